@@ -1,0 +1,229 @@
+/*
+ * voxel_c.h -- C ABI of libvoxel_c.so, the MI355X (gfx950) execution backend
+ * for VoxelVM.
+ *
+ * Part 1 declares the 64 symbols of the reference's C ABI with identical
+ * names and signatures (reference: bindings/voxel_c.h:14-114), so Go (cgo),
+ * Java (FFM) and C callers link unchanged.  Each block cites the reference
+ * declaration it replaces.  Differences in behaviour are listed per function:
+ * the reference's CPU loops are replaced by GPU kernels, and programs the GPU
+ * lowering does not recognise set voxel_last_error() instead of running.
+ *
+ * Part 2 adds voxel_gpu_* functions (no reference counterpart) for what the
+ * reference ABI cannot express: device-resident columns, f32/i32 columns,
+ * two-column predicates, u64 key sort with payload, i64 join with payload
+ * sum, per-group results, and batched/served queries.
+ *
+ * Conventions (all functions):
+ *  - no C++ exceptions cross this boundary;
+ *  - errors set a thread-local message readable with voxel_last_error();
+ *    functions returning int return 0 on success and -1 on error;
+ *  - "device pointer" means memory on the current HIP device (hipMalloc'ed,
+ *    or a torch CUDA tensor's data_ptr()).
+ */
+#pragma once
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ====================================================================== */
+/* Part 1 -- reference ABI                                                 */
+/* ====================================================================== */
+
+/* Engine (f64) -- reference bindings/voxel_c.h:14-24.
+ * add_segment borrows the caller's pointer exactly as the reference does
+ * (voxel_c.cpp:18-20); the rows a run needs are copied to HBM at each run
+ * (the caller may have changed them).  voxel_gpu_engine_set_resident() opts
+ * out of the re-copy.  run: fast-path programs (engine.hpp:84-141) and the
+ * Tier-0 loop programs of voxel_query / the MOV-prefixed tests run on the
+ * GPU with the reference's register effects; any other program, and loops
+ * that would not terminate in the reference, set voxel_last_error() and leave
+ * the registers untouched. */
+void*     voxel_engine_create_f64(void);
+void      voxel_engine_destroy(void* engine);
+size_t    voxel_engine_add_segment(void* engine, const double* data, size_t count);
+void      voxel_engine_load_program(void* engine, const uint32_t* code, size_t len);
+void      voxel_engine_run(void* engine);
+void      voxel_engine_run_parallel(void* engine, size_t seg_id, uint32_t threads, uint8_t result_reg);
+void      voxel_engine_set_scalar(void* engine, size_t reg, uint64_t value);
+uint64_t  voxel_engine_get_scalar(void* engine, size_t reg);
+void      voxel_engine_set_scalar_f64(void* engine, size_t reg, double value);
+double    voxel_engine_get_scalar_f64(void* engine, size_t reg);
+size_t    voxel_engine_k_lanes(void);
+
+/* Engine (i64) -- reference bindings/voxel_c.h:29-35. */
+void*     voxel_engine_create_i64(void);
+void      voxel_engine_destroy_i64(void* engine);
+size_t    voxel_engine_add_segment_i64(void* engine, const int64_t* data, size_t count);
+void      voxel_engine_load_program_i64(void* engine, const uint32_t* code, size_t len);
+void      voxel_engine_run_i64(void* engine);
+void      voxel_engine_set_scalar_i64(void* engine, size_t reg, uint64_t value);
+uint64_t  voxel_engine_get_scalar_i64(void* engine, size_t reg);
+
+/* JIT -- reference bindings/voxel_c.h:40-41 (voxel_c.cpp:80-101).  The fused
+ * VLOAD/VFILTER/VSUM/ADDF/ADD/CMP/JNZ loop (codegen/x86_64.hpp:1600-1716) is
+ * lowered to the GPU scan with the JIT's semantics (8-row iterations from
+ * R2 = data_len, ordered compares, initial accumulator ignored).  Returns 0.0
+ * with voxel_last_error() set where the reference returns 0.0 (no fusable
+ * loop) and where it would run away (data_len == 0). */
+double    voxel_jit_run(const uint32_t* code, size_t code_len,
+                        const double* data, size_t data_len, double threshold);
+
+/* Instruction encoding -- reference bindings/voxel_c.h:46-65 (pure). */
+uint32_t  voxel_instr_vload(uint8_t vd, uint8_t ra, uint8_t seg_id, uint8_t count);
+uint32_t  voxel_instr_vfilter_gt(uint8_t vd, uint8_t va, uint8_t rb);
+uint32_t  voxel_instr_vfilter_ge(uint8_t vd, uint8_t va, uint8_t rb);
+uint32_t  voxel_instr_vfilter_lt(uint8_t vd, uint8_t va, uint8_t rb);
+uint32_t  voxel_instr_vfilter_le(uint8_t vd, uint8_t va, uint8_t rb);
+uint32_t  voxel_instr_vsum(uint8_t rd, uint8_t va);
+uint32_t  voxel_instr_vred_min(uint8_t rd, uint8_t va);
+uint32_t  voxel_instr_vred_max(uint8_t rd, uint8_t va);
+uint32_t  voxel_instr_vcount(uint8_t rd, uint8_t va);
+uint32_t  voxel_instr_addf(uint8_t rd, uint8_t ra, uint8_t rb);
+uint32_t  voxel_instr_add(uint8_t rd, uint8_t ra, uint8_t rb);
+uint32_t  voxel_instr_sub(uint8_t rd, uint8_t ra, uint8_t rb);
+uint32_t  voxel_instr_mul(uint8_t rd, uint8_t ra, uint8_t rb);
+uint32_t  voxel_instr_mov(uint8_t rd, int16_t imm);
+uint32_t  voxel_instr_movr(uint8_t rd, uint8_t ra);
+uint32_t  voxel_instr_cmp(uint8_t ra, uint8_t rb);
+uint32_t  voxel_instr_jnz(int16_t offset);
+uint32_t  voxel_instr_jz(int16_t offset);
+uint32_t  voxel_instr_jmp(int16_t offset);
+uint32_t  voxel_instr_halt(void);
+
+/* NullBitmap -- reference bindings/voxel_c.h:70-76 (host-side utility). */
+void*     voxel_null_create(size_t count);
+void      voxel_null_destroy(void* bitmap);
+int       voxel_null_is_null(const void* bitmap, size_t row);
+void      voxel_null_set_null(void* bitmap, size_t row);
+void      voxel_null_set_valid(void* bitmap, size_t row);
+size_t    voxel_null_null_count(const void* bitmap);
+size_t    voxel_null_valid_count(const void* bitmap);
+
+/* HashAggregator<u32,f64> -- reference bindings/voxel_c.h:81-85.  Rows are
+ * accumulated into per-key states on the GPU (groupby kernels); group_count
+ * is the number of distinct keys seen, as in ops/aggregate.hpp. */
+void*     voxel_agg_create(void);
+void      voxel_agg_destroy(void* agg);
+void      voxel_agg_init(void* agg, size_t expected_groups);
+void      voxel_agg_accumulate(void* agg, const uint32_t* keys, const double* values, size_t count);
+size_t    voxel_agg_group_count(const void* agg);
+
+/* Sort & TopK -- reference bindings/voxel_c.h:90-92.  voxel_sort_* is the
+ * stable argsort of SortOperator (ops/sort.hpp:506-541), computed by the GPU
+ * radix sort on order-preserving f64 keys (-0.0 ordered equal to +0.0). */
+void      voxel_sort_ascending(const double* data, size_t count, uint32_t* indices_out);
+void      voxel_sort_descending(const double* data, size_t count, uint32_t* indices_out);
+void      voxel_topk(const double* data, size_t count, size_t k, double* out, int largest);
+
+/* DictionaryEncoding<f64> -- reference bindings/voxel_c.h:97-101 (host). */
+void*     voxel_dict_create(void);
+void      voxel_dict_destroy(void* dict);
+void      voxel_dict_build(void* dict, const double* data, size_t count);
+size_t    voxel_dict_cardinality(const void* dict);
+void      voxel_dict_decode_batch(void* dict, double* out, size_t offset, size_t count);
+
+/* ThreadPool -- reference bindings/voxel_c.h:106-109 (host). */
+void*     voxel_pool_create(uint32_t threads);
+void      voxel_pool_destroy(void* pool);
+void      voxel_pool_wait_all(void* pool);
+size_t    voxel_pool_thread_count(const void* pool);
+
+/* Error handling -- reference bindings/voxel_c.h:114.  Unlike the reference
+ * (whose buffer is never written, voxel_c.cpp:7,227) the message is set by
+ * every failing call; run, run_parallel, jit_run and the voxel_gpu_* calls
+ * clear it when they start, so it is empty after a successful call. */
+const char* voxel_last_error(void);
+
+/* ====================================================================== */
+/* Part 2 -- GPU extensions (no reference counterpart)                     */
+/* ====================================================================== */
+
+enum {
+    VOXEL_GPU_F64 = 0, VOXEL_GPU_F32 = 1, VOXEL_GPU_I64 = 2, VOXEL_GPU_I32 = 3,
+    VOXEL_GPU_U64 = 4, VOXEL_GPU_U32 = 5
+};
+/* Predicate codes: the reference's kCmp* values (opcodes.hpp:415-420). */
+enum {
+    VOXEL_CMP_EQ = 0, VOXEL_CMP_NE = 1, VOXEL_CMP_LT = 2, VOXEL_CMP_LE = 3,
+    VOXEL_CMP_GT = 4, VOXEL_CMP_GE = 5, VOXEL_CMP_TRUE = 6
+};
+
+/* 1 when a HIP device is usable, 0 otherwise (message in voxel_last_error). */
+int       voxel_gpu_available(void);
+/* Device memory helpers (for callers without another allocator). */
+void*     voxel_gpu_malloc(size_t bytes);
+void      voxel_gpu_free(void* dev);
+int       voxel_gpu_memcpy_h2d(void* dev, const void* host, size_t bytes);
+int       voxel_gpu_memcpy_d2h(void* host, const void* dev, size_t bytes);
+int       voxel_gpu_synchronize(void);
+
+/* Fills a device column with deterministic uniform values: row i uses the
+ * i-th output u of SplitMix64 seeded with `seed` (z = seed + (i+1) *
+ * 0x9E3779B97F4A7C15, then the SplitMix64 finaliser); floats get
+ * lo + (hi - lo) * ((u >> 11) * 2^-53) (f32: rounded), integers get
+ * lo + floor(u * (hi - lo) / 2^64).  stream may be NULL. */
+int       voxel_gpu_fill_uniform(int dtype, void* dev, size_t n, uint64_t seed, double lo, double hi,
+                                 void* stream);
+
+/* Typed engine for i32 columns (Engine<i32>, fast path only). dtype is
+ * VOXEL_GPU_I32, VOXEL_GPU_I64 or VOXEL_GPU_F64; add_segment_raw borrows. */
+void*     voxel_gpu_engine_create(int dtype);
+size_t    voxel_gpu_engine_add_segment_raw(void* engine, const void* data, size_t count);
+/* Declares a segment immutable until the engine is destroyed: it is copied
+ * to HBM once and not re-copied at each run. */
+int       voxel_gpu_engine_set_resident(void* engine, size_t seg_id, int resident);
+/* Attaches a device-resident column (no copies at all). */
+size_t    voxel_gpu_engine_add_device_segment(void* engine, const void* dev, size_t count);
+
+/* 1 when voxel_jit_run would lower the program, else 0 (reason in
+ * voxel_last_error); backs voxel_py.jit_compile. */
+int       voxel_gpu_jit_lowerable(const uint32_t* code, size_t len);
+/* voxel_jit_run with R2 given separately from the row count (the pybind
+ * jit_run form, voxel_bindings.cpp:342-366: R2 = total_count). */
+double    voxel_gpu_jit_run_r2(const uint32_t* code, size_t code_len, const double* data,
+                               size_t data_len, uint64_t r2, double threshold);
+/* How load_program lowered the engine's program: 1 fast path, 2 Tier-0
+ * loop, 0 not lowerable (run will fail with the reason). */
+int       voxel_gpu_engine_lowered(const void* engine);
+
+/* Relational filter + aggregate over device columns:
+ * rows with op_a(a[i], t_a) AND op_b(b[i], t_b); COUNT, SUM(a), MIN(a),
+ * MAX(a).  Thresholds are bit patterns of the column type (f32/i32 in the
+ * low 32 bits).  Float sums are compensated in the column's own type; MIN/MAX
+ * skip NaN. */
+typedef struct voxel_gpu_scan_desc {
+    int32_t  dtype;
+    int32_t  op_a, op_b;       /* op_b = VOXEL_CMP_TRUE when b is NULL */
+    int32_t  blocks;           /* 0 = library default */
+    const void* a;
+    const void* b;
+    uint64_t n;
+    uint64_t t_a, t_b;
+} voxel_gpu_scan_desc;
+
+typedef struct voxel_gpu_agg {
+    uint64_t count;
+    double   sum;              /* floats: s + c rounded to the column type */
+    double   sum_hi, sum_lo;   /* floats: the compensated pair (s, c) */
+    int64_t  isum;             /* ints: wrapping 64-bit sum (i32 widened) */
+    double   fmin, fmax;       /* floats */
+    int64_t  imin, imax;       /* ints */
+    int32_t  has_minmax;
+    int32_t  pad;
+} voxel_gpu_agg;
+
+/* Launch + wait + read back. stream may be NULL (library stream). */
+int       voxel_gpu_scan(const voxel_gpu_scan_desc* d, void* stream, voxel_gpu_agg* out);
+/* Enqueue only (result stays on the device until voxel_gpu_scan_result). */
+int       voxel_gpu_scan_launch(const voxel_gpu_scan_desc* d, void* stream);
+int       voxel_gpu_scan_result(void* stream, voxel_gpu_agg* out);
+
+#ifdef __cplusplus
+}
+#endif

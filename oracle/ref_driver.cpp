@@ -1,0 +1,204 @@
+// ref_driver.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A thin extern "C" driver, compiled together with the reference's own
+// header-only C++ (include/voxel/...) straight from the reference checkout by
+// oracle/Makefile into oracle/_ref/libvoxel_ref.so.  It only instantiates the
+// reference's classes and calls them; it contains no algorithm of its own.
+// Used to pin the oracle restatement (tests/) and as the timed CPU baseline
+// (bench.py cpu_baseline, kind "reference").  The product never loads it.
+#include "voxel/voxel.hpp"
+#include <bit>
+#include <cstring>
+#include <random>
+#include <vector>
+
+using namespace voxel;
+
+namespace {
+template <typename T>
+int run_engine(T* data, size_t n, const uint32_t* code, size_t len, uint64_t* regs) {
+    Engine<T> e;
+    e.AddSegment(data, n);
+    for (size_t r = 0; r < 64; ++r) e.ScalarReg(r) = regs[r];
+    std::vector<u32> v(code, code + len);
+    e.LoadProgram(v);
+    e.Run();
+    for (size_t r = 0; r < 64; ++r) regs[r] = e.ScalarReg(r);
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+// Input generators of the reference's examples (validate.cpp:17-29).
+void vr_mt_uniform_f64(uint64_t seed, size_t n, double lo, double hi, double* out) {
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<double> d(lo, hi);
+    for (size_t i = 0; i < n; ++i) out[i] = d(g);
+}
+void vr_mt_uniform_i64(uint64_t seed, size_t n, int64_t lo, int64_t hi, int64_t* out) {
+    std::mt19937_64 g(seed);
+    std::uniform_int_distribution<int64_t> d(lo, hi);
+    for (size_t i = 0; i < n; ++i) out[i] = d(g);
+}
+void vr_mt_uniform_i32(uint64_t seed, size_t n, int32_t lo, int32_t hi, int32_t* out) {
+    std::mt19937_64 g(seed);
+    std::uniform_int_distribution<int32_t> d(lo, hi);
+    for (size_t i = 0; i < n; ++i) out[i] = d(g);
+}
+
+// Engine<T>::LoadProgram + Run (fast path or Tier-0), registers in/out.
+int vr_engine_run_f64(double* d, size_t n, const uint32_t* c, size_t l, uint64_t* r) { return run_engine<f64>(d, n, c, l, r); }
+int vr_engine_run_i64(int64_t* d, size_t n, const uint32_t* c, size_t l, uint64_t* r) { return run_engine<i64>(d, n, c, l, r); }
+int vr_engine_run_i32(int32_t* d, size_t n, const uint32_t* c, size_t l, uint64_t* r) { return run_engine<i32>(d, n, c, l, r); }
+
+// Engine<f64>::RunParallel, registers in/out.
+void vr_engine_run_parallel_f64(double* d, size_t n, const uint32_t* c, size_t l, uint64_t* regs,
+                                uint32_t threads, uint8_t result_reg) {
+    Engine<f64> e;
+    e.AddSegment(d, n);
+    for (size_t r = 0; r < 64; ++r) e.ScalarReg(r) = regs[r];
+    std::vector<u32> v(c, c + l);
+    e.LoadProgram(v);
+    e.RunParallel(0, threads, result_reg);
+    for (size_t r = 0; r < 64; ++r) regs[r] = e.ScalarReg(r);
+}
+
+// The voxel_jit_run sequence (bindings/voxel_c.cpp:80-101), compile and
+// entry split so that a caller can time the entry alone.
+void* vr_jit_compile(const uint32_t* code, size_t len, size_t* code_size) {
+    auto compiler = codegen::CreateJitCompiler();
+    if (!compiler) return nullptr;
+    auto* f = new codegen::JitFunction();
+    if (!compiler->Compile(code, len, *f) || !f->IsValid()) { compiler->Release(*f); delete f; return nullptr; }
+    if (code_size) *code_size = f->CodeSize;
+    return f;
+}
+double vr_jit_entry(void* fn, const double* data, size_t n, uint64_t r2, double threshold) {
+    auto* f = static_cast<codegen::JitFunction*>(fn);
+    RegFile rf;
+    rf.Scalar(0) = 0;
+    rf.Scalar(1) = 0;
+    rf.Scalar(2) = r2;
+    rf.Scalar(3) = std::bit_cast<u64>(threshold);
+    f64* seg = const_cast<f64*>(data);
+    u64 cnt = n;
+    f->Entry(&rf, &seg, &cnt);
+    return std::bit_cast<f64>(rf.Scalar(0));
+}
+void vr_jit_release(void* fn) {
+    auto compiler = codegen::CreateJitCompiler();
+    auto* f = static_cast<codegen::JitFunction*>(fn);
+    if (compiler) compiler->Release(*f);
+    delete f;
+}
+
+// VectorFilter<T>::Apply{EQ,NE,LT,LE,GT,GE,Between}; op = kCmp code, 6 = Between.
+#define VR_BITMAP(NAME, T)                                                              \
+uint64_t NAME(const T* x, size_t n, int op, T t, T t2, uint64_t* words) {              \
+    ops::VectorFilter<T> f;                                                             \
+    typename ops::VectorFilter<T>::FilterResult r;                                      \
+    switch (op) {                                                                       \
+    case 0: r = f.ApplyEQ(x, n, t); break; case 1: r = f.ApplyNE(x, n, t); break;       \
+    case 2: r = f.ApplyLT(x, n, t); break; case 3: r = f.ApplyLE(x, n, t); break;       \
+    case 4: r = f.ApplyGT(x, n, t); break; case 5: r = f.ApplyGE(x, n, t); break;       \
+    default: r = f.ApplyBetween(x, n, t, t2); break; }                                  \
+    if (!r.Bitmap.empty()) std::memcpy(words, r.Bitmap.data(), r.Bitmap.size() * 8);    \
+    return r.PassCount;                                                                 \
+}
+VR_BITMAP(vr_bitmap_f64, double)
+VR_BITMAP(vr_bitmap_f32, float)
+VR_BITMAP(vr_bitmap_i64, int64_t)
+VR_BITMAP(vr_bitmap_i32, int32_t)
+
+// ColumnFilter::Init + AndWith/OrWith/AndNotWith; active is replaced.
+uint64_t vr_column_filter(uint64_t* active, const uint64_t* other, size_t rows, int combine) {
+    ops::ColumnFilter<f64> cf;
+    cf.Init(rows);
+    size_t nw = (rows + 63) / 64;
+    cf.AndWith(active, nw);
+    if (combine == 0) cf.AndWith(other, nw);
+    else if (combine == 1) cf.OrWith(other, nw);
+    else cf.AndNotWith(other, nw);
+    std::memcpy(active, cf.GetBitmap().data(), nw * 8);
+    return cf.GetActiveCount();
+}
+
+// SelectionVector::BuildFromBitmap.
+uint64_t vr_select_from_bitmap(const uint64_t* words, size_t rows, uint32_t* out) {
+    ops::SelectionVector<f64> sv;
+    sv.BuildFromBitmap(words, rows);
+    if (sv.Count) std::memcpy(out, sv.Indices.data(), sv.Count * 4);
+    return sv.Count;
+}
+
+// simd::scalar::scalar_filter_gt_* / scalar_sum_*.
+uint64_t vr_scalar_filter_gt_f64(double* dst, const double* src, double t, size_t n) { return simd::scalar::scalar_filter_gt_f64(dst, src, t, n); }
+uint64_t vr_scalar_filter_gt_f32(float* dst, const float* src, float t, size_t n) { return simd::scalar::scalar_filter_gt_f32(dst, src, t, n); }
+double vr_scalar_sum_f64(const double* src, size_t n) { return simd::scalar::scalar_sum_f64(src, n); }
+float vr_scalar_sum_f32(const float* src, size_t n) { return simd::scalar::scalar_sum_f32(src, n); }
+
+// RadixSort<u64>::Sort(data, indices, count).
+void vr_radix_sort_u64(uint64_t* keys, uint32_t* idx, size_t n) {
+    ops::RadixSort<u64> s;
+    s.Sort(keys, idx, n);
+}
+
+// SortOperator::SortAscending/Descending (what voxel_sort_* call).
+void vr_sort_f64(const double* data, size_t n, int descending, uint32_t* idx) {
+    Segment<f64> seg(const_cast<f64*>(data), n);
+    if (descending) ops::SortOperator::SortDescending(seg, idx);
+    else ops::SortOperator::SortAscending(seg, idx);
+}
+
+// HashJoin<i64>::Build + Probe, plus the harness's wrapping payload sum.
+uint64_t vr_hash_join_i64(const int64_t* build, size_t nb, const int64_t* payload,
+                          const int64_t* probe, size_t np, uint32_t* left, uint32_t* right,
+                          int64_t* payload_sum) {
+    Arena arena;
+    ops::HashJoin<i64> j(arena);
+    j.Build(build, nb);
+    std::vector<u32> l(left ? 0 : np), r(right ? 0 : np);
+    u32* lo = left ? left : l.data();
+    u32* ro = right ? right : r.data();
+    sz m = 0;
+    j.Probe(probe, np, lo, ro, m);
+    u64 s = 0;
+    if (payload) for (sz i = 0; i < m; ++i) s += static_cast<u64>(payload[ro[i]]);
+    if (payload_sum) *payload_sum = static_cast<i64>(s);
+    return m;
+}
+
+// HashAggregator<i32|u32, f64>::Init + Accumulate + Finalize (slot order).
+#define VR_AGG(NAME, K)                                                                  \
+size_t NAME(const K* keys, const double* vals, size_t n, size_t expected, size_t maxg,    \
+            K* out_key, uint64_t* out_count, double* out_sum, double* out_min,            \
+            double* out_max, double* out_mean, double* out_m2) {                          \
+    Arena arena;                                                                          \
+    ops::HashAggregator<K, f64> a(arena);                                                 \
+    a.Init(expected);                                                                     \
+    a.Accumulate(keys, vals, n);                                                          \
+    size_t g = a.GroupCount();                                                            \
+    if (g > maxg) return g;                                                               \
+    std::vector<typename ops::HashAggregator<K, f64>::AggregateState> st(g);              \
+    a.Finalize(out_key, st.data(), g);                                                    \
+    for (size_t i = 0; i < g; ++i) {                                                      \
+        out_count[i] = st[i].Count; out_sum[i] = st[i].Sum; out_min[i] = st[i].Min;       \
+        out_max[i] = st[i].Max; out_mean[i] = st[i].Mean; out_m2[i] = st[i].M2;           \
+    }                                                                                     \
+    return g;                                                                             \
+}
+VR_AGG(vr_hash_agg_i32_f64, int32_t)
+VR_AGG(vr_hash_agg_u32_f64, uint32_t)
+
+// TopKSelect (voxel_topk).
+void vr_topk_f64(const double* data, size_t n, size_t k, double* out, int largest) {
+    ops::TopKSelect(data, n, k, out, largest != 0);
+}
+
+// Instruction encoders (bytecode/instruction.hpp) for program-word fixtures.
+uint32_t vr_encode(uint8_t op, uint8_t rd, uint8_t ra, uint8_t rb, uint16_t imm) {
+    return Instruction::Encode(static_cast<Opcode>(op), rd, ra, rb, imm);
+}
+
+}  // extern "C"

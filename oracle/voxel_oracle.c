@@ -1,0 +1,594 @@
+/*
+ * voxel_oracle.c -- CPU restatement of the VoxelVM hot-path semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it may be loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * by nothing else.  The product library (voxel_amd/libvoxel_c.so) never links,
+ * loads or calls it.
+ *
+ * Every function restates one piece of the reference algorithm from its
+ * source, cited as file:line relative to the reference repository root.
+ * Functions are plain C99 (+ unsigned __int128 from gcc) so that they are easy
+ * to read next to the reference.
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   vo_mt64_*            first outputs of std::mt19937_64(42) quoted by the
+ *                        reference (docs/Benchmarks.md:7 ground truth 500,720)
+ *   vo_acc_*             exact: Python Fraction arithmetic on small inputs,
+ *                        cancellation cases, math.fsum
+ *   vo_fastpath_*        reference golden values (test_suite.cpp:254-321,
+ *                        c_test.cpp:8-36) + oracle/_ref Engine<T>::Run
+ *   vo_tier0_*           voxel_query golden values (sum/count/min/max bits)
+ *                        + oracle/_ref Engine<f64> Tier-0
+ *   vo_jit_f64           oracle/_ref JIT (jit_run) bit patterns
+ *   vo_run_parallel_*    oracle/_ref RunParallel bit pattern
+ *   vo_bitmap_*          test_suite.cpp:410-427 bits {2,3,6,7}, numpy, _ref
+ *   vo_radix_*           std-library stable argsort (numpy kind="stable"),
+ *                        golden keys {5,3,5,1,3,2^64-1,0}, _ref RadixSort
+ *   vo_mergesort_*       numpy stable argsort, c_test idx [1 3 0 2 4], _ref
+ *   vo_hashjoin_*        brute-force dict join, golden pairs, _ref HashJoin
+ *   vo_groupby_*         brute-force per-key loops, golden groups, _ref
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+/* ------------------------------------------------------------------------ */
+/* Random generators used to make the reference's own example inputs.      */
+/* ------------------------------------------------------------------------ */
+
+/* mt19937_64 as published by Matsumoto & Nishimura (2004); this is the
+ * std::mt19937_64 used by examples/validate.cpp:17-29, jit_test.cpp:15-30. */
+typedef struct { uint64_t mt[312]; int idx; } vo_mt64;
+
+void vo_mt64_seed(vo_mt64* s, uint64_t seed) {
+    s->mt[0] = seed;
+    for (int i = 1; i < 312; ++i)
+        s->mt[i] = 6364136223846793005ULL * (s->mt[i - 1] ^ (s->mt[i - 1] >> 62)) + (uint64_t)i;
+    s->idx = 312;
+}
+
+static void vo_mt64_twist(vo_mt64* s) {
+    const uint64_t upper = 0xFFFFFFFF80000000ULL, lower = 0x7FFFFFFFULL;
+    for (int i = 0; i < 312; ++i) {
+        uint64_t x = (s->mt[i] & upper) | (s->mt[(i + 1) % 312] & lower);
+        uint64_t xa = x >> 1;
+        if (x & 1) xa ^= 0xB5026F5AA96619E9ULL;
+        s->mt[i] = s->mt[(i + 156) % 312] ^ xa;
+    }
+    s->idx = 0;
+}
+
+uint64_t vo_mt64_next(vo_mt64* s) {
+    if (s->idx >= 312) vo_mt64_twist(s);
+    uint64_t x = s->mt[s->idx++];
+    x ^= (x >> 29) & 0x5555555555555555ULL;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+    x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+    x ^= x >> 43;
+    return x;
+}
+
+/* uniform_real_distribution<double>(lo, hi) over a 64-bit engine as the
+ * GNU C++ library computes it: one draw, converted to double and divided by
+ * 2^64 (generate_canonical<double,53>), clamped below 1, then lo + u*(hi-lo). */
+void vo_mt64_uniform_f64(uint64_t seed, size_t n, double lo, double hi, double* out) {
+    vo_mt64 s; vo_mt64_seed(&s, seed);
+    for (size_t i = 0; i < n; ++i) {
+        double u = (double)vo_mt64_next(&s) / 18446744073709551616.0;
+        if (u >= 1.0) u = nextafter(1.0, 0.0);
+        out[i] = u * (hi - lo) + lo;
+    }
+}
+
+/* uniform_int_distribution<int64_t/int32_t>(lo, hi) over a 64-bit engine:
+ * Lemire's nearly-divisionless bounded draw (ACM TOMACS 29(1), 2019) with a
+ * 128-bit product, which is what the GNU C++ library does for a 64-bit engine. */
+static uint64_t vo_lemire64(vo_mt64* s, uint64_t range) {
+    u128 prod = (u128)vo_mt64_next(s) * range;
+    uint64_t low = (uint64_t)prod;
+    if (low < range) {
+        uint64_t thresh = (0 - range) % range;
+        while (low < thresh) {
+            prod = (u128)vo_mt64_next(s) * range;
+            low = (uint64_t)prod;
+        }
+    }
+    return (uint64_t)(prod >> 64);
+}
+
+void vo_mt64_uniform_i64(uint64_t seed, size_t n, int64_t lo, int64_t hi, int64_t* out) {
+    vo_mt64 s; vo_mt64_seed(&s, seed);
+    uint64_t urange = (uint64_t)hi - (uint64_t)lo;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t r = (urange == UINT64_MAX) ? vo_mt64_next(&s) : vo_lemire64(&s, urange + 1);
+        out[i] = (int64_t)(r + (uint64_t)lo);
+    }
+}
+
+void vo_mt64_uniform_i32(uint64_t seed, size_t n, int32_t lo, int32_t hi, int32_t* out) {
+    vo_mt64 s; vo_mt64_seed(&s, seed);
+    uint64_t urange = (uint64_t)(uint32_t)hi - (uint64_t)(uint32_t)lo;
+    urange &= 0xFFFFFFFFULL;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t r = vo_lemire64(&s, urange + 1);
+        out[i] = (int32_t)(uint32_t)(r + (uint64_t)(uint32_t)lo);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact long accumulator for float sums (Kulisch-style fixed point).       */
+/* A double is m * 2^(e), |m| < 2^53; bit position p = e + 1074 >= 0.       */
+/* Digits are 32-bit values held in int64 with lazy carry propagation.      */
+/* ------------------------------------------------------------------------ */
+#define VO_ACC_DIGITS 72
+typedef struct { int64_t d[VO_ACC_DIGITS]; uint32_t pending; int32_t nonfinite; } vo_acc;
+
+void vo_acc_init(vo_acc* a) { memset(a, 0, sizeof(*a)); }
+
+static void vo_acc_norm(vo_acc* a) {
+    for (int i = 0; i < VO_ACC_DIGITS - 1; ++i) {
+        int64_t carry = a->d[i] >> 32;           /* arithmetic shift: floor division */
+        a->d[i] -= carry * 4294967296LL;
+        a->d[i + 1] += carry;
+    }
+    a->pending = 0;
+}
+
+void vo_acc_add(vo_acc* a, double x) {
+    if (x == 0.0) return;
+    if (!isfinite(x)) { a->nonfinite = 1; return; }
+    int e;
+    double m = frexp(x, &e);                      /* x = m * 2^e, 0.5 <= |m| < 1 */
+    int64_t mi = (int64_t)ldexp(m, 53);           /* exact: x = mi * 2^(e-53)     */
+    int pos = e - 53 + 1074;
+    uint64_t mag = (uint64_t)(mi < 0 ? -mi : mi);
+    if (pos < 0) { mag >>= -pos; pos = 0; }       /* subnormal: trailing zeros only */
+    u128 v = (u128)mag << (pos & 31);
+    int dig = pos >> 5;
+    for (int k = 0; k < 3; ++k) {
+        int64_t chunk = (int64_t)(uint32_t)(v >> (32 * k));
+        a->d[dig + k] += (mi < 0) ? -chunk : chunk;
+    }
+    if (++a->pending >= (1u << 29)) vo_acc_norm(a);
+}
+
+/* Canonical digits: d[i] in [0, 2^32) for i < top, top digit signed.
+ * Value = sum d[i] * 2^(32 i) * 2^-1074. */
+void vo_acc_export(vo_acc* a, int64_t* out) {
+    vo_acc_norm(a);
+    memcpy(out, a->d, sizeof(a->d));
+}
+
+int vo_acc_digits(void) { return VO_ACC_DIGITS; }
+
+/* ------------------------------------------------------------------------ */
+/* Comparison predicates: kCmp* codes of bytecode/opcodes.hpp:415-420       */
+/* (EQ 0, NE 1, LT 2, LE 3, GT 4, GE 5); 6 = no predicate.                 */
+/* C++ relational operators: NaN fails all but != (engine.hpp:124-131).    */
+/* ------------------------------------------------------------------------ */
+#define VO_PRED(op, x, t) \
+    ((op) == 0 ? ((x) == (t)) : (op) == 1 ? ((x) != (t)) : (op) == 2 ? ((x) < (t)) : \
+     (op) == 3 ? ((x) <= (t)) : (op) == 4 ? ((x) > (t)) : (op) == 5 ? ((x) >= (t)) : 1)
+
+/* Relational (SQL-style) filter+aggregate over one or two columns:
+ * rows i with pred_a(a[i], ta) && pred_b(b[i], tb); COUNT, exact SUM(a),
+ * MIN/MAX(a) over passing non-NaN values.  This is what
+ * VectorFilter::Apply* + ColumnFilter::AndWith (ops/filter.hpp:42-288,
+ * 540-547) select, aggregated without the lane/tail rules of the VM. */
+#define VO_DEF_REL(NAME, T, ISF)                                                        \
+void NAME(const T* a, const T* b, size_t n, int opa, T ta, int opb, T tb,              \
+          uint64_t* count, int64_t* digits, T* mn, T* mx, int64_t* isum, int* any) {    \
+    vo_acc acc; vo_acc_init(&acc);                                                      \
+    uint64_t c = 0; int64_t s = 0; int have = 0; T lo = 0, hi = 0;                      \
+    for (size_t i = 0; i < n; ++i) {                                                    \
+        T x = a[i];                                                                     \
+        if (!VO_PRED(opa, x, ta)) continue;                                             \
+        if (b && !VO_PRED(opb, b[i], tb)) continue;                                     \
+        ++c;                                                                            \
+        if (ISF) { vo_acc_add(&acc, (double)x); if (x != x) continue; }                 \
+        else s = (int64_t)((uint64_t)s + (uint64_t)(int64_t)x);                         \
+        if (!have) { lo = hi = x; have = 1; }                                           \
+        else { if (x < lo) lo = x; if (x > hi) hi = x; }                                \
+    }                                                                                   \
+    *count = c; if (digits) vo_acc_export(&acc, digits);                                \
+    *mn = lo; *mx = hi; *isum = s; *any = have;                                         \
+}
+VO_DEF_REL(vo_rel_f64, double, 1)
+VO_DEF_REL(vo_rel_f32, float, 1)
+VO_DEF_REL(vo_rel_i64, int64_t, 0)
+VO_DEF_REL(vo_rel_i32, int32_t, 0)
+
+/* Exact sum of a plain array (no predicate). */
+void vo_exact_sum_f64(const double* x, size_t n, int64_t* digits) {
+    vo_acc acc; vo_acc_init(&acc);
+    for (size_t i = 0; i < n; ++i) vo_acc_add(&acc, x[i]);
+    vo_acc_export(&acc, digits);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Interpreter fast path: Engine<T>::RunFastPath, exec/engine.hpp:109-141.  */
+/* filter opcode 0xC1..0xC6 (EQ NE LT LE GT GE), any other -> GT (line 130).*/
+/* total = min(seg.Count, (sz)(i64)R[total]); groups of kL = 32/sizeof(T)   */
+/* while off + kL <= total; lanes summed ((v0+v1)+v2)+... in T; acc += s.   */
+/* Integer T wraps (computed in unsigned arithmetic).  Returns final off.   */
+/* ------------------------------------------------------------------------ */
+static int vo_fast_pass_code(uint8_t fop) {
+    switch (fop) { case 0xC1: return 0; case 0xC2: return 1; case 0xC3: return 2;
+                   case 0xC4: return 3; case 0xC5: return 4; case 0xC6: return 5; default: return 4; }
+}
+
+#define VO_DEF_FAST(NAME, T, UT, KL, ISF)                                                 \
+uint64_t NAME(const T* data, uint64_t count, uint64_t reg_total, uint64_t reg_off,       \
+              T thresh, T acc0, uint8_t fop, T* acc_out) {                               \
+    uint64_t total = reg_total; if (count < total) total = count;                        \
+    uint64_t off = reg_off;                                                              \
+    int op = vo_fast_pass_code(fop);                                                     \
+    T acc = acc0;                                                                        \
+    while (off + KL <= total) {                                                          \
+        T s = 0;                                                                         \
+        for (int i = 0; i < KL; ++i) {                                                   \
+            T x = data[off + i];                                                         \
+            T v = VO_PRED(op, x, thresh) ? x : (T)0;                                     \
+            if (ISF) s = s + v; else s = (T)(UT)((UT)s + (UT)v);                         \
+        }                                                                                \
+        if (ISF) acc = acc + s; else acc = (T)(UT)((UT)acc + (UT)s);                     \
+        off += KL;                                                                       \
+    }                                                                                    \
+    *acc_out = acc;                                                                      \
+    return off;                                                                          \
+}
+VO_DEF_FAST(vo_fastpath_f64, double, uint64_t, 4, 1)
+VO_DEF_FAST(vo_fastpath_f32, float, uint32_t, 8, 1)
+VO_DEF_FAST(vo_fastpath_i64, int64_t, uint64_t, 4, 0)
+VO_DEF_FAST(vo_fastpath_i32, int32_t, uint32_t, 8, 0)
+
+/* ------------------------------------------------------------------------ */
+/* Tier-0 loop programs of the voxel_query shapes (bindings/voxel_query.py: */
+/* 74-135,155-171) and the MOV-prefixed test programs, Engine<f64/i64>.     */
+/* Handlers: VLOAD engine.hpp:1947-1969 (zero past Count and past the VLOAD */
+/* count field), VFILTER 2870-2924 (dst = pass ? src : 0), VSUM 2980-2987,  */
+/* VCOUNT 3097-3111 (lanes != 0 and not NaN), VRED_MIN/MAX 3071-3095,       */
+/* ADDF 1394-1401 (f64 on register bits), ADD 1286-1292 (i64), CMP          */
+/* 1603-1608 (flags of i64 wrapping difference), JLE/JGE 1758-1776, MOVR.   */
+/* The loop runs until off == total (JNZ): only called with                 */
+/* (total - off) a positive multiple of kL (otherwise it never ends).       */
+/* red: 0 VSUM, 1 VCOUNT, 2 VRED_MIN, 3 VRED_MAX.                           */
+/* accop: 0 ADDF, 1 ADD, 2 CMP/JLE/MOVR, 3 CMP/JGE/MOVR.                    */
+/* Returns R[acc] bits; *part_out = last group's R[part] bits.              */
+/* ------------------------------------------------------------------------ */
+static uint64_t vo_f64bits(double d) { uint64_t u; memcpy(&u, &d, 8); return u; }
+static double vo_bitsf64(uint64_t u) { double d; memcpy(&d, &u, 8); return d; }
+
+static uint64_t vo_tier0_accum(uint64_t acc, uint64_t part, int accop) {
+    if (accop == 0) return vo_f64bits(vo_bitsf64(acc) + vo_bitsf64(part));
+    if (accop == 1) return acc + part;
+    int64_t diff = (int64_t)(acc - part);                   /* CMP acc, part */
+    int z = diff == 0, sgn = diff < 0;
+    if (accop == 2) return (z || sgn) ? acc : part;         /* JLE +2 skips MOVR */
+    return (z || !sgn) ? acc : part;                        /* JGE +2 skips MOVR */
+}
+
+uint64_t vo_tier0_f64(const double* data, uint64_t count, uint64_t off, uint64_t total,
+                      int filt, double thresh, int vcount, int red, int accop,
+                      uint64_t acc, uint64_t* part_out) {
+    uint64_t part = 0;
+    if (vcount == 0) vcount = 4;
+    do {
+        double v[4];
+        for (int i = 0; i < 4; ++i) {
+            uint64_t idx = off + (uint64_t)i;
+            v[i] = (i < vcount && idx < count) ? data[idx] : 0.0;
+            if (filt != 6) v[i] = VO_PRED(filt, v[i], thresh) ? v[i] : 0.0;
+        }
+        if (red == 0) { double s = 0.0; for (int i = 0; i < 4; ++i) s += v[i]; part = vo_f64bits(s); }
+        else if (red == 1) { uint64_t c = 0; for (int i = 0; i < 4; ++i) if (!isnan(v[i]) && v[i] != 0.0) ++c; part = c; }
+        else if (red == 2) { double m = v[0]; for (int i = 1; i < 4; ++i) if (v[i] < m || isnan(v[i])) m = v[i]; part = vo_f64bits(m); }
+        else { double m = v[0]; for (int i = 1; i < 4; ++i) if (v[i] > m || isnan(m)) m = v[i]; part = vo_f64bits(m); }
+        acc = vo_tier0_accum(acc, part, accop);
+        off += 4;
+    } while (off != total);
+    *part_out = part;
+    return acc;
+}
+
+uint64_t vo_tier0_i64(const int64_t* data, uint64_t count, uint64_t off, uint64_t total,
+                      int filt, int64_t thresh, int vcount, int red, int accop,
+                      uint64_t acc, uint64_t* part_out) {
+    uint64_t part = 0;
+    if (vcount == 0) vcount = 4;
+    do {
+        int64_t v[4];
+        for (int i = 0; i < 4; ++i) {
+            uint64_t idx = off + (uint64_t)i;
+            v[i] = (i < vcount && idx < count) ? data[idx] : 0;
+            if (filt != 6) v[i] = VO_PRED(filt, v[i], thresh) ? v[i] : 0;
+        }
+        if (red == 0) { uint64_t s = 0; for (int i = 0; i < 4; ++i) s += (uint64_t)v[i]; part = s; }
+        else if (red == 1) { uint64_t c = 0; for (int i = 0; i < 4; ++i) if (v[i] != 0) ++c; part = c; }
+        else if (red == 2) { int64_t m = v[0]; for (int i = 1; i < 4; ++i) if (v[i] < m) m = v[i]; part = (uint64_t)m; }
+        else { int64_t m = v[0]; for (int i = 1; i < 4; ++i) if (v[i] > m) m = v[i]; part = (uint64_t)m; }
+        acc = vo_tier0_accum(acc, part, accop);
+        off += 4;
+    } while (off != total);
+    *part_out = part;
+    return acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* JIT fused loop, codegen/x86_64.hpp:1632-1716, as entered by voxel_jit_run */
+/* (bindings/voxel_c.cpp:80-101): iters = (R2 - off + 7) >> 3; each         */
+/* iteration adds masked x[8k+j] into lane j of accumulator A (j<4) or B;   */
+/* vcmppd predicates are ordered (NaN fails every one, NE included,         */
+/* line 1606); result = (L0+L1)+(L2+L3) with L = A+B; the initial R[acc] is */
+/* ignored.  Rows past data_len read as 0 here (the reference over-reads).  */
+/* ------------------------------------------------------------------------ */
+double vo_jit_f64(const double* data, uint64_t count, uint64_t r2, uint64_t off, uint8_t fop, double thresh) {
+    int op = vo_fast_pass_code(fop);
+    uint64_t iters = (r2 - off + 7) >> 3;
+    double A[4] = {0, 0, 0, 0}, B[4] = {0, 0, 0, 0};
+    for (uint64_t k = 0; k < iters; ++k) {
+        for (int j = 0; j < 8; ++j) {
+            uint64_t idx = off + 8 * k + (uint64_t)j;
+            double x = idx < count ? data[idx] : 0.0;
+            double v = (x == x && VO_PRED(op, x, thresh)) ? x : 0.0;
+            if (j < 4) A[j] += v; else B[j - 4] += v;
+        }
+    }
+    double L[4];
+    for (int j = 0; j < 4; ++j) L[j] = A[j] + B[j];
+    return (L[0] + L[1]) + (L[2] + L[3]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Engine<T>::RunParallel, engine.hpp:143-175, for fast-path programs:      */
+/* chunk = ceil(N/threads); worker t runs the fast path on rows             */
+/* [t*chunk, t*chunk+count) with every register copied (so the same R[off]  */
+/* and R[total], total clipped to the chunk's Count); combined = sum over   */
+/* workers of R[result] in T, in worker order, written to R[result].        */
+/* Only the result register's value is restated here.                       */
+/* ------------------------------------------------------------------------ */
+double vo_run_parallel_fast_f64(const double* data, uint64_t n, uint32_t threads,
+                                uint64_t reg_total, uint64_t reg_off, double thresh,
+                                double acc0, uint8_t fop) {
+    if (threads < 1) threads = 1;
+    uint64_t chunk = (n + threads - 1) / threads;
+    double combined = 0.0;
+    for (uint32_t t = 0; t < threads; ++t) {
+        uint64_t start = (uint64_t)t * chunk;
+        if (start >= n) continue;                    /* count == 0: worker never ran */
+        uint64_t cnt = (start + chunk > n) ? (n - start) : chunk;
+        double acc;
+        vo_fastpath_f64(data + start, cnt, reg_total, reg_off, thresh, acc0, fop, &acc);
+        combined += acc;
+    }
+    return combined;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Bitmap filters: VectorFilter<T>::Apply* (ops/filter.hpp:42-288): bit     */
+/* (i & 63) of word i/64 = pred(x_i); tail bits zero; PassCount = popcount. */
+/* op 6 = Between(lo=t, hi=t2).                                             */
+/* ------------------------------------------------------------------------ */
+#define VO_DEF_BITMAP(NAME, T)                                                        \
+uint64_t NAME(const T* x, size_t n, int op, T t, T t2, uint64_t* words) {            \
+    size_t nw = (n + 63) / 64; memset(words, 0, nw * 8);                              \
+    uint64_t pc = 0;                                                                  \
+    for (size_t i = 0; i < n; ++i) {                                                  \
+        int p = op == 6 ? (x[i] >= t && x[i] <= t2) : VO_PRED(op, x[i], t);           \
+        if (p) { words[i >> 6] |= 1ULL << (i & 63); ++pc; }                           \
+    }                                                                                 \
+    return pc;                                                                        \
+}
+VO_DEF_BITMAP(vo_bitmap_f64, double)
+VO_DEF_BITMAP(vo_bitmap_f32, float)
+VO_DEF_BITMAP(vo_bitmap_i64, int64_t)
+VO_DEF_BITMAP(vo_bitmap_i32, int32_t)
+
+/* ColumnFilter::Init/AndWith/OrWith/AndNotWith (ops/filter.hpp:528-570):   */
+/* combine = 0 AND, 1 OR, 2 ANDNOT; Init's all-ones bitmap is tail-masked;  */
+/* returns the active count.                                                */
+uint64_t vo_bitmap_combine(uint64_t* active, const uint64_t* other, size_t rows, int combine) {
+    size_t nw = (rows + 63) / 64;
+    uint64_t pc = 0;
+    for (size_t w = 0; w < nw; ++w) {
+        uint64_t v = combine == 0 ? (active[w] & other[w]) : combine == 1 ? (active[w] | other[w])
+                                                                          : (active[w] & ~other[w]);
+        if (w == nw - 1 && (rows & 63)) v &= (1ULL << (rows & 63)) - 1;
+        active[w] = v;
+        pc += (uint64_t)__builtin_popcountll(v);
+    }
+    return pc;
+}
+
+/* SelectionVector::BuildFromBitmap (ops/filter.hpp:378-399): ascending    */
+/* u32 row ids of set bits.                                                 */
+uint64_t vo_select_from_bitmap(const uint64_t* words, size_t rows, uint32_t* out) {
+    size_t nw = (rows + 63) / 64;
+    uint64_t k = 0;
+    for (size_t w = 0; w < nw; ++w) {
+        uint64_t v = words[w];
+        while (v) { out[k++] = (uint32_t)(w * 64 + (uint64_t)__builtin_ctzll(v)); v &= v - 1; }
+    }
+    return k;
+}
+
+/* scalar_filter_gt_* (simd/scalar.hpp:155,214,260,306): passing values in  */
+/* row order (op generalised to the six kCmp codes).                        */
+#define VO_DEF_COMPACT(NAME, T)                                                \
+uint64_t NAME(const T* x, size_t n, int op, T t, T* out) {                    \
+    uint64_t k = 0;                                                            \
+    for (size_t i = 0; i < n; ++i) if (VO_PRED(op, x[i], t)) out[k++] = x[i];  \
+    return k;                                                                  \
+}
+VO_DEF_COMPACT(vo_compact_f64, double)
+VO_DEF_COMPACT(vo_compact_f32, float)
+VO_DEF_COMPACT(vo_compact_i64, int64_t)
+VO_DEF_COMPACT(vo_compact_i32, int32_t)
+
+/* ------------------------------------------------------------------------ */
+/* RadixSort<u64>::Sort(data, indices, count), ops/sort.hpp:83-160: eight   */
+/* LSD passes over bytes 0..7: histogram, exclusive prefix, stable scatter  */
+/* of (key, payload).  Unsigned keys: no sign pass.                         */
+/* ------------------------------------------------------------------------ */
+void vo_radix_sort_u64(uint64_t* keys, uint32_t* idx, size_t n) {
+    if (n <= 1) return;
+    uint64_t* tk = (uint64_t*)malloc(n * 8);
+    uint32_t* ti = (uint32_t*)malloc(n * 4);
+    uint64_t *sk = keys, *dk = tk; uint32_t *si = idx, *di = ti;
+    for (int b = 0; b < 8; ++b) {
+        size_t hist[256] = {0};
+        int sh = 8 * b;
+        for (size_t i = 0; i < n; ++i) ++hist[(sk[i] >> sh) & 0xFF];
+        size_t pre[256]; pre[0] = 0;
+        for (int d = 1; d < 256; ++d) pre[d] = pre[d - 1] + hist[d - 1];
+        for (size_t i = 0; i < n; ++i) {
+            size_t p = pre[(sk[i] >> sh) & 0xFF]++;
+            dk[p] = sk[i]; di[p] = si[i];
+        }
+        uint64_t* t1 = sk; sk = dk; dk = t1;
+        uint32_t* t2 = si; si = di; di = t2;
+    }
+    /* eight passes: the data is back in the caller's buffers */
+    free(tk); free(ti);
+}
+
+/* MergeSort<f64>::Sort(data, idx, n) + SortOperator (ops/sort.hpp:196-213, */
+/* 303-338, 506-541): bottom-up stable merge, "<=" takes left (ascending),  */
+/* ">=" takes left (descending).  Writes the permutation of 0..n-1.         */
+void vo_mergesort_f64(const double* data, size_t n, int descending, uint32_t* idx_out) {
+    for (size_t i = 0; i < n; ++i) idx_out[i] = (uint32_t)i;
+    if (n <= 1) return;
+    double* v = (double*)malloc(n * 8); memcpy(v, data, n * 8);
+    double* tv = (double*)malloc(n * 8); uint32_t* ti = (uint32_t*)malloc(n * 4);
+    for (size_t w = 1; w < n; w *= 2) {
+        for (size_t left = 0; left < n; left += 2 * w) {
+            size_t mid = left + w < n ? left + w : n, right = left + 2 * w < n ? left + 2 * w : n;
+            if (mid >= right) continue;
+            size_t ll = mid - left;
+            memcpy(tv, v + left, ll * 8); memcpy(ti, idx_out + left, ll * 4);
+            size_t a = 0, b = mid, o = left;
+            while (a < ll && b < right) {
+                int takeLeft = descending ? (tv[a] >= v[b]) : (tv[a] <= v[b]);
+                if (takeLeft) { v[o] = tv[a]; idx_out[o++] = ti[a++]; }
+                else { v[o] = v[b]; idx_out[o++] = idx_out[b++]; }
+            }
+            while (a < ll) { v[o] = tv[a]; idx_out[o++] = ti[a++]; }
+        }
+    }
+    free(v); free(tv); free(ti);
+}
+
+/* ------------------------------------------------------------------------ */
+/* HashValue<i64> (ops/hash.hpp:328-333): the splitmix64 finaliser.          */
+/* HashJoin<i64> (ops/join.hpp:36-80,150-165): slots = nextpow2(max(2n,64)), */
+/* chains pushed at the head (so the first equal key met by Probe is the    */
+/* highest build row id with that key); Probe emits (i, row) for the first  */
+/* equal key in probe order.  Also returns the wrapping i64 sum of          */
+/* payload[row] over the matches (the join+SUM harness of the configs).     */
+/* ------------------------------------------------------------------------ */
+uint64_t vo_hash_i64(int64_t key) {
+    uint64_t x = (uint64_t)key;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+static size_t vo_next_pow2(size_t v) {
+    v--; v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16; v |= v >> 32; return v + 1;
+}
+
+uint64_t vo_hashjoin_i64(const int64_t* build, size_t nb, const int64_t* payload,
+                         const int64_t* probe, size_t np,
+                         uint32_t* left_out, uint32_t* right_out, int64_t* payload_sum) {
+    size_t slots = vo_next_pow2((nb * 2) > 64 ? nb * 2 : 64);
+    int64_t* head = (int64_t*)malloc(slots * 8);
+    int64_t* next = (int64_t*)malloc((nb ? nb : 1) * 8);
+    for (size_t s = 0; s < slots; ++s) head[s] = -1;
+    for (size_t i = 0; i < nb; ++i) {
+        size_t s = (size_t)(vo_hash_i64(build[i]) & (slots - 1));
+        next[i] = head[s]; head[s] = (int64_t)i;
+    }
+    uint64_t m = 0, sum = 0;
+    for (size_t i = 0; i < np; ++i) {
+        size_t s = (size_t)(vo_hash_i64(probe[i]) & (slots - 1));
+        for (int64_t e = head[s]; e >= 0; e = next[e]) {
+            if (build[e] == probe[i]) {
+                if (left_out) left_out[m] = (uint32_t)i;
+                if (right_out) right_out[m] = (uint32_t)e;
+                if (payload) sum += (uint64_t)payload[e];
+                ++m;
+                break;
+            }
+        }
+    }
+    free(head); free(next);
+    if (payload_sum) *payload_sum = (int64_t)sum;
+    return m;
+}
+
+/* ------------------------------------------------------------------------ */
+/* HashAggregator<TKey,f64> per-group state (ops/aggregate.hpp:76-140,       */
+/* 176-239): Sum in row order, Min/Max with strict compares (first of equal */
+/* values kept), Count, Welford Mean/M2.  The table layout only decides the */
+/* output order (slot order, Finalize 249-261); the oracle emits groups in  */
+/* ascending key order.  Also an exact SUM per group (accumulator digits).  */
+/* Keys are i32/u32 bit patterns given as uint32.  Returns group count.     */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint32_t key; uint64_t count; double sum, mn, mx, mean, m2; } vo_group;
+
+static int vo_cmp_u32(const void* a, const void* b) {
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b; return x < y ? -1 : x > y;
+}
+
+size_t vo_groupby_f64(const uint32_t* keys, const double* vals, size_t n, int keys_signed,
+                      size_t max_groups, uint32_t* out_key, uint64_t* out_count,
+                      double* out_sum, double* out_min, double* out_max,
+                      double* out_mean, double* out_m2, int64_t* out_digits) {
+    /* distinct keys, sorted */
+    uint32_t* uk = (uint32_t*)malloc((n ? n : 1) * 4);
+    memcpy(uk, keys, n * 4);
+    for (size_t i = 0; i < n && keys_signed; ++i) uk[i] ^= 0x80000000u;
+    qsort(uk, n, 4, vo_cmp_u32);
+    size_t g = 0;
+    for (size_t i = 0; i < n; ++i) if (i == 0 || uk[i] != uk[i - 1]) uk[g++] = uk[i];
+    if (g > max_groups) { free(uk); return g; }
+    vo_group* st = (vo_group*)calloc(g ? g : 1, sizeof(vo_group));
+    vo_acc* acc = (vo_acc*)calloc(g ? g : 1, sizeof(vo_acc));
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t k = keys_signed ? (keys[i] ^ 0x80000000u) : keys[i];
+        size_t lo = 0, hi = g;
+        while (lo + 1 < hi) { size_t mid = (lo + hi) / 2; if (uk[mid] <= k) lo = mid; else hi = mid; }
+        vo_group* s = &st[lo];
+        double v = vals[i];
+        if (s->count == 0) { s->sum = s->mn = s->mx = v; s->count = 1; s->mean = v; s->m2 = 0.0; }
+        else {
+            s->sum += v;
+            if (v < s->mn) s->mn = v;
+            if (v > s->mx) s->mx = v;
+            s->count++;
+            double d = v - s->mean; s->mean += d / (double)s->count; s->m2 += d * (v - s->mean);
+        }
+        vo_acc_add(&acc[lo], v);
+    }
+    for (size_t j = 0; j < g; ++j) {
+        out_key[j] = keys_signed ? (uk[j] ^ 0x80000000u) : uk[j];
+        out_count[j] = st[j].count; out_sum[j] = st[j].sum; out_min[j] = st[j].mn;
+        out_max[j] = st[j].mx; out_mean[j] = st[j].mean; out_m2[j] = st[j].m2;
+        if (out_digits) vo_acc_export(&acc[j], out_digits + j * VO_ACC_DIGITS);
+    }
+    free(uk); free(st); free(acc);
+    return g;
+}
+
+/* splitmix64 (Steele, Lea, Flood 2014), used for the synthetic columns.     */
+uint64_t vo_splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}

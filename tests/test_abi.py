@@ -1,0 +1,175 @@
+"""C-ABI surface checks that need no GPU: the library loads, exports every symbol
+include/voxel_c.h declares (the 64 of the reference's bindings/voxel_c.h among
+them), encodes instructions like the reference, lowers the reference's programs
+to the right plan kinds, and the Python surfaces emit the reference's words.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+from voxel_amd import _lib
+from voxel_amd._lib import lib
+
+REF_ABI = """voxel_engine_create_f64 voxel_engine_destroy voxel_engine_add_segment voxel_engine_load_program
+voxel_engine_run voxel_engine_run_parallel voxel_engine_set_scalar voxel_engine_get_scalar
+voxel_engine_set_scalar_f64 voxel_engine_get_scalar_f64 voxel_engine_k_lanes voxel_engine_create_i64
+voxel_engine_destroy_i64 voxel_engine_add_segment_i64 voxel_engine_load_program_i64 voxel_engine_run_i64
+voxel_engine_set_scalar_i64 voxel_engine_get_scalar_i64 voxel_jit_run voxel_instr_vload voxel_instr_vfilter_gt
+voxel_instr_vfilter_ge voxel_instr_vfilter_lt voxel_instr_vfilter_le voxel_instr_vsum voxel_instr_vred_min
+voxel_instr_vred_max voxel_instr_vcount voxel_instr_addf voxel_instr_add voxel_instr_sub voxel_instr_mul
+voxel_instr_mov voxel_instr_movr voxel_instr_cmp voxel_instr_jnz voxel_instr_jz voxel_instr_jmp voxel_instr_halt
+voxel_null_create voxel_null_destroy voxel_null_is_null voxel_null_set_null voxel_null_set_valid
+voxel_null_null_count voxel_null_valid_count voxel_agg_create voxel_agg_destroy voxel_agg_init
+voxel_agg_accumulate voxel_agg_group_count voxel_sort_ascending voxel_sort_descending voxel_topk
+voxel_dict_create voxel_dict_destroy voxel_dict_build voxel_dict_cardinality voxel_dict_decode_batch
+voxel_pool_create voxel_pool_destroy voxel_pool_wait_all voxel_pool_thread_count voxel_last_error""".split()
+
+
+def test_reference_abi_is_64_symbols_and_all_exported():
+    assert len(REF_ABI) == 64
+    declared = _lib.header_symbols()
+    for s in REF_ABI:
+        assert s in declared, s
+    for s in declared:
+        assert hasattr(lib, s), "declared but not exported: " + s
+
+
+def words(golden, name):
+    return [int(w, 16) for w in golden["programs"][name]]
+
+
+def test_instruction_encoders_match_reference_words(golden):
+    fast = words(golden, "fast")
+    got = [lib.voxel_instr_vload(0, 1, 0, 0), lib.voxel_instr_vfilter_gt(1, 0, 3), lib.voxel_instr_vsum(5, 1),
+           lib.voxel_instr_addf(0, 0, 5), lib.voxel_instr_add(1, 1, 4), lib.voxel_instr_cmp(1, 2),
+           lib.voxel_instr_jnz(-6), lib.voxel_instr_halt()]
+    assert got == fast
+    assert lib.voxel_instr_mov(4, 4) == words(golden, "vq_sum")[0]
+    assert lib.voxel_instr_vcount(5, 1) == words(golden, "vq_count")[3]
+    assert lib.voxel_instr_vred_min(5, 1) == words(golden, "vq_min")[3]
+    assert lib.voxel_instr_vred_max(5, 1) == words(golden, "vq_max")[3]
+    assert lib.voxel_instr_movr(0, 5) == words(golden, "vq_min")[6]
+    if o.R is not None:
+        rng = np.random.default_rng(0)
+        for _ in range(200):
+            rd, ra, rb = (int(v) for v in rng.integers(0, 16, 3))
+            imm = int(rng.integers(-2048, 2048))
+            assert lib.voxel_instr_sub(rd, ra, rb) == o.R.vr_encode(0x21, rd, ra, rb, 0)
+            assert lib.voxel_instr_mul(rd, ra, rb) == o.R.vr_encode(0x22, rd, ra, rb, 0)
+            assert lib.voxel_instr_mov(rd, imm) == o.R.vr_encode(0x10, rd, 0, 0, imm & 0xFFF)
+            assert lib.voxel_instr_jz(imm) == o.R.vr_encode(0x51, 0, 0, 0, imm & 0xFFF)
+            assert lib.voxel_instr_jmp(imm) == o.R.vr_encode(0x50, 0, 0, 0, imm & 0xFFF)
+            assert lib.voxel_instr_vfilter_lt(rd, ra, rb) == o.R.vr_encode(0xC3, rd, ra, rb, 0)
+            assert lib.voxel_instr_vfilter_le(rd, ra, rb) == o.R.vr_encode(0xC4, rd, ra, rb, 0)
+            assert lib.voxel_instr_vfilter_ge(rd, ra, rb) == o.R.vr_encode(0xC6, rd, ra, rb, 0)
+
+
+def lowered(code, i64=False):
+    e = lib.voxel_engine_create_i64() if i64 else lib.voxel_engine_create_f64()
+    c = np.array(code, np.uint32)
+    (lib.voxel_engine_load_program_i64 if i64 else lib.voxel_engine_load_program)(e, c.ctypes.data, c.size)
+    k = lib.voxel_gpu_engine_lowered(e)
+    (lib.voxel_engine_destroy_i64 if i64 else lib.voxel_engine_destroy)(e)
+    return k
+
+
+def test_lowering_classifies_reference_programs(golden):
+    assert lowered(words(golden, "fast")) == 1                       # DetectFastPath matches
+    for k in ("vq_sum", "vq_count", "vq_min", "vq_max", "vq_sum_lt"):
+        assert lowered(words(golden, k)) == 2, k                    # MOV prefix -> Tier-0 loop
+    assert lowered(words(golden, "i64_add"), i64=True) == 2
+    # the fast-path matcher ignores what follows word 6 (engine.hpp:84-107)
+    assert lowered(words(golden, "fast")[:7] + [0x00000002]) == 1
+    # programs that are not scan loops are refused
+    assert lowered([0x00000001]) == 0
+    assert lowered(words(golden, "vq_sum")[:-2] + [lib.voxel_instr_jnz(-5), 1]) == 0
+    bad = words(golden, "vq_sum")
+    bad[4] = lib.voxel_instr_addf(0, 0, 0)                          # accumulate reads acc twice
+    assert lowered(bad) == 0
+
+
+def test_jit_lowering_matches_fusion_detector(golden):
+    def ok(code):
+        c = np.array(code, np.uint32)
+        return lib.voxel_gpu_jit_lowerable(c.ctypes.data, c.size)
+    assert ok(words(golden, "fast")) == 1
+    assert ok(words(golden, "vq_sum")) == 1                         # MOV prefix still fuses in the JIT
+    assert ok(words(golden, "vq_count")) == 0                       # VCOUNT never fuses
+    assert "JIT lowering" in _lib.last_error()
+    assert ok(words(golden, "vq_sum_lt")) == 0                      # generic VFILTER never fuses
+
+
+def test_voxel_query_emits_reference_words(golden):
+    from voxel_amd import voxel_query as vq
+    x = np.arange(8, dtype=np.float64)
+    for name, build in (("vq_sum", lambda q: q.filter_gt(1).sum()), ("vq_count", lambda q: q.filter_gt(1).count()),
+                        ("vq_min", lambda q: q.filter_gt(1).min()), ("vq_max", lambda q: q.filter_gt(1).max()),
+                        ("vq_sum_lt", lambda q: q.filter_lt(1).sum())):
+        q = build(vq.from_numpy(x))
+        code = [vq._vx.Instruction.mov(4, 4).raw] + q.code + [1]
+        assert code == words(golden, name), name
+
+
+def test_voxel_py_instruction_words(golden):
+    from voxel_amd.voxel_py import Instruction as I
+    assert I.vfilter(1, 0, 3, 2).raw == words(golden, "vq_sum_lt")[2]
+    assert I.jle(2).raw == words(golden, "vq_min")[5] and I.jge(2).raw == words(golden, "vq_max")[5]
+    assert I.jnz(-8).raw == words(golden, "vq_min")[9]
+
+
+def test_host_utilities_match_reference_behaviour():
+    nb = lib.voxel_null_create(64)                                  # c_test.cpp Test 3
+    for r in (10, 20, 30):
+        lib.voxel_null_set_null(nb, r)
+    assert lib.voxel_null_null_count(nb) == 3 and lib.voxel_null_valid_count(nb) == 61
+    assert lib.voxel_null_is_null(nb, 20) == 1 and lib.voxel_null_is_null(nb, 21) == 0
+    lib.voxel_null_set_valid(nb, 20)
+    assert lib.voxel_null_null_count(nb) == 2
+    lib.voxel_null_destroy(nb)
+    nb = lib.voxel_null_create(70)
+    assert lib.voxel_null_null_count(nb) == 0 and lib.voxel_null_valid_count(nb) == 70
+    lib.voxel_null_destroy(nb)
+    d = lib.voxel_dict_create()
+    x = np.array([3.0, 1.0, 3.0, -0.0, 2.0, 0.0, 1.0], np.float64)
+    lib.voxel_dict_build(d, x.ctypes.data, x.size)
+    assert lib.voxel_dict_cardinality(d) == 4
+    out = np.zeros(x.size, np.float64)
+    lib.voxel_dict_decode_batch(d, out.ctypes.data, 0, x.size)
+    assert out.tolist() == [3.0, 1.0, 3.0, -0.0, 2.0, -0.0, 1.0]     # equal values share the first one
+    lib.voxel_dict_destroy(d)
+    p = lib.voxel_pool_create(3)
+    assert lib.voxel_pool_thread_count(p) == 3
+    lib.voxel_pool_wait_all(p)
+    lib.voxel_pool_destroy(p)
+    assert lib.voxel_engine_k_lanes() == 4
+
+
+def test_engine_registers_are_host_state():
+    e = lib.voxel_engine_create_f64()
+    lib.voxel_engine_set_scalar_f64(e, 3, 500.0)
+    assert lib.voxel_engine_get_scalar(e, 3) == o.f64_bits(500.0)
+    lib.voxel_engine_set_scalar(e, 64 + 7, 9)                       # index masked to 0x3F
+    assert lib.voxel_engine_get_scalar(e, 7) == 9
+    lib.voxel_engine_destroy(e)
+
+
+def test_no_gpu_means_loud_failure():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from voxel_amd import voxel_py as vx
+    e = vx.EngineF64()
+    x = np.arange(8, dtype=np.float64)
+    e.add_segment(x)
+    e.set_scalar(2, 8)
+    e.load_program([lib.voxel_instr_vload(0, 1, 0, 0), lib.voxel_instr_vfilter_gt(1, 0, 3),
+                    lib.voxel_instr_vsum(5, 1), lib.voxel_instr_addf(0, 0, 5), lib.voxel_instr_add(1, 1, 4),
+                    lib.voxel_instr_cmp(1, 2), lib.voxel_instr_jnz(-6), lib.voxel_instr_halt()])
+    with pytest.raises(RuntimeError, match="no usable HIP device"):
+        e.run()
+    assert e.get_scalar(0) == 0 and e.get_scalar(1) == 0            # registers untouched

@@ -1,0 +1,353 @@
+"""Pins the CPU oracle (oracle/voxel_oracle.c) before it is trusted as the checker.
+
+Each restated function is checked against (a) the golden vectors produced by the
+reference's own C++ (tests/golden/reference_vectors.json, made by
+tests/golden/make_golden.py from oracle/_ref), (b) the reference build itself
+where it is present, and (c) what mathematics fixes: exact rational sums,
+correctly rounded fsum, library stable sorts, brute-force joins and groupings.
+CPU only.
+"""
+from __future__ import annotations
+
+import math
+import random
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+
+B = lambda h: int(h, 16)  # noqa: E731
+
+
+def test_mt19937_64_matches_reference_examples(golden, x42):
+    g = golden["data_f64_seed42"]
+    assert [int(b) for b in x42[:3].view(np.uint64)] == [B(h) for h in g["first"]]
+    assert x42[0] == 755.15553295453901 and x42[1] == 639.0313938546974
+    xi = o.mt_uniform_i64(7, 5, 0, 1000)
+    assert xi[:3].tolist() == golden["i64_seed7"]["first"]
+    x32 = o.mt_uniform_i32(9, 5, 0, 1 << 30)
+    assert x32[:3].tolist() == golden["i32_seed9"]["first"]
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_mt_generators_match_libstdcxx_stream():
+    for seed in (1, 42, 12345):
+        a = o.mt_uniform_f64(seed, 20000, -3.0, 7.5)
+        b = np.empty_like(a)
+        o.R.vr_mt_uniform_f64(seed, a.size, -3.0, 7.5, o.ptr(b))
+        assert np.array_equal(a, b)
+        ai = o.mt_uniform_i64(seed, 20000, -5, 1 << 40)
+        bi = np.empty_like(ai)
+        o.R.vr_mt_uniform_i64(seed, ai.size, -5, 1 << 40, o.ptr(bi))
+        assert np.array_equal(ai, bi)
+        a3 = o.mt_uniform_i32(seed, 20000, -7, 12345)
+        b3 = np.empty_like(a3)
+        o.R.vr_mt_uniform_i32(seed, a3.size, -7, 12345, o.ptr(b3))
+        assert np.array_equal(a3, b3)
+
+
+def test_exact_accumulator_against_fractions():
+    rng = random.Random(5)
+    for trial in range(30):
+        vals = []
+        for _ in range(rng.randint(1, 60)):
+            e = rng.choice([rng.randint(-1074, -1000), rng.randint(-60, 60), rng.randint(900, 1023)])
+            m = rng.random() * rng.choice([-1, 1])
+            vals.append(math.ldexp(m, e))
+        vals += [5e-324, -5e-324, 1.7976931348623157e308, -1.7976931348623157e308]
+        x = np.array(vals, np.float64)
+        exact = sum((Fraction(v) for v in vals), Fraction(0))
+        assert o.exact_sum(x) == exact
+    # catastrophic cancellation
+    assert o.exact_sum(np.array([1e300, 1.0, -1e300, 2.0**-1074])) == 1 + Fraction(1, 2**1074)
+
+
+def test_exact_accumulator_rounds_like_fsum():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(200_000) * np.exp(rng.uniform(-30, 30, 200_000))
+    assert float(o.exact_sum(x)) == math.fsum(x.tolist())
+
+
+def test_filter_count_and_exact_sum_golden(x42):
+    r = o.relational(x42, o.CMP["GT"], 500.0)
+    assert r["count"] == 500_720                          # docs/Benchmarks.md:7
+    assert r["count"] == int(np.count_nonzero(x42 > 500.0))
+    assert abs(float(r["sum"]) - 375_538_740.218) < 1e-3
+    assert float(r["sum"]) == math.fsum(x42[x42 > 500.0].tolist())
+    assert r["min"] == x42[x42 > 500].min() and r["max"] == 999.99852628798419
+
+
+def test_relational_compound_against_numpy():
+    rng = np.random.default_rng(11)
+    for dt in (np.float64, np.float32, np.int64, np.int32):
+        if np.dtype(dt).kind == "f":
+            a = (rng.random(50_000) * 1000).astype(dt)
+            b = (rng.random(50_000) * 1000).astype(dt)
+            a[::97] = np.nan
+        else:
+            a = rng.integers(0, 2**31, 50_000).astype(dt)
+            b = rng.integers(0, 2**31, 50_000).astype(dt)
+        ta, tb = dt(a[7]) if not np.isnan(a[7]) else dt(300), dt(b[9])
+        for opa, fa in ((4, np.greater), (1, np.not_equal), (3, np.less_equal)):
+            m = fa(a, ta) & (b < tb)
+            r = o.relational(a, opa, ta, b, 2, tb)
+            assert r["count"] == int(m.sum())
+            sel = a[m]
+            if np.dtype(dt).kind == "f":
+                if not np.isnan(sel).any():
+                    assert r["sum"] == sum((Fraction(float(v)) for v in sel), Fraction(0))
+                fin = sel[~np.isnan(sel)]
+                assert (r["min"] is None) == (fin.size == 0)
+                if fin.size:
+                    assert r["min"] == fin.min() and r["max"] == fin.max()
+            else:
+                assert r["sum"] == int(sel.astype(np.int64).sum())
+                if sel.size:
+                    assert r["min"] == sel.min() and r["max"] == sel.max()
+
+
+def _fast(x, total, off=0, thr=500.0, acc0=0.0, fop=0xC5):
+    acc = np.zeros(1, np.float64)
+    end = o.O.vo_fastpath_f64(o.ptr(x), x.size, total, off, thr, acc0, fop, o.ptr(acc))
+    return int(acc.view(np.uint64)[0]), int(end)
+
+
+def test_fastpath_restatement_is_bit_exact(golden, x42):
+    g = golden["seed42"]
+    bits, end = _fast(x42, x42.size)
+    assert bits == B(g["fast"]["R0"]) and end == g["fast"]["R1"]
+    # a plain sequential sum differs in the last bits: the golden value pins the order
+    assert o.f64_bits(float(np.cumsum(np.where(x42 > 500, x42, 0.0))[-1])) != bits
+    # tail of 1003 rows, every row passes, 3 rows dropped
+    t = x42[:1003].copy()
+    bits, end = _fast(t, 1003, thr=0.0)
+    assert bits == B(g["tail_1003"]["R0"]) and end == 1000
+    # literal programs (test_suite.cpp:254-285, c_test.cpp:8-36)
+    d8 = np.arange(1, 9, dtype=np.float64)
+    assert o.bits_f64(_fast(d8, 8, thr=4.0)[0]) == 26.0 == golden["literals"]["engine_f64_fast"]
+
+
+def test_fastpath_i32_wraps(golden):
+    g = golden["i32_seed9"]
+    x = o.mt_uniform_i32(9, g["n"], 0, 1 << 30)
+    acc = np.zeros(1, np.int32)
+    end = o.O.vo_fastpath_i32(o.ptr(x), x.size, x.size, 0, 1 << 29, 0, 0xC5, o.ptr(acc))
+    assert (int(acc[0]) & 0xFFFFFFFFFFFFFFFF) == B(g["R0"]) and end == g["R1"]
+    exact = int(x[x > (1 << 29)].astype(np.int64).sum())
+    assert ((exact + 2**31) % 2**32) - 2**31 == int(acc[0])          # sum mod 2^32, sign-extended
+
+
+def _tier0(x, red, accop, filt=4, thr=500.0, total=None, off=0):
+    total = x.size if total is None else total
+    part = np.zeros(1, np.uint64)
+    fn = o.O.vo_tier0_f64 if x.dtype == np.float64 else o.O.vo_tier0_i64
+    r = fn(o.ptr(x), x.size, off, total, filt, thr, 0, red, accop, 0, o.ptr(part))
+    return int(r), int(part[0])
+
+
+def test_tier0_voxel_query_shapes(golden, x42):
+    g = golden["seed42"]
+    assert _tier0(x42, 0, 0) == (B(g["vq_sum"]["R0"]), B(g["vq_sum"]["R5"]))
+    assert _tier0(x42, 1, 1) == (B(g["vq_count"]["R0"]), B(g["vq_count"]["R5"]))
+    assert B(g["vq_count"]["R0"]) == 500_720
+    assert _tier0(x42, 2, 2) == (B(g["vq_min"]["R0"]), B(g["vq_min"]["R5"]))
+    assert _tier0(x42, 3, 3) == (B(g["vq_max"]["R0"]), B(g["vq_max"]["R5"]))
+    assert _tier0(x42, 0, 0, filt=2) == (B(g["vq_sum_lt"]["R0"]), B(g["vq_sum_lt"]["R5"]))
+    # Tier-0 sum equals the fast path's bits (same group order)
+    assert B(g["vq_sum"]["R0"]) == B(g["fast"]["R0"])
+
+
+def test_tier0_i64_add_variant(golden):
+    g = golden["i64_seed7"]
+    x = o.mt_uniform_i64(7, g["n"], 0, 1000)
+    assert _tier0(x, 0, 1, thr=500)[0] == g["R0"] == int(x[x > 500].sum())
+    d = np.array([10, 20, 30, 40], np.int64)
+    assert _tier0(d, 0, 1, thr=15)[0] == 90 == golden["literals"]["engine_i64"]
+
+
+def test_tier0_minmax_fold_is_a_bit_compare():
+    # negative values: the i64 compare of f64 bits is not the f64 order (Q10)
+    x = np.array([3.0, -1.0, 2.0, 5.0, -4.0, 7.0, 1.0, 0.5], np.float64)
+    r_max, _ = _tier0(x, 3, 3, filt=6)
+    r_min, _ = _tier0(x, 2, 2, filt=6)
+    # group maxima 5.0, 7.0 (bits positive), acc starts 0.0 -> 7.0
+    assert o.bits_f64(r_max) == 7.0
+    # group minima -1.0, -4.0: from 0.0, CMP 0 - bits(-1.0) > 0 -> take -1.0;
+    # then bits(-1.0) - bits(-4.0) < 0 (as i64) -> keep -1.0
+    assert o.bits_f64(r_min) == -1.0
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_tier0_against_reference_random_programs():
+    rng = np.random.default_rng(21)
+    progs = {}
+    import json, os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+    for k in ("vq_sum", "vq_count", "vq_min", "vq_max"):
+        progs[k] = np.array([int(w, 16) for w in g["programs"][k]], np.uint32)
+    kinds = {"vq_sum": (0, 0), "vq_count": (1, 1), "vq_min": (2, 2), "vq_max": (3, 3)}
+    for trial in range(12):
+        n = int(rng.integers(1, 400)) * 4
+        x = rng.standard_normal(n) * 100
+        x[rng.integers(0, n, 5)] = 0.0
+        if trial % 3 == 0:
+            x[rng.integers(0, n, 3)] = np.nan
+        thr = float(rng.standard_normal() * 50)
+        for k, (red, accop) in kinds.items():
+            regs = np.zeros(64, np.uint64)
+            regs[2], regs[3] = n, o.f64_bits(thr)
+            o.R.vr_engine_run_f64(o.ptr(x), n, o.ptr(progs[k]), progs[k].size, o.ptr(regs))
+            r, part = _tier0(x, red, accop, thr=thr)
+            assert (r, part) == (int(regs[0]), int(regs[5])), (trial, k)
+
+
+def test_jit_restatement(golden, x42):
+    g = golden["seed42"]
+    r = o.O.vo_jit_f64(o.ptr(x42), x42.size, x42.size, 0, 0xC5, 500.0)
+    assert o.f64_bits(r) == B(g["jit"]["result"])
+    r = o.O.vo_jit_f64(o.ptr(x42), 999992, 999992, 0, 0xC5, 500.0)
+    assert o.f64_bits(r) == B(g["jit_999992"]["result"])
+
+
+def test_run_parallel_restatement(golden, x42):
+    g = golden["seed42"]
+    for t in (8, 7):
+        r = o.O.vo_run_parallel_fast_f64(o.ptr(x42), x42.size, t, x42.size, 0, 500.0, 0.0, 0xC5)
+        assert o.f64_bits(r) == B(g["run_parallel_%d" % t]["R0"])
+
+
+def test_bitmap_and_selection(golden):
+    g = golden["bitmap_gt4"]
+    x = np.array(g["data"], np.float64)
+    w = np.zeros(1, np.uint64)
+    assert o.O.vo_bitmap_f64(o.ptr(x), x.size, 4, 4.0, 0.0, o.ptr(w)) == g["pass"] == 4
+    assert int(w[0]) == g["word"] == (1 << 2) | (1 << 3) | (1 << 6) | (1 << 7)
+    rng = np.random.default_rng(2)
+    a = rng.integers(0, 100, 10_001).astype(np.int32)
+    nw = (a.size + 63) // 64
+    wa, wb = np.zeros(nw, np.uint64), np.zeros(nw, np.uint64)
+    o.O.vo_bitmap_i32(o.ptr(a), a.size, 2, 30, 0, o.ptr(wa))
+    o.O.vo_bitmap_i32(o.ptr(a), a.size, 5, 10, 0, o.ptr(wb))
+    bits = np.unpackbits(wa.view(np.uint8), bitorder="little")[:a.size].astype(bool)
+    assert np.array_equal(bits, a < 30)
+    cnt = o.O.vo_bitmap_combine(o.ptr(wa), o.ptr(wb), a.size, 0)
+    m = (a < 30) & (a >= 10)
+    assert cnt == m.sum()
+    idx = np.zeros(a.size, np.uint32)
+    k = o.O.vo_select_from_bitmap(o.ptr(wa), a.size, o.ptr(idx))
+    assert np.array_equal(idx[:k], np.nonzero(m)[0])
+    if o.R is not None:
+        wr = np.zeros(nw, np.uint64)
+        assert o.R.vr_bitmap_i32(o.ptr(a), a.size, 2, 30, 0, o.ptr(wr)) == (a < 30).sum()
+        wr2 = np.zeros(nw, np.uint64)
+        o.R.vr_bitmap_i32(o.ptr(a), a.size, 5, 10, 0, o.ptr(wr2))
+        assert o.R.vr_column_filter(o.ptr(wr), o.ptr(wr2), a.size, 0) == cnt
+        assert np.array_equal(wr, wa)
+
+
+def test_radix_sort_restatement(golden):
+    g = golden["radix_u64"]
+    k = np.array(g["in"], np.uint64)
+    i = np.arange(k.size, dtype=np.uint32)
+    o.O.vo_radix_sort_u64(o.ptr(k), o.ptr(i), k.size)
+    assert k.tolist() == g["keys"] and i.tolist() == g["idx"]
+    rng = np.random.default_rng(4)
+    for gen in (lambda n: rng.integers(0, 2**64, n, dtype=np.uint64),
+                lambda n: rng.integers(0, 50, n).astype(np.uint64)):
+        keys = gen(20_000)
+        kk, ii = keys.copy(), np.arange(keys.size, dtype=np.uint32)
+        o.O.vo_radix_sort_u64(o.ptr(kk), o.ptr(ii), kk.size)
+        ref = np.argsort(keys, kind="stable")
+        assert np.array_equal(ii, ref) and np.array_equal(kk, keys[ref])
+        if o.R is not None:
+            k2, i2 = keys.copy(), np.arange(keys.size, dtype=np.uint32)
+            o.R.vr_radix_sort_u64(o.ptr(k2), o.ptr(i2), k2.size)
+            assert np.array_equal(i2, ii)
+
+
+def test_mergesort_restatement(golden):
+    for name in ("sort_c_test", "sort_ties"):
+        g = golden[name]
+        x = np.array(g["data"], np.float64)
+        for desc, key in ((0, "asc"), (1, "desc")):
+            idx = np.zeros(x.size, np.uint32)
+            o.O.vo_mergesort_f64(o.ptr(x), x.size, desc, o.ptr(idx))
+            assert idx.tolist() == g[key], (name, key)
+    rng = np.random.default_rng(8)
+    x = np.round(rng.standard_normal(30_000), 2)
+    x[::50] = -0.0
+    idx = np.zeros(x.size, np.uint32)
+    o.O.vo_mergesort_f64(o.ptr(x), x.size, 0, o.ptr(idx))
+    assert np.array_equal(idx, np.argsort(x, kind="stable"))
+    o.O.vo_mergesort_f64(o.ptr(x), x.size, 1, o.ptr(idx))
+    assert np.array_equal(idx, np.argsort(-x, kind="stable"))
+
+
+def test_hash_join_restatement(golden):
+    for name in ("join_small", "join_dups"):
+        g = golden[name]
+        b, p, pay = (np.array(g[k], np.int64) for k in ("build", "probe", "payload"))
+        lo, ro = np.zeros(p.size, np.uint32), np.zeros(p.size, np.uint32)
+        s = np.zeros(1, np.int64)
+        m = o.O.vo_hashjoin_i64(o.ptr(b), b.size, o.ptr(pay), o.ptr(p), p.size, o.ptr(lo), o.ptr(ro), o.ptr(s))
+        assert m == g["matches"] and lo[:m].tolist() == g["left"] and ro[:m].tolist() == g["right"]
+        assert int(s[0]) == g["payload_sum"]
+    # brute force: the first equal key on a chain is the highest build row
+    rng = np.random.default_rng(6)
+    b = rng.integers(0, 3000, 5000).astype(np.int64)
+    p = rng.integers(0, 4000, 20000).astype(np.int64)
+    pay = rng.integers(-2**62, 2**62, b.size).astype(np.int64)
+    last = {}
+    for i, k in enumerate(b.tolist()):
+        last[k] = i
+    exp = [(i, last[k]) for i, k in enumerate(p.tolist()) if k in last]
+    lo, ro = np.zeros(p.size, np.uint32), np.zeros(p.size, np.uint32)
+    s = np.zeros(1, np.int64)
+    m = o.O.vo_hashjoin_i64(o.ptr(b), b.size, o.ptr(pay), o.ptr(p), p.size, o.ptr(lo), o.ptr(ro), o.ptr(s))
+    assert list(zip(lo[:m].tolist(), ro[:m].tolist())) == exp
+    tot = sum(int(pay[r]) for _, r in exp)
+    assert int(s[0]) == ((tot + 2**63) % 2**64) - 2**63
+
+
+def _groupby(keys, vals, signed=True):
+    n = keys.size
+    G = 4096
+    ok, oc = np.zeros(G, np.uint32), np.zeros(G, np.uint64)
+    osum, omin, omax, omean, om2 = (np.zeros(G, np.float64) for _ in range(5))
+    dig = np.zeros(G * o.NDIG, np.int64)
+    g = o.O.vo_groupby_f64(o.ptr(keys.view(np.uint32)), o.ptr(vals), n, 1 if signed else 0, G, o.ptr(ok),
+                           o.ptr(oc), o.ptr(osum), o.ptr(omin), o.ptr(omax), o.ptr(omean), o.ptr(om2), o.ptr(dig))
+    return g, ok[:g].view(np.int32 if signed else np.uint32), oc[:g], osum[:g], omin[:g], omax[:g], omean[:g], om2[:g], dig
+
+
+def test_groupby_restatement(golden):
+    for name in ("agg_small", "agg_c_test"):
+        gg = golden[name]
+        keys, vals = np.array(gg["keys"], np.int32), np.array(gg["vals"], np.float64)
+        g, k, c, s, mn, mx, mean, m2, _ = _groupby(keys, vals)
+        ref = sorted(gg["groups"], key=lambda d: d["key"])
+        assert g == len(ref)
+        for i, d in enumerate(ref):
+            assert (k[i], c[i], s[i], mn[i], mx[i], mean[i], m2[i]) == (
+                d["key"], d["count"], d["sum"], d["min"], d["max"], d["mean"], d["m2"])
+    rng = np.random.default_rng(10)
+    keys = rng.integers(1, 266, 50_000).astype(np.int32)
+    vals = rng.lognormal(2.5, 0.6, keys.size)
+    g, k, c, s, mn, mx, mean, m2, dig = _groupby(keys, vals)
+    for i in range(g):
+        sel = vals[keys == k[i]]
+        assert c[i] == sel.size and mn[i] == sel.min() and mx[i] == sel.max()
+        assert float(o.digits_to_fraction(dig[i * o.NDIG:(i + 1) * o.NDIG])) == math.fsum(sel.tolist())
+    if o.R is not None:
+        G = 512
+        ok, oc = np.zeros(G, np.int32), np.zeros(G, np.uint64)
+        osum, omin, omax, omean, om2 = (np.zeros(G, np.float64) for _ in range(5))
+        ng = o.R.vr_hash_agg_i32_f64(o.ptr(keys), o.ptr(vals), keys.size, 128, G, o.ptr(ok), o.ptr(oc),
+                                     o.ptr(osum), o.ptr(omin), o.ptr(omax), o.ptr(omean), o.ptr(om2))
+        order = np.argsort(ok[:ng])
+        assert np.array_equal(ok[:ng][order], k)
+        assert np.array_equal(oc[:ng][order], c)
+        assert np.array_equal(osum[:ng][order], s)         # same row order: bit-exact
+        assert np.array_equal(omean[:ng][order], mean) and np.array_equal(om2[:ng][order], m2)

@@ -1,0 +1,386 @@
+"""GPU parity of the scan family (vx_scan.hip) through the C ABI.
+
+Bars (stated per test):
+* counts, integer sums, bit-pattern min/max, register effects: bit-exact with
+  the oracle / the reference's golden values;
+* f64 sums: |gpu - exact| <= 2 ulp(exact) against the oracle's exact long
+  accumulator, and within the reference's own 1e-12 relative criterion
+  (examples/validate.cpp:68) of its result;
+* f32 sums: |gpu - exact| <= 4 ulp_f32(exact) (compensated in f32).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+B = lambda h: int(h, 16)  # noqa: E731
+
+
+def close_f64(got: float, exact, ulps: float = 2.0) -> bool:
+    e = float(exact)
+    return abs(got - e) <= ulps * o.ulp(e) + 0.0
+
+
+def words(golden, name):
+    return [int(w, 16) for w in golden["programs"][name]]
+
+
+def engine_f64(x, code, regs):
+    from voxel_amd import voxel_py as vx
+    e = vx.EngineF64()
+    e.add_segment(x)
+    for k, v in regs.items():
+        e.set_scalar(k, v)
+    e.load_program(code)
+    return e
+
+
+# ---------------------------------------------------------------- fast path
+def test_fast_path_f64_golden(gpu, golden, x42):
+    g = golden["seed42"]
+    e = engine_f64(x42, words(golden, "fast"), {2: x42.size, 3: o.f64_bits(500.0)})
+    e.run()
+    exact = o.relational(x42, 4, 500.0)["sum"]
+    got = e.get_scalar_f64(0)
+    ref = o.bits_f64(B(g["fast"]["R0"]))
+    assert close_f64(got, exact), (got, float(exact))
+    assert abs(got - ref) <= 1e-12 * abs(ref)
+    assert e.get_scalar(1) == g["fast"]["R1"] == 1_000_000
+    print("fast path f64: gpu err %.3g ulp, reference err %.3g ulp" % (
+        abs(got - float(exact)) / o.ulp(float(exact)), abs(ref - float(exact)) / o.ulp(float(exact))))
+
+
+def test_fast_path_tail_and_offsets(gpu, golden, x42):
+    t = x42[:1003].copy()
+    e = engine_f64(t, words(golden, "fast"), {2: 1003, 3: o.f64_bits(0.0)})
+    e.run()
+    assert e.get_scalar(1) == 1000                                   # tail of 3 rows dropped
+    assert close_f64(e.get_scalar_f64(0), o.exact_sum(t[:1000]))
+    # nonzero start offset, initial accumulator, R[total] beyond Count
+    e = engine_f64(t, words(golden, "fast"), {0: o.f64_bits(1.5), 1: 5, 2: 10**9, 3: o.f64_bits(500.0)})
+    e.run()
+    acc = np.zeros(1, np.float64)
+    end = o.O.vo_fastpath_f64(o.ptr(t), t.size, 10**9, 5, 500.0, 1.5, 0xC5, o.ptr(acc))
+    assert e.get_scalar(1) == end == 1001
+    sel = t[5:1001]
+    assert close_f64(e.get_scalar_f64(0), o.exact_sum(np.concatenate([[1.5], sel[sel > 500]])))
+    # fewer than kL rows: no group runs, registers keep their values
+    e = engine_f64(t[:3].copy(), words(golden, "fast"), {0: o.f64_bits(-0.0), 2: 3, 3: o.f64_bits(0.0)})
+    e.run()
+    assert e.get_scalar(0) == o.f64_bits(-0.0) and e.get_scalar(1) == 0
+
+
+def test_fast_path_every_filter_opcode(gpu, golden):
+    rng = np.random.default_rng(1)
+    x = np.round(rng.standard_normal(4096) * 10)                     # many exact ties
+    x[::37] = np.nan
+    base = words(golden, "fast")
+    for fop in range(0xC1, 0xC7):
+        code = list(base)
+        code[1] = (code[1] & ~0xFF) | fop
+        e = engine_f64(x, code, {2: x.size, 3: o.f64_bits(3.0)})
+        e.run()
+        acc = np.zeros(1, np.float64)
+        o.O.vo_fastpath_f64(o.ptr(x), x.size, x.size, 0, 3.0, 0.0, fop, o.ptr(acc))
+        # oracle sums in the reference order; both are NaN for NE (NaN != 3 passes)
+        got, ref = e.get_scalar_f64(0), float(acc[0])
+        if math.isnan(ref):
+            assert math.isnan(got)
+        else:
+            m = {0xC1: x == 3, 0xC2: x != 3, 0xC3: x < 3, 0xC4: x <= 3, 0xC5: x > 3, 0xC6: x >= 3}[fop]
+            assert got == float(o.exact_sum(x[m]))                   # small integers: exact
+
+
+def test_fast_path_i32_wraps_bit_exact(gpu, golden):
+    from voxel_amd._lib import lib
+    g = golden["i32_seed9"]
+    x = o.mt_uniform_i32(9, g["n"], 0, 1 << 30)
+    e = lib.voxel_gpu_engine_create(3)
+    lib.voxel_gpu_engine_add_segment_raw(e, x.ctypes.data, x.size)
+    lib.voxel_engine_set_scalar(e, 2, x.size)
+    lib.voxel_engine_set_scalar(e, 3, 1 << 29)
+    c = np.array(words(golden, "fast"), np.uint32)
+    lib.voxel_engine_load_program(e, c.ctypes.data, c.size)
+    lib.voxel_engine_run(e)
+    assert lib.voxel_last_error() in (b"", None)
+    assert lib.voxel_engine_get_scalar(e, 0) == B(g["R0"])
+    lib.voxel_engine_destroy(e)
+
+
+def test_literal_programs(gpu, golden):
+    d8 = np.arange(1, 9, dtype=np.float64)
+    e = engine_f64(d8, words(golden, "fast"), {2: 8, 3: o.f64_bits(4.0)})      # c_test.cpp Test 1
+    e.run()
+    assert e.get_scalar_f64(0) == 26.0
+    e = engine_f64(d8, [0x00400410] + words(golden, "fast"), {2: 8, 3: o.f64_bits(4.0)})
+    e.run()                                                           # test_suite.cpp:254-285
+    assert e.get_scalar_f64(0) == 26.0 and e.get_scalar(4) == 4
+    from voxel_amd import voxel_py as vx
+    ei = vx.EngineI64()
+    ei.add_segment(np.array([10, 20, 30, 40], np.int64))
+    ei.set_scalar(2, 4)
+    ei.set_scalar(3, 15)
+    ei.load_program(words(golden, "i64_add"))
+    ei.run()
+    assert ei.get_scalar_i64(0) == 90                                 # test_suite.cpp:290-321
+
+
+# ---------------------------------------------------------------- Tier-0
+def test_voxel_query_shapes_golden(gpu, golden, x42):
+    g = golden["seed42"]
+    from voxel_amd import voxel_query as vq
+    exact = o.relational(x42, 4, 500.0)["sum"]
+    s = vq.from_numpy(x42).filter_gt(500.0).sum().run()
+    assert close_f64(s, exact)
+    assert abs(s - o.bits_f64(B(g["vq_sum"]["R0"]))) <= 1e-12 * s
+    c = vq.from_numpy(x42).filter_gt(500.0).count().run()
+    assert o.f64_bits(c) == B(g["vq_count"]["R0"]) == 500_720         # count bits read as f64 (Q9)
+    mn = vq.from_numpy(x42).filter_gt(500.0).min().run()
+    assert o.f64_bits(mn) == B(g["vq_min"]["R0"]) == 0                # Q10
+    mx = vq.from_numpy(x42).filter_gt(500.0).max().run()
+    assert o.f64_bits(mx) == B(g["vq_max"]["R0"])
+    lt = vq.from_numpy(x42).filter_lt(500.0).sum().run()
+    assert close_f64(lt, o.relational(x42, 2, 500.0)["sum"])
+    assert abs(lt - o.bits_f64(B(g["vq_sum_lt"]["R0"]))) <= 1e-12 * lt
+
+
+def test_tier0_register_effects(gpu, golden, x42):
+    g = golden["seed42"]
+    for name in ("vq_sum", "vq_count", "vq_min", "vq_max"):
+        e = engine_f64(x42, words(golden, name), {2: x42.size, 3: o.f64_bits(500.0)})
+        e.run()
+        assert e.get_scalar(1) == g[name]["R1"]
+        assert e.get_scalar(4) == g[name]["R4"]
+        assert e.get_scalar(5) == B(g[name]["R5"]), name               # last group's partial, bit-exact
+        if name != "vq_sum":
+            assert e.get_scalar(0) == B(g[name]["R0"]), name
+
+
+def test_tier0_i64_add_variant_golden(gpu, golden):
+    g = golden["i64_seed7"]
+    from voxel_amd import voxel_py as vx
+    x = o.mt_uniform_i64(7, g["n"], 0, 1000)
+    e = vx.EngineI64()
+    e.add_segment(x)
+    e.set_scalar(2, x.size)
+    e.set_scalar(3, 500)
+    e.load_program(words(golden, "i64_add"))
+    e.run()
+    assert e.get_scalar_i64(0) == g["R0"] == 374_259_792
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tier0_random_programs_against_oracle(gpu, golden, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(1, 3000)) * 4
+    x = rng.standard_normal(n) * 100
+    x[rng.integers(0, n, 7)] = 0.0
+    if seed % 2:
+        x[rng.integers(0, n, 4)] = np.nan
+    thr = float(rng.standard_normal() * 50)
+    shapes = {"vq_count": (1, 1), "vq_min": (2, 2), "vq_max": (3, 3), "vq_sum": (0, 0)}
+    for name, (red, accop) in shapes.items():
+        e = engine_f64(x, words(golden, name), {2: n, 3: o.f64_bits(thr)})
+        e.run()
+        part = np.zeros(1, np.uint64)
+        r = o.O.vo_tier0_f64(o.ptr(x), n, 0, n, 4, thr, 0, red, accop, 0, o.ptr(part))
+        assert e.get_scalar(5) == int(part[0]), name
+        if name == "vq_sum":
+            got = e.get_scalar_f64(0)
+            if math.isnan(o.bits_f64(int(r))):
+                assert math.isnan(got)
+            else:
+                sel = np.where(x > thr, x, 0.0)
+                assert close_f64(got, o.exact_sum(sel[~np.isnan(sel)]))
+        else:
+            assert e.get_scalar(0) == int(r), name
+
+
+def test_tier0_minmax_mixed_signs_uses_exact_fold(gpu, golden):
+    # values whose bit patterns straddle both wrap points: the GPU falls back
+    # to the in-order fold; result must equal the oracle's bits exactly
+    rng = np.random.default_rng(9)
+    for trial in range(4):
+        x = rng.standard_normal(4 * 1500) * 1e3
+        x[::11] = rng.choice([np.inf, -np.inf, 1e300, -1e300, 5e-324], x[::11].size)
+        for name, (red, accop) in (("vq_min", (2, 2)), ("vq_max", (3, 3))):
+            thr = -5e2
+            e = engine_f64(x, words(golden, name), {2: x.size, 3: o.f64_bits(thr)})
+            e.run()
+            part = np.zeros(1, np.uint64)
+            r = o.O.vo_tier0_f64(o.ptr(x), x.size, 0, x.size, 4, thr, 0, red, accop, 0, o.ptr(part))
+            assert e.get_scalar(0) == int(r) and e.get_scalar(5) == int(part[0]), (trial, name)
+
+
+def test_tier0_zero_padding_past_count(gpu, golden):
+    # R[total] past Count: VLOAD reads zeros there (engine.hpp:1959-1962)
+    x = np.array([5.0, -3.0, 7.0, 1.0, 2.0, 9.0], np.float64)
+    for name, (red, accop) in (("vq_count", (1, 1)), ("vq_max", (3, 3)), ("vq_min", (2, 2))):
+        e = engine_f64(x, words(golden, name), {2: 12, 3: o.f64_bits(-10.0)})
+        e.run()
+        part = np.zeros(1, np.uint64)
+        r = o.O.vo_tier0_f64(o.ptr(x), x.size, 0, 12, 4, -10.0, 0, red, accop, 0, o.ptr(part))
+        assert (e.get_scalar(0), e.get_scalar(5)) == (int(r), int(part[0])), name
+
+
+def test_tier0_nonterminating_loop_is_refused(gpu, golden):
+    x = np.arange(10, dtype=np.float64)
+    e = engine_f64(x, words(golden, "vq_sum"), {2: 10, 3: o.f64_bits(1.0)})   # 10 rows: 10 % 4 != 0
+    with pytest.raises(RuntimeError, match="would not terminate"):
+        e.run()
+    assert e.get_scalar(0) == 0 and e.get_scalar(1) == 0
+
+
+# ---------------------------------------------------------------- JIT
+def test_jit_run_golden(gpu, golden, x42):
+    from voxel_amd import voxel_py as vx
+    from voxel_amd._lib import lib
+    g = golden["seed42"]
+    c = np.array(words(golden, "fast"), np.uint32)
+    r = lib.voxel_jit_run(c.ctypes.data, c.size, x42.ctypes.data, x42.size, 500.0)
+    exact = o.relational(x42, 4, 500.0)["sum"]
+    assert close_f64(r, exact)
+    ref = o.bits_f64(B(g["jit"]["result"]))
+    assert abs(r - ref) <= 1e-12 * ref
+    assert close_f64(vx.jit_run(words(golden, "vq_sum"), x42, 500.0, x42.size), exact)
+    d8 = np.arange(1, 9, dtype=np.float64)
+    assert lib.voxel_jit_run(c.ctypes.data, c.size, d8.ctypes.data, 8, 4.0) == 26.0
+    # not fusable: 0.0 and an error, as the reference's C entry returns 0.0
+    cc = np.array(words(golden, "vq_count"), np.uint32)
+    assert lib.voxel_jit_run(cc.ctypes.data, cc.size, x42.ctypes.data, x42.size, 500.0) == 0.0
+    assert b"JIT lowering" in lib.voxel_last_error()
+    with pytest.raises(RuntimeError):
+        vx.jit_run(words(golden, "vq_count"), x42, 500.0, x42.size)
+
+
+def test_jit_ordered_compares_and_over_read(gpu, golden):
+    from voxel_amd._lib import lib
+    x = np.array([1.0, np.nan, 3.0, 4.0, np.nan, 6.0, 7.0, 8.0, 9.0, 10.0], np.float64)
+    base = words(golden, "fast")
+    for fop in range(0xC1, 0xC7):
+        code = np.array(base, np.uint32)
+        code[1] = (code[1] & ~0xFF) | fop
+        r = lib.voxel_jit_run(code.ctypes.data, code.size, x.ctypes.data, x.size, 4.0)
+        ref = o.O.vo_jit_f64(o.ptr(x), x.size, x.size, 0, fop, 4.0)
+        assert r == ref, hex(fop)                                    # NE fails NaN in the JIT
+
+
+# ---------------------------------------------------------------- run_parallel
+def test_run_parallel_golden(gpu, golden, x42):
+    g = golden["seed42"]
+    for t in (8, 7):
+        e = engine_f64(x42, words(golden, "fast"), {2: x42.size, 3: o.f64_bits(500.0)})
+        e.run_parallel(0, t, 0)
+        ref = o.bits_f64(B(g["run_parallel_%d" % t]["R0"]))
+        got = e.get_scalar_f64(0)
+        # each worker drops its own tail: compare with the exact sum of the rows kept
+        chunk = -(-x42.size // t)
+        kept = []
+        for w in range(t):
+            s = w * chunk
+            cnt = min(chunk, x42.size - s)
+            seg = x42[s:s + (cnt // 4) * 4]
+            kept.append(seg[seg > 500])
+        exact = o.exact_sum(np.concatenate(kept))
+        assert close_f64(got, exact, ulps=4) and abs(got - ref) <= 1e-12 * ref
+
+
+# ---------------------------------------------------------------- relational scans
+TORCH_DT = {np.float64: "float64", np.float32: "float32", np.int64: "int64", np.int32: "int32"}
+
+
+def dev(x):
+    import torch
+    return torch.from_numpy(x).cuda()
+
+
+def gen(dt, n, rng):
+    if np.dtype(dt).kind == "f":
+        return (rng.random(n) * 1000).astype(dt)
+    return rng.integers(0, 2**31, n).astype(dt)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32, np.int64, np.int32])
+@pytest.mark.parametrize("n", [0, 1, 5, 1023, 65_537, 2_000_003])
+def test_relational_compound_scan(gpu, dt, n):
+    from voxel_amd import scan
+    rng = np.random.default_rng(n + 7)
+    a, b = gen(dt, n, rng), gen(dt, n, rng)
+    if np.dtype(dt).kind == "f":
+        ta, tb = 292.8932188, 707.1067812
+    else:
+        ta, tb = int((1 - math.sqrt(0.5)) * 2**31), int(math.ceil(math.sqrt(0.5) * 2**31))
+    ta, tb = dt(ta), dt(tb)
+    r = scan.scan(dev(a), "gt", ta, dev(b), "lt", tb)
+    ref = o.relational(a, 4, ta, b, 2, tb)
+    assert r["count"] == ref["count"]
+    if np.dtype(dt).kind == "f":
+        if dt == np.float64:
+            assert close_f64(r["sum"], ref["sum"])
+        else:
+            e = float(ref["sum"])
+            assert abs(r["sum"] - e) <= 4 * float(np.spacing(np.float32(abs(e)))) + 0.0
+        assert (r["min"], r["max"]) == ((ref["min"], ref["max"]) if ref["min"] is not None else (None, None))
+    else:
+        assert r["sum"] == ref["sum"]
+        assert (r["min"], r["max"]) == ((int(ref["min"]), int(ref["max"])) if ref["count"] else (None, None))
+
+
+def test_relational_special_values_and_unaligned(gpu):
+    import torch
+    from voxel_amd import scan
+    x = np.array([np.nan, -0.0, 0.0, np.inf, -np.inf, 1.0, -1.0, 5e-324, np.nan, 2.0, 3.0], np.float64)
+    for op in ("eq", "ne", "lt", "le", "gt", "ge", "true"):
+        for t in (0.0, -0.0, 1.0, np.inf, np.nan):
+            opc = scan.OPS[op]
+            r = scan.scan(dev(x), op, t)
+            ref = o.relational(x, opc, t)
+            assert r["count"] == ref["count"], (op, t)
+    # unaligned start: a column view one element in (pointer not 16-byte aligned)
+    big = np.arange(1, 100_001, dtype=np.float64)
+    t = torch.from_numpy(big).cuda()[1:]
+    r = scan.scan(t, "gt", 10.0)
+    sel = big[1:][big[1:] > 10]
+    assert r["count"] == sel.size and r["sum"] == float(sel.sum()) and r["min"] == 11.0
+    # b shifted against a (different misalignment): scalar path
+    r = scan.scan(torch.from_numpy(big).cuda()[1:50001], "gt", 10.0, torch.from_numpy(big).cuda()[2:50002], "lt", 30000.0)
+    aa, bb = big[1:50001], big[2:50002]
+    assert r["count"] == int(((aa > 10) & (bb < 30000)).sum())
+
+
+def test_scan_is_deterministic(gpu):
+    from voxel_amd import scan
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(3_000_001) * 1e6
+    d = dev(a)
+    r1 = scan.scan(d, "gt", -1.0)
+    r2 = scan.scan(d, "gt", -1.0)
+    assert (r1["sum_hi"], r1["sum_lo"], r1["count"]) == (r2["sum_hi"], r2["sum_lo"], r2["count"])
+    assert close_f64(r1["sum"], o.relational(a, 4, -1.0)["sum"])
+
+
+def test_full_size_compound_properties(gpu):
+    """2^28-row f64 columns: count of (a>t1 AND b<t2) plus count of (a>t1 AND
+    b>=t2) equals count(a>t1); the sum splits the same way (within the stated
+    ulp bound); counts match torch's exact integer counts."""
+    import torch
+    from voxel_amd import scan
+    n = 1 << 28
+    g = torch.Generator(device="cuda").manual_seed(1)
+    a = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 1000
+    b = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 1000
+    t1, t2 = 292.8932188, 707.1067812
+    r = scan.scan(a, "gt", t1, b, "lt", t2)
+    r2 = scan.scan(a, "gt", t1, b, "ge", t2)
+    r1 = scan.scan(a, "gt", t1)
+    assert r["count"] + r2["count"] == r1["count"]
+    assert r["count"] == int(((a > t1) & (b < t2)).sum().item())
+    assert abs((r["sum"] + r2["sum"]) - r1["sum"]) <= 4 * o.ulp(r1["sum"])
+    assert r["min"] == float(a[(a > t1) & (b < t2)].min().item())
+    del a, b
+    torch.cuda.empty_cache()

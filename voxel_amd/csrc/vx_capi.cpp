@@ -1,0 +1,492 @@
+// vx_capi.cpp -- the reference C ABI's engine and JIT entry points
+// (bindings/voxel_c.h:14-41), plus the voxel_gpu_* scan extensions,
+// lowered to the GPU scan kernels of vx_scan.hip.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "../../include/voxel_c.h"
+#include "vx_kernels.hpp"
+#include "vx_lower.hpp"
+#include "vx_runtime.hpp"
+
+using namespace vx;
+
+namespace {
+
+// A segment: the reference's borrowed Segment<T>{Data, Count}
+// (data/segment.hpp:25-60), mirrored in HBM on demand.
+struct Seg {
+    const void* host = nullptr;      // borrowed caller memory (null for device segments)
+    const void* devext = nullptr;    // caller's device memory
+    size_t count = 0;
+    bool resident = false;           // copy once, never refresh
+    DevColumn col;
+};
+
+// Engine<T>: 64 scalar registers (core/registers.hpp:20) + segments + program.
+struct Engine {
+    int dtype = DT_F64;
+    uint64_t regs[64] = {};
+    std::vector<std::unique_ptr<Seg>> segs;
+    std::vector<uint32_t> code;
+    Plan plan;
+};
+
+size_t lanes_of(int dt) { return 32 / dtype_size(dt); }
+
+// Device pointer of a segment whose rows [begin, end) are current in HBM.
+const void* seg_rows(Seg& s, int dt, uint64_t begin, uint64_t end, StreamCtx* c) {
+    if (s.devext) return s.devext;
+    size_t es = dtype_size(dt);
+    if (!s.col.ensure(s.count * es)) return nullptr;
+    if (s.resident && s.col.valid) return s.col.dev;
+    if (s.resident) { begin = 0; end = s.count; }
+    if (end > begin) {
+        char* d = static_cast<char*>(s.col.dev) + begin * es;
+        const char* h = static_cast<const char*>(s.host) + begin * es;
+        if (!hip_ok(hipMemcpyAsync(d, h, (end - begin) * es, hipMemcpyHostToDevice, c->stream), "copy segment to HBM"))
+            return nullptr;
+    }
+    if (s.resident) s.col.valid = true;
+    return s.col.dev;
+}
+
+bool sync_read(StreamCtx* c, void* host_dst, const void* dev_src, size_t bytes) {
+    if (!hip_ok(hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, c->stream), "read back result"))
+        return false;
+    return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+// One scan launch into the stream's result slab, read back.
+bool scan_once(StreamCtx* c, const ScanArgs& g, ScanOut* out) {
+    if (g.end <= g.begin) {
+        std::memset(out, 0, sizeof(*out));
+        return true;
+    }
+    if (!hip_ok(launch_scan(g, c->scr, &c->dev->scan, c->stream), "scan launch")) return false;
+    if (!sync_read(c, &c->host->scan, &c->dev->scan, sizeof(ScanOut))) return false;
+    *out = c->host->scan;
+    return true;
+}
+
+double bits_f64(uint64_t b) { double d; std::memcpy(&d, &b, 8); return d; }
+uint64_t f64_bits(double d) { uint64_t b; std::memcpy(&b, &d, 8); return b; }
+float bits_f32(uint64_t b) { uint32_t u = (uint32_t)b; float f; std::memcpy(&f, &u, 4); return f; }
+
+// ---------------------------------------------------------------------------
+// Engine<T>::RunFastPath (exec/engine.hpp:109-141) on the GPU.
+// ---------------------------------------------------------------------------
+bool run_fast(Engine& e, StreamCtx* c) {
+    const Plan& p = e.plan;
+    if (p.seg >= e.segs.size()) { set_error("program reads segment %u, engine has %zu", p.seg, e.segs.size()); return false; }
+    Seg& s = *e.segs[p.seg];
+    const uint64_t kl = lanes_of(e.dtype);
+    uint64_t total = e.regs[p.tot] < s.count ? e.regs[p.tot] : s.count;
+    uint64_t off = e.regs[p.off];
+    uint64_t groups = (off <= total && total - off >= kl) ? (total - off) / kl : 0;
+    uint64_t end = off + groups * kl;
+    ScanOut r{};
+    if (groups > 0) {
+        const void* d = seg_rows(s, e.dtype, off, end, c);
+        if (!d) return false;
+        ScanArgs g{};
+        g.dtype = e.dtype; g.a = d; g.begin = off; g.end = end; g.mode = 0;
+        g.pa = make_pred(e.dtype, cmp_of_filter_op(p.fop), e.regs[p.thr]);
+        if (!scan_once(c, g, &r)) return false;
+    }
+    uint64_t accb = e.regs[p.acc];
+    if (e.dtype == DT_F64) {
+        double acc0 = bits_f64(accb);
+        double acc = groups ? finish_sum_f64(acc0, bits_f64(r.s), bits_f64(r.c)) : acc0;
+        e.regs[p.acc] = f64_bits(acc);
+    } else if (e.dtype == DT_I64) {
+        e.regs[p.acc] = accb + r.s;                        // wrapping i64 (engine.hpp:134-138)
+    } else {                                              // i32: accumulate in T, sign-extend (registers.hpp:57-62)
+        int32_t acc = (int32_t)(uint32_t)((uint32_t)accb + (uint32_t)r.s);
+        e.regs[p.acc] = (uint64_t)(int64_t)acc;
+    }
+    e.regs[p.off] = end;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Tier-0 loop programs (dispatch loop engine.hpp:76-80 over the handlers of
+// VLOAD/VFILTER/VSUM|VCOUNT|VRED_*/ADDF|ADD|CMP-select/ADD/CMP/JNZ).
+// ---------------------------------------------------------------------------
+bool run_tier0(Engine& e, StreamCtx* c) {
+    const Plan& p = e.plan;
+    if (e.dtype != DT_F64 && e.dtype != DT_I64) { set_error("Tier-0 loops are lowered for f64 and i64 engines only"); return false; }
+    uint64_t regs[64];
+    std::memcpy(regs, e.regs, sizeof(regs));
+    run_prefix(p.prefix, regs);
+    const uint64_t kl = 4;
+    if (regs[p.step] != kl) { set_error("loop step register R%u holds %llu, not the lane count %llu", p.step,
+                                        (unsigned long long)regs[p.step], (unsigned long long)kl); return false; }
+    if (p.vcount != 0 && p.vcount < kl) { set_error("VLOAD with a partial lane count is not lowered"); return false; }
+    if (p.seg >= e.segs.size()) { set_error("program reads segment %u, engine has %zu", p.seg, e.segs.size()); return false; }
+    uint64_t off = regs[p.off], total = regs[p.tot];
+    if (total <= off || (total - off) % kl != 0) {
+        set_error("the reference loop would not terminate: (R%u - R%u) = %lld is not a positive multiple of %llu "
+                  "(JNZ loops until offset == total)", p.tot, p.off, (long long)(total - off), (unsigned long long)kl);
+        return false;
+    }
+    // accepted (reduction, accumulate) pairs
+    bool f = e.dtype == DT_F64;
+    bool okpair = (p.red == 0 && p.accop == 0 && f) || (p.red == 0 && p.accop == 1 && !f) ||
+                  (p.red == 1 && p.accop == 1) || (p.red >= 2 && p.accop >= 2);
+    if (!okpair) { set_error("loop (reduction %u, accumulate %u) on this engine type is not lowered", p.red, p.accop); return false; }
+
+    Seg& s = *e.segs[p.seg];
+    uint64_t dend = total < s.count ? total : s.count;    // rows past Count load as 0
+    const void* d = seg_rows(s, e.dtype, off < dend ? off : dend, dend, c);
+    if (!d) return false;
+    uint64_t groups = (total - off) / kl;
+    uint64_t thrb = p.thr != 0xFF ? regs[p.thr] : 0;
+    PredDev pred = p.cmp == 6 ? make_pred(e.dtype, CMP_TRUE, 0) : p.cmp == 7 ? make_never_pred(e.dtype)
+                                                                             : make_pred(e.dtype, p.cmp, thrb);
+    GroupArgs ga{};
+    ga.dtype = e.dtype; ga.a = d; ga.count = s.count; ga.off = off; ga.groups = groups; ga.p = pred;
+    ga.use_table = 0; ga.red = p.red; ga.fold = p.accop; ga.acc0 = regs[p.acc];
+
+    uint64_t accb = regs[p.acc], last = 0;
+    if (p.red <= 1) {
+        ScanArgs g{};
+        g.dtype = e.dtype; g.a = d; g.begin = off < dend ? off : dend; g.end = dend;
+        g.pa = pred;
+        if (p.red == 1) { g.mode = 1; g.pb = make_nonzero_pred(e.dtype); g.table_b = 1; }
+        ScanOut r{};
+        if (g.end > g.begin) {
+            if (!hip_ok(launch_scan(g, c->scr, &c->dev->scan, c->stream), "scan launch")) return false;
+        } else {
+            std::memset(&c->host->scan, 0, sizeof(ScanOut));
+        }
+        ga.off = off + (groups - 1) * kl;
+        if (!hip_ok(launch_group_last(ga, &c->dev->last, c->stream), "last-group launch")) return false;
+        if (!sync_read(c, c->host, c->dev, sizeof(ResultSlab))) return false;
+        if (g.end > g.begin) r = c->host->scan;
+        last = c->host->last;
+        if (p.red == 0 && f) accb = f64_bits(finish_sum_f64(bits_f64(accb), bits_f64(r.s), bits_f64(r.c)));
+        else if (p.red == 0) accb = accb + r.s;
+        else accb = accb + r.cnt;
+    } else {
+        if (!hip_ok(launch_group_minmax(ga, c->scr, &c->dev->group, c->stream), "group launch")) return false;
+        if (!sync_read(c, &c->host->group, &c->dev->group, sizeof(GroupOut))) return false;
+        GroupOut go = c->host->group;
+        last = go.last;
+        const uint64_t half = 1ull << 63;
+        if ((uint64_t)(go.smax - go.smin) < half) {
+            // every value lies in a signed window narrower than 2^63: the
+            // wrapping CMP difference is the signed order
+            accb = p.accop == 2 ? (uint64_t)go.smin : (uint64_t)go.smax;
+        } else if (go.umax - go.umin < half) {
+            accb = p.accop == 2 ? go.umin : go.umax;
+        } else {
+            if (!hip_ok(launch_group_fold(ga, c->dev->fold, c->stream), "fold launch")) return false;
+            if (!sync_read(c, c->host->fold, c->dev->fold, 16)) return false;
+            accb = c->host->fold[0];
+            last = c->host->fold[1];
+        }
+    }
+    regs[p.acc] = accb;
+    regs[p.off] = total;
+    regs[p.part] = last;
+    std::memcpy(e.regs, regs, sizeof(regs));
+    return true;
+}
+
+Engine* eng(void* h) { return static_cast<Engine*>(h); }
+
+void engine_run(void* h) {
+    set_error("%s", "");
+    if (!h) { set_error("null engine"); return; }
+    Engine& e = *eng(h);
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(nullptr);
+    if (!c) return;
+    if (e.plan.kind == Plan::FAST) run_fast(e, c);
+    else if (e.plan.kind == Plan::TIER0) run_tier0(e, c);
+    else set_error("program not lowered to the GPU: %s", e.plan.why.c_str());
+}
+
+size_t engine_add(void* h, const void* data, size_t count) {
+    Engine& e = *eng(h);
+    auto s = std::make_unique<Seg>();
+    s->host = data;
+    s->count = count;
+    e.segs.push_back(std::move(s));
+    return e.segs.size() - 1;
+}
+
+void engine_load(void* h, const uint32_t* code, size_t len) {
+    Engine& e = *eng(h);
+    e.code.assign(code, code + len);
+    e.plan = lower_engine_program(e.code.data(), e.code.size());
+}
+
+void fill_agg(int dtype, const ScanOut& r, voxel_gpu_agg* out) {
+    std::memset(out, 0, sizeof(*out));
+    out->count = r.cnt;
+    if (dtype == DT_F64) {
+        double s = bits_f64(r.s), cc = bits_f64(r.c);
+        out->sum_hi = s; out->sum_lo = cc;
+        out->sum = finish_sum_f64(0.0, s, cc);
+        out->fmin = bits_f64(r.mn); out->fmax = bits_f64(r.mx);
+        out->has_minmax = r.cnt > 0 && out->fmin <= out->fmax;
+    } else if (dtype == DT_F32) {
+        float s = bits_f32(r.s), cc = bits_f32(r.c);
+        out->sum_hi = s; out->sum_lo = cc;
+        out->sum = finish_sum_f32(0.0f, s, cc);
+        out->fmin = bits_f32(r.mn); out->fmax = bits_f32(r.mx);
+        out->has_minmax = r.cnt > 0 && out->fmin <= out->fmax;
+    } else if (dtype == DT_I64) {
+        out->isum = (int64_t)r.s;
+        out->imin = (int64_t)r.mn; out->imax = (int64_t)r.mx;
+        out->has_minmax = r.cnt > 0;
+    } else {
+        out->isum = (int64_t)r.s;
+        out->imin = (int32_t)(uint32_t)r.mn; out->imax = (int32_t)(uint32_t)r.mx;
+        out->has_minmax = r.cnt > 0;
+    }
+}
+
+bool make_scan_args(const voxel_gpu_scan_desc* d, ScanArgs& g) {
+    if (!d) { set_error("null scan descriptor"); return false; }
+    if (d->dtype < DT_F64 || d->dtype > DT_I32) { set_error("scan dtype %d not supported", d->dtype); return false; }
+    if (!d->a && d->n) { set_error("scan: column a is null"); return false; }
+    if (d->op_a < 0 || d->op_a > 6 || d->op_b < 0 || d->op_b > 6) { set_error("scan: predicate code out of range"); return false; }
+    g = ScanArgs{};
+    g.dtype = d->dtype; g.a = d->a; g.b = d->b; g.begin = 0; g.end = d->n; g.blocks = d->blocks;
+    g.pa = make_pred(d->dtype, d->op_a, d->t_a);
+    if (d->b && d->op_b != CMP_TRUE) {
+        g.mode = 2;
+        g.pb = make_pred(d->dtype, d->op_b, d->t_b);
+    } else {
+        g.mode = 0;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- Engine (f64) : reference bindings/voxel_c.cpp:14-49 ----
+void* voxel_engine_create_f64(void) { auto* e = new Engine(); e->dtype = DT_F64; return e; }
+void voxel_engine_destroy(void* h) { delete eng(h); }
+size_t voxel_engine_add_segment(void* h, const double* data, size_t count) { return engine_add(h, data, count); }
+void voxel_engine_load_program(void* h, const uint32_t* code, size_t len) { engine_load(h, code, len); }
+void voxel_engine_run(void* h) { engine_run(h); }
+
+// Engine<f64>::RunParallel (engine.hpp:143-175): row chunks of ceil(N/threads),
+// each runs the fast path with copied registers (dropping its own tail), and
+// R[result] = sum over workers of their R[result] in T.  Workers whose chunk
+// starts at or past N are skipped (the reference reads out of bounds there).
+void voxel_engine_run_parallel(void* h, size_t seg_id, uint32_t threads, uint8_t result_reg) {
+    set_error("%s", "");
+    if (!h) { set_error("null engine"); return; }
+    Engine& e = *eng(h);
+    if (e.segs.empty() || seg_id >= e.segs.size()) return;          // engine.hpp:146
+    if (threads == 0) threads = std::thread::hardware_concurrency();
+    if (threads < 1) threads = 1;
+    if (e.plan.kind != Plan::FAST) { set_error("run_parallel is lowered for fast-path programs only: %s",
+                                               e.plan.kind == Plan::NONE ? e.plan.why.c_str() : "Tier-0 loop"); return; }
+    if (e.plan.seg != 0) { set_error("run_parallel workers hold one segment; the program reads segment %u", e.plan.seg); return; }
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(nullptr);
+    if (!c) return;
+    Seg& s = *e.segs[seg_id];
+    const uint64_t n = s.count, kl = lanes_of(e.dtype);
+    const uint64_t chunk = (n + threads - 1) / threads;
+    const uint64_t r_off = e.regs[e.plan.off], r_tot = e.regs[e.plan.tot];
+    PredDev pred = make_pred(e.dtype, cmp_of_filter_op(e.plan.fop), e.regs[e.plan.thr]);
+    const void* d = nullptr;
+    if (n) { d = seg_rows(s, e.dtype, 0, n, c); if (!d) return; }
+    // Each worker's R[result] after its fast path; summed in T in worker
+    // order.  R[acc] values (acc0 + S_t) are summed as a compensated pair.
+    const uint8_t rr = result_reg & 0x3F;
+    const bool res_is_off = rr == e.plan.off;
+    const bool res_is_acc = rr == e.plan.acc && !res_is_off;
+    double acc0 = bits_f64(e.regs[e.plan.acc]);
+    double hs = 0.0, hc = 0.0, plain = 0.0;
+    uint64_t isum = 0;
+    auto add = [&](double v) { double t = hs + v, bp = t - hs; hc += (hs - (t - bp)) + (v - bp); hs = t; };
+    for (uint32_t t = 0; t < threads; ++t) {
+        uint64_t start = (uint64_t)t * chunk;
+        if (start >= n) continue;
+        uint64_t cnt = (start + chunk > n) ? (n - start) : chunk;
+        uint64_t total = r_tot < cnt ? r_tot : cnt;
+        uint64_t groups = (r_off <= total && total - r_off >= kl) ? (total - r_off) / kl : 0;
+        uint64_t woff = r_off + groups * kl;
+        if (!res_is_acc) {
+            uint64_t v = res_is_off ? woff : e.regs[rr];
+            if (e.dtype == DT_F64) plain = plain + bits_f64(v); else isum += v;
+            continue;
+        }
+        ScanOut r{};
+        if (groups) {
+            ScanArgs g{};
+            g.dtype = e.dtype; g.a = d; g.begin = start + r_off; g.end = start + woff; g.pa = pred;
+            if (!scan_once(c, g, &r)) return;
+        }
+        if (e.dtype == DT_F64) {
+            add(acc0);
+            if (groups) { add(bits_f64(r.s)); add(bits_f64(r.c)); }
+        } else {
+            isum += e.regs[e.plan.acc] + r.s;
+        }
+    }
+    if (e.dtype == DT_F64) e.regs[rr] = f64_bits(res_is_acc ? hs + hc : plain);
+    else e.regs[rr] = isum;
+}
+
+void voxel_engine_set_scalar(void* h, size_t reg, uint64_t v) { eng(h)->regs[reg & 0x3F] = v; }
+uint64_t voxel_engine_get_scalar(void* h, size_t reg) { return eng(h)->regs[reg & 0x3F]; }
+void voxel_engine_set_scalar_f64(void* h, size_t reg, double v) { eng(h)->regs[reg & 0x3F] = f64_bits(v); }
+double voxel_engine_get_scalar_f64(void* h, size_t reg) { return bits_f64(eng(h)->regs[reg & 0x3F]); }
+size_t voxel_engine_k_lanes(void) { return 4; }
+
+// ---- Engine (i64) : reference bindings/voxel_c.cpp:54-77 ----
+void* voxel_engine_create_i64(void) { auto* e = new Engine(); e->dtype = DT_I64; return e; }
+void voxel_engine_destroy_i64(void* h) { delete eng(h); }
+size_t voxel_engine_add_segment_i64(void* h, const int64_t* data, size_t count) { return engine_add(h, data, count); }
+void voxel_engine_load_program_i64(void* h, const uint32_t* code, size_t len) { engine_load(h, code, len); }
+void voxel_engine_run_i64(void* h) { engine_run(h); }
+void voxel_engine_set_scalar_i64(void* h, size_t reg, uint64_t v) { eng(h)->regs[reg & 0x3F] = v; }
+uint64_t voxel_engine_get_scalar_i64(void* h, size_t reg) { return eng(h)->regs[reg & 0x3F]; }
+
+// ---- JIT : reference bindings/voxel_c.cpp:80-101 ----
+static double jit_run_impl(const uint32_t* code, size_t code_len, const double* data, size_t data_len,
+                           uint64_t r2_in, double threshold) {
+    set_error("%s", "");
+    JitPlan jp = lower_jit_program(code, code_len);
+    if (!jp.ok) { set_error("JIT lowering: %s", jp.why.c_str()); return 0.0; }
+    uint64_t regs[64] = {};
+    regs[2] = r2_in;
+    regs[3] = f64_bits(threshold);
+    run_prefix(jp.prefix, regs);
+    uint64_t off = regs[jp.off], r2 = regs[2];
+    uint64_t iters = (r2 - off + 7) >> 3;                 // x86_64.hpp:1659-1668
+    if (iters == 0) { set_error("the reference JIT loop would not terminate: no rows (R2 - offset + 7 < 8)"); return 0.0; }
+    uint64_t end = off + 8 * iters;
+    if (end > data_len) end = data_len;                   // rows past the segment read as 0
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(nullptr);
+    if (!c) return 0.0;
+    ScanOut r{};
+    if (end > off) {
+        // staging buffer kept for the life of the process (never freed at exit)
+        static DevColumn* stage_p = new DevColumn();
+        DevColumn& stage = *stage_p;
+        if (!stage.ensure(data_len * 8)) return 0.0;
+        if (!hip_ok(hipMemcpyAsync(static_cast<char*>(stage.dev) + off * 8, data + off, (end - off) * 8,
+                                   hipMemcpyHostToDevice, c->stream), "copy JIT data to HBM")) return 0.0;
+        int cmp = cmp_of_filter_op(jp.fop);
+        ScanArgs g{};
+        g.dtype = DT_F64; g.a = stage.dev; g.begin = off; g.end = end; g.mode = 0;
+        g.pa = make_pred(DT_F64, cmp, regs[jp.thr], true);
+        g.table_a = cmp == CMP_NE;                        // vcmppd NEQ_OQ: NaN fails
+        if (!scan_once(c, g, &r)) return 0.0;
+    }
+    return bits_f64(r.s) + bits_f64(r.c);                 // the initial R[acc] is not added
+}
+
+double voxel_jit_run(const uint32_t* code, size_t code_len, const double* data, size_t data_len, double threshold) {
+    return jit_run_impl(code, code_len, data, data_len, data_len, threshold);
+}
+
+double voxel_gpu_jit_run_r2(const uint32_t* code, size_t code_len, const double* data, size_t data_len,
+                            uint64_t r2, double threshold) {
+    return jit_run_impl(code, code_len, data, data_len, r2, threshold);
+}
+
+// ---- GPU extensions ----
+int voxel_gpu_available(void) { return runtime_ok() ? 1 : 0; }
+
+void* voxel_gpu_malloc(size_t bytes) {
+    if (!runtime_ok()) return nullptr;
+    void* p = nullptr;
+    if (!hip_ok(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc")) return nullptr;
+    return p;
+}
+void voxel_gpu_free(void* p) { if (p) (void)hipFree(p); }
+int voxel_gpu_memcpy_h2d(void* dev, const void* host, size_t bytes) {
+    return hip_ok(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D") ? 0 : -1;
+}
+int voxel_gpu_memcpy_d2h(void* host, const void* dev, size_t bytes) {
+    return hip_ok(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H") ? 0 : -1;
+}
+int voxel_gpu_synchronize(void) { return hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize") ? 0 : -1; }
+
+void* voxel_gpu_engine_create(int dtype) {
+    if (dtype != DT_F64 && dtype != DT_I64 && dtype != DT_I32) {
+        set_error("typed engine: dtype %d has no Engine<T> in the reference (f64, i64, i32 only)", dtype);
+        return nullptr;
+    }
+    auto* e = new Engine();
+    e->dtype = dtype;
+    return e;
+}
+size_t voxel_gpu_engine_add_segment_raw(void* h, const void* data, size_t count) { return engine_add(h, data, count); }
+int voxel_gpu_engine_set_resident(void* h, size_t seg, int resident) {
+    Engine& e = *eng(h);
+    if (seg >= e.segs.size()) { set_error("no segment %zu", seg); return -1; }
+    e.segs[seg]->resident = resident != 0;
+    e.segs[seg]->col.valid = false;
+    return 0;
+}
+size_t voxel_gpu_engine_add_device_segment(void* h, const void* dev, size_t count) {
+    Engine& e = *eng(h);
+    auto s = std::make_unique<Seg>();
+    s->devext = dev;
+    s->count = count;
+    e.segs.push_back(std::move(s));
+    return e.segs.size() - 1;
+}
+
+int voxel_gpu_jit_lowerable(const uint32_t* code, size_t len) {
+    set_error("%s", "");
+    JitPlan jp = lower_jit_program(code, len);
+    if (!jp.ok) set_error("JIT lowering: %s", jp.why.c_str());
+    return jp.ok ? 1 : 0;
+}
+
+int voxel_gpu_engine_lowered(const void* h) {
+    const Engine& e = *static_cast<const Engine*>(h);
+    return e.plan.kind == Plan::FAST ? 1 : e.plan.kind == Plan::TIER0 ? 2 : 0;
+}
+
+int voxel_gpu_scan_launch(const voxel_gpu_scan_desc* d, void* stream) {
+    set_error("%s", "");
+    ScanArgs g;
+    if (!make_scan_args(d, g)) return -1;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(stream);
+    if (!c) return -1;
+    c->last_dtype = d->dtype;
+    if (g.end <= g.begin) {
+        return hip_ok(hipMemsetAsync(&c->dev->scan, 0, sizeof(ScanOut), c->stream), "clear result") ? 0 : -1;
+    }
+    return hip_ok(launch_scan(g, c->scr, &c->dev->scan, c->stream), "scan launch") ? 0 : -1;
+}
+
+int voxel_gpu_scan_result(void* stream, voxel_gpu_agg* out) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(stream);
+    if (!c) return -1;
+    if (c->last_dtype < 0) { set_error("no scan launched on this stream"); return -1; }
+    if (!sync_read(c, &c->host->scan, &c->dev->scan, sizeof(ScanOut))) return -1;
+    fill_agg(c->last_dtype, c->host->scan, out);
+    return 0;
+}
+
+int voxel_gpu_scan(const voxel_gpu_scan_desc* d, void* stream, voxel_gpu_agg* out) {
+    if (voxel_gpu_scan_launch(d, stream) != 0) return -1;
+    return voxel_gpu_scan_result(stream, out);
+}
+
+}  // extern "C"

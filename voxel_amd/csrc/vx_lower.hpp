@@ -1,0 +1,76 @@
+// vx_lower.hpp -- recognises the bytecode loops the reference runs natively
+// and turns them into GPU plans.  Host C++ only.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace vx {
+
+// Instruction fields, bytecode/instruction.hpp:19-56:
+// [op:8][rd:4][ra:4][rb:4][imm12:12].
+inline uint8_t op_of(uint32_t w) { return w & 0xFF; }
+inline uint8_t rd_of(uint32_t w) { return (w >> 8) & 0xF; }
+inline uint8_t ra_of(uint32_t w) { return (w >> 12) & 0xF; }
+inline uint8_t rb_of(uint32_t w) { return (w >> 16) & 0xF; }
+inline uint16_t imm_of(uint32_t w) { return (w >> 20) & 0xFFF; }
+inline int16_t simm_of(uint32_t w) {
+    uint16_t v = imm_of(w);
+    return (v & 0x800) ? (int16_t)(v | 0xF000) : (int16_t)v;
+}
+
+enum Op : uint8_t {
+    OP_NOP = 0x00, OP_HALT = 0x01, OP_MOV = 0x10, OP_MOVR = 0x11, OP_ADD = 0x20, OP_ADDF = 0x2A,
+    OP_CMP = 0x40, OP_JNZ = 0x52, OP_JLE = 0x5A, OP_JGE = 0x5C, OP_VLOAD = 0x70,
+    OP_VFILTER = 0xC0, OP_VFILTER_EQ = 0xC1, OP_VFILTER_GE = 0xC6,
+    OP_VSUM = 0xD0, OP_VRED_MIN = 0xD5, OP_VRED_MAX = 0xD6, OP_VCOUNT = 0xD7,
+};
+
+// Dedicated filter opcode 0xC1..0xC6 -> kCmp code (EQ NE LT LE GT GE).
+inline int cmp_of_filter_op(uint8_t op) {
+    switch (op) {
+    case 0xC1: return 0; case 0xC2: return 1; case 0xC3: return 2;
+    case 0xC4: return 3; case 0xC5: return 4; case 0xC6: return 5;
+    default: return 4;   // RunFastPath's default branch (engine.hpp:130)
+    }
+}
+
+struct Plan {
+    enum Kind { NONE, FAST, TIER0 } kind = NONE;
+    std::string why;                 // reason when NONE
+
+    // FAST: Engine<T>::DetectFastPath (engine.hpp:84-107)
+    uint8_t seg = 0, thr = 3, acc = 0, off = 1, tot = 2;
+    uint8_t fop = 0xC5;
+
+    // TIER0: prefix + loop (voxel_query.py:74-135,155-171 shapes)
+    std::vector<uint32_t> prefix;    // MOV/MOVR/NOP words run before the loop
+    int cmp = 6;                     // kCmp code of the filter, 6 none, 7 never
+    uint8_t vcount = 0;              // VLOAD count field
+    uint8_t red = 0;                 // 0 VSUM, 1 VCOUNT, 2 VRED_MIN, 3 VRED_MAX
+    uint8_t accop = 0;               // 0 ADDF, 1 ADD, 2 keep-smaller (JLE), 3 keep-larger (JGE)
+    uint8_t part = 5, step = 4;
+};
+
+// Interpreter view: fast path first (exactly DetectFastPath), then the
+// Tier-0 loop shapes.
+Plan lower_engine_program(const uint32_t* code, size_t len);
+
+// JIT view (codegen/x86_64.hpp:1600-1631): the fused block may start after
+// a prefix of MOV/MOVR/NOP words; only the f64 filter+sum loop fuses.
+struct JitPlan {
+    bool ok = false;
+    std::string why;
+    std::vector<uint32_t> prefix;
+    uint8_t seg = 0, thr = 3, acc = 0, off = 1;
+    uint8_t fop = 0xC5;
+};
+JitPlan lower_jit_program(const uint32_t* code, size_t len);
+
+// Applies MOV/MOVR/NOP words to a register file (HandleMOV/HandleMOVR,
+// engine.hpp:1167-1178).
+void run_prefix(const std::vector<uint32_t>& prefix, uint64_t* regs);
+
+}  // namespace vx

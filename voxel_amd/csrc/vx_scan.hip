@@ -1,0 +1,526 @@
+// vx_scan.hip -- fused filter + aggregate scans for gfx950 (MI355X).
+//
+// Replaces the reference's scan loops:
+//   Engine<T>::RunFastPath            exec/engine.hpp:109-141
+//   JIT fused filter+sum loop         codegen/x86_64.hpp:1632-1716
+//   Tier-0 VLOAD/VFILTER/VSUM|VCOUNT  exec/engine.hpp:1947-1969,2870-2924,2980-3111
+//   VectorFilter::Apply* + ColumnFilter::AndWith (compound predicates)
+//                                     ops/filter.hpp:42-288,540-547
+// with one streaming kernel family:
+//   * 16-byte (dwordx4) coalesced loads, U vectors per column in flight per
+//     lane, grid-stride over resident workgroups (<= 8 per CU);
+//   * predicates as a closed range test (two compares -> SGPR lane masks);
+//   * floats: per-lane TwoSum-compensated (s, c) pairs kept in T;
+//     ints: wrapping 64-bit sums (exact);
+//   * wave64 butterfly reduction, one partial per workgroup, and a
+//     deterministic final reduction by the last workgroup to finish (ticket),
+//     so one launch produces the result and the order of the final combine
+//     depends only on the grid size.
+#include "vx_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#pragma clang fp contract(off)
+
+namespace vx {
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename T> struct TyInfo;
+template <> struct TyInfo<double>  { using V = double2;   static constexpr int E = 2; using S = double;   static constexpr bool F = true; };
+template <> struct TyInfo<float>   { using V = float4;    static constexpr int E = 4; using S = float;    static constexpr bool F = true; };
+template <> struct TyInfo<int64_t> { using V = longlong2; static constexpr int E = 2; using S = uint64_t; static constexpr bool F = false; };
+template <> struct TyInfo<int32_t> { using V = int4;      static constexpr int E = 4; using S = uint64_t; static constexpr bool F = false; };
+
+template <typename T> __device__ __forceinline__ T from_bits(uint64_t b);
+template <> __device__ __forceinline__ double from_bits<double>(uint64_t b) { return __longlong_as_double((long long)b); }
+template <> __device__ __forceinline__ float from_bits<float>(uint64_t b) { return __uint_as_float((uint32_t)b); }
+template <> __device__ __forceinline__ int64_t from_bits<int64_t>(uint64_t b) { return (int64_t)b; }
+template <> __device__ __forceinline__ int32_t from_bits<int32_t>(uint64_t b) { return (int32_t)(uint32_t)b; }
+template <> __device__ __forceinline__ uint64_t from_bits<uint64_t>(uint64_t b) { return b; }
+
+__device__ __forceinline__ uint64_t to_bits(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ uint64_t to_bits(float v) { return (uint64_t)__float_as_uint(v); }
+__device__ __forceinline__ uint64_t to_bits(int64_t v) { return (uint64_t)v; }
+__device__ __forceinline__ uint64_t to_bits(int32_t v) { return (uint64_t)(uint32_t)v; }
+__device__ __forceinline__ uint64_t to_bits(uint64_t v) { return v; }
+
+template <typename T> struct Lim;
+template <> struct Lim<double>  { __device__ static double lo() { return -__builtin_huge_val(); } __device__ static double hi() { return __builtin_huge_val(); } };
+template <> struct Lim<float>   { __device__ static float lo() { return -__builtin_huge_valf(); } __device__ static float hi() { return __builtin_huge_valf(); } };
+template <> struct Lim<int64_t> { __device__ static int64_t lo() { return INT64_MIN; } __device__ static int64_t hi() { return INT64_MAX; } };
+template <> struct Lim<int32_t> { __device__ static int32_t lo() { return INT32_MIN; } __device__ static int32_t hi() { return INT32_MAX; } };
+
+// Predicate, range or truth-table form (PredDev in vx_kernels.hpp).
+template <typename T, bool TABLE>
+struct Pred {
+    T lo, hi, t;
+    bool inv;
+    uint32_t table;
+    __device__ __forceinline__ void load(const PredDev& p) {
+        lo = from_bits<T>(p.lo); hi = from_bits<T>(p.hi); t = from_bits<T>(p.t);
+        inv = p.inv != 0; table = p.table;
+    }
+    __device__ __forceinline__ bool operator()(T x) const {
+        if constexpr (TABLE) {
+            uint32_t st = (x < t ? 1u : 0u) | (x == t ? 2u : 0u) | (x > t ? 4u : 0u);
+            return (table >> st) & 1u;
+        } else {
+            return ((x >= lo) & (x <= hi)) != inv;
+        }
+    }
+};
+
+// Compensated summation step (Knuth TwoSum): s + c tracks the exact sum.
+template <typename S>
+__device__ __forceinline__ void two_sum(S& s, S& c, S v) {
+    S t = s + v;
+    S bp = t - s;
+    S e = (s - (t - bp)) + (v - bp);
+    s = t;
+    c += e;
+}
+
+template <typename T>
+struct Acc {
+    using S = typename TyInfo<T>::S;
+    S s, c;
+    uint64_t n;
+    T mn, mx;
+    __device__ __forceinline__ void init() { s = S(0); c = S(0); n = 0; mn = Lim<T>::hi(); mx = Lim<T>::lo(); }
+    __device__ __forceinline__ void add(bool p, T x) {
+        n += p ? 1u : 0u;
+        if constexpr (TyInfo<T>::F) {
+            two_sum<S>(s, c, p ? x : T(0));
+            mn = p ? fmin(mn, x) : mn;
+            mx = p ? fmax(mx, x) : mx;
+        } else {
+            s += p ? (uint64_t)(int64_t)x : 0ull;
+            mn = (p && x < mn) ? x : mn;
+            mx = (p && x > mx) ? x : mx;
+        }
+    }
+    __device__ __forceinline__ void merge(const Acc& o) {
+        n += o.n;
+        if constexpr (TyInfo<T>::F) {
+            S cc = c + o.c;
+            two_sum<S>(s, cc, o.s);
+            c = cc;
+            mn = fmin(mn, o.mn);
+            mx = fmax(mx, o.mx);
+        } else {
+            s += o.s;
+            mn = o.mn < mn ? o.mn : mn;
+            mx = o.mx > mx ? o.mx : mx;
+        }
+    }
+    __device__ __forceinline__ void shfl_merge(int mask) {
+        Acc o;
+        o.s = __shfl_xor(s, mask, 64);
+        o.c = __shfl_xor(c, mask, 64);
+        o.n = __shfl_xor(n, mask, 64);
+        o.mn = __shfl_xor(mn, mask, 64);
+        o.mx = __shfl_xor(mx, mask, 64);
+        merge(o);
+    }
+    __device__ __forceinline__ void store(ScanOut* p) const {
+        p->s = to_bits(s); p->c = to_bits(c); p->cnt = n; p->mn = to_bits(mn); p->mx = to_bits(mx);
+    }
+    __device__ __forceinline__ void load(const ScanOut* p) {
+        s = from_bits<S>(p->s); c = from_bits<S>(p->c); n = p->cnt; mn = from_bits<T>(p->mn); mx = from_bits<T>(p->mx);
+    }
+};
+// Workgroup reduction of an Acc into thread 0 (deterministic order).
+template <typename T>
+__device__ __forceinline__ void block_reduce(Acc<T>& a, Acc<T>* lds) {
+    for (int m = 32; m > 0; m >>= 1) a.shfl_merge(m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) a.merge(lds[w]);
+    }
+}
+
+// Publishes this workgroup's partial and returns true in the workgroup that
+// finishes last; that workgroup then sees every other partial.
+__device__ __forceinline__ bool publish_and_check_last(uint32_t* ticket, bool* s_last) {
+    if (threadIdx.x == 0) {
+        __threadfence();
+        uint32_t prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *s_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    bool last = *s_last;
+    if (last) __threadfence();
+    return last;
+}
+
+template <typename T, int MODE, bool TA, bool TB, bool VEC>
+__global__ __launch_bounds__(kBlock)
+void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, uint64_t end,
+            PredDev pda, PredDev pdb, ScanOut* __restrict__ parts, uint32_t* __restrict__ ticket,
+            ScanOut* __restrict__ out) {
+    using V = typename std::conditional<VEC, typename TyInfo<T>::V, T>::type;
+    constexpr int E = VEC ? TyInfo<T>::E : 1;
+    constexpr int U = (MODE == 2) ? 4 : 8;
+
+    Pred<T, TA> pa; pa.load(pda);
+    Pred<T, TB> pb; pb.load(pdb);
+    Acc<T> acc; acc.init();
+
+    auto one = [&](T x, T y) {
+        bool p = pa(x);
+        if constexpr (MODE == 1) p = p & pb(x);
+        if constexpr (MODE == 2) p = p & pb(y);
+        acc.add(p, x);
+    };
+
+    // Scalar head (up to the first 16-byte boundary of a) and tail.
+    const uint64_t n = end - begin;
+    uint64_t head = 0;
+    if constexpr (VEC) {
+        uintptr_t addr = (uintptr_t)(a + begin);
+        head = ((16u - (addr & 15u)) & 15u) / sizeof(T);
+        if (head > n) head = n;
+    }
+    const uint64_t vb = begin + head;
+    const uint64_t nvec = (n - head) / E;
+    const uint64_t ts = vb + nvec * E;
+    const uint64_t ntail = end - ts;
+    const uint64_t gtid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (gtid < head + ntail) {
+        uint64_t i = gtid < head ? begin + gtid : ts + (gtid - head);
+        one(a[i], MODE == 2 ? b[i] : T(0));
+    }
+
+    const V* __restrict__ av = reinterpret_cast<const V*>(a + vb);
+    const V* __restrict__ bv = reinterpret_cast<const V*>(MODE == 2 ? b + vb : a + vb);
+    const uint64_t tile = (uint64_t)kBlock * U;
+    const uint64_t step = (uint64_t)gridDim.x * tile;
+    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < nvec; base += step) {
+        V xa[U], xb[U];
+        if (base + tile <= nvec) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) xa[u] = av[base + u * kBlock + threadIdx.x];
+            if constexpr (MODE == 2) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) xb[u] = bv[base + u * kBlock + threadIdx.x];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T* pxa = reinterpret_cast<const T*>(&xa[u]);
+                const T* pxb = reinterpret_cast<const T*>(&xb[u]);
+#pragma unroll
+                for (int e = 0; e < E; ++e) one(pxa[e], MODE == 2 ? pxb[e] : T(0));
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                uint64_t vi = base + u * kBlock + threadIdx.x;
+                if (vi < nvec) {
+                    xa[u] = av[vi];
+                    if constexpr (MODE == 2) xb[u] = bv[vi];
+                    const T* pxa = reinterpret_cast<const T*>(&xa[u]);
+                    const T* pxb = reinterpret_cast<const T*>(&xb[u]);
+#pragma unroll
+                    for (int e = 0; e < E; ++e) one(pxa[e], MODE == 2 ? pxb[e] : T(0));
+                }
+            }
+        }
+    }
+
+    __shared__ Acc<T> lds[kBlock / 64];
+    __shared__ bool s_last;
+    block_reduce(acc, lds);
+    if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
+    if (!publish_and_check_last(ticket, &s_last)) return;
+
+    Acc<T> r; r.init();
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += kBlock) {
+        Acc<T> o; o.load(&parts[i]);
+        r.merge(o);
+    }
+    __syncthreads();
+    block_reduce(r, lds);
+    if (threadIdx.x == 0) {
+        r.store(out);
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tier-0 VRED_MIN / VRED_MAX groups (engine.hpp:3071-3095) with the
+// CMP acc,part / JLE|JGE +2 / MOVR fold of the voxel_query min()/max()
+// programs (bindings/voxel_query.py:111-135).  Each group is 4 lanes of an
+// 8-byte type; lane = pred(x) ? x : 0 (VFILTER, engine.hpp:2870-2924), rows
+// past Count load as 0 (VLOAD, engine.hpp:1959-1962).
+// ---------------------------------------------------------------------------
+template <typename T, bool TAB>
+__device__ __forceinline__ uint64_t group_value(const T* __restrict__ a, uint64_t count, uint64_t row,
+                                                const Pred<T, TAB>& p, bool use_pred, int red) {
+    T v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint64_t idx = row + i;
+        T x = idx < count ? a[idx] : T(0);
+        v[i] = (!use_pred || p(x)) ? x : T(0);
+    }
+    if (red == 0) {                       // VSUM: T sum = 0; sum += lane (engine.hpp:2980-2987)
+        if constexpr (TyInfo<T>::F) {
+            T sm = T(0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sm = sm + v[i];
+            return to_bits(sm);
+        } else {
+            uint64_t sm = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sm += (uint64_t)(int64_t)v[i];
+            return sm;
+        }
+    }
+    if (red == 1) {                       // VCOUNT: lanes != 0 and not NaN (engine.hpp:3097-3111)
+        uint64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c += (v[i] == v[i] && v[i] != T(0)) ? 1u : 0u;
+        return c;
+    }
+    T m = v[0];
+    if (red == 2) {
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+            bool take = v[i] < m;
+            if constexpr (TyInfo<T>::F) take = take || (v[i] != v[i]);
+            m = take ? v[i] : m;
+        }
+    } else {
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+            bool take = v[i] > m;
+            if constexpr (TyInfo<T>::F) take = take || (m != m);
+            m = take ? v[i] : m;
+        }
+    }
+    return to_bits(m);
+}
+
+struct GAcc {
+    int64_t smin, smax;
+    uint64_t umin, umax;
+    __device__ __forceinline__ void init() { smin = INT64_MAX; smax = INT64_MIN; umin = UINT64_MAX; umax = 0; }
+    __device__ __forceinline__ void add(uint64_t v) {
+        int64_t s = (int64_t)v;
+        smin = s < smin ? s : smin; smax = s > smax ? s : smax;
+        umin = v < umin ? v : umin; umax = v > umax ? v : umax;
+    }
+    __device__ __forceinline__ void merge(const GAcc& o) {
+        smin = o.smin < smin ? o.smin : smin; smax = o.smax > smax ? o.smax : smax;
+        umin = o.umin < umin ? o.umin : umin; umax = o.umax > umax ? o.umax : umax;
+    }
+    __device__ __forceinline__ void shfl_merge(int m) {
+        GAcc o;
+        o.smin = __shfl_xor(smin, m, 64); o.smax = __shfl_xor(smax, m, 64);
+        o.umin = __shfl_xor(umin, m, 64); o.umax = __shfl_xor(umax, m, 64);
+        merge(o);
+    }
+};
+
+__device__ __forceinline__ void gblock_reduce(GAcc& a, GAcc* lds) {
+    for (int m = 32; m > 0; m >>= 1) a.shfl_merge(m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kBlock / 64; ++w) a.merge(lds[w]);
+}
+
+template <typename T, bool TAB>
+__global__ __launch_bounds__(kBlock)
+void k_group_minmax(const T* __restrict__ a, uint64_t count, uint64_t off, uint64_t groups,
+                    PredDev pd, int use_pred, int red, uint64_t acc0,
+                    GroupOut* __restrict__ parts, uint32_t* __restrict__ ticket, GroupOut* __restrict__ out) {
+    Pred<T, TAB> p; p.load(pd);
+    GAcc g; g.init();
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < groups; i += stride) {
+        uint64_t v = group_value<T, TAB>(a, count, off + 4 * i, p, use_pred != 0, red);
+        g.add(v);
+        if (i == groups - 1) out->last = v;
+    }
+    __shared__ GAcc lds[kBlock / 64];
+    __shared__ bool s_last;
+    gblock_reduce(g, lds);
+    if (threadIdx.x == 0) {
+        GroupOut* q = &parts[blockIdx.x];
+        q->smin = g.smin; q->smax = g.smax; q->umin = g.umin; q->umax = g.umax;
+    }
+    if (!publish_and_check_last(ticket, &s_last)) return;
+    GAcc r; r.init();
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += kBlock) {
+        GAcc o; o.smin = parts[i].smin; o.smax = parts[i].smax; o.umin = parts[i].umin; o.umax = parts[i].umax;
+        r.merge(o);
+    }
+    __syncthreads();
+    gblock_reduce(r, lds);
+    if (threadIdx.x == 0) {
+        r.add(acc0);
+        out->smin = r.smin; out->smax = r.smax; out->umin = r.umin; out->umax = r.umax;
+        out->groups = groups;
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Exact sequential fold in one wave: lanes compute 64 group values at a time,
+// then every lane applies them in row order (the loop of the reference).
+template <typename T, bool TAB>
+__global__ __launch_bounds__(64)
+void k_group_fold(const T* __restrict__ a, uint64_t count, uint64_t off, uint64_t groups,
+                  PredDev pd, int red, int fold, uint64_t acc0, uint64_t* __restrict__ result) {
+    Pred<T, TAB> p; p.load(pd);
+    uint64_t acc = acc0, last = 0;
+    for (uint64_t base = 0; base < groups; base += 64) {
+        uint64_t gi = base + threadIdx.x;
+        uint64_t v = gi < groups ? group_value<T, TAB>(a, count, off + 4 * gi, p, true, red) : 0;
+        uint64_t lim = groups - base < 64 ? groups - base : 64;
+        for (uint64_t k = 0; k < lim; ++k) {
+            uint64_t r = __shfl(v, (int)k, 64);
+            int64_t diff = (int64_t)(acc - r);          // CMP acc, part
+            bool take = (fold == 2) ? (diff > 0) : (diff < 0);
+            acc = take ? r : acc;
+            last = r;
+        }
+    }
+    if (threadIdx.x == 0) { result[0] = acc; result[1] = last; }
+}
+
+template <typename T, bool TAB>
+__global__ __launch_bounds__(64)
+void k_group_last(const T* __restrict__ a, uint64_t count, uint64_t row, PredDev pd, int red,
+                  uint64_t* __restrict__ result) {
+    Pred<T, TAB> p; p.load(pd);
+    if (threadIdx.x == 0) result[0] = group_value<T, TAB>(a, count, row, p, true, red);
+}
+
+template <typename T, int MODE, bool TA, bool TB>
+hipError_t launch_t(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
+    const T* a = static_cast<const T*>(g.a);
+    const T* b = static_cast<const T*>(g.b);
+    uint64_t n = g.end - g.begin;
+    bool vec = true;
+    if (MODE == 2 && (((uintptr_t)(a + g.begin) ^ (uintptr_t)(b + g.begin)) & 15u)) vec = false;
+    constexpr int U = (MODE == 2) ? 4 : 8;
+    uint64_t per_block = (uint64_t)kBlock * U * (16 / sizeof(T));
+    uint64_t want = (n + per_block - 1) / per_block;
+    uint64_t cap = g.blocks > 0 ? (uint64_t)g.blocks : (uint64_t)device_cu_count() * 8;
+    if (cap > scr.max_blocks) cap = scr.max_blocks;
+    uint64_t blocks = want < cap ? want : cap;
+    if (blocks < 1) blocks = 1;
+    ScanOut* parts = static_cast<ScanOut*>(scr.partials);
+    if (vec)
+        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                           a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
+    else
+        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, false>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                           a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_type(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
+    if (g.mode == 0) return g.table_a ? launch_t<T, 0, true, false>(g, scr, out, st)
+                                      : launch_t<T, 0, false, false>(g, scr, out, st);
+    if (g.mode == 1) return g.table_b ? launch_t<T, 1, false, true>(g, scr, out, st)
+                                      : launch_t<T, 1, false, false>(g, scr, out, st);
+    return launch_t<T, 2, false, false>(g, scr, out, st);
+}
+
+}  // namespace
+
+int device_cu_count() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+        cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    }
+    return cus;
+}
+
+hipError_t launch_scan(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
+    if (g.end < g.begin) return hipErrorInvalidValue;
+    if (g.mode == 2 && g.table_b) return hipErrorInvalidValue;
+    switch (g.dtype) {
+    case DT_F64: return launch_type<double>(g, scr, out, st);
+    case DT_F32: return launch_type<float>(g, scr, out, st);
+    case DT_I64: return launch_type<int64_t>(g, scr, out, st);
+    case DT_I32: return launch_type<int32_t>(g, scr, out, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_group_minmax(const GroupArgs& g, const Scratch& scr, GroupOut* out, hipStream_t st) {
+    if (g.groups == 0) return hipErrorInvalidValue;
+    uint64_t want = (g.groups + kBlock - 1) / kBlock;
+    uint64_t cap = g.blocks > 0 ? (uint64_t)g.blocks : (uint64_t)device_cu_count() * 8;
+    if (cap > scr.max_blocks) cap = scr.max_blocks;
+    unsigned blocks = (unsigned)(want < cap ? want : cap);
+    GroupOut* parts = static_cast<GroupOut*>(scr.partials);
+    int use_pred = 1;
+    if (g.dtype == DT_F64) {
+        if (g.use_table)
+            hipLaunchKernelGGL((k_group_minmax<double, true>), dim3(blocks), dim3(kBlock), 0, st,
+                               (const double*)g.a, g.count, g.off, g.groups, g.p, use_pred, g.red, g.acc0, parts, scr.ticket, out);
+        else
+            hipLaunchKernelGGL((k_group_minmax<double, false>), dim3(blocks), dim3(kBlock), 0, st,
+                               (const double*)g.a, g.count, g.off, g.groups, g.p, use_pred, g.red, g.acc0, parts, scr.ticket, out);
+    } else if (g.dtype == DT_I64) {
+        hipLaunchKernelGGL((k_group_minmax<int64_t, false>), dim3(blocks), dim3(kBlock), 0, st,
+                           (const int64_t*)g.a, g.count, g.off, g.groups, g.p, use_pred, g.red, g.acc0, parts, scr.ticket, out);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group_fold(const GroupArgs& g, uint64_t* result, hipStream_t st) {
+    if (g.groups == 0) return hipErrorInvalidValue;
+    if (g.dtype == DT_F64) {
+        if (g.use_table)
+            hipLaunchKernelGGL((k_group_fold<double, true>), dim3(1), dim3(64), 0, st,
+                               (const double*)g.a, g.count, g.off, g.groups, g.p, g.red, g.fold, g.acc0, result);
+        else
+            hipLaunchKernelGGL((k_group_fold<double, false>), dim3(1), dim3(64), 0, st,
+                               (const double*)g.a, g.count, g.off, g.groups, g.p, g.red, g.fold, g.acc0, result);
+    } else if (g.dtype == DT_I64) {
+        hipLaunchKernelGGL((k_group_fold<int64_t, false>), dim3(1), dim3(64), 0, st,
+                           (const int64_t*)g.a, g.count, g.off, g.groups, g.p, g.red, g.fold, g.acc0, result);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group_last(const GroupArgs& g, uint64_t* result, hipStream_t st) {
+    if (g.dtype == DT_F64) {
+        if (g.use_table)
+            hipLaunchKernelGGL((k_group_last<double, true>), dim3(1), dim3(64), 0, st,
+                               (const double*)g.a, g.count, g.off, g.p, g.red, result);
+        else
+            hipLaunchKernelGGL((k_group_last<double, false>), dim3(1), dim3(64), 0, st,
+                               (const double*)g.a, g.count, g.off, g.p, g.red, result);
+    } else if (g.dtype == DT_I64) {
+        hipLaunchKernelGGL((k_group_last<int64_t, false>), dim3(1), dim3(64), 0, st,
+                           (const int64_t*)g.a, g.count, g.off, g.p, g.red, result);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace vx

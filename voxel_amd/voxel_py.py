@@ -1,0 +1,232 @@
+"""voxel_py -- the Python module surface of VoxelVM (reference:
+bindings/voxel_bindings.cpp:25-396), backed by libvoxel_c on the GPU.
+
+Only the hot-path members are GPU work: EngineF64/EngineI64 ``run`` and
+``run_parallel``, ``jit_run``, ``sort_ascending``, ``HashAggregator``,
+``HashJoin``, ``VectorFilterF64``.  Everything here calls the C ABI in
+include/voxel_c.h through ctypes; nothing computes a query result in Python.
+
+Differences from the pybind module (documented per member):
+* ``add_segment`` keeps a reference to the array (the reference does not,
+  voxel_bindings.cpp:107-110, leaving a dangling pointer) and requires a
+  C-contiguous array of the engine's dtype (TypeError otherwise) instead of
+  silently converting to a temporary;
+* calls the GPU backend cannot lower raise RuntimeError with the reason.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+
+__version__ = "1.0.0"
+
+# Filter comparison constants, bytecode/opcodes.hpp:415-420 (voxel_bindings.cpp:48-51).
+CMP_EQ, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE = 0, 1, 2, 3, 4, 5
+
+
+class Opcode:
+    """The opcode values the pybind module exports (voxel_bindings.cpp:39-47)."""
+    NOP, HALT, MOV, ADD, SUB, MUL = 0x00, 0x01, 0x10, 0x20, 0x21, 0x22
+    CMP, JMP, JZ, JNZ = 0x40, 0x50, 0x51, 0x52
+    VLOAD, VSTORE, VADD, VFILTER_GT, VSUM, ADDF = 0x70, 0x71, 0x80, 0xC5, 0xD0, 0x2A
+
+
+def _word(op: int, rd: int = 0, ra: int = 0, rb: int = 0, imm: int = 0) -> int:
+    """[op:8][rd:4][ra:4][rb:4][imm12:12] (bytecode/instruction.hpp:64-71)."""
+    return ((op & 0xFF) | ((rd & 0xF) << 8) | ((ra & 0xF) << 12) | ((rb & 0xF) << 16)
+            | ((imm & 0xFFF) << 20))
+
+
+class Instruction:
+    """A 32-bit instruction word with the constructors voxel_py exposes."""
+
+    __slots__ = ("raw",)
+
+    def __init__(self, raw: int = 0):
+        self.raw = int(raw) & 0xFFFFFFFF
+
+    def op(self) -> int: return self.raw & 0xFF
+    def rd(self) -> int: return (self.raw >> 8) & 0xF
+    def ra(self) -> int: return (self.raw >> 12) & 0xF
+    def rb(self) -> int: return (self.raw >> 16) & 0xF
+    def imm12(self) -> int: return (self.raw >> 20) & 0xFFF
+
+    def __repr__(self) -> str:
+        return "Instruction(0x%08x)" % self.raw
+
+    # control / scalar
+    @staticmethod
+    def nop(): return Instruction(_word(0x00))
+    @staticmethod
+    def halt(): return Instruction(_word(0x01))
+    @staticmethod
+    def mov(rd, imm): return Instruction(_word(0x10, rd, 0, 0, imm))
+    @staticmethod
+    def movr(rd, ra): return Instruction(_word(0x11, rd, ra))
+    @staticmethod
+    def add(rd, ra, rb): return Instruction(_word(0x20, rd, ra, rb))
+    @staticmethod
+    def sub(rd, ra, rb): return Instruction(_word(0x21, rd, ra, rb))
+    @staticmethod
+    def mul(rd, ra, rb): return Instruction(_word(0x22, rd, ra, rb))
+    @staticmethod
+    def addf(rd, ra, rb): return Instruction(_word(0x2A, rd, ra, rb))
+    @staticmethod
+    def cmp(ra, rb): return Instruction(_word(0x40, 0, ra, rb))
+    @staticmethod
+    def jmp(off): return Instruction(_word(0x50, imm=off))
+    @staticmethod
+    def jz(off): return Instruction(_word(0x51, imm=off))
+    @staticmethod
+    def jnz(off): return Instruction(_word(0x52, imm=off))
+    @staticmethod
+    def jle(off): return Instruction(_word(0x5A, imm=off))
+    @staticmethod
+    def jge(off): return Instruction(_word(0x5C, imm=off))
+    # vector
+    @staticmethod
+    def vload(vd, ra, seg_id, count): return Instruction(_word(0x70, vd, ra, 0, ((seg_id & 0xF) << 8) | (count & 0xFF)))
+    @staticmethod
+    def vstore(vs, ra, seg_id, count): return Instruction(_word(0x71, vs, ra, 0, ((seg_id & 0xF) << 8) | (count & 0xFF)))
+    @staticmethod
+    def vadd(vd, va, vb): return Instruction(_word(0x80, vd, va, vb))
+    @staticmethod
+    def vsub(vd, va, vb): return Instruction(_word(0x81, vd, va, vb))
+    @staticmethod
+    def vmul(vd, va, vb): return Instruction(_word(0x82, vd, va, vb))
+    @staticmethod
+    def vmin(vd, va, vb): return Instruction(_word(0x87, vd, va, vb))
+    @staticmethod
+    def vmax(vd, va, vb): return Instruction(_word(0x88, vd, va, vb))
+    @staticmethod
+    def vfilter(vd, va, rb, mode): return Instruction(_word(0xC0, vd, va, rb, mode & 0x7))
+    @staticmethod
+    def vfilter_gt(vd, va, rb): return Instruction(_word(0xC5, vd, va, rb))
+    @staticmethod
+    def vfilter_ge(vd, va, rb): return Instruction(_word(0xC6, vd, va, rb))
+    @staticmethod
+    def vsum(rd, va): return Instruction(_word(0xD0, rd, va))
+    @staticmethod
+    def vred_min(rd, va): return Instruction(_word(0xD5, rd, va))
+    @staticmethod
+    def vred_max(rd, va): return Instruction(_word(0xD6, rd, va))
+    @staticmethod
+    def vcount(rd, va): return Instruction(_word(0xD7, rd, va))
+
+
+def _code_array(code) -> np.ndarray:
+    return np.ascontiguousarray(np.array([int(w) & 0xFFFFFFFF for w in code], dtype=np.uint32))
+
+
+def _as_column(arr, dtype) -> np.ndarray:
+    if not isinstance(arr, np.ndarray) or arr.dtype != dtype or not arr.flags.c_contiguous:
+        raise TypeError("add_segment needs a C-contiguous numpy array of %s" % np.dtype(dtype).name)
+    return arr
+
+
+class _Engine:
+    _dtype = np.float64
+    _create = "voxel_engine_create_f64"
+    _destroy = "voxel_engine_destroy"
+    _add = "voxel_engine_add_segment"
+    _load = "voxel_engine_load_program"
+    _run = "voxel_engine_run"
+    _set = "voxel_engine_set_scalar"
+    _get = "voxel_engine_get_scalar"
+
+    def __init__(self):
+        self._h = getattr(lib, self._create)()
+        self._keep = []          # borrowed arrays stay alive as long as the engine
+        self._code = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib, self._destroy)(h)
+            self._h = None
+
+    def add_segment(self, arr) -> int:
+        a = _as_column(arr, self._dtype)
+        self._keep.append(a)
+        return int(getattr(lib, self._add)(self._h, a.ctypes.data, a.size))
+
+    def load_program(self, code) -> None:
+        self._code = _code_array(code)
+        getattr(lib, self._load)(self._h, self._code.ctypes.data, self._code.size)
+
+    def run(self) -> None:
+        getattr(lib, self._run)(self._h)
+        check("run")
+
+    def get_scalar(self, i: int) -> int:
+        return int(getattr(lib, self._get)(self._h, i))
+
+    def set_scalar(self, i: int, v: int) -> None:
+        getattr(lib, self._set)(self._h, i, int(v) & 0xFFFFFFFFFFFFFFFF)
+
+
+class EngineF64(_Engine):
+    """Engine<f64> (voxel_bindings.cpp:105-171)."""
+    k_lanes = 4
+
+    def run_parallel(self, seg_id: int, threads: int = 0, result_reg: int = 0) -> None:
+        lib.voxel_engine_run_parallel(self._h, seg_id, threads, result_reg)
+        check("run_parallel")
+
+    def get_scalar_f64(self, i: int) -> float:
+        return float(lib.voxel_engine_get_scalar_f64(self._h, i))
+
+    def set_scalar_f64(self, i: int, v: float) -> None:
+        lib.voxel_engine_set_scalar_f64(self._h, i, float(v))
+
+
+class EngineI64(_Engine):
+    """Engine<i64> (voxel_bindings.cpp:174-189)."""
+    k_lanes = 4
+    _dtype = np.int64
+    _create = "voxel_engine_create_i64"
+    _destroy = "voxel_engine_destroy_i64"
+    _add = "voxel_engine_add_segment_i64"
+    _load = "voxel_engine_load_program_i64"
+    _run = "voxel_engine_run_i64"
+    _set = "voxel_engine_set_scalar_i64"
+    _get = "voxel_engine_get_scalar_i64"
+
+    def get_scalar_i64(self, i: int) -> int:
+        v = self.get_scalar(i)
+        return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def jit_compile(code) -> dict:
+    """voxel_bindings.cpp:329-340: reports whether the fused loop lowers.
+    No machine code exists: code_size and code_ptr are 0."""
+    c = _code_array(code)
+    ok = bool(lib.voxel_gpu_jit_lowerable(c.ctypes.data, c.size))
+    if not ok:
+        return {"compiled": False}
+    return {"compiled": True, "code_size": 0, "code_ptr": 0}
+
+
+def jit_run(code, data, threshold: float, total_count: int) -> float:
+    """voxel_bindings.cpp:342-366: R0 = R1 = 0, R2 = total_count,
+    R3 = threshold, one segment of len(data) rows; returns R0 as f64."""
+    c = _code_array(code)
+    d = _as_column(data, np.float64)
+    if not lib.voxel_gpu_jit_lowerable(c.ctypes.data, c.size):
+        raise RuntimeError("JIT compilation failed")
+    v = _jit_run_r2(c, d, float(threshold), int(total_count))
+    check("jit_run")
+    return v
+
+
+def _jit_run_r2(code: np.ndarray, data: np.ndarray, threshold: float, r2: int) -> float:
+    # voxel_jit_run sets R2 = data_len; the pybind form sets R2 = total_count
+    # and passes len(data) as the segment's row count.  Rows at or past
+    # len(data) read as zero; the loop bound comes from R2.
+    if r2 == data.size:
+        return float(lib.voxel_jit_run(code.ctypes.data, code.size, data.ctypes.data, data.size, threshold))
+    return float(lib.voxel_gpu_jit_run_r2(code.ctypes.data, code.size, data.ctypes.data, data.size,
+                                          C.c_uint64(r2 & 0xFFFFFFFFFFFFFFFF), threshold))
