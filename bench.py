@@ -120,10 +120,15 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
     import torch
-    from voxel_amd import scan
-    from voxel_amd._lib import lib
-    if not torch.cuda.is_available() or lib.voxel_gpu_available() != 1:
+    if not torch.cuda.is_available():
         print("no GPU available", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.init()
+    torch.empty(1, device="cuda")
+    from voxel_amd import scan
+    from voxel_amd._lib import lib, last_error
+    if lib.voxel_gpu_available() != 1:
+        print("libvoxel_c: " + last_error(), file=sys.stderr)
         sys.exit(1)
     stream = torch.cuda.current_stream().cuda_stream
     n = 1 << args.rows_log2
@@ -176,7 +181,7 @@ def main():
     out = {
         "metric": METRIC,
         "value": round(100.0 * gbs_wall / HBM_PEAK_GBS, 3),
-        "unit": "% of 8 TB/s HBM peak (f64 compound filter+SUM/COUNT/MIN/MAX, 2 columns x 2^%d rows)" % args.rows_log2,
+        "unit": "%% of 8 TB/s HBM peak (f64 compound filter+SUM/COUNT/MIN/MAX, 2 columns x 2^%d rows)" % args.rows_log2,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,

@@ -19,7 +19,9 @@
  *  - errors set a thread-local message readable with voxel_last_error();
  *    functions returning int return 0 on success and -1 on error;
  *  - "device pointer" means memory on the current HIP device (hipMalloc'ed,
- *    or a torch CUDA tensor's data_ptr()).
+ *    or a torch CUDA tensor's data_ptr());
+ *  - a `void* stream` argument is a hipStream_t; NULL means the device's
+ *    default stream (also used by every Part 1 function).
  */
 #pragma once
 
@@ -192,6 +194,15 @@ double    voxel_gpu_jit_run_r2(const uint32_t* code, size_t code_len, const doub
  * loop, 0 not lowerable (run will fail with the reason). */
 int       voxel_gpu_engine_lowered(const void* engine);
 
+/* Stable sort of (u64 key, u32 payload) pairs by key (RadixSort<u64>::Sort
+ * with indices, ops/sort.hpp:83-160) over device buffers; vals_in NULL
+ * means payload = row index.  Inputs are not modified.  n < 2^30. */
+int       voxel_gpu_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                                   uint32_t* vals_out, size_t n, void* stream);
+/* Stable f64 argsort over a device column (SortOperator order). */
+int       voxel_gpu_argsort_f64(const double* x_dev, size_t n, int descending, uint32_t* idx_out_dev,
+                                void* stream);
+
 /* Relational filter + aggregate over device columns:
  * rows with op_a(a[i], t_a) AND op_b(b[i], t_b); COUNT, SUM(a), MIN(a),
  * MAX(a).  Thresholds are bit patterns of the column type (f32/i32 in the
@@ -218,7 +229,7 @@ typedef struct voxel_gpu_agg {
     int32_t  pad;
 } voxel_gpu_agg;
 
-/* Launch + wait + read back. stream may be NULL (library stream). */
+/* Launch + wait + read back. */
 int       voxel_gpu_scan(const voxel_gpu_scan_desc* d, void* stream, voxel_gpu_agg* out);
 /* Enqueue only (result stays on the device until voxel_gpu_scan_result). */
 int       voxel_gpu_scan_launch(const voxel_gpu_scan_desc* d, void* stream);

@@ -32,6 +32,9 @@ def gpu():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    from voxel_amd._lib import lib
-    assert lib.voxel_gpu_available() == 1, "libvoxel_c found no usable GPU"
+    torch.cuda.init()
+    torch.empty(1, device="cuda")
+    torch.cuda.synchronize()
+    from voxel_amd._lib import lib, last_error
+    assert lib.voxel_gpu_available() == 1, "libvoxel_c found no usable GPU: " + last_error()
     return lib

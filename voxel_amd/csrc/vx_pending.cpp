@@ -9,7 +9,4 @@ void voxel_agg_destroy(void*) {}
 void voxel_agg_init(void*, size_t) { vx::set_error("voxel_agg_*: GPU group-by not built"); }
 void voxel_agg_accumulate(void*, const uint32_t*, const double*, size_t) { vx::set_error("voxel_agg_*: GPU group-by not built"); }
 size_t voxel_agg_group_count(const void*) { vx::set_error("voxel_agg_*: GPU group-by not built"); return 0; }
-void voxel_sort_ascending(const double*, size_t, uint32_t*) { vx::set_error("voxel_sort_*: GPU sort not built"); }
-void voxel_sort_descending(const double*, size_t, uint32_t*) { vx::set_error("voxel_sort_*: GPU sort not built"); }
-void voxel_topk(const double*, size_t, size_t, double*, int) { vx::set_error("voxel_topk: GPU top-k not built"); }
 }

@@ -16,7 +16,6 @@ namespace {
 thread_local char g_err[512] = "";
 
 struct Runtime {
-    bool tried = false;
     bool ok = false;
     int device = 0;
     StreamCtx own;
@@ -68,20 +67,20 @@ std::mutex& api_mutex() {
 bool runtime_ok() {
     Runtime& r = rt();
     std::lock_guard<std::mutex> g(g_init_mutex);
-    if (r.tried) {
-        if (!r.ok) set_error("no usable HIP device (voxel_amd needs an MI355X/gfx950 GPU)");
-        return r.ok;
-    }
-    r.tried = true;
+    if (r.ok) return true;
+    // (a failed bring-up is retried on the next call)
+    hipError_t ie = hipInit(0);
     int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
-        set_error("no usable HIP device (voxel_amd needs an MI355X/gfx950 GPU)");
+    hipError_t ce = hipGetDeviceCount(&n);
+    if (ie != hipSuccess || ce != hipSuccess || n < 1) {
+        set_error("no usable HIP device (voxel_amd needs an MI355X/gfx950 GPU): hipInit %s, hipGetDeviceCount %s, %d devices",
+                  hipGetErrorString(ie), hipGetErrorString(ce), n);
         return false;
     }
     if (!hip_ok(hipGetDevice(&r.device), "hipGetDevice")) return false;
-    hipStream_t s = nullptr;
-    if (!hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) return false;
-    if (!init_ctx(&r.own, s)) return false;
+    // NULL stream = the device's default stream, so that work a caller queued
+    // there (e.g. torch on its default stream) is ordered before ours.
+    if (!init_ctx(&r.own, nullptr)) return false;
     r.ok = true;
     return true;
 }

@@ -37,7 +37,7 @@ struct StreamCtx {
 
 // Returns false (and sets the error) when no usable HIP device exists.
 bool runtime_ok();
-// The library's own stream context, or the context of a caller stream.
+// Context of a caller stream; NULL selects the device's default stream.
 StreamCtx* ctx_for(void* stream);
 // Serialises library calls that share scratch (all C ABI entry points).
 std::mutex& api_mutex();
