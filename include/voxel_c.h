@@ -211,7 +211,9 @@ int       voxel_gpu_argsort_f64(const double* x_dev, size_t n, int descending, u
 typedef struct voxel_gpu_scan_desc {
     int32_t  dtype;
     int32_t  op_a, op_b;       /* op_b = VOXEL_CMP_TRUE when b is NULL */
-    int32_t  blocks;           /* 0 = library default */
+    int32_t  blocks;           /* workgroups; 0 = library default */
+    int32_t  unroll;           /* 16-byte loads in flight per lane and column: 2, 4, 8; 0 = default */
+    int32_t  nontemporal;      /* 1 = non-temporal (streaming) loads */
     const void* a;
     const void* b;
     uint64_t n;
@@ -234,6 +236,49 @@ int       voxel_gpu_scan(const voxel_gpu_scan_desc* d, void* stream, voxel_gpu_a
 /* Enqueue only (result stays on the device until voxel_gpu_scan_result). */
 int       voxel_gpu_scan_launch(const voxel_gpu_scan_desc* d, void* stream);
 int       voxel_gpu_scan_result(void* stream, voxel_gpu_agg* out);
+
+/* ---- bitmap filters, ColumnFilter algebra, compaction ------------------ */
+/* Predicate code 7 = Between: t <= x <= t2 (VectorFilter::ApplyBetween). */
+enum { VOXEL_CMP_BETWEEN = 7 };
+
+/* VectorFilter<T>::Apply* (ops/filter.hpp:42-288) over a device column:
+ * words_dev[i/64] bit (i%64) = pred(x_i), (n+63)/64 words, tail bits zero;
+ * *pass_count = number of set bits (FilterResult::PassCount). */
+int       voxel_gpu_filter_bitmap(int dtype, const void* col, size_t n, int op, uint64_t t, uint64_t t2,
+                                  uint64_t* words_dev, uint64_t* pass_count, void* stream);
+/* The compound predicate op_a(a, t_a) AND op_b(b, t_b) fused into one bitmap
+ * (= ApplyXX(a) then ColumnFilter::AndWith(ApplyYY(b))). */
+int       voxel_gpu_filter_bitmap2(int dtype, const void* a, const void* b, size_t n, int op_a, uint64_t t_a,
+                                   int op_b, uint64_t t_b, uint64_t* words_dev, uint64_t* pass_count,
+                                   void* stream);
+/* ColumnFilter::AndWith / OrWith / AndNotWith (combine 0 / 1 / 2) in place on
+ * active_dev, tail bits past `rows` cleared; *active_count = set bits. */
+int       voxel_gpu_bitmap_combine(uint64_t* active_dev, const uint64_t* other_dev, size_t rows, int combine,
+                                   uint64_t* active_count, void* stream);
+/* SelectionVector::BuildFromBitmap (ops/filter.hpp:378-399): ascending u32
+ * row ids of the set bits. */
+int       voxel_gpu_select_from_bitmap(const uint64_t* words_dev, size_t rows, uint32_t* idx_out_dev,
+                                       uint64_t* count, void* stream);
+/* scalar_filter_* (simd/scalar.hpp:155-333) generalised to every predicate:
+ * passing values in row order into values_out_dev and/or their row ids into
+ * idx_out_dev (either may be NULL). */
+int       voxel_gpu_compact(int dtype, const void* col, size_t n, int op, uint64_t t, void* values_out_dev,
+                            uint32_t* idx_out_dev, uint64_t* count, void* stream);
+
+/* ---- hash join (HashJoin<i64>, ops/join.hpp:36-80) ------------------------ */
+/* Builds a table over device keys (payload_dev may be NULL: payload = row). */
+void*     voxel_gpu_join_build(const int64_t* build_keys_dev, const int64_t* payload_dev, size_t nb, void* stream);
+void      voxel_gpu_join_destroy(void* join);
+/* Inner join: number of probe rows with a match, and the wrapping i64 sum of
+ * the matched build rows' payloads (first match = highest build row id among
+ * equal keys, as the reference's LIFO chains give). */
+int       voxel_gpu_join_probe_sum(void* join, const int64_t* probe_dev, size_t np, uint64_t* match_count,
+                                   int64_t* payload_sum, void* stream);
+int       voxel_gpu_join_probe_sum_launch(void* join, const int64_t* probe_dev, size_t np, void* stream);
+int       voxel_gpu_join_probe_sum_result(void* join, uint64_t* match_count, int64_t* payload_sum, void* stream);
+/* HashJoin::Probe's output: (probe row, build row) pairs in probe order. */
+int       voxel_gpu_join_probe_pairs(void* join, const int64_t* probe_dev, size_t np, uint32_t* left_dev,
+                                     uint32_t* right_dev, uint64_t* match_count, void* stream);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvoxel_c.so")
 HEADER = os.path.join(HERE, "..", "include", "voxel_c.h")
 
+# torch ships its own HIP runtime (same soname as /opt/rocm's).  Loading torch
+# first makes this library bind to that already-loaded runtime, so a process
+# that uses both holds one HIP runtime instead of two that contend for the GPU.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         "voxel_amd: %s is missing; build it with `python -m voxel_amd.build` "
@@ -27,6 +35,7 @@ P, U8, U16, U32, U64, I16, I32, I64, SZ, F64, INT = (
 
 class ScanDesc(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("op_a", C.c_int32), ("op_b", C.c_int32), ("blocks", C.c_int32),
+                ("unroll", C.c_int32), ("nontemporal", C.c_int32),
                 ("a", P), ("b", P), ("n", U64), ("t_a", U64), ("t_b", U64)]
 
 
@@ -128,6 +137,17 @@ SIGS = {
     "voxel_gpu_engine_lowered": (INT, [P]),
     "voxel_gpu_sort_pairs_u64": (INT, [P, P, P, P, SZ, P]),
     "voxel_gpu_argsort_f64": (INT, [P, SZ, INT, P, P]),
+    "voxel_gpu_filter_bitmap": (INT, [INT, P, SZ, INT, U64, U64, P, P, P]),
+    "voxel_gpu_filter_bitmap2": (INT, [INT, P, P, SZ, INT, U64, INT, U64, P, P, P]),
+    "voxel_gpu_bitmap_combine": (INT, [P, P, SZ, INT, P, P]),
+    "voxel_gpu_select_from_bitmap": (INT, [P, SZ, P, P, P]),
+    "voxel_gpu_compact": (INT, [INT, P, SZ, INT, U64, P, P, P, P]),
+    "voxel_gpu_join_build": (P, [P, P, SZ, P]),
+    "voxel_gpu_join_destroy": (None, [P]),
+    "voxel_gpu_join_probe_sum": (INT, [P, P, SZ, P, P, P]),
+    "voxel_gpu_join_probe_sum_launch": (INT, [P, P, SZ, P]),
+    "voxel_gpu_join_probe_sum_result": (INT, [P, P, P, P]),
+    "voxel_gpu_join_probe_pairs": (INT, [P, P, SZ, P, P, P, P]),
     "voxel_gpu_scan": (INT, [C.POINTER(ScanDesc), P, C.POINTER(Agg)]),
     "voxel_gpu_scan_launch": (INT, [C.POINTER(ScanDesc), P]),
     "voxel_gpu_scan_result": (INT, [P, C.POINTER(Agg)]),

@@ -65,6 +65,8 @@ struct ScanArgs {
     int table_b;           // pb uses the table form (mode 1/2)
     int table_a;           // pa uses the table form
     int blocks;            // 0 = pick
+    int unroll;            // 16-byte loads in flight per lane and column: 2, 4, 8 (0 = default)
+    int nontemporal;       // 1 = non-temporal loads
 };
 
 // Fused filter + COUNT/SUM/MIN/MAX over one range; result in *out_dev

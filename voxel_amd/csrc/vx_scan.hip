@@ -16,6 +16,7 @@
 //     deterministic final reduction by the last workgroup to finish (ticket),
 //     so one launch produces the result and the order of the final combine
 //     depends only on the grid size.
+#include "vx_device.hpp"
 #include "vx_kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -29,61 +30,6 @@ namespace vx {
 namespace {
 
 constexpr int kBlock = 256;
-
-template <typename T> struct TyInfo;
-template <> struct TyInfo<double>  { using V = double2;   static constexpr int E = 2; using S = double;   static constexpr bool F = true; };
-template <> struct TyInfo<float>   { using V = float4;    static constexpr int E = 4; using S = float;    static constexpr bool F = true; };
-template <> struct TyInfo<int64_t> { using V = longlong2; static constexpr int E = 2; using S = uint64_t; static constexpr bool F = false; };
-template <> struct TyInfo<int32_t> { using V = int4;      static constexpr int E = 4; using S = uint64_t; static constexpr bool F = false; };
-
-template <typename T> __device__ __forceinline__ T from_bits(uint64_t b);
-template <> __device__ __forceinline__ double from_bits<double>(uint64_t b) { return __longlong_as_double((long long)b); }
-template <> __device__ __forceinline__ float from_bits<float>(uint64_t b) { return __uint_as_float((uint32_t)b); }
-template <> __device__ __forceinline__ int64_t from_bits<int64_t>(uint64_t b) { return (int64_t)b; }
-template <> __device__ __forceinline__ int32_t from_bits<int32_t>(uint64_t b) { return (int32_t)(uint32_t)b; }
-template <> __device__ __forceinline__ uint64_t from_bits<uint64_t>(uint64_t b) { return b; }
-
-__device__ __forceinline__ uint64_t to_bits(double v) { return (uint64_t)__double_as_longlong(v); }
-__device__ __forceinline__ uint64_t to_bits(float v) { return (uint64_t)__float_as_uint(v); }
-__device__ __forceinline__ uint64_t to_bits(int64_t v) { return (uint64_t)v; }
-__device__ __forceinline__ uint64_t to_bits(int32_t v) { return (uint64_t)(uint32_t)v; }
-__device__ __forceinline__ uint64_t to_bits(uint64_t v) { return v; }
-
-template <typename T> struct Lim;
-template <> struct Lim<double>  { __device__ static double lo() { return -__builtin_huge_val(); } __device__ static double hi() { return __builtin_huge_val(); } };
-template <> struct Lim<float>   { __device__ static float lo() { return -__builtin_huge_valf(); } __device__ static float hi() { return __builtin_huge_valf(); } };
-template <> struct Lim<int64_t> { __device__ static int64_t lo() { return INT64_MIN; } __device__ static int64_t hi() { return INT64_MAX; } };
-template <> struct Lim<int32_t> { __device__ static int32_t lo() { return INT32_MIN; } __device__ static int32_t hi() { return INT32_MAX; } };
-
-// Predicate, range or truth-table form (PredDev in vx_kernels.hpp).
-template <typename T, bool TABLE>
-struct Pred {
-    T lo, hi, t;
-    bool inv;
-    uint32_t table;
-    __device__ __forceinline__ void load(const PredDev& p) {
-        lo = from_bits<T>(p.lo); hi = from_bits<T>(p.hi); t = from_bits<T>(p.t);
-        inv = p.inv != 0; table = p.table;
-    }
-    __device__ __forceinline__ bool operator()(T x) const {
-        if constexpr (TABLE) {
-            uint32_t st = (x < t ? 1u : 0u) | (x == t ? 2u : 0u) | (x > t ? 4u : 0u);
-            return (table >> st) & 1u;
-        } else {
-            return ((x >= lo) & (x <= hi)) != inv;
-        }
-    }
-};
-
-// Compensated summation step (Knuth TwoSum): s + c tracks the exact sum.
-template <typename S>
-__device__ __forceinline__ void two_sum(S& s, S& c, S v) {
-    S t = s + v;
-    S bp = t - s;
-    S e = (s - (t - bp)) + (v - bp);
-    s = t;
-    c += e;
-}
 
 template <typename T>
 struct Acc {
@@ -160,15 +106,27 @@ __device__ __forceinline__ bool publish_and_check_last(uint32_t* ticket, bool* s
     return last;
 }
 
-template <typename T, int MODE, bool TA, bool TB, bool VEC>
+// 16-byte vector load; NT marks it non-temporal (streamed, not kept in cache).
+template <typename V, bool NT>
+__device__ __forceinline__ V ld16(const V* p) {
+    if constexpr (NT && sizeof(V) == 16) {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        u4 r = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+        V v;
+        __builtin_memcpy(&v, &r, 16);
+        return v;
+    } else {
+        return *p;
+    }
+}
+
+template <typename T, int MODE, bool TA, bool TB, bool VEC, int U, bool NT>
 __global__ __launch_bounds__(kBlock)
 void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, uint64_t end,
             PredDev pda, PredDev pdb, ScanOut* __restrict__ parts, uint32_t* __restrict__ ticket,
             ScanOut* __restrict__ out) {
     using V = typename std::conditional<VEC, typename TyInfo<T>::V, T>::type;
     constexpr int E = VEC ? TyInfo<T>::E : 1;
-    constexpr int U = (MODE == 2) ? 4 : 8;
-
     Pred<T, TA> pa; pa.load(pda);
     Pred<T, TB> pb; pb.load(pdb);
     Acc<T> acc; acc.init();
@@ -206,10 +164,10 @@ void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, ui
         V xa[U], xb[U];
         if (base + tile <= nvec) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) xa[u] = av[base + u * kBlock + threadIdx.x];
+            for (int u = 0; u < U; ++u) xa[u] = ld16<V, NT>(&av[base + u * kBlock + threadIdx.x]);
             if constexpr (MODE == 2) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) xb[u] = bv[base + u * kBlock + threadIdx.x];
+                for (int u = 0; u < U; ++u) xb[u] = ld16<V, NT>(&bv[base + u * kBlock + threadIdx.x]);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -405,14 +363,13 @@ void k_group_last(const T* __restrict__ a, uint64_t count, uint64_t row, PredDev
     if (threadIdx.x == 0) result[0] = group_value<T, TAB>(a, count, row, p, true, red);
 }
 
-template <typename T, int MODE, bool TA, bool TB>
-hipError_t launch_t(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
+template <typename T, int MODE, bool TA, bool TB, int U, bool NT>
+hipError_t launch_u(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
     const T* a = static_cast<const T*>(g.a);
     const T* b = static_cast<const T*>(g.b);
     uint64_t n = g.end - g.begin;
     bool vec = true;
     if (MODE == 2 && (((uintptr_t)(a + g.begin) ^ (uintptr_t)(b + g.begin)) & 15u)) vec = false;
-    constexpr int U = (MODE == 2) ? 4 : 8;
     uint64_t per_block = (uint64_t)kBlock * U * (16 / sizeof(T));
     uint64_t want = (n + per_block - 1) / per_block;
     uint64_t cap = g.blocks > 0 ? (uint64_t)g.blocks : (uint64_t)device_cu_count() * 8;
@@ -421,12 +378,27 @@ hipError_t launch_t(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStre
     if (blocks < 1) blocks = 1;
     ScanOut* parts = static_cast<ScanOut*>(scr.partials);
     if (vec)
-        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, true, U, NT>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
                            a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
     else
-        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, false>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, false, U, NT>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
                            a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
     return hipGetLastError();
+}
+
+// Unroll (16-byte loads in flight per lane and column) and non-temporal
+// loads are tunable; unroll 0 selects the default for the mode.
+template <typename T, int MODE, bool TA, bool TB>
+hipError_t launch_t(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
+    int u = g.unroll > 0 ? g.unroll : (MODE == 2 ? 4 : 8);
+    bool nt = g.nontemporal > 0;
+    if (TA || TB) { u = 4; nt = false; }    // table predicates: compat paths, one variant
+    switch (u) {
+    case 2: return nt ? launch_u<T, MODE, TA, TB, 2, true>(g, scr, out, st) : launch_u<T, MODE, TA, TB, 2, false>(g, scr, out, st);
+    case 4: return nt ? launch_u<T, MODE, TA, TB, 4, true>(g, scr, out, st) : launch_u<T, MODE, TA, TB, 4, false>(g, scr, out, st);
+    case 8: return nt ? launch_u<T, MODE, TA, TB, 8, true>(g, scr, out, st) : launch_u<T, MODE, TA, TB, 8, false>(g, scr, out, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename T>
