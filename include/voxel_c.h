@@ -213,7 +213,9 @@ typedef struct voxel_gpu_scan_desc {
     int32_t  op_a, op_b;       /* op_b = VOXEL_CMP_TRUE when b is NULL */
     int32_t  blocks;           /* workgroups; 0 = library default */
     int32_t  unroll;           /* 16-byte loads in flight per lane and column: 2, 4, 8; 0 = default */
-    int32_t  nontemporal;      /* 1 = non-temporal (streaming) loads */
+    int32_t  nontemporal;      /* 1 = non-temporal (streaming) loads, -1 = cached loads, 0 = default */
+    int32_t  layout;           /* 0 = grid-stride tiles, 1 = contiguous run of tiles per workgroup */
+    int32_t  pad_;
     const void* a;
     const void* b;
     uint64_t n;
@@ -279,6 +281,24 @@ int       voxel_gpu_join_probe_sum_result(void* join, uint64_t* match_count, int
 /* HashJoin::Probe's output: (probe row, build row) pairs in probe order. */
 int       voxel_gpu_join_probe_pairs(void* join, const int64_t* probe_dev, size_t np, uint32_t* left_dev,
                                      uint32_t* right_dev, uint64_t* match_count, void* stream);
+
+/* ---- group-by (HashAggregator<TKey,f64>, ops/aggregate.hpp:71-301) ----- */
+/* GROUP BY key: COUNT, SUM (compensated), AVG = SUM/COUNT, MIN, MAX of an f64
+ * column.  key_dtype VOXEL_GPU_I32, VOXEL_GPU_U32 or VOXEL_GPU_I64; output
+ * device arrays (any may be NULL) of max_groups entries, ascending key order.
+ * *ngroups is set even when it exceeds max_groups (then -1 is returned). */
+int       voxel_gpu_groupby_f64(int key_dtype, const void* keys_dev, const double* vals_dev, size_t n,
+                                size_t max_groups, int64_t* out_keys, uint64_t* out_count, double* out_sum,
+                                double* out_avg, double* out_min, double* out_max, uint64_t* ngroups,
+                                void* stream);
+/* Same result through the sort-based path whatever the key range. */
+int       voxel_gpu_groupby_f64_general(int key_dtype, const void* keys_dev, const double* vals_dev, size_t n,
+                                        size_t max_groups, int64_t* out_keys, uint64_t* out_count,
+                                        double* out_sum, double* out_avg, double* out_min, double* out_max,
+                                        uint64_t* ngroups, void* stream);
+/* Per-group results of a voxel_agg_* aggregator, into host arrays. */
+int       voxel_gpu_agg_results(void* agg, int64_t* keys, uint64_t* counts, double* sums, double* avgs,
+                                double* mins, double* maxs, size_t max_groups);
 
 #ifdef __cplusplus
 }

@@ -62,7 +62,14 @@ def test_bitmap_filter_all_ops(gpu, golden, dt, n):
         rc = lib.voxel_gpu_filter_bitmap(DT[np.dtype(dt)], d.data_ptr(), n, op, tbits(dt, t), tbits(dt, t2),
                                          words.data_ptr(), pc.ctypes.data, None)
         assert rc == 0
-        w_ref, pc_ref = oracle_bitmap(x, 6 if op == 7 else op, t, t2)
+        if op == 6:                      # no predicate: every row
+            w_ref = np.zeros(nw, np.uint64)
+            bits = np.zeros(nw * 64, np.uint8)
+            bits[:n] = 1
+            w_ref = np.packbits(bits, bitorder="little").view(np.uint64)
+            pc_ref = n
+        else:
+            w_ref, pc_ref = oracle_bitmap(x, 6 if op == 7 else op, t, t2)
         assert int(pc[0]) == pc_ref, op
         assert np.array_equal(host(words, nw, np.uint64), w_ref), op
 
@@ -72,7 +79,8 @@ def test_bitmap_golden_and_compound(gpu, golden):
     g = golden["bitmap_gt4"]
     x = np.array(g["data"], np.float64)
     words, pc = u64(1), np.zeros(1, np.uint64)
-    lib.voxel_gpu_filter_bitmap(0, dev(x).data_ptr(), x.size, 4, tbits(np.float64, 4.0), 0, words.data_ptr(),
+    dx = dev(x)
+    lib.voxel_gpu_filter_bitmap(0, dx.data_ptr(), x.size, 4, tbits(np.float64, 4.0), 0, words.data_ptr(),
                                 pc.ctypes.data, None)
     assert int(words[0].item()) == g["word"] and int(pc[0]) == g["pass"]
     rng = np.random.default_rng(1)
@@ -80,7 +88,8 @@ def test_bitmap_golden_and_compound(gpu, golden):
     a, b = rng.random(n) * 1000, rng.random(n) * 1000
     nw = (n + 63) // 64
     words = u64(nw)
-    lib.voxel_gpu_filter_bitmap2(0, dev(a).data_ptr(), dev(b).data_ptr(), n, 4, tbits(np.float64, 292.8932188), 2,
+    da, db = dev(a), dev(b)                # keep both alive for the call
+    lib.voxel_gpu_filter_bitmap2(0, da.data_ptr(), db.data_ptr(), n, 4, tbits(np.float64, 292.8932188), 2,
                                  tbits(np.float64, 707.1067812), words.data_ptr(), pc.ctypes.data, None)
     wa, _ = oracle_bitmap(a, 4, 292.8932188)
     wb, _ = oracle_bitmap(b, 2, 707.1067812)

@@ -1,0 +1,172 @@
+"""GPU parity of the group-by (vx_groupby.hip) through the C ABI.
+
+Bars: COUNT, MIN, MAX bit-exact with the oracle (vo_groupby_f64, pinned against
+the reference's HashAggregator through oracle/_ref); SUM within 2 ulp of the
+exact per-group sum (oracle long accumulator); AVG = SUM/COUNT of those.
+Groups are compared in key order (the reference emits slot order,
+ops/aggregate.hpp:249-261).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_groups(keys, vals, signed=True):
+    n = keys.size
+    G = 1 << 16
+    ok, oc = np.zeros(G, np.uint32), np.zeros(G, np.uint64)
+    osum, omin, omax, omean, om2 = (np.zeros(G, np.float64) for _ in range(5))
+    dig = np.zeros(G * o.NDIG, np.int64)
+    k32 = keys.astype(np.int32) if signed else keys.astype(np.uint32)
+    g = o.O.vo_groupby_f64(o.ptr(k32.view(np.uint32)), o.ptr(vals), n, 1 if signed else 0, G, o.ptr(ok), o.ptr(oc),
+                           o.ptr(osum), o.ptr(omin), o.ptr(omax), o.ptr(omean), o.ptr(om2), o.ptr(dig))
+    exact = [o.digits_to_fraction(dig[i * o.NDIG:(i + 1) * o.NDIG]) for i in range(g)]
+    kk = ok[:g].view(np.int32) if signed else ok[:g]
+    return kk.astype(np.int64), oc[:g], exact, omin[:g], omax[:g]
+
+
+def gpu_groupby(keys, vals, key_dtype=3, general=False, max_groups=1 << 16):
+    import torch
+    from voxel_amd._lib import lib
+    kd = torch.from_numpy(keys).cuda()
+    vd = torch.from_numpy(vals).cuda()
+    out = {k: torch.zeros(max_groups, dtype=torch.int64 if k in ("key", "count") else torch.float64, device="cuda")
+           for k in ("key", "count", "sum", "avg", "min", "max")}
+    ng = np.zeros(1, np.uint64)
+    fn = lib.voxel_gpu_groupby_f64_general if general else lib.voxel_gpu_groupby_f64
+    rc = fn(key_dtype, kd.data_ptr(), vd.data_ptr(), keys.size, max_groups, *(out[k].data_ptr() for k in
+            ("key", "count", "sum", "avg", "min", "max")), ng.ctypes.data, None)
+    assert rc == 0, lib.voxel_last_error()
+    g = int(ng[0])
+    return {k: v[:g].cpu().numpy() for k, v in out.items()}
+
+
+def check(res, keys, vals, signed=True):
+    k, c, exact, mn, mx = oracle_groups(keys, vals, signed)
+    assert np.array_equal(res["key"], k)
+    assert np.array_equal(res["count"].astype(np.uint64), c)
+    assert np.array_equal(res["min"], mn) and np.array_equal(res["max"], mx)
+    for i in range(k.size):
+        e = float(exact[i])
+        assert abs(res["sum"][i] - e) <= 2 * o.ulp(e), (i, res["sum"][i], e)
+        assert abs(res["avg"][i] - e / c[i]) <= 4 * o.ulp(e / c[i])
+
+
+def test_agg_abi_golden(gpu, golden):
+    from voxel_amd._lib import lib
+    for name in ("agg_small", "agg_c_test"):
+        g = golden[name]
+        keys = np.array(g["keys"], np.uint32)
+        vals = np.array(g["vals"], np.float64)
+        a = lib.voxel_agg_create()
+        lib.voxel_agg_init(a, 10)
+        lib.voxel_agg_accumulate(a, keys.ctypes.data, vals.ctypes.data, keys.size)
+        assert lib.voxel_last_error() in (b"", None)
+        ref = sorted(g["groups"], key=lambda d: d["key"])
+        assert lib.voxel_agg_group_count(a) == len(ref)
+        n = len(ref)
+        k, c = np.zeros(n, np.int64), np.zeros(n, np.uint64)
+        s, av, mn, mx = (np.zeros(n, np.float64) for _ in range(4))
+        assert lib.voxel_gpu_agg_results(a, *(x.ctypes.data for x in (k, c, s, av, mn, mx)), n) == 0
+        for i, d in enumerate(ref):
+            assert (k[i], c[i], s[i], mn[i], mx[i]) == (d["key"], d["count"], d["sum"], d["min"], d["max"])
+            assert av[i] == d["sum"] / d["count"]
+        lib.voxel_agg_destroy(a)
+
+
+def test_agg_abi_batches_merge(gpu):
+    from voxel_amd._lib import lib
+    rng = np.random.default_rng(4)
+    a = lib.voxel_agg_create()
+    lib.voxel_agg_init(a, 128)
+    allk, allv = [], []
+    for b in range(5):
+        n = int(rng.integers(1, 50_000))
+        keys = rng.integers(0, 300 if b % 2 else 5000, n).astype(np.uint32)   # dense and sparse batches
+        vals = rng.standard_normal(n) * 100
+        lib.voxel_agg_accumulate(a, keys.ctypes.data, vals.ctypes.data, n)
+        assert lib.voxel_last_error() in (b"", None)
+        allk.append(keys); allv.append(vals)
+    keys, vals = np.concatenate(allk), np.concatenate(allv)
+    kk, c, exact, mn, mx = oracle_groups(keys, vals, signed=False)
+    G = lib.voxel_agg_group_count(a)
+    assert G == kk.size
+    k, cc = np.zeros(G, np.int64), np.zeros(G, np.uint64)
+    s, av, mmn, mmx = (np.zeros(G, np.float64) for _ in range(4))
+    assert lib.voxel_gpu_agg_results(a, *(x.ctypes.data for x in (k, cc, s, av, mmn, mmx)), G) == 0
+    assert np.array_equal(k, kk) and np.array_equal(cc, c) and np.array_equal(mmn, mn) and np.array_equal(mmx, mx)
+    for i in range(G):
+        assert abs(s[i] - float(exact[i])) <= 2 * o.ulp(float(exact[i]))
+    lib.voxel_agg_init(a, 16)                       # Init starts over
+    assert lib.voxel_agg_group_count(a) == 0
+    lib.voxel_agg_destroy(a)
+
+
+@pytest.mark.parametrize("n", [1, 100, 65_537, 3_000_000])
+def test_groupby_zones_dense_and_general(gpu, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(1, 266, n).astype(np.int32)              # zone 1..265
+    vals = rng.lognormal(2.5, 0.6, n)                             # fare
+    dense = gpu_groupby(keys, vals)
+    check(dense, keys, vals)
+    general = gpu_groupby(keys, vals, general=True)
+    check(general, keys, vals)
+    assert np.array_equal(dense["count"], general["count"]) and np.array_equal(dense["min"], general["min"])
+
+
+def test_groupby_sparse_and_signed_keys(gpu):
+    rng = np.random.default_rng(7)
+    n = 200_000
+    keys = rng.integers(-2**31, 2**31, n).astype(np.int32)
+    keys[::3] = keys[5]                                            # one heavy group
+    vals = rng.standard_normal(n)
+    vals[::101] = -0.0
+    check(gpu_groupby(keys, vals, max_groups=n), keys, vals)
+    ukeys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    check(gpu_groupby(ukeys, vals, key_dtype=5, max_groups=n), ukeys, vals, signed=False)
+
+
+def test_groupby_skewed_single_group_and_deterministic(gpu):
+    n = 1_000_003
+    keys = np.full(n, 42, np.int32)                                # every lane of a wave conflicts
+    vals = np.random.default_rng(1).standard_normal(n) * 1e6
+    r1 = gpu_groupby(keys, vals)
+    r2 = gpu_groupby(keys, vals)
+    check(r1, keys, vals)
+    assert r1["sum"].tobytes() == r2["sum"].tobytes()              # bit-identical run to run
+
+
+def test_groupby_full_size_trip_table(gpu):
+    """3e8-row trip table (zone i32 uniform 1..265, fare f64): counts add up
+    to the row count, and the per-zone sums add up to the compensated total of
+    the fare column (voxel_gpu_scan with no predicate) within the stated bound."""
+    import torch
+    from voxel_amd import scan
+    from voxel_amd._lib import lib
+    n = 300_000_000
+    zone = torch.empty(n, dtype=torch.int32, device="cuda")
+    fare = torch.empty(n, dtype=torch.float64, device="cuda")
+    assert lib.voxel_gpu_fill_uniform(3, zone.data_ptr(), n, 21, 1.0, 266.0, None) == 0
+    assert lib.voxel_gpu_fill_uniform(0, fare.data_ptr(), n, 22, 2.0, 80.0, None) == 0
+    torch.cuda.synchronize()
+    G = 512
+    out = [torch.zeros(G, dtype=torch.int64 if i < 2 else torch.float64, device="cuda") for i in range(6)]
+    ng = np.zeros(1, np.uint64)
+    assert lib.voxel_gpu_groupby_f64(3, zone.data_ptr(), fare.data_ptr(), n, G, *(t.data_ptr() for t in out),
+                                     ng.ctypes.data, None) == 0
+    g = int(ng[0])
+    assert g == 265
+    assert int(out[1][:g].sum().item()) == n
+    tot = scan.scan(fare, "true", 0.0)
+    s = math.fsum(out[2][:g].cpu().tolist())
+    assert abs(s - tot["sum"]) <= 1e-12 * tot["sum"]
+    cnt_ref = torch.bincount(zone.to(torch.int64), minlength=266)[1:266]
+    assert torch.equal(cnt_ref.to(torch.int64), out[1][:g].to(torch.int64))

@@ -82,7 +82,7 @@ def test_fast_path_every_filter_opcode(gpu, golden):
     base = words(golden, "fast")
     for fop in range(0xC1, 0xC7):
         code = list(base)
-        code[1] = (code[1] & ~0xFF) | fop
+        code[1] = (int(code[1]) & 0xFFFFFF00) | fop
         e = engine_f64(x, code, {2: x.size, 3: o.f64_bits(3.0)})
         e.run()
         acc = np.zeros(1, np.float64)
@@ -264,7 +264,7 @@ def test_jit_ordered_compares_and_over_read(gpu, golden):
     base = words(golden, "fast")
     for fop in range(0xC1, 0xC7):
         code = np.array(base, np.uint32)
-        code[1] = (code[1] & ~0xFF) | fop
+        code[1] = (int(code[1]) & 0xFFFFFF00) | fop
         r = lib.voxel_jit_run(code.ctypes.data, code.size, x.ctypes.data, x.size, 4.0)
         ref = o.O.vo_jit_f64(o.ptr(x), x.size, x.size, 0, fop, 4.0)
         assert r == ref, hex(fop)                                    # NE fails NaN in the JIT

@@ -35,7 +35,7 @@ P, U8, U16, U32, U64, I16, I32, I64, SZ, F64, INT = (
 
 class ScanDesc(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("op_a", C.c_int32), ("op_b", C.c_int32), ("blocks", C.c_int32),
-                ("unroll", C.c_int32), ("nontemporal", C.c_int32),
+                ("unroll", C.c_int32), ("nontemporal", C.c_int32), ("layout", C.c_int32), ("pad_", C.c_int32),
                 ("a", P), ("b", P), ("n", U64), ("t_a", U64), ("t_b", U64)]
 
 
@@ -148,6 +148,9 @@ SIGS = {
     "voxel_gpu_join_probe_sum_launch": (INT, [P, P, SZ, P]),
     "voxel_gpu_join_probe_sum_result": (INT, [P, P, P, P]),
     "voxel_gpu_join_probe_pairs": (INT, [P, P, SZ, P, P, P, P]),
+    "voxel_gpu_groupby_f64": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
+    "voxel_gpu_groupby_f64_general": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
+    "voxel_gpu_agg_results": (INT, [P, P, P, P, P, P, P, SZ]),
     "voxel_gpu_scan": (INT, [C.POINTER(ScanDesc), P, C.POINTER(Agg)]),
     "voxel_gpu_scan_launch": (INT, [C.POINTER(ScanDesc), P]),
     "voxel_gpu_scan_result": (INT, [P, C.POINTER(Agg)]),

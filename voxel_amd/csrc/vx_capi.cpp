@@ -179,7 +179,7 @@ bool run_tier0(Engine& e, StreamCtx* c) {
         GroupOut go = c->host->group;
         last = go.last;
         const uint64_t half = 1ull << 63;
-        if ((uint64_t)(go.smax - go.smin) < half) {
+        if ((uint64_t)go.smax - (uint64_t)go.smin < half) {      // unsigned: no signed overflow
             // every value lies in a signed window narrower than 2^63: the
             // wrapping CMP difference is the signed order
             accb = p.accop == 2 ? (uint64_t)go.smin : (uint64_t)go.smax;
@@ -261,7 +261,7 @@ bool make_scan_args(const voxel_gpu_scan_desc* d, ScanArgs& g) {
     if (d->op_a < 0 || d->op_a > 6 || d->op_b < 0 || d->op_b > 6) { set_error("scan: predicate code out of range"); return false; }
     g = ScanArgs{};
     g.dtype = d->dtype; g.a = d->a; g.b = d->b; g.begin = 0; g.end = d->n; g.blocks = d->blocks;
-    g.unroll = d->unroll; g.nontemporal = d->nontemporal;
+    g.unroll = d->unroll; g.nontemporal = d->nontemporal; g.layout = d->layout;
     if (g.unroll != 0 && g.unroll != 2 && g.unroll != 4 && g.unroll != 8) { set_error("scan: unroll must be 2, 4 or 8"); return false; }
     g.pa = make_pred(d->dtype, d->op_a, d->t_a);
     if (d->b && d->op_b != CMP_TRUE) {

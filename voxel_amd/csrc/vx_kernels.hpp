@@ -66,7 +66,8 @@ struct ScanArgs {
     int table_a;           // pa uses the table form
     int blocks;            // 0 = pick
     int unroll;            // 16-byte loads in flight per lane and column: 2, 4, 8 (0 = default)
-    int nontemporal;       // 1 = non-temporal loads
+    int nontemporal;       // 1 = non-temporal loads, -1 = cached loads, 0 = default (non-temporal)
+    int layout;            // 0 = grid-stride tiles, 1 = one contiguous run of tiles per workgroup
 };
 
 // Fused filter + COUNT/SUM/MIN/MAX over one range; result in *out_dev

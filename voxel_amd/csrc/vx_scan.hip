@@ -120,7 +120,7 @@ __device__ __forceinline__ V ld16(const V* p) {
     }
 }
 
-template <typename T, int MODE, bool TA, bool TB, bool VEC, int U, bool NT>
+template <typename T, int MODE, bool TA, bool TB, bool VEC, int U, bool NT, bool CONTIG>
 __global__ __launch_bounds__(kBlock)
 void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, uint64_t end,
             PredDev pda, PredDev pdb, ScanOut* __restrict__ parts, uint32_t* __restrict__ ticket,
@@ -158,9 +158,23 @@ void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, ui
 
     const V* __restrict__ av = reinterpret_cast<const V*>(a + vb);
     const V* __restrict__ bv = reinterpret_cast<const V*>(MODE == 2 ? b + vb : a + vb);
+    // tiles of kBlock*U vectors: grid-stride over tiles, or (CONTIG) one
+    // contiguous run of tiles per workgroup
     const uint64_t tile = (uint64_t)kBlock * U;
-    const uint64_t step = (uint64_t)gridDim.x * tile;
-    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < nvec; base += step) {
+    uint64_t first, stop, step;
+    if constexpr (CONTIG) {
+        const uint64_t ntiles = (nvec + tile - 1) / tile;
+        const uint64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+        first = (uint64_t)blockIdx.x * per * tile;
+        stop = first + per * tile;
+        if (stop > nvec) stop = nvec;
+        step = tile;
+    } else {
+        first = (uint64_t)blockIdx.x * tile;
+        stop = nvec;
+        step = (uint64_t)gridDim.x * tile;
+    }
+    for (uint64_t base = first; base < stop; base += step) {
         V xa[U], xb[U];
         if (base + tile <= nvec) {
 #pragma unroll
@@ -363,7 +377,7 @@ void k_group_last(const T* __restrict__ a, uint64_t count, uint64_t row, PredDev
     if (threadIdx.x == 0) result[0] = group_value<T, TAB>(a, count, row, p, true, red);
 }
 
-template <typename T, int MODE, bool TA, bool TB, int U, bool NT>
+template <typename T, int MODE, bool TA, bool TB, int U, bool NT, bool CONTIG>
 hipError_t launch_u(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
     const T* a = static_cast<const T*>(g.a);
     const T* b = static_cast<const T*>(g.b);
@@ -372,16 +386,16 @@ hipError_t launch_u(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStre
     if (MODE == 2 && (((uintptr_t)(a + g.begin) ^ (uintptr_t)(b + g.begin)) & 15u)) vec = false;
     uint64_t per_block = (uint64_t)kBlock * U * (16 / sizeof(T));
     uint64_t want = (n + per_block - 1) / per_block;
-    uint64_t cap = g.blocks > 0 ? (uint64_t)g.blocks : (uint64_t)device_cu_count() * 8;
+    uint64_t cap = g.blocks > 0 ? (uint64_t)g.blocks : (uint64_t)device_cu_count() * 4;
     if (cap > scr.max_blocks) cap = scr.max_blocks;
     uint64_t blocks = want < cap ? want : cap;
     if (blocks < 1) blocks = 1;
     ScanOut* parts = static_cast<ScanOut*>(scr.partials);
     if (vec)
-        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, true, U, NT>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, true, U, NT, CONTIG>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
                            a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
     else
-        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, false, U, NT>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, false, U, NT, CONTIG>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
                            a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
     return hipGetLastError();
 }
@@ -390,15 +404,22 @@ hipError_t launch_u(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStre
 // loads are tunable; unroll 0 selects the default for the mode.
 template <typename T, int MODE, bool TA, bool TB>
 hipError_t launch_t(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
-    int u = g.unroll > 0 ? g.unroll : (MODE == 2 ? 4 : 8);
-    bool nt = g.nontemporal > 0;
-    if (TA || TB) { u = 4; nt = false; }    // table predicates: compat paths, one variant
-    switch (u) {
-    case 2: return nt ? launch_u<T, MODE, TA, TB, 2, true>(g, scr, out, st) : launch_u<T, MODE, TA, TB, 2, false>(g, scr, out, st);
-    case 4: return nt ? launch_u<T, MODE, TA, TB, 4, true>(g, scr, out, st) : launch_u<T, MODE, TA, TB, 4, false>(g, scr, out, st);
-    case 8: return nt ? launch_u<T, MODE, TA, TB, 8, true>(g, scr, out, st) : launch_u<T, MODE, TA, TB, 8, false>(g, scr, out, st);
-    default: return hipErrorInvalidValue;
+    int u = g.unroll > 0 ? g.unroll : 4;
+    bool nt = g.nontemporal >= 0;             // default: non-temporal
+    bool contig = g.layout > 0;
+    if (TA || TB) { u = 4; nt = false; contig = false; }    // table predicates: compat paths, one variant
+#define VX_SCAN_U(UU)                                                                                   \
+    if (u == UU) {                                                                                      \
+        if (contig) return nt ? launch_u<T, MODE, TA, TB, UU, true, true>(g, scr, out, st)              \
+                              : launch_u<T, MODE, TA, TB, UU, false, true>(g, scr, out, st);            \
+        return nt ? launch_u<T, MODE, TA, TB, UU, true, false>(g, scr, out, st)                         \
+                  : launch_u<T, MODE, TA, TB, UU, false, false>(g, scr, out, st);                       \
     }
+    VX_SCAN_U(2)
+    VX_SCAN_U(4)
+    VX_SCAN_U(8)
+#undef VX_SCAN_U
+    return hipErrorInvalidValue;
 }
 
 template <typename T>

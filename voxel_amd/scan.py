@@ -44,7 +44,7 @@ def _dev(col):
 
 
 def make_desc(a, op_a, t_a, b=None, op_b=TRUE, t_b=0, blocks: int = 0, unroll: int = 0,
-              nontemporal: int = 0) -> ScanDesc:
+              nontemporal: int = 0, layout: int = 0) -> ScanDesc:
     pa, n, dt = _dev(a)
     pb = None
     if b is not None:
@@ -55,7 +55,7 @@ def make_desc(a, op_a, t_a, b=None, op_b=TRUE, t_b=0, blocks: int = 0, unroll: i
     op_b = OPS.get(op_b, op_b) if isinstance(op_b, str) else op_b
     d = ScanDesc()
     d.dtype, d.op_a, d.op_b, d.blocks = dt, op_a, op_b if b is not None else TRUE, blocks
-    d.unroll, d.nontemporal = unroll, nontemporal
+    d.unroll, d.nontemporal, d.layout = unroll, nontemporal, layout
     d.a, d.b, d.n = pa, pb, n
     d.t_a, d.t_b = threshold_bits(dt, t_a), threshold_bits(dt, t_b)
     return d
