@@ -300,6 +300,48 @@ int       voxel_gpu_groupby_f64_general(int key_dtype, const void* keys_dev, con
 int       voxel_gpu_agg_results(void* agg, int64_t* keys, uint64_t* counts, double* sums, double* avgs,
                                 double* mins, double* maxs, size_t max_groups);
 
+/* ---- learned-CDF sort (no counterpart in the reference; same output as
+ * voxel_gpu_sort_pairs_u64, ops/sort.hpp:83-160) ---------------------------- */
+/* FP32 MLP 1 -> 32 -> 32 -> 1 (ReLU), parameters packed as
+ * W1[32] b1[32] W2[32][32] b2[32] W3[32] b3 (voxel_gpu_mlp_param_count()
+ * floats, device memory). */
+int       voxel_gpu_mlp_param_count(void);
+int       voxel_gpu_mlp_init(float* params_dev, uint64_t seed);
+int       voxel_gpu_mlp_forward(const float* params_dev, const float* x_dev, size_t n, float* y_dev, void* stream);
+/* Gradient of the mean squared error over n points (grad_dev: param_count
+ * floats) and the loss itself (loss_dev: one float); either may be NULL. */
+int       voxel_gpu_mlp_grad(const float* params_dev, const float* x_dev, const float* y_dev, size_t n,
+                             float* grad_dev, float* loss_dev, void* stream);
+/* steps of Adam (beta 0.9/0.999, eps 1e-8) at learning rate lr; the loss
+ * before the last step into *loss_out_host (may be NULL). */
+int       voxel_gpu_mlp_train(float* params_dev, const float* x_dev, const float* y_dev, size_t n, int steps,
+                              float lr, float* loss_out_host, void* stream);
+
+typedef struct voxel_gpu_learned_stats {
+    int32_t  model;             /* 0 MLP CDF, 1 equal-width (linear in the key) */
+    int32_t  log_grid;          /* model input: 1 = float bits of key - min */
+    int32_t  fell_back;         /* 1 = whole input sorted by the plain radix sort */
+    int32_t  partition_passes;  /* onesweep passes over the coarse bucket id */
+    int32_t  train_steps;
+    float    final_loss;
+    uint64_t buckets, cells;
+    uint64_t max_bucket, big_buckets, big_rows, big_unsorted;
+    uint64_t longest_fine_run, long_runs;
+    float    sample_ms, train_ms, table_ms, partition_ms, bucket_ms, fixup_ms;
+} voxel_gpu_learned_stats;
+/* Stable sort of (u64 key, u32 payload) pairs, bit-identical to
+ * voxel_gpu_sort_pairs_u64: keys are partitioned by a monotone bucket id
+ * derived from a learned CDF (model 0) or a linear one (model 1), each
+ * bucket is sorted in LDS.  vals_in NULL = row index.  n < 2^30.
+ * stats may be NULL. */
+int       voxel_gpu_sort_pairs_u64_learned(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                                           uint32_t* vals_out, size_t n, int model,
+                                           voxel_gpu_learned_stats* stats, void* stream);
+/* Tuning of the learned sort: log2 sample size (20), training points
+ * (16384), Adam steps (200; -1 = none), log2 rows per bucket (11, max 11);
+ * 0 keeps the default. */
+int       voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, int rows_per_bucket_log2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 def oracle_groups(keys, vals, signed=True):
     n = keys.size
-    G = 1 << 16
+    G = max(1 << 16, n)
     ok, oc = np.zeros(G, np.uint32), np.zeros(G, np.uint64)
     osum, omin, omax, omean, om2 = (np.zeros(G, np.float64) for _ in range(5))
     dig = np.zeros(G * o.NDIG, np.int64)

@@ -351,3 +351,25 @@ def test_groupby_restatement(golden):
         assert np.array_equal(oc[:ng][order], c)
         assert np.array_equal(osum[:ng][order], s)         # same row order: bit-exact
         assert np.array_equal(omean[:ng][order], mean) and np.array_equal(om2[:ng][order], m2)
+
+
+def test_mlp_oracle_pinned_by_finite_differences():
+    """The CPU model of the learned-CDF network: its gradient equals central
+    finite differences of its loss, and its forward equals a unit-by-unit loop."""
+    from oracle import mlp
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal(mlp.N_PARAMS) * 0.5
+    x = rng.random(40)
+    y = np.sort(rng.random(40))
+    g = mlp.grad(p, x, y)
+    h = 1e-6
+    for i in list(range(0, 64, 7)) + list(range(64, 1088, 97)) + list(range(1088, mlp.N_PARAMS, 5)):
+        e = np.zeros(mlp.N_PARAMS)
+        e[i] = h
+        fd = (mlp.loss(p + e, x, y) - mlp.loss(p - e, x, y)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+    w1, b1, w2, b2, w3, b3 = mlp.unpack(p)
+    for xi, yi in zip(x[:5], mlp.forward(p, x[:5])):
+        h1 = [max(w1[j] * xi + b1[j], 0.0) for j in range(32)]
+        h2 = [max(sum(w2[k][j] * h1[j] for j in range(32)) + b2[k], 0.0) for k in range(32)]
+        assert abs(sum(w3[k] * h2[k] for k in range(32)) + b3 - yi) < 1e-12

@@ -45,6 +45,18 @@ class Agg(C.Structure):
                 ("has_minmax", C.c_int32), ("pad", C.c_int32)]
 
 
+class LearnedStats(C.Structure):
+    _fields_ = [("model", C.c_int32), ("log_grid", C.c_int32), ("fell_back", C.c_int32),
+                ("partition_passes", C.c_int32), ("train_steps", C.c_int32), ("final_loss", C.c_float),
+                ("buckets", U64), ("cells", U64), ("max_bucket", U64), ("big_buckets", U64), ("big_rows", U64),
+                ("big_unsorted", U64), ("longest_fine_run", U64), ("long_runs", U64),
+                ("sample_ms", C.c_float), ("train_ms", C.c_float), ("table_ms", C.c_float),
+                ("partition_ms", C.c_float), ("bucket_ms", C.c_float), ("fixup_ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 SIGS = {
     # engine (f64)
     "voxel_engine_create_f64": (P, []),
@@ -151,6 +163,13 @@ SIGS = {
     "voxel_gpu_groupby_f64": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
     "voxel_gpu_groupby_f64_general": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
     "voxel_gpu_agg_results": (INT, [P, P, P, P, P, P, P, SZ]),
+    "voxel_gpu_mlp_param_count": (INT, []),
+    "voxel_gpu_mlp_init": (INT, [P, U64]),
+    "voxel_gpu_mlp_forward": (INT, [P, P, SZ, P, P]),
+    "voxel_gpu_mlp_grad": (INT, [P, P, P, SZ, P, P, P]),
+    "voxel_gpu_mlp_train": (INT, [P, P, P, SZ, INT, C.c_float, P, P]),
+    "voxel_gpu_sort_pairs_u64_learned": (INT, [P, P, P, P, SZ, INT, P, P]),
+    "voxel_gpu_learned_tune": (INT, [INT, INT, INT, INT]),
     "voxel_gpu_scan": (INT, [C.POINTER(ScanDesc), P, C.POINTER(Agg)]),
     "voxel_gpu_scan_launch": (INT, [C.POINTER(ScanDesc), P]),
     "voxel_gpu_scan_result": (INT, [P, C.POINTER(Agg)]),

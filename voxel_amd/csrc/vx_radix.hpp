@@ -13,22 +13,40 @@ struct LsdDigit {
     }
 };
 
-// Digit p of a monotone bucket id computed from the key by a splitter/slope
-// table (learned CDF partitioner, vx_learned.hip): bucket(k) is
-// non-decreasing in k, so a stable sort by bucket followed by a stable sort
-// inside each bucket sorts by key.
+// Digit p of a monotone "fine bucket" id fb(k) computed from the key with a
+// piecewise-linear table (learned CDF partitioner, vx_learned.hip).  Keys are
+// clamped to [kmin, kmax] and mapped to a grid coordinate G(k): the offset
+// k - kmin itself (linear grid), or the bit pattern of (float)(k - kmin)
+// (logarithmic grid: 2^23 steps per octave).  Cell c = G >> cshift covers
+// fine buckets [b0[c], b0[c+1]]; inside it fb = b0[c] + floor(slope[c] *
+// (G - c << cshift)), clamped to b0[c+1].  Every step is non-decreasing in
+// k, so fb is non-decreasing in k: a stable sort by fb followed by a stable
+// sort inside each fb run sorts by key.  Partition pass p uses bits
+// [8 + 8p, 16 + 8p) of fb (the coarse bucket is fb >> 8; the low 8 bits
+// split a bucket once more inside LDS).
 struct TableDigit {
-    const uint64_t* cell_lo;    // key at the start of each cell (ascending), ncells entries
-    const uint32_t* cell_b0;    // bucket at the start of each cell (non-decreasing)
-    const float* cell_slope;    // buckets per key unit inside the cell
+    const uint32_t* b0;         // ncells + 1 entries, non-decreasing, b0[ncells] = total fine buckets
+    const float* slope;         // ncells entries
+    uint64_t kmin, kmax;
     uint32_t ncells;
-    uint32_t nbuckets;
-    int shift_bits;             // bits of bucket id per digit pass (8)
-    uint64_t kmin;              // smallest key (cell grid origin)
-    int cell_shift;             // cell index = float-bits grid (see vx_learned.hip)
-    __device__ uint32_t bucket(uint64_t k) const;
+    int cshift;
+    int logmode;
+    uint32_t nfine;             // total fine buckets
+    __device__ __forceinline__ uint32_t fine(uint64_t k) const {
+        uint64_t kc = k < kmin ? kmin : (k > kmax ? kmax : k);
+        uint64_t d = kc - kmin;
+        uint64_t g = logmode ? (uint64_t)__float_as_uint((float)d) : d;
+        uint64_t c = g >> cshift;
+        if (c >= ncells) c = ncells - 1;
+        uint64_t off = g - (c << cshift);
+        uint32_t lo = b0[c], hi = b0[c + 1];
+        float f = floorf(slope[c] * (float)off);
+        uint32_t add = f <= 0.0f ? 0u : (f >= (float)(hi - lo) ? hi - lo : (uint32_t)f);
+        uint32_t fb = lo + add;
+        return fb < nfine ? fb : nfine - 1;
+    }
     __device__ __forceinline__ uint32_t digit(uint64_t k, int p) const {
-        return (bucket(k) >> (shift_bits * p)) & 0xFFu;
+        return (fine(k) >> (8 + 8 * p)) & 0xFFu;
     }
 };
 

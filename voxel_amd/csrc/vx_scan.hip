@@ -386,7 +386,7 @@ hipError_t launch_u(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStre
     if (MODE == 2 && (((uintptr_t)(a + g.begin) ^ (uintptr_t)(b + g.begin)) & 15u)) vec = false;
     uint64_t per_block = (uint64_t)kBlock * U * (16 / sizeof(T));
     uint64_t want = (n + per_block - 1) / per_block;
-    uint64_t cap = g.blocks > 0 ? (uint64_t)g.blocks : (uint64_t)device_cu_count() * 4;
+    uint64_t cap = g.blocks > 0 ? (uint64_t)g.blocks : (uint64_t)device_cu_count() * 2;
     if (cap > scr.max_blocks) cap = scr.max_blocks;
     uint64_t blocks = want < cap ? want : cap;
     if (blocks < 1) blocks = 1;
@@ -404,7 +404,9 @@ hipError_t launch_u(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStre
 // loads are tunable; unroll 0 selects the default for the mode.
 template <typename T, int MODE, bool TA, bool TB>
 hipError_t launch_t(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {
-    int u = g.unroll > 0 ? g.unroll : 4;
+    // defaults from the MI355X sweep (2^28-row compound scans): 2 workgroups
+    // per CU, non-temporal loads, 4 (8-byte types) / 8 (4-byte types) loads
+    int u = g.unroll > 0 ? g.unroll : (sizeof(T) == 8 ? 4 : 8);
     bool nt = g.nontemporal >= 0;             // default: non-temporal
     bool contig = g.layout > 0;
     if (TA || TB) { u = 4; nt = false; contig = false; }    // table predicates: compat paths, one variant
