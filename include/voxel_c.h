@@ -331,14 +331,15 @@ typedef struct voxel_gpu_learned_stats {
 } voxel_gpu_learned_stats;
 /* Stable sort of (u64 key, u32 payload) pairs, bit-identical to
  * voxel_gpu_sort_pairs_u64: keys are partitioned by a monotone bucket id
- * derived from a learned CDF (model 0) or a linear one (model 1), each
- * bucket is sorted in LDS.  vals_in NULL = row index.  n < 2^30.
- * stats may be NULL. */
+ * derived from a learned CDF (model 0: the MLP, trained on the GPU on the
+ * logit of a sample's CDF, routing to 4096 linear leaf models) or a linear
+ * one (model 1, no model), each bucket is sorted in LDS.  vals_in NULL = row
+ * index.  n < 2^30.  stats may be NULL. */
 int       voxel_gpu_sort_pairs_u64_learned(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
                                            uint32_t* vals_out, size_t n, int model,
                                            voxel_gpu_learned_stats* stats, void* stream);
 /* Tuning of the learned sort: log2 sample size (20), training points
- * (16384), Adam steps (200; -1 = none), log2 rows per bucket (11, max 11);
+ * (4096), Adam steps (100; -1 = none), log2 rows per bucket (11, max 11);
  * 0 keeps the default. */
 int       voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, int rows_per_bucket_log2);
 

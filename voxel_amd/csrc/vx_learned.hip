@@ -47,6 +47,7 @@ constexpr int kParams = 32 + 32 + 32 * 32 + 32 + 32 + 1;   // 1153
 constexpr int oW1 = 0, oB1 = 32, oW2 = 64, oB2 = 64 + 1024, oW3 = oB2 + 32, oB3 = oW3 + 32;
 constexpr int kGradThreads = 64;
 constexpr int kCap = 4096;                   // rows a workgroup sorts in LDS
+constexpr uint32_t kLeaves = 4096;           // linear models under the MLP
 
 namespace {
 
@@ -141,15 +142,30 @@ __global__ __launch_bounds__(kGradThreads) void k_mlp_grad(const float* __restri
     }
 }
 
-// Sums the workgroup gradients (fixed order) into grad; Adam step when adam != 0.
+// Sums the workgroup gradients into grad (fixed order: wave w adds partials
+// w, w + 4, ..., then the four wave sums in order); Adam step when adam != 0.
+// 64 parameters per workgroup, one per lane.
 __global__ __launch_bounds__(256) void k_mlp_reduce(float* __restrict__ Pg, float* __restrict__ m, float* __restrict__ v,
                                                     const float* __restrict__ part, uint32_t nblocks, int adam,
                                                     float lr, float b1c, float b2c, float* __restrict__ grad_out,
                                                     float* __restrict__ loss_out, float inv_n) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > kParams) return;
+    __shared__ float ws[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
     float g = 0.0f;
-    for (uint32_t b = 0; b < nblocks; ++b) g += part[(uint64_t)b * (kParams + 1) + p];
+    if (p <= kParams) {
+        uint32_t b = w;
+        for (; b + 12 < nblocks; b += 16) {
+            float a0 = part[(uint64_t)b * (kParams + 1) + p], a1 = part[(uint64_t)(b + 4) * (kParams + 1) + p];
+            float a2 = part[(uint64_t)(b + 8) * (kParams + 1) + p], a3 = part[(uint64_t)(b + 12) * (kParams + 1) + p];
+            g += a0; g += a1; g += a2; g += a3;
+        }
+        for (; b < nblocks; b += 4) g += part[(uint64_t)b * (kParams + 1) + p];
+    }
+    ws[w][lane] = g;
+    __syncthreads();
+    if (w != 0 || p > kParams) return;
+    g = ((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane];
     if (p == kParams) {
         if (loss_out) *loss_out = g * inv_n;
         return;
@@ -189,17 +205,15 @@ __global__ __launch_bounds__(256) void k_quantiles(const uint64_t* __restrict__ 
     }
 }
 
-__device__ __forceinline__ double grid_value(uint64_t g, int logmode) {
-    return logmode ? (double)__uint_as_float((uint32_t)g) : (double)g;
-}
-
 __device__ __forceinline__ uint64_t grid_of(uint64_t k, uint64_t kmin, uint64_t kmax, int logmode) {
     uint64_t kc = k < kmin ? kmin : (k > kmax ? kmax : k);
-    uint64_t d = kc - kmin;
-    return logmode ? (uint64_t)__float_as_uint((float)d) : d;
+    return TableDigit::grid(kc - kmin, logmode);
 }
 
-// training pairs: T points evenly spaced in rank over the sorted sample
+// Training pairs: T points evenly spaced in rank over the sorted sample,
+// (grid position / grid max, logit of the rank fraction).  Fitting the logit
+// keeps the relative error of the CDF small in both tails, where a fit of
+// the CDF itself leaves flat stretches that pile keys into a few buckets.
 __global__ __launch_bounds__(256) void k_train_data(const uint64_t* __restrict__ ss, uint64_t s, uint64_t t,
                                                     uint64_t kmin, uint64_t kmax, int logmode, double gmax,
                                                     float* __restrict__ x, float* __restrict__ y) {
@@ -208,60 +222,140 @@ __global__ __launch_bounds__(256) void k_train_data(const uint64_t* __restrict__
         uint64_t r = j * s / t;
         uint64_t g = grid_of(ss[r], kmin, kmax, logmode);
         x[j] = gmax > 0 ? (float)((double)g / gmax) : 0.0f;
-        y[j] = (float)(((double)r + 0.5) / (double)s);
+        const double q = ((double)r + 0.5) / (double)s;
+        y[j] = (float)log(q / (1.0 - q));
     }
 }
 
-// CDF estimate at every cell start; model 0 = MLP, 1 = linear in the key
-__global__ __launch_bounds__(256) void k_cells(const float* __restrict__ Pg, uint32_t ncells, int cshift, int logmode,
-                                               double gmax, double range, int model, float* __restrict__ yc) {
+// Root model at every cell start c (grid c << cshift), c = 0..ncells-1, and
+// R[ncells] = 1.  model 0: sigmoid of the MLP, rescaled so that its values at
+// the sample minimum and maximum are 0 and 1; model 1: linear in the key.
+// Clamped to [0, 1].
+__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+__global__ __launch_bounds__(256) void k_root_cells(const float* __restrict__ Pg, uint32_t ncells, int cshift,
+                                                    double gmax, int model, float* __restrict__ R) {
     __shared__ float P[kParams];
+    __shared__ float ends[2];
     for (int i = threadIdx.x; i < kParams; i += blockDim.x) P[i] = Pg[i];
     __syncthreads();
+    if (model == 0 && threadIdx.x < 2) {
+        float h1[kH], h2[kH];
+        ends[threadIdx.x] = sigmoidf(mlp_fwd(P, threadIdx.x ? 1.0f : 0.0f, h1, h2));
+    }
+    __syncthreads();
+    const float y0 = ends[0], inv = model == 0 && ends[1] > ends[0] ? 1.0f / (ends[1] - ends[0]) : 1.0f;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += stride) {
-        uint64_t g = c << cshift;
-        float y;
-        if (model == 0) {
-            float h1[kH], h2[kH];
-            y = mlp_fwd(P, gmax > 0 ? (float)((double)g / gmax) : 0.0f, h1, h2);
-        } else {
-            y = range > 0 ? (float)(grid_value(g, logmode) / range) : 0.0f;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= ncells; c += stride) {
+        float y = 1.0f;
+        if (c < ncells) {
+            float x = gmax > 0 ? (float)((double)(c << cshift) / gmax) : 0.0f;
+            if (model == 0) {
+                float h1[kH], h2[kH];
+                y = (sigmoidf(mlp_fwd(P, x, h1, h2)) - y0) * inv;
+            } else {
+                y = x;
+            }
         }
-        yc[c] = fminf(fmaxf(y, 0.0f), 1.0f);
+        R[c] = fminf(fmaxf(y, 0.0f), 1.0f);
     }
 }
 
-// one workgroup: prefix max of yc, fine-bucket starts b0 and slopes
-__global__ __launch_bounds__(1024) void k_table(const float* __restrict__ yc, uint32_t ncells, uint32_t nfine,
-                                                int cshift, uint32_t* __restrict__ b0, float* __restrict__ slope) {
-    __shared__ float part[1024];
-    const uint32_t per = (ncells + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per, hi = min(lo + per, ncells);
-    float m = 0.0f;
-    for (uint32_t c = lo; c < hi; ++c) m = fmaxf(m, yc[c]);
+// One workgroup: in-place inclusive prefix max of y[0..n).
+template <class T>
+__global__ __launch_bounds__(1024) void k_prefix_max(T* __restrict__ y, uint32_t n) {
+    __shared__ T part[1024];
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t lo = min(threadIdx.x * per, n), hi = min(lo + per, n);
+    T m = 0;
+    for (uint32_t c = lo; c < hi; ++c) m = y[c] > m ? y[c] : m;
     part[threadIdx.x] = m;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        float t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0.0f;
+        T t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : (T)0;
         __syncthreads();
-        part[threadIdx.x] = fmaxf(part[threadIdx.x], t);
+        part[threadIdx.x] = part[threadIdx.x] > t ? part[threadIdx.x] : t;
         __syncthreads();
     }
-    float run = threadIdx.x ? part[threadIdx.x - 1] : 0.0f;
+    T run = threadIdx.x ? part[threadIdx.x - 1] : (T)0;
     for (uint32_t c = lo; c < hi; ++c) {
-        run = fmaxf(run, yc[c]);
-        double f = floor((double)run * (double)nfine);
-        uint32_t b = f >= (double)nfine ? nfine : (uint32_t)f;
-        b0[c] = c == 0 ? 0u : b;
+        run = y[c] > run ? y[c] : run;
+        y[c] = run;
     }
-    if (threadIdx.x == 0) b0[ncells] = nfine;
-    __syncthreads();
-    const double width = ldexp(1.0, cshift);
-    for (uint32_t c = lo; c < hi; ++c) {
-        uint32_t a = b0[c], b = b0[c + 1];
-        slope[c] = (float)((double)(b - a) / width);
+}
+
+// Root value at grid position g: linear between the cell-start values.
+__device__ __forceinline__ float root_at(const float* __restrict__ R, uint32_t ncells, int cshift, uint64_t g) {
+    uint64_t c = g >> cshift;
+    if (c >= ncells) c = ncells - 1;
+    float f = (float)(g - (c << cshift)) * __uint_as_float((uint32_t)(127 - cshift) << 23);
+    f = f > 1.0f ? 1.0f : f;
+    return R[c] + (R[c + 1] - R[c]) * f;
+}
+
+__device__ __forceinline__ uint32_t leaf_of(float r, uint32_t L) {
+    float f = floorf(r * (float)L);
+    return f <= 0.0f ? 0u : (f >= (float)(L - 1) ? L - 1 : (uint32_t)f);
+}
+
+// lo[l] = first sample (sorted) routed to a leaf >= l, l = 0..L
+__global__ __launch_bounds__(256) void k_leaf_lo(const uint64_t* __restrict__ ss, uint64_t s, const float* __restrict__ R,
+                                                 uint32_t ncells, int cshift, uint64_t kmin, uint64_t kmax, int logmode,
+                                                 uint32_t L, uint32_t* __restrict__ lo) {
+    uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > L) return;
+    uint64_t a = 0, z = s;
+    while (a < z) {
+        uint64_t mid = a + (z - a) / 2;
+        if (leaf_of(root_at(R, ncells, cshift, grid_of(ss[mid], kmin, kmax, logmode)), L) < l) a = mid + 1;
+        else z = mid;
     }
+    lo[l] = (uint32_t)a;
+}
+
+// CDF at every cell start from the leaf the root routes it to: the leaf's
+// samples [lo, hi) are joined by a line from (grid of its first sample,
+// lo / s) to (grid of its last sample, hi / s).
+__global__ __launch_bounds__(256) void k_leaf_cells(const uint64_t* __restrict__ ss, uint64_t s, const float* __restrict__ R,
+                                                    uint32_t ncells, int cshift, uint64_t kmin, uint64_t kmax, int logmode,
+                                                    uint32_t L, const uint32_t* __restrict__ lo, double* __restrict__ y) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += stride) {
+        const uint64_t g = c << cshift;
+        const uint32_t l = leaf_of(R[c], L);
+        const uint32_t a = lo[l], b = lo[l + 1];
+        const double y0 = (double)a / (double)s, y1 = (double)b / (double)s;
+        double v = y0;
+        if (b > a) {
+            const uint64_t g0 = grid_of(ss[a], kmin, kmax, logmode), g1 = grid_of(ss[b - 1], kmin, kmax, logmode);
+            if (g <= g0) v = y0;
+            else if (g >= g1) v = y1;
+            else v = y0 + (y1 - y0) * ((double)(g - g0) / (double)(g1 - g0));
+        }
+        y[c] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_widen(const float* __restrict__ R, uint32_t n, double* __restrict__ y) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) y[c] = (double)R[c];
+}
+
+// cell[c] = {first fine bucket, fine buckets in the cell} from the
+// non-decreasing CDF y at the cell starts (first bucket of cell 0 is 0, the
+// last cell ends at nfine)
+__global__ __launch_bounds__(256) void k_table(const double* __restrict__ y, uint32_t ncells, uint32_t nfine,
+                                               uint2* __restrict__ cell) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    auto start = [&](uint32_t i) -> uint32_t {
+        if (i == 0) return 0u;
+        if (i >= ncells) return nfine;
+        double f = floor(y[i] * (double)nfine);
+        return f <= 0.0 ? 0u : (f >= (double)nfine ? nfine : (uint32_t)f);
+    };
+    uint32_t a = start(c);
+    cell[c] = make_uint2(a, start(c + 1) - a);
 }
 
 // start[b] = first row whose coarse bucket is >= b (rows are in bucket order)
@@ -296,6 +390,8 @@ __global__ __launch_bounds__(256) void k_bucket_sort(const uint64_t* __restrict_
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t sst[257], scn[256], longrun[256];
     __shared__ uint32_t nlong;
+    __shared__ uint8_t dig8[CAP];                  // fine digit of each LDS slot
+    __shared__ uint8_t islong[256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, t = threadIdx.x;
     uint32_t longest = 0, nlong_runs = 0;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
@@ -346,6 +442,7 @@ __global__ __launch_bounds__(256) void k_bucket_sort(const uint64_t* __restrict_
                 uint32_t pos = sst[dd[r]] + wcnt[w][dd[r]] + rk[r];
                 kb[pos] = k[r];
                 vb[pos] = v[r];
+                dig8[pos] = (uint8_t)dd[r];
             }
         }
         __syncthreads();
@@ -363,15 +460,13 @@ __global__ __launch_bounds__(256) void k_bucket_sort(const uint64_t* __restrict_
         } else {
             longrun[atomicAdd(&nlong, 1u)] = t;
         }
+        islong[t] = tot > kShortRun;
         __syncthreads();
         const uint32_t nl = nlong;
         nlong_runs += nl;
         // rows of short runs are final in LDS; rows of long runs are ranked below
-        for (uint32_t i = t; i < m; i += 256) {
-            uint32_t a = 0, z = 256;                   // run of row i: last d with sst[d] <= i
-            while (z - a > 1) { uint32_t mid = (a + z) >> 1; if (sst[mid] <= i) a = mid; else z = mid; }
-            if (sst[a + 1] - sst[a] <= kShortRun) { kout[s + i] = kb[i]; vout[s + i] = vb[i]; }
-        }
+        for (uint32_t i = t; i < m; i += 256)
+            if (!islong[dig8[i]]) { kout[s + i] = kb[i]; vout[s + i] = vb[i]; }
         for (uint32_t q = 0; q < nl; ++q) {
             const uint32_t d = longrun[q], rlo = sst[d], rhi = sst[d + 1];
             for (uint32_t i = rlo + t; i < rhi; i += 256) {
@@ -408,6 +503,20 @@ __global__ __launch_bounds__(256) void k_big_check(const uint64_t* __restrict__ 
     }
 }
 
+// Row moves (src row, dst row, rows) per list entry: dir 0 copies
+// a[src..] -> b[dst..], dir 1 copies a[dst..] -> b[src..].
+__global__ __launch_bounds__(256) void k_move(const uint64_t* __restrict__ ka, const uint32_t* __restrict__ va,
+                                              uint64_t* __restrict__ kb, uint32_t* __restrict__ vb,
+                                              const uint64_t* __restrict__ mv, uint64_t nmv, int dir) {
+    for (uint64_t j = blockIdx.x; j < nmv; j += gridDim.x) {
+        const uint64_t s = mv[3 * j + (dir ? 1 : 0)], d = mv[3 * j + (dir ? 0 : 1)], len = mv[3 * j + 2];
+        for (uint64_t i = threadIdx.x; i < len; i += 256) {
+            kb[d + i] = ka[s + i];
+            vb[d + i] = va[s + i];
+        }
+    }
+}
+
 unsigned gridn(uint64_t n, unsigned cap) {
     uint64_t b = (n + 255) / 256;
     return (unsigned)(b ? (b < cap ? b : cap) : 1);
@@ -421,7 +530,8 @@ struct LearnedBufs {
     float* tx = nullptr; float* ty = nullptr; uint64_t t_cap = 0;
     float* grad = nullptr; float* loss = nullptr; float* loss_host = nullptr;
     uint64_t* samp = nullptr; uint64_t* samp2 = nullptr; uint32_t* sidx = nullptr; uint32_t* sidx2 = nullptr; uint64_t s_cap = 0;
-    float* yc = nullptr; uint32_t* b0 = nullptr; float* slope = nullptr;
+    float* R = nullptr; double* yd = nullptr; uint32_t* leaf_lo = nullptr;
+    uint2* cell = nullptr;
     uint64_t* k1 = nullptr; uint64_t* k2 = nullptr; uint32_t* v1 = nullptr; uint32_t* v2 = nullptr; uint64_t cap = 0;
     uint64_t* start = nullptr; uint32_t* big = nullptr; uint64_t nb_cap = 0;
     unsigned long long* stat = nullptr;
@@ -430,11 +540,19 @@ struct LearnedBufs {
     uint64_t* qdev = nullptr; uint64_t* qhost = nullptr;
     unsigned long long* stat_host = nullptr;
     float* phost = nullptr;
+    uint64_t* cdev = nullptr; uint64_t* chost = nullptr; uint32_t* cflag = nullptr; uint32_t* cflag_host = nullptr;
+    uint64_t c_cap = 0;                       // chunk list entries
     hipEvent_t ev[8] = {};
+    // training loop as one graph (rebuilt when its buffers or shape change)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void* graph_key[7] = {};
+    uint64_t graph_T = 0;
+    int graph_steps = 0;
 };
 
 struct LearnedTune {
-    int sample_log2 = 20, train_points = 16384, train_steps = 200, bucket_log2 = 11;
+    int sample_log2 = 20, train_points = 4096, train_steps = 100, bucket_log2 = 11;
 };
 static LearnedTune g_tune;
 
@@ -452,8 +570,8 @@ static bool ensure_model(LearnedBufs& b) {
     if (b.P) return true;
     bool ok = galloc((void**)&b.P, kParams * 4) && galloc((void**)&b.m, kParams * 4) && galloc((void**)&b.v, kParams * 4) &&
               galloc((void**)&b.grad, kParams * 4) && galloc((void**)&b.loss, 64) &&
-              galloc((void**)&b.yc, (65536 + 1) * 4) && galloc((void**)&b.b0, (65536 + 2) * 4) &&
-              galloc((void**)&b.slope, (65536 + 1) * 4) && galloc((void**)&b.stat, 64) &&
+              galloc((void**)&b.R, (65536 + 2) * 4) && galloc((void**)&b.yd, (65536 + 2) * 8) &&
+              galloc((void**)&b.leaf_lo, (kLeaves + 2) * 4) && galloc((void**)&b.cell, (65536 + 2) * 8) && galloc((void**)&b.stat, 64) &&
               hip_ok(hipHostMalloc((void**)&b.loss_host, 64, hipHostMallocDefault), "hipHostMalloc") &&
               hip_ok(hipHostMalloc((void**)&b.skip_host, 64, hipHostMallocDefault), "hipHostMalloc") &&
               hip_ok(hipHostMalloc((void**)&b.qhost, 1025 * 8, hipHostMallocDefault), "hipHostMalloc") &&
@@ -461,9 +579,27 @@ static bool ensure_model(LearnedBufs& b) {
               hip_ok(hipHostMalloc((void**)&b.phost, kParams * 4, hipHostMallocDefault), "hipHostMalloc") &&
               galloc((void**)&b.qdev, 1025 * 8) &&
               galloc((void**)&b.w.hist, 8 * 256 * 8) && galloc((void**)&b.w.offsets, 8 * 256 * 8) &&
-              galloc((void**)&b.w.skip, 64) && galloc((void**)&b.w.ticket, 64);
+              galloc((void**)&b.w.skip, 64);
     for (int i = 0; ok && i < 8; ++i) ok = hip_ok(hipEventCreate(&b.ev[i]), "hipEventCreate");
     return ok;
+}
+
+constexpr uint64_t kChunk = 65536;           // rows per chunk of the fix-up lists
+
+static bool ensure_chunks(LearnedBufs& b, uint64_t nch) {
+    if (nch <= b.c_cap) return true;
+    if (b.cdev) (void)hipFree(b.cdev);
+    if (b.cflag) (void)hipFree(b.cflag);
+    if (b.chost) (void)hipHostFree(b.chost);
+    if (b.cflag_host) (void)hipHostFree(b.cflag_host);
+    b.cdev = nullptr; b.cflag = nullptr; b.chost = nullptr; b.cflag_host = nullptr; b.c_cap = 0;
+    uint64_t cap = nch < 1024 ? 1024 : nch * 2;
+    if (!galloc((void**)&b.cdev, cap * 24) || !galloc((void**)&b.cflag, cap * 4) ||
+        !hip_ok(hipHostMalloc((void**)&b.chost, cap * 24, hipHostMallocDefault), "hipHostMalloc") ||
+        !hip_ok(hipHostMalloc((void**)&b.cflag_host, cap * 4, hipHostMallocDefault), "hipHostMalloc"))
+        return false;
+    b.c_cap = cap;
+    return true;
 }
 
 static bool ensure_train(LearnedBufs& b, uint64_t t) {
@@ -485,7 +621,8 @@ static bool ensure_train(LearnedBufs& b, uint64_t t) {
 }
 
 // Deterministic initialisation: ReLU breakpoints of the first layer spread
-// over [0, 1], He-scaled second layer, small output layer.
+// over the input range (so that the ends are not flat), He-scaled second
+// layer, small output layer.
 static void init_params(std::vector<float>& P, uint64_t seed) {
     P.assign(kParams, 0.0f);
     uint64_t z = seed;
@@ -503,8 +640,9 @@ static void init_params(std::vector<float>& P, uint64_t seed) {
         return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
     };
     for (int j = 0; j < kH; ++j) {
-        double w = 1.0 + next() * 3.0;
-        double brk = (j + 0.5) / kH;
+        // rising and falling units alternately, breakpoints spread over [-0.05, 1.05]
+        double w = (j % 2 ? -1.0 : 1.0) * (1.0 + next() * 3.0);
+        double brk = -0.05 + 1.1 * (j + 0.5) / kH;
         P[oW1 + j] = (float)w;
         P[oB1 + j] = (float)(-w * brk);
     }
@@ -520,9 +658,67 @@ static bool mlp_grad(LearnedBufs& b, const float* P, const float* x, const float
     uint64_t blocks = (n + kGradThreads - 1) / kGradThreads;
     hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)blocks), dim3(kGradThreads), 0, st, P, x, y, n, 1.0f / (float)n, b.part);
     float b1c = 1.0f - std::pow(0.9f, (float)step), b2c = 1.0f - std::pow(0.999f, (float)step);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((kParams + 1 + 255) / 256), dim3(256), 0, st, const_cast<float*>(P), b.m, b.v,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kParams + 1 + 63) / 64), dim3(256), 0, st, const_cast<float*>(P), b.m, b.v,
                        b.part, (uint32_t)blocks, adam, lr, b1c, b2c, grad, loss, 1.0f / (float)n);
     return hip_ok(hipGetLastError(), "mlp launch");
+}
+
+static float train_lr(int s, int steps) { return s <= (steps * 2) / 3 ? 0.02f : 0.004f; }
+
+// The Adam loop of the learned sort (steps x {gradient, reduce + update}) as
+// a graph of kernel nodes: built once per buffer set, then one launch.
+static bool train_graph(LearnedBufs& b, uint64_t T, int steps, hipStream_t st) {
+    const void* key[7] = {b.P, b.tx, b.ty, b.part, b.m, b.v, b.loss};
+    bool same = b.graph_exec && b.graph_T == T && b.graph_steps == steps;
+    for (int i = 0; same && i < 7; ++i) same = key[i] == b.graph_key[i];
+    if (!same) {
+        if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
+        if (b.graph) (void)hipGraphDestroy(b.graph);
+        b.graph_exec = nullptr;
+        b.graph = nullptr;
+        if (!hip_ok(hipGraphCreate(&b.graph, 0), "hipGraphCreate")) return false;
+        const float* Pc = b.P;
+        float* Pm = b.P;
+        const float* x = b.tx;
+        const float* y = b.ty;
+        uint64_t n = T;
+        float inv_n = 1.0f / (float)T;
+        float* part = b.part;
+        const float* partc = b.part;
+        uint32_t nblocks = (uint32_t)((T + kGradThreads - 1) / kGradThreads);
+        int adam = 1;
+        float* gnull = nullptr;
+        hipGraphNode_t prev = nullptr;
+        for (int s = 1; s <= steps; ++s) {
+            float lr = train_lr(s, steps);
+            float b1c = 1.0f - std::pow(0.9f, (float)s), b2c = 1.0f - std::pow(0.999f, (float)s);
+            void* ga[] = {(void*)&Pc, (void*)&x, (void*)&y, (void*)&n, (void*)&inv_n, (void*)&part};
+            hipKernelNodeParams kp = {};
+            kp.func = reinterpret_cast<void*>(&k_mlp_grad);
+            kp.gridDim = dim3(nblocks);
+            kp.blockDim = dim3(kGradThreads);
+            kp.sharedMemBytes = 0;
+            kp.kernelParams = ga;
+            kp.extra = nullptr;
+            hipGraphNode_t gn, rn;
+            if (!hip_ok(hipGraphAddKernelNode(&gn, b.graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp), "graph node"))
+                return false;
+            void* ra[] = {(void*)&Pm, (void*)&b.m, (void*)&b.v, (void*)&partc, (void*)&nblocks, (void*)&adam,
+                          (void*)&lr, (void*)&b1c, (void*)&b2c, (void*)&gnull, (void*)&b.loss, (void*)&inv_n};
+            kp.func = reinterpret_cast<void*>(&k_mlp_reduce);
+            kp.gridDim = dim3((kParams + 1 + 63) / 64);
+            kp.blockDim = dim3(256);
+            kp.kernelParams = ra;
+            if (!hip_ok(hipGraphAddKernelNode(&rn, b.graph, &gn, 1, &kp), "graph node")) return false;
+            prev = rn;
+        }
+        if (!hip_ok(hipGraphInstantiate(&b.graph_exec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate"))
+            return false;
+        for (int i = 0; i < 7; ++i) b.graph_key[i] = key[i];
+        b.graph_T = T;
+        b.graph_steps = steps;
+    }
+    return hip_ok(hipGraphLaunch(b.graph_exec, st), "hipGraphLaunch");
 }
 
 }  // namespace vx
@@ -592,8 +788,8 @@ int voxel_gpu_mlp_param_count(void) { return kParams; }
 int voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, int rows_per_bucket_log2) {
     std::lock_guard<std::mutex> lk(api_mutex());
     g_tune.sample_log2 = sample_log2 > 0 ? sample_log2 : 20;
-    g_tune.train_points = train_points > 0 ? train_points : 16384;
-    g_tune.train_steps = train_steps >= 0 ? (train_steps ? train_steps : 200) : 0;
+    g_tune.train_points = train_points > 0 ? train_points : 4096;
+    g_tune.train_steps = train_steps >= 0 ? (train_steps ? train_steps : 100) : 0;
     g_tune.bucket_log2 = rows_per_bucket_log2 > 0 ? rows_per_bucket_log2 : 11;
     if (g_tune.bucket_log2 > 11) g_tune.bucket_log2 = 11;     // a bucket must fit the LDS sort on average
     return 0;
@@ -636,7 +832,7 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         if (stats) *stats = S;
         return 0;
     }
-    uint64_t words = radix_tiles(n) * 256;
+    uint64_t words = radix_work_words(n);
     if (words > b.w.status_cap) {
         if (b.w.status) (void)hipFree(b.w.status);
         b.w.status = nullptr; b.w.status_cap = 0;
@@ -685,7 +881,7 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     const uint64_t* q = b.qhost;
     const uint64_t kmin = q[0], kmax = q[1024];
     const uint64_t span = kmax - kmin;
-    auto logg = [&](uint64_t k) { float f = (float)(k - kmin); uint32_t u; std::memcpy(&u, &f, 4); return (double)u; };
+    auto logg = [&](uint64_t k) { return (double)TableDigit::grid(k - kmin, 1); };
     double dev_lin = 0, dev_log = 0;
     const double glog = logg(kmax);
     for (int i = 0; i <= 1024; ++i) {
@@ -713,19 +909,28 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         if (!hip_ok(launch_zero32((uint32_t*)b.m, kParams, st), "zero") || !hip_ok(launch_zero32((uint32_t*)b.v, kParams, st), "zero"))
             return -1;
         const int steps = g_tune.train_steps;
-        for (int s = 1; s <= steps; ++s) {
-            float lr = s <= (steps * 2) / 3 ? 0.02f : 0.004f;
-            if (!mlp_grad(b, b.P, b.tx, b.ty, T, nullptr, b.loss, 1, lr, s, st)) return -1;
-        }
+        if (steps > 0 && !train_graph(b, T, steps, st)) return -1;
         S.train_steps = steps;
     }
     hipEventRecord(b.ev[2], st);
-    // 4. fine-bucket table
-    hipLaunchKernelGGL(k_cells, dim3(gridn(ncells, 256)), dim3(256), 0, st, b.P, ncells, cshift, logmode,
-                       (double)gmax_i, (double)span, model, b.yc);
-    hipLaunchKernelGGL(k_table, dim3(1), dim3(1024), 0, st, b.yc, ncells, nfine, cshift, b.b0, b.slope);
+    // 4. CDF at the cell starts -> fine-bucket table.  model 0: MLP root routes
+    //    each cell to one of kLeaves linear models fitted to the sample;
+    //    model 1: linear in the key (no model).
+    hipLaunchKernelGGL(k_root_cells, dim3(gridn(ncells + 1, 256)), dim3(256), 0, st, b.P, ncells, cshift,
+                       (double)gmax_i, model, b.R);
+    hipLaunchKernelGGL(k_prefix_max<float>, dim3(1), dim3(1024), 0, st, b.R, ncells + 1);
+    if (model == 0) {
+        hipLaunchKernelGGL(k_leaf_lo, dim3((kLeaves + 256) / 256), dim3(256), 0, st, b.samp2, SS, b.R, ncells, cshift,
+                           kmin, kmax, logmode, kLeaves, b.leaf_lo);
+        hipLaunchKernelGGL(k_leaf_cells, dim3(gridn(ncells, 256)), dim3(256), 0, st, b.samp2, SS, b.R, ncells, cshift,
+                           kmin, kmax, logmode, kLeaves, b.leaf_lo, b.yd);
+    } else {
+        hipLaunchKernelGGL(k_widen, dim3((ncells + 255) / 256), dim3(256), 0, st, b.R, ncells, b.yd);
+    }
+    hipLaunchKernelGGL(k_prefix_max<double>, dim3(1), dim3(1024), 0, st, b.yd, ncells);
+    hipLaunchKernelGGL(k_table, dim3((ncells + 255) / 256), dim3(256), 0, st, b.yd, ncells, nfine, b.cell);
     if (!hip_ok(hipGetLastError(), "table")) return -1;
-    TableDigit dig{b.b0, b.slope, kmin, kmax, ncells, cshift, logmode, nfine};
+    TableDigit dig{b.cell, kmin, kmax, ncells, cshift, logmode, nfine, std::ldexp(1.0f, -cshift)};
     hipEventRecord(b.ev[3], st);
     // 5. stable partition by coarse bucket (LSD over its bytes)
     const uint32_t* vin = payload();
@@ -757,7 +962,11 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
                        (uint32_t)nb, dig, b.big, b.stat);
     if (!hip_ok(hipGetLastError(), "bucket sort")) return -1;
     hipEventRecord(b.ev[5], st);
-    // 7. buckets over the LDS capacity
+    // 7. buckets over the LDS capacity: those holding one key value are in
+    //    stable order already; the others are gathered (in bucket order),
+    //    sorted together by the radix sort -- key ranges of buckets do not
+    //    overlap, so one stable sort of the concatenation sorts each bucket --
+    //    and scattered back.
     std::vector<uint64_t> starts(nb + 1);
     std::vector<uint32_t> bigf(nb);
     if (!hip_ok(hipMemcpyAsync(starts.data(), b.start, (nb + 1) * 8, hipMemcpyDeviceToHost, st), "read bounds") ||
@@ -767,7 +976,7 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         return -1;
     S.longest_fine_run = b.stat_host[0];
     S.long_runs = b.stat_host[1];
-    std::vector<uint64_t> chunks;
+    std::vector<uint64_t> chunks;            // (row start, row end, bucket first row)
     std::vector<uint32_t> chunk_bucket;
     uint64_t maxb = 0;
     for (uint64_t i = 0; i < nb; ++i) {
@@ -776,53 +985,60 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         if (!bigf[i]) continue;
         S.big_buckets++;
         S.big_rows += m;
-        for (uint64_t s = starts[i]; s < starts[i + 1]; s += 65536) {
-            chunks.push_back(s);
-            chunks.push_back(std::min(s + 65536, starts[i + 1]));
+        for (uint64_t r = starts[i]; r < starts[i + 1]; r += kChunk) {
+            chunks.push_back(r);
+            chunks.push_back(std::min(r + kChunk, starts[i + 1]));
             chunks.push_back(starts[i]);
             chunk_bucket.push_back((uint32_t)i);
         }
     }
     S.max_bucket = maxb;
     std::vector<uint32_t> nonuniform;
-    if (!chunks.empty()) {
-        uint64_t nch = chunk_bucket.size();
-        uint64_t* dch = nullptr;
-        uint32_t* dfl = nullptr;
-        if (!galloc((void**)&dch, chunks.size() * 8)) return -1;
-        if (!galloc((void**)&dfl, nch * 4)) { (void)hipFree(dch); return -1; }
-        std::vector<uint32_t> fl(nch);
-        bool ok = hip_ok(hipMemcpyAsync(dch, chunks.data(), chunks.size() * 8, hipMemcpyHostToDevice, st), "copy chunks") &&
-                  hip_ok(launch_zero32(dfl, nch, st), "zero");
+    if (!chunk_bucket.empty()) {
+        const uint64_t nch = chunk_bucket.size();
+        if (!ensure_chunks(b, nch)) return -1;
+        std::memcpy(b.chost, chunks.data(), nch * 24);
+        bool ok = hip_ok(hipMemcpyAsync(b.cdev, b.chost, nch * 24, hipMemcpyHostToDevice, st), "copy chunks") &&
+                  hip_ok(launch_zero32(b.cflag, nch, st), "zero");
         if (ok) {
-            hipLaunchKernelGGL(k_big_check, dim3((unsigned)std::min<uint64_t>(nch, 8192)), dim3(256), 0, st, kout, dch, nch, dfl);
-            ok = hip_ok(hipGetLastError(), "big check");
+            hipLaunchKernelGGL(k_big_check, dim3((unsigned)std::min<uint64_t>(nch, 8192)), dim3(256), 0, st, kout,
+                               b.cdev, nch, b.cflag);
+            ok = hip_ok(hipGetLastError(), "big check") &&
+                 hip_ok(hipMemcpyAsync(b.cflag_host, b.cflag, nch * 4, hipMemcpyDeviceToHost, st), "read") &&
+                 hip_ok(hipStreamSynchronize(st), "sync");
         }
-        if (ok) ok = hip_ok(hipMemcpyAsync(fl.data(), dfl, nch * 4, hipMemcpyDeviceToHost, st), "read") &&
-                     hip_ok(hipStreamSynchronize(st), "sync");
-        (void)hipFree(dch);
-        (void)hipFree(dfl);
         if (!ok) return -1;
         for (uint64_t j = 0; j < nch; ++j)
-            if (fl[j] && (nonuniform.empty() || nonuniform.back() != chunk_bucket[j])) nonuniform.push_back(chunk_bucket[j]);
+            if (b.cflag_host[j] && (nonuniform.empty() || nonuniform.back() != chunk_bucket[j]))
+                nonuniform.push_back(chunk_bucket[j]);
     }
     S.big_unsorted = nonuniform.size();
     uint64_t unsorted_rows = 0;
     for (uint32_t bi : nonuniform) unsorted_rows += starts[bi + 1] - starts[bi];
-    if (nonuniform.size() > 64 || unsorted_rows > n / 4) {
+    if (unsorted_rows > n / 2) {
         // the model did not split the keys: plain radix sort of the input
         S.fell_back = 1;
         const uint32_t* v0 = payload();
         if (!v0 || sort_pairs_u64(kin, v0, kout, vout, n, c) < 0) return -1;
-    } else {
-        for (uint32_t bi : nonuniform) {
-            // k1/v1 are free now: sort the segment there, copy it back
-            uint64_t s = starts[bi], m = starts[bi + 1] - starts[bi];
-            if (sort_pairs_u64(kout + s, vout + s, b.k1, b.v1, m, c) < 0) return -1;
-            if (!hip_ok(hipMemcpyAsync(kout + s, b.k1, m * 8, hipMemcpyDeviceToDevice, st), "copy") ||
-                !hip_ok(hipMemcpyAsync(vout + s, b.v1, m * 4, hipMemcpyDeviceToDevice, st), "copy"))
-                return -1;
-        }
+    } else if (unsorted_rows) {
+        // move list (source row, destination row, rows) in kChunk pieces
+        std::vector<uint64_t> mv;
+        uint64_t at = 0;
+        for (uint32_t bi : nonuniform)
+            for (uint64_t r = starts[bi]; r < starts[bi + 1]; r += kChunk) {
+                uint64_t len = std::min<uint64_t>(kChunk, starts[bi + 1] - r);
+                mv.push_back(r); mv.push_back(at); mv.push_back(len);
+                at += len;
+            }
+        const uint64_t nmv = mv.size() / 3;
+        if (!ensure_chunks(b, nmv)) return -1;
+        std::memcpy(b.chost, mv.data(), nmv * 24);
+        if (!hip_ok(hipMemcpyAsync(b.cdev, b.chost, nmv * 24, hipMemcpyHostToDevice, st), "copy moves")) return -1;
+        const unsigned mb = (unsigned)std::min<uint64_t>(nmv, 65535);
+        hipLaunchKernelGGL(k_move, dim3(mb), dim3(256), 0, st, kout, vout, b.k1, b.v1, b.cdev, nmv, 0);
+        if (!hip_ok(hipGetLastError(), "gather") || sort_pairs_u64(b.k1, b.v1, b.k2, b.v2, at, c) < 0) return -1;
+        hipLaunchKernelGGL(k_move, dim3(mb), dim3(256), 0, st, b.k2, b.v2, kout, vout, b.cdev, nmv, 1);
+        if (!hip_ok(hipGetLastError(), "scatter")) return -1;
     }
     hipEventRecord(b.ev[6], st);
     if (!hip_ok(hipEventSynchronize(b.ev[6]), "sync")) return -1;

@@ -3,21 +3,25 @@
 // (ops/sort.hpp:83-160) and, through an order-preserving key transform,
 // MergeSort<f64>/SortOperator argsorts (ops/sort.hpp:196-213, 506-541).
 //
-// Structure (single-pass "onesweep" digit passes):
+// Structure:
 //   1. k_hist: one read of the keys builds all eight 8-bit digit histograms
 //      (per-wave LDS counters, then global 64-bit counters);
 //   2. k_scan_hist: exclusive digit offsets per pass; a pass whose digit is
 //      the same for every key is skipped (a stable pass over one digit is
 //      the identity permutation);
-//   3. k_onesweep per executed pass: each workgroup takes the next tile id
-//      from an atomic ticket, ranks its 4096 keys stably (wave ballots of the
-//      digit bits + per-wave LDS counters), publishes its per-digit tile count
-//      and looks back over earlier tiles' status words (decoupled look-back,
-//      flag|count in one 32-bit word) for the exclusive prefix, then writes
-//      keys and payloads through LDS so that each digit run is written by
-//      consecutive lanes.
-// Tile ids are taken in launch order, so every tile a workgroup waits on is
-// owned by a workgroup that already runs: the look-back always completes.
+//   3. per executed pass (reduce-then-scan, no inter-workgroup waits): the
+//      keys are cut into at most 2048 ranges of whole 4096-key tiles;
+//      k_upsweep counts each range's digits, k_range_scan gives every range
+//      its first output position per digit, and k_downsweep (one workgroup
+//      per range, tiles in order) ranks each tile's keys stably (wave
+//      ballots of the digit bits + per-wave LDS counters) and writes keys and
+//      payloads through LDS so that each digit run is written by consecutive
+//      lanes.
+// A decoupled look-back over 4096-key tiles (one status word per tile and
+// digit) was measured at about 0.1 TB/s here: the status loads are coherent
+// across the eight XCDs and the look-back walks them one at a time; the
+// upsweep costs one extra read of the keys (8 of 32 bytes per key and pass)
+// and leaves every workgroup independent.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -110,7 +114,7 @@ hipError_t launch_iota(uint32_t* idx, uint64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
-uint64_t radix_tiles(uint64_t n) { return (n + kTile - 1) / kTile; }
+uint64_t radix_work_words(uint64_t n) { return radix_ranges(n) * 256 * 2; }
 
 hipError_t launch_scan_hist(const unsigned long long* ghist, int passes, uint64_t n, unsigned long long* offsets,
                             uint32_t* skip, hipStream_t st) {

@@ -11,42 +11,55 @@ struct LsdDigit {
     __host__ __device__ __forceinline__ uint32_t digit(uint64_t k, int p) const {
         return (uint32_t)(k >> (8 * p)) & 0xFFu;
     }
+    // all passes' digits from one evaluation: bits(k) then digit p of it
+    __host__ __device__ __forceinline__ uint64_t bits(uint64_t k) const { return k; }
+    __host__ __device__ __forceinline__ uint32_t digit_of_bits(uint64_t b, int p) const {
+        return (uint32_t)(b >> (8 * p)) & 0xFFu;
+    }
 };
 
 // Digit p of a monotone "fine bucket" id fb(k) computed from the key with a
 // piecewise-linear table (learned CDF partitioner, vx_learned.hip).  Keys are
-// clamped to [kmin, kmax] and mapped to a grid coordinate G(k): the offset
-// k - kmin itself (linear grid), or the bit pattern of (float)(k - kmin)
-// (logarithmic grid: 2^23 steps per octave).  Cell c = G >> cshift covers
-// fine buckets [b0[c], b0[c+1]]; inside it fb = b0[c] + floor(slope[c] *
-// (G - c << cshift)), clamped to b0[c+1].  Every step is non-decreasing in
-// k, so fb is non-decreasing in k: a stable sort by fb followed by a stable
-// sort inside each fb run sorts by key.  Partition pass p uses bits
-// [8 + 8p, 16 + 8p) of fb (the coarse bucket is fb >> 8; the low 8 bits
-// split a bucket once more inside LDS).
+// clamped to [kmin, kmax] and mapped to a grid coordinate G(d), d = k - kmin:
+// d itself (linear grid), or the bit pattern of (float)d + 1 minus that of
+// 1.0f (logarithmic grid: 2^23 steps per octave, G(0) = 0).  Cell c = G >>
+// cshift covers fine buckets [lo, lo + width] (cell[c] = {lo, width}, lo of
+// cell c + 1 = lo + width); inside it fb = lo + floor(width * (G - c <<
+// cshift) * 2^-cshift), clamped to lo + width.  Every step is
+// non-decreasing in k, so fb is non-decreasing in k: a stable sort by fb
+// followed by a stable sort inside each fb run sorts by key.  Partition pass
+// p uses bits [8 + 8p, 16 + 8p) of fb (the coarse bucket is fb >> 8; the low
+// 8 bits split a bucket once more inside LDS).
 struct TableDigit {
-    const uint32_t* b0;         // ncells + 1 entries, non-decreasing, b0[ncells] = total fine buckets
-    const float* slope;         // ncells entries
+    const uint2* cell;          // ncells entries {first fine bucket, fine buckets in the cell}
     uint64_t kmin, kmax;
     uint32_t ncells;
     int cshift;
     int logmode;
     uint32_t nfine;             // total fine buckets
+    float inv_width;            // 2^-cshift
+    __host__ __device__ static __forceinline__ uint64_t grid(uint64_t d, int logmode) {
+        if (!logmode) return d;
+        float f = (float)d + 1.0f;
+        return (uint64_t)(__builtin_bit_cast(uint32_t, f) - 0x3F800000u);
+    }
     __device__ __forceinline__ uint32_t fine(uint64_t k) const {
         uint64_t kc = k < kmin ? kmin : (k > kmax ? kmax : k);
-        uint64_t d = kc - kmin;
-        uint64_t g = logmode ? (uint64_t)__float_as_uint((float)d) : d;
+        uint64_t g = grid(kc - kmin, logmode);
         uint64_t c = g >> cshift;
         if (c >= ncells) c = ncells - 1;
-        uint64_t off = g - (c << cshift);
-        uint32_t lo = b0[c], hi = b0[c + 1];
-        float f = floorf(slope[c] * (float)off);
-        uint32_t add = f <= 0.0f ? 0u : (f >= (float)(hi - lo) ? hi - lo : (uint32_t)f);
-        uint32_t fb = lo + add;
+        const uint2 e = cell[c];
+        float f = floorf((float)e.y * ((float)(g - (c << cshift)) * inv_width));
+        uint32_t add = f <= 0.0f ? 0u : (f >= (float)e.y ? e.y : (uint32_t)f);
+        uint32_t fb = e.x + add;
         return fb < nfine ? fb : nfine - 1;
     }
     __device__ __forceinline__ uint32_t digit(uint64_t k, int p) const {
         return (fine(k) >> (8 + 8 * p)) & 0xFFu;
+    }
+    __device__ __forceinline__ uint64_t bits(uint64_t k) const { return fine(k); }
+    __device__ __forceinline__ uint32_t digit_of_bits(uint64_t b, int p) const {
+        return (uint32_t)(b >> (8 + 8 * p)) & 0xFFu;
     }
 };
 
@@ -55,12 +68,11 @@ struct RadixWork {
     unsigned long long* hist = nullptr;     // 8 x 256 digit counts
     unsigned long long* offsets = nullptr;  // 8 x 256 exclusive digit offsets
     uint32_t* skip = nullptr;               // 8 flags: pass moves nothing
-    uint32_t* status = nullptr;             // tiles x 256 look-back words
-    uint32_t* ticket = nullptr;             // tile id counter
+    uint32_t* status = nullptr;             // per-range digit counts and offsets (radix_work_words)
     uint64_t status_cap = 0;                // words
 };
 
-uint64_t radix_tiles(uint64_t n);
+uint64_t radix_work_words(uint64_t n);     // words of RadixWork::status a pass over n keys needs
 hipError_t launch_zero32(uint32_t* p, uint64_t n, hipStream_t st);
 hipError_t launch_f64_keys(const double* x, uint64_t n, int descending, uint64_t* keys, uint32_t* idx, hipStream_t st);
 hipError_t launch_iota(uint32_t* idx, uint64_t n, hipStream_t st);

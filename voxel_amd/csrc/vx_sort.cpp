@@ -47,10 +47,9 @@ bool ensure_work(SortBufs& b, uint64_t n) {
         if (!hip_ok(hipMalloc((void**)&w.hist, 8 * 256 * 8), "hipMalloc(hist)")) return false;
         if (!hip_ok(hipMalloc((void**)&w.offsets, 8 * 256 * 8), "hipMalloc(offsets)")) return false;
         if (!hip_ok(hipMalloc((void**)&w.skip, 64), "hipMalloc(skip)")) return false;
-        if (!hip_ok(hipMalloc((void**)&w.ticket, 64), "hipMalloc(ticket)")) return false;
         if (!hip_ok(hipHostMalloc((void**)&b.skip_host, 64, hipHostMallocDefault), "hipHostMalloc(skip)")) return false;
     }
-    uint64_t words = radix_tiles(n) * 256;
+    uint64_t words = radix_work_words(n);
     if (words > w.status_cap) {
         if (w.status) (void)hipFree(w.status);
         w.status = nullptr;
