@@ -321,12 +321,13 @@ typedef struct voxel_gpu_learned_stats {
     int32_t  model;             /* 0 MLP CDF, 1 equal-width (linear in the key) */
     int32_t  log_grid;          /* model input: 1 = float bits of key - min */
     int32_t  fell_back;         /* 1 = whole input sorted by the plain radix sort */
-    int32_t  partition_passes;  /* onesweep passes over the coarse bucket id */
+    int32_t  partition_passes;  /* digit passes over the bucket id */
     int32_t  train_steps;
-    float    final_loss;
-    uint64_t buckets, cells;
-    uint64_t max_bucket, big_buckets, big_rows, big_unsorted;
-    uint64_t longest_fine_run, long_runs;
+    float    final_loss;        /* mean squared error of the logit CDF fit */
+    uint64_t buckets, cells, groups;
+    uint64_t longest_bucket;    /* rows of the largest bucket sorted by a wave */
+    uint64_t widest_group;      /* rows of the largest group of buckets */
+    uint64_t big_groups, big_rows, big_unsorted;  /* groups over a wave's capacity */
     float    sample_ms, train_ms, table_ms, partition_ms, bucket_ms, fixup_ms;
 } voxel_gpu_learned_stats;
 /* Stable sort of (u64 key, u32 payload) pairs, bit-identical to

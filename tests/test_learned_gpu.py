@@ -133,7 +133,7 @@ def test_learned_model_beats_equal_width_buckets(gpu):
     assert np.array_equal(k0, k1) and np.array_equal(v0, v1)
     assert s0["fell_back"] == 0 and s0["big_rows"] < n // 100, str(s0)
     assert s1["big_rows"] > n // 4 or s1["fell_back"] == 1, str(s1)
-    assert s0["max_bucket"] * 10 < s1["max_bucket"], str((s0, s1))
+    assert s0["widest_group"] * 10 < s1["widest_group"], str((s0, s1))
     assert s0["final_loss"] < 0.02, str(s0)            # mean squared error of the logit CDF
 
 

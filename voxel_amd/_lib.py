@@ -48,8 +48,8 @@ class Agg(C.Structure):
 class LearnedStats(C.Structure):
     _fields_ = [("model", C.c_int32), ("log_grid", C.c_int32), ("fell_back", C.c_int32),
                 ("partition_passes", C.c_int32), ("train_steps", C.c_int32), ("final_loss", C.c_float),
-                ("buckets", U64), ("cells", U64), ("max_bucket", U64), ("big_buckets", U64), ("big_rows", U64),
-                ("big_unsorted", U64), ("longest_fine_run", U64), ("long_runs", U64),
+                ("buckets", U64), ("cells", U64), ("groups", U64), ("longest_bucket", U64), ("widest_group", U64),
+                ("big_groups", U64), ("big_rows", U64), ("big_unsorted", U64),
                 ("sample_ms", C.c_float), ("train_ms", C.c_float), ("table_ms", C.c_float),
                 ("partition_ms", C.c_float), ("bucket_ms", C.c_float), ("fixup_ms", C.c_float)]
 

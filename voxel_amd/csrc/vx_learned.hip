@@ -3,25 +3,31 @@
 // ops/sort.hpp:83-160); fewer passes over HBM when the key distribution can
 // be learned.
 //
-//  1. sample S keys at SplitMix64-random positions, sort the sample (radix);
+//  1. sample S keys at SplitMix64-random positions, sort the sample (radix),
+//     read 1025 quantiles;
 //  2. pick the grid (linear offsets, or float bit patterns = logarithmic)
 //     under which the sample CDF is closest to a straight line;
-//  3. train a 1->32->32->1 FP32 ReLU MLP on (grid position, rank) pairs of the
-//     sorted sample with this file's own forward/backward kernels and Adam;
-//  4. evaluate the MLP at every grid cell start, make it non-decreasing
-//     (prefix max), and turn it into the fine-bucket table of TableDigit
-//     (vx_radix.hpp): fb(k) is non-decreasing in k by construction;
-//  5. stable-partition the pairs by coarse bucket fb >> 8 with the onesweep
-//     digit passes (vx_onesweep.hpp) -- ceil(log2(buckets) / 8) passes instead
-//     of eight;
-//  6. one workgroup per bucket (<= kCap rows): rank by fb & 255 in LDS
-//     (stable), then insertion-sort each fine run by key (stable), write back;
-//  7. buckets larger than kCap are checked for a single key value (then they
-//     are already in stable order); any other one is sorted by the LSD radix
-//     sort, and when those are too many the whole input is sorted by the LSD
-//     radix sort instead (reported in the statistics).
-// The equal-width variant (model = 1) replaces step 3-4 by a CDF that is
-// linear in the key over [min, max] (the method without the network).
+//  3. train a 1->32->32->1 FP32 ReLU MLP on (grid position, logit of rank)
+//     pairs of the sorted sample with this file's own forward/backward
+//     kernels and Adam (the step loop is one graph launch);
+//  4. CDF at every grid cell start: the MLP (sigmoid, rescaled to [0, 1] over
+//     the sample range) routes the cell to one of 4096 linear leaf models
+//     fitted to the sample (a two-stage recursive model); made non-decreasing
+//     and turned into the bucket table of TableDigit (vx_radix.hpp): the
+//     bucket id fb(k) is non-decreasing in k by construction;
+//  5. stable partition of the pairs by fb with the digit passes of
+//     vx_onesweep.hpp -- ceil(log2(buckets) / 8) passes instead of eight;
+//     buckets are ~32-256 rows on average;
+//  6. bucket starts (marks + suffix minimum), then one wave per group of
+//     consecutive buckets (<= 512 rows) ranks every row inside its bucket in
+//     LDS (key, then row order) and writes it to its place;
+//  7. larger groups are checked for a single key value (then they are in
+//     stable order already); the others are gathered, sorted together by the
+//     LSD radix sort and scattered back, and when they hold most of the rows
+//     the whole input is sorted by the LSD radix sort instead (reported in
+//     the statistics).
+// The equal-width variant (model = 1) replaces steps 3-4 by a CDF that is
+// linear in the key over the sample's [min, max] (the method without a model).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -46,8 +52,9 @@ constexpr int kH = 32;                       // hidden width
 constexpr int kParams = 32 + 32 + 32 * 32 + 32 + 32 + 1;   // 1153
 constexpr int oW1 = 0, oB1 = 32, oW2 = 64, oB2 = 64 + 1024, oW3 = oB2 + 32, oB3 = oW3 + 32;
 constexpr int kGradThreads = 64;
-constexpr int kCap = 4096;                   // rows a workgroup sorts in LDS
 constexpr uint32_t kLeaves = 4096;           // linear models under the MLP
+constexpr int kMinBucketLog2 = 5;            // average bucket rows are kept at or above 32
+constexpr uint32_t kBigCap = 65536;          // groups over a wave's capacity that are fixed up
 
 namespace {
 
@@ -358,136 +365,168 @@ __global__ __launch_bounds__(256) void k_table(const double* __restrict__ y, uin
     cell[c] = make_uint2(a, start(c + 1) - a);
 }
 
-// start[b] = first row whose coarse bucket is >= b (rows are in bucket order)
-__global__ __launch_bounds__(256) void k_bounds(const uint64_t* __restrict__ keys, uint64_t n, TableDigit dig,
-                                                uint32_t nb, uint64_t* __restrict__ start) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += stride) {
-        uint64_t lo = 0, hi = n;
-        while (lo < hi) {
-            uint64_t mid = lo + (hi - lo) / 2;
-            if ((dig.fine(keys[mid]) >> 8) < b) lo = mid + 1;
-            else hi = mid;
-        }
-        start[b] = lo;
+// start[f] = first row of bucket f, for every non-empty bucket (rows are
+// in bucket order after the partition, fbs = their bucket ids); other
+// entries keep the fill value n.
+constexpr int kMarkRows = 8;
+__global__ __launch_bounds__(256) void k_mark_starts(const uint32_t* __restrict__ fbs, uint64_t n,
+                                                     uint32_t* __restrict__ start) {
+    const uint64_t r0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kMarkRows;
+    if (r0 >= n) return;
+    uint32_t f[kMarkRows];
+    if (r0 + kMarkRows <= n) {                       // 32-byte aligned: two 16-byte loads
+        const uint4 a = *reinterpret_cast<const uint4*>(fbs + r0);
+        const uint4 b = *reinterpret_cast<const uint4*>(fbs + r0 + 4);
+        f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w; f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
+    } else {
+#pragma unroll
+        for (int u = 0; u < kMarkRows; ++u) f[u] = r0 + u < n ? fbs[r0 + u] : 0u;
+    }
+    uint32_t prev = r0 ? fbs[r0 - 1] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int u = 0; u < kMarkRows; ++u) {
+        if (r0 + u < n && f[u] != prev) start[f[u]] = (uint32_t)(r0 + u);
+        prev = f[u];
     }
 }
 
-// Per-bucket sort of buckets of at most CAP rows: stable rank by the low 8
-// bits of the fine bucket in LDS, then every fine run sorted by key (stable):
-// runs of up to kShortRun rows by one lane with insertion sort, longer runs
-// by the whole workgroup, each row's rank counted over the run.
-constexpr uint32_t kShortRun = 32;
+// Suffix minimum over start[0..m): empty buckets take the start of the next
+// non-empty one.  Three steps over chunks of kMinChunk entries.
+constexpr uint32_t kMinChunk = 8192;
+__global__ __launch_bounds__(256) void k_min_chunks(const uint32_t* __restrict__ a, uint64_t m, uint32_t* __restrict__ cmin) {
+    __shared__ uint32_t sm[256];
+    const uint64_t s = (uint64_t)blockIdx.x * kMinChunk;
+    uint32_t v = 0xFFFFFFFFu;
+    for (uint64_t i = s + threadIdx.x; i < s + kMinChunk && i < m; i += 256) v = min(v, a[i]);
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) sm[threadIdx.x] = min(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cmin[blockIdx.x] = sm[0];
+}
 
-template <int CAP>
-__global__ __launch_bounds__(256) void k_bucket_sort(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                     const uint64_t* __restrict__ start, uint32_t nb, TableDigit dig,
-                                                     uint32_t* __restrict__ big, unsigned long long* __restrict__ stat) {
-    constexpr int R = CAP / 256;
-    __shared__ uint64_t kb[CAP];
-    __shared__ uint32_t vb[CAP];
-    __shared__ uint32_t wcnt[4][256];
-    __shared__ uint32_t sst[257], scn[256], longrun[256];
-    __shared__ uint32_t nlong;
-    __shared__ uint8_t dig8[CAP];                  // fine digit of each LDS slot
-    __shared__ uint8_t islong[256];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, t = threadIdx.x;
-    uint32_t longest = 0, nlong_runs = 0;
-    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
-        const uint64_t s = start[b], e = start[b + 1], m = e - s;
+// one workgroup: carry[c] = min of cmin over chunks after c
+__global__ __launch_bounds__(256) void k_min_carry(const uint32_t* __restrict__ cmin, uint32_t nc, uint32_t* __restrict__ carry) {
+    __shared__ uint32_t sm[256];
+    const uint32_t t = threadIdx.x, per = (nc + 255) / 256;
+    const uint32_t lo = min(t * per, nc), hi = min(lo + per, nc);
+    uint32_t v = 0xFFFFFFFFu;
+    for (uint32_t c = lo; c < hi; ++c) v = min(v, cmin[c]);
+    sm[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {                 // inclusive suffix min over threads
+        uint32_t x = t + o < 256 ? sm[t + o] : 0xFFFFFFFFu;
+        __syncthreads();
+        sm[t] = min(sm[t], x);
+        __syncthreads();
+    }
+    uint32_t run = t + 1 < 256 ? sm[t + 1] : 0xFFFFFFFFu;
+    for (uint32_t c = hi; c-- > lo;) {
+        carry[c] = run;
+        run = min(run, cmin[c]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_min_apply(uint32_t* __restrict__ a, uint64_t m, const uint32_t* __restrict__ carry) {
+    __shared__ uint32_t sm[256];
+    constexpr uint32_t per = kMinChunk / 256;
+    const uint64_t s = (uint64_t)blockIdx.x * kMinChunk + (uint64_t)threadIdx.x * per;
+    uint32_t v[per];
+    uint32_t run = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = per - 1; j >= 0; --j) {
+        v[j] = s + j < m ? a[s + j] : 0xFFFFFFFFu;
+        run = min(run, v[j]);
+    }
+    sm[threadIdx.x] = run;
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    for (int o = 1; o < 256; o <<= 1) {
+        uint32_t x = t + o < 256 ? sm[t + o] : 0xFFFFFFFFu;
+        __syncthreads();
+        sm[t] = min(sm[t], x);
+        __syncthreads();
+    }
+    run = min(carry[blockIdx.x], t + 1 < 256 ? sm[t + 1] : 0xFFFFFFFFu);
+#pragma unroll
+    for (int j = per - 1; j >= 0; --j) {
+        run = min(run, v[j]);
+        if (s + j < m) a[s + j] = run;
+    }
+}
+
+// One wave per group of G consecutive buckets (<= kWaveRows rows): the rows
+// go to LDS, every row's rank inside its bucket is counted over the bucket's
+// rows (key, then row order: stable), and the row is written to its place.
+// Larger groups are copied through unchanged and listed in biglist.
+constexpr int kWaveRows = 512;
+__global__ __launch_bounds__(256) void k_group_sort(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                    const uint32_t* __restrict__ start, uint32_t nb, uint32_t G,
+                                                    uint32_t ngroups, uint32_t* __restrict__ biglist, uint32_t big_cap,
+                                                    unsigned long long* __restrict__ stat) {
+    __shared__ uint64_t kb[4][kWaveRows];
+    __shared__ uint32_t vb[4][kWaveRows];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t waves = gridDim.x * 4;
+    uint32_t longest = 0, widest = 0;
+    for (uint32_t g = blockIdx.x * 4 + w; g < ngroups; g += waves) {
+        const uint32_t b0 = g * G, b1 = min(b0 + G, nb);
+        const uint32_t s = start[b0], e = start[b1], m = e - s;
+        widest = m > widest ? m : widest;
         if (m == 0) continue;
-        if (m > CAP || m == 1) {
-            for (uint64_t i = t; i < m; i += 256) { kout[s + i] = kin[s + i]; vout[s + i] = vin[s + i]; }
-            if (m > 1 && t == 0) big[b] = 1;
+        if (m > kWaveRows || m == 1) {
+            for (uint32_t i = lane; i < m; i += 64) { kout[s + i] = kin[s + i]; vout[s + i] = vin[s + i]; }
+            if (m > 1 && lane == 0) {
+                uint32_t slot = atomicAdd((uint32_t*)&stat[2], 1u);
+                if (slot < big_cap) biglist[slot] = g;
+            }
             continue;
         }
-        for (int i = t; i < 4 * 256; i += 256) (&wcnt[0][0])[i] = 0;
-        if (t == 0) nlong = 0;
-        __syncthreads();
-        uint64_t k[R];
-        uint32_t v[R], rk[R], dd[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            uint64_t idx = (uint64_t)w * (64 * R) + r * 64 + lane;
-            bool valid = idx < m;
-            k[r] = valid ? kin[s + idx] : 0;
-            v[r] = valid ? vin[s + idx] : 0;
-            dd[r] = valid ? (dig.fine(k[r]) & 0xFFu) : 0u;
-            uint64_t peers = match_digit(dd[r], valid);
-            uint32_t before = valid ? wcnt[w][dd[r]] : 0;
-            uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
-            if (valid && below == 0) wcnt[w][dd[r]] = before + (uint32_t)__popcll(peers);
-            rk[r] = before + below;
-        }
-        __syncthreads();
-        uint32_t tot = 0;
-#pragma unroll
-        for (int ww = 0; ww < 4; ++ww) { uint32_t c = wcnt[ww][t]; wcnt[ww][t] = tot; tot += c; }
-        scn[t] = tot;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            uint32_t x = t >= off ? scn[t - off] : 0;
-            __syncthreads();
-            scn[t] += x;
-            __syncthreads();
-        }
-        sst[t] = scn[t] - tot;
-        if (t == 255) sst[256] = scn[255];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            uint64_t idx = (uint64_t)w * (64 * R) + r * 64 + lane;
-            if (idx < m) {
-                uint32_t pos = sst[dd[r]] + wcnt[w][dd[r]] + rk[r];
-                kb[pos] = k[r];
-                vb[pos] = v[r];
-                dig8[pos] = (uint8_t)dd[r];
+        for (uint32_t i = lane; i < m; i += 64) { kb[w][i] = kin[s + i]; vb[w][i] = vin[s + i]; }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t i = lane; i < m; i += 64) {
+            // bucket of row i: last b in [b0, b1) with start[b] <= s + i
+            uint32_t lo = 0, hi = m;
+            for (uint32_t bb = b0 + 1; bb < b1; ++bb) {
+                const uint32_t x = start[bb] - s;
+                if (x <= i) lo = x;
+                else { hi = x; break; }
             }
-        }
-        __syncthreads();
-        const uint32_t lo = sst[t], hi = lo + tot;
-        if (tot > longest) longest = tot;
-        if (tot <= kShortRun) {
-            for (uint32_t i = lo + 1; i < hi; ++i) {
-                uint64_t key = kb[i];
-                uint32_t val = vb[i];
-                uint32_t j = i;
-                while (j > lo && kb[j - 1] > key) { kb[j] = kb[j - 1]; vb[j] = vb[j - 1]; --j; }
-                kb[j] = key;
-                vb[j] = val;
+            const uint64_t key = kb[w][i];
+            uint32_t rank = 0;
+            uint32_t j = lo;
+            for (; j + 4 <= hi; j += 4) {
+                const uint64_t k0 = kb[w][j], k1 = kb[w][j + 1], k2 = kb[w][j + 2], k3 = kb[w][j + 3];
+                rank += (k0 < key) || (k0 == key && j < i);
+                rank += (k1 < key) || (k1 == key && j + 1 < i);
+                rank += (k2 < key) || (k2 == key && j + 2 < i);
+                rank += (k3 < key) || (k3 == key && j + 3 < i);
             }
-        } else {
-            longrun[atomicAdd(&nlong, 1u)] = t;
-        }
-        islong[t] = tot > kShortRun;
-        __syncthreads();
-        const uint32_t nl = nlong;
-        nlong_runs += nl;
-        // rows of short runs are final in LDS; rows of long runs are ranked below
-        for (uint32_t i = t; i < m; i += 256)
-            if (!islong[dig8[i]]) { kout[s + i] = kb[i]; vout[s + i] = vb[i]; }
-        for (uint32_t q = 0; q < nl; ++q) {
-            const uint32_t d = longrun[q], rlo = sst[d], rhi = sst[d + 1];
-            for (uint32_t i = rlo + t; i < rhi; i += 256) {
-                const uint64_t key = kb[i];
-                uint32_t rank = 0;
-                for (uint32_t j = rlo; j < rhi; ++j) {
-                    uint64_t kj = kb[j];
-                    rank += (kj < key) || (kj == key && j < i);
-                }
-                kout[s + rlo + rank] = key;
-                vout[s + rlo + rank] = vb[i];
+            for (; j < hi; ++j) {
+                const uint64_t kj = kb[w][j];
+                rank += (kj < key) || (kj == key && j < i);
             }
+            if (hi - lo > longest) longest = hi - lo;
+            kout[s + lo + rank] = key;
+            vout[s + lo + rank] = vb[w][i];
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     for (int msk = 32; msk > 0; msk >>= 1) {
         uint32_t o = __shfl_xor(longest, msk, 64);
         longest = o > longest ? o : longest;
     }
-    if (lane == 0) atomicMax(&stat[0], (unsigned long long)longest);
-    if (t == 0 && nlong_runs) atomicAdd(&stat[1], (unsigned long long)nlong_runs);
+    if (lane == 0) {
+        atomicMax(&stat[0], (unsigned long long)longest);
+        atomicMax(&stat[1], (unsigned long long)widest);
+    }
 }
 
 // chunk list entries: (row start, row end, bucket first row); flags[j] = 1 when
@@ -517,6 +556,20 @@ __global__ __launch_bounds__(256) void k_move(const uint64_t* __restrict__ ka, c
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// list entries (first bucket, end bucket, -) -> (first row, end row, -)
+__global__ __launch_bounds__(256) void k_gather_starts(const uint32_t* __restrict__ start, uint64_t* __restrict__ list,
+                                                       uint64_t nl) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nl) return;
+    list[3 * j] = start[list[3 * j]];
+    list[3 * j + 1] = start[list[3 * j + 1]];
+}
+
 unsigned gridn(uint64_t n, unsigned cap) {
     uint64_t b = (n + 255) / 256;
     return (unsigned)(b ? (b < cap ? b : cap) : 1);
@@ -533,7 +586,9 @@ struct LearnedBufs {
     float* R = nullptr; double* yd = nullptr; uint32_t* leaf_lo = nullptr;
     uint2* cell = nullptr;
     uint64_t* k1 = nullptr; uint64_t* k2 = nullptr; uint32_t* v1 = nullptr; uint32_t* v2 = nullptr; uint64_t cap = 0;
-    uint64_t* start = nullptr; uint32_t* big = nullptr; uint64_t nb_cap = 0;
+    uint32_t* f1 = nullptr; uint32_t* f2 = nullptr;      // carried bucket ids
+    uint32_t* start = nullptr; uint32_t* cmin = nullptr; uint64_t nb_cap = 0;
+    uint32_t* biglist = nullptr; uint32_t* biglist_host = nullptr;
     unsigned long long* stat = nullptr;
     RadixWork w;
     uint32_t* skip_host = nullptr;
@@ -552,7 +607,7 @@ struct LearnedBufs {
 };
 
 struct LearnedTune {
-    int sample_log2 = 20, train_points = 4096, train_steps = 100, bucket_log2 = 11;
+    int sample_log2 = 20, train_points = 4096, train_steps = 100, bucket_log2 = 8;
 };
 static LearnedTune g_tune;
 
@@ -577,7 +632,8 @@ static bool ensure_model(LearnedBufs& b) {
               hip_ok(hipHostMalloc((void**)&b.qhost, 1025 * 8, hipHostMallocDefault), "hipHostMalloc") &&
               hip_ok(hipHostMalloc((void**)&b.stat_host, 64, hipHostMallocDefault), "hipHostMalloc") &&
               hip_ok(hipHostMalloc((void**)&b.phost, kParams * 4, hipHostMallocDefault), "hipHostMalloc") &&
-              galloc((void**)&b.qdev, 1025 * 8) &&
+              galloc((void**)&b.qdev, 1025 * 8) && galloc((void**)&b.biglist, kBigCap * 4) &&
+              hip_ok(hipHostMalloc((void**)&b.biglist_host, kBigCap * 4, hipHostMallocDefault), "hipHostMalloc") &&
               galloc((void**)&b.w.hist, 8 * 256 * 8) && galloc((void**)&b.w.offsets, 8 * 256 * 8) &&
               galloc((void**)&b.w.skip, 64);
     for (int i = 0; ok && i < 8; ++i) ok = hip_ok(hipEventCreate(&b.ev[i]), "hipEventCreate");
@@ -790,8 +846,9 @@ int voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, i
     g_tune.sample_log2 = sample_log2 > 0 ? sample_log2 : 20;
     g_tune.train_points = train_points > 0 ? train_points : 4096;
     g_tune.train_steps = train_steps >= 0 ? (train_steps ? train_steps : 100) : 0;
-    g_tune.bucket_log2 = rows_per_bucket_log2 > 0 ? rows_per_bucket_log2 : 11;
-    if (g_tune.bucket_log2 > 11) g_tune.bucket_log2 = 11;     // a bucket must fit the LDS sort on average
+    g_tune.bucket_log2 = rows_per_bucket_log2 > 0 ? rows_per_bucket_log2 : 8;
+    if (g_tune.bucket_log2 > 8) g_tune.bucket_log2 = 8;       // a bucket group must fit one wave's LDS on average
+    if (g_tune.bucket_log2 < kMinBucketLog2) g_tune.bucket_log2 = kMinBucketLog2;
     return 0;
 }
 
@@ -810,11 +867,11 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     LearnedBufs& b = lbufs(c);
     if (!ensure_model(b)) return -1;
     if (n > b.cap) {
-        for (void* p : {(void*)b.k1, (void*)b.k2, (void*)b.v1, (void*)b.v2})
+        for (void* p : {(void*)b.k1, (void*)b.k2, (void*)b.v1, (void*)b.v2, (void*)b.f1, (void*)b.f2})
             if (p) (void)hipFree(p);
-        b.k1 = b.k2 = nullptr; b.v1 = b.v2 = nullptr; b.cap = 0;
+        b.k1 = b.k2 = nullptr; b.v1 = b.v2 = nullptr; b.f1 = b.f2 = nullptr; b.cap = 0;
         if (!galloc((void**)&b.k1, n * 8) || !galloc((void**)&b.k2, n * 8) || !galloc((void**)&b.v1, n * 4) ||
-            !galloc((void**)&b.v2, n * 4))
+            !galloc((void**)&b.v2, n * 4) || !galloc((void**)&b.f1, n * 4) || !galloc((void**)&b.f2, n * 4))
             return -1;
         b.cap = n;
     }
@@ -852,17 +909,23 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         b.s_cap = SS;
     }
     if (!ensure_train(b, T)) return -1;
-    uint64_t nb = 1;
-    while ((nb << g_tune.bucket_log2) < n) nb <<= 1;                 // ~2^bucket_log2 rows per bucket
-    const uint32_t nfine = (uint32_t)(nb * 256);
-    int cbits = 0;
-    while ((1ull << cbits) < nb) ++cbits;
-    const int parts = (cbits + 7) / 8;
+    // Buckets: as few partition passes as keep the average bucket at or
+    // below 2^max_log2 rows, then as many buckets as those passes address
+    // without going below 2^min_log2 rows on average.
+    auto clog2 = [](uint64_t x) { int b = 0; while ((1ull << b) < x) ++b; return b; };
+    const int bits_min = std::max(1, clog2((n + (1ull << g_tune.bucket_log2) - 1) >> g_tune.bucket_log2));
+    const int parts = (bits_min + 7) / 8;
+    const int bits = std::max(bits_min, std::min(8 * parts, clog2(n >> kMinBucketLog2)));
+    const uint64_t nb = 1ull << bits;
+    const uint32_t nfine = (uint32_t)nb;
+    const uint32_t G = (uint32_t)std::max<uint64_t>(1, 64 / std::max<uint64_t>(1, n >> bits));
+    const uint32_t ngroups = (uint32_t)((nb + G - 1) / G);
+    const uint64_t nchunks_min = (nb + 1 + kMinChunk - 1) / kMinChunk;
     if (nb + 1 > b.nb_cap) {
         if (b.start) (void)hipFree(b.start);
-        if (b.big) (void)hipFree(b.big);
-        b.start = nullptr; b.big = nullptr; b.nb_cap = 0;
-        if (!galloc((void**)&b.start, (nb + 1) * 8) || !galloc((void**)&b.big, (nb + 1) * 4)) return -1;
+        if (b.cmin) (void)hipFree(b.cmin);
+        b.start = nullptr; b.cmin = nullptr; b.nb_cap = 0;
+        if (!galloc((void**)&b.start, (nb + 1) * 4) || !galloc((void**)&b.cmin, nchunks_min * 8)) return -1;
         b.nb_cap = nb + 1;
     }
     S.buckets = nb;
@@ -932,70 +995,99 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     if (!hip_ok(hipGetLastError(), "table")) return -1;
     TableDigit dig{b.cell, kmin, kmax, ncells, cshift, logmode, nfine, std::ldexp(1.0f, -cshift)};
     hipEventRecord(b.ev[3], st);
-    // 5. stable partition by coarse bucket (LSD over its bytes)
+    // 5. stable partition by bucket id (LSD over its bytes); pass 0 computes
+    //    the id once and every pass carries it next to the rows
     const uint32_t* vin = payload();
     if (!vin) return -1;
-    if (!hip_ok(radix_histogram(kin, n, dig, parts, b.w, st), "model histogram")) return -1;
-    if (!hip_ok(hipMemcpyAsync(b.skip_host, b.w.skip, 32, hipMemcpyDeviceToHost, st), "read skip") ||
-        !hip_ok(hipStreamSynchronize(st), "sync"))
-        return -1;
     const uint64_t* ks = kin;
     const uint32_t* vs = vin;
+    const uint32_t* fs = nullptr;
     int executed = 0;
     for (int p = 0; p < parts; ++p) {
-        if (b.skip_host[p]) continue;
         uint64_t* kd = (ks == b.k1) ? b.k2 : b.k1;
         uint32_t* vd = (vs == b.v1) ? b.v2 : b.v1;          // v2 may hold the row index: read once, then free
-        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, p, b.w, st), "model pass")) return -1;
+        uint32_t* fd = (fs == b.f1) ? b.f2 : b.f1;
+        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, p, b.w, st, fs, fd), "model pass")) return -1;
         ks = kd;
         vs = vd;
+        fs = fd;
         ++executed;
     }
     S.partition_passes = executed;
     hipEventRecord(b.ev[4], st);
-    // 6. bucket bounds and per-bucket sort into kout/vout
-    hipLaunchKernelGGL(k_bounds, dim3(gridn(nb + 1, 4096)), dim3(256), 0, st, ks, (uint64_t)n, dig, (uint32_t)nb, b.start);
-    if (!hip_ok(launch_zero32(b.big, nb + 1, st), "zero") || !hip_ok(launch_zero32((uint32_t*)b.stat, 4, st), "zero"))
-        return -1;
-    unsigned sblocks = (unsigned)std::min<uint64_t>(nb, (uint64_t)device_cu_count() * 8);
-    hipLaunchKernelGGL((k_bucket_sort<kCap>), dim3(sblocks), dim3(256), 0, st, ks, vs, kout, vout, b.start,
-                       (uint32_t)nb, dig, b.big, b.stat);
+    // 6. bucket starts and per-group sort into kout/vout
+    hipLaunchKernelGGL(k_fill32, dim3(gridn(nb + 1, 8192)), dim3(256), 0, st, b.start, nb + 1, (uint32_t)n);
+    hipLaunchKernelGGL(k_mark_starts, dim3((unsigned)((n + 256 * kMarkRows - 1) / (256 * kMarkRows))), dim3(256), 0, st,
+                       fs, (uint64_t)n, b.start);
+    {
+        const uint32_t nc = (uint32_t)nchunks_min;
+        uint32_t* carry = b.cmin + nc;
+        hipLaunchKernelGGL(k_min_chunks, dim3(nc), dim3(256), 0, st, b.start, nb + 1, b.cmin);
+        hipLaunchKernelGGL(k_min_carry, dim3(1), dim3(256), 0, st, b.cmin, nc, carry);
+        hipLaunchKernelGGL(k_min_apply, dim3(nc), dim3(256), 0, st, b.start, nb + 1, carry);
+    }
+    if (!hip_ok(launch_zero32((uint32_t*)b.stat, 16, st), "zero")) return -1;
+    const unsigned gblocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((ngroups + 3) / 4, (uint64_t)device_cu_count() * 16));
+    hipLaunchKernelGGL(k_group_sort, dim3(gblocks), dim3(256), 0, st, ks, vs, kout, vout, b.start, (uint32_t)nb, G,
+                       ngroups, b.biglist, kBigCap, b.stat);
     if (!hip_ok(hipGetLastError(), "bucket sort")) return -1;
     hipEventRecord(b.ev[5], st);
-    // 7. buckets over the LDS capacity: those holding one key value are in
-    //    stable order already; the others are gathered (in bucket order),
-    //    sorted together by the radix sort -- key ranges of buckets do not
-    //    overlap, so one stable sort of the concatenation sorts each bucket --
+    // 7. groups over a wave's capacity: those holding one key value are in
+    //    stable order already; the others are gathered (in group order),
+    //    sorted together by the radix sort -- key ranges of groups do not
+    //    overlap, so one stable sort of the concatenation sorts each group --
     //    and scattered back.
-    std::vector<uint64_t> starts(nb + 1);
-    std::vector<uint32_t> bigf(nb);
-    if (!hip_ok(hipMemcpyAsync(starts.data(), b.start, (nb + 1) * 8, hipMemcpyDeviceToHost, st), "read bounds") ||
-        !hip_ok(hipMemcpyAsync(bigf.data(), b.big, nb * 4, hipMemcpyDeviceToHost, st), "read flags") ||
-        !hip_ok(hipMemcpyAsync(b.stat_host, b.stat, 16, hipMemcpyDeviceToHost, st), "read stats") ||
+    if (!hip_ok(hipMemcpyAsync(b.stat_host, b.stat, 24, hipMemcpyDeviceToHost, st), "read stats") ||
         !hip_ok(hipStreamSynchronize(st), "sync"))
         return -1;
-    S.longest_fine_run = b.stat_host[0];
-    S.long_runs = b.stat_host[1];
-    std::vector<uint64_t> chunks;            // (row start, row end, bucket first row)
-    std::vector<uint32_t> chunk_bucket;
+    S.longest_bucket = b.stat_host[0];
+    S.widest_group = b.stat_host[1];
+    S.groups = ngroups;
+    const uint64_t nbig = b.stat_host[2] & 0xFFFFFFFFull;
+    bool overflow = nbig > kBigCap;
+    std::vector<uint32_t> bigs;
+    std::vector<uint32_t> gstart;               // start rows of the big groups: (first, end) pairs
+    if (nbig && !overflow) {
+        if (!hip_ok(hipMemcpyAsync(b.biglist_host, b.biglist, nbig * 4, hipMemcpyDeviceToHost, st), "read list") ||
+            !hip_ok(hipStreamSynchronize(st), "sync"))
+            return -1;
+        bigs.assign(b.biglist_host, b.biglist_host + nbig);
+        std::sort(bigs.begin(), bigs.end());
+        // group bounds: start[g * G] and start[min(g * G + G, nb)]
+        if (!ensure_chunks(b, nbig)) return -1;
+        for (uint64_t j = 0; j < nbig; ++j) {
+            b.chost[3 * j] = (uint64_t)bigs[j] * G;
+            b.chost[3 * j + 1] = std::min<uint64_t>((uint64_t)bigs[j] * G + G, nb);
+            b.chost[3 * j + 2] = 0;
+        }
+        if (!hip_ok(hipMemcpyAsync(b.cdev, b.chost, nbig * 24, hipMemcpyHostToDevice, st), "copy list")) return -1;
+        hipLaunchKernelGGL(k_gather_starts, dim3((unsigned)((nbig + 255) / 256)), dim3(256), 0, st, b.start, b.cdev, nbig);
+        if (!hip_ok(hipGetLastError(), "starts") ||
+            !hip_ok(hipMemcpyAsync(b.chost, b.cdev, nbig * 24, hipMemcpyDeviceToHost, st), "read starts") ||
+            !hip_ok(hipStreamSynchronize(st), "sync"))
+            return -1;
+        gstart.resize(2 * nbig);
+        for (uint64_t j = 0; j < nbig; ++j) { gstart[2 * j] = (uint32_t)b.chost[3 * j]; gstart[2 * j + 1] = (uint32_t)b.chost[3 * j + 1]; }
+    }
+    std::vector<uint64_t> chunks;            // (row start, row end, group first row)
+    std::vector<uint32_t> chunk_group;
     uint64_t maxb = 0;
-    for (uint64_t i = 0; i < nb; ++i) {
-        uint64_t m = starts[i + 1] - starts[i];
-        maxb = std::max(maxb, m);
-        if (!bigf[i]) continue;
-        S.big_buckets++;
-        S.big_rows += m;
-        for (uint64_t r = starts[i]; r < starts[i + 1]; r += kChunk) {
+    for (uint64_t j = 0; j < (overflow ? 0 : nbig); ++j) {
+        const uint64_t r0 = gstart[2 * j], r1 = gstart[2 * j + 1];
+        maxb = std::max(maxb, r1 - r0);
+        S.big_groups++;
+        S.big_rows += r1 - r0;
+        for (uint64_t r = r0; r < r1; r += kChunk) {
             chunks.push_back(r);
-            chunks.push_back(std::min(r + kChunk, starts[i + 1]));
-            chunks.push_back(starts[i]);
-            chunk_bucket.push_back((uint32_t)i);
+            chunks.push_back(std::min(r + kChunk, r1));
+            chunks.push_back(r0);
+            chunk_group.push_back((uint32_t)j);
         }
     }
-    S.max_bucket = maxb;
-    std::vector<uint32_t> nonuniform;
-    if (!chunk_bucket.empty()) {
-        const uint64_t nch = chunk_bucket.size();
+    (void)maxb;
+    std::vector<uint32_t> nonuniform;            // indexes into bigs
+    if (!chunk_group.empty()) {
+        const uint64_t nch = chunk_group.size();
         if (!ensure_chunks(b, nch)) return -1;
         std::memcpy(b.chost, chunks.data(), nch * 24);
         bool ok = hip_ok(hipMemcpyAsync(b.cdev, b.chost, nch * 24, hipMemcpyHostToDevice, st), "copy chunks") &&
@@ -1009,13 +1101,13 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         }
         if (!ok) return -1;
         for (uint64_t j = 0; j < nch; ++j)
-            if (b.cflag_host[j] && (nonuniform.empty() || nonuniform.back() != chunk_bucket[j]))
-                nonuniform.push_back(chunk_bucket[j]);
+            if (b.cflag_host[j] && (nonuniform.empty() || nonuniform.back() != chunk_group[j]))
+                nonuniform.push_back(chunk_group[j]);
     }
     S.big_unsorted = nonuniform.size();
     uint64_t unsorted_rows = 0;
-    for (uint32_t bi : nonuniform) unsorted_rows += starts[bi + 1] - starts[bi];
-    if (unsorted_rows > n / 2) {
+    for (uint32_t j : nonuniform) unsorted_rows += gstart[2 * j + 1] - gstart[2 * j];
+    if (overflow || unsorted_rows > n / 2) {
         // the model did not split the keys: plain radix sort of the input
         S.fell_back = 1;
         const uint32_t* v0 = payload();
@@ -1024,9 +1116,9 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         // move list (source row, destination row, rows) in kChunk pieces
         std::vector<uint64_t> mv;
         uint64_t at = 0;
-        for (uint32_t bi : nonuniform)
-            for (uint64_t r = starts[bi]; r < starts[bi + 1]; r += kChunk) {
-                uint64_t len = std::min<uint64_t>(kChunk, starts[bi + 1] - r);
+        for (uint32_t j : nonuniform)
+            for (uint64_t r = gstart[2 * j]; r < gstart[2 * j + 1]; r += kChunk) {
+                uint64_t len = std::min<uint64_t>(kChunk, gstart[2 * j + 1] - r);
                 mv.push_back(r); mv.push_back(at); mv.push_back(len);
                 at += len;
             }

@@ -57,7 +57,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 // digits, k_range_scan turns the counts into each range's first output
 // position per digit, and k_downsweep (one workgroup per range, tiles in
 // order) scatters stably.  No workgroup waits on another.
-constexpr uint64_t kMaxRanges = 2048;
+constexpr uint64_t kMaxRanges = 2048;     // and words of RadixWork::status: 2 x ranges x 256 + 256
 
 __host__ __device__ inline uint64_t radix_tiles_per_range(uint64_t n) {
     uint64_t tiles = (n + kTile - 1) / kTile;
@@ -72,34 +72,54 @@ __host__ __device__ inline uint64_t radix_ranges(uint64_t n) {
     return r ? r : 1;
 }
 
+// Digit source of a pass: the key through the digit function, or (CIN) a
+// carried 32-bit digit word whose byte `pass` is the digit.
+template <class Digit, bool CIN>
+__device__ __forceinline__ uint32_t pass_digit(const Digit& dig, uint64_t key, uint32_t word, int pass) {
+    if constexpr (CIN) return (word >> (8 * pass)) & 0xFFu;
+    else return dig.digit(key, pass);
+}
+
 // cnt[d * nr + range] = keys of the range with digit d
-template <class Digit>
-__global__ __launch_bounds__(kThreads) void k_upsweep(const uint64_t* __restrict__ keys, uint64_t n, Digit dig, int pass,
-                                                      uint64_t range_len, uint32_t nr, uint32_t* __restrict__ cnt) {
+template <class Digit, bool CIN>
+__global__ __launch_bounds__(kThreads) void k_upsweep(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ fin,
+                                                      uint64_t n, Digit dig, int pass, uint64_t range_len, uint32_t nr,
+                                                      uint32_t* __restrict__ cnt) {
     __shared__ uint32_t h[kWaves][256];
     const int w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&h[0][0])[i] = 0;
     __syncthreads();
     const uint64_t s = (uint64_t)blockIdx.x * range_len;
     const uint64_t e = s + range_len < n ? s + range_len : n;
-    for (uint64_t i0 = s; i0 < e; i0 += kThreads) {         // block-uniform trip count
-        const uint64_t i = i0 + threadIdx.x;
-        const bool valid = i < e;
-        const uint32_t d = valid ? dig.digit(keys[i], pass) : 0u;
-        const uint64_t peers = match_digit(d, valid);
-        if (valid && (peers & lanemask_lt()) == 0) h[w][d] += (uint32_t)__popcll(peers);
+    constexpr int U = 8;                                    // keys per lane in flight
+    for (uint64_t i0 = s; i0 < e; i0 += kThreads * U) {      // block-uniform trip count
+        uint64_t kk[U];
+        uint32_t dd[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kThreads + threadIdx.x;
+            if constexpr (CIN) dd[u] = i < e ? fin[i] : 0u;
+            else kk[u] = i < e ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) dd[u] = pass_digit<Digit, CIN>(dig, CIN ? 0 : kk[u], dd[u], pass);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool valid = i0 + (uint64_t)u * kThreads + threadIdx.x < e;
+            const uint64_t peers = match_digit(dd[u], valid);
+            if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[w][dd[u]], (uint32_t)__popcll(peers));
+        }
     }
     __syncthreads();
     const int d = threadIdx.x;
     cnt[(uint64_t)d * nr + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
-// One workgroup per digit: off[d * nr + r] = offsets[pass][d] + keys with
-// digit d in ranges before r.  (Internal linkage: every file that
+// One workgroup per digit: off[d * nr + r] = keys with digit d in ranges
+// before r; tot[d] = keys with digit d.  (Internal linkage: every file that
 // instantiates the passes gets its own copy.)
 static __global__ __launch_bounds__(kThreads) void k_range_scan(const uint32_t* __restrict__ cnt, uint32_t nr,
-                                                         const unsigned long long* __restrict__ offsets, int pass,
-                                                         uint32_t* __restrict__ off) {
+                                                                uint32_t* __restrict__ off, uint32_t* __restrict__ tot) {
     __shared__ uint32_t sc[kThreads];
     const uint32_t d = blockIdx.x, t = threadIdx.x;
     const uint32_t per = (nr + kThreads - 1) / kThreads;
@@ -115,63 +135,39 @@ static __global__ __launch_bounds__(kThreads) void k_range_scan(const uint32_t* 
         sc[t] += x;
         __syncthreads();
     }
-    uint32_t run = (uint32_t)offsets[pass * 256 + d] + sc[t] - sum;
+    uint32_t run = sc[t] - sum;
     uint32_t* o = off + (uint64_t)d * nr;
     for (uint32_t r = lo; r < hi; ++r) {
         o[r] = run;
         run += c[r];
     }
+    if (t == kThreads - 1) tot[d] = sc[t];
 }
 
-template <class Digit>
+// Stable scatter of one range's tiles.  Keys, payloads (and with CIN the
+// carried digit words) of a tile are loaded together, ranked by digit, placed
+// in LDS in digit order and written out so that each digit run is written
+// by consecutive lanes.  COUT writes the digit word of every row (carried
+// from fin, or dig.bits(key)) next to it.
+template <class Digit, bool CIN, bool COUT>
 __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                        uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                        const uint32_t* __restrict__ fin, uint64_t* __restrict__ kout,
+                                                        uint32_t* __restrict__ vout, uint32_t* __restrict__ fout,
                                                         uint64_t n, Digit dig, int pass, uint64_t tiles_per_range,
-                                                        uint32_t nr, const uint32_t* __restrict__ off) {
-    __shared__ uint64_t buf[kTile];                 // keys, then payloads (as u32)
+                                                        uint32_t nr, const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ tot) {
+    __shared__ uint64_t kb[kTile];
+    __shared__ uint32_t vb[kTile];
+    __shared__ uint32_t fb[(CIN || COUT) ? kTile : 1];   // digit words of the LDS slots
     __shared__ uint32_t wcnt[kWaves][256];          // per-wave digit counters -> wave prefixes
     __shared__ uint32_t lstart[256];                // tile-local start of each digit
     __shared__ uint32_t run[256];                   // output position of the digit's next key
     __shared__ uint32_t scan_s[256];
-    __shared__ uint8_t dig8[kTile];                 // digit of each LDS slot
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, d = threadIdx.x;
-    run[d] = off[(uint64_t)d * nr + blockIdx.x];
-    for (uint64_t t = 0; t < tiles_per_range; ++t) {
-        const uint64_t tbase = ((uint64_t)blockIdx.x * tiles_per_range + t) * kTile;
-        if (tbase >= n) break;                                          // block-uniform
-        for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
-        __syncthreads();
-        const uint64_t base = tbase + (uint64_t)w * (64 * kRounds);
-        uint64_t k[kRounds];
-        uint32_t rank[kRounds];
-#pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            uint64_t idx = base + r * 64 + lane;
-            k[r] = idx < n ? kin[idx] : 0;
-        }
-#pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            uint64_t idx = base + r * 64 + lane;
-            bool valid = idx < n;
-            uint32_t dd = dig.digit(k[r], pass);
-            uint64_t peers = match_digit(dd, valid);
-            uint32_t before = valid ? wcnt[w][dd] : 0;
-            uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
-            // the lowest lane of each peer group adds the group's size
-            if (valid && below == 0) wcnt[w][dd] = before + (uint32_t)__popcll(peers);
-            rank[r] = (before + below) | (dd << 16);        // rank < 4096, digit above it
-        }
-        __syncthreads();
-        // per digit: wave prefixes, tile count, tile-local digit start
-        uint32_t tot = 0;
-#pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) {
-            uint32_t c = wcnt[ww][d];
-            wcnt[ww][d] = tot;
-            tot += c;
-        }
-        scan_s[d] = tot;
+    {   // digit base: exclusive scan of the digit totals
+        const uint32_t t0 = tot[d];
+        scan_s[d] = t0;
         __syncthreads();
         for (int o = 1; o < 256; o <<= 1) {
             uint32_t x = d >= o ? scan_s[d - o] : 0;
@@ -179,53 +175,110 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restri
             scan_s[d] += x;
             __syncthreads();
         }
-        lstart[d] = scan_s[d] - tot;
-        __syncthreads();
-        // keys into LDS in digit order (stable: wave prefix + rank)
-        uint32_t lpos[kRounds];
+        run[d] = scan_s[d] - t0 + off[(uint64_t)d * nr + blockIdx.x];
+    }
+    for (uint64_t t = 0; t < tiles_per_range; ++t) {
+        const uint64_t tbase = ((uint64_t)blockIdx.x * tiles_per_range + t) * kTile;
+        if (tbase >= n) break;                                          // block-uniform
+        for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
+        const uint64_t base = tbase + (uint64_t)w * (64 * kRounds);
+        uint64_t k[kRounds];
+        uint32_t v[kRounds], f[kRounds];
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
-            uint64_t idx = base + r * 64 + lane;
-            lpos[r] = kTile;
+            const uint64_t idx = base + r * 64 + lane;
+            const bool ok = idx < n;
+            k[r] = ok ? kin[idx] : 0;
+            v[r] = ok ? vin[idx] : 0;
+            if constexpr (CIN) f[r] = ok ? fin[idx] : 0u;
+        }
+        __syncthreads();                                                // wcnt cleared
+        uint32_t rank[kRounds];
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            if constexpr (COUT && !CIN) f[r] = (uint32_t)dig.bits(k[r]);
+            if constexpr (COUT || CIN) rank[r] = (f[r] >> (8 * pass)) & 0xFFu;
+            else rank[r] = dig.digit(k[r], pass);                       // all digits first: loads in flight together
+        }
+        // rank inside the wave: the lowest lane of each group of equal digits
+        // adds the group's size to the wave's counter (LDS atomics of one wave
+        // are performed in issue order, so round r + 1 sees round r's adds)
+        // and hands the counter's old value to its group
+        uint32_t leader[kRounds];
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const uint64_t idx = base + r * 64 + lane;
+            const bool valid = idx < n;
+            const uint32_t dd = rank[r];
+            const uint64_t peers = match_digit(dd, valid);
+            const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+            uint32_t old = 0;
+            if (valid && below == 0) old = atomicAdd(&wcnt[w][dd], (uint32_t)__popcll(peers));
+            leader[r] = old;
+            rank[r] = below | (dd << 16) | (valid ? ((uint32_t)(__ffsll((unsigned long long)peers) - 1) << 24) : 0u);
+        }
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const uint32_t before = (uint32_t)__shfl((int)leader[r], (int)(rank[r] >> 24), 64);
+            rank[r] = ((rank[r] & 0xFFu) + before) | (rank[r] & 0x00FF0000u);   // rank < 4096, digit in bits 16..23
+        }
+        __syncthreads();
+        // per digit: wave prefixes, tile count, tile-local digit start
+        uint32_t tcount = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) {
+            uint32_t c = wcnt[ww][d];
+            wcnt[ww][d] = tcount;
+            tcount += c;
+        }
+        {   // exclusive scan of the 256 tile counts: wave scans, then the wave totals
+            uint32_t x = tcount;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) scan_s[w] = x;
+            __syncthreads();
+            uint32_t add = 0;
+#pragma unroll
+            for (int ww = 0; ww < kWaves; ++ww) add += ww < w ? scan_s[ww] : 0u;
+            lstart[d] = x - tcount + add;
+        }
+        __syncthreads();
+        // rows into LDS in digit order (stable: wave prefix + rank)
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const uint64_t idx = base + r * 64 + lane;
             if (idx < n) {
                 const uint32_t dd = rank[r] >> 16;
-                lpos[r] = lstart[dd] + wcnt[w][dd] + (rank[r] & 0xFFFFu);
-                if (lpos[r] < kTile) {
-                    buf[lpos[r]] = k[r];
-                    dig8[lpos[r]] = (uint8_t)dd;
+                const uint32_t lp = lstart[dd] + wcnt[w][dd] + (rank[r] & 0xFFFFu);
+                if (lp < kTile) {
+                    kb[lp] = k[r];
+                    vb[lp] = v[r];
+                    if constexpr (CIN || COUT) fb[lp] = f[r];
                 }
             }
         }
         __syncthreads();
         const uint32_t tn = tbase + kTile <= n ? kTile : (uint32_t)(n - tbase);
-        uint64_t gpos[kRounds];
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
-            uint32_t i = r * kThreads + threadIdx.x;
-            gpos[r] = n;
+            const uint32_t i = r * kThreads + threadIdx.x;
             if (i < tn) {
-                uint64_t key = buf[i];
-                uint32_t dd = dig8[i];
-                gpos[r] = (uint64_t)run[dd] + (i - lstart[dd]);
-                if (gpos[r] < n) kout[gpos[r]] = key;     // always true when the counts match
+                uint32_t dd;
+                if constexpr (CIN || COUT) dd = (fb[i] >> (8 * pass)) & 0xFFu;
+                else dd = dig.digit(kb[i], pass);
+                const uint64_t g = (uint64_t)run[dd] + (i - lstart[dd]);
+                if (g < n) {                                  // always true when the counts match
+                    kout[g] = kb[i];
+                    vout[g] = vb[i];
+                    if constexpr (COUT) fout[g] = fb[i];
+                }
             }
         }
         __syncthreads();
-        // payloads through the same LDS slots
-        uint32_t* vb = reinterpret_cast<uint32_t*>(buf);
-#pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            uint64_t idx = base + r * 64 + lane;
-            if (idx < n && lpos[r] < kTile) vb[lpos[r]] = vin[idx];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            uint32_t i = r * kThreads + threadIdx.x;
-            if (i < tn && gpos[r] < n) vout[gpos[r]] = vb[i];
-        }
-        run[d] += tot;
-        __syncthreads();
+        run[d] += tcount;
     }
 }
 
@@ -246,21 +299,37 @@ hipError_t radix_histogram(const uint64_t* keys, uint64_t n, const Digit& dig, i
     return launch_scan_hist(w.hist, passes, n, w.offsets, w.skip, st);
 }
 
-// One stable digit pass; needs radix_histogram's offsets for this pass and
-// w.status of at least radix_work_words(n) words.
+// One stable digit pass; needs w.status of at least radix_work_words(n)
+// words.  fin: carried digit words of kin's rows (byte `pass` is the digit)
+// instead of dig; fout: write the rows' digit words (carried, or
+// dig.bits(key)) next to kout.
 template <class Digit>
 hipError_t radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
-                      const Digit& dig, int pass, RadixWork& w, hipStream_t st) {
+                      const Digit& dig, int pass, RadixWork& w, hipStream_t st, const uint32_t* fin,
+                      uint32_t* fout) {
     if (n == 0) return hipSuccess;
     const uint64_t per = radix_tiles_per_range(n), nr = radix_ranges(n);
     uint32_t* cnt = w.status;
     uint32_t* off = w.status + nr * 256;
-    hipLaunchKernelGGL((k_upsweep<Digit>), dim3((unsigned)nr), dim3(kThreads), 0, st, kin, n, dig, pass,
-                       per * kTile, (uint32_t)nr, cnt);
-    hipLaunchKernelGGL(k_range_scan, dim3(256), dim3(kThreads), 0, st, cnt, (uint32_t)nr,
-                       (const unsigned long long*)w.offsets, pass, off);
-    hipLaunchKernelGGL((k_downsweep<Digit>), dim3((unsigned)nr), dim3(kThreads), 0, st, kin, vin, kout, vout, n,
-                       dig, pass, per, (uint32_t)nr, off);
+    uint32_t* tot = w.status + nr * 512;
+    if (fin) hipLaunchKernelGGL((k_upsweep<Digit, true>), dim3((unsigned)nr), dim3(kThreads), 0, st, kin, fin, n, dig,
+                                pass, per * kTile, (uint32_t)nr, cnt);
+    else hipLaunchKernelGGL((k_upsweep<Digit, false>), dim3((unsigned)nr), dim3(kThreads), 0, st, kin, fin, n, dig,
+                            pass, per * kTile, (uint32_t)nr, cnt);
+    hipLaunchKernelGGL(k_range_scan, dim3(256), dim3(kThreads), 0, st, cnt, (uint32_t)nr, off, tot);
+    const dim3 g((unsigned)nr), b(kThreads);
+    if (fin && fout)
+        hipLaunchKernelGGL((k_downsweep<Digit, true, true>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig, pass,
+                           per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+    else if (fin)
+        hipLaunchKernelGGL((k_downsweep<Digit, true, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
+                           pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+    else if (fout)
+        hipLaunchKernelGGL((k_downsweep<Digit, false, true>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
+                           pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+    else
+        hipLaunchKernelGGL((k_downsweep<Digit, false, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
+                           pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
     return hipGetLastError();
 }
 

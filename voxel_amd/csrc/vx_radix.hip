@@ -114,7 +114,7 @@ hipError_t launch_iota(uint32_t* idx, uint64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
-uint64_t radix_work_words(uint64_t n) { return radix_ranges(n) * 256 * 2; }
+uint64_t radix_work_words(uint64_t n) { return radix_ranges(n) * 256 * 2 + 256; }
 
 hipError_t launch_scan_hist(const unsigned long long* ghist, int passes, uint64_t n, unsigned long long* offsets,
                             uint32_t* skip, hipStream_t st) {
@@ -124,6 +124,6 @@ hipError_t launch_scan_hist(const unsigned long long* ghist, int passes, uint64_
 
 template hipError_t radix_histogram<LsdDigit>(const uint64_t*, uint64_t, const LsdDigit&, int, RadixWork&, hipStream_t);
 template hipError_t radix_pass<LsdDigit>(const uint64_t*, const uint32_t*, uint64_t*, uint32_t*, uint64_t,
-                                         const LsdDigit&, int, RadixWork&, hipStream_t);
+                                         const LsdDigit&, int, RadixWork&, hipStream_t, const uint32_t*, uint32_t*);
 
 }  // namespace vx

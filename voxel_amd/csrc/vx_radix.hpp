@@ -28,15 +28,14 @@ struct LsdDigit {
 // cshift) * 2^-cshift), clamped to lo + width.  Every step is
 // non-decreasing in k, so fb is non-decreasing in k: a stable sort by fb
 // followed by a stable sort inside each fb run sorts by key.  Partition pass
-// p uses bits [8 + 8p, 16 + 8p) of fb (the coarse bucket is fb >> 8; the low
-// 8 bits split a bucket once more inside LDS).
+// p uses bits [8p, 8p + 8) of fb.
 struct TableDigit {
     const uint2* cell;          // ncells entries {first fine bucket, fine buckets in the cell}
     uint64_t kmin, kmax;
     uint32_t ncells;
     int cshift;
     int logmode;
-    uint32_t nfine;             // total fine buckets
+    uint32_t nfine;             // total buckets
     float inv_width;            // 2^-cshift
     __host__ __device__ static __forceinline__ uint64_t grid(uint64_t d, int logmode) {
         if (!logmode) return d;
@@ -55,11 +54,11 @@ struct TableDigit {
         return fb < nfine ? fb : nfine - 1;
     }
     __device__ __forceinline__ uint32_t digit(uint64_t k, int p) const {
-        return (fine(k) >> (8 + 8 * p)) & 0xFFu;
+        return (fine(k) >> (8 * p)) & 0xFFu;
     }
     __device__ __forceinline__ uint64_t bits(uint64_t k) const { return fine(k); }
     __device__ __forceinline__ uint32_t digit_of_bits(uint64_t b, int p) const {
-        return (uint32_t)(b >> (8 + 8 * p)) & 0xFFu;
+        return (uint32_t)(b >> (8 * p)) & 0xFFu;
     }
 };
 
@@ -83,6 +82,7 @@ hipError_t radix_histogram(const uint64_t* keys, uint64_t n, const Digit& dig, i
                            hipStream_t st);
 template <class Digit>
 hipError_t radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
-                      const Digit& dig, int pass, RadixWork& w, hipStream_t st);
+                      const Digit& dig, int pass, RadixWork& w, hipStream_t st, const uint32_t* fin,
+                      uint32_t* fout);
 
 }  // namespace vx

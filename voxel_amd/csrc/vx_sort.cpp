@@ -109,7 +109,7 @@ int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uin
         bool to_out = ((np - 1 - i) % 2) == 0;
         uint64_t* kd = to_out ? kout : b.ktmp;
         uint32_t* vd = to_out ? vout : b.vtmp;
-        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, passes[i], b.w, st), "radix pass")) return -1;
+        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, passes[i], b.w, st, nullptr, nullptr), "radix pass")) return -1;
         ks = kd;
         vs = vd;
     }
