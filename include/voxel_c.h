@@ -344,6 +344,52 @@ int       voxel_gpu_sort_pairs_u64_learned(const uint64_t* keys_in, const uint32
  * 0 keeps the default. */
 int       voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, int rows_per_bucket_log2);
 
+/* ---- batched queries (no counterpart in the reference: it runs one program
+ * per Engine::Run, exec/engine.hpp:109-141) ------------------------------- */
+/* A batch engine runs up to max_queries scan descriptors (voxel_gpu_scan's,
+ * blocks/unroll/layout fields ignored) in one persistent kernel launched
+ * through a HIP graph on the engine's own stream.  Each query is cut into
+ * tiles of tile_rows rows; a query's result depends only on its rows and
+ * tile_rows (bit-identical alone or in any mix).  One launch in flight per
+ * engine; an engine is not thread-safe. */
+void*     voxel_gpu_batch_create(size_t max_queries);
+void      voxel_gpu_batch_destroy(void* batch);
+/* unroll 2/4/8, tile_rows multiple of 4096, blocks_per_cu 1..8; 0 keeps. */
+int       voxel_gpu_batch_config(void* batch, int unroll, uint32_t tile_rows, int blocks_per_cu);
+int       voxel_gpu_batch_get_config(void* batch, int* unroll, uint32_t* tile_rows, int* blocks_per_cu);
+int       voxel_gpu_batch_launch(void* batch, const voxel_gpu_scan_desc* q, size_t nq);
+int       voxel_gpu_batch_done(void* batch);            /* 1 when the launched batch finished */
+/* Waits and writes nq results; *ms (may be NULL) = device time of the batch. */
+int       voxel_gpu_batch_wait(void* batch, voxel_gpu_agg* out, float* ms);
+int       voxel_gpu_batch_run(void* batch, const voxel_gpu_scan_desc* q, size_t nq, voxel_gpu_agg* out,
+                              float* ms);
+/* Times every configuration of unroll {2,4,8} x tile_rows {16K..256K} x
+ * blocks_per_cu {1,2,4} on these queries (median of reps), checks that each
+ * gives the same per-query results (counts/ints/min/max identical, float sums
+ * within 4 ulp), and keeps the fastest. */
+int       voxel_gpu_batch_autotune(void* batch, const voxel_gpu_scan_desc* q, size_t nq, int reps,
+                                   double* best_ms, int* configs_tried);
+
+/* ---- serving: continuous batching over resident columns ----------------- */
+typedef struct voxel_gpu_serve_stats {
+    uint64_t queries, batches, bytes, max_batch_seen, admission_waits;
+    double   makespan_s, qps, gbps, mean_batch;
+    double   p50_s, p99_s, mean_s, max_s;   /* arrival -> result on the host */
+} voxel_gpu_serve_stats;
+/* inflight batch engines (1..8); a query is admitted while the scratch of the
+ * queries on the device (64 B per tile + result + descriptor) stays within
+ * memory_budget_bytes (a query larger than the budget runs alone). */
+void*     voxel_gpu_server_create(size_t max_batch, uint64_t memory_budget_bytes, int inflight);
+void      voxel_gpu_server_destroy(void* server);
+int       voxel_gpu_server_config(void* server, int unroll, uint32_t tile_rows, int blocks_per_cu);
+int       voxel_gpu_server_autotune(void* server, const voxel_gpu_scan_desc* q, size_t nq, int reps,
+                                    double* best_ms, int* configs_tried);
+/* Replays n queries arriving at arrival_s[i] seconds (non-decreasing) after
+ * the call starts.  mode 0: continuous batching; mode 1: one synchronous
+ * voxel_gpu_scan per query.  out[i], latency_s[i] (may be NULL), stats. */
+int       voxel_gpu_server_replay(void* server, const voxel_gpu_scan_desc* q, const double* arrival_s, size_t n,
+                                  int mode, voxel_gpu_agg* out, double* latency_s, voxel_gpu_serve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

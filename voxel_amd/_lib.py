@@ -57,6 +57,15 @@ class LearnedStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ServeStats(C.Structure):
+    _fields_ = [("queries", U64), ("batches", U64), ("bytes", U64), ("max_batch_seen", U64),
+                ("admission_waits", U64), ("makespan_s", F64), ("qps", F64), ("gbps", F64), ("mean_batch", F64),
+                ("p50_s", F64), ("p99_s", F64), ("mean_s", F64), ("max_s", F64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 SIGS = {
     # engine (f64)
     "voxel_engine_create_f64": (P, []),
@@ -170,6 +179,20 @@ SIGS = {
     "voxel_gpu_mlp_train": (INT, [P, P, P, SZ, INT, C.c_float, P, P]),
     "voxel_gpu_sort_pairs_u64_learned": (INT, [P, P, P, P, SZ, INT, P, P]),
     "voxel_gpu_learned_tune": (INT, [INT, INT, INT, INT]),
+    "voxel_gpu_batch_create": (P, [SZ]),
+    "voxel_gpu_batch_destroy": (None, [P]),
+    "voxel_gpu_batch_config": (INT, [P, INT, U32, INT]),
+    "voxel_gpu_batch_get_config": (INT, [P, P, P, P]),
+    "voxel_gpu_batch_launch": (INT, [P, P, SZ]),
+    "voxel_gpu_batch_done": (INT, [P]),
+    "voxel_gpu_batch_wait": (INT, [P, P, P]),
+    "voxel_gpu_batch_run": (INT, [P, P, SZ, P, P]),
+    "voxel_gpu_batch_autotune": (INT, [P, P, SZ, INT, P, P]),
+    "voxel_gpu_server_create": (P, [SZ, U64, INT]),
+    "voxel_gpu_server_destroy": (None, [P]),
+    "voxel_gpu_server_config": (INT, [P, INT, U32, INT]),
+    "voxel_gpu_server_autotune": (INT, [P, P, SZ, INT, P, P]),
+    "voxel_gpu_server_replay": (INT, [P, P, P, SZ, INT, P, P, P]),
     "voxel_gpu_scan": (INT, [C.POINTER(ScanDesc), P, C.POINTER(Agg)]),
     "voxel_gpu_scan_launch": (INT, [C.POINTER(ScanDesc), P]),
     "voxel_gpu_scan_result": (INT, [P, C.POINTER(Agg)]),

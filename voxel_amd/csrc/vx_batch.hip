@@ -1,0 +1,428 @@
+// vx_batch.hip -- many filter/aggregate queries in one persistent kernel.
+//
+// The reference runs one program per Engine::Run call (exec/engine.hpp:
+// 109-141, one column, one predicate); a batch here is a list of scan
+// descriptors (the voxel_gpu_scan descriptor: column(s), dtype, predicate(s))
+// over columns of any length.  Every query is cut into tiles of tile_rows
+// rows; the tiles of all queries form one global list.  Resident workgroups
+// take tiles from an atomic counter, reduce a tile with the scan's
+// compensated COUNT/SUM/MIN/MAX accumulator (vx_acc.hpp) and store one
+// partial per tile; the workgroup that completes a query's last tile (a
+// per-query ticket) combines that query's partials in tile order.  A query's
+// result therefore depends only on its own rows and tile_rows: it is the
+// same bits alone or in any mix of queries.
+//
+// Host side: BatchRunner keeps pinned descriptor/result buffers and a graph
+// of three nodes -- descriptor upload, kernel, result download -- whose copy
+// sizes are updated per launch; the kernel's counters reset themselves, so
+// the graph holds no memset node.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/voxel_c.h"
+#include "vx_acc.hpp"
+#include "vx_batch.hpp"
+#include "vx_device.hpp"
+#include "vx_kernels.hpp"
+#include "vx_radix.hpp"
+#include "vx_runtime.hpp"
+
+#pragma clang fp contract(off)
+
+namespace vx {
+namespace {
+
+template <typename T, int U, bool NT>
+__device__ __forceinline__ void batch_tile(const BatchQuery& q, uint64_t s, uint64_t e, uint64_t tile,
+                                           ScanOut* __restrict__ parts, ScanOut* __restrict__ results,
+                                           uint32_t qi, uint32_t* __restrict__ ticket) {
+    using V = typename TyInfo<T>::V;
+    constexpr int E = TyInfo<T>::E;
+    __shared__ Acc<T> lds[kBlock / 64];
+    __shared__ bool s_last;
+    Pred<T, false> pa; pa.load(q.pa);
+    Pred<T, false> pb; pb.load(q.pb);
+    const bool two = q.mode == 2;
+    const T* __restrict__ a = reinterpret_cast<const T*>(q.a);
+    const T* __restrict__ b = reinterpret_cast<const T*>(two ? q.b : q.a);
+    Acc<T> acc; acc.init();
+    auto one = [&](T x, T y) {
+        bool p = pa(x);
+        if (two) p = p & pb(y);
+        acc.add(p, x);
+    };
+    const bool aligned = ((reinterpret_cast<uintptr_t>(a + s) | reinterpret_cast<uintptr_t>(b + s)) & 15u) == 0;
+    const uint64_t nvec = aligned ? (e - s) / E : 0;
+    const V* __restrict__ av = reinterpret_cast<const V*>(a + s);
+    const V* __restrict__ bv = reinterpret_cast<const V*>(b + s);
+    for (uint64_t base = 0; base < nvec; base += (uint64_t)kBlock * U) {
+        V xa[U], xb[U];
+        if (base + (uint64_t)kBlock * U <= nvec) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) xa[u] = ld16<V, NT>(&av[base + u * kBlock + threadIdx.x]);
+            if (two) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) xb[u] = ld16<V, NT>(&bv[base + u * kBlock + threadIdx.x]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T* pxa = reinterpret_cast<const T*>(&xa[u]);
+                const T* pxb = reinterpret_cast<const T*>(&xb[u]);
+#pragma unroll
+                for (int k = 0; k < E; ++k) one(pxa[k], two ? pxb[k] : T(0));
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t vi = base + u * kBlock + threadIdx.x;
+                if (vi < nvec) {
+                    xa[u] = av[vi];
+                    if (two) xb[u] = bv[vi];
+                    const T* pxa = reinterpret_cast<const T*>(&xa[u]);
+                    const T* pxb = reinterpret_cast<const T*>(&xb[u]);
+#pragma unroll
+                    for (int k = 0; k < E; ++k) one(pxa[k], two ? pxb[k] : T(0));
+                }
+            }
+        }
+    }
+    for (uint64_t i = s + nvec * E + threadIdx.x; i < e; i += kBlock) one(a[i], two ? b[i] : T(0));
+
+    block_reduce(acc, lds);
+    if (threadIdx.x == 0) {
+        acc.store(&parts[tile]);
+        __threadfence();
+        const uint32_t prev = __hip_atomic_fetch_add(&ticket[qi], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = prev == q.ntiles - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        Acc<T> r; r.init();
+        for (uint32_t i = threadIdx.x; i < q.ntiles; i += kBlock) {
+            Acc<T> o; o.load(&parts[q.tile0 + i]);
+            r.merge(o);
+        }
+        __syncthreads();
+        block_reduce(r, lds);
+        if (threadIdx.x == 0) {
+            r.store(&results[qi]);
+            __hip_atomic_store(&ticket[qi], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+}
+
+template <int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchHeader* __restrict__ hdr, const BatchQuery* __restrict__ qs,
+                                                  uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t s_tile;
+    const uint32_t total = hdr->total_tiles, nq = hdr->nq, tile_rows = hdr->tile_rows;
+    ScanOut* parts = hdr->parts;
+    ScanOut* results = hdr->results;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        __syncthreads();
+        if (tile >= total) break;                               // block-uniform
+        uint32_t lo = 0, hi = nq;                                // last query with tile0 <= tile
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (qs[mid].tile0 <= tile) lo = mid; else hi = mid;
+        }
+        const BatchQuery q = qs[lo];
+        const uint64_t s = (uint64_t)(tile - q.tile0) * tile_rows;
+        const uint64_t e = s + tile_rows < q.n ? s + tile_rows : q.n;
+        switch (q.dtype) {
+            case DT_F64: batch_tile<double, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
+            case DT_F32: batch_tile<float, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
+            case DT_I64: batch_tile<int64_t, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
+            default: batch_tile<int32_t, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
+        }
+    }
+    // the last workgroup to leave resets the tile counter for the next launch
+    if (threadIdx.x == 0) {
+        const uint32_t prev = __hip_atomic_fetch_add(&ctr[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gridDim.x - 1) {
+            __hip_atomic_store(&ctr[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctr[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+void* batch_kernel(int unroll) {
+    switch (unroll) {
+        case 2: return reinterpret_cast<void*>(&k_batch<2, true>);
+        case 8: return reinterpret_cast<void*>(&k_batch<8, true>);
+        default: return reinterpret_cast<void*>(&k_batch<4, true>);
+    }
+}
+
+}  // namespace
+
+BatchRunner::BatchRunner(size_t max_queries) : max_q_(max_queries ? max_queries : 1) {
+    if (!runtime_ok()) return;
+    const size_t qbytes = sizeof(BatchHeader) + max_q_ * sizeof(BatchQuery);
+    ok_ = hip_ok(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate") &&
+          hip_ok(hipHostMalloc((void**)&hq_, qbytes, hipHostMallocDefault), "hipHostMalloc") &&
+          hip_ok(hipMalloc((void**)&dq_, qbytes), "hipMalloc") &&
+          hip_ok(hipHostMalloc((void**)&hres_, max_q_ * sizeof(ScanOut), hipHostMallocDefault), "hipHostMalloc") &&
+          hip_ok(hipMalloc((void**)&dres_, max_q_ * sizeof(ScanOut)), "hipMalloc") &&
+          hip_ok(hipMalloc((void**)&ctr_, (max_q_ + 2) * 4), "hipMalloc") &&
+          hip_ok(hipMemset(ctr_, 0, (max_q_ + 2) * 4), "hipMemset") &&
+          hip_ok(hipEventCreate(&ev0_), "hipEventCreate") && hip_ok(hipEventCreate(&ev1_), "hipEventCreate") &&
+          ensure_parts(4096);
+    dtypes_.assign(max_q_, 0);
+}
+
+BatchRunner::~BatchRunner() {
+    if (inflight_) (void)hipStreamSynchronize(st_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (hq_) (void)hipHostFree(hq_);
+    if (hres_) (void)hipHostFree(hres_);
+    if (dq_) (void)hipFree(dq_);
+    if (dres_) (void)hipFree(dres_);
+    if (parts_) (void)hipFree(parts_);
+    if (ctr_) (void)hipFree(ctr_);
+    if (st_) (void)hipStreamDestroy(st_);
+}
+
+bool BatchRunner::set_config(const BatchConfig& c) {
+    if (c.unroll != 2 && c.unroll != 4 && c.unroll != 8) { set_error("batch: unroll must be 2, 4 or 8"); return false; }
+    if (c.tile_rows < 4096 || c.tile_rows % 4096 != 0) { set_error("batch: tile_rows must be a multiple of 4096"); return false; }
+    if (c.blocks_per_cu < 1 || c.blocks_per_cu > 8) { set_error("batch: blocks_per_cu must be 1..8"); return false; }
+    if (inflight_) { set_error("batch: a launch is in flight"); return false; }
+    if (c.unroll != cfg_.unroll || c.blocks_per_cu != cfg_.blocks_per_cu) graph_valid_ = false;
+    cfg_ = c;
+    return true;
+}
+
+uint64_t BatchRunner::tiles_of(uint64_t n) const {
+    uint64_t t = (n + cfg_.tile_rows - 1) / cfg_.tile_rows;
+    return t ? t : 1;
+}
+
+bool BatchRunner::ensure_parts(uint64_t tiles) {
+    if (tiles <= parts_cap_) return true;
+    uint64_t cap = std::max<uint64_t>(tiles, parts_cap_ * 2);
+    if (parts_) (void)hipFree(parts_);
+    parts_ = nullptr;
+    parts_cap_ = 0;
+    if (!hip_ok(hipMalloc((void**)&parts_, cap * sizeof(ScanOut)), "hipMalloc(batch partials)")) return false;
+    parts_cap_ = cap;
+    return true;
+}
+
+bool BatchRunner::build_graph() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    if (!hip_ok(hipGraphCreate(&graph_, 0), "hipGraphCreate")) return false;
+    const size_t qbytes = sizeof(BatchHeader) + max_q_ * sizeof(BatchQuery);
+    if (!hip_ok(hipGraphAddMemcpyNode1D(&n_up_, graph_, nullptr, 0, dq_, hq_, qbytes, hipMemcpyHostToDevice),
+                "graph upload node"))
+        return false;
+    const BatchHeader* hdr = reinterpret_cast<const BatchHeader*>(dq_);
+    const BatchQuery* qs = reinterpret_cast<const BatchQuery*>(dq_ + sizeof(BatchHeader));
+    uint32_t* ctr = ctr_;
+    void* args[] = {(void*)&hdr, (void*)&qs, (void*)&ctr};
+    hipKernelNodeParams kp = {};
+    kp.func = batch_kernel(cfg_.unroll);
+    kp.gridDim = dim3((unsigned)(device_cu_count() * cfg_.blocks_per_cu));
+    kp.blockDim = dim3(kBlock);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    if (!hip_ok(hipGraphAddKernelNode(&n_kern_, graph_, &n_up_, 1, &kp), "graph kernel node")) return false;
+    if (!hip_ok(hipGraphAddMemcpyNode1D(&n_down_, graph_, &n_kern_, 1, hres_, dres_, max_q_ * sizeof(ScanOut),
+                                        hipMemcpyDeviceToHost), "graph download node"))
+        return false;
+    if (!hip_ok(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate")) return false;
+    graph_valid_ = true;
+    return true;
+}
+
+bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
+    if (!ok_) { set_error("batch: not initialised"); return false; }
+    if (inflight_) { set_error("batch: a launch is in flight"); return false; }
+    if (nq == 0 || nq > max_q_) { set_error("batch: %zu queries (1..%zu allowed)", nq, max_q_); return false; }
+    BatchHeader* h = reinterpret_cast<BatchHeader*>(hq_);
+    BatchQuery* qs = reinterpret_cast<BatchQuery*>(hq_ + sizeof(BatchHeader));
+    uint64_t tiles = 0;
+    for (size_t i = 0; i < nq; ++i) {
+        ScanArgs g;
+        if (!make_scan_args(&q[i], g)) { set_error("batch query %zu: %s", i, last_error()); return false; }
+        BatchQuery& d = qs[i];
+        std::memset(&d, 0, sizeof(d));
+        d.a = g.a;
+        d.b = g.mode == 2 ? g.b : g.a;
+        d.n = q[i].n;
+        d.pa = g.pa;
+        d.pb = g.mode == 2 ? g.pb : g.pa;
+        d.dtype = g.dtype;
+        d.mode = g.mode;
+        d.tile0 = tiles;
+        d.ntiles = (uint32_t)tiles_of(d.n);
+        tiles += d.ntiles;
+        dtypes_[i] = g.dtype;
+    }
+    if (tiles >= (1ull << 32)) { set_error("batch: too many tiles"); return false; }
+    if (!ensure_parts(tiles)) return false;
+    std::memset(h, 0, sizeof(*h));
+    h->nq = (uint32_t)nq;
+    h->total_tiles = (uint32_t)tiles;
+    h->tile_rows = cfg_.tile_rows;
+    h->parts = parts_;
+    h->results = dres_;
+    if (!graph_valid_ && !build_graph()) return false;
+    const size_t up = sizeof(BatchHeader) + nq * sizeof(BatchQuery);
+    if (!hip_ok(hipGraphExecMemcpyNodeSetParams1D(exec_, n_up_, dq_, hq_, up, hipMemcpyHostToDevice), "graph upload size") ||
+        !hip_ok(hipGraphExecMemcpyNodeSetParams1D(exec_, n_down_, hres_, dres_, nq * sizeof(ScanOut), hipMemcpyDeviceToHost),
+                "graph download size"))
+        return false;
+    if (!hip_ok(hipEventRecord(ev0_, st_), "hipEventRecord") || !hip_ok(hipGraphLaunch(exec_, st_), "hipGraphLaunch") ||
+        !hip_ok(hipEventRecord(ev1_, st_), "hipEventRecord"))
+        return false;
+    nq_ = nq;
+    inflight_ = true;
+    return true;
+}
+
+bool BatchRunner::done() {
+    if (!inflight_) return true;
+    return hipEventQuery(ev1_) == hipSuccess;
+}
+
+bool BatchRunner::wait(voxel_gpu_agg* out) {
+    if (!inflight_) { set_error("batch: nothing launched"); return false; }
+    inflight_ = false;
+    if (!hip_ok(hipEventSynchronize(ev1_), "batch wait")) return false;
+    for (size_t i = 0; i < nq_; ++i) fill_agg(dtypes_[i], hres_[i], &out[i]);
+    return true;
+}
+
+float BatchRunner::last_ms() {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess) return -1.0f;
+    return ms;
+}
+
+}  // namespace vx
+
+using namespace vx;
+
+extern "C" {
+
+void* voxel_gpu_batch_create(size_t max_queries) {
+    set_error("%s", "");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    auto* r = new BatchRunner(max_queries);
+    if (!r->ok()) { delete r; return nullptr; }
+    return r;
+}
+
+void voxel_gpu_batch_destroy(void* h) { delete static_cast<BatchRunner*>(h); }
+
+int voxel_gpu_batch_config(void* h, int unroll, uint32_t tile_rows, int blocks_per_cu) {
+    set_error("%s", "");
+    BatchRunner* r = static_cast<BatchRunner*>(h);
+    BatchConfig c = r->config();
+    if (unroll) c.unroll = unroll;
+    if (tile_rows) c.tile_rows = tile_rows;
+    if (blocks_per_cu) c.blocks_per_cu = blocks_per_cu;
+    return r->set_config(c) ? 0 : -1;
+}
+
+int voxel_gpu_batch_get_config(void* h, int* unroll, uint32_t* tile_rows, int* blocks_per_cu) {
+    const BatchConfig& c = static_cast<BatchRunner*>(h)->config();
+    if (unroll) *unroll = c.unroll;
+    if (tile_rows) *tile_rows = c.tile_rows;
+    if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
+    return 0;
+}
+
+int voxel_gpu_batch_launch(void* h, const voxel_gpu_scan_desc* q, size_t nq) {
+    set_error("%s", "");
+    return static_cast<BatchRunner*>(h)->launch(q, nq) ? 0 : -1;
+}
+
+int voxel_gpu_batch_done(void* h) { return static_cast<BatchRunner*>(h)->done() ? 1 : 0; }
+
+int voxel_gpu_batch_wait(void* h, voxel_gpu_agg* out, float* ms) {
+    set_error("%s", "");
+    BatchRunner* r = static_cast<BatchRunner*>(h);
+    if (!r->wait(out)) return -1;
+    if (ms) *ms = r->last_ms();
+    return 0;
+}
+
+int voxel_gpu_batch_run(void* h, const voxel_gpu_scan_desc* q, size_t nq, voxel_gpu_agg* out, float* ms) {
+    if (voxel_gpu_batch_launch(h, q, nq) != 0) return -1;
+    return voxel_gpu_batch_wait(h, out, ms);
+}
+
+// Autotuner: times every (unroll, tile_rows, blocks_per_cu) of the search
+// space on the given queries (median of reps launches each) and keeps the
+// fastest.  Per-query results of every configuration are checked against
+// the first one: counts, integer sums and min/max identical, float sums
+// within 4 ulp of the compensated value (only the tile split differs).
+int voxel_gpu_batch_autotune(void* h, const voxel_gpu_scan_desc* q, size_t nq, int reps, double* best_ms,
+                             int* configs_tried) {
+    set_error("%s", "");
+    BatchRunner* r = static_cast<BatchRunner*>(h);
+    if (reps < 1) reps = 3;
+    const int unrolls[] = {2, 4, 8};
+    const uint32_t tiles[] = {16384, 32768, 65536, 131072, 262144};
+    const int bpcs[] = {1, 2, 4};
+    std::vector<voxel_gpu_agg> ref(nq), cur(nq);
+    BatchConfig best = r->config();
+    double best_t = 1e300;
+    int tried = 0;
+    bool have_ref = false;
+    for (int u : unrolls)
+        for (uint32_t t : tiles)
+            for (int b : bpcs) {
+                BatchConfig c{u, t, b};
+                if (!r->set_config(c)) return -1;
+                std::vector<float> ms;
+                for (int k = 0; k < reps + 1; ++k) {
+                    if (!r->launch(q, nq) || !r->wait(cur.data())) return -1;
+                    if (k > 0) ms.push_back(r->last_ms());
+                }
+                ++tried;
+                if (!have_ref) { ref = cur; have_ref = true; }
+                for (size_t i = 0; i < nq; ++i) {
+                    const voxel_gpu_agg& x = cur[i];
+                    const voxel_gpu_agg& y = ref[i];
+                    bool same = x.count == y.count && x.isum == y.isum && x.imin == y.imin && x.imax == y.imax &&
+                                x.has_minmax == y.has_minmax &&
+                                (!x.has_minmax || (x.fmin == y.fmin && x.fmax == y.fmax) ||
+                                 (std::isnan(x.fmin) && std::isnan(y.fmin)));
+                    const double tol = 4.0 * std::fabs(std::nextafter(y.sum, INFINITY) - y.sum) + 1e-300;
+                    same = same && (x.sum == y.sum || std::fabs(x.sum - y.sum) <= tol);
+                    if (!same) {
+                        set_error("autotune: query %zu differs under unroll %d tile %u blocks/CU %d", i, u, t, b);
+                        return -1;
+                    }
+                }
+                std::sort(ms.begin(), ms.end());
+                const double med = ms[ms.size() / 2];
+                if (med < best_t) { best_t = med; best = c; }
+            }
+    if (!r->set_config(best)) return -1;
+    if (best_ms) *best_ms = best_t;
+    if (configs_tried) *configs_tried = tried;
+    return 0;
+}
+
+}  // extern "C"

@@ -228,6 +228,11 @@ void engine_load(void* h, const uint32_t* code, size_t len) {
     e.plan = lower_engine_program(e.code.data(), e.code.size());
 }
 
+}  // namespace
+
+// Result conversion and descriptor checks shared with the batch and serving
+// paths (declared in vx_runtime.hpp).
+namespace vx {
 void fill_agg(int dtype, const ScanOut& r, voxel_gpu_agg* out) {
     std::memset(out, 0, sizeof(*out));
     out->count = r.cnt;
@@ -273,7 +278,8 @@ bool make_scan_args(const voxel_gpu_scan_desc* d, ScanArgs& g) {
     return true;
 }
 
-}  // namespace
+
+}  // namespace vx
 
 extern "C" {
 

@@ -7,6 +7,7 @@
 
 #include <mutex>
 
+#include "../../include/voxel_c.h"
 #include "vx_kernels.hpp"
 
 namespace vx {
@@ -56,6 +57,11 @@ PredDev make_never_pred(int dtype);
 // Combines a compensated pair with an initial value and rounds to T.
 double finish_sum_f64(double acc0, double s, double c);
 float finish_sum_f32(float acc0, float s, float c);
+
+// Scan result (device form) -> voxel_gpu_agg for a column dtype.
+void fill_agg(int dtype, const ScanOut& r, voxel_gpu_agg* out);
+// Checks a scan descriptor and lowers it; false with the error set.
+bool make_scan_args(const voxel_gpu_scan_desc* d, ScanArgs& g);
 
 // Device buffer that mirrors a borrowed host column.
 struct DevColumn {

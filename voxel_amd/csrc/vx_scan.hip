@@ -16,6 +16,7 @@
 //     deterministic final reduction by the last workgroup to finish (ticket),
 //     so one launch produces the result and the order of the final combine
 //     depends only on the grid size.
+#include "vx_acc.hpp"
 #include "vx_device.hpp"
 #include "vx_kernels.hpp"
 
@@ -28,97 +29,6 @@
 
 namespace vx {
 namespace {
-
-constexpr int kBlock = 256;
-
-template <typename T>
-struct Acc {
-    using S = typename TyInfo<T>::S;
-    S s, c;
-    uint64_t n;
-    T mn, mx;
-    __device__ __forceinline__ void init() { s = S(0); c = S(0); n = 0; mn = Lim<T>::hi(); mx = Lim<T>::lo(); }
-    __device__ __forceinline__ void add(bool p, T x) {
-        n += p ? 1u : 0u;
-        if constexpr (TyInfo<T>::F) {
-            two_sum<S>(s, c, p ? x : T(0));
-            mn = p ? fmin(mn, x) : mn;
-            mx = p ? fmax(mx, x) : mx;
-        } else {
-            s += p ? (uint64_t)(int64_t)x : 0ull;
-            mn = (p && x < mn) ? x : mn;
-            mx = (p && x > mx) ? x : mx;
-        }
-    }
-    __device__ __forceinline__ void merge(const Acc& o) {
-        n += o.n;
-        if constexpr (TyInfo<T>::F) {
-            S cc = c + o.c;
-            two_sum<S>(s, cc, o.s);
-            c = cc;
-            mn = fmin(mn, o.mn);
-            mx = fmax(mx, o.mx);
-        } else {
-            s += o.s;
-            mn = o.mn < mn ? o.mn : mn;
-            mx = o.mx > mx ? o.mx : mx;
-        }
-    }
-    __device__ __forceinline__ void shfl_merge(int mask) {
-        Acc o;
-        o.s = __shfl_xor(s, mask, 64);
-        o.c = __shfl_xor(c, mask, 64);
-        o.n = __shfl_xor(n, mask, 64);
-        o.mn = __shfl_xor(mn, mask, 64);
-        o.mx = __shfl_xor(mx, mask, 64);
-        merge(o);
-    }
-    __device__ __forceinline__ void store(ScanOut* p) const {
-        p->s = to_bits(s); p->c = to_bits(c); p->cnt = n; p->mn = to_bits(mn); p->mx = to_bits(mx);
-    }
-    __device__ __forceinline__ void load(const ScanOut* p) {
-        s = from_bits<S>(p->s); c = from_bits<S>(p->c); n = p->cnt; mn = from_bits<T>(p->mn); mx = from_bits<T>(p->mx);
-    }
-};
-// Workgroup reduction of an Acc into thread 0 (deterministic order).
-template <typename T>
-__device__ __forceinline__ void block_reduce(Acc<T>& a, Acc<T>* lds) {
-    for (int m = 32; m > 0; m >>= 1) a.shfl_merge(m);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) lds[wid] = a;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kBlock / 64; ++w) a.merge(lds[w]);
-    }
-}
-
-// Publishes this workgroup's partial and returns true in the workgroup that
-// finishes last; that workgroup then sees every other partial.
-__device__ __forceinline__ bool publish_and_check_last(uint32_t* ticket, bool* s_last) {
-    if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *s_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    bool last = *s_last;
-    if (last) __threadfence();
-    return last;
-}
-
-// 16-byte vector load; NT marks it non-temporal (streamed, not kept in cache).
-template <typename V, bool NT>
-__device__ __forceinline__ V ld16(const V* p) {
-    if constexpr (NT && sizeof(V) == 16) {
-        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-        u4 r = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
-        V v;
-        __builtin_memcpy(&v, &r, 16);
-        return v;
-    } else {
-        return *p;
-    }
-}
 
 template <typename T, int MODE, bool TA, bool TB, bool VEC, int U, bool NT, bool CONTIG>
 __global__ __launch_bounds__(kBlock)
