@@ -1,13 +1,26 @@
-"""Benchmark of the fused scan-aggregate hot path on one MI355X.
+"""Benchmark of the VoxelVM hot path on one MI355X.
 
-Workload (one step = one pass of the hot path over one batch): two 2^28-row
+Headline (one step = one pass of the hot path over one batch): two 2^28-row
 f64 columns a, b resident in HBM, uniform in [0, 1000) (SplitMix64 rows,
 seeds 1 and 2), compound predicate a > t1 AND b < t2 at 50 % selectivity
 (t1 = 292.8932188, t2 = 707.1067812), returning COUNT, SUM(a), MIN(a), MAX(a)
-in one fused launch (voxel_gpu_scan_launch).  A breakdown over f64/f32/i64/i32
-and 1/50/99 % selectivity is printed in the same JSON line.
+in one fused launch (voxel_gpu_scan_launch).  The same JSON line carries
+* breakdown: f64/f32/i64/i32 x 1/50/99 % selectivity;
+* sort: 2^29 u64 keys + u32 row-index payload, plain LSD radix vs the
+  learned-CDF sort, uniform / Zipf(1.1, 2^32) / lognormal(20, 2) /
+  95 %-sorted keys, Gkeys/s and output equality;
+* join: 2^24 build keys (i64 payload), 2^28 probes (90 % drawn Zipf(1.0)
+  over the build keys, 10 % absent), match count + SUM(payload), Gprobes/s;
+* groupby: SUM/AVG(fare) GROUP BY zone over 3e8 rows (i32 zone 1..265,
+  f64 fare lognormal(2.5, 0.6));
+* serve: 4096 filter+aggregate queries over 64 resident f64/i64 columns
+  (lengths lognormal, 1K..64M rows) batched vs one launch per query, at
+  saturation and as a 20k queries/s Poisson arrival stream.
+Zipf keys use the continuous inverse CDF of the power law (rank =
+floor((1 - u(1 - N^(1-s)))^(1/(1-s)))), lognormal keys floor(exp(20 + 2Z)).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-breakdown]
+                    [--no-sort] [--no-join] [--no-groupby] [--no-serve]
 
 value = % of the 8 TB/s HBM peak achieved by the f64 compound scan, from the
 whole timed region (K launches bracketed by device synchronisation);
@@ -51,6 +64,11 @@ def parse():
     p.add_argument("--rows-log2", type=int, default=28)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-breakdown", action="store_true")
+    p.add_argument("--no-sort", action="store_true")
+    p.add_argument("--no-join", action="store_true")
+    p.add_argument("--no-groupby", action="store_true")
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--sort-log2", type=int, default=29)
     return p.parse_args()
 
 
@@ -73,6 +91,224 @@ def time_scan(desc, steps, warmup, stream_handle):
     return wall, kernel_ms
 
 
+def _sync_time(fn, reps=3, warmup=1):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def sort_keys(dist: str, n: int, seed: int):
+    """Synthetic u64 keys (as int64 tensors) of the four sort workloads."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    if dist == "uniform":
+        return torch.randint(-2**63, 2**63 - 1, (n,), dtype=torch.int64, device="cuda", generator=g)
+    if dist == "zipf":
+        s_, N = 1.1, float(2**32)
+        u = torch.rand(n, dtype=torch.float64, device="cuda", generator=g)
+        r = torch.pow(1.0 - u * (1.0 - N ** (1.0 - s_)), 1.0 / (1.0 - s_))
+        return torch.clamp(torch.floor(r) - 1.0, 0.0, N - 1.0).to(torch.int64)
+    if dist == "lognormal":
+        z = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+        return torch.clamp(torch.floor(torch.exp(20.0 + 2.0 * z)), 0.0, 2.0**62).to(torch.int64)
+    if dist == "sorted95":
+        k = torch.randint(0, 2**62, (n,), dtype=torch.int64, device="cuda", generator=g)
+        k = torch.sort(k).values
+        m = int(0.05 * n / 2)
+        i = torch.randint(0, n, (m,), device="cuda", generator=g)
+        j = torch.randint(0, n, (m,), device="cuda", generator=g)
+        ki, kj = k[i].clone(), k[j].clone()
+        k[i] = kj
+        k[j] = ki
+        return k
+    raise ValueError(dist)
+
+
+def bench_sort(log2n: int) -> dict:
+    """Plain LSD radix vs learned-CDF sort of 2^log2n (u64 key, u32 row index)."""
+    import ctypes as C
+    import torch
+    from voxel_amd._lib import LearnedStats, lib
+    n = 1 << log2n
+    out = {"n": n, "payload": "u32 row index", "algorithmic_bytes": 2 * n * 12,
+           "note": "Gkeys/s = n / wall time of one sort call (inputs resident); bytes = one read + one write "
+                   "of key+payload"}
+    ko_l = torch.empty(n, dtype=torch.int64, device="cuda")
+    vo_l = torch.empty(n, dtype=torch.int32, device="cuda")
+    ko_r = torch.empty(n, dtype=torch.int64, device="cuda")
+    vo_r = torch.empty(n, dtype=torch.int32, device="cuda")
+    speedups = []
+    for dist in ("uniform", "zipf", "lognormal", "sorted95"):
+        k = sort_keys(dist, n, 7)
+        torch.cuda.synchronize()
+        st = LearnedStats()
+
+        def lsd():
+            if lib.voxel_gpu_sort_pairs_u64(k.data_ptr(), None, ko_r.data_ptr(), vo_r.data_ptr(), n, None) != 0:
+                raise RuntimeError("lsd sort failed")
+
+        def learned():
+            if lib.voxel_gpu_sort_pairs_u64_learned(k.data_ptr(), None, ko_l.data_ptr(), vo_l.data_ptr(), n, 0,
+                                                    C.byref(st), None) != 0:
+                raise RuntimeError("learned sort failed")
+
+        t_lsd = _sync_time(lsd)
+        t_learned = _sync_time(learned)
+        same = bool(torch.equal(ko_l, ko_r) and torch.equal(vo_l, vo_r))
+        d = st.as_dict()
+        speedups.append(t_lsd / t_learned)
+        out[dist] = {"lsd_gkeys_per_s": round(n / t_lsd / 1e9, 3), "learned_gkeys_per_s": round(n / t_learned / 1e9, 3),
+                     "lsd_ms": round(t_lsd * 1e3, 3), "learned_ms": round(t_learned * 1e3, 3),
+                     "speedup": round(t_lsd / t_learned, 3), "bit_exact": same,
+                     "learned_achieved_gb_per_s": round(2 * n * 12 / t_learned / 1e9, 1),
+                     "partition_passes": d["partition_passes"], "buckets": d["buckets"], "fell_back": d["fell_back"],
+                     "big_rows": d["big_rows"], "train_ms": round(d["train_ms"], 3),
+                     "final_logit_mse": round(d["final_loss"], 5)}
+        del k
+        torch.cuda.empty_cache()
+    out["speedup_geomean"] = round(math.exp(sum(math.log(x) for x in speedups) / len(speedups)), 3)
+    del ko_l, vo_l, ko_r, vo_r
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_join() -> dict:
+    """Build 2^24 distinct i64 keys + i64 payloads; probe 2^28 keys (90 % hits
+    drawn Zipf(1.0) over the build rows through a fixed permutation, 10 %
+    absent: top bit set, build keys have it clear)."""
+    import ctypes as C
+    import torch
+    from voxel_amd._lib import lib
+    nb, npr = 1 << 24, 1 << 28
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    while True:
+        bk = torch.randint(0, 2**63 - 1, (nb,), dtype=torch.int64, device="cuda", generator=g)
+        if torch.unique(bk).numel() == nb:
+            break
+    pay = torch.randint(-2**63, 2**63 - 1, (nb,), dtype=torch.int64, device="cuda", generator=g)
+    perm = torch.randperm(nb, device="cuda", generator=g)
+    u = torch.rand(npr, dtype=torch.float64, device="cuda", generator=g)
+    rank = torch.clamp(torch.floor(torch.pow(float(nb), u)), 1, nb).to(torch.int64)      # Zipf(1.0) ranks
+    row = perm[rank - 1]
+    hit = torch.rand(npr, device="cuda", generator=g) < 0.9
+    miss = torch.randint(0, 2**63 - 1, (npr,), dtype=torch.int64, device="cuda", generator=g) | (-2**63)
+    probe = torch.where(hit, bk[row], miss)
+    exp_count = int(hit.sum().item())
+    exp_sum = int(torch.where(hit, pay[row], torch.zeros_like(pay[row])).sum().item())
+    del u, rank, miss
+    torch.cuda.synchronize()
+    holder = {}
+
+    def build():
+        if "j" in holder:
+            lib.voxel_gpu_join_destroy(holder.pop("j"))
+        holder["j"] = lib.voxel_gpu_join_build(bk.data_ptr(), pay.data_ptr(), nb, None)
+        if not holder["j"]:
+            raise RuntimeError("join build failed")
+
+    t_build = _sync_time(build)
+    cnt, sm = C.c_uint64(), C.c_int64()
+
+    def probe_fn():
+        if lib.voxel_gpu_join_probe_sum(holder["j"], probe.data_ptr(), npr, C.byref(cnt), C.byref(sm), None) != 0:
+            raise RuntimeError("join probe failed")
+
+    t_probe = _sync_time(probe_fn)
+    lib.voxel_gpu_join_destroy(holder.pop("j"))
+    ok = cnt.value == exp_count and sm.value == exp_sum
+    out = {"build_rows": nb, "probe_rows": npr, "gprobes_per_s": round(npr / t_probe / 1e9, 3),
+           "probe_ms": round(t_probe * 1e3, 3), "build_ms": round(t_build * 1e3, 3),
+           "build_mrows_per_s": round(nb / t_build / 1e6, 1),
+           "probe_algorithmic_gb_per_s": round(npr * 24 / t_probe / 1e9, 1),
+           "match_count": cnt.value, "payload_sum": sm.value, "matches_expected": ok}
+    del bk, pay, perm, row, hit, probe
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_groupby() -> dict:
+    import torch
+    from voxel_amd._lib import lib
+    out = {}
+    for rows in (3_000_000, 300_000_000):
+        zone = torch.empty(rows, dtype=torch.int32, device="cuda")
+        assert lib.voxel_gpu_fill_uniform(3, zone.data_ptr(), rows, 21, 1.0, 266.0, None) == 0
+        g = torch.Generator(device="cuda")
+        g.manual_seed(22)
+        fare = torch.exp(2.5 + 0.6 * torch.randn(rows, dtype=torch.float64, device="cuda", generator=g))
+        G = 512
+        res = [torch.zeros(G, dtype=torch.int64 if i < 2 else torch.float64, device="cuda") for i in range(6)]
+        import numpy as np
+        ng = np.zeros(1, np.uint64)
+
+        def run():
+            if lib.voxel_gpu_groupby_f64(3, zone.data_ptr(), fare.data_ptr(), rows, G, *(t.data_ptr() for t in res),
+                                         ng.ctypes.data, None) != 0:
+                raise RuntimeError("groupby failed")
+
+        t = _sync_time(run)
+        out["rows_%d" % rows] = {"ms": round(t * 1e3, 3), "groups": int(ng[0]),
+                                 "gb_per_s": round(rows * 12 / t / 1e9, 1),
+                                 "count_total_ok": int(res[1][: int(ng[0])].sum().item()) == rows}
+        del zone, fare, res
+        torch.cuda.empty_cache()
+    return out
+
+
+def bench_serve() -> dict:
+    import numpy as np
+    import torch
+    from voxel_amd import scan
+    from voxel_amd._lib import lib
+    from voxel_amd.serve import Server
+    rng = np.random.default_rng(2024)
+    ncol, nq = 64, 4096
+    lengths = np.clip(np.floor(np.exp(rng.normal(math.log(2**20), 2, ncol))), 1000, 64 * 2**20).astype(np.int64)
+    cols = []
+    for i, n in enumerate(lengths):
+        f = i < 32
+        c = torch.empty(int(n), dtype=torch.float64 if f else torch.int64, device="cuda")
+        assert lib.voxel_gpu_fill_uniform(0 if f else 2, c.data_ptr(), int(n), 100 + i, 0.0,
+                                          1000.0 if f else float(2**31), None) == 0
+        cols.append(c)
+    torch.cuda.synchronize()
+    pick = rng.integers(0, ncol, nq)
+    sel = rng.uniform(0.01, 0.99, nq)
+    qs = [scan.make_desc(cols[int(i)], "lt", float(s * 1000.0) if int(i) < 32 else int(s * 2**31))
+          for i, s in zip(pick, sel)]
+    arrivals = np.cumsum(rng.exponential(1 / 20000.0, nq))
+    srv = Server(max_batch=nq, memory_budget=256 << 20, inflight=2)
+    tune = srv.autotune(qs[:256], reps=2)
+    res = {}
+    for mode, name in ((0, "batched"), (1, "unbatched")):
+        for load, arr in (("saturated", np.zeros(nq)), ("poisson_20k_qps", arrivals)):
+            out, lat, st = srv.replay_raw(qs, arr, mode=mode)
+            res["%s_%s" % (name, load)] = {"queries_per_s": round(st["qps"], 1), "gb_per_s": round(st["gbps"], 1),
+                                          "p50_us": round(st["p50_s"] * 1e6, 1), "p99_us": round(st["p99_s"] * 1e6, 1),
+                                          "makespan_ms": round(st["makespan_s"] * 1e3, 3), "batches": st["batches"],
+                                          "mean_batch": round(st["mean_batch"], 2)}
+    res["throughput_ratio_saturated"] = round(res["batched_saturated"]["queries_per_s"] /
+                                              res["unbatched_saturated"]["queries_per_s"], 3)
+    res["autotune"] = tune
+    res["workload"] = ("4096 queries (x < t, selectivity U[1%%,99%%]) over 64 resident columns (32 f64, 32 i64), "
+                       "lengths floor(exp(N(ln 2^20, 2))) clipped to [1e3, 2^26], %.1f GB scanned" %
+                       (sum(int(lengths[int(i)]) * 8 for i in pick) / 1e9))
+    srv.close()
+    del cols, qs
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(n_rows_log2: int = 27):
     """The reference's fastest CPU path for filter+sum: its x86-64 JIT fused
     loop (codegen/x86_64.hpp:1632-1716), built from the reference sources into
@@ -89,19 +325,22 @@ def cpu_baseline(n_rows_log2: int = 27):
     if not fn:
         return None
     ts = []
+    got = 0.0
     for i in range(5):
         t0 = time.perf_counter()
-        o.R.vr_jit_entry(fn, o.ptr(x), n, n, 500.0)
+        got = o.R.vr_jit_entry(fn, o.ptr(x), n, n, 500.0)
         ts.append(time.perf_counter() - t0)
     o.R.vr_jit_release(fn)
     med = statistics.median(ts[2:])
+    want = float(x[x > 500.0].sum())
     gbs = n * 8 / med / 1e9
     return {"value": round(100.0 * gbs / HBM_PEAK_GBS, 4), "unit": "% of 8 TB/s HBM peak",
             "gb_per_s": round(gbs, 3), "rows_per_s": round(n / med, 1), "cores": 1, "kind": "reference",
             "sample": "reference JIT fused filter+sum (oracle/_ref vr_jit_entry = voxel_jit_run loop), "
                       "2^%d f64 rows uniform [0,1000), x > 500, single column (8 B/row), "
                       "median of 3 after 2 warmups, one core" % n_rows_log2,
-            "host_cpu": _cpu_model(), "nproc": os.cpu_count()}
+            "host_cpu": _cpu_model(), "nproc": os.cpu_count(),
+            "result_check": {"jit_sum": got, "numpy_sum": want, "rel_diff": abs(got - want) / abs(want)}}
 
 
 def _cpu_model():
@@ -206,6 +445,14 @@ def main():
     }
     del cols
     torch.cuda.empty_cache()
+    if not args.no_sort:
+        out["sort"] = bench_sort(args.sort_log2)
+    if not args.no_join:
+        out["join"] = bench_join()
+    if not args.no_groupby:
+        out["groupby"] = bench_groupby()
+    if not args.no_serve:
+        out["serve"] = bench_serve()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
