@@ -69,6 +69,9 @@ def parse():
     p.add_argument("--no-groupby", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--sort-log2", type=int, default=29)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a device-to-device copy of one 2^rows-log2 f64 column (the known-size "
+                        "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes)")
     return p.parse_args()
 
 
@@ -171,8 +174,9 @@ def bench_sort(log2n: int) -> dict:
                      "speedup": round(t_lsd / t_learned, 3), "bit_exact": same,
                      "learned_achieved_gb_per_s": round(2 * n * 12 / t_learned / 1e9, 1),
                      "partition_passes": d["partition_passes"], "buckets": d["buckets"], "fell_back": d["fell_back"],
-                     "big_rows": d["big_rows"], "train_ms": round(d["train_ms"], 3),
-                     "final_logit_mse": round(d["final_loss"], 5)}
+                     "big_rows": d["big_rows"], "final_logit_mse": round(d["final_loss"], 5),
+                     "phase_ms": {k: round(d[k + "_ms"], 3) for k in ("sample", "train", "table", "partition",
+                                                                       "bucket", "fixup")}}
         del k
         torch.cuda.empty_cache()
     out["speedup_geomean"] = round(math.exp(sum(math.log(x) for x in speedups) / len(speedups)), 3)
@@ -343,6 +347,21 @@ def cpu_baseline(n_rows_log2: int = 27):
             "result_check": {"jit_sum": got, "numpy_sum": want, "rel_diff": abs(got - want) / abs(want)}}
 
 
+def _traffic(rows_log2: int):
+    """HBM bytes per launch of the headline kernel measured by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a copy of
+    known size), as recorded in profiles/scan_f64_compound_traffic.json for
+    this workload; None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", "scan_f64_compound_traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if d.get("rows_log2") != rows_log2:
+        return None
+    return d.get("traffic_bytes_per_launch")
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -402,6 +421,14 @@ def main():
     gbs_wall = algo_bytes / (wall / args.steps) / 1e9
     gbs_kernel = algo_bytes / (kernel_ms * 1e-3) / 1e9
 
+    if args.calibrate_copy:
+        src = column("f64", 1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del dst
+
     breakdown = {}
     if not args.no_breakdown:
         for dt_name in ("f64", "f32", "i64", "i32"):
@@ -437,7 +464,7 @@ def main():
         "gb_per_s": round(gbs_wall, 1),
         "rows_per_s": round(n / (wall / args.steps), 1),
         "roofline": {"bound": "hbm", "achieved": round(gbs_kernel, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs_kernel / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs_kernel / HBM_PEAK_GBS, 4), "traffic": _traffic(args.rows_log2),
                      "kernel": "k_scan<double,2,...> (vx_scan.hip)", "algorithmic_bytes_per_launch": algo_bytes,
                      "avg_kernel_ms": round(kernel_ms, 4)},
         "breakdown": breakdown,

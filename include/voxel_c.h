@@ -340,8 +340,8 @@ int       voxel_gpu_sort_pairs_u64_learned(const uint64_t* keys_in, const uint32
                                            uint32_t* vals_out, size_t n, int model,
                                            voxel_gpu_learned_stats* stats, void* stream);
 /* Tuning of the learned sort: log2 sample size (20), training points
- * (4096), Adam steps (100; -1 = none), log2 rows per bucket (11, max 11);
- * 0 keeps the default. */
+ * (4096), Adam steps (60; -1 = none), log2 of the largest average bucket
+ * (8, range 5..8); 0 keeps the default. */
 int       voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, int rows_per_bucket_log2);
 
 /* ---- batched queries (no counterpart in the reference: it runs one program

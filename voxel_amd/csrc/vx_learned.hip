@@ -457,11 +457,57 @@ __global__ __launch_bounds__(256) void k_min_apply(uint32_t* __restrict__ a, uin
     }
 }
 
-// One wave per group of G consecutive buckets (<= kWaveRows rows): the rows
-// go to LDS, every row's rank inside its bucket is counted over the bucket's
-// rows (key, then row order: stable), and the row is written to its place.
-// Larger groups are copied through unchanged and listed in biglist.
 constexpr int kWaveRows = 512;
+constexpr uint32_t kMaxGroup = 63;           // buckets per wave group (their starts sit in LDS)
+
+// Rank sort of rows [s, s + m) (m <= kWaveRows) by one wave: the rows go to
+// the wave's LDS slice, every row's rank inside its bucket is counted over
+// the bucket's rows (key, then row order: stable) and the row is written to
+// its place.  The nbk buckets of the range start at sb[0..nbk) (relative to
+// s, sb[0] = 0, sb[nbk] = m; in LDS).
+__device__ __forceinline__ void wave_rank_sort(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                               const uint32_t* sb, uint32_t nbk, uint32_t s, uint32_t m,
+                                               uint64_t* kb, uint32_t* vb, int lane, uint32_t& longest) {
+    for (uint32_t i = lane; i < m; i += 64) { kb[i] = kin[s + i]; vb[i] = vin[s + i]; }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t i = lane; i < m; i += 64) {
+        // bucket of row i: last q with sb[q] <= i (binary search in LDS)
+        uint32_t a = 0, z = nbk;
+        while (z - a > 1) {
+            const uint32_t mid = (a + z) >> 1;
+            if (sb[mid] <= i) a = mid; else z = mid;
+        }
+        const uint32_t lo = sb[a], hi = sb[a + 1];
+        const uint64_t key = kb[i];
+        uint32_t rank = 0;
+        uint32_t j = lo;
+        for (; j + 4 <= hi; j += 4) {
+            const uint64_t k0 = kb[j], k1 = kb[j + 1], k2 = kb[j + 2], k3 = kb[j + 3];
+            rank += (k0 < key) || (k0 == key && j < i);
+            rank += (k1 < key) || (k1 == key && j + 1 < i);
+            rank += (k2 < key) || (k2 == key && j + 2 < i);
+            rank += (k3 < key) || (k3 == key && j + 3 < i);
+        }
+        for (; j < hi; ++j) {
+            const uint64_t kj = kb[j];
+            rank += (kj < key) || (kj == key && j < i);
+        }
+        if (hi - lo > longest) longest = hi - lo;
+        kout[s + lo + rank] = key;
+        vout[s + lo + rank] = vb[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave per group of G consecutive buckets.  A group of at most
+// kWaveRows rows is rank-sorted at once; a larger group bucket by bucket,
+// and buckets over kWaveRows rows are left in place and listed in biglist
+// (the fix-up handles them with parallel kernels).
 __global__ __launch_bounds__(256) void k_group_sort(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                     const uint32_t* __restrict__ start, uint32_t nb, uint32_t G,
@@ -469,59 +515,49 @@ __global__ __launch_bounds__(256) void k_group_sort(const uint64_t* __restrict__
                                                     unsigned long long* __restrict__ stat) {
     __shared__ uint64_t kb[4][kWaveRows];
     __shared__ uint32_t vb[4][kWaveRows];
+    __shared__ uint32_t sbs[4][kMaxGroup + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t waves = gridDim.x * 4;
     uint32_t longest = 0, widest = 0;
+    uint32_t* sb = sbs[w];
     for (uint32_t g = blockIdx.x * 4 + w; g < ngroups; g += waves) {
-        const uint32_t b0 = g * G, b1 = min(b0 + G, nb);
-        const uint32_t s = start[b0], e = start[b1], m = e - s;
-        widest = m > widest ? m : widest;
+        const uint32_t b0 = g * G, b1 = min(b0 + G, nb), nbk = b1 - b0;
+        const uint32_t s = start[b0];
+        if (lane <= (int)nbk) sb[lane] = start[b0 + lane] - s;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t m = sb[nbk];
         if (m == 0) continue;
-        if (m > kWaveRows || m == 1) {
-            for (uint32_t i = lane; i < m; i += 64) { kout[s + i] = kin[s + i]; vout[s + i] = vin[s + i]; }
-            if (m > 1 && lane == 0) {
-                uint32_t slot = atomicAdd((uint32_t*)&stat[2], 1u);
-                if (slot < big_cap) biglist[slot] = g;
+        if (m <= kWaveRows) {
+            widest = m > widest ? m : widest;
+            if (m == 1) {
+                if (lane == 0) { kout[s] = kin[s]; vout[s] = vin[s]; }
+            } else {
+                wave_rank_sort(kin, vin, kout, vout, sb, nbk, s, m, kb[w], vb[w], lane, longest);
             }
             continue;
         }
-        for (uint32_t i = lane; i < m; i += 64) { kb[w][i] = kin[s + i]; vb[w][i] = vin[s + i]; }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t i = lane; i < m; i += 64) {
-            // bucket of row i: last b in [b0, b1) with start[b] <= s + i
-            uint32_t lo = 0, hi = m;
-            for (uint32_t bb = b0 + 1; bb < b1; ++bb) {
-                const uint32_t x = start[bb] - s;
-                if (x <= i) lo = x;
-                else { hi = x; break; }
+        for (uint32_t q = 0; q < nbk; ++q) {
+            const uint32_t bs = s + sb[q], mb = sb[q + 1] - sb[q];
+            widest = mb > widest ? mb : widest;
+            if (mb == 0) continue;
+            if (mb == 1) {
+                if (lane == 0) { kout[bs] = kin[bs]; vout[bs] = vin[bs]; }
+            } else if (mb <= kWaveRows) {
+                const uint32_t one[2] = {0u, mb};
+                wave_rank_sort(kin, vin, kout, vout, one, 1, bs, mb, kb[w], vb[w], lane, longest);
+            } else if (lane == 0) {
+                const uint32_t slot = atomicAdd((uint32_t*)&stat[2], 1u);
+                if (slot < big_cap) biglist[slot] = b0 + q;
             }
-            const uint64_t key = kb[w][i];
-            uint32_t rank = 0;
-            uint32_t j = lo;
-            for (; j + 4 <= hi; j += 4) {
-                const uint64_t k0 = kb[w][j], k1 = kb[w][j + 1], k2 = kb[w][j + 2], k3 = kb[w][j + 3];
-                rank += (k0 < key) || (k0 == key && j < i);
-                rank += (k1 < key) || (k1 == key && j + 1 < i);
-                rank += (k2 < key) || (k2 == key && j + 2 < i);
-                rank += (k3 < key) || (k3 == key && j + 3 < i);
-            }
-            for (; j < hi; ++j) {
-                const uint64_t kj = kb[w][j];
-                rank += (kj < key) || (kj == key && j < i);
-            }
-            if (hi - lo > longest) longest = hi - lo;
-            kout[s + lo + rank] = key;
-            vout[s + lo + rank] = vb[w][i];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     for (int msk = 32; msk > 0; msk >>= 1) {
         uint32_t o = __shfl_xor(longest, msk, 64);
         longest = o > longest ? o : longest;
+        o = __shfl_xor(widest, msk, 64);
+        widest = o > widest ? o : widest;
     }
     if (lane == 0) {
         atomicMax(&stat[0], (unsigned long long)longest);
@@ -607,7 +643,7 @@ struct LearnedBufs {
 };
 
 struct LearnedTune {
-    int sample_log2 = 20, train_points = 4096, train_steps = 100, bucket_log2 = 8;
+    int sample_log2 = 20, train_points = 4096, train_steps = 60, bucket_log2 = 8;
 };
 static LearnedTune g_tune;
 
@@ -719,7 +755,7 @@ static bool mlp_grad(LearnedBufs& b, const float* P, const float* x, const float
     return hip_ok(hipGetLastError(), "mlp launch");
 }
 
-static float train_lr(int s, int steps) { return s <= (steps * 2) / 3 ? 0.02f : 0.004f; }
+static float train_lr(int s, int steps) { return s <= (steps * 2) / 3 ? 0.03f : 0.006f; }
 
 // The Adam loop of the learned sort (steps x {gradient, reduce + update}) as
 // a graph of kernel nodes: built once per buffer set, then one launch.
@@ -845,7 +881,7 @@ int voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, i
     std::lock_guard<std::mutex> lk(api_mutex());
     g_tune.sample_log2 = sample_log2 > 0 ? sample_log2 : 20;
     g_tune.train_points = train_points > 0 ? train_points : 4096;
-    g_tune.train_steps = train_steps >= 0 ? (train_steps ? train_steps : 100) : 0;
+    g_tune.train_steps = train_steps >= 0 ? (train_steps ? train_steps : 60) : 0;
     g_tune.bucket_log2 = rows_per_bucket_log2 > 0 ? rows_per_bucket_log2 : 8;
     if (g_tune.bucket_log2 > 8) g_tune.bucket_log2 = 8;       // a bucket group must fit one wave's LDS on average
     if (g_tune.bucket_log2 < kMinBucketLog2) g_tune.bucket_log2 = kMinBucketLog2;
@@ -918,7 +954,8 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     const int bits = std::max(bits_min, std::min(8 * parts, clog2(n >> kMinBucketLog2)));
     const uint64_t nb = 1ull << bits;
     const uint32_t nfine = (uint32_t)nb;
-    const uint32_t G = (uint32_t)std::max<uint64_t>(1, 64 / std::max<uint64_t>(1, n >> bits));
+    // a wave takes ~256 rows at a time (loads of 4 rows per lane in flight)
+    const uint32_t G = (uint32_t)std::min<uint64_t>(kMaxGroup, std::max<uint64_t>(1, 256 / std::max<uint64_t>(1, n >> bits)));
     const uint32_t ngroups = (uint32_t)((nb + G - 1) / G);
     const uint64_t nchunks_min = (nb + 1 + kMinChunk - 1) / kMinChunk;
     if (nb + 1 > b.nb_cap) {
@@ -1032,11 +1069,12 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
                        ngroups, b.biglist, kBigCap, b.stat);
     if (!hip_ok(hipGetLastError(), "bucket sort")) return -1;
     hipEventRecord(b.ev[5], st);
-    // 7. groups over a wave's capacity: those holding one key value are in
-    //    stable order already; the others are gathered (in group order),
-    //    sorted together by the radix sort -- key ranges of groups do not
-    //    overlap, so one stable sort of the concatenation sorts each group --
-    //    and scattered back.
+    // 7. buckets over a wave's capacity (still in partition order in ks/vs):
+    //    those holding one key value are in stable order already and are
+    //    copied; the others are gathered (in bucket order), sorted together
+    //    by the radix sort -- key ranges of buckets do not overlap, so one
+    //    stable sort of the concatenation sorts each bucket -- and scattered
+    //    back.
     if (!hip_ok(hipMemcpyAsync(b.stat_host, b.stat, 24, hipMemcpyDeviceToHost, st), "read stats") ||
         !hip_ok(hipStreamSynchronize(st), "sync"))
         return -1;
@@ -1044,20 +1082,18 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     S.widest_group = b.stat_host[1];
     S.groups = ngroups;
     const uint64_t nbig = b.stat_host[2] & 0xFFFFFFFFull;
-    bool overflow = nbig > kBigCap;
-    std::vector<uint32_t> bigs;
-    std::vector<uint32_t> gstart;               // start rows of the big groups: (first, end) pairs
+    const bool overflow = nbig > kBigCap;
+    std::vector<uint64_t> rng;                    // (first row, end row) per big bucket, bucket order
     if (nbig && !overflow) {
         if (!hip_ok(hipMemcpyAsync(b.biglist_host, b.biglist, nbig * 4, hipMemcpyDeviceToHost, st), "read list") ||
             !hip_ok(hipStreamSynchronize(st), "sync"))
             return -1;
-        bigs.assign(b.biglist_host, b.biglist_host + nbig);
+        std::vector<uint32_t> bigs(b.biglist_host, b.biglist_host + nbig);
         std::sort(bigs.begin(), bigs.end());
-        // group bounds: start[g * G] and start[min(g * G + G, nb)]
         if (!ensure_chunks(b, nbig)) return -1;
         for (uint64_t j = 0; j < nbig; ++j) {
-            b.chost[3 * j] = (uint64_t)bigs[j] * G;
-            b.chost[3 * j + 1] = std::min<uint64_t>((uint64_t)bigs[j] * G + G, nb);
+            b.chost[3 * j] = bigs[j];
+            b.chost[3 * j + 1] = (uint64_t)bigs[j] + 1;
             b.chost[3 * j + 2] = 0;
         }
         if (!hip_ok(hipMemcpyAsync(b.cdev, b.chost, nbig * 24, hipMemcpyHostToDevice, st), "copy list")) return -1;
@@ -1066,34 +1102,31 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
             !hip_ok(hipMemcpyAsync(b.chost, b.cdev, nbig * 24, hipMemcpyDeviceToHost, st), "read starts") ||
             !hip_ok(hipStreamSynchronize(st), "sync"))
             return -1;
-        gstart.resize(2 * nbig);
-        for (uint64_t j = 0; j < nbig; ++j) { gstart[2 * j] = (uint32_t)b.chost[3 * j]; gstart[2 * j + 1] = (uint32_t)b.chost[3 * j + 1]; }
+        rng.resize(2 * nbig);
+        for (uint64_t j = 0; j < nbig; ++j) { rng[2 * j] = b.chost[3 * j]; rng[2 * j + 1] = b.chost[3 * j + 1]; }
     }
-    std::vector<uint64_t> chunks;            // (row start, row end, group first row)
-    std::vector<uint32_t> chunk_group;
-    uint64_t maxb = 0;
-    for (uint64_t j = 0; j < (overflow ? 0 : nbig); ++j) {
-        const uint64_t r0 = gstart[2 * j], r1 = gstart[2 * j + 1];
-        maxb = std::max(maxb, r1 - r0);
+    std::vector<uint64_t> chunks;                 // (row start, row end, bucket first row)
+    std::vector<uint32_t> chunk_bucket;
+    for (uint64_t j = 0; j < rng.size() / 2; ++j) {
+        const uint64_t r0 = rng[2 * j], r1 = rng[2 * j + 1];
         S.big_groups++;
         S.big_rows += r1 - r0;
         for (uint64_t r = r0; r < r1; r += kChunk) {
             chunks.push_back(r);
             chunks.push_back(std::min(r + kChunk, r1));
             chunks.push_back(r0);
-            chunk_group.push_back((uint32_t)j);
+            chunk_bucket.push_back((uint32_t)j);
         }
     }
-    (void)maxb;
-    std::vector<uint32_t> nonuniform;            // indexes into bigs
-    if (!chunk_group.empty()) {
-        const uint64_t nch = chunk_group.size();
+    std::vector<char> mixed(rng.size() / 2, 0);
+    if (!chunk_bucket.empty()) {
+        const uint64_t nch = chunk_bucket.size();
         if (!ensure_chunks(b, nch)) return -1;
         std::memcpy(b.chost, chunks.data(), nch * 24);
         bool ok = hip_ok(hipMemcpyAsync(b.cdev, b.chost, nch * 24, hipMemcpyHostToDevice, st), "copy chunks") &&
                   hip_ok(launch_zero32(b.cflag, nch, st), "zero");
         if (ok) {
-            hipLaunchKernelGGL(k_big_check, dim3((unsigned)std::min<uint64_t>(nch, 8192)), dim3(256), 0, st, kout,
+            hipLaunchKernelGGL(k_big_check, dim3((unsigned)std::min<uint64_t>(nch, 8192)), dim3(256), 0, st, ks,
                                b.cdev, nch, b.cflag);
             ok = hip_ok(hipGetLastError(), "big check") &&
                  hip_ok(hipMemcpyAsync(b.cflag_host, b.cflag, nch * 4, hipMemcpyDeviceToHost, st), "read") &&
@@ -1101,36 +1134,50 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         }
         if (!ok) return -1;
         for (uint64_t j = 0; j < nch; ++j)
-            if (b.cflag_host[j] && (nonuniform.empty() || nonuniform.back() != chunk_group[j]))
-                nonuniform.push_back(chunk_group[j]);
+            if (b.cflag_host[j]) mixed[chunk_bucket[j]] = 1;
     }
-    S.big_unsorted = nonuniform.size();
     uint64_t unsorted_rows = 0;
-    for (uint32_t j : nonuniform) unsorted_rows += gstart[2 * j + 1] - gstart[2 * j];
+    for (size_t j = 0; j < mixed.size(); ++j)
+        if (mixed[j]) { S.big_unsorted++; unsorted_rows += rng[2 * j + 1] - rng[2 * j]; }
     if (overflow || unsorted_rows > n / 2) {
         // the model did not split the keys: plain radix sort of the input
         S.fell_back = 1;
         const uint32_t* v0 = payload();
         if (!v0 || sort_pairs_u64(kin, v0, kout, vout, n, c) < 0) return -1;
-    } else if (unsorted_rows) {
-        // move list (source row, destination row, rows) in kChunk pieces
-        std::vector<uint64_t> mv;
+    } else if (!rng.empty()) {
+        // moves (source row, destination row, rows) in kChunk pieces:
+        // single-key buckets straight to kout, the others packed into the free
+        // ping-pong buffers
+        uint64_t* ks_w = const_cast<uint64_t*>(ks);
+        uint32_t* vs_w = const_cast<uint32_t*>(vs);
+        uint64_t* kt = ks_w == b.k1 ? b.k2 : b.k1;
+        uint32_t* vt = vs_w == b.v1 ? b.v2 : b.v1;
+        std::vector<uint64_t> same, pack;
         uint64_t at = 0;
-        for (uint32_t j : nonuniform)
-            for (uint64_t r = gstart[2 * j]; r < gstart[2 * j + 1]; r += kChunk) {
-                uint64_t len = std::min<uint64_t>(kChunk, gstart[2 * j + 1] - r);
-                mv.push_back(r); mv.push_back(at); mv.push_back(len);
-                at += len;
+        for (size_t j = 0; j < mixed.size(); ++j)
+            for (uint64_t r = rng[2 * j]; r < rng[2 * j + 1]; r += kChunk) {
+                const uint64_t len = std::min<uint64_t>(kChunk, rng[2 * j + 1] - r);
+                if (mixed[j]) { pack.push_back(r); pack.push_back(at); pack.push_back(len); at += len; }
+                else { same.push_back(r); same.push_back(r); same.push_back(len); }
             }
-        const uint64_t nmv = mv.size() / 3;
-        if (!ensure_chunks(b, nmv)) return -1;
-        std::memcpy(b.chost, mv.data(), nmv * 24);
-        if (!hip_ok(hipMemcpyAsync(b.cdev, b.chost, nmv * 24, hipMemcpyHostToDevice, st), "copy moves")) return -1;
-        const unsigned mb = (unsigned)std::min<uint64_t>(nmv, 65535);
-        hipLaunchKernelGGL(k_move, dim3(mb), dim3(256), 0, st, kout, vout, b.k1, b.v1, b.cdev, nmv, 0);
-        if (!hip_ok(hipGetLastError(), "gather") || sort_pairs_u64(b.k1, b.v1, b.k2, b.v2, at, c) < 0) return -1;
-        hipLaunchKernelGGL(k_move, dim3(mb), dim3(256), 0, st, b.k2, b.v2, kout, vout, b.cdev, nmv, 1);
-        if (!hip_ok(hipGetLastError(), "scatter")) return -1;
+        const uint64_t ns = same.size() / 3, np_ = pack.size() / 3;
+        if (!ensure_chunks(b, ns + np_)) return -1;
+        std::memcpy(b.chost, same.data(), ns * 24);
+        std::memcpy(b.chost + 3 * ns, pack.data(), np_ * 24);
+        if (!hip_ok(hipMemcpyAsync(b.cdev, b.chost, (ns + np_) * 24, hipMemcpyHostToDevice, st), "copy moves")) return -1;
+        if (ns) {
+            hipLaunchKernelGGL(k_move, dim3((unsigned)std::min<uint64_t>(ns, 65535)), dim3(256), 0, st, ks, vs, kout, vout,
+                               b.cdev, ns, 0);
+            if (!hip_ok(hipGetLastError(), "copy single-key buckets")) return -1;
+        }
+        if (np_) {
+            const uint64_t* mv = b.cdev + 3 * ns;
+            const unsigned mb = (unsigned)std::min<uint64_t>(np_, 65535);
+            hipLaunchKernelGGL(k_move, dim3(mb), dim3(256), 0, st, ks, vs, kt, vt, mv, np_, 0);
+            if (!hip_ok(hipGetLastError(), "gather") || sort_pairs_u64(kt, vt, ks_w, vs_w, at, c) < 0) return -1;
+            hipLaunchKernelGGL(k_move, dim3(mb), dim3(256), 0, st, ks, vs, kout, vout, mv, np_, 1);
+            if (!hip_ok(hipGetLastError(), "scatter")) return -1;
+        }
     }
     hipEventRecord(b.ev[6], st);
     if (!hip_ok(hipEventSynchronize(b.ev[6]), "sync")) return -1;

@@ -177,21 +177,28 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restri
         }
         run[d] = scan_s[d] - t0 + off[(uint64_t)d * nr + blockIdx.x];
     }
-    for (uint64_t t = 0; t < tiles_per_range; ++t) {
-        const uint64_t tbase = ((uint64_t)blockIdx.x * tiles_per_range + t) * kTile;
-        if (tbase >= n) break;                                          // block-uniform
-        for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
-        const uint64_t base = tbase + (uint64_t)w * (64 * kRounds);
-        uint64_t k[kRounds];
-        uint32_t v[kRounds], f[kRounds];
+    // rows of a tile: k/v/f registers; the next tile's rows are loaded into
+    // them as soon as the current tile sits in LDS, so the loads overlap the
+    // scatter of the current tile
+    uint64_t k[kRounds];
+    uint32_t v[kRounds], f[kRounds];
+    auto load_tile = [&](uint64_t tb) {
+        const uint64_t b0 = tb + (uint64_t)w * (64 * kRounds);
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
-            const uint64_t idx = base + r * 64 + lane;
+            const uint64_t idx = b0 + r * 64 + lane;
             const bool ok = idx < n;
             k[r] = ok ? kin[idx] : 0;
             v[r] = ok ? vin[idx] : 0;
             if constexpr (CIN) f[r] = ok ? fin[idx] : 0u;
         }
+    };
+    load_tile((uint64_t)blockIdx.x * tiles_per_range * kTile);
+    for (uint64_t t = 0; t < tiles_per_range; ++t) {
+        const uint64_t tbase = ((uint64_t)blockIdx.x * tiles_per_range + t) * kTile;
+        if (tbase >= n) break;                                          // block-uniform
+        for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
+        const uint64_t base = tbase + (uint64_t)w * (64 * kRounds);
         __syncthreads();                                                // wcnt cleared
         uint32_t rank[kRounds];
 #pragma unroll
@@ -260,6 +267,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restri
                 }
             }
         }
+        if (t + 1 < tiles_per_range && tbase + kTile < n) load_tile(tbase + kTile);
         __syncthreads();
         const uint32_t tn = tbase + kTile <= n ? kTile : (uint32_t)(n - tbase);
 #pragma unroll
