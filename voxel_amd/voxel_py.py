@@ -230,3 +230,151 @@ def _jit_run_r2(code: np.ndarray, data: np.ndarray, threshold: float, r2: int) -
         return float(lib.voxel_jit_run(code.ctypes.data, code.size, data.ctypes.data, data.size, threshold))
     return float(lib.voxel_gpu_jit_run_r2(code.ctypes.data, code.size, data.ctypes.data, data.size,
                                           C.c_uint64(r2 & 0xFFFFFFFFFFFFFFFF), threshold))
+
+
+# ---------------------------------------------------------------------------
+# Operators (voxel_bindings.cpp:224-300).  Host arrays in, host arrays out;
+# the work runs on the GPU through the C ABI.
+# ---------------------------------------------------------------------------
+def _convert(arr, dtype) -> np.ndarray:
+    """py::array_t<T> argument conversion (forcecast): any array-like becomes a
+    C-contiguous array of T."""
+    return np.ascontiguousarray(arr, dtype=dtype)
+
+
+class _Dev:
+    """A device copy of a host array (freed with the object)."""
+
+    def __init__(self, host: np.ndarray):
+        self.n = host.size
+        self.bytes = max(host.nbytes, 8)
+        self.ptr = lib.voxel_gpu_malloc(self.bytes)
+        if not self.ptr:
+            check("voxel_gpu_malloc")
+            raise RuntimeError("voxel_gpu_malloc failed")
+        if host.nbytes and lib.voxel_gpu_memcpy_h2d(self.ptr, host.ctypes.data, host.nbytes) != 0:
+            check("voxel_gpu_memcpy_h2d")
+
+    @staticmethod
+    def empty(nbytes: int) -> "_Dev":
+        return _Dev(np.zeros(max(nbytes, 8), np.uint8))
+
+    def to_host(self, out: np.ndarray) -> np.ndarray:
+        if out.nbytes and lib.voxel_gpu_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes) != 0:
+            check("voxel_gpu_memcpy_d2h")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib.voxel_gpu_free(self.ptr)
+            self.ptr = None
+
+
+class VectorFilterF64:
+    """VectorFilter<f64> (ops/filter.hpp:42-78): apply_gt -> (bitmap words, pass count)."""
+
+    def apply_gt(self, arr, threshold: float):
+        x = _convert(arr, np.float64)
+        words = np.zeros((x.size + 63) // 64, np.uint64)
+        d = _Dev(x)
+        w = _Dev.empty(words.nbytes)
+        cnt = np.zeros(1, np.uint64)
+        tb = int(np.array([threshold], np.float64).view(np.uint64)[0])
+        if lib.voxel_gpu_filter_bitmap(0, d.ptr, x.size, CMP_GT, tb, 0, w.ptr, cnt.ctypes.data, None) != 0:
+            check("apply_gt")
+        w.to_host(words)
+        return words, int(cnt[0])
+
+
+class Arena:
+    """Arena (voxel_bindings.cpp:235-237).  The GPU operators allocate device
+    memory themselves; the object is kept for signature compatibility."""
+
+    def __init__(self, block_size: int = 65536):
+        self.block_size = block_size
+
+
+class HashAggregator:
+    """HashAggregator<u32, f64> (ops/aggregate.hpp:71-301) over voxel_agg_*."""
+
+    def __init__(self, arena: Arena):
+        self._arena = arena
+        self.h = lib.voxel_agg_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.voxel_agg_destroy(self.h)
+            self.h = None
+
+    def init(self, expected_groups: int) -> None:
+        lib.voxel_agg_init(self.h, expected_groups)
+
+    def accumulate(self, keys, values) -> None:
+        k = _convert(keys, np.uint32)
+        v = _convert(values, np.float64)
+        lib.voxel_agg_accumulate(self.h, k.ctypes.data, v.ctypes.data, min(k.size, v.size))
+        check("accumulate")
+
+    def group_count(self) -> int:
+        return int(lib.voxel_agg_group_count(self.h))
+
+
+def sort_ascending(arr) -> np.ndarray:
+    """SortOperator::SortAscending (ops/sort.hpp:506-523): stable argsort, u32."""
+    x = _convert(arr, np.float64)
+    out = np.zeros(x.size, np.uint32)
+    lib.voxel_sort_ascending(x.ctypes.data, x.size, out.ctypes.data)
+    check("sort_ascending")
+    return out
+
+
+def topk_select(arr, k: int, largest: bool) -> np.ndarray:
+    """TopKSelect (ops/sort.hpp): the k largest (or smallest) values, sorted."""
+    x = _convert(arr, np.float64)
+    out = np.zeros(k, np.float64)
+    lib.voxel_topk(x.ctypes.data, x.size, k, out.ctypes.data, 1 if largest else 0)
+    check("topk_select")
+    return out
+
+
+class HashJoin:
+    """HashJoin<u32> (ops/join.hpp:16-214) on the GPU join (keys widened to
+    i64, which keeps key equality and the first-match rule)."""
+
+    def __init__(self, arena: Arena):
+        self._arena = arena
+        self.h = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.voxel_gpu_join_destroy(self.h)
+            self.h = None
+
+    def build(self, keys) -> None:
+        k = _convert(keys, np.uint32).astype(np.int64)
+        d = _Dev(k)
+        if self.h:
+            lib.voxel_gpu_join_destroy(self.h)
+        self.h = lib.voxel_gpu_join_build(d.ptr, None, k.size, None)
+        if not self.h:
+            check("HashJoin.build")
+            raise RuntimeError("HashJoin.build failed")
+
+    def probe(self, probe_keys):
+        p = _convert(probe_keys, np.uint32).astype(np.int64)
+        if self.h is None:
+            return np.zeros(0, np.uint32), np.zeros(0, np.uint32), 0
+        d = _Dev(p)
+        left = _Dev.empty(p.size * 4)
+        right = _Dev.empty(p.size * 4)
+        m = np.zeros(1, np.uint64)
+        if lib.voxel_gpu_join_probe_pairs(self.h, d.ptr, p.size, left.ptr, right.ptr, m.ctypes.data, None) != 0:
+            check("HashJoin.probe")
+        n = int(m[0])
+        lo = left.to_host(np.zeros(n, np.uint32))
+        ro = right.to_host(np.zeros(n, np.uint32))
+        return lo, ro, n
+
+    def contains(self, key: int) -> bool:
+        _, _, n = self.probe(np.array([key], np.uint32))
+        return n > 0
