@@ -214,7 +214,10 @@ typedef struct voxel_gpu_scan_desc {
     int32_t  blocks;           /* workgroups; 0 = library default */
     int32_t  unroll;           /* 16-byte loads in flight per lane and column: 2, 4, 8; 0 = default */
     int32_t  nontemporal;      /* 1 = non-temporal (streaming) loads, -1 = cached loads, 0 = default */
-    int32_t  layout;           /* 0 = grid-stride tiles, 1 = contiguous run of tiles per workgroup */
+    int32_t  layout;           /* 0 = default (grid-stride tiles, partials combined by a second kernel),
+                                  1 = contiguous tiles per workgroup + last-workgroup combine,
+                                  2 = grid-stride + second kernel, 3 = contiguous + second kernel,
+                                  4 = grid-stride + last-workgroup combine */
     int32_t  pad_;
     const void* a;
     const void* b;

@@ -10,7 +10,7 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvoxel_c.so")
+LIB_PATH = os.environ.get("VOXEL_LIB") or os.path.join(HERE, "libvoxel_c.so")
 HEADER = os.path.join(HERE, "..", "include", "voxel_c.h")
 
 # torch ships its own HIP runtime (same soname as /opt/rocm's).  Loading torch

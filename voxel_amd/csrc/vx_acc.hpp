@@ -18,15 +18,18 @@ template <typename T>
 struct Acc {
     using S = typename TyInfo<T>::S;
     S s, c;
-    uint64_t n;
+    uint64_t n;        // rows counted by merges
+    uint32_t k;        // rows counted by add (folded into n before any merge)
     T mn, mx;
-    __device__ __forceinline__ void init() { s = S(0); c = S(0); n = 0; mn = Lim<T>::hi(); mx = Lim<T>::lo(); }
+    __device__ __forceinline__ void init() { s = S(0); c = S(0); n = 0; k = 0; mn = Lim<T>::hi(); mx = Lim<T>::lo(); }
+    __device__ __forceinline__ void fold() { n += k; k = 0; }
     __device__ __forceinline__ void add(bool p, T x) {
-        n += p ? 1u : 0u;
+        k += p ? 1u : 0u;
         if constexpr (TyInfo<T>::F) {
             two_sum<S>(s, c, p ? x : T(0));
-            mn = p ? fmin(mn, x) : mn;
-            mx = p ? fmax(mx, x) : mx;
+            // NaN fails both compares, so NaN rows do not reach MIN/MAX
+            mn = (p && x < mn) ? x : mn;
+            mx = (p && x > mx) ? x : mx;
         } else {
             s += p ? (uint64_t)(int64_t)x : 0ull;
             mn = (p && x < mn) ? x : mn;
@@ -34,13 +37,13 @@ struct Acc {
         }
     }
     __device__ __forceinline__ void merge(const Acc& o) {
-        n += o.n;
+        n += o.n + o.k;
         if constexpr (TyInfo<T>::F) {
             S cc = c + o.c;
             two_sum<S>(s, cc, o.s);
             c = cc;
-            mn = fmin(mn, o.mn);
-            mx = fmax(mx, o.mx);
+            mn = o.mn < mn ? o.mn : mn;
+            mx = o.mx > mx ? o.mx : mx;
         } else {
             s += o.s;
             mn = o.mn < mn ? o.mn : mn;
@@ -49,6 +52,7 @@ struct Acc {
     }
     __device__ __forceinline__ void shfl_merge(int mask) {
         Acc o;
+        o.k = 0;
         o.s = __shfl_xor(s, mask, 64);
         o.c = __shfl_xor(c, mask, 64);
         o.n = __shfl_xor(n, mask, 64);
@@ -57,15 +61,17 @@ struct Acc {
         merge(o);
     }
     __device__ __forceinline__ void store(ScanOut* p) const {
-        p->s = to_bits(s); p->c = to_bits(c); p->cnt = n; p->mn = to_bits(mn); p->mx = to_bits(mx);
+        p->s = to_bits(s); p->c = to_bits(c); p->cnt = n + k; p->mn = to_bits(mn); p->mx = to_bits(mx);
     }
     __device__ __forceinline__ void load(const ScanOut* p) {
-        s = from_bits<S>(p->s); c = from_bits<S>(p->c); n = p->cnt; mn = from_bits<T>(p->mn); mx = from_bits<T>(p->mx);
+        s = from_bits<S>(p->s); c = from_bits<S>(p->c); n = p->cnt; k = 0;
+        mn = from_bits<T>(p->mn); mx = from_bits<T>(p->mx);
     }
 };
 // Workgroup reduction of an Acc into thread 0 (deterministic order).
 template <typename T>
 __device__ __forceinline__ void block_reduce(Acc<T>& a, Acc<T>* lds) {
+    a.fold();
     for (int m = 32; m > 0; m >>= 1) a.shfl_merge(m);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) lds[wid] = a;

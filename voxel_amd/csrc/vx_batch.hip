@@ -8,14 +8,15 @@
 // take tiles from an atomic counter, reduce a tile with the scan's
 // compensated COUNT/SUM/MIN/MAX accumulator (vx_acc.hpp) and store one
 // partial per tile; the workgroup that completes a query's last tile (a
-// per-query ticket) combines that query's partials in tile order.  A query's
-// result therefore depends only on its own rows and tile_rows: it is the
-// same bits alone or in any mix of queries.
+// per-query pass after the kernel boundary, k_batch_finish) combines each
+// query's partials in tile order.  A query's result therefore depends only
+// on its own rows and tile_rows: it is the same bits alone or in any mix of
+// queries.
 //
 // Host side: BatchRunner keeps pinned descriptor/result buffers and a graph
-// of three nodes -- descriptor upload, kernel, result download -- whose copy
-// sizes are updated per launch; the kernel's counters reset themselves, so
-// the graph holds no memset node.
+// of four nodes -- descriptor upload, tile kernel, finish kernel, result
+// download -- whose copy sizes are updated per launch; the finish kernel
+// resets the tile counter, so the graph holds no memset node.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -39,12 +40,10 @@ namespace {
 
 template <typename T, int U, bool NT>
 __device__ __forceinline__ void batch_tile(const BatchQuery& q, uint64_t s, uint64_t e, uint64_t tile,
-                                           ScanOut* __restrict__ parts, ScanOut* __restrict__ results,
-                                           uint32_t qi, uint32_t* __restrict__ ticket) {
+                                           ScanOut* __restrict__ parts) {
     using V = typename TyInfo<T>::V;
     constexpr int E = TyInfo<T>::E;
     __shared__ Acc<T> lds[kBlock / 64];
-    __shared__ bool s_last;
     Pred<T, false> pa; pa.load(q.pa);
     Pred<T, false> pb; pb.load(q.pb);
     const bool two = q.mode == 2;
@@ -92,66 +91,83 @@ __device__ __forceinline__ void batch_tile(const BatchQuery& q, uint64_t s, uint
         }
     }
     for (uint64_t i = s + nvec * E + threadIdx.x; i < e; i += kBlock) one(a[i], two ? b[i] : T(0));
-
     block_reduce(acc, lds);
-    if (threadIdx.x == 0) {
-        acc.store(&parts[tile]);
-        __threadfence();
-        const uint32_t prev = __hip_atomic_fetch_add(&ticket[qi], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = prev == q.ntiles - 1;
-    }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        Acc<T> r; r.init();
-        for (uint32_t i = threadIdx.x; i < q.ntiles; i += kBlock) {
-            Acc<T> o; o.load(&parts[q.tile0 + i]);
-            r.merge(o);
-        }
-        __syncthreads();
-        block_reduce(r, lds);
-        if (threadIdx.x == 0) {
-            r.store(&results[qi]);
-            __hip_atomic_store(&ticket[qi], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    if (threadIdx.x == 0) acc.store(&parts[tile]);
     __syncthreads();
 }
 
+// Tiles of all queries: resident workgroups take tile ids from ctr[0] and
+// store one partial per tile.  No workgroup waits on or reads another's
+// partial here (that would need an agent-scope release and acquire per tile:
+// an L2 write-back and invalidate on this multi-XCD part); k_batch_finish
+// combines them after the kernel boundary.
 template <int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchHeader* __restrict__ hdr, const BatchQuery* __restrict__ qs,
                                                   uint32_t* __restrict__ ctr) {
     __shared__ uint32_t s_tile;
     const uint32_t total = hdr->total_tiles, nq = hdr->nq, tile_rows = hdr->tile_rows;
     ScanOut* parts = hdr->parts;
-    ScanOut* results = hdr->results;
+    uint32_t lo = 0;                                             // query of the block's previous tile
     for (;;) {
         if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const uint32_t tile = s_tile;
         __syncthreads();
         if (tile >= total) break;                               // block-uniform
-        uint32_t lo = 0, hi = nq;                                // last query with tile0 <= tile
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (qs[mid].tile0 <= tile) lo = mid; else hi = mid;
+        // last query with tile0 <= tile: a block's tiles increase, so step
+        // forward from the previous one, and jump by binary search when far
+        if (lo + 1 < nq && qs[lo + 1].tile0 <= tile) {
+            if (lo + 2 < nq && qs[lo + 2].tile0 <= tile) {
+                uint32_t a = lo + 2, z = nq;
+                while (z - a > 1) {
+                    const uint32_t mid = (a + z) >> 1;
+                    if (qs[mid].tile0 <= tile) a = mid; else z = mid;
+                }
+                lo = a;
+            } else {
+                lo = lo + 1;
+            }
         }
         const BatchQuery q = qs[lo];
         const uint64_t s = (uint64_t)(tile - q.tile0) * tile_rows;
         const uint64_t e = s + tile_rows < q.n ? s + tile_rows : q.n;
         switch (q.dtype) {
-            case DT_F64: batch_tile<double, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
-            case DT_F32: batch_tile<float, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
-            case DT_I64: batch_tile<int64_t, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
-            default: batch_tile<int32_t, U, NT>(q, s, e, tile, parts, results, lo, ctr + 2); break;
+            case DT_F64: batch_tile<double, U, NT>(q, s, e, tile, parts); break;
+            case DT_F32: batch_tile<float, U, NT>(q, s, e, tile, parts); break;
+            case DT_I64: batch_tile<int64_t, U, NT>(q, s, e, tile, parts); break;
+            default: batch_tile<int32_t, U, NT>(q, s, e, tile, parts); break;
         }
     }
-    // the last workgroup to leave resets the tile counter for the next launch
-    if (threadIdx.x == 0) {
-        const uint32_t prev = __hip_atomic_fetch_add(&ctr[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == gridDim.x - 1) {
-            __hip_atomic_store(&ctr[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctr[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ void finish_query(const BatchQuery& q, const ScanOut* __restrict__ parts,
+                                             ScanOut* __restrict__ out) {
+    __shared__ Acc<T> lds[kBlock / 64];
+    Acc<T> r; r.init();
+    for (uint32_t i = threadIdx.x; i < q.ntiles; i += kBlock) {
+        Acc<T> o; o.load(&parts[q.tile0 + i]);
+        r.merge(o);
+    }
+    block_reduce(r, lds);
+    if (threadIdx.x == 0) r.store(out);
+    __syncthreads();
+}
+
+// One workgroup per query (grid-stride): the query's tile partials in a fixed
+// order -> its result, in the caller's order; resets the tile counter.
+__global__ __launch_bounds__(kBlock) void k_batch_finish(const BatchHeader* __restrict__ hdr,
+                                                         const BatchQuery* __restrict__ qs, uint32_t* __restrict__ ctr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = 0;
+    const uint32_t nq = hdr->nq;
+    for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        const BatchQuery q = qs[qi];
+        ScanOut* out = &hdr->results[q.orig];
+        switch (q.dtype) {
+            case DT_F64: finish_query<double>(q, hdr->parts, out); break;
+            case DT_F32: finish_query<float>(q, hdr->parts, out); break;
+            case DT_I64: finish_query<int64_t>(q, hdr->parts, out); break;
+            default: finish_query<int32_t>(q, hdr->parts, out); break;
         }
     }
 }
@@ -244,7 +260,12 @@ bool BatchRunner::build_graph() {
     kp.kernelParams = args;
     kp.extra = nullptr;
     if (!hip_ok(hipGraphAddKernelNode(&n_kern_, graph_, &n_up_, 1, &kp), "graph kernel node")) return false;
-    if (!hip_ok(hipGraphAddMemcpyNode1D(&n_down_, graph_, &n_kern_, 1, hres_, dres_, max_q_ * sizeof(ScanOut),
+    hipKernelNodeParams fp = kp;
+    fp.func = reinterpret_cast<void*>(&k_batch_finish);
+    fp.gridDim = dim3((unsigned)std::min<size_t>(max_q_, (size_t)device_cu_count() * 8));
+    hipGraphNode_t n_fin;
+    if (!hip_ok(hipGraphAddKernelNode(&n_fin, graph_, &n_kern_, 1, &fp), "graph finish node")) return false;
+    if (!hip_ok(hipGraphAddMemcpyNode1D(&n_down_, graph_, &n_fin, 1, hres_, dres_, max_q_ * sizeof(ScanOut),
                                         hipMemcpyDeviceToHost), "graph download node"))
         return false;
     if (!hip_ok(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate")) return false;
@@ -258,11 +279,18 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
     if (nq == 0 || nq > max_q_) { set_error("batch: %zu queries (1..%zu allowed)", nq, max_q_); return false; }
     BatchHeader* h = reinterpret_cast<BatchHeader*>(hq_);
     BatchQuery* qs = reinterpret_cast<BatchQuery*>(hq_ + sizeof(BatchHeader));
+    // tile ids are handed out largest query first, so the last tiles the
+    // workgroups take are the small ones and no long tile runs alone at the
+    // end; results go back to the caller's order (descriptor field orig)
+    order_.resize(nq);
+    for (size_t i = 0; i < nq; ++i) order_[i] = (uint32_t)i;
+    std::stable_sort(order_.begin(), order_.end(), [&](uint32_t x, uint32_t y) { return q[x].n > q[y].n; });
     uint64_t tiles = 0;
-    for (size_t i = 0; i < nq; ++i) {
+    for (size_t j = 0; j < nq; ++j) {
+        const size_t i = order_[j];
         ScanArgs g;
         if (!make_scan_args(&q[i], g)) { set_error("batch query %zu: %s", i, last_error()); return false; }
-        BatchQuery& d = qs[i];
+        BatchQuery& d = qs[j];
         std::memset(&d, 0, sizeof(d));
         d.a = g.a;
         d.b = g.mode == 2 ? g.b : g.a;
@@ -273,6 +301,7 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
         d.mode = g.mode;
         d.tile0 = tiles;
         d.ntiles = (uint32_t)tiles_of(d.n);
+        d.orig = (uint32_t)i;
         tiles += d.ntiles;
         dtypes_[i] = g.dtype;
     }

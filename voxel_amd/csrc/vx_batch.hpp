@@ -23,7 +23,7 @@ struct BatchQuery {
     uint64_t tile0;          // first global tile of the query
     PredDev pa, pb;
     int32_t dtype, mode;     // mode 0: pa on a; 2: pa on a and pb on b
-    uint32_t ntiles, pad;
+    uint32_t ntiles, orig;   // tiles; index of the query in the caller's list
     uint64_t pad2[2];
 };
 static_assert(sizeof(BatchQuery) == 128, "BatchQuery layout");
@@ -81,6 +81,7 @@ class BatchRunner {
     hipGraphNode_t n_up_ = nullptr, n_kern_ = nullptr, n_down_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<int> dtypes_;
+    std::vector<uint32_t> order_;
     size_t nq_ = 0;
     bool inflight_ = false;
     bool graph_valid_ = false;

@@ -267,6 +267,7 @@ bool make_scan_args(const voxel_gpu_scan_desc* d, ScanArgs& g) {
     g = ScanArgs{};
     g.dtype = d->dtype; g.a = d->a; g.b = d->b; g.begin = 0; g.end = d->n; g.blocks = d->blocks;
     g.unroll = d->unroll; g.nontemporal = d->nontemporal; g.layout = d->layout;
+    if (g.layout < 0 || g.layout > 4) { set_error("scan: layout must be 0..4"); return false; }
     if (g.unroll != 0 && g.unroll != 2 && g.unroll != 4 && g.unroll != 8) { set_error("scan: unroll must be 2, 4 or 8"); return false; }
     g.pa = make_pred(d->dtype, d->op_a, d->t_a);
     if (d->b && d->op_b != CMP_TRUE) {

@@ -67,7 +67,11 @@ struct ScanArgs {
     int blocks;            // 0 = pick
     int unroll;            // 16-byte loads in flight per lane and column: 2, 4, 8 (0 = default)
     int nontemporal;       // 1 = non-temporal loads, -1 = cached loads, 0 = default (non-temporal)
-    int layout;            // 0 = grid-stride tiles, 1 = one contiguous run of tiles per workgroup
+    // tiles / combine: 0 = default (grid-stride tiles, partials combined by a
+    // second kernel), 1 = contiguous run of tiles per workgroup + last-
+    // workgroup combine, 2 = grid-stride + second kernel, 3 = contiguous +
+    // second kernel, 4 = grid-stride + last-workgroup combine
+    int layout;
 };
 
 // Fused filter + COUNT/SUM/MIN/MAX over one range; result in *out_dev

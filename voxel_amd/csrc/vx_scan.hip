@@ -30,7 +30,7 @@
 namespace vx {
 namespace {
 
-template <typename T, int MODE, bool TA, bool TB, bool VEC, int U, bool NT, bool CONTIG>
+template <typename T, int MODE, bool TA, bool TB, bool VEC, int U, bool NT, bool CONTIG, bool FIN = true>
 __global__ __launch_bounds__(kBlock)
 void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, uint64_t end,
             PredDev pda, PredDev pdb, ScanOut* __restrict__ parts, uint32_t* __restrict__ ticket,
@@ -120,6 +120,7 @@ void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, ui
     __shared__ bool s_last;
     block_reduce(acc, lds);
     if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
+    if constexpr (!FIN) return;                 // k_scan_final combines after the kernel boundary
     if (!publish_and_check_last(ticket, &s_last)) return;
 
     Acc<T> r; r.init();
@@ -133,6 +134,21 @@ void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, ui
         r.store(out);
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// One workgroup: the partials of a k_scan<..., FIN = false> launch, merged in
+// the same order as the last-workgroup combine.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_final(const ScanOut* __restrict__ parts, uint32_t nblocks,
+                                                       ScanOut* __restrict__ out) {
+    __shared__ Acc<T> lds[kBlock / 64];
+    Acc<T> r; r.init();
+    for (uint32_t i = threadIdx.x; i < nblocks; i += kBlock) {
+        Acc<T> o; o.load(&parts[i]);
+        r.merge(o);
+    }
+    block_reduce(r, lds);
+    if (threadIdx.x == 0) r.store(out);
 }
 
 // ---------------------------------------------------------------------------
@@ -301,6 +317,20 @@ hipError_t launch_u(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStre
     uint64_t blocks = want < cap ? want : cap;
     if (blocks < 1) blocks = 1;
     ScanOut* parts = static_cast<ScanOut*>(scr.partials);
+    // the last-workgroup combine needs an agent-scope release/acquire per
+    // workgroup (an L2 write-back + invalidate on this multi-XCD part); a
+    // second one-workgroup kernel after the kernel boundary avoids it
+    const bool split = g.layout == 0 || g.layout == 2 || g.layout == 3;
+    if (split) {
+        if (vec)
+            hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, true, U, NT, CONTIG, false>), dim3((unsigned)blocks), dim3(kBlock),
+                               0, st, a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
+        else
+            hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, false, U, NT, CONTIG, false>), dim3((unsigned)blocks), dim3(kBlock),
+                               0, st, a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
+        hipLaunchKernelGGL((k_scan_final<T>), dim3(1), dim3(kBlock), 0, st, (const ScanOut*)parts, (uint32_t)blocks, out);
+        return hipGetLastError();
+    }
     if (vec)
         hipLaunchKernelGGL((k_scan<T, MODE, TA, TB, true, U, NT, CONTIG>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
                            a, b, g.begin, g.end, g.pa, g.pb, parts, scr.ticket, out);
@@ -318,7 +348,7 @@ hipError_t launch_t(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStre
     // per CU, non-temporal loads, 4 (8-byte types) / 8 (4-byte types) loads
     int u = g.unroll > 0 ? g.unroll : (sizeof(T) == 8 ? 4 : 8);
     bool nt = g.nontemporal >= 0;             // default: non-temporal
-    bool contig = g.layout > 0;
+    bool contig = g.layout == 1 || g.layout == 3;
     if (TA || TB) { u = 4; nt = false; contig = false; }    // table predicates: compat paths, one variant
 #define VX_SCAN_U(UU)                                                                                   \
     if (u == UU) {                                                                                      \
