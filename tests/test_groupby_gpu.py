@@ -134,6 +134,21 @@ def test_groupby_sparse_and_signed_keys(gpu):
     check(gpu_groupby(ukeys, vals, key_dtype=5, max_groups=n), ukeys, vals, signed=False)
 
 
+@pytest.mark.parametrize("rare", [5, 380, 1_000_000])
+def test_groupby_keys_missed_by_the_window_sample(gpu, rare):
+    """The dense window comes from a sample of 65536 rows at i * n / 65536;
+    rows 1 and 2 are never sampled for n = 10^6, so their keys fall outside
+    the window: the pass is redone with the exact range (dense when it still
+    spans at most 512 keys, the sort path otherwise) and matches the oracle."""
+    rng = np.random.default_rng(rare)
+    n = 1_000_000
+    keys = rng.integers(100, 200, n).astype(np.int32)
+    keys[1] = rare
+    keys[2] = 150 + rare % 7
+    vals = rng.standard_normal(n) * 10
+    check(gpu_groupby(keys, vals), keys, vals)
+
+
 def test_groupby_skewed_single_group_and_deterministic(gpu):
     n = 1_000_003
     keys = np.full(n, 42, np.int32)                                # every lane of a wave conflicts

@@ -4,14 +4,17 @@
 // and the voxel_agg_* C ABI (bindings/voxel_c.cpp:167-182).
 //
 // Dense path (key range of at most kMaxDense consecutive values, e.g. the
-// 265 taxi zones): every wave owns a private copy of the bins in LDS.  A
-// wave takes 64 rows at a time; lanes holding the same bin are ranked by
-// lane (wave ballots over the bin bits) and update the bin one rank after
-// the other, so no two lanes of a wave write one bin at once and every bin
-// sees its rows in row order.  Bins hold a TwoSum-compensated (s, c) pair.
-// The 4 wave copies merge in wave order; workgroups write partials that one
-// workgroup per bin merges in workgroup order.  The result does not depend
-// on scheduling: it is bit-identical from run to run.
+// 265 taxi zones): every wave owns a private copy of the bins in LDS, laid
+// out as separate arrays (sum, compensation, min, max, u32 count).  Rows are
+// added with LDS atomics: ds_add_rtn_f64 returns the bin's sum before the
+// row's add, so the rounding error of that add is recovered exactly (TwoSum
+// of the returned value and the row) and added to the compensation word;
+// min, max and count are ds_min_f64 / ds_max_f64 / ds_add_u32.  Only one
+// wave writes a copy and the LDS unit applies a wave instruction's lanes in
+// a fixed order, so the result does not depend on scheduling and is
+// bit-identical from run to run.  The wave copies merge in wave order;
+// workgroups write partials that one workgroup per bin merges in workgroup
+// order.
 //
 // General path (any key domain): keys become order-preserving u64 and are
 // sorted with their row ids by the radix sort (vx_radix.hip); group starts
@@ -21,9 +24,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <limits>
 #include <vector>
 
 #include "../../include/voxel_c.h"
+#include "vx_acc.hpp"
 #include "vx_compact.hpp"
 #include "vx_device.hpp"
 #include "vx_groupby.hpp"
@@ -36,9 +41,11 @@
 namespace vx {
 namespace {
 
-constexpr int kGThreads = 256;
+constexpr int kGThreads = 512;
 constexpr int kGWaves = kGThreads / 64;
-constexpr int kGUnroll = 4;
+constexpr int kGChunk = 256;          // rows per wave step: 4 per lane
+constexpr int kGUnroll = 2;           // wave steps in flight
+constexpr size_t kGBinBytes = 4 * sizeof(double) + sizeof(uint32_t);
 
 __device__ __forceinline__ void bin_init(GBin& b) {
     b.s = 0.0; b.c = 0.0; b.cnt = 0;
@@ -61,52 +68,128 @@ __device__ __forceinline__ void bin_merge(GBin& a, const GBin& o) {
     a.mx = fmax(a.mx, o.mx);
 }
 
-__device__ __forceinline__ uint64_t g_lanemask_lt() {
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
+// Per-wave bin arrays in LDS: sum, compensation, (min, max) pairs and u32
+// counts, each indexed by wave copy * bins + bin.  A row costs one
+// ds_add_rtn_f64, one ds_read_b128 of its (min, max) pair and one ds_add_u32;
+// the compensation add is skipped when the rounding error is zero, and the
+// min / max atomics are issued only when the row beats the pair just read
+// (only this wave writes the copy, so the pair read is current).
+struct WaveBins {
+    double* s;
+    double* c;
+    double2* mm;
+    uint32_t* cnt;
+    __device__ __forceinline__ void add(uint32_t i, double v) const {
+        double old = __hip_atomic_fetch_add(&s[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        double t = old + v;
+        double bp = t - old;
+        double e = (old - (t - bp)) + (v - bp);
+        if (e != 0.0) __hip_atomic_fetch_add(&c[i], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const double2 p = mm[i];
+        if (v < p.x) __hip_atomic_fetch_min(&mm[i].x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (v > p.y) __hip_atomic_fetch_max(&mm[i].y, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&cnt[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ __forceinline__ GBin get(uint32_t i) const {
+        GBin b;
+        const double2 p = mm[i];
+        b.s = s[i]; b.c = c[i]; b.mn = p.x; b.mx = p.y; b.cnt = cnt[i];
+        return b;
+    }
+};
 
-template <typename K>
+// Rows [0, n): wave steps of kGChunk rows go to waves round-robin (wave
+// wid takes steps wid, wid + waves, ...).  VEC: keys and values 16-byte
+// aligned, so a lane reads its 4 consecutive rows with non-temporal 16-byte
+// loads.  CHECK: the key window [kmin, kmin + nbins) came from a sample;
+// rows outside it go to a spare bin (index nbins) that is never read and
+// are counted in range[2], and the exact key range is folded into range[0]
+// (min) and range[1] (max).  Each wave copy holds nbins + 1 bins.
+template <typename K, bool VEC, bool CHECK>
 __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ keys, const double* __restrict__ vals,
-                                                        uint64_t n, int64_t kmin, uint32_t nbins, int nbits,
-                                                        GBin* __restrict__ parts) {
-    extern __shared__ GBin bins[];               // kGWaves * nbins
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (uint32_t i = threadIdx.x; i < kGWaves * nbins; i += kGThreads) bin_init(bins[i]);
+                                                        uint64_t n, int64_t kmin, uint32_t nbins,
+                                                        GBin* __restrict__ parts, long long* __restrict__ range) {
+    extern __shared__ double gsm[];
+    const uint32_t nb1 = nbins + 1;
+    const uint32_t tot = kGWaves * nb1;
+    WaveBins all{gsm, gsm + tot, reinterpret_cast<double2*>(gsm + 2 * tot), reinterpret_cast<uint32_t*>(gsm + 4 * tot)};
+    for (uint32_t i = threadIdx.x; i < tot; i += kGThreads) {
+        all.s[i] = 0.0; all.c[i] = 0.0; all.cnt[i] = 0;
+        all.mm[i] = make_double2(__builtin_huge_val(), -__builtin_huge_val());
+    }
     __syncthreads();
-    GBin* mine = bins + (size_t)w * nbins;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t off = (uint32_t)w * nb1;
+    WaveBins mine{all.s + off, all.c + off, all.mm + off, all.cnt + off};
+    K klo = std::numeric_limits<K>::max(), khi = std::numeric_limits<K>::lowest();
+    uint32_t miss = 0;
+    auto row = [&](K k, double v) {
+        uint32_t b = (uint32_t)((int64_t)k - kmin);
+        if constexpr (CHECK) {
+            klo = k < klo ? k : klo;
+            khi = k > khi ? k : khi;
+            miss += b >= nbins ? 1u : 0u;
+            b = b < nbins ? b : nbins;
+        }
+        mine.add(b, v);
+    };
     const uint64_t waves = (uint64_t)gridDim.x * kGWaves;
     const uint64_t wid = (uint64_t)blockIdx.x * kGWaves + w;
-    for (uint64_t base = wid * 64 * kGUnroll; base < n; base += waves * 64 * kGUnroll) {
-        uint32_t b[kGUnroll];
-        double v[kGUnroll];
-        bool ok[kGUnroll];
+    const uint64_t steps = n / kGChunk;
+    uint64_t st = wid;
+    for (; st + (kGUnroll - 1) * waves < steps; st += kGUnroll * waves) {
+        K kk[kGUnroll][4];
+        double v[kGUnroll][4];
 #pragma unroll
         for (int u = 0; u < kGUnroll; ++u) {
-            uint64_t row = base + (uint64_t)u * 64 + lane;
-            ok[u] = row < n;
-            b[u] = ok[u] ? (uint32_t)((int64_t)keys[row] - kmin) : 0u;
-            v[u] = ok[u] ? vals[row] : 0.0;
+            const uint64_t base = (st + (uint64_t)u * waves) * kGChunk;
+            if constexpr (VEC) {
+                uint4 k4 = ld16<uint4, true>(reinterpret_cast<const uint4*>(keys + base + 4 * lane));
+                double2 v0 = ld16<double2, true>(reinterpret_cast<const double2*>(vals + base + 4 * lane));
+                double2 v1 = ld16<double2, true>(reinterpret_cast<const double2*>(vals + base + 4 * lane + 2));
+                kk[u][0] = (K)k4.x; kk[u][1] = (K)k4.y; kk[u][2] = (K)k4.z; kk[u][3] = (K)k4.w;
+                v[u][0] = v0.x; v[u][1] = v0.y; v[u][2] = v1.x; v[u][3] = v1.y;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    kk[u][j] = keys[base + 64 * j + lane];
+                    v[u][j] = vals[base + 64 * j + lane];
+                }
+            }
         }
 #pragma unroll
-        for (int u = 0; u < kGUnroll; ++u) {
-            uint64_t peers = __ballot(ok[u]);
-            for (int bit = 0; bit < nbits; ++bit) {
-                uint64_t m = __ballot((b[u] >> bit) & 1u);
-                peers &= ((b[u] >> bit) & 1u) ? m : ~m;
-            }
-            int rank = ok[u] ? __popcll(peers & g_lanemask_lt()) : 64;
-            for (int r = 0;; ++r) {
-                if (rank == r) bin_add(mine[b[u]], v[u]);
-                if (!__any(rank > r && rank < 64)) break;
-            }
+        for (int u = 0; u < kGUnroll; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) row(kk[u][j], v[u][j]);
+    }
+    for (; st < steps; st += waves) {
+        const uint64_t base = st * kGChunk;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) row(keys[base + 64 * j + lane], vals[base + 64 * j + lane]);
+    }
+    if (wid == waves - 1)                                   // rows past the last whole step
+        for (uint64_t r = steps * kGChunk + lane; r < n; r += 64) row(keys[r], vals[r]);
+    if constexpr (CHECK) {
+        int64_t l = (int64_t)klo, h = (int64_t)khi;
+        for (int m = 32; m > 0; m >>= 1) {
+            const int64_t x = __shfl_xor(l, m, 64), y = __shfl_xor(h, m, 64);
+            l = x < l ? x : l;
+            h = y > h ? y : h;
+        }
+        uint32_t msum = miss;
+        if (__ballot(miss != 0))
+            for (int m = 32; m > 0; m >>= 1) msum += __shfl_xor(msum, m, 64);
+        if (lane == 0) {
+            atomicMin(&range[0], (long long)l);
+            atomicMax(&range[1], (long long)h);
+            if (msum) atomicAdd((unsigned long long*)&range[2], (unsigned long long)msum);
         }
     }
     __syncthreads();
     GBin* out = parts + (size_t)blockIdx.x * nbins;
     for (uint32_t i = threadIdx.x; i < nbins; i += kGThreads) {
-        GBin acc = bins[i];
-        for (int ww = 1; ww < kGWaves; ++ww) bin_merge(acc, bins[(size_t)ww * nbins + i]);
+        GBin acc = all.get(i);
+        for (int ww = 1; ww < kGWaves; ++ww) bin_merge(acc, all.get((uint32_t)ww * nb1 + i));
         out[i] = acc;
     }
 }
@@ -127,25 +210,34 @@ __global__ __launch_bounds__(256) void k_gb_final(const GBin* __restrict__ parts
     if (threadIdx.x == 0) out[bin] = red[0];
 }
 
+// range[0..1] = key range of a sample: rows i * n / S for i < S (every row
+// when n <= S), one row per thread; range[0..4] start as (max, min, max,
+// min, 0) from k_range_init, and the checked dense kernel folds the exact
+// range and the rows outside the window into range[2..4].
+constexpr uint32_t kSample = 65536;
+__global__ void k_range_init(long long* range) {
+    range[0] = INT64_MAX; range[1] = INT64_MIN; range[2] = INT64_MAX; range[3] = INT64_MIN; range[4] = 0;
+}
 template <typename K>
-__global__ __launch_bounds__(256) void k_key_minmax(const K* __restrict__ keys, uint64_t n,
-                                                    long long* __restrict__ mm) {
+__global__ __launch_bounds__(256) void k_key_sample(const K* __restrict__ keys, uint64_t n,
+                                                    long long* __restrict__ range) {
+    const uint64_t S = n < kSample ? n : kSample;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     int64_t lo = INT64_MAX, hi = INT64_MIN;
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        int64_t k = (int64_t)keys[i];
-        lo = k < lo ? k : lo;
-        hi = k > hi ? k : hi;
+    if (i < S) {
+        const uint64_t r = S == n ? i : (uint64_t)(((unsigned __int128)i * n) / S);
+        lo = hi = (int64_t)keys[r];
     }
     for (int m = 32; m > 0; m >>= 1) {
-        int64_t a = __shfl_xor(lo, m, 64), b = __shfl_xor(hi, m, 64);
-        lo = a < lo ? a : lo;
-        hi = b > hi ? b : hi;
+        const int64_t x = __shfl_xor(lo, m, 64), y = __shfl_xor(hi, m, 64);
+        lo = x < lo ? x : lo;
+        hi = y > hi ? y : hi;
     }
-    if ((threadIdx.x & 63) == 0) { atomicMin(&mm[0], (long long)lo); atomicMax(&mm[1], (long long)hi); }
+    if ((threadIdx.x & 63) == 0 && lo <= hi) {
+        atomicMin(&range[0], (long long)lo);
+        atomicMax(&range[1], (long long)hi);
+    }
 }
-
-__global__ void k_mm_init(long long* mm) { mm[0] = INT64_MAX; mm[1] = INT64_MIN; }
 
 // non-empty dense bins -> compacted group records (ascending key)
 struct DenseEmit {
@@ -319,6 +411,54 @@ static int64_t group_sorted(GroupbyWork& w, uint64_t n, const double* vals, cons
     return (int64_t)ng;
 }
 
+// One dense pass with key window [kmin, kmin + nbins).  check: the window
+// is a guess; rows outside it are counted (w.mm_host[4] after return) and
+// the exact key range comes back in w.mm_host[2..3].
+static int64_t dense_pass(GroupbyWork& w, int key_dtype, const void* keys, const double* vals, uint64_t n,
+                          int64_t kmin, uint32_t nbins, bool check, int64_t* okey, GBin* orec, StreamCtx* c) {
+    hipStream_t st = c->stream;
+    const size_t lds = (size_t)kGWaves * (nbins + 1) * kGBinBytes;
+    const bool vec = ((uintptr_t)keys & 15) == 0 && ((uintptr_t)vals & 15) == 0;
+    const bool i32 = key_dtype == DT_I32;
+    const void* fns[2][2][2] = {
+        {{(const void*)k_gb_dense<uint32_t, false, false>, (const void*)k_gb_dense<uint32_t, false, true>},
+         {(const void*)k_gb_dense<uint32_t, true, false>, (const void*)k_gb_dense<uint32_t, true, true>}},
+        {{(const void*)k_gb_dense<int32_t, false, false>, (const void*)k_gb_dense<int32_t, false, true>},
+         {(const void*)k_gb_dense<int32_t, true, false>, (const void*)k_gb_dense<int32_t, true, true>}}};
+    const void* fn = fns[i32][vec][check];
+    int occ = 0;
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kGThreads, lds), "occupancy")) return -1;
+    if (occ < 1) occ = 1;
+    unsigned blocks = (unsigned)device_cu_count() * (unsigned)occ;
+    const uint64_t need = (n + (uint64_t)kGChunk * kGUnroll * kGWaves - 1) / ((uint64_t)kGChunk * kGUnroll * kGWaves);
+    if (need < blocks) blocks = (unsigned)(need ? need : 1);
+    // a wave's bin counts are u32: it sees at most ceil(steps / waves) steps plus the tail
+    const uint64_t per_wave =
+        (n / kGChunk + (uint64_t)blocks * kGWaves - 1) / ((uint64_t)blocks * kGWaves) * kGChunk + kGChunk;
+    if (per_wave >= (1ull << 32)) {
+        set_error("group-by: %llu rows exceed the dense path's per-wave count", (unsigned long long)n);
+        return -1;
+    }
+    const uint64_t recs = (uint64_t)blocks * nbins;
+    if (recs > w.parts_cap) {
+        if (w.parts) (void)hipFree(w.parts);
+        w.parts = nullptr; w.parts_cap = 0;
+        if (!hip_ok(hipMalloc((void**)&w.parts, recs * sizeof(GBin)), "hipMalloc(partials)")) return -1;
+        w.parts_cap = recs;
+    }
+    long long* range = w.mm + 2;                  // exact min, max, rows outside the window
+    void* args[] = {(void*)&keys, (void*)&vals, (void*)&n, (void*)&kmin, (void*)&nbins, (void*)&w.parts, (void*)&range};
+    if (!hip_ok(hipLaunchKernel(fn, dim3(blocks), dim3(kGThreads), args, lds, st), "group-by launch")) return -1;
+    hipLaunchKernelGGL(k_gb_final, dim3(nbins), dim3(256), 0, st, w.parts, blocks, nbins, w.bins);
+    if (!hip_ok(hipGetLastError(), "group-by final launch")) return -1;
+    if (check && !hip_ok(hipMemcpyAsync(w.mm_host + 2, w.mm + 2, 24, hipMemcpyDeviceToHost, st), "read key range"))
+        return -1;
+    DenseEmit src{w.bins, kmin, okey, orec};
+    uint64_t ng = 0;
+    if (!run_compact(src, nbins, c, &ng)) return -1;      // synchronises the stream
+    return (int64_t)ng;
+}
+
 // Group-by over device columns; group records in ascending key order.
 int64_t groupby_f64(int key_dtype, const void* keys, const double* vals, uint64_t n, int64_t** okey_out,
                     GBin** orec_out, StreamCtx* c, int force_general) {
@@ -338,45 +478,31 @@ int64_t groupby_f64(int key_dtype, const void* keys, const double* vals, uint64_
         if (!hip_ok(hipHostMalloc((void**)&w.mm_host, 64, hipHostMallocDefault), "hipHostMalloc")) return -1;
         if (!hip_ok(hipMalloc((void**)&w.bins, kMaxDense * sizeof(GBin)), "hipMalloc")) return -1;
     }
-    hipLaunchKernelGGL(k_mm_init, dim3(1), dim3(1), 0, st, w.mm);
-    unsigned mblocks = (unsigned)device_cu_count() * 4;
-    if (key_dtype == DT_I32) hipLaunchKernelGGL(k_key_minmax<int32_t>, dim3(mblocks), dim3(256), 0, st, (const int32_t*)keys, n, w.mm);
-    else if (key_dtype == DT_U32) hipLaunchKernelGGL(k_key_minmax<uint32_t>, dim3(mblocks), dim3(256), 0, st, (const uint32_t*)keys, n, w.mm);
-    else if (key_dtype == DT_I64) hipLaunchKernelGGL(k_key_minmax<int64_t>, dim3(mblocks), dim3(256), 0, st, (const int64_t*)keys, n, w.mm);
-    else { set_error("group-by: key dtype %d not supported (i32, u32, i64)", key_dtype); return -1; }
-    if (!hip_ok(hipMemcpyAsync(w.mm_host, w.mm, 16, hipMemcpyDeviceToHost, st), "read key range") ||
-        !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+    if (key_dtype != DT_I32 && key_dtype != DT_U32 && key_dtype != DT_I64) {
+        set_error("group-by: key dtype %d not supported (i32, u32, i64)", key_dtype);
         return -1;
-    const int64_t kmin = w.mm_host[0], kmax = w.mm_host[1];
-    const uint64_t span = (uint64_t)(kmax - kmin) + 1;
-    if (!force_general && k32 && span <= (uint64_t)kMaxDense) {
-        uint32_t nbins = (uint32_t)span;
-        int nbits = 0;
-        while ((1u << nbits) < nbins) ++nbits;
-        unsigned blocks = (unsigned)device_cu_count() * 4;
-        uint64_t need = (n + 64 * kGUnroll * kGWaves - 1) / (64 * kGUnroll * kGWaves);
-        if (need < blocks) blocks = (unsigned)(need ? need : 1);
-        uint64_t recs = (uint64_t)blocks * nbins;
-        if (recs > w.parts_cap) {
-            if (w.parts) (void)hipFree(w.parts);
-            w.parts = nullptr; w.parts_cap = 0;
-            if (!hip_ok(hipMalloc((void**)&w.parts, recs * sizeof(GBin)), "hipMalloc(partials)")) return -1;
-            w.parts_cap = recs;
+    }
+    if (!force_general && k32) {
+        // key window from a sample of kSample rows (all rows when n <= kSample)
+        const unsigned sblocks = (unsigned)(((n < kSample ? n : kSample) + 255) / 256);
+        hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, st, w.mm);
+        if (key_dtype == DT_I32) hipLaunchKernelGGL(k_key_sample<int32_t>, dim3(sblocks), dim3(256), 0, st, (const int32_t*)keys, n, w.mm);
+        else hipLaunchKernelGGL(k_key_sample<uint32_t>, dim3(sblocks), dim3(256), 0, st, (const uint32_t*)keys, n, w.mm);
+        if (!hip_ok(hipGetLastError(), "sample launch") ||
+            !hip_ok(hipMemcpyAsync(w.mm_host, w.mm, 16, hipMemcpyDeviceToHost, st), "read key range") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+            return -1;
+        int64_t kmin = w.mm_host[0];
+        uint64_t span = (uint64_t)(w.mm_host[1] - kmin) + 1;
+        bool exact = n <= kSample;
+        while (span <= (uint64_t)kMaxDense) {
+            int64_t ng = dense_pass(w, key_dtype, keys, vals, n, kmin, (uint32_t)span, !exact, okey, orec, c);
+            if (ng < 0) return -1;
+            if (exact || w.mm_host[4] == 0) return ng;
+            kmin = w.mm_host[2];                     // a key outside the sampled window: exact range known now
+            span = (uint64_t)(w.mm_host[3] - kmin) + 1;
+            exact = true;
         }
-        size_t lds = (size_t)kGWaves * nbins * sizeof(GBin);
-        if (key_dtype == DT_I32)
-            hipLaunchKernelGGL(k_gb_dense<int32_t>, dim3(blocks), dim3(kGThreads), lds, st, (const int32_t*)keys, vals, n,
-                               kmin, nbins, nbits, w.parts);
-        else
-            hipLaunchKernelGGL(k_gb_dense<uint32_t>, dim3(blocks), dim3(kGThreads), lds, st, (const uint32_t*)keys, vals,
-                               n, kmin, nbins, nbits, w.parts);
-        if (!hip_ok(hipGetLastError(), "group-by launch")) return -1;
-        hipLaunchKernelGGL(k_gb_final, dim3(nbins), dim3(256), 0, st, w.parts, blocks, nbins, w.bins);
-        if (!hip_ok(hipGetLastError(), "group-by final launch")) return -1;
-        DenseEmit src{w.bins, kmin, okey, orec};
-        uint64_t ng = 0;
-        if (!run_compact(src, nbins, c, &ng)) return -1;
-        return (int64_t)ng;
     }
     // general path: sort (key, row) and reduce segments
     if (n >= (1ull << 30)) { set_error("group-by: general path limited to 2^30 rows"); return -1; }

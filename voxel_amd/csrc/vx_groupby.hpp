@@ -4,9 +4,9 @@
 
 namespace vx {
 
-// Largest key range handled by the LDS-privatised dense path: four wave
-// copies of kMaxDense 40-byte bins fit in 64 KB of LDS.
-constexpr int kMaxDense = 384;
+// Largest key range handled by the LDS-privatised dense path: eight wave
+// copies of kMaxDense + 1 36-byte bins (144 KB) fit in a CU's 160 KB of LDS.
+constexpr int kMaxDense = 512;
 
 // Aggregate state of one group: compensated sum pair, count, min, max.
 struct GBin {
