@@ -211,30 +211,47 @@ def bench_join() -> dict:
     exp_sum = int(torch.where(hit, pay[row], torch.zeros_like(pay[row])).sum().item())
     del u, rank, miss
     torch.cuda.synchronize()
-    holder = {}
+    from voxel_amd._lib import JoinStats
+    layouts = {}
+    for mode, name in ((0, "auto"), (1, "one_table"), (3, "learned_range"), (2, "hash")):
+        holder = {}
+        st = JoinStats()
 
-    def build():
-        if "j" in holder:
-            lib.voxel_gpu_join_destroy(holder.pop("j"))
-        holder["j"] = lib.voxel_gpu_join_build(bk.data_ptr(), pay.data_ptr(), nb, None)
-        if not holder["j"]:
-            raise RuntimeError("join build failed")
+        def build():
+            if "j" in holder:
+                lib.voxel_gpu_join_destroy(holder.pop("j"))
+            holder["j"] = lib.voxel_gpu_join_build_ex(bk.data_ptr(), pay.data_ptr(), nb, mode, C.byref(st), None)
+            if not holder["j"]:
+                raise RuntimeError("join build failed")
 
-    t_build = _sync_time(build)
-    cnt, sm = C.c_uint64(), C.c_int64()
+        t_build = _sync_time(build)
+        cnt, sm = C.c_uint64(), C.c_int64()
 
-    def probe_fn():
-        if lib.voxel_gpu_join_probe_sum(holder["j"], probe.data_ptr(), npr, C.byref(cnt), C.byref(sm), None) != 0:
-            raise RuntimeError("join probe failed")
+        def probe_fn():
+            if lib.voxel_gpu_join_probe_sum(holder["j"], probe.data_ptr(), npr, C.byref(cnt), C.byref(sm), None) != 0:
+                raise RuntimeError("join probe failed")
 
-    t_probe = _sync_time(probe_fn)
-    lib.voxel_gpu_join_destroy(holder.pop("j"))
-    ok = cnt.value == exp_count and sm.value == exp_sum
-    out = {"build_rows": nb, "probe_rows": npr, "gprobes_per_s": round(npr / t_probe / 1e9, 3),
-           "probe_ms": round(t_probe * 1e3, 3), "build_ms": round(t_build * 1e3, 3),
-           "build_mrows_per_s": round(nb / t_build / 1e6, 1),
-           "probe_algorithmic_gb_per_s": round(npr * 24 / t_probe / 1e9, 1),
-           "match_count": cnt.value, "payload_sum": sm.value, "matches_expected": ok}
+        t_probe = _sync_time(probe_fn)
+        lib.voxel_gpu_join_destroy(holder.pop("j"))
+        d = st.as_dict()
+        layouts[name] = {"gprobes_per_s": round(npr / t_probe / 1e9, 3), "probe_ms": round(t_probe * 1e3, 3),
+                         "build_ms": round(t_build * 1e3, 3), "mode": d["mode"], "fell_back": d["fell_back"],
+                         "partitions": d["partitions"], "max_partition_keys": d["max_partition_keys"],
+                         "min_partition_keys": d["min_partition_keys"],
+                         "matches_expected": cnt.value == exp_count and sm.value == exp_sum}
+        if mode == 3:
+            layouts[name]["model_train_ms"] = round(d["model"]["train_ms"], 3)
+            layouts[name]["model_final_logit_mse"] = round(d["model"]["final_loss"], 5)
+    a = layouts["auto"]
+    out = {"build_rows": nb, "probe_rows": npr, "gprobes_per_s": a["gprobes_per_s"],
+           "probe_ms": a["probe_ms"], "build_ms": a["build_ms"],
+           "build_mrows_per_s": round(nb / (a["build_ms"] * 1e-3) / 1e6, 1),
+           "probe_algorithmic_gb_per_s": round(npr * 24 / (a["probe_ms"] * 1e-3) / 1e9, 1),
+           "match_count": exp_count, "payload_sum": exp_sum,
+           "matches_expected": all(v["matches_expected"] for v in layouts.values()),
+           "layouts": layouts,
+           "note": "auto = one 2^25-slot table, direct probes; hash / learned_range = 256 sub-tables of 2^17 "
+                   "slots, probes partitioned by sub-table (one digit pass) before probing"}
     del bk, pay, perm, row, hit, probe
     torch.cuda.empty_cache()
     return out
@@ -469,7 +486,7 @@ def main():
         "rows_per_s": round(n / (wall / args.steps), 1),
         "roofline": {"bound": "hbm", "achieved": round(gbs_kernel, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs_kernel / HBM_PEAK_GBS, 4), "traffic": _traffic(args.rows_log2),
-                     "kernel": "k_scan<double,2,...> (vx_scan.hip)", "algorithmic_bytes_per_launch": algo_bytes,
+                     "kernel": "k_scan<double,2,...> + k_scan_final (vx_scan.hip), one op", "algorithmic_bytes_per_launch": algo_bytes,
                      "avg_kernel_ms": round(kernel_ms, 4)},
         "breakdown": breakdown,
         "result_check": {"count": res["count"], "sum": res["sum"], "min": res["min"], "max": res["max"]},

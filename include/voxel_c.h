@@ -347,6 +347,32 @@ int       voxel_gpu_sort_pairs_u64_learned(const uint64_t* keys_in, const uint32
  * (8, range 5..8); 0 keeps the default. */
 int       voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, int rows_per_bucket_log2);
 
+/* ---- partitioned hash join (HashJoin<i64>, ops/join.hpp:36-80, with the
+ * table cut into sub-tables; same results as voxel_gpu_join_build) ------- */
+typedef struct voxel_gpu_join_stats {
+    int32_t  mode;                   /* layout in effect: 1 one table, 2 hash, 3 learned range, 4 equal-width range */
+    int32_t  fell_back;              /* mode whose sub-tables overflowed (the table was rebuilt as one), 0 none */
+    uint32_t partitions;             /* sub-tables */
+    uint32_t pad;
+    uint64_t slots_per_partition;
+    uint64_t max_partition_keys;     /* distinct keys in the fullest / emptiest sub-table */
+    uint64_t min_partition_keys;
+    float    build_ms;
+    float    pad2;
+    voxel_gpu_learned_stats model;   /* range modes: the CDF model that partitions the keys */
+} voxel_gpu_join_stats;
+/* mode 0: auto (one table: measured fastest on MI355X); 1: one table;
+ * partitioned layouts, sub-tables of 2^17 slots (at most 256): 2: sub-table
+ * = top bits of the key's hash; 3: learned range partition (the learned
+ * sort's MLP CDF fitted to the build keys); 4: equal-width range partition
+ * (no model).  A range partition whose sub-table overflows falls back to one
+ * table (stats->fell_back).  With a partitioned layout, probe batches of 2^20
+ * rows or more are partitioned by sub-table before probing (one stable digit
+ * pass) unless voxel_gpu_join_set_probe_partitioning(j, 0). */
+void*     voxel_gpu_join_build_ex(const int64_t* build_keys_dev, const int64_t* payload_dev, size_t nb, int mode,
+                                  voxel_gpu_join_stats* stats, void* stream);
+int       voxel_gpu_join_set_probe_partitioning(void* join, int on);
+
 /* ---- batched queries (no counterpart in the reference: it runs one program
  * per Engine::Run, exec/engine.hpp:109-141) ------------------------------- */
 /* A batch engine runs up to max_queries scan descriptors (voxel_gpu_scan's,

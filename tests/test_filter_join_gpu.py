@@ -199,6 +199,81 @@ def test_join_random_vs_oracle(gpu, nb, np_):
     assert r["left"] == lo[:m].tolist() and r["right"] == ro[:m].tolist()
 
 
+def join_mode(build, payload, probe, mode, partition_probes=1):
+    import ctypes as C
+    import torch
+    from voxel_amd._lib import JoinStats, lib
+    bd, pd, pr = dev(build), dev(payload), dev(probe)
+    st = JoinStats()
+    j = lib.voxel_gpu_join_build_ex(bd.data_ptr(), pd.data_ptr(), build.size, mode, C.byref(st), None)
+    assert j, lib.voxel_last_error()
+    assert lib.voxel_gpu_join_set_probe_partitioning(j, partition_probes) == 0
+    m, s = np.zeros(1, np.uint64), np.zeros(1, np.int64)
+    assert lib.voxel_gpu_join_probe_sum(j, pr.data_ptr(), probe.size, m.ctypes.data, s.ctypes.data, None) == 0
+    lo = torch.zeros(probe.size, dtype=torch.int32, device="cuda")
+    ro = torch.zeros(probe.size, dtype=torch.int32, device="cuda")
+    k = np.zeros(1, np.uint64)
+    assert lib.voxel_gpu_join_probe_pairs(j, pr.data_ptr(), probe.size, lo.data_ptr(), ro.data_ptr(),
+                                          k.ctypes.data, None) == 0
+    lib.voxel_gpu_join_destroy(j)
+    kk = int(k[0])
+    return {"matches": int(m[0]), "sum": int(s[0]), "pairs": kk, "left": host(lo, kk, np.uint32),
+            "right": host(ro, kk, np.uint32), "stats": st.as_dict()}
+
+
+def oracle_join(build, payload, probe):
+    lo, ro = np.zeros(max(probe.size, 1), np.uint32), np.zeros(max(probe.size, 1), np.uint32)
+    s = np.zeros(1, np.int64)
+    m = o.O.vo_hashjoin_i64(o.ptr(build), build.size, o.ptr(payload), o.ptr(probe), probe.size, o.ptr(lo),
+                            o.ptr(ro), o.ptr(s))
+    return m, int(s[0]), lo[:m], ro[:m]
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_join_partitioned_layouts_vs_oracle(gpu, mode):
+    """One table, hash sub-tables, learned and equal-width range sub-tables:
+    match count, payload sum and (probe, build) pairs equal the oracle's, with
+    the probes partitioned by sub-table (2^20 + 3 rows) and in input order."""
+    rng = np.random.default_rng(40 + mode)
+    nb, np_ = 300_000, (1 << 20) + 3
+    build = rng.integers(-2**40, 2**40, nb).astype(np.int64)
+    build[rng.integers(0, nb, 20_000)] = build[rng.integers(0, nb, 20_000)]    # duplicate keys
+    build[7] = np.iinfo(np.int64).min                                        # the empty-slot sentinel
+    payload = rng.integers(-2**62, 2**62, nb).astype(np.int64)
+    probe = np.where(rng.random(np_) < 0.7, build[rng.integers(0, nb, np_)], rng.integers(-2**41, 2**41, np_))
+    probe[::1001] = np.iinfo(np.int64).min
+    m, s, lo, ro = oracle_join(build, payload, probe)
+    for pp in (1, 0):
+        r = join_mode(build, payload, probe, mode, pp)
+        assert r["stats"]["mode"] == mode and r["stats"]["fell_back"] == 0
+        assert r["stats"]["partitions"] == (1 if mode == 1 else 8)
+        assert r["matches"] == m == r["pairs"] and r["sum"] == s
+        assert np.array_equal(r["left"], lo) and np.array_equal(r["right"], ro)
+
+
+def test_join_learned_partition_gain_and_equal_width_fallback(gpu):
+    """Build keys floor(exp(20 + 2Z)): equal-width key ranges put nearly all
+    keys into the first sub-table, which overflows, and the table falls back to
+    one table; the learned CDF splits them evenly (fullest sub-table at most
+    1.5x the mean) and needs no fallback.  Both give the oracle's result."""
+    rng = np.random.default_rng(77)
+    nb, np_ = 300_000, 400_000
+    build = np.unique(np.floor(np.exp(20 + 2 * rng.standard_normal(nb))).astype(np.int64))
+    payload = rng.integers(-2**62, 2**62, build.size).astype(np.int64)
+    probe = np.where(rng.random(np_) < 0.9, build[rng.integers(0, build.size, np_)], -rng.integers(1, 2**40, np_))
+    m, s, lo, ro = oracle_join(build, payload, probe)
+    eq = join_mode(build, payload, probe, 4)
+    assert eq["stats"]["fell_back"] == 4 and eq["stats"]["mode"] == 1
+    le = join_mode(build, payload, probe, 3)
+    st = le["stats"]
+    assert st["fell_back"] == 0 and st["mode"] == 3 and st["partitions"] == 8
+    assert st["max_partition_keys"] <= 1.5 * build.size / st["partitions"], st
+    assert st["model"]["train_steps"] > 0 and st["model"]["log_grid"] == 1
+    for r in (eq, le):
+        assert r["matches"] == m and r["sum"] == s
+        assert np.array_equal(r["left"], lo) and np.array_equal(r["right"], ro)
+
+
 def test_join_config_shape_properties(gpu):
     """Build 2^24 distinct keys (bit 63 clear) with i64 payloads, probe 2^28:
     a probe set built from build keys (all match) and their bit-63-set copies

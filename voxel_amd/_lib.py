@@ -57,6 +57,17 @@ class LearnedStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class JoinStats(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("fell_back", C.c_int32), ("partitions", C.c_uint32), ("pad", C.c_uint32),
+                ("slots_per_partition", U64), ("max_partition_keys", U64), ("min_partition_keys", U64),
+                ("build_ms", C.c_float), ("pad2", C.c_float), ("model", LearnedStats)]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("pad") and f != "model"}
+        d["model"] = self.model.as_dict()
+        return d
+
+
 class ServeStats(C.Structure):
     _fields_ = [("queries", U64), ("batches", U64), ("bytes", U64), ("max_batch_seen", U64),
                 ("admission_waits", U64), ("makespan_s", F64), ("qps", F64), ("gbps", F64), ("mean_batch", F64),
@@ -169,6 +180,8 @@ SIGS = {
     "voxel_gpu_join_probe_sum_launch": (INT, [P, P, SZ, P]),
     "voxel_gpu_join_probe_sum_result": (INT, [P, P, P, P]),
     "voxel_gpu_join_probe_pairs": (INT, [P, P, SZ, P, P, P, P]),
+    "voxel_gpu_join_build_ex": (P, [P, P, SZ, INT, P, P]),
+    "voxel_gpu_join_set_probe_partitioning": (INT, [P, INT]),
     "voxel_gpu_groupby_f64": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
     "voxel_gpu_groupby_f64_general": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
     "voxel_gpu_agg_results": (INT, [P, P, P, P, P, P, P, SZ]),

@@ -194,12 +194,14 @@ __device__ __forceinline__ uint64_t sm_mix(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// s keys at SplitMix64 positions, as order bits key ^ flip
 __global__ __launch_bounds__(256) void k_sample(const uint64_t* __restrict__ keys, uint64_t n, uint64_t s,
-                                                uint64_t seed, uint64_t* __restrict__ out, uint32_t* __restrict__ idx) {
+                                                uint64_t seed, uint64_t flip, uint64_t* __restrict__ out,
+                                                uint32_t* __restrict__ idx) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s; i += stride) {
         uint64_t pos = __umul64hi(sm_mix(seed + (i + 1) * 0x9E3779B97F4A7C15ull), n);
-        out[i] = keys[pos];
+        out[i] = keys[pos] ^ flip;
         idx[i] = (uint32_t)i;
     }
 }
@@ -813,6 +815,120 @@ static bool train_graph(LearnedBufs& b, uint64_t T, int steps, hipStream_t st) {
     return hip_ok(hipGraphLaunch(b.graph_exec, st), "hipGraphLaunch");
 }
 
+// Steps 1-4 of the learned sort: sample n keys (as order bits key ^ flip),
+// pick the grid, train the MLP (model 0) and build the table that maps a key
+// to one of nfine buckets, non-decreasing in the key.  Records b.ev[1..3]
+// (b.ev[0] is the caller's); the table lives in b.cell.
+static bool learned_table(LearnedBufs& b, const uint64_t* kin, uint64_t n, uint64_t flip, int model, uint32_t nfine,
+                          voxel_gpu_learned_stats& S, TableDigit& dig, StreamCtx* c) {
+    hipStream_t st = c->stream;
+    uint64_t SS = 1ull << g_tune.sample_log2;                        // sample size
+    if (SS > n / 4) SS = n / 4;
+    uint64_t T = std::min<uint64_t>(SS, (uint64_t)g_tune.train_points);   // training points
+    if (SS > b.s_cap) {
+        for (void* p : {(void*)b.samp, (void*)b.samp2, (void*)b.sidx, (void*)b.sidx2})
+            if (p) (void)hipFree(p);
+        b.samp = b.samp2 = nullptr; b.sidx = b.sidx2 = nullptr; b.s_cap = 0;
+        if (!galloc((void**)&b.samp, SS * 8) || !galloc((void**)&b.samp2, SS * 8) || !galloc((void**)&b.sidx, SS * 4) ||
+            !galloc((void**)&b.sidx2, SS * 4))
+            return false;
+        b.s_cap = SS;
+    }
+    if (!ensure_train(b, T)) return false;
+    // 1. sample, sort it, read 1025 quantiles
+    hipLaunchKernelGGL(k_sample, dim3(gridn(SS, 4096)), dim3(256), 0, st, kin, (uint64_t)n, SS, 0x5EEDull, flip, b.samp,
+                       b.sidx);
+    if (!hip_ok(hipGetLastError(), "sample")) return false;
+    if (sort_pairs_u64(b.samp, b.sidx, b.samp2, b.sidx2, SS, c) < 0) return false;
+    hipLaunchKernelGGL(k_quantiles, dim3(5), dim3(256), 0, st, b.samp2, SS, b.qdev);
+    if (!hip_ok(hipGetLastError(), "quantiles") ||
+        !hip_ok(hipMemcpyAsync(b.qhost, b.qdev, 1025 * 8, hipMemcpyDeviceToHost, st), "read quantiles") ||
+        !hip_ok(hipStreamSynchronize(st), "sync"))
+        return false;
+    // 2. grid: linear or logarithmic, whichever the sample CDF is straighter on
+    const uint64_t* q = b.qhost;
+    const uint64_t kmin = q[0], kmax = q[1024];
+    const uint64_t span = kmax - kmin;
+    auto logg = [&](uint64_t k) { return (double)TableDigit::grid(k - kmin, 1); };
+    double dev_lin = 0, dev_log = 0;
+    const double glog = logg(kmax);
+    for (int i = 0; i <= 1024; ++i) {
+        double target = i / 1024.0;
+        if (span) dev_lin = std::max(dev_lin, std::fabs((double)(q[i] - kmin) / (double)span - target));
+        if (glog > 0) dev_log = std::max(dev_log, std::fabs(logg(q[i]) / glog - target));
+    }
+    const int logmode = (model == 0 && dev_log < dev_lin) ? 1 : 0;
+    S.log_grid = logmode;
+    const uint64_t gmax_i = logmode ? (uint64_t)glog : span;
+    int gbits = 0;
+    while (gbits < 64 && (gmax_i >> gbits) != 0) ++gbits;
+    const int cshift = gbits > 16 ? gbits - 16 : 0;
+    const uint32_t ncells = (uint32_t)(gmax_i >> cshift) + 1;
+    S.cells = ncells;
+    hipEventRecord(b.ev[1], st);
+    // 3. train the MLP on (grid position, sample rank)
+    if (model == 0) {
+        std::vector<float> P;
+        init_params(P, 0xC0FFEEull);
+        std::memcpy(b.phost, P.data(), kParams * 4);
+        if (!hip_ok(hipMemcpyAsync(b.P, b.phost, kParams * 4, hipMemcpyHostToDevice, st), "copy params")) return false;
+        hipLaunchKernelGGL(k_train_data, dim3(gridn(T, 1024)), dim3(256), 0, st, b.samp2, SS, T, kmin, kmax, logmode,
+                           (double)gmax_i, b.tx, b.ty);
+        if (!hip_ok(launch_zero32((uint32_t*)b.m, kParams, st), "zero") || !hip_ok(launch_zero32((uint32_t*)b.v, kParams, st), "zero"))
+            return false;
+        const int steps = g_tune.train_steps;
+        if (steps > 0 && !train_graph(b, T, steps, st)) return false;
+        S.train_steps = steps;
+    }
+    hipEventRecord(b.ev[2], st);
+    // 4. CDF at the cell starts -> fine-bucket table.  model 0: MLP root routes
+    //    each cell to one of kLeaves linear models fitted to the sample;
+    //    model 1: linear in the key (no model).
+    hipLaunchKernelGGL(k_root_cells, dim3(gridn(ncells + 1, 256)), dim3(256), 0, st, b.P, ncells, cshift,
+                       (double)gmax_i, model, b.R);
+    hipLaunchKernelGGL(k_prefix_max<float>, dim3(1), dim3(1024), 0, st, b.R, ncells + 1);
+    if (model == 0) {
+        hipLaunchKernelGGL(k_leaf_lo, dim3((kLeaves + 256) / 256), dim3(256), 0, st, b.samp2, SS, b.R, ncells, cshift,
+                           kmin, kmax, logmode, kLeaves, b.leaf_lo);
+        hipLaunchKernelGGL(k_leaf_cells, dim3(gridn(ncells, 256)), dim3(256), 0, st, b.samp2, SS, b.R, ncells, cshift,
+                           kmin, kmax, logmode, kLeaves, b.leaf_lo, b.yd);
+    } else {
+        hipLaunchKernelGGL(k_widen, dim3((ncells + 255) / 256), dim3(256), 0, st, b.R, ncells, b.yd);
+    }
+    hipLaunchKernelGGL(k_prefix_max<double>, dim3(1), dim3(1024), 0, st, b.yd, ncells);
+    hipLaunchKernelGGL(k_table, dim3((ncells + 255) / 256), dim3(256), 0, st, b.yd, ncells, nfine, b.cell);
+    if (!hip_ok(hipGetLastError(), "table")) return false;
+    dig = TableDigit{b.cell, kmin, kmax, ncells, cshift, logmode, nfine, std::ldexp(1.0f, -cshift)};
+    hipEventRecord(b.ev[3], st);
+    return true;
+}
+
+// The learned CDF of n keys as a partitioner into nfine buckets (for the
+// range-partitioned join): the table is copied to cell_out (65538 entries).
+bool learned_partitioner(const uint64_t* kin, uint64_t n, uint64_t flip, int model, uint32_t nfine, uint2* cell_out,
+                         TableDigit* dig, voxel_gpu_learned_stats* stats, StreamCtx* c) {
+    voxel_gpu_learned_stats S;
+    std::memset(&S, 0, sizeof(S));
+    S.model = model;
+    S.buckets = nfine;
+    LearnedBufs& b = lbufs(c);
+    if (!ensure_model(b)) return false;
+    hipEventRecord(b.ev[0], c->stream);
+    if (!learned_table(b, kin, n, flip, model, nfine, S, *dig, c)) return false;
+    if (!hip_ok(hipMemcpyAsync(cell_out, b.cell, (size_t)(dig->ncells + 1) * sizeof(uint2), hipMemcpyDeviceToDevice,
+                               c->stream), "copy partition table"))
+        return false;
+    dig->cell = cell_out;
+    if (!hip_ok(hipEventSynchronize(b.ev[3]), "sync")) return false;
+    float ms[3] = {};
+    for (int i = 0; i < 3; ++i) (void)hipEventElapsedTime(&ms[i], b.ev[i], b.ev[i + 1]);
+    S.sample_ms = ms[0]; S.train_ms = ms[1]; S.table_ms = ms[2];
+    if (model == 0 && S.train_steps > 0 && hip_ok(hipMemcpy(b.loss_host, b.loss, 4, hipMemcpyDeviceToHost), "read loss"))
+        S.final_loss = *b.loss_host;
+    if (stats) *stats = S;
+    return true;
+}
+
 }  // namespace vx
 
 using namespace vx;
@@ -932,19 +1048,6 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         if (!galloc((void**)&b.w.status, words * 4)) return -1;
         b.w.status_cap = words;
     }
-    uint64_t SS = 1ull << g_tune.sample_log2;                        // sample size
-    if (SS > n / 4) SS = n / 4;
-    uint64_t T = std::min<uint64_t>(SS, (uint64_t)g_tune.train_points);   // training points
-    if (SS > b.s_cap) {
-        for (void* p : {(void*)b.samp, (void*)b.samp2, (void*)b.sidx, (void*)b.sidx2})
-            if (p) (void)hipFree(p);
-        b.samp = b.samp2 = nullptr; b.sidx = b.sidx2 = nullptr; b.s_cap = 0;
-        if (!galloc((void**)&b.samp, SS * 8) || !galloc((void**)&b.samp2, SS * 8) || !galloc((void**)&b.sidx, SS * 4) ||
-            !galloc((void**)&b.sidx2, SS * 4))
-            return -1;
-        b.s_cap = SS;
-    }
-    if (!ensure_train(b, T)) return -1;
     // Buckets: as few partition passes as keep the average bucket at or
     // below 2^max_log2 rows, then as many buckets as those passes address
     // without going below 2^min_log2 rows on average.
@@ -968,70 +1071,8 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     S.buckets = nb;
 
     hipEventRecord(b.ev[0], st);
-    // 1. sample, sort it, read 1025 quantiles
-    hipLaunchKernelGGL(k_sample, dim3(gridn(SS, 4096)), dim3(256), 0, st, kin, (uint64_t)n, SS, 0x5EEDull, b.samp, b.sidx);
-    if (!hip_ok(hipGetLastError(), "sample")) return -1;
-    if (sort_pairs_u64(b.samp, b.sidx, b.samp2, b.sidx2, SS, c) < 0) return -1;
-    hipLaunchKernelGGL(k_quantiles, dim3(5), dim3(256), 0, st, b.samp2, SS, b.qdev);
-    if (!hip_ok(hipGetLastError(), "quantiles") ||
-        !hip_ok(hipMemcpyAsync(b.qhost, b.qdev, 1025 * 8, hipMemcpyDeviceToHost, st), "read quantiles") ||
-        !hip_ok(hipStreamSynchronize(st), "sync"))
-        return -1;
-    // 2. grid: linear or logarithmic, whichever the sample CDF is straighter on
-    const uint64_t* q = b.qhost;
-    const uint64_t kmin = q[0], kmax = q[1024];
-    const uint64_t span = kmax - kmin;
-    auto logg = [&](uint64_t k) { return (double)TableDigit::grid(k - kmin, 1); };
-    double dev_lin = 0, dev_log = 0;
-    const double glog = logg(kmax);
-    for (int i = 0; i <= 1024; ++i) {
-        double target = i / 1024.0;
-        if (span) dev_lin = std::max(dev_lin, std::fabs((double)(q[i] - kmin) / (double)span - target));
-        if (glog > 0) dev_log = std::max(dev_log, std::fabs(logg(q[i]) / glog - target));
-    }
-    const int logmode = (model == 0 && dev_log < dev_lin) ? 1 : 0;
-    S.log_grid = logmode;
-    const uint64_t gmax_i = logmode ? (uint64_t)glog : span;
-    int gbits = 0;
-    while (gbits < 64 && (gmax_i >> gbits) != 0) ++gbits;
-    const int cshift = gbits > 16 ? gbits - 16 : 0;
-    const uint32_t ncells = (uint32_t)(gmax_i >> cshift) + 1;
-    S.cells = ncells;
-    hipEventRecord(b.ev[1], st);
-    // 3. train the MLP on (grid position, sample rank)
-    if (model == 0) {
-        std::vector<float> P;
-        init_params(P, 0xC0FFEEull);
-        std::memcpy(b.phost, P.data(), kParams * 4);
-        if (!hip_ok(hipMemcpyAsync(b.P, b.phost, kParams * 4, hipMemcpyHostToDevice, st), "copy params")) return -1;
-        hipLaunchKernelGGL(k_train_data, dim3(gridn(T, 1024)), dim3(256), 0, st, b.samp2, SS, T, kmin, kmax, logmode,
-                           (double)gmax_i, b.tx, b.ty);
-        if (!hip_ok(launch_zero32((uint32_t*)b.m, kParams, st), "zero") || !hip_ok(launch_zero32((uint32_t*)b.v, kParams, st), "zero"))
-            return -1;
-        const int steps = g_tune.train_steps;
-        if (steps > 0 && !train_graph(b, T, steps, st)) return -1;
-        S.train_steps = steps;
-    }
-    hipEventRecord(b.ev[2], st);
-    // 4. CDF at the cell starts -> fine-bucket table.  model 0: MLP root routes
-    //    each cell to one of kLeaves linear models fitted to the sample;
-    //    model 1: linear in the key (no model).
-    hipLaunchKernelGGL(k_root_cells, dim3(gridn(ncells + 1, 256)), dim3(256), 0, st, b.P, ncells, cshift,
-                       (double)gmax_i, model, b.R);
-    hipLaunchKernelGGL(k_prefix_max<float>, dim3(1), dim3(1024), 0, st, b.R, ncells + 1);
-    if (model == 0) {
-        hipLaunchKernelGGL(k_leaf_lo, dim3((kLeaves + 256) / 256), dim3(256), 0, st, b.samp2, SS, b.R, ncells, cshift,
-                           kmin, kmax, logmode, kLeaves, b.leaf_lo);
-        hipLaunchKernelGGL(k_leaf_cells, dim3(gridn(ncells, 256)), dim3(256), 0, st, b.samp2, SS, b.R, ncells, cshift,
-                           kmin, kmax, logmode, kLeaves, b.leaf_lo, b.yd);
-    } else {
-        hipLaunchKernelGGL(k_widen, dim3((ncells + 255) / 256), dim3(256), 0, st, b.R, ncells, b.yd);
-    }
-    hipLaunchKernelGGL(k_prefix_max<double>, dim3(1), dim3(1024), 0, st, b.yd, ncells);
-    hipLaunchKernelGGL(k_table, dim3((ncells + 255) / 256), dim3(256), 0, st, b.yd, ncells, nfine, b.cell);
-    if (!hip_ok(hipGetLastError(), "table")) return -1;
-    TableDigit dig{b.cell, kmin, kmax, ncells, cshift, logmode, nfine, std::ldexp(1.0f, -cshift)};
-    hipEventRecord(b.ev[3], st);
+    TableDigit dig;
+    if (!learned_table(b, kin, n, 0, model, nfine, S, dig, c)) return -1;
     // 5. stable partition by bucket id (LSD over its bytes); pass 0 computes
     //    the id once and every pass carries it next to the rows
     const uint32_t* vin = payload();

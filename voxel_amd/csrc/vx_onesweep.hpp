@@ -149,7 +149,7 @@ static __global__ __launch_bounds__(kThreads) void k_range_scan(const uint32_t* 
 // in LDS in digit order and written out so that each digit run is written
 // by consecutive lanes.  COUT writes the digit word of every row (carried
 // from fin, or dig.bits(key)) next to it.
-template <class Digit, bool CIN, bool COUT>
+template <class Digit, bool CIN, bool COUT, bool PAY = true>
 __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                         const uint32_t* __restrict__ fin, uint64_t* __restrict__ kout,
                                                         uint32_t* __restrict__ vout, uint32_t* __restrict__ fout,
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restri
                                                         uint32_t nr, const uint32_t* __restrict__ off,
                                                         const uint32_t* __restrict__ tot) {
     __shared__ uint64_t kb[kTile];
-    __shared__ uint32_t vb[kTile];
+    __shared__ uint32_t vb[PAY ? kTile : 1];
     __shared__ uint32_t fb[(CIN || COUT) ? kTile : 1];   // digit words of the LDS slots
     __shared__ uint32_t wcnt[kWaves][256];          // per-wave digit counters -> wave prefixes
     __shared__ uint32_t lstart[256];                // tile-local start of each digit
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restri
             const uint64_t idx = b0 + r * 64 + lane;
             const bool ok = idx < n;
             k[r] = ok ? kin[idx] : 0;
-            v[r] = ok ? vin[idx] : 0;
+            if constexpr (PAY) v[r] = ok ? vin[idx] : 0;
             if constexpr (CIN) f[r] = ok ? fin[idx] : 0u;
         }
     };
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restri
                 const uint32_t lp = lstart[dd] + wcnt[w][dd] + (rank[r] & 0xFFFFu);
                 if (lp < kTile) {
                     kb[lp] = k[r];
-                    vb[lp] = v[r];
+                    if constexpr (PAY) vb[lp] = v[r];
                     if constexpr (CIN || COUT) fb[lp] = f[r];
                 }
             }
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restri
                 const uint64_t g = (uint64_t)run[dd] + (i - lstart[dd]);
                 if (g < n) {                                  // always true when the counts match
                     kout[g] = kb[i];
-                    vout[g] = vb[i];
+                    if constexpr (PAY) vout[g] = vb[i];
                     if constexpr (COUT) fout[g] = fb[i];
                 }
             }
@@ -310,12 +310,14 @@ hipError_t radix_histogram(const uint64_t* keys, uint64_t n, const Digit& dig, i
 // One stable digit pass; needs w.status of at least radix_work_words(n)
 // words.  fin: carried digit words of kin's rows (byte `pass` is the digit)
 // instead of dig; fout: write the rows' digit words (carried, or
-// dig.bits(key)) next to kout.
+// dig.bits(key)) next to kout.  vin and vout null: keys only (no carried
+// digit words then).
 template <class Digit>
 hipError_t radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
                       const Digit& dig, int pass, RadixWork& w, hipStream_t st, const uint32_t* fin,
                       uint32_t* fout) {
     if (n == 0) return hipSuccess;
+    if ((!vin || !vout) && (vin || vout || fin || fout)) return hipErrorInvalidValue;
     const uint64_t per = radix_tiles_per_range(n), nr = radix_ranges(n);
     uint32_t* cnt = w.status;
     uint32_t* off = w.status + nr * 256;
@@ -326,7 +328,10 @@ hipError_t radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, 
                             pass, per * kTile, (uint32_t)nr, cnt);
     hipLaunchKernelGGL(k_range_scan, dim3(256), dim3(kThreads), 0, st, cnt, (uint32_t)nr, off, tot);
     const dim3 g((unsigned)nr), b(kThreads);
-    if (fin && fout)
+    if (!vin || !vout) {
+        hipLaunchKernelGGL((k_downsweep<Digit, false, false, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n,
+                           dig, pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+    } else if (fin && fout)
         hipLaunchKernelGGL((k_downsweep<Digit, true, true>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig, pass,
                            per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
     else if (fin)
