@@ -3,15 +3,18 @@
 // The reference runs one program per Engine::Run call (exec/engine.hpp:
 // 109-141, one column, one predicate); a batch here is a list of scan
 // descriptors (the voxel_gpu_scan descriptor: column(s), dtype, predicate(s))
-// over columns of any length.  Every query is cut into tiles of tile_rows
-// rows; the tiles of all queries form one global list.  Resident workgroups
-// take tiles from an atomic counter, reduce a tile with the scan's
-// compensated COUNT/SUM/MIN/MAX accumulator (vx_acc.hpp) and store one
-// partial per tile; the workgroup that completes a query's last tile (a
-// per-query pass after the kernel boundary, k_batch_finish) combines each
-// query's partials in tile order.  A query's result therefore depends only
-// on its own rows and tile_rows: it is the same bits alone or in any mix of
-// queries.
+// over columns of any length.  Queries that read the same rows (same
+// column(s), length and element type) are grouped into jobs of up to
+// kBatchShare queries: a job loads each row once and feeds every member's
+// predicate and accumulator (a shared scan).  Every job is cut into tiles of
+// tile_rows rows; the tiles of all jobs form one global list.  Resident
+// workgroups take tiles from an atomic counter, reduce a tile with the
+// scan's compensated COUNT/SUM/MIN/MAX accumulator (vx_acc.hpp), one per
+// member, and store one partial per member and tile; a per-query pass after
+// the kernel boundary (k_batch_finish) combines each query's partials in
+// tile order.  A query's accumulator sees the same rows in the same order
+// whether it shares its job or runs alone, so its result depends only on
+// its own rows and tile_rows: it is the same bits alone or in any mix.
 //
 // Host side: BatchRunner keeps pinned descriptor/result buffers and a graph
 // of four nodes -- descriptor upload, tile kernel, finish kernel, result
@@ -23,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/voxel_c.h"
@@ -38,22 +42,38 @@
 namespace vx {
 namespace {
 
+// Rows [s, e) of a job, tile t of it: every member's predicate(s) and
+// accumulator on each row loaded; member m's partial goes to its own slot
+// tile0 + t.
 template <typename T, int U, bool NT>
-__device__ __forceinline__ void batch_tile(const BatchQuery& q, uint64_t s, uint64_t e, uint64_t tile,
-                                           ScanOut* __restrict__ parts) {
+__device__ __forceinline__ void batch_tile(const BatchJob& job, const BatchQuery* __restrict__ qs, uint64_t s,
+                                           uint64_t e, uint32_t t, ScanOut* __restrict__ parts) {
     using V = typename TyInfo<T>::V;
     constexpr int E = TyInfo<T>::E;
+    constexpr int G = kBatchShare;
     __shared__ Acc<T> lds[kBlock / 64];
-    Pred<T, false> pa; pa.load(q.pa);
-    Pred<T, false> pb; pb.load(q.pb);
-    const bool two = q.mode == 2;
-    const T* __restrict__ a = reinterpret_cast<const T*>(q.a);
-    const T* __restrict__ b = reinterpret_cast<const T*>(two ? q.b : q.a);
-    Acc<T> acc; acc.init();
+    const int nm = (int)job.nmem;
+    Pred<T, false> pa[G], pb[G];
+    Acc<T> acc[G];
+#pragma unroll
+    for (int m = 0; m < G; ++m) {
+        acc[m].init();
+        if (m < nm) {
+            pa[m].load(qs[job.mem[m]].pa);
+            pb[m].load(qs[job.mem[m]].pb);
+        }
+    }
+    const bool two = job.mode == 2;
+    const T* __restrict__ a = reinterpret_cast<const T*>(job.a);
+    const T* __restrict__ b = reinterpret_cast<const T*>(two ? job.b : job.a);
     auto one = [&](T x, T y) {
-        bool p = pa(x);
-        if (two) p = p & pb(y);
-        acc.add(p, x);
+#pragma unroll
+        for (int m = 0; m < G; ++m)
+            if (m < nm) {
+                bool p = pa[m](x);
+                if (two) p = p & pb[m](y);
+                acc[m].add(p, x);
+            }
     };
     const bool aligned = ((reinterpret_cast<uintptr_t>(a + s) | reinterpret_cast<uintptr_t>(b + s)) & 15u) == 0;
     const uint64_t nvec = aligned ? (e - s) / E : 0;
@@ -91,51 +111,57 @@ __device__ __forceinline__ void batch_tile(const BatchQuery& q, uint64_t s, uint
         }
     }
     for (uint64_t i = s + nvec * E + threadIdx.x; i < e; i += kBlock) one(a[i], two ? b[i] : T(0));
-    block_reduce(acc, lds);
-    if (threadIdx.x == 0) acc.store(&parts[tile]);
-    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < G; ++m)
+        if (m < nm) {
+            block_reduce(acc[m], lds);
+            if (threadIdx.x == 0) acc[m].store(&parts[qs[job.mem[m]].tile0 + t]);
+            __syncthreads();
+        }
 }
 
-// Tiles of all queries: resident workgroups take tile ids from ctr[0] and
-// store one partial per tile.  No workgroup waits on or reads another's
-// partial here (that would need an agent-scope release and acquire per tile:
-// an L2 write-back and invalidate on this multi-XCD part); k_batch_finish
-// combines them after the kernel boundary.
+// Tiles of all jobs: resident workgroups take tile ids from ctr[0] and
+// store one partial per member and tile.  No workgroup waits on or reads
+// another's partial here (that would need an agent-scope release and
+// acquire per tile: an L2 write-back and invalidate on this multi-XCD part);
+// k_batch_finish combines them after the kernel boundary.
 template <int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchHeader* __restrict__ hdr, const BatchQuery* __restrict__ qs,
                                                   uint32_t* __restrict__ ctr) {
     __shared__ uint32_t s_tile;
-    const uint32_t total = hdr->total_tiles, nq = hdr->nq, tile_rows = hdr->tile_rows;
+    const uint32_t total = hdr->total_tiles, nj = hdr->njobs, tile_rows = hdr->tile_rows;
+    const BatchJob* __restrict__ jobs = hdr->jobs;
     ScanOut* parts = hdr->parts;
-    uint32_t lo = 0;                                             // query of the block's previous tile
+    uint32_t lo = 0;                                             // job of the block's previous tile
     for (;;) {
         if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const uint32_t tile = s_tile;
         __syncthreads();
         if (tile >= total) break;                               // block-uniform
-        // last query with tile0 <= tile: a block's tiles increase, so step
+        // last job with tile0 <= tile: a block's tiles increase, so step
         // forward from the previous one, and jump by binary search when far
-        if (lo + 1 < nq && qs[lo + 1].tile0 <= tile) {
-            if (lo + 2 < nq && qs[lo + 2].tile0 <= tile) {
-                uint32_t a = lo + 2, z = nq;
+        if (lo + 1 < nj && jobs[lo + 1].tile0 <= tile) {
+            if (lo + 2 < nj && jobs[lo + 2].tile0 <= tile) {
+                uint32_t a = lo + 2, z = nj;
                 while (z - a > 1) {
                     const uint32_t mid = (a + z) >> 1;
-                    if (qs[mid].tile0 <= tile) a = mid; else z = mid;
+                    if (jobs[mid].tile0 <= tile) a = mid; else z = mid;
                 }
                 lo = a;
             } else {
                 lo = lo + 1;
             }
         }
-        const BatchQuery q = qs[lo];
-        const uint64_t s = (uint64_t)(tile - q.tile0) * tile_rows;
-        const uint64_t e = s + tile_rows < q.n ? s + tile_rows : q.n;
-        switch (q.dtype) {
-            case DT_F64: batch_tile<double, U, NT>(q, s, e, tile, parts); break;
-            case DT_F32: batch_tile<float, U, NT>(q, s, e, tile, parts); break;
-            case DT_I64: batch_tile<int64_t, U, NT>(q, s, e, tile, parts); break;
-            default: batch_tile<int32_t, U, NT>(q, s, e, tile, parts); break;
+        const BatchJob job = jobs[lo];
+        const uint32_t t = (uint32_t)(tile - job.tile0);
+        const uint64_t s = (uint64_t)t * tile_rows;
+        const uint64_t e = s + tile_rows < job.n ? s + tile_rows : job.n;
+        switch (job.dtype) {
+            case DT_F64: batch_tile<double, U, NT>(job, qs, s, e, t, parts); break;
+            case DT_F32: batch_tile<float, U, NT>(job, qs, s, e, t, parts); break;
+            case DT_I64: batch_tile<int64_t, U, NT>(job, qs, s, e, t, parts); break;
+            default: batch_tile<int32_t, U, NT>(job, qs, s, e, t, parts); break;
         }
     }
 }
@@ -184,7 +210,7 @@ void* batch_kernel(int unroll) {
 
 BatchRunner::BatchRunner(size_t max_queries) : max_q_(max_queries ? max_queries : 1) {
     if (!runtime_ok()) return;
-    const size_t qbytes = sizeof(BatchHeader) + max_q_ * sizeof(BatchQuery);
+    const size_t qbytes = sizeof(BatchHeader) + max_q_ * (sizeof(BatchQuery) + sizeof(BatchJob));
     ok_ = hip_ok(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate") &&
           hip_ok(hipHostMalloc((void**)&hq_, qbytes, hipHostMallocDefault), "hipHostMalloc") &&
           hip_ok(hipMalloc((void**)&dq_, qbytes), "hipMalloc") &&
@@ -244,7 +270,7 @@ bool BatchRunner::build_graph() {
     exec_ = nullptr;
     graph_ = nullptr;
     if (!hip_ok(hipGraphCreate(&graph_, 0), "hipGraphCreate")) return false;
-    const size_t qbytes = sizeof(BatchHeader) + max_q_ * sizeof(BatchQuery);
+    const size_t qbytes = sizeof(BatchHeader) + max_q_ * (sizeof(BatchQuery) + sizeof(BatchJob));
     if (!hip_ok(hipGraphAddMemcpyNode1D(&n_up_, graph_, nullptr, 0, dq_, hq_, qbytes, hipMemcpyHostToDevice),
                 "graph upload node"))
         return false;
@@ -279,42 +305,97 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
     if (nq == 0 || nq > max_q_) { set_error("batch: %zu queries (1..%zu allowed)", nq, max_q_); return false; }
     BatchHeader* h = reinterpret_cast<BatchHeader*>(hq_);
     BatchQuery* qs = reinterpret_cast<BatchQuery*>(hq_ + sizeof(BatchHeader));
-    // tile ids are handed out largest query first, so the last tiles the
-    // workgroups take are the small ones and no long tile runs alone at the
-    // end; results go back to the caller's order (descriptor field orig)
-    order_.resize(nq);
-    for (size_t i = 0; i < nq; ++i) order_[i] = (uint32_t)i;
-    std::stable_sort(order_.begin(), order_.end(), [&](uint32_t x, uint32_t y) { return q[x].n > q[y].n; });
-    uint64_t tiles = 0;
-    for (size_t j = 0; j < nq; ++j) {
-        const size_t i = order_[j];
-        ScanArgs g;
-        if (!make_scan_args(&q[i], g)) { set_error("batch query %zu: %s", i, last_error()); return false; }
-        BatchQuery& d = qs[j];
-        std::memset(&d, 0, sizeof(d));
-        d.a = g.a;
-        d.b = g.mode == 2 ? g.b : g.a;
-        d.n = q[i].n;
-        d.pa = g.pa;
-        d.pb = g.mode == 2 ? g.pb : g.pa;
-        d.dtype = g.dtype;
-        d.mode = g.mode;
-        d.tile0 = tiles;
-        d.ntiles = (uint32_t)tiles_of(d.n);
-        d.orig = (uint32_t)i;
-        tiles += d.ntiles;
-        dtypes_[i] = g.dtype;
+    // Jobs: queries reading the same rows (column(s), length, element type)
+    // share a scan, up to kBatchShare per job, in the caller's order.  Tile
+    // ids are handed out largest job first, so the last tiles the workgroups
+    // take are the small ones and no long tile runs alone at the end; results
+    // go back to the caller's order (descriptor field orig).
+    std::vector<ScanArgs> args(nq);
+    for (size_t i = 0; i < nq; ++i)
+        if (!make_scan_args(&q[i], args[i])) { set_error("batch query %zu: %s", i, last_error()); return false; }
+    jobs_.clear();
+    std::vector<std::vector<uint32_t>> members;
+    {
+        struct Key {
+            const void* a; const void* b; uint64_t n; int dtype, mode;
+            bool operator==(const Key& o) const {
+                return a == o.a && b == o.b && n == o.n && dtype == o.dtype && mode == o.mode;
+            }
+        };
+        struct KeyHash {
+            size_t operator()(const Key& k) const {
+                return std::hash<const void*>()(k.a) ^ (std::hash<const void*>()(k.b) * 31) ^
+                       (std::hash<uint64_t>()(k.n) * 131) ^ (size_t)(k.dtype * 7 + k.mode);
+            }
+        };
+        std::unordered_map<Key, size_t, KeyHash> open;      // key -> job still taking members
+        for (size_t i = 0; i < nq; ++i) {
+            const ScanArgs& g = args[i];
+            Key k{g.a, g.mode == 2 ? g.b : nullptr, q[i].n, g.dtype, g.mode};
+            auto it = open.find(k);
+            if (it == open.end() || members[it->second].size() == (size_t)kBatchShare) {
+                BatchJob j;
+                std::memset(&j, 0, sizeof(j));
+                j.a = g.a;
+                j.b = g.mode == 2 ? g.b : g.a;
+                j.n = q[i].n;
+                j.dtype = g.dtype;
+                j.mode = g.mode;
+                j.ntiles = (uint32_t)tiles_of(j.n);
+                jobs_.push_back(j);
+                members.emplace_back();
+                open[k] = jobs_.size() - 1;
+                it = open.find(k);
+            }
+            members[it->second].push_back((uint32_t)i);
+        }
     }
-    if (tiles >= (1ull << 32)) { set_error("batch: too many tiles"); return false; }
-    if (!ensure_parts(tiles)) return false;
+    const size_t nj = jobs_.size();
+    order_.resize(nj);
+    for (size_t i = 0; i < nj; ++i) order_[i] = (uint32_t)i;
+    std::stable_sort(order_.begin(), order_.end(), [&](uint32_t x, uint32_t y) { return jobs_[x].n > jobs_[y].n; });
+    BatchJob* jd = reinterpret_cast<BatchJob*>(hq_ + sizeof(BatchHeader) + nq * sizeof(BatchQuery));
+    uint64_t tiles = 0, parts = 0;
+    uint32_t qn = 0;                                    // query records in job order
+    for (size_t jj = 0; jj < nj; ++jj) {
+        BatchJob j = jobs_[order_[jj]];
+        j.tile0 = tiles;
+        tiles += j.ntiles;
+        const std::vector<uint32_t>& mem = members[order_[jj]];
+        j.nmem = (uint32_t)mem.size();
+        for (size_t m = 0; m < mem.size(); ++m) {
+            const uint32_t i = mem[m];
+            const ScanArgs& g = args[i];
+            BatchQuery& d = qs[qn];
+            std::memset(&d, 0, sizeof(d));
+            d.a = g.a;
+            d.b = g.mode == 2 ? g.b : g.a;
+            d.n = q[i].n;
+            d.pa = g.pa;
+            d.pb = g.mode == 2 ? g.pb : g.pa;
+            d.dtype = g.dtype;
+            d.mode = g.mode;
+            d.tile0 = parts;
+            d.ntiles = j.ntiles;
+            d.orig = i;
+            parts += j.ntiles;
+            dtypes_[i] = g.dtype;
+            j.mem[m] = qn++;
+        }
+        jd[jj] = j;
+    }
+    if (tiles >= (1ull << 32) || parts >= (1ull << 32)) { set_error("batch: too many tiles"); return false; }
+    if (!ensure_parts(parts)) return false;
     std::memset(h, 0, sizeof(*h));
     h->nq = (uint32_t)nq;
     h->total_tiles = (uint32_t)tiles;
     h->tile_rows = cfg_.tile_rows;
+    h->njobs = (uint32_t)nj;
     h->parts = parts_;
     h->results = dres_;
+    h->jobs = reinterpret_cast<const BatchJob*>(dq_ + sizeof(BatchHeader) + nq * sizeof(BatchQuery));
     if (!graph_valid_ && !build_graph()) return false;
-    const size_t up = sizeof(BatchHeader) + nq * sizeof(BatchQuery);
+    const size_t up = sizeof(BatchHeader) + nq * sizeof(BatchQuery) + nj * sizeof(BatchJob);
     if (!hip_ok(hipGraphExecMemcpyNodeSetParams1D(exec_, n_up_, dq_, hq_, up, hipMemcpyHostToDevice), "graph upload size") ||
         !hip_ok(hipGraphExecMemcpyNodeSetParams1D(exec_, n_down_, hres_, dres_, nq * sizeof(ScanOut), hipMemcpyDeviceToHost),
                 "graph download size"))

@@ -15,12 +15,17 @@
 
 namespace vx {
 
+// Queries of one batch that read the same rows (same column(s), length and
+// element type) share a scan: one job loads a tile once and feeds up to
+// kBatchShare queries' predicates and accumulators.
+constexpr int kBatchShare = 4;
+
 // Device descriptor of one query (128 bytes).
 struct BatchQuery {
     const void* a;
     const void* b;
     uint64_t n;
-    uint64_t tile0;          // first global tile of the query
+    uint64_t tile0;          // first tile partial of the query
     PredDev pa, pb;
     int32_t dtype, mode;     // mode 0: pa on a; 2: pa on a and pb on b
     uint32_t ntiles, orig;   // tiles; index of the query in the caller's list
@@ -28,12 +33,26 @@ struct BatchQuery {
 };
 static_assert(sizeof(BatchQuery) == 128, "BatchQuery layout");
 
-// Header in front of the descriptors (64 bytes).
+// One shared scan (64 bytes): its rows, tiles and member queries.
+struct BatchJob {
+    const void* a;
+    const void* b;
+    uint64_t n;
+    uint64_t tile0;          // first global tile of the job
+    int32_t dtype, mode;
+    uint32_t ntiles, nmem;
+    uint32_t mem[kBatchShare];   // members: indices into the query records
+};
+static_assert(sizeof(BatchJob) == 64, "BatchJob layout");
+
+// Header in front of the descriptors (64 bytes); the job records follow the
+// nq query records.
 struct BatchHeader {
-    uint32_t nq, total_tiles, tile_rows, pad;
-    ScanOut* parts;          // one partial per tile
+    uint32_t nq, total_tiles, tile_rows, njobs;
+    ScanOut* parts;          // one partial per query and tile
     ScanOut* results;        // one result per query
-    uint64_t pad2[4];
+    const BatchJob* jobs;
+    uint64_t pad2[3];
 };
 static_assert(sizeof(BatchHeader) == 64, "BatchHeader layout");
 
@@ -82,6 +101,7 @@ class BatchRunner {
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<int> dtypes_;
     std::vector<uint32_t> order_;
+    std::vector<BatchJob> jobs_;
     size_t nq_ = 0;
     bool inflight_ = false;
     bool graph_valid_ = false;
