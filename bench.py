@@ -470,7 +470,7 @@ def main():
         for dt_name in ("f64", "f32", "i64", "i32"):
             for sel in (0.01, 0.50, 0.99):
                 d, e = desc_for(dt_name, sel)
-                _, kms = time_scan(d, max(5, args.steps // 2), 2, stream)
+                _, kms = time_scan(d, max(5, args.steps // 2), 2, stream, settle_s=0.2)
                 r = scan.result(d.dtype, stream)
                 gb = 2 * n * e / (kms * 1e-3) / 1e9
                 breakdown["%s_sel%02d" % (dt_name, int(sel * 100))] = {
