@@ -320,6 +320,62 @@ uint64_t vo_tier0_i64(const int64_t* data, uint64_t count, uint64_t off, uint64_
 }
 
 /* ------------------------------------------------------------------------ */
+/* Tier-0 loops with several filters and reductions, Engine<f64/i64>: per   */
+/* kL = 4 group, VLOAD as above, then filters in program order, filter j    */
+/* reading the loaded vector (fsrc[j] < 0) or filter fsrc[j]'s output       */
+/* (VFilterImpl, engine.hpp:1090-1110: lane = pred(lane, t) ? lane : 0;     */
+/* fcmp 7 = never passes), then reduction block b of the loaded vector      */
+/* (bsrc[b] < 0) or a filter's output, folded into acc[b] as above.        */
+/* The handlers are the ones vo_tier0_* restates; registers are distinct.   */
+/* acc[] in/out (register bits); part[] = last group's partials.            */
+/* ------------------------------------------------------------------------ */
+static uint64_t vo_red_f64(const double* v, int red) {
+    if (red == 0) { double s = 0.0; for (int i = 0; i < 4; ++i) s += v[i]; return vo_f64bits(s); }
+    if (red == 1) { uint64_t c = 0; for (int i = 0; i < 4; ++i) if (!isnan(v[i]) && v[i] != 0.0) ++c; return c; }
+    double m = v[0];
+    for (int i = 1; i < 4; ++i)
+        if (red == 2 ? (v[i] < m || isnan(v[i])) : (v[i] > m || isnan(m))) m = v[i];
+    return vo_f64bits(m);
+}
+
+static uint64_t vo_red_i64(const int64_t* v, int red) {
+    if (red == 0) { uint64_t s = 0; for (int i = 0; i < 4; ++i) s += (uint64_t)v[i]; return s; }
+    if (red == 1) { uint64_t c = 0; for (int i = 0; i < 4; ++i) if (v[i] != 0) ++c; return c; }
+    int64_t m = v[0];
+    for (int i = 1; i < 4; ++i)
+        if (red == 2 ? v[i] < m : v[i] > m) m = v[i];
+    return (uint64_t)m;
+}
+
+#define VO_DEF_TIER0_MULTI(NAME, T, RED)                                                    \
+void NAME(const T* data, uint64_t count, uint64_t off, uint64_t total, int vcount,          \
+          int nf, const int* fcmp, const T* fthr, const int* fsrc,                          \
+          int nb, const int* bsrc, const int* red, const int* accop,                        \
+          uint64_t* acc, uint64_t* part) {                                                   \
+    T ld[4], fv[4][4];                                                                       \
+    if (vcount == 0) vcount = 4;                                                             \
+    do {                                                                                     \
+        for (int i = 0; i < 4; ++i) {                                                        \
+            uint64_t idx = off + (uint64_t)i;                                                \
+            ld[i] = (i < vcount && idx < count) ? data[idx] : (T)0;                          \
+        }                                                                                    \
+        for (int j = 0; j < nf; ++j) {                                                       \
+            const T* src = fsrc[j] < 0 ? ld : fv[fsrc[j]];                                   \
+            for (int i = 0; i < 4; ++i)                                                      \
+                fv[j][i] = (fcmp[j] != 7 && VO_PRED(fcmp[j], src[i], fthr[j])) ? src[i] : (T)0; \
+        }                                                                                    \
+        for (int b = 0; b < nb; ++b) {                                                       \
+            const T* v = bsrc[b] < 0 ? ld : fv[bsrc[b]];                                     \
+            part[b] = RED(v, red[b]);                                                        \
+            acc[b] = vo_tier0_accum(acc[b], part[b], accop[b]);                              \
+        }                                                                                    \
+        off += 4;                                                                            \
+    } while (off != total);                                                                  \
+}
+VO_DEF_TIER0_MULTI(vo_tier0_multi_f64, double, vo_red_f64)
+VO_DEF_TIER0_MULTI(vo_tier0_multi_i64, int64_t, vo_red_i64)
+
+/* ------------------------------------------------------------------------ */
 /* JIT fused loop, codegen/x86_64.hpp:1632-1716, as entered by voxel_jit_run */
 /* (bindings/voxel_c.cpp:80-101): iters = (R2 - off + 7) >> 3; each         */
 /* iteration adds masked x[8k+j] into lane j of accumulator A (j<4) or B;   */

@@ -93,6 +93,33 @@ def test_lowering_classifies_reference_programs(golden):
     assert lowered(bad) == 0
 
 
+def test_lowering_accepts_filter_chains_and_several_reductions():
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(5)
+    for trial in range(30):
+        code, spec = tg.make_program(rng, trial % 2 == 0, allow_ne=True)
+        assert lowered(list(code), i64=trial % 2 == 1) == 2, spec
+    W = tg.w
+    base = [W(0x10, rd=4, imm=4), W(0x70, rd=0, ra=1)]
+    tail = [W(0x20, rd=1, ra=1, rb=4), W(0x40, ra=1, rb=2)]
+    def loop(body):
+        words = base + body + tail
+        return words + [W(0x52, imm=(1 - len(words)) & 0xFFF), W(0x01)]
+    f1 = W(0xC5, rd=1, ra=0, rb=3)
+    f2 = W(0xC3, rd=2, ra=1, rb=6)
+    f3 = W(0xC6, rd=3, ra=2, rb=3)
+    red = [W(0xD0, rd=5, ra=2), W(0x2A, rd=0, ra=0, rb=5)]
+    assert lowered(loop([f1, f2] + red)) == 2
+    assert lowered(loop([f1, f2, f3, W(0xD0, rd=5, ra=3), W(0x2A, rd=0, ra=0, rb=5)])) == 0   # three deep
+    assert lowered(loop([f1, W(0xD0, rd=5, ra=7), W(0x2A, rd=0, ra=0, rb=5)])) == 0          # V7 undefined
+    assert lowered(loop([f1, W(0xC3, rd=1, ra=0, rb=6)] + red)) == 0                          # V1 redefined
+    two = [W(0xD0, rd=5, ra=1), W(0x2A, rd=0, ra=0, rb=5), W(0xD7, rd=7, ra=1), W(0x20, rd=0, ra=0, rb=7)]
+    assert lowered(loop([f1] + two)) == 0                                                     # shared accumulator
+    two[3] = W(0x20, rd=8, ra=8, rb=7)
+    assert lowered(loop([f1] + two)) == 2
+    assert lowered(loop([W(0xC5, rd=1, ra=0, rb=5), W(0xD0, rd=5, ra=1), W(0x2A, rd=0, ra=0, rb=5)])) == 0  # threshold written
+
+
 def test_jit_lowering_matches_fusion_detector(golden):
     def ok(code):
         c = np.array(code, np.uint32)

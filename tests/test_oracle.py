@@ -203,6 +203,40 @@ def test_tier0_against_reference_random_programs():
             assert (r, part) == (int(regs[0]), int(regs[5])), (trial, k)
 
 
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_multi_restatement_against_reference_engine(is_f64):
+    """vo_tier0_multi_* (several filters, chained or not, and up to three
+    reductions per loop) against the reference engine running the same
+    words: every accumulator and partial register bit-identical."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(31 if is_f64 else 32)
+    for trial in range(40):
+        n = int(rng.integers(1, 300)) * 4
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 1)
+            x[rng.integers(0, n, 5)] = 0.0
+            if trial % 4 == 0:
+                x[rng.integers(0, n, 3)] = np.nan
+            thr = [float(np.round(rng.standard_normal() * 60, 1)) for _ in range(2)]
+            tbits = [o.f64_bits(t) for t in thr]
+        else:
+            x = rng.integers(-1000, 1000, n).astype(np.int64)
+            x[rng.integers(0, n, 5)] = 0
+            thr = [int(rng.integers(-600, 600)) for _ in range(2)]
+            tbits = [t & (2**64 - 1) for t in thr]
+        code, spec = tg.make_program(rng, is_f64, allow_ne=True)
+        regs = np.zeros(64, np.uint64)
+        regs[2], regs[3], regs[6] = n, tbits[0], tbits[1]
+        run = o.R.vr_engine_run_f64 if is_f64 else o.R.vr_engine_run_i64
+        run(o.ptr(x), n, o.ptr(code), code.size, o.ptr(regs))
+        acc, part = tg.oracle_run(o, x, n, spec, thr, is_f64)
+        for b in range(len(spec["red"])):
+            assert int(regs[tg.ACCS[b]]) == acc[b], (trial, b, spec)
+            assert int(regs[tg.PARTS[b]]) == part[b], (trial, b, spec)
+        assert int(regs[1]) == n
+
+
 def test_jit_restatement(golden, x42):
     g = golden["seed42"]
     r = o.O.vo_jit_f64(o.ptr(x42), x42.size, x42.size, 0, 0xC5, 500.0)

@@ -201,6 +201,76 @@ def test_tier0_random_programs_against_oracle(gpu, golden, seed):
             assert e.get_scalar(0) == int(r), name
 
 
+def _block_vector(x, spec, thr, b):
+    """The reduced vector of block b: filters of its chain applied in order."""
+    ops = {0: np.equal, 1: np.not_equal, 2: np.less, 3: np.less_equal, 4: np.greater, 5: np.greater_equal}
+    chain = []
+    j = spec["bsrc"][b]
+    while j >= 0:
+        chain.append(j)
+        j = spec["fsrc"][j]
+    v = x.copy()
+    with np.errstate(invalid="ignore"):
+        for j in reversed(chain):
+            v = np.where(ops[spec["fcmp"][j]](v, thr[j]), v, 0)
+    return v
+
+
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_filter_chains_and_several_reductions(gpu, is_f64):
+    """Loops with up to two chained VFILTERs and up to three reductions
+    (generated words): every partial register, count, min and max equal the
+    oracle's bits (vo_tier0_multi_*, pinned against the reference engine);
+    f64 sums within 2 ulp of the exact sum of the block's vector, i64 sums
+    exact.  A chain joining != with another filter is refused."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(61 if is_f64 else 62)
+    refused = 0
+    for trial in range(24):
+        n = int(rng.integers(1, 2000)) * 4
+        if is_f64:
+            x = rng.standard_normal(n) * 100
+            x[rng.integers(0, n, 7)] = 0.0
+            if trial % 3 == 0:
+                x[rng.integers(0, n, 3)] = np.nan
+            thr = [float(rng.standard_normal() * 60) for _ in range(2)]
+            tb = [o.f64_bits(t) for t in thr]
+            e = vx.EngineF64()
+        else:
+            x = rng.integers(-2**40, 2**40, n).astype(np.int64)
+            x[rng.integers(0, n, 7)] = 0
+            thr = [int(rng.integers(-2**39, 2**39)) for _ in range(2)]
+            tb = [t & (2**64 - 1) for t in thr]
+            e = vx.EngineI64()
+        code, spec = tg.make_program(rng, is_f64, allow_ne=True)
+        e.add_segment(x)
+        for r, v in ((2, n), (3, tb[0]), (6, tb[1])):
+            e.set_scalar(r, v)
+        e.load_program([int(c) for c in code])
+        if not tg.mergeable(spec):
+            with pytest.raises(RuntimeError, match="filter chain"):
+                e.run()
+            refused += 1
+            continue
+        e.run()
+        acc, part = tg.oracle_run(o, x, n, spec, thr, is_f64)
+        assert e.get_scalar(1) == n
+        for b, red in enumerate(spec["red"]):
+            assert e.get_scalar(tg.PARTS[b]) == part[b], (trial, b, spec)
+            got = e.get_scalar(tg.ACCS[b])
+            if red == 0 and is_f64:
+                g = o.bits_f64(got)
+                if math.isnan(o.bits_f64(acc[b])):
+                    assert math.isnan(g)
+                else:
+                    v = _block_vector(x, spec, thr, b)
+                    assert close_f64(g, o.exact_sum(v)), (trial, b, g)
+            else:
+                assert got == acc[b], (trial, b, spec)
+    assert refused < 24
+
+
 def test_tier0_minmax_mixed_signs_uses_exact_fold(gpu, golden):
     # values whose bit patterns straddle both wrap points: the GPU falls back
     # to the in-order fold; result must equal the oracle's bits exactly

@@ -1,0 +1,96 @@
+"""Random Tier-0 loop programs with several filters and reductions, for the
+oracle pinning test (against the reference engine in oracle/_ref) and the GPU
+parity test.  Words follow bytecode/instruction.hpp:19-56
+([op:8][rd:4][ra:4][rb:4][imm12:12]) with the opcode values of
+bytecode/opcodes.hpp.
+
+Registers: R1 offset, R2 total, R4 step (= 4 lanes, set by the program),
+R3 / R6 filter thresholds, partials R5 R7 R9 R11, accumulators R0 R8 R10 R12.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FILTER_OPS = {0: 0xC1, 1: 0xC2, 2: 0xC3, 3: 0xC4, 4: 0xC5, 5: 0xC6}   # kCmp code -> VFILTER_xx
+RED_OPS = {0: 0xD0, 1: 0xD7, 2: 0xD5, 3: 0xD6}                      # VSUM VCOUNT VRED_MIN VRED_MAX
+PARTS = (5, 7, 9, 11)
+ACCS = (0, 8, 10, 12)
+THR_REGS = (3, 6)
+
+
+def w(op, rd=0, ra=0, rb=0, imm=0):
+    return (op & 0xFF) | (rd & 0xF) << 8 | (ra & 0xF) << 12 | (rb & 0xF) << 16 | (imm & 0xFFF) << 20
+
+
+def make_program(rng, is_f64=True, allow_ne=False):
+    """Returns (words, spec) where spec describes the filters and blocks in
+    the oracle's terms (vo_tier0_multi_*)."""
+    code = [w(0x10, rd=4, imm=4)]                       # MOV R4, 4
+    vload = len(code)
+    code.append(w(0x70, rd=0, ra=1, imm=0))              # VLOAD V0, R1, segment 0, all lanes
+    nf = int(rng.integers(0, 3))
+    fcmp, fsrc = [], []
+    for j in range(nf):
+        cmp = int(rng.choice([2, 3, 4, 5] + ([1] if allow_ne else [])))
+        src = -1 if j == 0 or rng.random() < 0.3 else 0            # chain onto filter 0 or read V0
+        fcmp.append(cmp)
+        fsrc.append(src)
+        code.append(w(FILTER_OPS[cmp], rd=1 + j, ra=0 if src < 0 else 1 + src, rb=THR_REGS[j]))
+    nb = int(rng.integers(1, 4))
+    bsrc, reds, accops = [], [], []
+    for b in range(nb):
+        src = int(rng.integers(-1, nf)) if nf else -1
+        red = int(rng.integers(0, 4))
+        vec = 0 if src < 0 else 1 + src
+        code.append(w(RED_OPS[red], rd=PARTS[b], ra=vec))
+        if red == 0:
+            accop = 0 if is_f64 else 1
+            code.append(w(0x2A if is_f64 else 0x20, rd=ACCS[b], ra=ACCS[b], rb=PARTS[b]))
+        elif red == 1:
+            accop = 1
+            code.append(w(0x20, rd=ACCS[b], ra=ACCS[b], rb=PARTS[b]))
+        else:
+            accop = 2 if red == 2 else 3
+            code.append(w(0x40, ra=ACCS[b], rb=PARTS[b]))                       # CMP acc, part
+            code.append(w(0x5A if red == 2 else 0x5C, imm=2))                    # JLE|JGE +2
+            code.append(w(0x11, rd=ACCS[b], ra=PARTS[b]))                        # MOVR acc, part
+        bsrc.append(src)
+        reds.append(red)
+        accops.append(accop)
+    code.append(w(0x20, rd=1, ra=1, rb=4))               # ADD R1, R1, R4
+    code.append(w(0x40, ra=1, rb=2))                     # CMP R1, R2
+    jnz = len(code)
+    code.append(w(0x52, imm=(vload - jnz) & 0xFFF))      # JNZ back to VLOAD
+    code.append(w(0x01))                                 # HALT
+    spec = {"fcmp": fcmp, "fsrc": fsrc, "bsrc": bsrc, "red": reds, "accop": accops}
+    return np.array(code, np.uint32), spec
+
+
+def oracle_run(o, x, total, spec, thr, is_f64=True):
+    """vo_tier0_multi_* on the spec; returns (acc bits, part bits) per block."""
+    nf, nb = len(spec["fcmp"]), len(spec["red"])
+    fcmp = np.array(spec["fcmp"] + [0], np.int32)
+    fsrc = np.array(spec["fsrc"] + [0], np.int32)
+    fthr = np.array(list(thr[:nf]) + [0], np.float64 if is_f64 else np.int64)
+    bsrc = np.array(spec["bsrc"], np.int32)
+    red = np.array(spec["red"], np.int32)
+    accop = np.array(spec["accop"], np.int32)
+    acc = np.zeros(nb, np.uint64)
+    part = np.zeros(nb, np.uint64)
+    fn = o.O.vo_tier0_multi_f64 if is_f64 else o.O.vo_tier0_multi_i64
+    fn(o.ptr(x), x.size, 0, total, 0, nf, o.ptr(fcmp), o.ptr(fthr), o.ptr(fsrc), nb, o.ptr(bsrc), o.ptr(red),
+       o.ptr(accop), o.ptr(acc), o.ptr(part))
+    return [int(a) for a in acc], [int(p) for p in part]
+
+
+def mergeable(spec):
+    """True when every block's filter chain is one range (no != in a chain of two)."""
+    for b, src in enumerate(spec["bsrc"]):
+        chain = []
+        j = src
+        while j >= 0:
+            chain.append(spec["fcmp"][j])
+            j = spec["fsrc"][j]
+        if len(chain) == 2 and 1 in chain:
+            return False
+    return True

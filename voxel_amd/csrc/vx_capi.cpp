@@ -116,8 +116,69 @@ bool run_fast(Engine& e, StreamCtx* c) {
 
 // ---------------------------------------------------------------------------
 // Tier-0 loop programs (dispatch loop engine.hpp:76-80 over the handlers of
-// VLOAD/VFILTER/VSUM|VCOUNT|VRED_*/ADDF|ADD|CMP-select/ADD/CMP/JNZ).
+// VLOAD/VFILTER/VSUM|VCOUNT|VRED_*/ADDF|ADD|CMP-select/ADD/CMP/JNZ).  A
+// chain of filters zeroes a lane unless every filter passes (VFilterImpl,
+// engine.hpp:1090-1110, applied to a lane already zeroed keeps it zero), so
+// a reduced vector is "conjunction of its filters ? x : 0"; each reduction
+// block is run as its own scan or group pass.
 // ---------------------------------------------------------------------------
+namespace {
+bool run_tier0_block(const Plan::Block& b, const PredDev& pred, int dtype, const void* d, uint64_t count,
+                     uint64_t off, uint64_t dend, uint64_t groups, uint64_t* regs, StreamCtx* c) {
+    const uint64_t kl = 4;
+    const bool f = dtype == DT_F64;
+    const bool okpair = (b.red == 0 && b.accop == 0 && f) || (b.red == 0 && b.accop == 1 && !f) ||
+                        (b.red == 1 && b.accop == 1) || (b.red >= 2 && b.accop >= 2);
+    if (!okpair) { set_error("loop (reduction %u, accumulate %u) on this engine type is not lowered", b.red, b.accop); return false; }
+    GroupArgs ga{};
+    ga.dtype = dtype; ga.a = d; ga.count = count; ga.off = off; ga.groups = groups; ga.p = pred;
+    ga.use_table = 0; ga.red = b.red; ga.fold = b.accop; ga.acc0 = regs[b.acc];
+
+    uint64_t accb = regs[b.acc], last = 0;
+    if (b.red <= 1) {
+        ScanArgs g{};
+        g.dtype = dtype; g.a = d; g.begin = off < dend ? off : dend; g.end = dend;
+        g.pa = pred;
+        if (b.red == 1) { g.mode = 1; g.pb = make_nonzero_pred(dtype); g.table_b = 1; }
+        ScanOut r{};
+        if (g.end > g.begin) {
+            if (!hip_ok(launch_scan(g, c->scr, &c->dev->scan, c->stream), "scan launch")) return false;
+        } else {
+            std::memset(&c->host->scan, 0, sizeof(ScanOut));
+        }
+        ga.off = off + (groups - 1) * kl;
+        if (!hip_ok(launch_group_last(ga, &c->dev->last, c->stream), "last-group launch")) return false;
+        if (!sync_read(c, c->host, c->dev, sizeof(ResultSlab))) return false;
+        if (g.end > g.begin) r = c->host->scan;
+        last = c->host->last;
+        if (b.red == 0 && f) accb = f64_bits(finish_sum_f64(bits_f64(accb), bits_f64(r.s), bits_f64(r.c)));
+        else if (b.red == 0) accb = accb + r.s;
+        else accb = accb + r.cnt;
+    } else {
+        if (!hip_ok(launch_group_minmax(ga, c->scr, &c->dev->group, c->stream), "group launch")) return false;
+        if (!sync_read(c, &c->host->group, &c->dev->group, sizeof(GroupOut))) return false;
+        GroupOut go = c->host->group;
+        last = go.last;
+        const uint64_t half = 1ull << 63;
+        if ((uint64_t)go.smax - (uint64_t)go.smin < half) {      // unsigned: no signed overflow
+            // every value lies in a signed window narrower than 2^63: the
+            // wrapping CMP difference is the signed order
+            accb = b.accop == 2 ? (uint64_t)go.smin : (uint64_t)go.smax;
+        } else if (go.umax - go.umin < half) {
+            accb = b.accop == 2 ? go.umin : go.umax;
+        } else {
+            if (!hip_ok(launch_group_fold(ga, c->dev->fold, c->stream), "fold launch")) return false;
+            if (!sync_read(c, c->host->fold, c->dev->fold, 16)) return false;
+            accb = c->host->fold[0];
+            last = c->host->fold[1];
+        }
+    }
+    regs[b.acc] = accb;
+    regs[b.part] = last;
+    return true;
+}
+}  // namespace
+
 bool run_tier0(Engine& e, StreamCtx* c) {
     const Plan& p = e.plan;
     if (e.dtype != DT_F64 && e.dtype != DT_I64) { set_error("Tier-0 loops are lowered for f64 and i64 engines only"); return false; }
@@ -135,66 +196,33 @@ bool run_tier0(Engine& e, StreamCtx* c) {
                   "(JNZ loops until offset == total)", p.tot, p.off, (long long)(total - off), (unsigned long long)kl);
         return false;
     }
-    // accepted (reduction, accumulate) pairs
-    bool f = e.dtype == DT_F64;
-    bool okpair = (p.red == 0 && p.accop == 0 && f) || (p.red == 0 && p.accop == 1 && !f) ||
-                  (p.red == 1 && p.accop == 1) || (p.red >= 2 && p.accop >= 2);
-    if (!okpair) { set_error("loop (reduction %u, accumulate %u) on this engine type is not lowered", p.red, p.accop); return false; }
-
+    // each block's predicate: the conjunction of its filter chain
+    std::vector<PredDev> preds(p.blocks.size());
+    for (size_t k = 0; k < p.blocks.size(); ++k) {
+        const Plan::Block& b = p.blocks[k];
+        PredDev pr = make_pred(e.dtype, CMP_TRUE, 0);
+        for (int j = 0; j < b.nf; ++j) {
+            const Plan::Filt& fl = p.filts[b.f[j]];
+            const PredDev q = fl.cmp == 7 ? make_never_pred(e.dtype) : make_pred(e.dtype, fl.cmp, regs[fl.thr]);
+            if (j == 0) { pr = q; continue; }
+            PredDev m;
+            if (!merge_preds(e.dtype, pr, q, &m)) {
+                set_error("a filter chain with a != filter and another filter is not lowered");
+                return false;
+            }
+            pr = m;
+        }
+        pr.table = 0;
+        preds[k] = pr;
+    }
     Seg& s = *e.segs[p.seg];
     uint64_t dend = total < s.count ? total : s.count;    // rows past Count load as 0
     const void* d = seg_rows(s, e.dtype, off < dend ? off : dend, dend, c);
     if (!d) return false;
-    uint64_t groups = (total - off) / kl;
-    uint64_t thrb = p.thr != 0xFF ? regs[p.thr] : 0;
-    PredDev pred = p.cmp == 6 ? make_pred(e.dtype, CMP_TRUE, 0) : p.cmp == 7 ? make_never_pred(e.dtype)
-                                                                             : make_pred(e.dtype, p.cmp, thrb);
-    GroupArgs ga{};
-    ga.dtype = e.dtype; ga.a = d; ga.count = s.count; ga.off = off; ga.groups = groups; ga.p = pred;
-    ga.use_table = 0; ga.red = p.red; ga.fold = p.accop; ga.acc0 = regs[p.acc];
-
-    uint64_t accb = regs[p.acc], last = 0;
-    if (p.red <= 1) {
-        ScanArgs g{};
-        g.dtype = e.dtype; g.a = d; g.begin = off < dend ? off : dend; g.end = dend;
-        g.pa = pred;
-        if (p.red == 1) { g.mode = 1; g.pb = make_nonzero_pred(e.dtype); g.table_b = 1; }
-        ScanOut r{};
-        if (g.end > g.begin) {
-            if (!hip_ok(launch_scan(g, c->scr, &c->dev->scan, c->stream), "scan launch")) return false;
-        } else {
-            std::memset(&c->host->scan, 0, sizeof(ScanOut));
-        }
-        ga.off = off + (groups - 1) * kl;
-        if (!hip_ok(launch_group_last(ga, &c->dev->last, c->stream), "last-group launch")) return false;
-        if (!sync_read(c, c->host, c->dev, sizeof(ResultSlab))) return false;
-        if (g.end > g.begin) r = c->host->scan;
-        last = c->host->last;
-        if (p.red == 0 && f) accb = f64_bits(finish_sum_f64(bits_f64(accb), bits_f64(r.s), bits_f64(r.c)));
-        else if (p.red == 0) accb = accb + r.s;
-        else accb = accb + r.cnt;
-    } else {
-        if (!hip_ok(launch_group_minmax(ga, c->scr, &c->dev->group, c->stream), "group launch")) return false;
-        if (!sync_read(c, &c->host->group, &c->dev->group, sizeof(GroupOut))) return false;
-        GroupOut go = c->host->group;
-        last = go.last;
-        const uint64_t half = 1ull << 63;
-        if ((uint64_t)go.smax - (uint64_t)go.smin < half) {      // unsigned: no signed overflow
-            // every value lies in a signed window narrower than 2^63: the
-            // wrapping CMP difference is the signed order
-            accb = p.accop == 2 ? (uint64_t)go.smin : (uint64_t)go.smax;
-        } else if (go.umax - go.umin < half) {
-            accb = p.accop == 2 ? go.umin : go.umax;
-        } else {
-            if (!hip_ok(launch_group_fold(ga, c->dev->fold, c->stream), "fold launch")) return false;
-            if (!sync_read(c, c->host->fold, c->dev->fold, 16)) return false;
-            accb = c->host->fold[0];
-            last = c->host->fold[1];
-        }
-    }
-    regs[p.acc] = accb;
+    const uint64_t groups = (total - off) / kl;
+    for (size_t k = 0; k < p.blocks.size(); ++k)
+        if (!run_tier0_block(p.blocks[k], preds[k], e.dtype, d, s.count, off, dend, groups, regs, c)) return false;
     regs[p.off] = total;
-    regs[p.part] = last;
     std::memcpy(e.regs, regs, sizeof(regs));
     return true;
 }

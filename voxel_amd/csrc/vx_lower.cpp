@@ -48,66 +48,90 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     Plan p;
     if (detect_fast_path(c, len, p)) return p;
 
-    // Tier-0 loop: prefix; VLOAD; [VFILTER*]; reduce; acc update; ADD off; CMP off,tot; JNZ; [HALT]
+    // Tier-0 loop: prefix; VLOAD; then filters (each reading the loaded
+    // vector or a filter's output, at most two deep) and reduction blocks
+    // (VSUM|VCOUNT|VRED_* of any of those vectors, then ADDF|ADD or
+    // CMP/JLE|JGE/MOVR into an accumulator); ADD off; CMP off,tot; JNZ; [HALT]
     size_t i = 0;
     while (i < len && is_prefix_op(op_of(c[i]))) p.prefix.push_back(c[i++]);
     const size_t L = i;
     if (i >= len || op_of(c[i]) != OP_VLOAD) return none("no VLOAD loop: program is not a lowerable scan loop");
     uint32_t vl = c[i++];
-    uint8_t vA = rd_of(vl), rOff = ra_of(vl);
+    const uint8_t vA = rd_of(vl), rOff = ra_of(vl);
     p.seg = (imm_of(vl) >> 8) & 0xF;
     p.vcount = imm_of(vl) & 0xFF;
     p.off = rOff;
 
-    uint8_t vB = vA;
-    p.cmp = 6;
-    p.thr = 0xFF;
-    if (i < len && op_of(c[i]) >= OP_VFILTER && op_of(c[i]) <= OP_VFILTER_GE) {
-        uint32_t f = c[i++];
-        if (ra_of(f) != vA) return none("VFILTER does not read the loaded vector");
-        if (op_of(f) == OP_VFILTER) {
-            int mode = imm_of(f) & 0x7;           // HandleVFILTER, engine.hpp:2870-2894
-            p.cmp = mode <= 5 ? mode : 7;
-        } else {
-            p.cmp = cmp_of_filter_op(op_of(f));
+    // vector register -> filters applied to it (-1: not defined in the loop)
+    int chain_len[16];
+    uint8_t chain[16][2];
+    for (int v = 0; v < 16; ++v) chain_len[v] = -1;
+    chain_len[vA] = 0;
+    while (i < len) {
+        const uint32_t w = c[i];
+        const uint8_t op = op_of(w);
+        if (op >= OP_VFILTER && op <= OP_VFILTER_GE) {
+            const uint8_t src = ra_of(w), dst = rd_of(w);
+            if (chain_len[src] < 0) return none("VFILTER reads a vector the loop does not define");
+            if (chain_len[dst] >= 0) return none("VFILTER overwrites a vector the loop already defines");
+            if (chain_len[src] >= 2) return none("more than two chained VFILTERs");
+            if (p.filts.size() >= 4) return none("more than four VFILTERs");
+            Plan::Filt f;
+            if (op == OP_VFILTER) {
+                int mode = imm_of(w) & 0x7;        // HandleVFILTER, engine.hpp:2870-2894
+                f.cmp = mode <= 5 ? mode : 7;
+            } else {
+                f.cmp = cmp_of_filter_op(op);
+            }
+            f.thr = rb_of(w);
+            chain_len[dst] = chain_len[src] + 1;
+            for (int k = 0; k < chain_len[src]; ++k) chain[dst][k] = chain[src][k];
+            chain[dst][chain_len[src]] = (uint8_t)p.filts.size();
+            p.filts.push_back(f);
+            ++i;
+            continue;
         }
-        vB = rd_of(f);
-        p.thr = rb_of(f);
+        uint8_t red;
+        if (op == OP_VSUM) red = 0;
+        else if (op == OP_VCOUNT) red = 1;
+        else if (op == OP_VRED_MIN) red = 2;
+        else if (op == OP_VRED_MAX) red = 3;
+        else break;
+        const uint8_t src = ra_of(w), rP = rd_of(w);
+        if (chain_len[src] < 0) return none("reduction reads a vector the loop does not define");
+        if (p.blocks.size() >= 4) return none("more than four reductions in the loop");
+        Plan::Block b;
+        b.red = red;
+        b.part = rP;
+        b.nf = (uint8_t)chain_len[src];
+        b.f[0] = chain_len[src] > 0 ? chain[src][0] : 0;
+        b.f[1] = chain_len[src] > 1 ? chain[src][1] : 0;
+        ++i;
+        if (i >= len) return none("loop body ends early");
+        const uint32_t aw = c[i];
+        if (op_of(aw) == OP_ADDF || op_of(aw) == OP_ADD) {
+            b.acc = rd_of(aw);
+            bool ok = (ra_of(aw) == b.acc && rb_of(aw) == rP) || (rb_of(aw) == b.acc && ra_of(aw) == rP);
+            if (!ok) return none("accumulate is not acc = acc + part");
+            b.accop = op_of(aw) == OP_ADDF ? 0 : 1;
+            i += 1;
+        } else if (op_of(aw) == OP_CMP && i + 2 < len) {
+            const uint32_t jw = c[i + 1], mw = c[i + 2];
+            b.acc = ra_of(aw);
+            bool ok = rb_of(aw) == rP && (op_of(jw) == OP_JLE || op_of(jw) == OP_JGE) && simm_of(jw) == 2 &&
+                      op_of(mw) == OP_MOVR && rd_of(mw) == b.acc && ra_of(mw) == rP;
+            if (!ok) return none("compare-select accumulate is not CMP acc,part; JLE|JGE +2; MOVR acc,part");
+            b.accop = op_of(jw) == OP_JLE ? 2 : 3;
+            i += 3;
+        } else {
+            return none("loop accumulate is not ADDF/ADD/CMP-select");
+        }
+        p.blocks.push_back(b);
     }
-    if (i >= len) return none("loop body ends early");
-    uint32_t rw = c[i++];
-    switch (op_of(rw)) {
-    case OP_VSUM: p.red = 0; break;
-    case OP_VCOUNT: p.red = 1; break;
-    case OP_VRED_MIN: p.red = 2; break;
-    case OP_VRED_MAX: p.red = 3; break;
-    default: return none("loop reduction is not VSUM/VCOUNT/VRED_MIN/VRED_MAX");
+    if (p.blocks.empty()) {
+        if (i >= len) return none("loop body ends early");
+        return none("loop reduction is not VSUM/VCOUNT/VRED_MIN/VRED_MAX");
     }
-    if (ra_of(rw) != vB) return none("reduction does not read the filtered vector");
-    uint8_t rP = rd_of(rw);
-    p.part = rP;
-
-    if (i >= len) return none("loop body ends early");
-    uint32_t aw = c[i];
-    uint8_t rAcc;
-    if (op_of(aw) == OP_ADDF || op_of(aw) == OP_ADD) {
-        rAcc = rd_of(aw);
-        bool ok = (ra_of(aw) == rAcc && rb_of(aw) == rP) || (rb_of(aw) == rAcc && ra_of(aw) == rP);
-        if (!ok) return none("accumulate is not acc = acc + part");
-        p.accop = op_of(aw) == OP_ADDF ? 0 : 1;
-        i += 1;
-    } else if (op_of(aw) == OP_CMP && i + 2 < len) {
-        uint32_t jw = c[i + 1], mw = c[i + 2];
-        rAcc = ra_of(aw);
-        bool ok = rb_of(aw) == rP && (op_of(jw) == OP_JLE || op_of(jw) == OP_JGE) && simm_of(jw) == 2 &&
-                  op_of(mw) == OP_MOVR && rd_of(mw) == rAcc && ra_of(mw) == rP;
-        if (!ok) return none("compare-select accumulate is not CMP acc,part; JLE|JGE +2; MOVR acc,part");
-        p.accop = op_of(jw) == OP_JLE ? 2 : 3;
-        i += 3;
-    } else {
-        return none("loop accumulate is not ADDF/ADD/CMP-select");
-    }
-    p.acc = rAcc;
 
     if (i + 2 >= len) return none("loop control missing");
     uint32_t adw = c[i], cw = c[i + 1], jw = c[i + 2];
@@ -121,6 +145,13 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     if (op_of(jw) != OP_JNZ) return none("loop does not end in JNZ");
     long target = (long)(i + 2) + simm_of(jw);
     if (target < 0 || target > (long)L) return none("loop branch does not return to VLOAD");
+    // registers the loop writes: every block's partial and accumulator, the offset
+    bool wr[16] = {};
+    wr[rOff] = true;
+    for (const Plan::Block& b : p.blocks) {
+        if (wr[b.part] || wr[b.acc] || b.part == b.acc) return none("loop registers overlap");
+        wr[b.part] = wr[b.acc] = true;
+    }
     // Words between the branch target and VLOAD run again each iteration;
     // they must be prefix moves that do not touch loop registers (so running
     // them again changes nothing), e.g. voxel_query's unfiltered loops that
@@ -128,9 +159,8 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     for (long k = target; k < (long)L; ++k) {
         uint32_t w = c[k];
         uint8_t op = op_of(w);
-        auto loopreg = [&](uint8_t r) { return r == rP || r == rAcc || r == rOff; };
-        if (op == OP_MOV && loopreg(rd_of(w))) return none("loop re-runs a move into a loop register");
-        if (op == OP_MOVR && (loopreg(rd_of(w)) || loopreg(ra_of(w)) || rd_of(w) == ra_of(w)))
+        if (op == OP_MOV && wr[rd_of(w)]) return none("loop re-runs a move into a loop register");
+        if (op == OP_MOVR && (wr[rd_of(w)] || wr[ra_of(w)] || rd_of(w) == ra_of(w)))
             return none("loop re-runs a register move that is not idempotent");
     }
     if (target < (long)L) {
@@ -146,10 +176,9 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     p.tot = rTot;
 
     // Registers written by the loop must not be inputs of the loop.
-    auto written = [&](uint8_t r) { return r == rP || r == rAcc || r == rOff; };
-    if (rP == rAcc || rAcc == rOff || rP == rOff) return none("loop registers overlap");
-    if (written(rStep) || written(rTot) || (p.thr != 0xFF && written(p.thr)))
-        return none("loop overwrites its own bound, step or threshold");
+    if (wr[rStep] || wr[rTot]) return none("loop overwrites its own bound, step or threshold");
+    for (const Plan::Filt& f : p.filts)
+        if (f.cmp != 7 && wr[f.thr]) return none("loop overwrites its own bound, step or threshold");
     p.kind = Plan::TIER0;
     return p;
 }

@@ -45,13 +45,24 @@ struct Plan {
     uint8_t seg = 0, thr = 3, acc = 0, off = 1, tot = 2;
     uint8_t fop = 0xC5;
 
-    // TIER0: prefix + loop (voxel_query.py:74-135,155-171 shapes)
+    // TIER0: prefix + loop (voxel_query.py:74-135,155-171 shapes, and their
+    // generalisation: up to two chained VFILTERs, up to four reductions)
     std::vector<uint32_t> prefix;    // MOV/MOVR/NOP words run before the loop
-    int cmp = 6;                     // kCmp code of the filter, 6 none, 7 never
     uint8_t vcount = 0;              // VLOAD count field
-    uint8_t red = 0;                 // 0 VSUM, 1 VCOUNT, 2 VRED_MIN, 3 VRED_MAX
-    uint8_t accop = 0;               // 0 ADDF, 1 ADD, 2 keep-smaller (JLE), 3 keep-larger (JGE)
-    uint8_t part = 5, step = 4;
+    uint8_t step = 4;
+    struct Filt {
+        int cmp;                     // kCmp code, 7 never
+        uint8_t thr;                 // threshold register
+    };
+    struct Block {                   // reduction of one vector + its accumulate
+        uint8_t red;                 // 0 VSUM, 1 VCOUNT, 2 VRED_MIN, 3 VRED_MAX
+        uint8_t accop;               // 0 ADDF, 1 ADD, 2 keep-smaller (JLE), 3 keep-larger (JGE)
+        uint8_t part, acc;           // partial and accumulator registers
+        uint8_t nf;                  // filters applied to the reduced vector (0..2) ...
+        uint8_t f[2];                // ... as indices into filts, in program order
+    };
+    std::vector<Filt> filts;
+    std::vector<Block> blocks;
 };
 
 // Interpreter view: fast path first (exactly DetectFastPath), then the

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <unordered_map>
 
 namespace vx {
@@ -190,6 +191,39 @@ PredDev make_nonzero_pred(int dtype) {
 }
 
 PredDev make_never_pred(int dtype) { return make_pred(dtype, -1, 0); }
+
+namespace {
+template <typename T>
+bool merge_range(const PredDev& a, const PredDev& b, PredDev* out) {
+    const T alo = of_bits<T>(a.lo), ahi = of_bits<T>(a.hi), blo = of_bits<T>(b.lo), bhi = of_bits<T>(b.hi);
+    const bool a_empty = !(alo <= ahi), b_empty = !(blo <= bhi);
+    const bool a_true = a.inv && a_empty, b_true = b.inv && b_empty;
+    if (a_true) { *out = b; return true; }
+    if (b_true) { *out = a; return true; }
+    if (a.inv || b.inv) return false;                 // x != t: the conjunction is no single range
+    PredDev p = a;
+    const T lo = alo < blo ? blo : alo, hi = ahi < bhi ? ahi : bhi;
+    p.lo = bits_of<T>(lo);
+    p.hi = bits_of<T>(hi);
+    p.inv = 0;
+    if (a_empty || b_empty || !(lo <= hi)) p = make_pred(std::is_same<T, double>::value ? DT_F64 :
+                                                         std::is_same<T, float>::value ? DT_F32 :
+                                                         std::is_same<T, int64_t>::value ? DT_I64 : DT_I32, -1, 0);
+    p.table = 0;
+    *out = p;
+    return true;
+}
+}  // namespace
+
+bool merge_preds(int dtype, const PredDev& a, const PredDev& b, PredDev* out) {
+    switch (dtype) {
+    case DT_F64: return merge_range<double>(a, b, out);
+    case DT_F32: return merge_range<float>(a, b, out);
+    case DT_I64: return merge_range<int64_t>(a, b, out);
+    case DT_I32: return merge_range<int32_t>(a, b, out);
+    default: return false;
+    }
+}
 
 double finish_sum_f64(double acc0, double s, double c) {
     if (!std::isfinite(s)) return acc0 + s;

@@ -53,6 +53,9 @@ PredDev make_pred(int dtype, int cmp, uint64_t tbits, bool ordered = false);
 PredDev make_nonzero_pred(int dtype);
 // A predicate that never passes (generic VFILTER modes 6 and 7).
 PredDev make_never_pred(int dtype);
+// The conjunction of two range predicates (non-table form) as one range
+// predicate; false when it is not one (a "!= t" with anything but "true").
+bool merge_preds(int dtype, const PredDev& a, const PredDev& b, PredDev* out);
 
 // Combines a compensated pair with an initial value and rounds to T.
 double finish_sum_f64(double acc0, double s, double c);
