@@ -376,6 +376,67 @@ VO_DEF_TIER0_MULTI(vo_tier0_multi_f64, double, vo_red_f64)
 VO_DEF_TIER0_MULTI(vo_tier0_multi_i64, int64_t, vo_red_i64)
 
 /* ------------------------------------------------------------------------ */
+/* Straight-line scalar words (engine.hpp:1167-1178 MOV/MOVR, 1286-1380     */
+/* integer arithmetic on register bits as i64, 1394-1446 f64 arithmetic,    */
+/* 1457-1505 bitwise; opcodes.hpp:24-60).  Returns 1 when every word ran,    */
+/* 0 when DIV/MOD/DIVF by zero stopped the program (Running_ = false),      */
+/* -1 for INT64_MIN / -1 (a trap in the reference's host) or an opcode      */
+/* outside this set.                                                        */
+/* ------------------------------------------------------------------------ */
+/* x86 SSE/AVX scalar arithmetic returns the first NaN operand, quieted,
+   when an operand is NaN (the reference binary's ADDF..DIVF). */
+static uint64_t vo_f64_op(unsigned op, uint64_t a, uint64_t b) {
+    double fa = vo_bitsf64(a), fb = vo_bitsf64(b), r;
+    if (isnan(fa)) return a | 0x0008000000000000ull;
+    if (isnan(fb)) return b | 0x0008000000000000ull;
+    r = op == 0x2A ? fa + fb : op == 0x2B ? fa - fb : op == 0x2C ? fa * fb : fa / fb;
+    return vo_f64bits(r);
+}
+
+int vo_scalar_run(const uint32_t* w, size_t n, uint64_t* r) {
+    for (size_t k = 0; k < n; ++k) {
+        uint32_t x = w[k];
+        unsigned op = x & 0xFF, rd = (x >> 8) & 0xF, ra = (x >> 12) & 0xF, rb = (x >> 16) & 0xF;
+        int imm = (int)((x >> 20) & 0xFFF);
+        if (imm & 0x800) imm -= 0x1000;
+        uint64_t a = r[ra], b = r[rb];
+        int64_t sa = (int64_t)a, sb = (int64_t)b;
+        double fa = vo_bitsf64(a), fb = vo_bitsf64(b);
+        switch (op) {
+        case 0x00: break;
+        case 0x10: r[rd] = (uint64_t)(int64_t)imm; break;
+        case 0x11: r[rd] = a; break;
+        case 0x20: r[rd] = a + b; break;
+        case 0x21: r[rd] = a - b; break;
+        case 0x22: r[rd] = a * b; break;
+        case 0x23: case 0x24:
+            if (sb == 0) return 0;
+            if (sa == INT64_MIN && sb == -1) return -1;
+            r[rd] = (uint64_t)(op == 0x23 ? sa / sb : sa % sb);
+            break;
+        case 0x25: r[rd] = 0 - a; break;
+        case 0x26: r[rd] = sa < 0 ? 0 - a : a; break;
+        case 0x27: r[rd] = (uint64_t)(sa < sb ? sa : sb); break;
+        case 0x28: r[rd] = (uint64_t)(sa > sb ? sa : sb); break;
+        case 0x29: r[rd] = (uint64_t)((sa >> 1) + (sb >> 1) + ((sa & 1) + (sb & 1)) / 2); break;
+        case 0x2A: case 0x2B: case 0x2C: r[rd] = vo_f64_op(op, a, b); break;
+        case 0x2D: if (fb == 0.0) return 0; r[rd] = vo_f64_op(op, a, b); break;
+        case 0x2E: r[rd] = a ^ 0x8000000000000000ull; break;
+        case 0x2F: r[rd] = a & 0x7FFFFFFFFFFFFFFFull; break;
+        case 0x30: r[rd] = a & b; break;
+        case 0x31: r[rd] = a | b; break;
+        case 0x32: r[rd] = a ^ b; break;
+        case 0x33: r[rd] = ~a; break;
+        case 0x34: r[rd] = a << (b & 0x3F); break;
+        case 0x35: r[rd] = a >> (b & 0x3F); break;
+        case 0x36: r[rd] = (uint64_t)(sa >> (b & 0x3F)); break;
+        default: return -1;
+        }
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
 /* JIT fused loop, codegen/x86_64.hpp:1632-1716, as entered by voxel_jit_run */
 /* (bindings/voxel_c.cpp:80-101): iters = (R2 - off + 7) >> 3; each         */
 /* iteration adds masked x[8k+j] into lane j of accumulator A (j<4) or B;   */

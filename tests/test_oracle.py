@@ -237,6 +237,36 @@ def test_tier0_multi_restatement_against_reference_engine(is_f64):
         assert int(regs[1]) == n
 
 
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_scalar_words_restatement_against_reference_engine():
+    """vo_scalar_run against the reference engine on straight-line programs
+    of integer, f64 and bitwise words (including DIV/MOD/DIVF by zero, which
+    stop the program): all 16 scalar registers bit-identical."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(41)
+    x = np.zeros(4, np.float64)
+    checked = stopped = 0
+    while checked < 300:
+        regs = np.zeros(64, np.uint64)
+        for r in range(16):
+            kind = rng.integers(0, 4)
+            regs[r] = (int(rng.integers(-50, 50)) & (2**64 - 1) if kind == 0 else
+                       o.f64_bits(float(rng.standard_normal() * 1e3)) if kind == 1 else
+                       int(rng.integers(0, 2**63)) if kind == 2 else 0)
+        words = np.array(tg.scalar_words(rng, int(rng.integers(1, 12)), range(16), range(16)) + [tg.w(0x01)],
+                         np.uint32)
+        mine = regs.copy()
+        rc = o.O.vo_scalar_run(o.ptr(words), words.size - 1, o.ptr(mine))
+        if rc < 0:
+            continue                                   # INT64_MIN / -1: the reference traps
+        ref = regs.copy()
+        o.R.vr_engine_run_f64(o.ptr(x), x.size, o.ptr(words), words.size, o.ptr(ref))
+        assert np.array_equal(mine[:16], ref[:16]), [hex(int(v)) for v in words]
+        checked += 1
+        stopped += rc == 0
+    assert stopped > 0
+
+
 def test_jit_restatement(golden, x42):
     g = golden["seed42"]
     r = o.O.vo_jit_f64(o.ptr(x42), x42.size, x42.size, 0, 0xC5, 500.0)

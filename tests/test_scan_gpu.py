@@ -271,6 +271,67 @@ def test_tier0_filter_chains_and_several_reductions(gpu, is_f64):
     assert refused < 24
 
 
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_scalar_prefix_and_epilogue(gpu, is_f64):
+    """Scalar words (integer, f64 and bitwise arithmetic, DIV/MOD/DIVF by
+    zero stopping the program) before and after a multi-reduction loop: all
+    16 scalar registers equal the oracle's (vo_scalar_run, pinned against the
+    reference engine, around vo_tier0_multi_*).  f64 engines use the exact
+    reductions (VCOUNT, VRED_MIN/MAX) so every register compares bit for bit."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(71 if is_f64 else 72)
+    done = stopped = 0
+    while done < 20:
+        n = int(rng.integers(1, 1500)) * 4
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 2)
+            x[rng.integers(0, n, 7)] = 0.0
+            thr = [float(np.round(rng.standard_normal() * 60, 2)) for _ in range(2)]
+            tb = [o.f64_bits(t) for t in thr]
+        else:
+            x = rng.integers(-2**40, 2**40, n).astype(np.int64)
+            thr = [int(rng.integers(-2**39, 2**39)) for _ in range(2)]
+            tb = [t & (2**64 - 1) for t in thr]
+        regs0 = np.zeros(64, np.uint64)
+        for r in (0, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15):
+            regs0[r] = int(rng.integers(-40, 40)) & (2**64 - 1) if rng.random() < 0.5 else o.f64_bits(float(rng.standard_normal()))
+        regs0[1], regs0[2], regs0[3], regs0[6] = 0, n, tb[0], tb[1]
+        pre = tg.scalar_words(rng, int(rng.integers(0, 6)), range(16), tg.SCRATCH + tg.ACCS)
+        epi = tg.scalar_words(rng, int(rng.integers(0, 6)), range(16), range(16))
+        code, spec = tg.make_program(rng, is_f64, reds=(1, 2, 3) if is_f64 else (0, 1, 2, 3), prefix=pre, epilogue=epi)
+        # expected registers
+        exp = regs0.copy()
+        pw = np.array(pre + [0], np.uint32)
+        rc = o.O.vo_scalar_run(o.ptr(pw), len(pre), o.ptr(exp))
+        if rc < 0:
+            continue
+        if rc == 1:
+            exp[4] = 4
+            nb = len(spec["red"])
+            acc0 = [int(exp[tg.ACCS[b]]) for b in range(nb)]
+            t_now = [o.bits_f64(int(exp[3])), o.bits_f64(int(exp[6]))] if is_f64 else \
+                    [int(np.int64(np.uint64(exp[3]))), int(np.int64(np.uint64(exp[6])))]
+            acc, part = tg.oracle_run(o, x, n, spec, t_now, is_f64, acc0=acc0)
+            for b in range(nb):
+                exp[tg.ACCS[b]], exp[tg.PARTS[b]] = acc[b], part[b]
+            exp[1] = n
+            ew = np.array(epi + [0], np.uint32)
+            if o.O.vo_scalar_run(o.ptr(ew), len(epi), o.ptr(exp)) < 0:
+                continue
+        else:
+            stopped += 1
+        e = vx.EngineF64() if is_f64 else vx.EngineI64()
+        e.add_segment(x)
+        for r in range(16):
+            e.set_scalar(r, int(regs0[r]))
+        e.load_program([int(c) for c in code])
+        e.run()
+        got = [e.get_scalar(r) for r in range(16)]
+        assert got == [int(v) for v in exp[:16]], (done, [hex(int(c)) for c in code])
+        done += 1
+
+
 def test_tier0_minmax_mixed_signs_uses_exact_fold(gpu, golden):
     # values whose bit patterns straddle both wrap points: the GPU falls back
     # to the in-order fold; result must equal the oracle's bits exactly

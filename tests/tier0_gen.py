@@ -22,10 +22,26 @@ def w(op, rd=0, ra=0, rb=0, imm=0):
     return (op & 0xFF) | (rd & 0xF) << 8 | (ra & 0xF) << 12 | (rb & 0xF) << 16 | (imm & 0xFFF) << 20
 
 
-def make_program(rng, is_f64=True, allow_ne=False):
+SCALAR_OPS = [0x00, 0x10, 0x11] + list(range(0x20, 0x30)) + list(range(0x30, 0x37))
+SCRATCH = (13, 14, 15)
+
+
+def scalar_words(rng, k, reads, writes):
+    """k random scalar words (NOP MOV MOVR, integer, f64 and bitwise
+    arithmetic) reading registers `reads` and writing `writes`."""
+    out = []
+    for _ in range(k):
+        op = int(rng.choice(SCALAR_OPS))
+        imm = int(rng.integers(-2048, 2048)) if op == 0x10 else 0
+        out.append(w(op, rd=int(rng.choice(writes)), ra=int(rng.choice(reads)), rb=int(rng.choice(reads)), imm=imm))
+    return out
+
+
+def make_program(rng, is_f64=True, allow_ne=False, reds=(0, 1, 2, 3), prefix=(), epilogue=()):
     """Returns (words, spec) where spec describes the filters and blocks in
-    the oracle's terms (vo_tier0_multi_*)."""
-    code = [w(0x10, rd=4, imm=4)]                       # MOV R4, 4
+    the oracle's terms (vo_tier0_multi_*); prefix / epilogue: scalar words
+    placed before the loop / between the loop and HALT."""
+    code = list(prefix) + [w(0x10, rd=4, imm=4)]        # ...; MOV R4, 4
     vload = len(code)
     code.append(w(0x70, rd=0, ra=1, imm=0))              # VLOAD V0, R1, segment 0, all lanes
     nf = int(rng.integers(0, 3))
@@ -37,10 +53,10 @@ def make_program(rng, is_f64=True, allow_ne=False):
         fsrc.append(src)
         code.append(w(FILTER_OPS[cmp], rd=1 + j, ra=0 if src < 0 else 1 + src, rb=THR_REGS[j]))
     nb = int(rng.integers(1, 4))
-    bsrc, reds, accops = [], [], []
+    bsrc, reds_used, accops = [], [], []
     for b in range(nb):
         src = int(rng.integers(-1, nf)) if nf else -1
-        red = int(rng.integers(0, 4))
+        red = int(rng.choice(reds))
         vec = 0 if src < 0 else 1 + src
         code.append(w(RED_OPS[red], rd=PARTS[b], ra=vec))
         if red == 0:
@@ -55,19 +71,21 @@ def make_program(rng, is_f64=True, allow_ne=False):
             code.append(w(0x5A if red == 2 else 0x5C, imm=2))                    # JLE|JGE +2
             code.append(w(0x11, rd=ACCS[b], ra=PARTS[b]))                        # MOVR acc, part
         bsrc.append(src)
-        reds.append(red)
+        reds_used.append(red)
         accops.append(accop)
     code.append(w(0x20, rd=1, ra=1, rb=4))               # ADD R1, R1, R4
     code.append(w(0x40, ra=1, rb=2))                     # CMP R1, R2
     jnz = len(code)
     code.append(w(0x52, imm=(vload - jnz) & 0xFFF))      # JNZ back to VLOAD
+    code += list(epilogue)
     code.append(w(0x01))                                 # HALT
-    spec = {"fcmp": fcmp, "fsrc": fsrc, "bsrc": bsrc, "red": reds, "accop": accops}
+    spec = {"fcmp": fcmp, "fsrc": fsrc, "bsrc": bsrc, "red": reds_used, "accop": accops}
     return np.array(code, np.uint32), spec
 
 
-def oracle_run(o, x, total, spec, thr, is_f64=True):
-    """vo_tier0_multi_* on the spec; returns (acc bits, part bits) per block."""
+def oracle_run(o, x, total, spec, thr, is_f64=True, acc0=None):
+    """vo_tier0_multi_* on the spec; returns (acc bits, part bits) per block
+    (acc0: the accumulators' values before the loop, default 0)."""
     nf, nb = len(spec["fcmp"]), len(spec["red"])
     fcmp = np.array(spec["fcmp"] + [0], np.int32)
     fsrc = np.array(spec["fsrc"] + [0], np.int32)
@@ -75,7 +93,7 @@ def oracle_run(o, x, total, spec, thr, is_f64=True):
     bsrc = np.array(spec["bsrc"], np.int32)
     red = np.array(spec["red"], np.int32)
     accop = np.array(spec["accop"], np.int32)
-    acc = np.zeros(nb, np.uint64)
+    acc = np.array(acc0 if acc0 is not None else [0] * nb, np.uint64)
     part = np.zeros(nb, np.uint64)
     fn = o.O.vo_tier0_multi_f64 if is_f64 else o.O.vo_tier0_multi_i64
     fn(o.ptr(x), x.size, 0, total, 0, nf, o.ptr(fcmp), o.ptr(fthr), o.ptr(fsrc), nb, o.ptr(bsrc), o.ptr(red),

@@ -184,7 +184,12 @@ bool run_tier0(Engine& e, StreamCtx* c) {
     if (e.dtype != DT_F64 && e.dtype != DT_I64) { set_error("Tier-0 loops are lowered for f64 and i64 engines only"); return false; }
     uint64_t regs[64];
     std::memcpy(regs, e.regs, sizeof(regs));
-    run_prefix(p.prefix, regs);
+    const int pre = run_scalar(p.prefix, regs);
+    if (pre < 0) return false;
+    if (pre == 0) {                                       // a division by zero stopped the program
+        std::memcpy(e.regs, regs, sizeof(regs));
+        return true;
+    }
     const uint64_t kl = 4;
     if (regs[p.step] != kl) { set_error("loop step register R%u holds %llu, not the lane count %llu", p.step,
                                         (unsigned long long)regs[p.step], (unsigned long long)kl); return false; }
@@ -223,6 +228,7 @@ bool run_tier0(Engine& e, StreamCtx* c) {
     for (size_t k = 0; k < p.blocks.size(); ++k)
         if (!run_tier0_block(p.blocks[k], preds[k], e.dtype, d, s.count, off, dend, groups, regs, c)) return false;
     regs[p.off] = total;
+    if (run_scalar(p.epilogue, regs) < 0) return false;
     std::memcpy(e.regs, regs, sizeof(regs));
     return true;
 }

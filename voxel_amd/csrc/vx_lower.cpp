@@ -1,6 +1,11 @@
 // vx_lower.cpp -- bytecode recognition (see vx_lower.hpp).
 #include "vx_lower.hpp"
 
+#include <cmath>
+#include <cstring>
+
+#include "vx_runtime.hpp"
+
 namespace vx {
 
 namespace {
@@ -13,6 +18,65 @@ Plan none(const char* why) {
     return p;
 }
 }  // namespace
+
+bool host_op(uint8_t op) {
+    return op == OP_NOP || op == OP_MOV || op == OP_MOVR || (op >= 0x20 && op <= 0x2F) || (op >= 0x30 && op <= 0x36);
+}
+
+int run_scalar(const std::vector<uint32_t>& words, uint64_t* regs) {
+    auto f64 = [](uint64_t b) { double d; std::memcpy(&d, &b, 8); return d; };
+    auto bits = [](double d) { uint64_t b; std::memcpy(&b, &d, 8); return b; };
+    // The reference's x86 scalar arithmetic returns the first NaN operand,
+    // quieted, when an operand is NaN.
+    auto fop = [&](uint8_t op, uint64_t a, uint64_t b) -> uint64_t {
+        const double fa = f64(a), fb = f64(b);
+        if (std::isnan(fa)) return a | 0x0008000000000000ull;
+        if (std::isnan(fb)) return b | 0x0008000000000000ull;
+        return bits(op == 0x2A ? fa + fb : op == 0x2B ? fa - fb : op == 0x2C ? fa * fb : fa / fb);
+    };
+    for (uint32_t w : words) {
+        const uint8_t op = op_of(w), rd = rd_of(w), ra = ra_of(w), rb = rb_of(w);
+        const uint64_t a = regs[ra], b = regs[rb];
+        const int64_t sa = (int64_t)a, sb = (int64_t)b;
+        switch (op) {
+        case OP_NOP: break;
+        case OP_MOV: regs[rd] = (uint64_t)(int64_t)simm_of(w); break;       // HandleMOV: sign-extended imm12
+        case OP_MOVR: regs[rd] = a; break;
+        case 0x20: regs[rd] = a + b; break;                                  // ADD (wrapping i64)
+        case 0x21: regs[rd] = a - b; break;                                  // SUB
+        case 0x22: regs[rd] = a * b; break;                                  // MUL
+        case 0x23:                                                           // DIV
+        case 0x24:                                                           // MOD
+            if (sb == 0) return 0;
+            if (sa == INT64_MIN && sb == -1) { set_error("%s", "integer division INT64_MIN / -1 traps in the reference"); return -1; }
+            regs[rd] = (uint64_t)(op == 0x23 ? sa / sb : sa % sb);
+            break;
+        case 0x25: regs[rd] = 0 - a; break;                                  // NEG
+        case 0x26: regs[rd] = sa < 0 ? 0 - a : a; break;                     // ABS
+        case 0x27: regs[rd] = (uint64_t)(sa < sb ? sa : sb); break;          // MIN
+        case 0x28: regs[rd] = (uint64_t)(sa > sb ? sa : sb); break;          // MAX
+        case 0x29:                                                           // AVG
+            regs[rd] = (uint64_t)((sa >> 1) + (sb >> 1) + ((sa & 1) + (sb & 1)) / 2);
+            break;
+        case 0x2A: case 0x2B: case 0x2C: regs[rd] = fop(op, a, b); break;   // ADDF SUBF MULF
+        case 0x2D:                                                           // DIVF
+            if (f64(b) == 0.0) return 0;
+            regs[rd] = fop(op, a, b);
+            break;
+        case 0x2E: regs[rd] = a ^ 0x8000000000000000ull; break;              // NEGF: sign flip
+        case 0x2F: regs[rd] = a & 0x7FFFFFFFFFFFFFFFull; break;              // ABSF: sign clear
+        case 0x30: regs[rd] = a & b; break;                                  // AND
+        case 0x31: regs[rd] = a | b; break;                                  // OR
+        case 0x32: regs[rd] = a ^ b; break;                                  // XOR
+        case 0x33: regs[rd] = ~a; break;                                     // NOT
+        case 0x34: regs[rd] = a << (b & 0x3F); break;                        // SHL
+        case 0x35: regs[rd] = a >> (b & 0x3F); break;                        // SHR
+        case 0x36: regs[rd] = (uint64_t)(sa >> (b & 0x3F)); break;           // SAR
+        default: set_error("%s", "word is not a scalar instruction"); return -1;
+        }
+    }
+    return 1;
+}
 
 void run_prefix(const std::vector<uint32_t>& prefix, uint64_t* regs) {
     for (uint32_t w : prefix) {
@@ -53,7 +117,7 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     // (VSUM|VCOUNT|VRED_* of any of those vectors, then ADDF|ADD or
     // CMP/JLE|JGE/MOVR into an accumulator); ADD off; CMP off,tot; JNZ; [HALT]
     size_t i = 0;
-    while (i < len && is_prefix_op(op_of(c[i]))) p.prefix.push_back(c[i++]);
+    while (i < len && host_op(op_of(c[i]))) p.prefix.push_back(c[i++]);
     const size_t L = i;
     if (i >= len || op_of(c[i]) != OP_VLOAD) return none("no VLOAD loop: program is not a lowerable scan loop");
     uint32_t vl = c[i++];
@@ -159,6 +223,7 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     for (long k = target; k < (long)L; ++k) {
         uint32_t w = c[k];
         uint8_t op = op_of(w);
+        if (op != OP_MOV && op != OP_MOVR && op != OP_NOP) return none("loop re-runs a scalar instruction that is not a move");
         if (op == OP_MOV && wr[rd_of(w)]) return none("loop re-runs a move into a loop register");
         if (op == OP_MOVR && (wr[rd_of(w)] || wr[ra_of(w)] || rd_of(w) == ra_of(w)))
             return none("loop re-runs a register move that is not idempotent");
@@ -171,6 +236,7 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
                     return none("loop re-runs a register move that is not idempotent");
     }
     i += 3;
+    while (i < len && host_op(op_of(c[i]))) p.epilogue.push_back(c[i++]);
     if (i < len && op_of(c[i]) != OP_HALT) return none("instructions after the loop are not lowerable");
     p.step = rStep;
     p.tot = rTot;

@@ -47,7 +47,8 @@ struct Plan {
 
     // TIER0: prefix + loop (voxel_query.py:74-135,155-171 shapes, and their
     // generalisation: up to two chained VFILTERs, up to four reductions)
-    std::vector<uint32_t> prefix;    // MOV/MOVR/NOP words run before the loop
+    std::vector<uint32_t> prefix;    // scalar words run before the loop (host_op)
+    std::vector<uint32_t> epilogue;  // scalar words run after it, up to HALT
     uint8_t vcount = 0;              // VLOAD count field
     uint8_t step = 4;
     struct Filt {
@@ -83,5 +84,18 @@ JitPlan lower_jit_program(const uint32_t* code, size_t len);
 // Applies MOV/MOVR/NOP words to a register file (HandleMOV/HandleMOVR,
 // engine.hpp:1167-1178).
 void run_prefix(const std::vector<uint32_t>& prefix, uint64_t* regs);
+
+// Scalar words a lowered Tier-0 program may run before or after its loop:
+// NOP, MOV, MOVR, ADD SUB MUL DIV MOD NEG ABS MIN MAX AVG, ADDF SUBF MULF
+// DIVF NEGF ABSF, AND OR XOR NOT SHL SHR SAR (engine.hpp:1167-1178,
+// 1286-1505; opcodes.hpp:24-60).  Condition flags are not kept: no lowered
+// program reads them outside its loop.
+bool host_op(uint8_t op);
+
+// Runs host_op words in order on a register file.  Returns 1 when all ran,
+// 0 when a DIV/MOD/DIVF by zero stopped the program (the reference clears
+// Running_ there and runs nothing further), -1 (error set) on a division the
+// reference's host would trap on (INT64_MIN / -1).
+int run_scalar(const std::vector<uint32_t>& words, uint64_t* regs);
 
 }  // namespace vx
