@@ -191,7 +191,8 @@ int       voxel_gpu_jit_lowerable(const uint32_t* code, size_t len);
 double    voxel_gpu_jit_run_r2(const uint32_t* code, size_t code_len, const double* data,
                                size_t data_len, uint64_t r2, double threshold);
 /* How load_program lowered the engine's program: 1 fast path, 2 Tier-0
- * loop, 0 not lowerable (run will fail with the reason). */
+ * loop, 3 straight-line scalar and segment-aggregate (AGG_*) words, 0 not
+ * lowerable (run will fail with the reason). */
 int       voxel_gpu_engine_lowered(const void* engine);
 
 /* Stable sort of (u64 key, u32 payload) pairs by key (RadixSort<u64>::Sort

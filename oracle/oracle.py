@@ -61,6 +61,9 @@ _SIGS = {
     "vo_tier0_multi_f64": (None, [P, U64, U64, U64, INT, INT, P, P, P, INT, P, P, P, P, P]),
     "vo_tier0_multi_i64": (None, [P, U64, U64, U64, INT, INT, P, P, P, INT, P, P, P, P, P]),
     "vo_scalar_run": (INT, [P, SZ, P]),
+    "vo_agg_create": (P, [INT]),
+    "vo_agg_destroy": (None, [P]),
+    "vo_straight_run": (INT, [P, P, U64, P, SZ, P]),
     "vo_jit_f64": (F64, [P, U64, U64, U64, U8, F64]),
     "vo_run_parallel_fast_f64": (F64, [P, U64, U32, U64, U64, F64, F64, U8]),
     "vo_bitmap_f64": (U64, [P, SZ, INT, F64, F64, P]),

@@ -437,6 +437,209 @@ int vo_scalar_run(const uint32_t* w, size_t n, uint64_t* r) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Segment aggregates AGG_COUNT..AGG_PERCENTILE (engine.hpp:499-698,        */
+/* handlers 3229-3283) on Engine<f64/i64>, with the engine's 16 AggState    */
+/* slots (engine.hpp:347-372), inside straight-line programs of scalar      */
+/* words (vo_scalar_run) and AGG words.  AGG word: rd = slot and result     */
+/* register, ra = segment (one here: 0), rb = register holding the start    */
+/* row, imm12 = row count (0: all; AGG_PERCENTILE also reads the percent    */
+/* from imm12 & 0xFF).  Sums and Welford run in row order as in the         */
+/* reference.  The distinct set is a sorted array (its size is what the     */
+/* reference returns).  AGG_SUM_DISTINCT and AGG_MODE are not restated:     */
+/* their results depend on unordered_map iteration order.                   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t count; double sumf; int64_t sumi;
+    uint64_t minv, maxv, firstv, lastv;          /* T bits */
+    double m2, mean;
+    int init, first_set, has_set;
+    uint64_t* set; size_t nset, capset;          /* sorted distinct T bits (f64: +0 for -0) */
+} vo_aggslot;
+
+typedef struct { vo_aggslot s[16]; int is_f64; } vo_aggstate;
+
+static void vo_slot_reset(vo_aggslot* a, int is_f64) {
+    free(a->set);
+    memset(a, 0, sizeof(*a));
+    a->minv = is_f64 ? vo_f64bits(1.7976931348623157e308) : (uint64_t)INT64_MAX;
+    a->maxv = is_f64 ? vo_f64bits(-1.7976931348623157e308) : (uint64_t)INT64_MIN;
+}
+
+void* vo_agg_create(int is_f64) {
+    vo_aggstate* st = (vo_aggstate*)calloc(1, sizeof(vo_aggstate));
+    st->is_f64 = is_f64;
+    for (int i = 0; i < 16; ++i) vo_slot_reset(&st->s[i], is_f64);
+    return st;
+}
+
+void vo_agg_destroy(void* p) {
+    vo_aggstate* st = (vo_aggstate*)p;
+    if (!st) return;
+    for (int i = 0; i < 16; ++i) free(st->s[i].set);
+    free(st);
+}
+
+static int vo_cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* order key of a T value (f64: -0 as +0) */
+static uint64_t vo_okey(uint64_t bits, int is_f64) {
+    if (!is_f64) return bits ^ 0x8000000000000000ull;
+    if (bits == 0x8000000000000000ull) bits = 0;
+    return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+}
+static uint64_t vo_unkey(uint64_t k, int is_f64) {
+    if (!is_f64) return k ^ 0x8000000000000000ull;
+    return (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+}
+
+/* returns 1 ran, -1 not restated (or NaN where the reference's order is unspecified) */
+static int vo_agg_word(vo_aggstate* st, const uint64_t* data, uint64_t count, uint32_t w, uint64_t* r) {
+    const int F = st->is_f64;
+    unsigned op = w & 0xFF, rd = (w >> 8) & 0xF, ra = (w >> 12) & 0xF, rb = (w >> 16) & 0xF, imm = (w >> 20) & 0xFFF;
+    if (ra >= 1) return 1;                                   /* segment id past the engine's segments: PC++ only */
+    uint64_t start = r[rb];
+    uint64_t cnt = imm == 0 ? 0xFFFFFFFFull : (uint64_t)(imm & 0xFFF);
+    uint64_t end = start + cnt;
+    if (end > count) end = count;
+    vo_aggslot* s = &st->s[rd & 0xF];
+    #define X(i) (F ? vo_bitsf64(data[i]) : (double)(int64_t)data[i])
+    switch (op) {
+    case 0xE0:                                                           /* COUNT */
+        if (!s->init) { s->count = 0; s->init = 1; }
+        for (uint64_t i = start; i < end; ++i)
+            if (F ? (vo_bitsf64(data[i]) != 0.0) : (data[i] != 0)) s->count++;
+        r[rd] = s->count;
+        return 1;
+    case 0xE1:                                                           /* SUM */
+        if (!s->init) { s->sumf = 0; s->sumi = 0; s->init = 1; }
+        if (F) { for (uint64_t i = start; i < end; ++i) s->sumf += vo_bitsf64(data[i]); r[rd] = vo_f64bits(s->sumf); }
+        else { for (uint64_t i = start; i < end; ++i) s->sumi = (int64_t)((uint64_t)s->sumi + data[i]); r[rd] = (uint64_t)s->sumi; }
+        return 1;
+    case 0xE2: {                                                         /* AVG */
+        if (!s->init) { s->count = 0; s->sumf = 0; s->sumi = 0; s->init = 1; }
+        uint64_t n = 0;
+        if (F) {
+            for (uint64_t i = start; i < end; ++i) { s->sumf += vo_bitsf64(data[i]); n++; }
+            s->count += n;
+            r[rd] = vo_f64bits(s->count ? s->sumf / (double)s->count : 0.0);
+        } else {
+            for (uint64_t i = start; i < end; ++i) { s->sumi = (int64_t)((uint64_t)s->sumi + data[i]); n++; }
+            s->count += n;
+            r[rd] = (uint64_t)(s->count ? s->sumi / (int64_t)s->count : 0);
+        }
+        return 1;
+    }
+    case 0xE3: case 0xE4:                                                /* MIN, MAX */
+        if (!s->init) {
+            if (op == 0xE3) s->minv = F ? vo_f64bits(1.7976931348623157e308) : (uint64_t)INT64_MAX;
+            else s->maxv = F ? vo_f64bits(-1.7976931348623157e308) : (uint64_t)INT64_MIN;
+            s->init = 1;
+        }
+        for (uint64_t i = start; i < end; ++i) {
+            if (op == 0xE3) {
+                if (F ? vo_bitsf64(data[i]) < vo_bitsf64(s->minv) : (int64_t)data[i] < (int64_t)s->minv) s->minv = data[i];
+            } else {
+                if (F ? vo_bitsf64(data[i]) > vo_bitsf64(s->maxv) : (int64_t)data[i] > (int64_t)s->maxv) s->maxv = data[i];
+            }
+        }
+        r[rd] = op == 0xE3 ? s->minv : s->maxv;
+        return 1;
+    case 0xE5:                                                           /* FIRST */
+        if (!s->init) { s->first_set = 0; s->init = 1; }
+        if (!s->first_set && end > start) { s->firstv = data[start]; s->first_set = 1; }
+        r[rd] = s->firstv;
+        return 1;
+    case 0xE6:                                                           /* LAST */
+        if (!s->init) s->init = 1;
+        if (end > start) s->lastv = data[end - 1];
+        r[rd] = s->lastv;
+        return 1;
+    case 0xE7: case 0xE8: {                                              /* STDDEV, VARIANCE */
+        if (!s->init) { s->count = 0; s->sumf = 0; s->sumi = 0; s->mean = 0; s->m2 = 0; s->init = 1; }
+        for (uint64_t i = start; i < end; ++i) {
+            double v = X(i);
+            s->count++;
+            double d = v - s->mean;
+            s->mean += d / (double)s->count;
+            double d2 = v - s->mean;
+            s->m2 += d * d2;
+        }
+        double var = s->count > 1 ? s->m2 / (double)s->count : 0.0;
+        double res = op == 0xE7 ? sqrt(var) : var;
+        r[rd] = F ? vo_f64bits(res) : (uint64_t)(int64_t)res;
+        return 1;
+    }
+    case 0xE9: {                                                         /* COUNT_DISTINCT */
+        if (!s->init) { s->has_set = 1; s->nset = 0; s->init = 1; }
+        if (!s->has_set) return -1;                                      /* the reference dereferences a null set */
+        for (uint64_t i = start; i < end; ++i) {
+            if (F && isnan(vo_bitsf64(data[i]))) return -1;              /* NaN keys: unordered_map keeps each */
+            uint64_t k = vo_okey(data[i], F);
+            size_t lo = 0, hi = s->nset;
+            while (lo < hi) { size_t mid = (lo + hi) / 2; if (s->set[mid] < k) lo = mid + 1; else hi = mid; }
+            if (lo < s->nset && s->set[lo] == k) continue;
+            if (s->nset == s->capset) {
+                s->capset = s->capset ? 2 * s->capset : 64;
+                s->set = (uint64_t*)realloc(s->set, s->capset * 8);
+            }
+            memmove(s->set + lo + 1, s->set + lo, (s->nset - lo) * 8);
+            s->set[lo] = k;
+            s->nset++;
+        }
+        r[rd] = s->nset;
+        return 1;
+    }
+    case 0xEB: case 0xED: {                                              /* MEDIAN, PERCENTILE */
+        uint64_t n = end > start ? end - start : 0;
+        if (n == 0) { r[rd] = 0; return 1; }
+        uint64_t* k = (uint64_t*)malloc(n * 8);
+        for (uint64_t i = 0; i < n; ++i) {
+            if (F && isnan(vo_bitsf64(data[start + i]))) { free(k); return -1; }   /* nth_element order unspecified */
+            k[i] = vo_okey(data[start + i], F);
+        }
+        qsort(k, n, 8, vo_cmp_u64);
+        if (op == 0xED) {
+            double pct = (double)(imm & 0xFF) / 100.0;
+            uint64_t idx = (uint64_t)(pct * (double)n);
+            if (idx >= n) idx = n - 1;
+            r[rd] = vo_unkey(k[idx], F);
+        } else if (n % 2 == 0) {
+            uint64_t a = vo_unkey(k[n / 2], F), b = vo_unkey(k[n / 2 - 1], F);
+            r[rd] = F ? vo_f64bits((vo_bitsf64(a) + vo_bitsf64(b)) * 0.5) : (uint64_t)(((int64_t)a + (int64_t)b) / 2);
+        } else {
+            r[rd] = vo_unkey(k[n / 2], F);
+        }
+        free(k);
+        return 1;
+    }
+    default:
+        return -1;
+    }
+    #undef X
+}
+
+/* Straight-line program of scalar and AGG words (stops at HALT or at a
+   division by zero).  Returns 1 ran, 0 stopped by a division by zero,
+   -1 a word outside the restated set. */
+int vo_straight_run(void* state, const uint64_t* data, uint64_t count, const uint32_t* w, size_t n, uint64_t* r) {
+    vo_aggstate* st = (vo_aggstate*)state;
+    for (size_t k = 0; k < n; ++k) {
+        unsigned op = w[k] & 0xFF;
+        if (op == 0x01) return 1;
+        if (op >= 0xE0 && op <= 0xED) {
+            if (vo_agg_word(st, data, count, w[k], r) < 0) return -1;
+            continue;
+        }
+        int rc = vo_scalar_run(&w[k], 1, r);
+        if (rc != 1) return rc;
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
 /* JIT fused loop, codegen/x86_64.hpp:1632-1716, as entered by voxel_jit_run */
 /* (bindings/voxel_c.cpp:80-101): iters = (R2 - off + 7) >> 3; each         */
 /* iteration adds masked x[8k+j] into lane j of accumulator A (j<4) or B;   */

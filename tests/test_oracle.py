@@ -267,6 +267,45 @@ def test_scalar_words_restatement_against_reference_engine():
     assert stopped > 0
 
 
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_segment_aggregates_restatement_against_reference_engine(is_f64):
+    """vo_straight_run (AGG_COUNT..AGG_COUNT_DISTINCT, AGG_MEDIAN,
+    AGG_PERCENTILE with the engine's persistent slots, mixed with scalar
+    words) against the reference engine running the same words, several
+    programs in a row on one engine state: all 16 scalar registers
+    bit-identical."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(51 if is_f64 else 52)
+    done = 0
+    for trial in range(30):
+        n = int(rng.integers(1, 3000))
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 1e3, 3)
+            x[rng.integers(0, n, 4)] = 0.0
+            if trial % 5 == 0:
+                x[rng.integers(0, n, 2)] = np.inf
+        else:
+            x = rng.integers(-10**6, 10**6, n).astype(np.int64)
+            x[rng.integers(0, n, 4)] = 0
+        st = o.O.vo_agg_create(1 if is_f64 else 0)
+        regs = np.zeros(64, np.uint64)
+        ref_regs = np.zeros(64, np.uint64)
+        progs = [tg.straight_program(rng, n) for _ in range(3)]
+        # the reference keeps its AggState slots across runs of one engine:
+        # run the three programs back to back on one engine as one program
+        whole = np.concatenate([p[:-1] for p in progs] + [progs[-1][-1:]]).astype(np.uint32)
+        rc = o.O.vo_straight_run(st, o.ptr(x.view(np.uint64)), n, o.ptr(whole), whole.size, o.ptr(regs))
+        o.O.vo_agg_destroy(st)
+        if rc < 0:
+            continue
+        run = o.R.vr_engine_run_f64 if is_f64 else o.R.vr_engine_run_i64
+        run(o.ptr(x), n, o.ptr(whole), whole.size, o.ptr(ref_regs))
+        assert np.array_equal(regs[:16], ref_regs[:16]), (trial, [hex(int(v)) for v in whole])
+        done += 1
+    assert done >= 20
+
+
 def test_jit_restatement(golden, x42):
     g = golden["seed42"]
     r = o.O.vo_jit_f64(o.ptr(x42), x42.size, x42.size, 0, 0xC5, 500.0)

@@ -332,6 +332,86 @@ def test_tier0_scalar_prefix_and_epilogue(gpu, is_f64):
         done += 1
 
 
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_segment_aggregates_against_oracle(gpu, is_f64):
+    """Straight-line programs of AGG_* words (persistent slots across three
+    runs of one engine) and scalar words: counts, MIN/MAX, FIRST/LAST,
+    distinct counts, MEDIAN and PERCENTILE bit-identical with the oracle
+    (vo_straight_run, pinned against the reference engine); f64 SUM/AVG
+    within 1e-12 of the sum of |x| (the reference adds in row order, the GPU
+    a compensated sum), STDDEV/VARIANCE within 1e-9 relative (Welford in row
+    order vs merged workgroup moments; the i64 engine's truncated integer
+    within 1)."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(81 if is_f64 else 82)
+    inexact = {0xE1, 0xE2, 0xE7, 0xE8} if is_f64 else {0xE7, 0xE8}
+    runs = 0
+    for trial in range(12):
+        writers = {}
+        n = int(rng.integers(1, 20_000))
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 1e3, 3)
+            x[rng.integers(0, n, 4)] = 0.0
+            if trial % 4 == 0:
+                x[rng.integers(0, n, 2)] = np.inf
+        else:
+            x = rng.integers(-10**6, 10**6, n).astype(np.int64)
+            x[rng.integers(0, n, 4)] = 0
+        scale = float(np.abs(x[np.isfinite(x)]).sum()) if is_f64 else 0.0
+        e = vx.EngineF64() if is_f64 else vx.EngineI64()
+        e.add_segment(x)
+        st = o.O.vo_agg_create(1 if is_f64 else 0)
+        regs = np.zeros(64, np.uint64)
+        try:
+            for k in range(3):
+                code = tg.straight_program(rng, n, ops=tg.AGG_OPS, scalar_reads=(1, 2, 3, 13, 14, 15))
+                rc = o.O.vo_straight_run(st, o.ptr(x.view(np.uint64)), n, o.ptr(code), code.size, o.ptr(regs))
+                assert rc >= 0
+                e.load_program([int(c) for c in code])
+                e.run()
+                writers.update(tg.last_agg_writers(code))
+                for r in range(16):
+                    got, want = e.get_scalar(r), int(regs[r])
+                    if got == want:
+                        continue
+                    op = writers.get(r)
+                    assert op in inexact, (trial, k, r, hex(op or 0), got, want)
+                    if is_f64:
+                        g, w = o.bits_f64(got), o.bits_f64(want)
+                        if op in (0xE1, 0xE2):
+                            assert abs(g - w) <= 1e-12 * scale, (trial, k, r, g, w)
+                        else:
+                            assert abs(g - w) <= 1e-9 * abs(w), (trial, k, r, g, w)
+                    else:
+                        assert abs(int(np.int64(np.uint64(got))) - int(np.int64(np.uint64(want)))) <= 1
+                runs += 1
+        finally:
+            o.O.vo_agg_destroy(st)
+    assert runs == 36
+
+
+def test_segment_aggregate_programs_are_lowered_or_refused(gpu):
+    """AGG_SUM_DISTINCT / AGG_MODE are refused at load; NaN keys in
+    AGG_COUNT_DISTINCT are refused at run time with the registers untouched."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    from voxel_amd._lib import lib
+    x = np.array([1.0, np.nan, 2.0, 2.0])
+    e = vx.EngineF64()
+    e.add_segment(x)
+    e.load_program([tg.w(0xEC, rd=4, ra=0, rb=1), 1])
+    with pytest.raises(RuntimeError, match="AGG_MODE"):
+        e.run()
+    e.load_program([tg.w(0xE9, rd=4, ra=0, rb=1), 1])
+    with pytest.raises(RuntimeError, match="NaN"):
+        e.run()
+    assert e.get_scalar(4) == 0
+    e.load_program([tg.w(0xE0, rd=4, ra=0, rb=1), tg.w(0xE3, rd=5, ra=0, rb=1), 1])
+    e.run()
+    assert e.get_scalar(4) == 4 and o.bits_f64(e.get_scalar(5)) == 1.0     # NaN counts as != 0; MIN skips it
+
+
 def test_tier0_minmax_mixed_signs_uses_exact_fold(gpu, golden):
     # values whose bit patterns straddle both wrap points: the GPU falls back
     # to the in-order fold; result must equal the oracle's bits exactly

@@ -112,3 +112,35 @@ def mergeable(spec):
         if len(chain) == 2 and 1 in chain:
             return False
     return True
+
+
+AGG_OPS = list(range(0xE0, 0xEA)) + [0xEB, 0xED]          # AGG_COUNT..COUNT_DISTINCT, MEDIAN, PERCENTILE
+
+
+def straight_program(rng, n, k=8, ops=AGG_OPS, scalar_reads=tuple(range(16))):
+    """Straight-line program: MOVs of start rows into R1..R3, then k words
+    mixing AGG words (slots R4..R8, R9 for AGG_COUNT_DISTINCT, start register
+    R1..R3, row counts 0..4095 with 0 = all, a few naming a missing segment)
+    and scalar arithmetic on the scratch registers."""
+    code = [w(0x10, rd=r, imm=int(rng.integers(0, min(n, 2047) + 1))) for r in (1, 2, 3)]
+    for _ in range(k):
+        if rng.random() < 0.75:
+            op = int(rng.choice(ops))
+            imm = 0 if rng.random() < 0.3 else int(rng.integers(1, 4096))
+            seg = 0 if rng.random() < 0.95 else 1
+            slot = 9 if op == 0xE9 else int(rng.integers(4, 9))      # the distinct set keeps slot 9 to itself
+            code.append(w(op, rd=slot, ra=seg, rb=int(rng.integers(1, 4)), imm=imm))
+        else:
+            code += scalar_words(rng, 1, scalar_reads, SCRATCH)
+    code.append(w(0x01))
+    return np.array(code, np.uint32)
+
+
+def last_agg_writers(words):
+    """register -> opcode of the last AGG word that writes it"""
+    out = {}
+    for x in words:
+        op = int(x) & 0xFF
+        if 0xE0 <= op <= 0xED and ((int(x) >> 12) & 0xF) == 0:
+            out[(int(x) >> 8) & 0xF] = op
+    return out

@@ -13,6 +13,12 @@
 #include "vx_kernels.hpp"
 #include "vx_lower.hpp"
 #include "vx_runtime.hpp"
+#include "vx_radix.hpp"
+#include "vx_segagg.hpp"
+
+namespace vx {
+int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n, StreamCtx* c);
+}
 
 using namespace vx;
 
@@ -29,12 +35,31 @@ struct Seg {
 };
 
 // Engine<T>: 64 scalar registers (core/registers.hpp:20) + segments + program.
+// One AggState slot (engine.hpp:347-372): the fields the aggregates share.
+// The f64 sum is kept as a compensated pair; the distinct set as sorted
+// order keys in device memory.
+struct AggSlot {
+    uint64_t count = 0;
+    double sh = 0.0, sl = 0.0;       // SumF64
+    int64_t sumi = 0;                // SumI64
+    uint64_t minv = 0, maxv = 0;     // MinVal / MaxVal bits (set from the engine type)
+    uint64_t firstv = 0, lastv = 0;
+    double mean = 0.0, m2 = 0.0;
+    bool init = false, first_set = false, has_set = false;
+    uint64_t* set = nullptr;         // distinct order keys, ascending
+    uint64_t nset = 0, setcap = 0;
+    ~AggSlot() { if (set) (void)hipFree(set); }
+};
+
 struct Engine {
     int dtype = DT_F64;
     uint64_t regs[64] = {};
     std::vector<std::unique_ptr<Seg>> segs;
     std::vector<uint32_t> code;
     Plan plan;
+    AggSlot agg[16];
+    bool agg_ready = false;
+    SegaggBufs segagg;
 };
 
 size_t lanes_of(int dt) { return 32 / dtype_size(dt); }
@@ -233,6 +258,239 @@ bool run_tier0(Engine& e, StreamCtx* c) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Straight-line programs: scalar words on the host, segment aggregates
+// (Engine<T>::HandleAggregate, engine.hpp:499-698) on the GPU, with the
+// engine's 16 persistent slots.  Sums, averages and moments are compensated
+// or merged in a fixed order (within a few ulp of the reference's row-order
+// f64 sums and Welford updates); counts, min/max, first/last, distinct
+// counts, median and percentile are exact.
+// ---------------------------------------------------------------------------
+namespace {
+void dd_add(double& hi, double& lo, double s, double c) {
+    if (!std::isfinite(hi) || !std::isfinite(s)) { hi = hi + s; lo = 0.0; return; }
+    const double t = hi + s, bp = t - hi, e = (hi - (t - bp)) + (s - bp);
+    const double l = lo + e + c;
+    hi = t + l;
+    lo = l - (hi - t);
+}
+
+bool read_bits(StreamCtx* c, const void* dev, uint64_t row, int dtype, uint64_t* out) {
+    const size_t es = dtype_size(dtype);
+    uint64_t v = 0;
+    if (!sync_read(c, &c->host->fold[0], static_cast<const char*>(dev) + row * es, es)) return false;
+    std::memcpy(&v, &c->host->fold[0], es);
+    *out = v;
+    return true;
+}
+
+bool run_agg(Engine& e, uint32_t w, uint64_t* regs, StreamCtx* c) {
+    const uint8_t op = op_of(w), rd = rd_of(w), seg = ra_of(w), rb = rb_of(w);
+    const uint16_t imm = imm_of(w);
+    if (seg >= e.segs.size()) return true;                       // missing segment: PC++ only
+    Seg& s = *e.segs[seg];
+    const bool F = e.dtype == DT_F64;
+    const uint64_t start = regs[rb];
+    const uint64_t cnt = imm == 0 ? 0xFFFFFFFFull : (uint64_t)(imm & 0xFFF);
+    uint64_t end = start + cnt;
+    if (end > s.count) end = s.count;
+    const uint64_t n = end > start ? end - start : 0;
+    AggSlot& a = e.agg[rd & 0xF];
+    const size_t es = dtype_size(e.dtype);
+    const char* d = nullptr;
+    if (n) {
+        const void* base = seg_rows(s, e.dtype, start, end, c);
+        if (!base) return false;
+        d = static_cast<const char*>(base) + start * es;
+    }
+    auto scan = [&](int cmp, uint64_t tbits, ScanOut* r) {
+        ScanArgs g{};
+        g.dtype = e.dtype; g.a = d; g.begin = 0; g.end = n; g.mode = 0;
+        g.pa = make_pred(e.dtype, cmp, tbits);
+        return scan_once(c, g, r);
+    };
+    const double dbl_max = 1.7976931348623157e308;
+    switch (op) {
+    case 0xE0: {                                                 // COUNT: values != 0 (NaN counts)
+        if (!a.init) { a.count = 0; a.init = true; }
+        if (n) { ScanOut r{}; if (!scan(CMP_NE, 0, &r)) return false; a.count += r.cnt; }
+        regs[rd] = a.count;
+        return true;
+    }
+    case 0xE1:                                                   // SUM
+    case 0xE2: {                                                 // AVG
+        if (!a.init) {
+            if (op == 0xE2) a.count = 0;
+            a.sh = a.sl = 0.0; a.sumi = 0; a.init = true;
+        }
+        if (n) {
+            ScanOut r{};
+            if (!scan(CMP_TRUE, 0, &r)) return false;
+            if (F) dd_add(a.sh, a.sl, bits_f64(r.s), bits_f64(r.c));
+            else a.sumi = (int64_t)((uint64_t)a.sumi + r.s);
+        }
+        if (op == 0xE1) {
+            regs[rd] = F ? f64_bits(a.sh + a.sl) : (uint64_t)a.sumi;
+        } else {
+            a.count += n;
+            if (F) regs[rd] = f64_bits(a.count ? (a.sh + a.sl) / (double)a.count : 0.0);
+            else regs[rd] = (uint64_t)(a.count ? a.sumi / (int64_t)a.count : 0);
+        }
+        return true;
+    }
+    case 0xE3:
+    case 0xE4: {                                                 // MIN, MAX (row < current, NaN never taken)
+        if (!a.init) {
+            if (op == 0xE3) a.minv = F ? f64_bits(dbl_max) : (uint64_t)INT64_MAX;
+            else a.maxv = F ? f64_bits(-dbl_max) : (uint64_t)INT64_MIN;
+            a.init = true;
+        }
+        if (n) {
+            ScanOut r{};
+            if (!scan(CMP_TRUE, 0, &r)) return false;
+            voxel_gpu_agg g{};
+            fill_agg(e.dtype, r, &g);
+            if (F && g.has_minmax) {
+                if (op == 0xE3 && g.fmin < bits_f64(a.minv)) a.minv = f64_bits(g.fmin);
+                if (op == 0xE4 && g.fmax > bits_f64(a.maxv)) a.maxv = f64_bits(g.fmax);
+            } else if (!F) {
+                if (op == 0xE3 && g.imin < (int64_t)a.minv) a.minv = (uint64_t)g.imin;
+                if (op == 0xE4 && g.imax > (int64_t)a.maxv) a.maxv = (uint64_t)g.imax;
+            }
+        }
+        regs[rd] = op == 0xE3 ? a.minv : a.maxv;
+        return true;
+    }
+    case 0xE5:                                                   // FIRST
+        if (!a.init) { a.first_set = false; a.init = true; }
+        if (!a.first_set && n) {
+            if (!read_bits(c, d, 0, e.dtype, &a.firstv)) return false;
+            a.first_set = true;
+        }
+        regs[rd] = a.firstv;
+        return true;
+    case 0xE6:                                                   // LAST
+        if (!a.init) a.init = true;
+        if (n && !read_bits(c, d, n - 1, e.dtype, &a.lastv)) return false;
+        regs[rd] = a.lastv;
+        return true;
+    case 0xE7:
+    case 0xE8: {                                                 // STDDEV, VARIANCE: Welford, merged with Chan's formula
+        if (!a.init) { a.count = 0; a.sh = a.sl = 0.0; a.sumi = 0; a.mean = 0.0; a.m2 = 0.0; a.init = true; }
+        if (n) {
+            Moments m;
+            if (!segagg_moments(e.dtype, d, n, &m, e.segagg, c)) return false;
+            const double na = (double)a.count, nb = m.n, nt = na + nb;
+            const double dlt = m.mean - a.mean;
+            if (a.count == 0) { a.mean = m.mean; a.m2 = m.m2; }
+            else { a.mean = a.mean + dlt * (nb / nt); a.m2 = a.m2 + m.m2 + dlt * dlt * (na * (nb / nt)); }
+            a.count += n;
+        }
+        const double var = a.count > 1 ? a.m2 / (double)a.count : 0.0;
+        const double res = op == 0xE7 ? std::sqrt(var) : var;
+        regs[rd] = F ? f64_bits(res) : (uint64_t)(int64_t)res;
+        return true;
+    }
+    case 0xE9: {                                                 // COUNT_DISTINCT
+        if (!a.init) { a.has_set = true; a.nset = 0; a.init = true; }
+        if (!a.has_set) { set_error("AGG_COUNT_DISTINCT on a slot another aggregate started: the reference dereferences a null set"); return false; }
+        if (n) {
+            if (F) {
+                ScanOut r{};
+                if (!scan(CMP_LE, f64_bits(INFINITY), &r)) return false;     // non-NaN rows
+                if (r.cnt != n) { set_error("AGG_COUNT_DISTINCT over NaN values is not lowered (each NaN is its own key in the reference)"); return false; }
+            }
+            SegaggBufs& b = e.segagg;
+            if (!b.ensure(n + a.nset)) return false;                           // no reallocation below
+            if (!segagg_sorted_keys(e.dtype, d, n, b, c)) return false;
+            uint64_t nu = 0;
+            if (!segagg_unique(b.k2, n, b.k1, &nu, c)) return false;          // new distinct keys in k1
+            uint64_t total = nu + a.nset;
+            if (a.nset) {                                                      // merge: sort old + new, unique again
+                if (!hip_ok(hipMemcpyAsync(b.k1 + nu, a.set, a.nset * 8, hipMemcpyDeviceToDevice, c->stream), "copy set") ||
+                    !hip_ok(launch_iota(b.v1, total, c->stream), "iota"))
+                    return false;
+                if (sort_pairs_u64(b.k1, b.v1, b.k2, b.v2, total, c) < 0) return false;
+                if (!segagg_unique(b.k2, total, b.k1, &nu, c)) return false;
+            }
+            if (nu > a.setcap) {
+                if (a.set) (void)hipFree(a.set);
+                a.set = nullptr; a.setcap = 0;
+                if (!hip_ok(hipMalloc((void**)&a.set, nu * 8), "hipMalloc(distinct set)")) return false;
+                a.setcap = nu;
+            }
+            if (nu && !hip_ok(hipMemcpyAsync(a.set, b.k1, nu * 8, hipMemcpyDeviceToDevice, c->stream), "copy set")) return false;
+            a.nset = nu;
+        }
+        regs[rd] = a.nset;
+        return true;
+    }
+    case 0xEB:
+    case 0xED: {                                                 // MEDIAN, PERCENTILE (no slot state)
+        if (n == 0) { regs[rd] = 0; return true; }
+        if (F) {
+            ScanOut r{};
+            if (!scan(CMP_LE, f64_bits(INFINITY), &r)) return false;
+            if (r.cnt != n) { set_error("AGG_MEDIAN / AGG_PERCENTILE over NaN values is not lowered (the reference's nth_element order is unspecified)"); return false; }
+        }
+        if (!segagg_sorted_keys(e.dtype, d, n, e.segagg, c)) return false;
+        auto key_at = [&](uint64_t i, uint64_t* v) {
+            uint64_t k;
+            if (!sync_read(c, &c->host->fold[0], e.segagg.k2 + i, 8)) return false;
+            std::memcpy(&k, &c->host->fold[0], 8);
+            *v = order_unkey(k, e.dtype);
+            return true;
+        };
+        if (op == 0xED) {
+            const double pct = (double)(imm & 0xFF) / 100.0;
+            uint64_t idx = (uint64_t)(pct * (double)n);
+            if (idx >= n) idx = n - 1;
+            uint64_t v;
+            if (!key_at(idx, &v)) return false;
+            regs[rd] = v;
+        } else if (n % 2 == 0) {
+            uint64_t hi, lo;
+            if (!key_at(n / 2, &hi) || !key_at(n / 2 - 1, &lo)) return false;
+            regs[rd] = F ? f64_bits((bits_f64(hi) + bits_f64(lo)) * 0.5) : (uint64_t)(((int64_t)hi + (int64_t)lo) / 2);
+        } else {
+            uint64_t v;
+            if (!key_at(n / 2, &v)) return false;
+            regs[rd] = v;
+        }
+        return true;
+    }
+    default:
+        set_error("aggregate word 0x%02x is not lowered", op);
+        return false;
+    }
+}
+}  // namespace
+
+bool run_straight(Engine& e, StreamCtx* c) {
+    if (e.dtype != DT_F64 && e.dtype != DT_I64) { set_error("segment aggregates are lowered for f64 and i64 engines only"); return false; }
+    if (!e.agg_ready) {                                          // AggState defaults (engine.hpp:347-360)
+        const bool F = e.dtype == DT_F64;
+        for (AggSlot& a : e.agg) {
+            a.minv = F ? f64_bits(1.7976931348623157e308) : (uint64_t)INT64_MAX;
+            a.maxv = F ? f64_bits(-1.7976931348623157e308) : (uint64_t)INT64_MIN;
+        }
+        e.agg_ready = true;
+    }
+    uint64_t regs[64];
+    std::memcpy(regs, e.regs, sizeof(regs));
+    for (uint32_t w : e.plan.words) {
+        if (agg_op(op_of(w))) {
+            if (!run_agg(e, w, regs, c)) return false;
+            continue;
+        }
+        const int rc = run_scalar(std::vector<uint32_t>{w}, regs);
+        if (rc < 0) return false;
+        if (rc == 0) break;                                      // division by zero stops the program
+    }
+    std::memcpy(e.regs, regs, sizeof(regs));
+    return true;
+}
+
 Engine* eng(void* h) { return static_cast<Engine*>(h); }
 
 void engine_run(void* h) {
@@ -244,6 +502,7 @@ void engine_run(void* h) {
     if (!c) return;
     if (e.plan.kind == Plan::FAST) run_fast(e, c);
     else if (e.plan.kind == Plan::TIER0) run_tier0(e, c);
+    else if (e.plan.kind == Plan::STRAIGHT) run_straight(e, c);
     else set_error("program not lowered to the GPU: %s", e.plan.why.c_str());
 }
 
@@ -500,7 +759,7 @@ int voxel_gpu_jit_lowerable(const uint32_t* code, size_t len) {
 
 int voxel_gpu_engine_lowered(const void* h) {
     const Engine& e = *static_cast<const Engine*>(h);
-    return e.plan.kind == Plan::FAST ? 1 : e.plan.kind == Plan::TIER0 ? 2 : 0;
+    return e.plan.kind == Plan::FAST ? 1 : e.plan.kind == Plan::TIER0 ? 2 : e.plan.kind == Plan::STRAIGHT ? 3 : 0;
 }
 
 int voxel_gpu_scan_launch(const voxel_gpu_scan_desc* d, void* stream) {

@@ -112,6 +112,28 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     Plan p;
     if (detect_fast_path(c, len, p)) return p;
 
+    // Straight-line program of scalar words and segment aggregates: the
+    // interpreter runs it word by word to HALT (engine.hpp:76-80).
+    {
+        size_t k = 0;
+        bool any_agg = false;
+        for (; k < len && op_of(c[k]) != OP_HALT; ++k) {
+            const uint8_t op = op_of(c[k]);
+            if (agg_op(op)) {
+                if (op == 0xEA || op == 0xEC)
+                    return none("AGG_SUM_DISTINCT and AGG_MODE are not lowered (their results follow hash-map order)");
+                any_agg = true;
+                continue;
+            }
+            if (!host_op(op)) break;
+        }
+        if (any_agg && (k == len || op_of(c[k]) == OP_HALT)) {
+            p.kind = Plan::STRAIGHT;
+            p.words.assign(c, c + k);
+            return p;
+        }
+    }
+
     // Tier-0 loop: prefix; VLOAD; then filters (each reading the loaded
     // vector or a filter's output, at most two deep) and reduction blocks
     // (VSUM|VCOUNT|VRED_* of any of those vectors, then ADDF|ADD or

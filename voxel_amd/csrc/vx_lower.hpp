@@ -38,7 +38,8 @@ inline int cmp_of_filter_op(uint8_t op) {
 }
 
 struct Plan {
-    enum Kind { NONE, FAST, TIER0 } kind = NONE;
+    enum Kind { NONE, FAST, TIER0, STRAIGHT } kind = NONE;
+    std::vector<uint32_t> words;     // STRAIGHT: scalar and AGG_* words up to HALT
     std::string why;                 // reason when NONE
 
     // FAST: Engine<T>::DetectFastPath (engine.hpp:84-107)
@@ -91,6 +92,9 @@ void run_prefix(const std::vector<uint32_t>& prefix, uint64_t* regs);
 // 1286-1505; opcodes.hpp:24-60).  Condition flags are not kept: no lowered
 // program reads them outside its loop.
 bool host_op(uint8_t op);
+
+// Segment aggregate words AGG_COUNT..AGG_PERCENTILE (opcodes.hpp:228-241).
+inline bool agg_op(uint8_t op) { return op >= 0xE0 && op <= 0xED; }
 
 // Runs host_op words in order on a register file.  Returns 1 when all ran,
 // 0 when a DIV/MOD/DIVF by zero stopped the program (the reference clears

@@ -1,0 +1,169 @@
+// vx_segagg.hip -- device helpers of the segment aggregates (vx_segagg.hpp).
+//
+// Moments: every thread runs Welford's update over its rows (grid stride),
+// the 256 threads of a workgroup are merged pairwise in thread order with
+// Chan's formula, and the host merges the workgroups' partials in workgroup
+// order, so the result does not depend on scheduling.  Order keys and the
+// distinct set use the radix sort (vx_radix.hip) and the stream compaction
+// (vx_compact.hpp).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+
+#include "vx_compact.hpp"
+#include "vx_kernels.hpp"
+#include "vx_radix.hpp"
+#include "vx_runtime.hpp"
+#include "vx_segagg.hpp"
+
+#pragma clang fp contract(off)
+
+namespace vx {
+
+int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n, StreamCtx* c);
+
+namespace {
+
+constexpr int kMomThreads = 256;
+constexpr unsigned kMomBlocksMax = 1024;
+
+__host__ __device__ inline Moments chan_merge(const Moments& a, const Moments& b) {
+    if (a.n == 0.0) return b;
+    if (b.n == 0.0) return a;
+    const double n = a.n + b.n;
+    const double d = b.mean - a.mean;
+    Moments r;
+    r.n = n;
+    r.mean = a.mean + d * (b.n / n);
+    r.m2 = a.m2 + b.m2 + d * d * (a.n * (b.n / n));
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMomThreads) void k_moments(const T* __restrict__ x, uint64_t n, Moments* __restrict__ parts) {
+    __shared__ Moments red[kMomThreads];
+    Moments m{0.0, 0.0, 0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kMomThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kMomThreads + threadIdx.x; i < n; i += stride) {
+        const double v = (double)x[i];
+        m.n += 1.0;
+        const double d = v - m.mean;
+        m.mean += d / m.n;
+        m.m2 += d * (v - m.mean);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kMomThreads / 2; s > 0; s >>= 1) {          // pairs (t, t + s): thread order preserved
+        if ((int)threadIdx.x < s) red[threadIdx.x] = chan_merge(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_i64_keys(const int64_t* __restrict__ x, uint64_t n, uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ idx) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        keys[i] = (uint64_t)x[i] ^ 0x8000000000000000ull;
+        idx[i] = (uint32_t)i;
+    }
+}
+
+// first row of each run of equal keys -> its key
+struct UniqueSrc {
+    const uint64_t* k;
+    uint64_t* out;
+    __device__ __forceinline__ bool pass(uint64_t i) const { return i == 0 || k[i] != k[i - 1]; }
+    __device__ __forceinline__ void emit(uint64_t i, uint64_t j) const { out[j] = k[i]; }
+};
+
+unsigned gridn(uint64_t n, unsigned cap) {
+    const uint64_t b = (n + 255) / 256;
+    return (unsigned)(b ? (b < cap ? b : cap) : 1);
+}
+
+}  // namespace
+
+SegaggBufs::~SegaggBufs() {
+    for (void* p : {(void*)k1, (void*)k2, (void*)v1, (void*)v2, (void*)parts})
+        if (p) (void)hipFree(p);
+    if (parts_host) (void)hipHostFree(parts_host);
+}
+
+bool SegaggBufs::ensure(uint64_t n) {
+    if (!parts) {
+        if (!hip_ok(hipMalloc((void**)&parts, kMomBlocksMax * sizeof(Moments)), "hipMalloc") ||
+            !hip_ok(hipHostMalloc((void**)&parts_host, kMomBlocksMax * sizeof(Moments), hipHostMallocDefault),
+                    "hipHostMalloc"))
+            return false;
+    }
+    if (n <= cap) return true;
+    for (void* p : {(void*)k1, (void*)k2, (void*)v1, (void*)v2})
+        if (p) (void)hipFree(p);
+    k1 = k2 = nullptr;
+    v1 = v2 = nullptr;
+    cap = 0;
+    if (!hip_ok(hipMalloc((void**)&k1, n * 8), "hipMalloc(aggregate keys)") ||
+        !hip_ok(hipMalloc((void**)&k2, n * 8), "hipMalloc(aggregate keys)") ||
+        !hip_ok(hipMalloc((void**)&v1, n * 4), "hipMalloc(aggregate rows)") ||
+        !hip_ok(hipMalloc((void**)&v2, n * 4), "hipMalloc(aggregate rows)"))
+        return false;
+    cap = n;
+    return true;
+}
+
+bool segagg_moments(int dtype, const void* x, uint64_t n, Moments* out, SegaggBufs& b, StreamCtx* c) {
+    *out = Moments{0.0, 0.0, 0.0};
+    if (n == 0) return true;
+    if (!b.ensure(0)) return false;
+    unsigned blocks = (unsigned)device_cu_count() * 2;
+    if (blocks > kMomBlocksMax) blocks = kMomBlocksMax;
+    const uint64_t need = (n + kMomThreads - 1) / kMomThreads;
+    if (need < blocks) blocks = (unsigned)need;
+    if (dtype == DT_F64)
+        hipLaunchKernelGGL(k_moments<double>, dim3(blocks), dim3(kMomThreads), 0, c->stream, (const double*)x, n, b.parts);
+    else
+        hipLaunchKernelGGL(k_moments<int64_t>, dim3(blocks), dim3(kMomThreads), 0, c->stream, (const int64_t*)x, n, b.parts);
+    if (!hip_ok(hipGetLastError(), "moments launch") ||
+        !hip_ok(hipMemcpyAsync(b.parts_host, b.parts, blocks * sizeof(Moments), hipMemcpyDeviceToHost, c->stream),
+                "read moments") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+        return false;
+    Moments m{0.0, 0.0, 0.0};
+    for (unsigned i = 0; i < blocks; ++i) m = chan_merge(m, b.parts_host[i]);
+    *out = m;
+    return true;
+}
+
+bool segagg_sorted_keys(int dtype, const void* x, uint64_t n, SegaggBufs& b, StreamCtx* c) {
+    if (n >= (1ull << 30)) { set_error("aggregate: %llu rows exceed the sort's 2^30 limit", (unsigned long long)n); return false; }
+    if (!b.ensure(n)) return false;
+    if (dtype == DT_F64) {
+        if (!hip_ok(launch_f64_keys((const double*)x, n, 0, b.k1, b.v1, c->stream), "key launch")) return false;
+    } else {
+        hipLaunchKernelGGL(k_i64_keys, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, (const int64_t*)x, n, b.k1, b.v1);
+        if (!hip_ok(hipGetLastError(), "key launch")) return false;
+    }
+    return sort_pairs_u64(b.k1, b.v1, b.k2, b.v2, n, c) >= 0;
+}
+
+bool segagg_unique(const uint64_t* k, uint64_t n, uint64_t* out, uint64_t* nout, StreamCtx* c) {
+    *nout = 0;
+    if (n == 0) return true;
+    UniqueSrc src{k, out};
+    return run_compact(src, n, c, nout);
+}
+
+uint64_t order_key(uint64_t bits, int dtype) {
+    if (dtype != DT_F64) return bits ^ 0x8000000000000000ull;
+    if (bits == 0x8000000000000000ull) bits = 0;
+    return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+}
+
+uint64_t order_unkey(uint64_t key, int dtype) {
+    if (dtype != DT_F64) return key ^ 0x8000000000000000ull;
+    return (key >> 63) ? (key & 0x7FFFFFFFFFFFFFFFull) : ~key;
+}
+
+}  // namespace vx

@@ -1,0 +1,44 @@
+// vx_segagg.hpp -- device helpers of the segment aggregates AGG_* (the
+// reference's Engine<T>::HandleAggregate, exec/engine.hpp:499-698): Welford
+// moments, order keys, sort and distinct keys over a device column range.
+// The slot logic (AggState semantics) is host code in vx_capi.cpp.
+#pragma once
+#include <stdint.h>
+
+namespace vx {
+
+struct StreamCtx;
+
+// Moments of rows as f64: count, mean, sum of squared deviations.
+struct Moments {
+    double n, mean, m2;
+};
+
+// Device work buffers of one engine's aggregates (grown as needed).
+struct SegaggBufs {
+    uint64_t* k1 = nullptr;
+    uint64_t* k2 = nullptr;
+    uint32_t* v1 = nullptr;
+    uint32_t* v2 = nullptr;
+    uint64_t cap = 0;
+    Moments* parts = nullptr;        // per-workgroup moments
+    Moments* parts_host = nullptr;
+    ~SegaggBufs();
+    bool ensure(uint64_t n);
+};
+
+// x[0, n) (dtype DT_F64 or DT_I64) -> moments, merged in workgroup order.
+bool segagg_moments(int dtype, const void* x, uint64_t n, Moments* out, SegaggBufs& b, StreamCtx* c);
+
+// x[0, n) -> its order keys, ascending, in b.k2 (f64: -0 as +0; i64: sign
+// bit flipped).  n < 2^30.
+bool segagg_sorted_keys(int dtype, const void* x, uint64_t n, SegaggBufs& b, StreamCtx* c);
+
+// Ascending keys k[0, n) -> the distinct ones in out (device), *nout.
+bool segagg_unique(const uint64_t* k, uint64_t n, uint64_t* out, uint64_t* nout, StreamCtx* c);
+
+// Order key <-> value bits (host).
+uint64_t order_key(uint64_t bits, int dtype);
+uint64_t order_unkey(uint64_t key, int dtype);
+
+}  // namespace vx
