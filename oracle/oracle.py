@@ -61,6 +61,10 @@ _SIGS = {
     "vo_tier0_multi_f64": (None, [P, U64, U64, U64, INT, INT, P, P, P, INT, P, P, P, P, P]),
     "vo_tier0_multi_i64": (None, [P, U64, U64, U64, INT, INT, P, P, P, INT, P, P, P, P, P]),
     "vo_scalar_run": (INT, [P, SZ, P]),
+    "vo_tier0_expr_f64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P, P, P, P, P]),
+    "vo_tier0_expr_i64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P, P, P, P, P]),
+    "vo_tier0_expr_col_f64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P]),
+    "vo_tier0_expr_col_i64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P]),
     "vo_agg_create": (P, [INT]),
     "vo_agg_destroy": (None, [P]),
     "vo_straight_run": (INT, [P, P, U64, P, SZ, P]),

@@ -376,6 +376,85 @@ VO_DEF_TIER0_MULTI(vo_tier0_multi_f64, double, vo_red_f64)
 VO_DEF_TIER0_MULTI(vo_tier0_multi_i64, int64_t, vo_red_i64)
 
 /* ------------------------------------------------------------------------ */
+/* Tier-0 loops with vector arithmetic (engine.hpp:2221-2350 VADD VSUB VMUL */
+/* VDIV VNEG VABS VMIN VMAX, 2473-2590 VSADD VSSUB VSMUL VSDIV VSMIN VSMAX)  */
+/* and filters, per kL = 4 group: VLOAD into register vload_dst (zero past  */
+/* Count), then ops in program order on 16 vector registers; op word        */
+/* k: code[5k..5k+4] = opcode, dst, a, b, filter kCmp; sval[k] = the scalar */
+/* operand / threshold (register value as T).  Reduction block b reduces    */
+/* register bvreg[b] and folds into acc[b] as in vo_tier0_*.  f64 division  */
+/* by zero gives a quiet NaN (VDIV per lane, VSDIV for every lane).         */
+/* ------------------------------------------------------------------------ */
+#define VO_DEF_EXPR_LANE(NAME, T, ISF)                                                      \
+static T NAME(int op, int cmp, T x, T y, T s) {                                              \
+    switch (op) {                                                                            \
+    case 0x80: return ISF ? x + y : (T)((uint64_t)x + (uint64_t)y);                          \
+    case 0x81: return ISF ? x - y : (T)((uint64_t)x - (uint64_t)y);                          \
+    case 0x82: return ISF ? x * y : (T)((uint64_t)x * (uint64_t)y);                          \
+    case 0x83: return (y == (T)0) ? (T)NAN : x / y;                                          \
+    case 0x85: return ISF ? -x : (T)(0 - (uint64_t)x);                                       \
+    case 0x86: return ISF ? (T)fabs((double)x) : (x < (T)0 ? (T)(0 - (uint64_t)x) : x);      \
+    case 0x87: return ISF ? ((x < y || x != x) ? x : y) : (x < y ? x : y);                   \
+    case 0x88: return ISF ? ((x > y || y != y) ? x : y) : (x > y ? x : y);                   \
+    case 0x90: return ISF ? x + s : (T)((uint64_t)x + (uint64_t)s);                          \
+    case 0x91: return ISF ? x - s : (T)((uint64_t)x - (uint64_t)s);                          \
+    case 0x92: return ISF ? x * s : (T)((uint64_t)x * (uint64_t)s);                          \
+    case 0x93: return (s == (T)0) ? (T)NAN : x / s;                                          \
+    case 0x95: return ISF ? ((x < s || x != x) ? x : s) : (x < s ? x : s);                   \
+    case 0x96: return ISF ? ((x > s || s != s) ? x : s) : (x > s ? x : s);                   \
+    default: return (cmp != 7 && VO_PRED(cmp, x, s)) ? x : (T)0;                            \
+    }                                                                                        \
+}
+VO_DEF_EXPR_LANE(vo_expr_lane_f64, double, 1)
+VO_DEF_EXPR_LANE(vo_expr_lane_i64, int64_t, 0)
+
+/* one kL = 4 group of the body's vector words on the 16 vector registers */
+#define VO_DEF_EXPR_GROUP(NAME, T, LANE)                                                     \
+static void NAME(T v[16][4], const T* data, uint64_t count, uint64_t off, int vload_dst,    \
+                 int nops, const int* code, const T* sval) {                                 \
+    for (int i = 0; i < 4; ++i) {                                                            \
+        uint64_t idx = off + (uint64_t)i;                                                    \
+        v[vload_dst][i] = idx < count ? data[idx] : (T)0;                                    \
+    }                                                                                        \
+    for (int k = 0; k < nops; ++k) {                                                         \
+        const int* c = code + 5 * k;                                                         \
+        T r[4];                                                                              \
+        for (int i = 0; i < 4; ++i) r[i] = LANE(c[0], c[4], v[c[2]][i], v[c[3]][i], sval[k]); \
+        for (int i = 0; i < 4; ++i) v[c[1]][i] = r[i];                                       \
+    }                                                                                        \
+}
+VO_DEF_EXPR_GROUP(vo_expr_group_f64, double, vo_expr_lane_f64)
+VO_DEF_EXPR_GROUP(vo_expr_group_i64, int64_t, vo_expr_lane_i64)
+
+#define VO_DEF_TIER0_EXPR(NAME, COLNAME, T, GROUP, RED)                                      \
+void NAME(const T* data, uint64_t count, uint64_t off, uint64_t total, int vload_dst,       \
+          int nops, const int* code, const T* sval, int nb, const int* bvreg,                \
+          const int* red, const int* accop, uint64_t* acc, uint64_t* part) {                  \
+    T v[16][4];                                                                              \
+    memset(v, 0, sizeof(v));                                                                 \
+    do {                                                                                     \
+        GROUP(v, data, count, off, vload_dst, nops, code, sval);                             \
+        for (int b = 0; b < nb; ++b) {                                                       \
+            part[b] = RED(v[bvreg[b]], red[b]);                                              \
+            acc[b] = vo_tier0_accum(acc[b], part[b], accop[b]);                              \
+        }                                                                                    \
+        off += 4;                                                                            \
+    } while (off != total);                                                                  \
+}                                                                                            \
+/* the lanes of vector register `target` over rows [off, total) (test aid) */               \
+void COLNAME(const T* data, uint64_t count, uint64_t off, uint64_t total, int vload_dst,    \
+             int nops, const int* code, const T* sval, int target, T* out) {                 \
+    T v[16][4];                                                                              \
+    memset(v, 0, sizeof(v));                                                                 \
+    for (uint64_t g = 0; off + 4 * g < total; ++g) {                                         \
+        GROUP(v, data, count, off + 4 * g, vload_dst, nops, code, sval);                     \
+        for (int i = 0; i < 4; ++i) out[4 * g + (uint64_t)i] = v[target][i];                \
+    }                                                                                        \
+}
+VO_DEF_TIER0_EXPR(vo_tier0_expr_f64, vo_tier0_expr_col_f64, double, vo_expr_group_f64, vo_red_f64)
+VO_DEF_TIER0_EXPR(vo_tier0_expr_i64, vo_tier0_expr_col_i64, int64_t, vo_expr_group_i64, vo_red_i64)
+
+/* ------------------------------------------------------------------------ */
 /* Straight-line scalar words (engine.hpp:1167-1178 MOV/MOVR, 1286-1380     */
 /* integer arithmetic on register bits as i64, 1394-1446 f64 arithmetic,    */
 /* 1457-1505 bitwise; opcodes.hpp:24-60).  Returns 1 when every word ran,    */

@@ -120,6 +120,33 @@ def test_lowering_accepts_filter_chains_and_several_reductions():
     assert lowered(loop([W(0xC5, rd=1, ra=0, rb=5), W(0xD0, rd=5, ra=1), W(0x2A, rd=0, ra=0, rb=5)])) == 0  # threshold written
 
 
+def test_lowering_accepts_vector_arithmetic():
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(6)
+    for trial in range(30):
+        is_f64 = trial % 2 == 0
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 9)))
+        assert lowered(list(code), i64=not is_f64) == 2, spec
+    W = tg.w
+    base = [W(0x10, rd=4, imm=4), W(0x70, rd=0, ra=1)]
+    tail = [W(0x20, rd=1, ra=1, rb=4), W(0x40, ra=1, rb=2)]
+    def loop(body):
+        words = base + body + tail
+        return words + [W(0x52, imm=(1 - len(words)) & 0xFFF), W(0x01)]
+    red = [W(0xD0, rd=5, ra=2), W(0x2A, rd=0, ra=0, rb=5)]
+    mul = W(0x92, rd=1, ra=0, rb=3)                                  # V1 = V0 * R3
+    assert lowered(loop([mul, W(0xC5, rd=2, ra=1, rb=6)] + red)) == 2
+    # filters three deep behind arithmetic are evaluated per row
+    chain = [mul, W(0xC5, rd=2, ra=1, rb=6), W(0xC3, rd=3, ra=2, rb=6), W(0xC6, rd=4, ra=3, rb=6)]
+    assert lowered(loop(chain + [W(0xD0, rd=5, ra=4), W(0x2A, rd=0, ra=0, rb=5)])) == 2
+    assert lowered(loop([W(0x80, rd=2, ra=0, rb=7)] + red)) == 0                   # V7 undefined
+    assert lowered(loop([mul, W(0x80, rd=1, ra=0, rb=0)] + red)) == 0             # V1 redefined
+    assert lowered(loop([W(0x92, rd=2, ra=0, rb=5)] + red)) == 0                  # scalar operand written by the loop
+    assert lowered(loop([W(0x84, rd=2, ra=0, rb=0)] + red)) == 0                  # VMOD is not lowered
+    div = loop([W(0x93, rd=2, ra=0, rb=3)] + red)
+    assert lowered(div) == 2 and lowered(div, i64=True) == 0                     # integer division refused
+
+
 def test_jit_lowering_matches_fusion_detector(golden):
     def ok(code):
         c = np.array(code, np.uint32)

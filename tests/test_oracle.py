@@ -306,6 +306,43 @@ def test_segment_aggregates_restatement_against_reference_engine(is_f64):
     assert done >= 20
 
 
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_expr_restatement_against_reference_engine(is_f64):
+    """vo_tier0_expr_* (loops with VADD..VMAX, VSADD..VSMAX and filters on any
+    computed vector, up to three reductions) against the reference engine
+    running the same words: every accumulator and partial bit-identical."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(91 if is_f64 else 92)
+    for trial in range(60):
+        n = int(rng.integers(1, 300)) * 4
+        regs = np.zeros(64, np.uint64)
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 2)
+            x[rng.integers(0, n, 6)] = 0.0
+            if trial % 4 == 0:
+                x[rng.integers(0, n, 2)] = np.nan
+            for r in tg.SCAL_REGS:
+                regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)) if rng.random() < 0.9 else 0.0)
+        else:
+            x = rng.integers(-1000, 1000, n).astype(np.int64)
+            for r in tg.SCAL_REGS:
+                regs[r] = int(rng.integers(-50, 50)) & (2**64 - 1)
+        regs[2] = n
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 7)))
+        acc, part = tg.oracle_expr(o, x, n, spec, regs, is_f64)
+        ref = regs.copy()
+        run = o.R.vr_engine_run_f64 if is_f64 else o.R.vr_engine_run_i64
+        run(o.ptr(x), n, o.ptr(code), code.size, o.ptr(ref))
+        for b in range(len(spec["red"])):
+            assert int(ref[tg.ACCS[b]]) == acc[b], (trial, b, spec)
+            assert int(ref[tg.PARTS[b]]) == part[b], (trial, b, spec)
+            if spec["red"][b] == 1:     # the lanes of the reduced register (vo_tier0_expr_col_*)
+                col = tg.oracle_expr_col(o, x, n, spec, regs, spec["bvreg"][b], is_f64)
+                live = col[~np.isnan(col)] if is_f64 else col          # VCOUNT counts ordered != 0
+                assert int(np.count_nonzero(live)) == int(ref[tg.ACCS[b]]), (trial, b, spec)
+
+
 def test_jit_restatement(golden, x42):
     g = golden["seed42"]
     r = o.O.vo_jit_f64(o.ptr(x42), x42.size, x42.size, 0, 0xC5, 500.0)

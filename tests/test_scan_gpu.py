@@ -272,6 +272,74 @@ def test_tier0_filter_chains_and_several_reductions(gpu, is_f64):
 
 
 @pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_vector_arithmetic(gpu, is_f64):
+    """Loops with vector arithmetic (VADD..VMAX, VSADD..VSMAX) and filters on
+    computed vectors, up to three reductions (generated words): partials,
+    counts, min and max equal the oracle's bits (vo_tier0_expr_*, pinned
+    against the reference engine), NaN results compared as NaN; f64 sums
+    within 2 ulp of the exact sum of the reduced register's lanes; i64 sums
+    exact.  Integer vector division is refused."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(71 if is_f64 else 72)
+
+    def same(got, want):
+        if is_f64 and math.isnan(o.bits_f64(want)):
+            return math.isnan(o.bits_f64(got))
+        return got == want
+
+    for trial in range(30):
+        n = int(rng.integers(1, 3000)) * 4
+        regs = np.zeros(64, np.uint64)
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 2)
+            x[rng.integers(0, n, 6)] = 0.0
+            if trial % 4 == 0:
+                x[rng.integers(0, n, 2)] = np.nan
+            for r in tg.SCAL_REGS:
+                regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)) if rng.random() < 0.9 else 0.0)
+            e = vx.EngineF64()
+        else:
+            x = rng.integers(-2**40, 2**40, n).astype(np.int64)
+            x[rng.integers(0, n, 6)] = 0
+            for r in tg.SCAL_REGS:
+                regs[r] = int(rng.integers(-2**20, 2**20)) & (2**64 - 1)
+            e = vx.EngineI64()
+        regs[2] = n
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 9)))
+        e.add_segment(x)
+        for r in (2,) + tg.SCAL_REGS:
+            e.set_scalar(r, int(regs[r]))
+        e.load_program([int(c) for c in code])
+        e.run()
+        acc, part = tg.oracle_expr(o, x, n, spec, regs, is_f64)
+        assert e.get_scalar(1) == n
+        for b, red in enumerate(spec["red"]):
+            assert same(e.get_scalar(tg.PARTS[b]), part[b]), (trial, b, spec)
+            got = e.get_scalar(tg.ACCS[b])
+            if red == 0 and is_f64:
+                g = o.bits_f64(got)
+                if math.isnan(o.bits_f64(acc[b])):
+                    assert math.isnan(g)
+                else:
+                    col = tg.oracle_expr_col(o, x, n, spec, regs, spec["bvreg"][b], is_f64)
+                    assert close_f64(g, o.exact_sum(col)), (trial, b, g)
+            else:
+                assert same(got, acc[b]), (trial, b, spec)
+    if not is_f64:
+        e = vx.EngineI64()
+        x = np.arange(1, 65, dtype=np.int64)
+        e.add_segment(x)
+        e.set_scalar(2, 64)
+        e.set_scalar(3, 7)
+        e.load_program([tg.w(0x10, rd=4, imm=4), tg.w(0x70, rd=0, ra=1), tg.w(0x93, rd=1, ra=0, rb=3),
+                        tg.w(0xD0, rd=5, ra=1), tg.w(0x20, rd=8, ra=8, rb=5), tg.w(0x20, rd=1, ra=1, rb=4),
+                        tg.w(0x40, ra=1, rb=2), tg.w(0x52, imm=(1 - 8) & 0xFFF), tg.w(0x01)])
+        with pytest.raises(RuntimeError, match="integer vector division"):
+            e.run()
+
+
+@pytest.mark.parametrize("is_f64", [True, False])
 def test_tier0_scalar_prefix_and_epilogue(gpu, is_f64):
     """Scalar words (integer, f64 and bitwise arithmetic, DIV/MOD/DIVF by
     zero stopping the program) before and after a multi-reduction loop: all

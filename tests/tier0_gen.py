@@ -144,3 +144,106 @@ def last_agg_writers(words):
         if 0xE0 <= op <= 0xED and ((int(x) >> 12) & 0xF) == 0:
             out[(int(x) >> 8) & 0xF] = op
     return out
+
+
+VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x87, 0x88]             # VADD VSUB VMUL VDIV VMIN VMAX
+VEC_UN = [0x85, 0x86]                                       # VNEG VABS
+VEC_SCAL = [0x90, 0x91, 0x92, 0x93, 0x95, 0x96]            # VSADD VSSUB VSMUL VSDIV VSMIN VSMAX
+SCAL_REGS = (3, 6, 13)                                      # scalar operands / thresholds
+
+
+def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
+    """Loop with vector arithmetic and filters on V0 = VLOAD and up to three
+    reductions.  Returns (words, spec) for vo_tier0_expr_* (division only on
+    f64 engines)."""
+    code = [w(0x10, rd=4, imm=4)]
+    vload = len(code)
+    code.append(w(0x70, rd=0, ra=1, imm=0))
+    defined = [0]
+    ops = []
+    for k in range(nops):
+        dst = k + 1
+        kind = rng.random()
+        a = int(rng.choice(defined))
+        b = int(rng.choice(defined))
+        sreg = int(rng.choice(SCAL_REGS))
+        if kind < 0.3:
+            op = int(rng.choice([o for o in VEC_BIN if is_f64 or o != 0x83]))
+            code.append(w(op, rd=dst, ra=a, rb=b))
+            ops.append((op, dst, a, b, 0, -1))
+        elif kind < 0.4:
+            op = int(rng.choice(VEC_UN))
+            code.append(w(op, rd=dst, ra=a))
+            ops.append((op, dst, a, 0, 0, -1))
+        elif kind < 0.75:
+            op = int(rng.choice([o for o in VEC_SCAL if is_f64 or o != 0x93]))
+            code.append(w(op, rd=dst, ra=a, rb=sreg))
+            ops.append((op, dst, a, 0, 0, sreg))
+        else:
+            cmp = int(rng.choice([1, 2, 3, 4, 5]))
+            code.append(w(FILTER_OPS[cmp], rd=dst, ra=a, rb=sreg))
+            ops.append((0xC0 | (cmp + 1), dst, a, 0, cmp, sreg))
+        defined.append(dst)
+    nb = int(rng.integers(1, 4))
+    bvreg, reds_used, accops = [], [], []
+    for b in range(nb):
+        vreg = int(rng.choice(defined[1:] if len(defined) > 1 else defined))
+        red = int(rng.choice(reds))
+        code.append(w(RED_OPS[red], rd=PARTS[b], ra=vreg))
+        if red == 0:
+            accop = 0 if is_f64 else 1
+            code.append(w(0x2A if is_f64 else 0x20, rd=ACCS[b], ra=ACCS[b], rb=PARTS[b]))
+        elif red == 1:
+            accop = 1
+            code.append(w(0x20, rd=ACCS[b], ra=ACCS[b], rb=PARTS[b]))
+        else:
+            accop = 2 if red == 2 else 3
+            code.append(w(0x40, ra=ACCS[b], rb=PARTS[b]))
+            code.append(w(0x5A if red == 2 else 0x5C, imm=2))
+            code.append(w(0x11, rd=ACCS[b], ra=PARTS[b]))
+        bvreg.append(vreg)
+        reds_used.append(red)
+        accops.append(accop)
+    code.append(w(0x20, rd=1, ra=1, rb=4))
+    code.append(w(0x40, ra=1, rb=2))
+    jnz = len(code)
+    code.append(w(0x52, imm=(vload - jnz) & 0xFFF))
+    code.append(w(0x01))
+    return np.array(code, np.uint32), {"ops": ops, "bvreg": bvreg, "red": reds_used, "accop": accops}
+
+
+def _expr_words(spec, regs, is_f64):
+    ops = spec["ops"]
+    code = np.zeros(5 * max(len(ops), 1), np.int32)
+    dt = np.float64 if is_f64 else np.int64
+    sval = np.zeros(max(len(ops), 1), dt)
+    for k, (op, dst, a, b, cmp, sreg) in enumerate(ops):
+        code[5 * k:5 * k + 5] = (op, dst, a, b, cmp)
+        if sreg >= 0:
+            sval[k] = np.uint64(regs[sreg]).view(dt)
+    return code, sval
+
+
+def oracle_expr_col(o, x, total, spec, regs, target, is_f64=True):
+    """Lanes of vector register `target` over rows [0, total) (vo_tier0_expr_col_*)."""
+    code, sval = _expr_words(spec, regs, is_f64)
+    out = np.zeros(total, np.float64 if is_f64 else np.int64)
+    fn = o.O.vo_tier0_expr_col_f64 if is_f64 else o.O.vo_tier0_expr_col_i64
+    fn(o.ptr(x), x.size, 0, total, 0, len(spec["ops"]), o.ptr(code), o.ptr(sval), target, o.ptr(out))
+    return out
+
+
+def oracle_expr(o, x, total, spec, regs, is_f64=True, acc0=None):
+    """vo_tier0_expr_* on the spec; scalar operands from regs (register bits)."""
+    ops = spec["ops"]
+    code, sval = _expr_words(spec, regs, is_f64)
+    nb = len(spec["red"])
+    bvreg = np.array(spec["bvreg"], np.int32)
+    red = np.array(spec["red"], np.int32)
+    accop = np.array(spec["accop"], np.int32)
+    acc = np.array(acc0 if acc0 is not None else [0] * nb, np.uint64)
+    part = np.zeros(nb, np.uint64)
+    fn = o.O.vo_tier0_expr_f64 if is_f64 else o.O.vo_tier0_expr_i64
+    fn(o.ptr(x), x.size, 0, total, 0, len(ops), o.ptr(code), o.ptr(sval), nb, o.ptr(bvreg), o.ptr(red),
+       o.ptr(accop), o.ptr(acc), o.ptr(part))
+    return [int(a) for a in acc], [int(p) for p in part]

@@ -60,6 +60,7 @@ struct Engine {
     AggSlot agg[16];
     bool agg_ready = false;
     SegaggBufs segagg;
+    DevColumn expr;                  // a reduced vector of a loop with vector arithmetic
 };
 
 size_t lanes_of(int dt) { return 32 / dtype_size(dt); }
@@ -204,6 +205,46 @@ bool run_tier0_block(const Plan::Block& b, const PredDev& pred, int dtype, const
 }
 }  // namespace
 
+// Loops with vector arithmetic: the body's vector words are evaluated per row
+// (lanes are independent) into the reduced register's values over every row
+// the loop visits, [off, total), rows past Count loading 0; each block then
+// reduces that column as an unfiltered loop would.
+bool run_tier0_expr(Engine& e, uint64_t* regs, uint64_t off, uint64_t total, StreamCtx* c) {
+    const Plan& p = e.plan;
+    const uint64_t kl = 4;
+    if (e.dtype == DT_I64)
+        for (const Plan::VOp& v : p.vops)
+            if (v.op == 0x83 || v.op == 0x93) {
+                set_error("integer vector division is not lowered");
+                return false;
+            }
+    const uint64_t n = total - off;
+    if (n > (1ull << 31)) { set_error("a loop with vector arithmetic over %llu rows (more than 2^31) is not lowered",
+                                      (unsigned long long)n); return false; }
+    Seg& s = *e.segs[p.seg];
+    const uint64_t dend = total < s.count ? total : s.count;
+    const void* d = seg_rows(s, e.dtype, off < dend ? off : dend, dend, c);
+    if (!d) return false;
+    if (!e.expr.ensure(n * 8)) return false;
+    ExprProg prog{};
+    prog.nops = (int32_t)p.vops.size();
+    prog.vload = p.vload_dst;
+    for (size_t k = 0; k < p.vops.size(); ++k) {
+        const Plan::VOp& v = p.vops[k];
+        prog.ops[k] = ExprOp{v.op, v.dst, v.a, v.b, v.cmp, v.sreg != 0xFF ? regs[v.sreg] : 0};
+    }
+    const PredDev all = make_pred(e.dtype, CMP_TRUE, 0);
+    for (const Plan::Block& b : p.blocks) {
+        prog.target = b.vreg;
+        if (!expr_eval(e.dtype, d, s.count, off, n, prog, e.expr.dev, c)) return false;
+        if (!run_tier0_block(b, all, e.dtype, e.expr.dev, n, 0, n, n / kl, regs, c)) return false;
+    }
+    regs[p.off] = total;
+    if (run_scalar(p.epilogue, regs) < 0) return false;
+    std::memcpy(e.regs, regs, sizeof(uint64_t) * 64);
+    return true;
+}
+
 bool run_tier0(Engine& e, StreamCtx* c) {
     const Plan& p = e.plan;
     if (e.dtype != DT_F64 && e.dtype != DT_I64) { set_error("Tier-0 loops are lowered for f64 and i64 engines only"); return false; }
@@ -226,6 +267,7 @@ bool run_tier0(Engine& e, StreamCtx* c) {
                   "(JNZ loops until offset == total)", p.tot, p.off, (long long)(total - off), (unsigned long long)kl);
         return false;
     }
+    if (p.has_arith) return run_tier0_expr(e, regs, off, total, c);
     // each block's predicate: the conjunction of its filter chain
     std::vector<PredDev> preds(p.blocks.size());
     for (size_t k = 0; k < p.blocks.size(); ++k) {
@@ -759,6 +801,9 @@ int voxel_gpu_jit_lowerable(const uint32_t* code, size_t len) {
 
 int voxel_gpu_engine_lowered(const void* h) {
     const Engine& e = *static_cast<const Engine*>(h);
+    if (e.plan.kind == Plan::TIER0 && e.plan.has_arith && e.dtype == DT_I64)
+        for (const Plan::VOp& v : e.plan.vops)
+            if (v.op == 0x83 || v.op == 0x93) return 0;      // integer vector division is not lowered
     return e.plan.kind == Plan::FAST ? 1 : e.plan.kind == Plan::TIER0 ? 2 : e.plan.kind == Plan::STRAIGHT ? 3 : 0;
 }
 

@@ -148,20 +148,27 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     p.vcount = imm_of(vl) & 0xFF;
     p.off = rOff;
 
-    // vector register -> filters applied to it (-1: not defined in the loop)
+    // vector register -> filters applied to it (-1: not defined in the loop,
+    // -2: computed by vector arithmetic)
     int chain_len[16];
     uint8_t chain[16][2];
     for (int v = 0; v < 16; ++v) chain_len[v] = -1;
     chain_len[vA] = 0;
+    p.vload_dst = vA;
+    bool chain_ok = true;
+    auto defined = [&](uint8_t v) { return chain_len[v] != -1; };
+    auto vec_arith = [](uint8_t op) {          // lowered vector arithmetic (engine.hpp:2221-2350, 2473-2590)
+        return (op >= 0x80 && op <= 0x83) || op == 0x85 || op == 0x86 || op == 0x87 || op == 0x88 ||
+               (op >= 0x90 && op <= 0x93) || op == 0x95 || op == 0x96;
+    };
     while (i < len) {
         const uint32_t w = c[i];
         const uint8_t op = op_of(w);
         if (op >= OP_VFILTER && op <= OP_VFILTER_GE) {
             const uint8_t src = ra_of(w), dst = rd_of(w);
-            if (chain_len[src] < 0) return none("VFILTER reads a vector the loop does not define");
-            if (chain_len[dst] >= 0) return none("VFILTER overwrites a vector the loop already defines");
-            if (chain_len[src] >= 2) return none("more than two chained VFILTERs");
-            if (p.filts.size() >= 4) return none("more than four VFILTERs");
+            if (!defined(src)) return none("VFILTER reads a vector the loop does not define");
+            if (defined(dst)) return none("VFILTER overwrites a vector the loop already defines");
+            if (p.vops.size() >= 16) return none("more than sixteen vector words in the loop");
             Plan::Filt f;
             if (op == OP_VFILTER) {
                 int mode = imm_of(w) & 0x7;        // HandleVFILTER, engine.hpp:2870-2894
@@ -170,10 +177,28 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
                 f.cmp = cmp_of_filter_op(op);
             }
             f.thr = rb_of(w);
-            chain_len[dst] = chain_len[src] + 1;
-            for (int k = 0; k < chain_len[src]; ++k) chain[dst][k] = chain[src][k];
-            chain[dst][chain_len[src]] = (uint8_t)p.filts.size();
+            if (chain_len[src] >= 0 && chain_len[src] < 2 && p.filts.size() < 4) {
+                chain_len[dst] = chain_len[src] + 1;
+                for (int k = 0; k < chain_len[src]; ++k) chain[dst][k] = chain[src][k];
+                chain[dst][chain_len[src]] = (uint8_t)p.filts.size();
+            } else {
+                if (chain_len[src] >= 0) chain_ok = false;   // too deep for one range predicate
+                chain_len[dst] = -2;
+            }
             p.filts.push_back(f);
+            p.vops.push_back(Plan::VOp{op, dst, src, 0, f.thr, f.cmp});
+            ++i;
+            continue;
+        }
+        if (vec_arith(op)) {
+            const uint8_t dst = rd_of(w), a = ra_of(w), b = rb_of(w);
+            const bool scal = op >= 0x90, unary = op == 0x85 || op == 0x86;
+            if (!defined(a) || (!scal && !unary && !defined(b))) return none("vector arithmetic reads a vector the loop does not define");
+            if (defined(dst)) return none("vector arithmetic overwrites a vector the loop already defines");
+            if (p.vops.size() >= 16) return none("more than sixteen vector words in the loop");
+            p.vops.push_back(Plan::VOp{op, dst, a, (uint8_t)(scal || unary ? 0 : b), (uint8_t)(scal ? b : 0xFF), 0});
+            chain_len[dst] = -2;
+            p.has_arith = true;
             ++i;
             continue;
         }
@@ -184,12 +209,13 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
         else if (op == OP_VRED_MAX) red = 3;
         else break;
         const uint8_t src = ra_of(w), rP = rd_of(w);
-        if (chain_len[src] < 0) return none("reduction reads a vector the loop does not define");
+        if (!defined(src)) return none("reduction reads a vector the loop does not define");
         if (p.blocks.size() >= 4) return none("more than four reductions in the loop");
         Plan::Block b;
         b.red = red;
         b.part = rP;
-        b.nf = (uint8_t)chain_len[src];
+        b.vreg = src;
+        b.nf = chain_len[src] > 0 ? (uint8_t)chain_len[src] : 0;
         b.f[0] = chain_len[src] > 0 ? chain[src][0] : 0;
         b.f[1] = chain_len[src] > 1 ? chain[src][1] : 0;
         ++i;
@@ -218,6 +244,7 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
         if (i >= len) return none("loop body ends early");
         return none("loop reduction is not VSUM/VCOUNT/VRED_MIN/VRED_MAX");
     }
+    if (!p.has_arith && !chain_ok) return none("more than two chained VFILTERs");
 
     if (i + 2 >= len) return none("loop control missing");
     uint32_t adw = c[i], cw = c[i + 1], jw = c[i + 2];
@@ -267,6 +294,8 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     if (wr[rStep] || wr[rTot]) return none("loop overwrites its own bound, step or threshold");
     for (const Plan::Filt& f : p.filts)
         if (f.cmp != 7 && wr[f.thr]) return none("loop overwrites its own bound, step or threshold");
+    for (const Plan::VOp& v : p.vops)
+        if (v.sreg != 0xFF && v.cmp != 7 && wr[v.sreg]) return none("loop overwrites a scalar operand of its vector arithmetic");
     p.kind = Plan::TIER0;
     return p;
 }

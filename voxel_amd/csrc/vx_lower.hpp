@@ -62,9 +62,21 @@ struct Plan {
         uint8_t part, acc;           // partial and accumulator registers
         uint8_t nf;                  // filters applied to the reduced vector (0..2) ...
         uint8_t f[2];                // ... as indices into filts, in program order
+        uint8_t vreg;                // the reduced vector register
     };
     std::vector<Filt> filts;
     std::vector<Block> blocks;
+    // Loops with vector arithmetic: every vector word of the body in program
+    // order (arithmetic and filters), evaluated per row into the reduced
+    // registers' values.
+    struct VOp {
+        uint8_t op, dst, a, b;       // opcode, vector registers
+        uint8_t sreg;                // scalar operand / threshold register (0xFF none)
+        int cmp;                     // filters: kCmp code, 7 never
+    };
+    bool has_arith = false;
+    uint8_t vload_dst = 0;
+    std::vector<VOp> vops;
 };
 
 // Interpreter view: fast path first (exactly DetectFastPath), then the

@@ -83,7 +83,77 @@ unsigned gridn(uint64_t n, unsigned cap) {
     return (unsigned)(b ? (b < cap ? b : cap) : 1);
 }
 
+template <typename T>
+__device__ __forceinline__ bool expr_pass(int cmp, T x, T t) {
+    switch (cmp) {                            // VFilterImpl predicates (engine.hpp:2896-2924)
+    case 0: return x == t;
+    case 1: return x != t;
+    case 2: return x < t;
+    case 3: return x <= t;
+    case 4: return x > t;
+    case 5: return x >= t;
+    default: return false;
+    }
+}
+
+// Row-wise evaluation of the body's vector words (engine.hpp:2221-2350,
+// 2473-2590, 1090-1110): each lane of a vector word depends on the same lane
+// of its operands only.
+template <typename T>
+__global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, uint64_t count, uint64_t off,
+                                                   uint64_t n, ExprProg prog, T* __restrict__ out) {
+    constexpr bool F = sizeof(T) == 8 && (T)0.5 != (T)0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        T v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = T(0);
+        v[prog.vload] = off + i < count ? col[off + i] : T(0);
+        for (int k = 0; k < prog.nops; ++k) {
+            const ExprOp& o = prog.ops[k];
+            const T x = v[o.a], y = v[o.b];
+            T s;
+            __builtin_memcpy(&s, &o.s, sizeof(T));
+            T r;
+            switch (o.op) {
+            case 0x80: r = F ? x + y : (T)((uint64_t)x + (uint64_t)y); break;                   // VADD
+            case 0x81: r = F ? x - y : (T)((uint64_t)x - (uint64_t)y); break;                   // VSUB
+            case 0x82: r = F ? x * y : (T)((uint64_t)x * (uint64_t)y); break;                   // VMUL
+            case 0x83: r = y == T(0) ? (T)__builtin_nan("") : x / y; break;                      // VDIV (f64)
+            case 0x85: r = F ? -x : (T)(0 - (uint64_t)x); break;                                 // VNEG
+            case 0x86:                                                                          // VABS
+                if constexpr (F) r = __builtin_fabs(x);
+                else r = x < T(0) ? (T)(0 - (uint64_t)x) : x;
+                break;
+            case 0x87: r = F ? ((x < y || x != x) ? x : y) : (x < y ? x : y); break;             // VMIN
+            case 0x88: r = F ? ((x > y || y != y) ? x : y) : (x > y ? x : y); break;             // VMAX
+            case 0x90: r = F ? x + s : (T)((uint64_t)x + (uint64_t)s); break;                   // VSADD
+            case 0x91: r = F ? x - s : (T)((uint64_t)x - (uint64_t)s); break;                   // VSSUB
+            case 0x92: r = F ? x * s : (T)((uint64_t)x * (uint64_t)s); break;                   // VSMUL
+            case 0x93: r = s == T(0) ? (T)__builtin_nan("") : x / s; break;                      // VSDIV (f64)
+            case 0x95: r = F ? ((x < s || x != x) ? x : s) : (x < s ? x : s); break;             // VSMIN
+            case 0x96: r = F ? ((x > s || s != s) ? x : s) : (x > s ? x : s); break;             // VSMAX
+            default: r = expr_pass<T>(o.cmp, x, s) ? x : T(0); break;                            // VFILTER_*
+            }
+            v[o.dst] = r;
+        }
+        out[i] = v[prog.target];
+    }
+}
+
 }  // namespace
+
+bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,
+               StreamCtx* c) {
+    if (n == 0) return true;
+    if (dtype == DT_F64)
+        hipLaunchKernelGGL(k_expr_eval<double>, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, (const double*)col, count,
+                           off, n, prog, (double*)out);
+    else
+        hipLaunchKernelGGL(k_expr_eval<int64_t>, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, (const int64_t*)col,
+                           count, off, n, prog, (int64_t*)out);
+    return hip_ok(hipGetLastError(), "vector expression launch");
+}
 
 SegaggBufs::~SegaggBufs() {
     for (void* p : {(void*)k1, (void*)k2, (void*)v1, (void*)v2, (void*)parts})

@@ -37,6 +37,23 @@ bool segagg_sorted_keys(int dtype, const void* x, uint64_t n, SegaggBufs& b, Str
 // Ascending keys k[0, n) -> the distinct ones in out (device), *nout.
 bool segagg_unique(const uint64_t* k, uint64_t n, uint64_t* out, uint64_t* nout, StreamCtx* c);
 
+// Vector words of a Tier-0 loop body evaluated per row (the lanes of the
+// reference's vector registers are independent for these words).
+struct ExprOp {
+    uint8_t op, dst, a, b;           // opcode (VADD..VMAX, VSADD..VSMAX, VFILTER_*), vector registers
+    int32_t cmp;                     // filters: kCmp code, 7 never
+    uint64_t s;                      // scalar operand / threshold bits
+};
+struct ExprProg {
+    ExprOp ops[16];
+    int32_t nops, vload, target, pad;
+};
+
+// out[i] = register `prog.target` after the words, for rows off + i, i < n
+// (the loaded value is 0 for rows at or past count).  dtype DT_F64 / DT_I64.
+bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,
+               StreamCtx* c);
+
 // Order key <-> value bits (host).
 uint64_t order_key(uint64_t bits, int dtype);
 uint64_t order_unkey(uint64_t key, int dtype);
