@@ -20,7 +20,7 @@ Zipf keys use the continuous inverse CDF of the power law (rank =
 floor((1 - u(1 - N^(1-s)))^(1/(1-s)))), lognormal keys floor(exp(20 + 2Z)).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-breakdown]
-                    [--no-sort] [--no-join] [--no-groupby] [--no-serve]
+                    [--no-sort] [--no-join] [--no-groupby] [--no-serve] [--no-window]
 
 value = % of the 8 TB/s HBM peak achieved by the f64 compound scan, from the
 whole timed region (K launches bracketed by device synchronisation);
@@ -68,6 +68,7 @@ def parse():
     p.add_argument("--no-join", action="store_true")
     p.add_argument("--no-groupby", action="store_true")
     p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--no-window", action="store_true")
     p.add_argument("--sort-log2", type=int, default=29)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a device-to-device copy of one 2^rows-log2 f64 column (the known-size "
@@ -293,6 +294,72 @@ def bench_groupby() -> dict:
     return out
 
 
+# The reference's own figures for its window family (docs/Interpreter.md:58-65:
+# 1M f64 rows, window 20; quantile window 100), milliseconds on its CPU.
+WINDOW_DOC_MS = {"sum": 12.0, "mean": 15.0, "delta": 2.0, "stddev": 39.0, "variance": 37.0, "quantile": 1000.0}
+
+
+def _window_calls(lib, rows: int):
+    return (("delta", lambda x, o: lib.voxel_gpu_window_delta_f64(x, rows, o, rows, None)),
+            ("sum", lambda x, o: lib.voxel_gpu_window_sum_f64(x, rows, o, rows, 20, None)),
+            ("mean", lambda x, o: lib.voxel_gpu_window_mean_f64(x, rows, o, rows, 20, None)),
+            ("stddev", lambda x, o: lib.voxel_gpu_window_stddev_f64(x, rows, o, rows, 20, None)),
+            ("variance", lambda x, o: lib.voxel_gpu_window_variance_f64(x, rows, o, rows, 20, None)),
+            ("quantile", lambda x, o: lib.voxel_gpu_window_quantile_f64(x, rows, o, rows, 100, 0.05, None)))
+
+
+def bench_window() -> dict:
+    """EngineF64 window family on HBM-resident columns: the reference's 1M-row
+    table (window 20, quantile window 100) and a 2^27-row column."""
+    import torch
+    from voxel_amd._lib import lib
+    out = {}
+    for rows in (1_000_000, 1 << 27):
+        x = torch.empty(rows, dtype=torch.float64, device="cuda")
+        assert lib.voxel_gpu_fill_uniform(0, x.data_ptr(), rows, 31, 0.0, 1000.0, None) == 0
+        y = torch.empty(rows, dtype=torch.float64, device="cuda")
+        res = {}
+        for kind, call in _window_calls(lib, rows):
+            if kind == "quantile" and rows > 1_000_000:
+                continue
+
+            def run():
+                if call(x.data_ptr(), y.data_ptr()) < 0:
+                    raise RuntimeError("window %s failed" % kind)
+
+            t = _sync_time(run, reps=5)
+            r = {"ms": round(t * 1e3, 4), "gb_per_s": round(rows * 16 / t / 1e9, 1)}
+            if rows == 1_000_000:
+                r["reference_doc_ms"] = WINDOW_DOC_MS[kind]
+            res[kind] = r
+        out["rows_%d" % rows] = res
+        del x, y
+        torch.cuda.empty_cache()
+    return out
+
+
+def cpu_window_baseline() -> dict:
+    """The reference's window methods (Engine<f64>::Window*, built from its
+    sources into oracle/_ref) on one host core: 1M rows, window 20 (quantile:
+    window 100 on the first 100K rows, scaled to 1M)."""
+    import numpy as np
+    from oracle import oracle as o
+    if o.R is None:
+        return None
+    x = np.random.default_rng(3).random(1_000_000) * 1000.0
+    y = np.zeros_like(x)
+    res = {}
+    for k, kind in enumerate(("delta", "sum", "mean", "stddev", "variance", "quantile")):
+        n, w, scale = (100_000, 100, 10.0) if kind == "quantile" else (x.size, 20, 1.0)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            o.R.vr_window_f64(k, o.ptr(x), n, o.ptr(y), n, w, 0.05)
+            ts.append(time.perf_counter() - t0)
+        res[kind] = {"ms_1m": round(statistics.median(ts) * scale * 1e3, 3)}
+    return {"cores": 1, "kind": "reference", "host_cpu": _cpu_model(), "kinds": res}
+
+
 def bench_serve() -> dict:
     import numpy as np
     import torch
@@ -514,10 +581,14 @@ def main():
         out["join"] = bench_join()
     if not args.no_groupby:
         out["groupby"] = bench_groupby()
+    if not args.no_window:
+        out["window"] = bench_window()
     if not args.no_serve:
         out["serve"] = bench_serve()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        if not args.no_window and out["cpu_baseline"] is not None:
+            out["cpu_baseline"]["window"] = cpu_window_baseline()
     print(json.dumps(out))
 
 

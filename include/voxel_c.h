@@ -304,6 +304,36 @@ int       voxel_gpu_groupby_f64_general(int key_dtype, const void* keys_dev, con
 int       voxel_gpu_agg_results(void* agg, int64_t* keys, uint64_t* counts, double* sums, double* avgs,
                                 double* mins, double* maxs, size_t max_groups);
 
+/* ---- window family (Engine<f64>::WindowDelta .. WindowQuantile and
+ * TimeWindowSum, exec/engine.hpp:183-300; voxel_py EngineF64.window_*,
+ * bindings/voxel_bindings.cpp:127-180) ------------------------------------- */
+/* Device columns in and out.  Each returns what the reference method returns
+ * (the number of outputs written, 0 in its argument-check cases) and writes
+ * out[0, N) only; -1 on error (window outside 0..255; window_quantile with a
+ * NaN inside a window).  Sums, means, variances and standard deviations are
+ * bit-identical to the reference's left-to-right window loops.  The first
+ * five are stream-ordered (no host wait); window_quantile and
+ * time_window_sum wait for the stream. */
+int64_t   voxel_gpu_window_delta_f64(const double* data_dev, size_t count, double* out_dev, size_t out_len,
+                                     void* stream);
+int64_t   voxel_gpu_window_sum_f64(const double* data_dev, size_t count, double* out_dev, size_t out_len,
+                                   int window, void* stream);
+int64_t   voxel_gpu_window_mean_f64(const double* data_dev, size_t count, double* out_dev, size_t out_len,
+                                    int window, void* stream);
+int64_t   voxel_gpu_window_stddev_f64(const double* data_dev, size_t count, double* out_dev, size_t out_len,
+                                      int window, void* stream);
+int64_t   voxel_gpu_window_variance_f64(const double* data_dev, size_t count, double* out_dev, size_t out_len,
+                                        int window, void* stream);
+int64_t   voxel_gpu_window_quantile_f64(const double* data_dev, size_t count, double* out_dev, size_t out_len,
+                                        int window, double quantile, void* stream);
+/* Adds each row's value onto out[(ts - start_time) / duration_sec] for buckets
+ * in [0, out_len) (out is read, not cleared); returns the largest bucket
+ * written + 1 (1 when none).  Bucket sums are compensated (within a few ulp
+ * of the exact sum; the reference adds in row order). */
+int64_t   voxel_gpu_time_window_sum_f64(const int64_t* ts_dev, const double* vals_dev, size_t count,
+                                        int64_t start_time, int64_t duration_sec, double* out_dev, size_t out_len,
+                                        void* stream);
+
 /* ---- learned-CDF sort (no counterpart in the reference; same output as
  * voxel_gpu_sort_pairs_u64, ops/sort.hpp:83-160) ---------------------------- */
 /* FP32 MLP 1 -> 32 -> 32 -> 1 (ReLU), parameters packed as

@@ -196,6 +196,23 @@ void vr_topk_f64(const double* data, size_t n, size_t k, double* out, int larges
     ops::TopKSelect(data, n, k, out, largest != 0);
 }
 
+// Window family, static methods of Engine<f64> (exec/engine.hpp:183-300).
+int64_t vr_window_f64(int kind, const double* d, size_t n, double* out, size_t out_len, int window, double q) {
+    const u8 w = static_cast<u8>(window);
+    switch (kind) {
+    case 0: return (int64_t)Engine<f64>::WindowDelta(d, n, out, out_len);
+    case 1: return (int64_t)Engine<f64>::WindowSum(d, n, out, out_len, w);
+    case 2: return (int64_t)Engine<f64>::WindowMean(d, n, out, out_len, w);
+    case 3: return (int64_t)Engine<f64>::WindowStdDev(d, n, out, out_len, w);
+    case 4: return (int64_t)Engine<f64>::WindowVariance(d, n, out, out_len, w);
+    default: return (int64_t)Engine<f64>::WindowQuantile(d, n, out, out_len, w, q);
+    }
+}
+int64_t vr_time_window_sum_f64(const int64_t* ts, const double* v, size_t n, int64_t start, int64_t dur, double* out,
+                               size_t out_len) {
+    return (int64_t)Engine<f64>::TimeWindowSum(ts, v, n, start, dur, out, out_len);
+}
+
 // Instruction encoders (bytecode/instruction.hpp) for program-word fixtures.
 uint32_t vr_encode(uint8_t op, uint8_t rd, uint8_t ra, uint8_t rb, uint16_t imm) {
     return Instruction::Encode(static_cast<Opcode>(op), rd, ra, rb, imm);

@@ -991,3 +991,73 @@ uint64_t vo_splitmix64(uint64_t x) {
     x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
     return x ^ (x >> 31);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Window family, the static methods of Engine<f64> (engine.hpp:183-300):   */
+/* kind 0 WindowDelta, 1 WindowSum, 2 WindowMean, 3 WindowStdDev,           */
+/* 4 WindowVariance, 5 WindowQuantile.  Returns the number of outputs       */
+/* written; each window is summed left to right as in the reference.        */
+/* Quantile: the lo-th and (lo+1)-th smallest of the first min(window, 128) */
+/* values (nth_element then the minimum right of it); inputs without NaN.   */
+/* ------------------------------------------------------------------------ */
+static int vo_cmp_f64(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return x < y ? -1 : x > y;
+}
+
+int64_t vo_window_f64(int kind, const double* d, size_t n, double* out, size_t out_len, int window, double q) {
+    size_t w = (size_t)window, N, j, k;
+    if (kind == 0) {
+        if (n < 2) return 0;
+        N = n - 1 < out_len ? n - 1 : out_len;
+        for (j = 0; j < N; ++j) out[j] = d[j + 1] - d[j];
+        return (int64_t)N;
+    }
+    if (kind == 1 || kind == 2) { if (w == 0 || n < w) return 0; }
+    else if (kind == 3 || kind == 4) { if (w < 2 || n < w) return 0; }
+    else if (w < 2 || n < w || q < 0.0 || q > 1.0) return 0;
+    N = n - w + 1 < out_len ? n - w + 1 : out_len;
+    for (j = 0; j < N; ++j) {
+        if (kind <= 4) {
+            double s = 0.0;
+            for (k = 0; k < w; ++k) s += d[j + k];
+            if (kind == 1) { out[j] = s; continue; }
+            if (kind == 2) { out[j] = s / (double)w; continue; }
+            double mean = s / (double)w, ssq = 0.0;
+            for (k = 0; k < w; ++k) { double e = d[j + k] - mean; ssq += e * e; }
+            out[j] = kind == 3 ? sqrt(ssq / (double)w) : ssq / (double)w;
+        } else {
+            double buf[128];
+            size_t ww = w < 128 ? w : 128, lo, hi;
+            double pos, vlo, vhi;
+            memcpy(buf, d + j, ww * sizeof(double));
+            qsort(buf, ww, sizeof(double), vo_cmp_f64);
+            pos = q * (double)(ww - 1);
+            lo = (size_t)pos;
+            if (lo >= ww) lo = ww - 1;
+            hi = lo + 1 < ww ? lo + 1 : lo;
+            vlo = buf[lo];
+            if (lo == hi) { out[j] = vlo; continue; }
+            vhi = buf[lo + 1];
+            out[j] = vlo + (vhi - vlo) * (pos - (double)lo);
+        }
+    }
+    return (int64_t)N;
+}
+
+/* TimeWindowSum (engine.hpp:229-241): bucket = (ts - start) / duration     */
+/* (truncating, so rows up to one duration before start fall in bucket 0); */
+/* rows in [0, out_len) are added onto out in row order.                    */
+int64_t vo_time_window_sum_f64(const int64_t* ts, const double* v, size_t n, int64_t start, int64_t dur, double* out,
+                               size_t out_len) {
+    size_t i, maxb = 0;
+    if (dur <= 0 || n == 0 || out_len == 0) return 0;
+    for (i = 0; i < n; ++i) {
+        int64_t b = (int64_t)((uint64_t)ts[i] - (uint64_t)start) / dur;
+        if (b >= 0 && (size_t)b < out_len) {
+            out[b] += v[i];
+            if ((size_t)b > maxb) maxb = (size_t)b;
+        }
+    }
+    return (int64_t)(maxb + 1);
+}

@@ -513,3 +513,81 @@ def test_mlp_oracle_pinned_by_finite_differences():
         h1 = [max(w1[j] * xi + b1[j], 0.0) for j in range(32)]
         h2 = [max(sum(w2[k][j] * h1[j] for j in range(32)) + b2[k], 0.0) for k in range(32)]
         assert abs(sum(w3[k] * h2[k] for k in range(32)) + b3 - yi) < 1e-12
+
+
+WINDOW_KINDS = ("delta", "sum", "mean", "stddev", "variance", "quantile")
+
+
+def _window(lib_fn, kind, x, out_len, window, q, fill=-7.0):
+    out = np.full(out_len, fill)
+    n = lib_fn(WINDOW_KINDS.index(kind), o.ptr(x), x.size, o.ptr(out), out_len, window, q)
+    return n, out
+
+
+def test_window_family_closed_forms():
+    """vo_window_f64 against numpy on integer-valued data (every window sum
+    exact): sums = convolution, delta = diff, mean = sum / w, variance and
+    stddev = the two-pass population formulas, quantile = linear
+    interpolation between sorted neighbours; the reference's 0-return cases."""
+    rng = np.random.default_rng(3)
+    x = rng.integers(-1000, 1000, 500).astype(np.float64)
+    for w in (1, 2, 5, 20, 128, 200):
+        n, out = _window(o.O.vo_window_f64, "sum", x, 600, w, 0.0)
+        assert n == x.size - w + 1
+        assert np.array_equal(out[:n], np.convolve(x, np.ones(w), "valid"))
+        assert np.all(out[n:] == -7.0)                      # outputs past N untouched
+        n, out = _window(o.O.vo_window_f64, "mean", x, 600, w, 0.0)
+        assert np.array_equal(out[:n], np.convolve(x, np.ones(w), "valid") / w)
+    n, out = _window(o.O.vo_window_f64, "delta", x, 600, 0, 0.0)
+    assert n == 499 and np.array_equal(out[:n], np.diff(x))
+    win = np.lib.stride_tricks.sliding_window_view(x, 20)
+    n, out = _window(o.O.vo_window_f64, "variance", x, 600, 20, 0.0)
+    assert np.allclose(out[:n], win.var(axis=1), rtol=1e-13, atol=0)
+    n, out = _window(o.O.vo_window_f64, "stddev", x, 600, 20, 0.0)
+    assert np.allclose(out[:n], win.std(axis=1), rtol=1e-13, atol=0)
+    for w, q in ((20, 0.5), (100, 0.05), (200, 0.95), (7, 1.0), (7, 0.0)):
+        n, out = _window(o.O.vo_window_f64, "quantile", x, 600, w, q)
+        ww = min(w, 128)
+        srt = np.sort(np.lib.stride_tricks.sliding_window_view(x, w)[:, :ww], axis=1)
+        pos = q * (ww - 1)
+        lo = int(pos)
+        hi = min(lo + 1, ww - 1)
+        want = srt[:, lo] + (srt[:, hi] - srt[:, lo]) * (pos - lo)
+        assert n == x.size - w + 1 and np.array_equal(out[:n], want), (w, q)
+    # zero-output cases (engine.hpp:198, 246, 262, 279)
+    assert _window(o.O.vo_window_f64, "sum", x, 600, 0, 0.0)[0] == 0
+    assert _window(o.O.vo_window_f64, "stddev", x, 600, 1, 0.0)[0] == 0
+    assert _window(o.O.vo_window_f64, "quantile", x, 600, 5, 1.5)[0] == 0
+    assert _window(o.O.vo_window_f64, "sum", x[:3], 600, 4, 0.0)[0] == 0
+    assert _window(o.O.vo_window_f64, "delta", x[:1], 600, 0, 0.0)[0] == 0
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_window_family_restatement_against_reference():
+    """vo_window_f64 / vo_time_window_sum_f64 against Engine<f64>'s static
+    window methods: every output bit-identical, same return value."""
+    rng = np.random.default_rng(4)
+    for trial in range(40):
+        n = int(rng.integers(1, 3000))
+        x = rng.standard_normal(n) * 10.0 ** int(rng.integers(-3, 6))
+        kind = WINDOW_KINDS[trial % 6]
+        w = int(rng.choice([0, 1, 2, 3, 20, 100, 128, 129, 255]))
+        q = float(rng.choice([0.0, 0.05, 0.5, 0.99, 1.0, -0.5, 2.0]))
+        out_len = int(rng.integers(0, n + 10))
+        a, oa = _window(o.O.vo_window_f64, kind, x, out_len, w, q)
+        b, ob = _window(o.R.vr_window_f64, kind, x, out_len, w, q)
+        assert a == b and np.array_equal(oa.view(np.uint64), ob.view(np.uint64)), (trial, kind, w, q)
+    for trial in range(20):
+        n = int(rng.integers(0, 5000))
+        ts = np.sort(rng.integers(-500, 100000, n)).astype(np.int64)
+        if trial % 3 == 0:
+            rng.shuffle(ts)
+        v = rng.standard_normal(n) * 100
+        start = int(rng.integers(-1000, 1000))
+        dur = int(rng.choice([-5, 0, 1, 7, 3600]))
+        out_len = int(rng.integers(0, 200))
+        o1 = rng.standard_normal(out_len)
+        o2 = o1.copy()
+        a = o.O.vo_time_window_sum_f64(o.ptr(ts), o.ptr(v), n, start, dur, o.ptr(o1), out_len)
+        b = o.R.vr_time_window_sum_f64(o.ptr(ts), o.ptr(v), n, start, dur, o.ptr(o2), out_len)
+        assert a == b and np.array_equal(o1.view(np.uint64), o2.view(np.uint64)), trial

@@ -184,6 +184,13 @@ SIGS = {
     "voxel_gpu_join_set_probe_partitioning": (INT, [P, INT]),
     "voxel_gpu_groupby_f64": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
     "voxel_gpu_groupby_f64_general": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
+    "voxel_gpu_window_delta_f64": (I64, [P, SZ, P, SZ, P]),
+    "voxel_gpu_window_sum_f64": (I64, [P, SZ, P, SZ, INT, P]),
+    "voxel_gpu_window_mean_f64": (I64, [P, SZ, P, SZ, INT, P]),
+    "voxel_gpu_window_stddev_f64": (I64, [P, SZ, P, SZ, INT, P]),
+    "voxel_gpu_window_variance_f64": (I64, [P, SZ, P, SZ, INT, P]),
+    "voxel_gpu_window_quantile_f64": (I64, [P, SZ, P, SZ, INT, F64, P]),
+    "voxel_gpu_time_window_sum_f64": (I64, [P, P, SZ, I64, I64, P, SZ, P]),
     "voxel_gpu_agg_results": (INT, [P, P, P, P, P, P, P, SZ]),
     "voxel_gpu_mlp_param_count": (INT, []),
     "voxel_gpu_mlp_init": (INT, [P, U64]),

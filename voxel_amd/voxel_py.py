@@ -182,6 +182,40 @@ class EngineF64(_Engine):
     def set_scalar_f64(self, i: int, v: float) -> None:
         lib.voxel_engine_set_scalar_f64(self._h, i, float(v))
 
+    # Window family (voxel_bindings.cpp:127-180, Engine<f64>::Window*,
+    # exec/engine.hpp:183-300): host arrays in, the first N outputs written
+    # into `out`, N returned; the work runs on the GPU.
+    def window_delta(self, data, out) -> int:
+        return _window(lib.voxel_gpu_window_delta_f64, data, out)
+
+    def window_sum(self, data, out, window) -> int:
+        return _window(lib.voxel_gpu_window_sum_f64, data, out, _u8(window))
+
+    def window_mean(self, data, out, window) -> int:
+        return _window(lib.voxel_gpu_window_mean_f64, data, out, _u8(window))
+
+    def window_stddev(self, data, out, window) -> int:
+        return _window(lib.voxel_gpu_window_stddev_f64, data, out, _u8(window))
+
+    def window_variance(self, data, out, window) -> int:
+        return _window(lib.voxel_gpu_window_variance_f64, data, out, _u8(window))
+
+    def window_quantile(self, data, out, window, quantile) -> int:
+        return _window(lib.voxel_gpu_window_quantile_f64, data, out, _u8(window), float(quantile))
+
+    def time_window_sum(self, timestamps, values, start_time, duration_sec, out) -> int:
+        ts = _convert(timestamps, np.int64)
+        v = _convert(values, np.float64)
+        o = _out_array(out)
+        cnt = min(ts.size, v.size)
+        dts, dv, do = _Dev(ts), _Dev(v), _Dev(o)
+        n = int(lib.voxel_gpu_time_window_sum_f64(dts.ptr, dv.ptr, cnt, int(start_time), int(duration_sec), do.ptr,
+                                                  o.size, None))
+        if n < 0:
+            check("time_window_sum")
+        do.to_host(o)
+        return n
+
 
 class EngineI64(_Engine):
     """Engine<i64> (voxel_bindings.cpp:174-189)."""
@@ -268,6 +302,34 @@ class _Dev:
         if getattr(self, "ptr", None):
             lib.voxel_gpu_free(self.ptr)
             self.ptr = None
+
+
+def _u8(v) -> int:
+    """pybind11's u8 argument conversion: an int in 0..255, else TypeError."""
+    if isinstance(v, (bool, float)) or not isinstance(v, (int, np.integer)) or not 0 <= int(v) <= 255:
+        raise TypeError("incompatible function arguments: window must be an integer in 0..255")
+    return int(v)
+
+
+def _out_array(out) -> np.ndarray:
+    """py::array_t<f64> output: written in place when it already is a
+    C-contiguous f64 array, else into a converted copy (as pybind11 does)."""
+    if isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous and out.flags.writeable:
+        return out
+    return _convert(out, np.float64)
+
+
+def _window(fn, data, out, *args) -> int:
+    x = _convert(data, np.float64)
+    o = _out_array(out)
+    d = _Dev(x)
+    od = _Dev.empty(o.nbytes)
+    n = int(fn(d.ptr, x.size, od.ptr, o.size, *args, None))
+    if n < 0:
+        check("window")
+    if n > 0 and lib.voxel_gpu_memcpy_d2h(o.ctypes.data, od.ptr, n * 8) != 0:
+        check("voxel_gpu_memcpy_d2h")
+    return n
 
 
 class VectorFilterF64:
