@@ -20,7 +20,7 @@ Zipf keys use the continuous inverse CDF of the power law (rank =
 floor((1 - u(1 - N^(1-s)))^(1/(1-s)))), lognormal keys floor(exp(20 + 2Z)).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-breakdown]
-                    [--no-sort] [--no-join] [--no-groupby] [--no-serve] [--no-window]
+                    [--no-sort] [--no-join] [--no-groupby] [--no-serve] [--no-window] [--no-nulls]
 
 value = % of the 8 TB/s HBM peak achieved by the f64 compound scan, from the
 whole timed region (K launches bracketed by device synchronisation);
@@ -69,6 +69,7 @@ def parse():
     p.add_argument("--no-groupby", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-window", action="store_true")
+    p.add_argument("--no-nulls", action="store_true")
     p.add_argument("--sort-log2", type=int, default=29)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a device-to-device copy of one 2^rows-log2 f64 column (the known-size "
@@ -291,6 +292,63 @@ def bench_groupby() -> dict:
                                  "count_total_ok": int(res[1][: int(ng[0])].sum().item()) == rows}
         del zone, fare, res
         torch.cuda.empty_cache()
+    return out
+
+
+def bench_nulls(rows_log2: int = 28) -> dict:
+    """Scans over nullable columns: the headline compound filter + aggregate
+    (2 f64 columns of 2^28 rows) with a validity bitmap per column (about 10 %
+    nulls), NullableSegment SumValid/MinValid/MaxValid and CompactValid."""
+    import ctypes as C
+    import torch
+    from voxel_amd import scan as vs
+    from voxel_amd._lib import Agg, lib
+    n = 1 << rows_log2
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    b = torch.empty(n, dtype=torch.float64, device="cuda")
+    assert lib.voxel_gpu_fill_uniform(0, a.data_ptr(), n, 1, 0.0, 1000.0, None) == 0
+    assert lib.voxel_gpu_fill_uniform(0, b.data_ptr(), n, 2, 0.0, 1000.0, None) == 0
+    nw = (n + 63) // 64
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+
+    def words():
+        bits = torch.rand(nw, 64, device="cuda", generator=g) < 0.9
+        w = torch.zeros(nw, dtype=torch.int64, device="cuda")
+        for k in range(64):
+            w |= bits[:, k].to(torch.int64) << k
+        return w
+
+    wa, wb = words(), words()
+    d = vs.make_desc(a, vs.GT, 292.8932188, b, vs.LT, 707.1067812)
+    r = Agg()
+
+    def run():
+        if lib.voxel_gpu_scan_valid(C.byref(d), C.c_void_p(wa.data_ptr()), C.c_void_p(wb.data_ptr()), None,
+                                    C.byref(r)) != 0:
+            raise RuntimeError("scan_valid failed")
+
+    def run_plain():
+        if lib.voxel_gpu_scan(C.byref(d), None, C.byref(r)) != 0:
+            raise RuntimeError("scan failed")
+
+    t = _sync_time(run, reps=5)
+    tp = _sync_time(run_plain, reps=5)
+    bytes_ = n * 16 + 2 * nw * 8
+    out = {"scan_valid_compound_2x2^%d" % rows_log2: {
+        "ms": round(t * 1e3, 4), "gb_per_s": round(bytes_ / t / 1e9, 1),
+        "frac_of_peak": round(bytes_ / t / 1e9 / HBM_PEAK_GBS, 4), "count": int(r.count),
+        "same_scan_without_bitmaps_ms": round(tp * 1e3, 4)}}
+    t = _sync_time(lambda: vs.nullable_reduce(a, wa), reps=5)
+    out["nullable_reduce_2^%d" % rows_log2] = {"ms": round(t * 1e3, 4),
+                                               "gb_per_s": round((n * 8 + nw * 8) / t / 1e9, 1)}
+    o_ = torch.empty_like(a)
+    m = [0]
+    t = _sync_time(lambda: m.__setitem__(0, vs.nullable_compact(a, wa, o_)), reps=5)
+    out["nullable_compact_2^%d" % rows_log2] = {"ms": round(t * 1e3, 4), "kept": m[0],
+                                                "gb_per_s": round((n * 8 + nw * 8 + m[0] * 8) / t / 1e9, 1)}
+    del a, b, wa, wb, o_
+    torch.cuda.empty_cache()
     return out
 
 
@@ -583,6 +641,8 @@ def main():
         out["groupby"] = bench_groupby()
     if not args.no_window:
         out["window"] = bench_window()
+    if not args.no_nulls:
+        out["nulls"] = bench_nulls(args.rows_log2)
     if not args.no_serve:
         out["serve"] = bench_serve()
     if not args.no_cpu:

@@ -304,6 +304,33 @@ int       voxel_gpu_groupby_f64_general(int key_dtype, const void* keys_dev, con
 int       voxel_gpu_agg_results(void* agg, int64_t* keys, uint64_t* counts, double* sums, double* avgs,
                                 double* mins, double* maxs, size_t max_groups);
 
+/* ---- validity bitmaps (NullBitmap, data/nulls.hpp:16-235; NullableSegment<T>,
+ * nulls.hpp:241-420) over device columns ------------------------------------ */
+/* Validity words in NullBitmap's layout: bit (r & 63) of word r >> 6, 1 = valid,
+ * (n + 63) / 64 words (NullBitmap::Bits copied to HBM as is).  NULL = all rows
+ * valid. */
+/* voxel_gpu_scan over nullable columns: a row takes part only when its bit in
+ * valid_a (and, for a predicate on column b, in valid_b) is 1.  The filter
+ * bitmap ANDed with the validity words (ColumnFilter over the same layout,
+ * ops/filter.hpp:540-547), then COUNT/SUM/MIN/MAX, in one pass. */
+int       voxel_gpu_scan_valid(const voxel_gpu_scan_desc* d, const uint64_t* valid_a, const uint64_t* valid_b,
+                               void* stream, voxel_gpu_agg* out);
+typedef struct voxel_gpu_nullable_stats {
+    uint64_t valid_count, null_count;        /* ValidCount / NullCount (rows < n) */
+    double   sum, min, max;                  /* f64/f32: SumValid (compensated, rounded to the column type),
+                                                MinValid, MaxValid (NaN = NullSentinel when no row is valid) */
+    int64_t  isum, imin, imax;               /* i64/i32: SumValid (wrapping in the column type), MinValid,
+                                                MaxValid (lowest() when no row is valid) */
+} voxel_gpu_nullable_stats;
+/* NullableSegment<T>::SumValid / MinValid / MaxValid / ValidCount / NullCountVal
+ * of a device column (dtype VOXEL_GPU_F64/F32/I64/I32). */
+int       voxel_gpu_nullable_reduce(int dtype, const void* data_dev, const uint64_t* valid_dev, size_t n,
+                                    voxel_gpu_nullable_stats* out, void* stream);
+/* NullableSegment<T>::CompactValid: the valid rows' values in row order into
+ * out_dev (n entries of room), *count = how many (4- or 8-byte dtypes). */
+int       voxel_gpu_nullable_compact(int dtype, const void* data_dev, const uint64_t* valid_dev, size_t n,
+                                     void* out_dev, uint64_t* count, void* stream);
+
 /* ---- window family (Engine<f64>::WindowDelta .. WindowQuantile and
  * TimeWindowSum, exec/engine.hpp:183-300; voxel_py EngineF64.window_*,
  * bindings/voxel_bindings.cpp:127-180) ------------------------------------- */

@@ -66,6 +66,8 @@ _SIGS = {
     "vo_tier0_expr_col_f64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P]),
     "vo_tier0_expr_col_i64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P]),
     "vo_window_f64": (I64, [INT, P, SZ, P, SZ, INT, F64]),
+    "vo_nullable_f64": (None, [P, SZ, P, P, P, P]),
+    "vo_nullable_i64": (None, [P, SZ, P, P, P, P]),
     "vo_time_window_sum_f64": (I64, [P, P, SZ, I64, I64, P, SZ]),
     "vo_agg_create": (P, [INT]),
     "vo_agg_destroy": (None, [P]),
@@ -122,6 +124,8 @@ _RSIGS = {
     "vr_topk_f64": (None, [P, SZ, SZ, P, INT]),
     "vr_encode": (U32, [U8, U8, U8, U8, C.c_uint16]),
     "vr_window_f64": (I64, [INT, P, SZ, P, SZ, INT, F64]),
+    "vr_nullable_f64": (None, [P, SZ, P, P, P, P]),
+    "vr_nullable_i64": (None, [P, SZ, P, P, P, P]),
     "vr_time_window_sum_f64": (I64, [P, P, SZ, I64, I64, P, SZ]),
 }
 if R is not None:

@@ -26,6 +26,23 @@ int run_engine(T* data, size_t n, const uint32_t* code, size_t len, uint64_t* re
     for (size_t r = 0; r < 64; ++r) regs[r] = e.ScalarReg(r);
     return 0;
 }
+// NullableSegment<T> (data/nulls.hpp:241-420) over caller data and
+// NullBitmap words: res = {SumValid, MinValid, MaxValid}, counts =
+// {ValidCount, NullCountVal, rows after CompactValid}, compact = those rows.
+template <typename T>
+void nullable_run(T* data, size_t n, const uint64_t* words, T* res, uint64_t* counts, T* compact) {
+    NullableSegment<T> s(data, n);
+    s.Nulls.CopyFrom(words, (n + 63) / 64);
+    res[0] = s.SumValid();
+    res[1] = s.MinValid();
+    res[2] = s.MaxValid();
+    counts[0] = s.ValidCount();
+    counts[1] = s.NullCountVal();
+    Arena arena;
+    s.CompactValid(arena);
+    counts[2] = s.Data.Count;
+    if (s.Data.Count) std::memcpy(compact, s.Data.Data, s.Data.Count * sizeof(T));
+}
 }  // namespace
 
 extern "C" {
@@ -194,6 +211,13 @@ VR_AGG(vr_hash_agg_u32_f64, uint32_t)
 // TopKSelect (voxel_topk).
 void vr_topk_f64(const double* data, size_t n, size_t k, double* out, int largest) {
     ops::TopKSelect(data, n, k, out, largest != 0);
+}
+
+void vr_nullable_f64(double* d, size_t n, const uint64_t* w, double* res, uint64_t* counts, double* compact) {
+    nullable_run<f64>(d, n, w, res, counts, compact);
+}
+void vr_nullable_i64(int64_t* d, size_t n, const uint64_t* w, int64_t* res, uint64_t* counts, int64_t* compact) {
+    nullable_run<i64>(d, n, w, res, counts, compact);
 }
 
 // Window family, static methods of Engine<f64> (exec/engine.hpp:183-300).

@@ -1061,3 +1061,28 @@ int64_t vo_time_window_sum_f64(const int64_t* ts, const double* v, size_t n, int
     }
     return (int64_t)(maxb + 1);
 }
+
+/* ------------------------------------------------------------------------ */
+/* NullableSegment<T> (data/nulls.hpp:241-420) over data and NullBitmap      */
+/* words (bit r & 63 of word r >> 6, 1 = valid; rows >= n ignored):          */
+/* res = {SumValid (row order, in T), MinValid, MaxValid (first valid value, */
+/* then replaced only by a smaller / larger one; NullSentinel = NaN / lowest */
+/* when no row is valid)}, counts = {ValidCount, NullCount, rows kept by     */
+/* CompactValid}, compact = the valid rows in order.                         */
+/* ------------------------------------------------------------------------ */
+#define VO_DEF_NULLABLE(NAME, T, SENT)                                                       \
+void NAME(const T* d, size_t n, const uint64_t* w, T* res, uint64_t* counts, T* compact) {  \
+    T sum = 0, mn = SENT, mx = SENT;                                                         \
+    size_t valid = 0, i;                                                                     \
+    for (i = 0; i < n; ++i) {                                                                \
+        if (!((w[i >> 6] >> (i & 63)) & 1)) continue;                                        \
+        sum += d[i];                                                                         \
+        if (valid == 0) { mn = d[i]; mx = d[i]; }                                            \
+        else { if (d[i] < mn) mn = d[i]; if (d[i] > mx) mx = d[i]; }                         \
+        compact[valid++] = d[i];                                                             \
+    }                                                                                        \
+    res[0] = sum; res[1] = mn; res[2] = mx;                                                  \
+    counts[0] = valid; counts[1] = n - valid; counts[2] = valid;                             \
+}
+VO_DEF_NULLABLE(vo_nullable_f64, double, (double)NAN)
+VO_DEF_NULLABLE(vo_nullable_i64, int64_t, INT64_MIN)

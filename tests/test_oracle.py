@@ -591,3 +591,53 @@ def test_window_family_restatement_against_reference():
         a = o.O.vo_time_window_sum_f64(o.ptr(ts), o.ptr(v), n, start, dur, o.ptr(o1), out_len)
         b = o.R.vr_time_window_sum_f64(o.ptr(ts), o.ptr(v), n, start, dur, o.ptr(o2), out_len)
         assert a == b and np.array_equal(o1.view(np.uint64), o2.view(np.uint64)), trial
+
+
+def _random_validity(rng, n, trial):
+    nw = max((n + 63) // 64, 1)
+    kind = trial % 5
+    if kind == 0:
+        w = np.full(nw, ~np.uint64(0), np.uint64)                     # all valid
+    elif kind == 1:
+        w = np.zeros(nw, np.uint64)                                   # all null
+    else:
+        w = rng.integers(0, 2**63, nw, dtype=np.int64).astype(np.uint64) * np.uint64(2) + \
+            rng.integers(0, 2, nw).astype(np.uint64)
+        if kind == 3:
+            w[rng.integers(0, nw, max(nw // 3, 1))] = 0               # whole null words
+    return w                                                          # bits past n may be set
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_nullable_restatement_against_reference(is_f64):
+    """vo_nullable_* against NullableSegment<T> (SumValid, MinValid,
+    MaxValid, ValidCount, NullCountVal, CompactValid) on the same data and
+    NullBitmap words: every value bit-identical."""
+    rng = np.random.default_rng(8 if is_f64 else 9)
+    for trial in range(40):
+        n = int(rng.choice([0, 1, 63, 64, 65, 1000, 4097]))
+        if is_f64:
+            x = rng.standard_normal(n) * 100
+            if trial % 7 == 3 and n:
+                x[rng.integers(0, n, 3)] = np.nan
+                x[0] = np.nan                                         # a NaN first valid value
+        else:
+            x = rng.integers(-2**40, 2**40, n).astype(np.int64)
+        w = _random_validity(rng, n, trial)
+        dt = np.float64 if is_f64 else np.int64
+        r1, r2 = np.zeros(3, dt), np.zeros(3, dt)
+        c1, c2 = np.zeros(3, np.uint64), np.zeros(3, np.uint64)
+        k1, k2 = np.zeros(max(n, 1), dt), np.zeros(max(n, 1), dt)
+        f_o = o.O.vo_nullable_f64 if is_f64 else o.O.vo_nullable_i64
+        f_r = o.R.vr_nullable_f64 if is_f64 else o.R.vr_nullable_i64
+        f_o(o.ptr(x), n, o.ptr(w), o.ptr(r1), o.ptr(c1), o.ptr(k1))
+        xr = x.copy()
+        f_r(o.ptr(xr), n, o.ptr(w), o.ptr(r2), o.ptr(c2), o.ptr(k2))
+        assert np.array_equal(c1, c2), (trial, c1, c2)
+        assert np.array_equal(r1.view(np.uint64), r2.view(np.uint64)), (trial, r1, r2)
+        m = int(c1[2])
+        assert np.array_equal(k1[:m].view(np.uint64), k2[:m].view(np.uint64)), trial
+        # closed form: the valid rows are exactly the set bits below n
+        bits = np.unpackbits(w.view(np.uint8), bitorder="little")[:n].astype(bool)
+        assert int(c1[0]) == int(bits.sum()) and np.array_equal(k1[:m].view(np.uint64), x[bits].view(np.uint64))

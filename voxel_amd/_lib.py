@@ -57,6 +57,12 @@ class LearnedStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class NullableStats(C.Structure):
+    """voxel_gpu_nullable_stats."""
+    _fields_ = [("valid_count", U64), ("null_count", U64), ("sum", F64), ("min", F64), ("max", F64),
+                ("isum", I64), ("imin", I64), ("imax", I64)]
+
+
 class JoinStats(C.Structure):
     _fields_ = [("mode", C.c_int32), ("fell_back", C.c_int32), ("partitions", C.c_uint32), ("pad", C.c_uint32),
                 ("slots_per_partition", U64), ("max_partition_keys", U64), ("min_partition_keys", U64),
@@ -184,6 +190,9 @@ SIGS = {
     "voxel_gpu_join_set_probe_partitioning": (INT, [P, INT]),
     "voxel_gpu_groupby_f64": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
     "voxel_gpu_groupby_f64_general": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
+    "voxel_gpu_scan_valid": (INT, [P, P, P, P, P]),
+    "voxel_gpu_nullable_reduce": (INT, [INT, P, P, SZ, P, P]),
+    "voxel_gpu_nullable_compact": (INT, [INT, P, P, SZ, P, P, P]),
     "voxel_gpu_window_delta_f64": (I64, [P, SZ, P, SZ, P]),
     "voxel_gpu_window_sum_f64": (I64, [P, SZ, P, SZ, INT, P]),
     "voxel_gpu_window_mean_f64": (I64, [P, SZ, P, SZ, INT, P]),

@@ -77,6 +77,11 @@ struct ScanArgs {
 // Fused filter + COUNT/SUM/MIN/MAX over one range; result in *out_dev
 // (device memory, ScanOut).  Returns hipSuccess or the launch error.
 hipError_t launch_scan(const ScanArgs& args, const Scratch& scr, ScanOut* out_dev, hipStream_t stream);
+// The same over nullable columns: rows whose validity bit (va for a, vb for
+// b; NULL = all valid; words indexed from row 0, args.begin a multiple of 64)
+// is 0 are skipped.  Range predicates only (no table form).
+hipError_t launch_scan_valid(const ScanArgs& args, const uint64_t* va, const uint64_t* vb, const Scratch& scr,
+                             ScanOut* out_dev, hipStream_t stream);
 
 // Tier-0 min/max loop (VRED_MIN/MAX + CMP/JLE|JGE/MOVR fold) over groups of
 // 4 eight-byte lanes starting at row off; rows >= count read as 0.

@@ -152,6 +152,146 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const ScanOut* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// Scans over nullable columns.  A row takes part only when its validity bit
+// is 1 (NullBitmap layout, data/nulls.hpp:16-47: bit r & 63 of word r >> 6,
+// 1 = valid; NULL = every row valid), for column a and, when given, column
+// b; the predicates and aggregates then run as in k_scan.  This is the
+// reference's filter bitmap ANDed with the validity words (ColumnFilter over
+// the same word layout, ops/filter.hpp:540-547) followed by the aggregate, in
+// one pass.  The validity words add 1/64 of the column's bytes per row of an
+// 8-byte type; each lane reads the word(s) of its rows, which its wave
+// neighbours share from the same cache line.
+// ---------------------------------------------------------------------------
+__device__ uint64_t g_all_valid = ~0ull;       // read only
+// 16-byte loads in flight per lane and column: one column needs twice the
+// loads of two to keep as many bytes in flight
+template <int MODE> constexpr int kValidUnroll = MODE == 2 ? 4 : 8;
+
+template <typename T, int MODE, bool VEC>
+__global__ __launch_bounds__(kBlock)
+void k_scan_valid(const T* __restrict__ a, const T* __restrict__ b, const uint64_t* __restrict__ va,
+                  const uint64_t* __restrict__ vbm, uint64_t n, PredDev pda, PredDev pdb,
+                  ScanOut* __restrict__ parts) {
+    using V = typename std::conditional<VEC, typename TyInfo<T>::V, T>::type;
+    constexpr int E = VEC ? TyInfo<T>::E : 1;
+    constexpr int U = kValidUnroll<MODE>;
+    Pred<T, false> pa; pa.load(pda);
+    Pred<T, false> pb; pb.load(pdb);
+    Acc<T> acc; acc.init();
+    // An absent bitmap reads one all-valid word (index masked to 0), so the
+    // loop has no branch on it.
+    const uint64_t* wa = va ? va : &g_all_valid;
+    const uint64_t* wb = vbm ? vbm : &g_all_valid;
+    const uint64_t ka = va ? ~0ull : 0ull, kb = vbm ? ~0ull : 0ull;
+    auto one = [&](T x, T y, bool valid) {
+        bool p = pa(x) & valid;
+        if constexpr (MODE == 1) p = p & pb(x);
+        if constexpr (MODE == 2) p = p & pb(y);
+        acc.add(p, x);
+    };
+    auto bit = [](uint64_t w, uint64_t r) -> bool { return (w >> (r & 63)) & 1ull; };
+    uint64_t head = 0;
+    if constexpr (VEC) {
+        head = ((16u - ((uintptr_t)a & 15u)) & 15u) / sizeof(T);
+        if (head > n) head = n;
+    }
+    const uint64_t nvec = (n - head) / E;
+    const uint64_t ts = head + nvec * E;
+    const uint64_t gtid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (gtid < head + (n - ts)) {
+        const uint64_t i = gtid < head ? gtid : ts + (gtid - head);
+        one(a[i], MODE == 2 ? b[i] : T(0), bit(wa[(i >> 6) & ka], i) & (MODE != 2 || bit(wb[(i >> 6) & kb], i)));
+    }
+    const V* __restrict__ av = reinterpret_cast<const V*>(a + head);
+    const V* __restrict__ bv = reinterpret_cast<const V*>(MODE == 2 ? b + head : a + head);
+    const uint64_t tile = (uint64_t)kBlock * U;
+    // The E rows of a lane's vector lie in at most two words (two when a
+    // misaligned column start puts them across a word boundary); both are
+    // loaded with the data, and the wave's lanes share their cache lines.
+    auto step = [&](uint64_t vi, const V& xa, const V& xb, uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1) {
+        const T* pxa = reinterpret_cast<const T*>(&xa);
+        const T* pxb = reinterpret_cast<const T*>(&xb);
+        const uint64_t r0 = head + vi * E;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint64_t r = r0 + e;
+            const bool lo = (r >> 6) == (r0 >> 6);
+            one(pxa[e], MODE == 2 ? pxb[e] : T(0), bit(lo ? a0 : a1, r) & bit(lo ? b0 : b1, r));
+        }
+    };
+    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < nvec; base += (uint64_t)gridDim.x * tile) {
+        V xa[U], xb[U];
+        uint64_t a0[U], a1[U], b0[U], b1[U];
+        if (base + tile <= nvec) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {            // every load of the tile is issued before any is used
+                const uint64_t vi = base + u * kBlock + threadIdx.x;
+                const uint64_t r0 = head + vi * E;
+                xa[u] = ld16<V, true>(&av[vi]);
+                if constexpr (MODE == 2) xb[u] = ld16<V, true>(&bv[vi]);
+                a0[u] = wa[(r0 >> 6) & ka];
+                a1[u] = wa[((r0 + E - 1) >> 6) & ka];
+                if constexpr (MODE == 2) {              // column b's bitmap: only with a predicate on b
+                    b0[u] = wb[(r0 >> 6) & kb];
+                    b1[u] = wb[((r0 + E - 1) >> 6) & kb];
+                } else {
+                    b0[u] = b1[u] = ~0ull;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) step(base + u * kBlock + threadIdx.x, xa[u], xb[u], a0[u], a1[u], b0[u], b1[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t vi = base + u * kBlock + threadIdx.x;
+                if (vi < nvec) {
+                    const uint64_t r0 = head + vi * E;
+                    xa[u] = av[vi];
+                    if constexpr (MODE == 2) xb[u] = bv[vi];
+                    step(vi, xa[u], xb[u], wa[(r0 >> 6) & ka], wa[((r0 + E - 1) >> 6) & ka],
+                         MODE == 2 ? wb[(r0 >> 6) & kb] : ~0ull, MODE == 2 ? wb[((r0 + E - 1) >> 6) & kb] : ~0ull);
+                }
+            }
+        }
+    }
+    __shared__ Acc<T> lds[kBlock / 64];
+    block_reduce(acc, lds);
+    if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
+}
+
+template <typename T, int MODE>
+hipError_t launch_valid_t(const ScanArgs& g, const uint64_t* va, const uint64_t* vb, const Scratch& scr,
+                          ScanOut* out, hipStream_t st) {
+    const T* a = static_cast<const T*>(g.a) + g.begin;
+    const T* b = MODE == 2 ? static_cast<const T*>(g.b) + g.begin : nullptr;
+    const uint64_t n = g.end - g.begin;
+    const bool vec = !(MODE == 2 && (((uintptr_t)a ^ (uintptr_t)b) & 15u));
+    const uint64_t per_block = (uint64_t)kBlock * kValidUnroll<MODE> * (16 / sizeof(T));
+    uint64_t want = (n + per_block - 1) / per_block;
+    uint64_t cap = (uint64_t)device_cu_count() * (MODE == 2 ? 2 : 4);
+    if (cap > scr.max_blocks) cap = scr.max_blocks;
+    uint64_t blocks = want < cap ? want : cap;
+    if (blocks < 1) blocks = 1;
+    ScanOut* parts = static_cast<ScanOut*>(scr.partials);
+    if (vec)
+        hipLaunchKernelGGL((k_scan_valid<T, MODE, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a, b, va, vb, n,
+                           g.pa, g.pb, parts);
+    else
+        hipLaunchKernelGGL((k_scan_valid<T, MODE, false>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a, b, va, vb,
+                           n, g.pa, g.pb, parts);
+    hipLaunchKernelGGL((k_scan_final<T>), dim3(1), dim3(kBlock), 0, st, (const ScanOut*)parts, (uint32_t)blocks, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_valid_type(const ScanArgs& g, const uint64_t* va, const uint64_t* vb, const Scratch& scr,
+                             ScanOut* out, hipStream_t st) {
+    if (g.mode == 0) return launch_valid_t<T, 0>(g, va, vb, scr, out, st);
+    if (g.mode == 1) return launch_valid_t<T, 1>(g, va, vb, scr, out, st);
+    return launch_valid_t<T, 2>(g, va, vb, scr, out, st);
+}
+
+// ---------------------------------------------------------------------------
 // Tier-0 VRED_MIN / VRED_MAX groups (engine.hpp:3071-3095) with the
 // CMP acc,part / JLE|JGE +2 / MOVR fold of the voxel_query min()/max()
 // programs (bindings/voxel_query.py:111-135).  Each group is 4 lanes of an
@@ -385,6 +525,21 @@ int device_cu_count() {
         cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     }
     return cus;
+}
+
+hipError_t launch_scan_valid(const ScanArgs& g, const uint64_t* va, const uint64_t* vb, const Scratch& scr,
+                             ScanOut* out, hipStream_t st) {
+    if (g.end < g.begin || g.table_a || g.table_b) return hipErrorInvalidValue;
+    if (g.begin & 63) return hipErrorInvalidValue;          // validity words start at row 0 of the range
+    const uint64_t* wa = va ? va + (g.begin >> 6) : nullptr;
+    const uint64_t* wb = vb ? vb + (g.begin >> 6) : nullptr;
+    switch (g.dtype) {
+    case DT_F64: return launch_valid_type<double>(g, wa, wb, scr, out, st);
+    case DT_F32: return launch_valid_type<float>(g, wa, wb, scr, out, st);
+    case DT_I64: return launch_valid_type<int64_t>(g, wa, wb, scr, out, st);
+    case DT_I32: return launch_valid_type<int32_t>(g, wa, wb, scr, out, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_scan(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipStream_t st) {

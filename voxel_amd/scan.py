@@ -72,14 +72,67 @@ def _result(dt: int, r: Agg) -> dict:
     return out
 
 
-def scan(a, op_a, t_a, b=None, op_b=TRUE, t_b=0, stream=None, blocks: int = 0) -> dict:
-    """Runs one filter+aggregate and returns {count, sum, min, max}."""
+def _words(valid):
+    """Device pointer of a validity-word tensor (int64/uint64, NullBitmap layout) or None."""
+    if valid is None:
+        return None
+    if not valid.is_cuda or not valid.is_contiguous() or valid.element_size() != 8:
+        raise TypeError("validity words must be a contiguous 64-bit device tensor")
+    return C.c_void_p(valid.data_ptr())
+
+
+def scan(a, op_a, t_a, b=None, op_b=TRUE, t_b=0, stream=None, blocks: int = 0, valid_a=None, valid_b=None) -> dict:
+    """Runs one filter+aggregate and returns {count, sum, min, max}.  With
+    validity words (bit i % 64 of word i // 64, 1 = valid) for a and/or b,
+    rows that are null in either column are skipped (voxel_gpu_scan_valid)."""
     d = make_desc(a, op_a, t_a, b, op_b, t_b, blocks)
     r = Agg()
     st = None if stream is None else C.c_void_p(stream)
-    if lib.voxel_gpu_scan(C.byref(d), st, C.byref(r)) != 0:
+    if valid_a is not None or valid_b is not None:
+        for v in (valid_a, valid_b):
+            if v is not None and v.numel() < (d.n + 63) // 64:
+                raise ValueError("validity words cover fewer rows than the column")
+        if lib.voxel_gpu_scan_valid(C.byref(d), _words(valid_a), _words(valid_b), st, C.byref(r)) != 0:
+            check("voxel_gpu_scan_valid")
+    elif lib.voxel_gpu_scan(C.byref(d), st, C.byref(r)) != 0:
         check("voxel_gpu_scan")
     return _result(d.dtype, r)
+
+
+def nullable_reduce(col, valid=None, stream=None) -> dict:
+    """NullableSegment<T>::SumValid / MinValid / MaxValid / ValidCount /
+    NullCountVal (data/nulls.hpp:241-420) of a device column."""
+    from ._lib import NullableStats
+    p, n, dt = _dev(col)
+    if valid is not None and valid.numel() < (n + 63) // 64:
+        raise ValueError("validity words cover fewer rows than the column")
+    r = NullableStats()
+    st = None if stream is None else C.c_void_p(stream)
+    if lib.voxel_gpu_nullable_reduce(dt, C.c_void_p(p), _words(valid), n, C.byref(r), st) != 0:
+        check("voxel_gpu_nullable_reduce")
+    out = {"valid_count": int(r.valid_count), "null_count": int(r.null_count)}
+    if dt in (F64, F32):
+        out.update(sum=r.sum, min=r.min, max=r.max)
+    else:
+        out.update(sum=int(r.isum), min=int(r.imin), max=int(r.imax))
+    return out
+
+
+def nullable_compact(col, valid, out, stream=None) -> int:
+    """NullableSegment<T>::CompactValid: the valid rows of ``col`` in order
+    into ``out`` (a device tensor of the same dtype, len(col) entries);
+    returns how many."""
+    p, n, dt = _dev(col)
+    po, no, dto = _dev(out)
+    if no < n or dto != dt:
+        raise ValueError("out must hold len(col) entries of the column's dtype")
+    if valid.numel() < (n + 63) // 64:
+        raise ValueError("validity words cover fewer rows than the column")
+    cnt = C.c_uint64(0)
+    st = None if stream is None else C.c_void_p(stream)
+    if lib.voxel_gpu_nullable_compact(dt, C.c_void_p(p), _words(valid), n, C.c_void_p(po), C.byref(cnt), st) != 0:
+        check("voxel_gpu_nullable_compact")
+    return int(cnt.value)
 
 
 def launch(desc: ScanDesc, stream=None) -> None:
