@@ -21,6 +21,7 @@ floor((1 - u(1 - N^(1-s)))^(1/(1-s)))), lognormal keys floor(exp(20 + 2Z)).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-breakdown]
                     [--no-sort] [--no-join] [--no-groupby] [--no-serve] [--no-window] [--no-nulls]
+                    [--no-encodings]
 
 value = % of the 8 TB/s HBM peak achieved by the f64 compound scan, from the
 whole timed region (K launches bracketed by device synchronisation);
@@ -70,6 +71,7 @@ def parse():
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-window", action="store_true")
     p.add_argument("--no-nulls", action="store_true")
+    p.add_argument("--no-encodings", action="store_true")
     p.add_argument("--sort-log2", type=int, default=29)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a device-to-device copy of one 2^rows-log2 f64 column (the known-size "
@@ -348,6 +350,60 @@ def bench_nulls(rows_log2: int = 28) -> dict:
     out["nullable_compact_2^%d" % rows_log2] = {"ms": round(t * 1e3, 4), "kept": m[0],
                                                 "gb_per_s": round((n * 8 + nw * 8 + m[0] * 8) / t / 1e9, 1)}
     del a, b, wa, wb, o_
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_encodings(rows_log2: int = 28) -> dict:
+    """Filter + aggregate scans that read encoded columns (2^28 rows): the
+    dictionary (1000 distinct f64, u32 indices), frame-of-reference (i32
+    codes), 12-bit packed u32, RLE (runs of 16) and delta (i64) forms, beside
+    the plain f64 scan of the same rows."""
+    import ctypes as C
+    import torch
+    from voxel_amd import encoding as enc
+    from voxel_amd import scan as vs
+    from voxel_amd._lib import lib
+    n = 1 << rows_log2
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    out = {}
+    d = torch.sort(torch.randn(1000, dtype=torch.float64, device="cuda", generator=g) * 100).values
+    ix = torch.randint(0, 1000, (n,), dtype=torch.int32, device="cuda", generator=g)
+    plain = d[ix.long()]
+    t = float(d[500].item())
+
+    def entry(name, fn, nbytes, rows=n):
+        ms = _sync_time(fn, reps=5) * 1e3
+        out[name] = {"ms": round(ms, 4), "rows_per_s": round(rows / ms * 1e3, 1),
+                     "encoded_gb_per_s": round(nbytes / ms / 1e6, 1),
+                     "frac_of_peak": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}
+
+    dcol = enc.dictionary(d, ix)
+    entry("plain_f64_scan", lambda: vs.scan(plain, vs.GT, t), n * 8)
+    entry("dictionary_scan", lambda: dcol.scan(vs.GT, t), n * 4 + 8000)
+    del plain
+    torch.cuda.empty_cache()
+    codes = torch.randint(0, 2**31 - 1, (n,), dtype=torch.int32, device="cuda", generator=g)
+    fcol = enc.frame_of_reference(codes, 1000.0, 1e-6)
+    entry("frame_of_reference_scan", lambda: fcol.scan(vs.GT, 2000.0), n * 4)
+    del codes, fcol
+    words = torch.randint(-2**62, 2**62, (n * 12 // 64,), dtype=torch.int64, device="cuda", generator=g)
+    bcol = enc.bitpacked(words, 12, n)
+    entry("bitpacked12_scan", lambda: bcol.scan(vs.LT, 1000), n * 12 // 8)
+    del words, bcol
+    nr = n // 16
+    vals = torch.randn(nr, dtype=torch.float64, device="cuda", generator=g)
+    lens = torch.full((nr,), 16, dtype=torch.int32, device="cuda")
+    rcol = enc.rle(vals, lens)
+    entry("rle16_scan", lambda: rcol.scan(vs.GT, 0.0), nr * 12)
+    del vals, lens, rcol
+    deltas = torch.randint(-1000, 1000, (n,), dtype=torch.int64, device="cuda", generator=g)
+    dl = enc.delta(deltas, 0)
+    entry("delta_i64_decode_and_scan", lambda: dl.scan(vs.GT, 0), n * 8)
+    o_ = torch.empty(n, dtype=torch.float64, device="cuda")
+    entry("dictionary_decode", lambda: dcol.decode(0, n, o_), n * 12)
+    del deltas, dl, o_, dcol, d, ix
     torch.cuda.empty_cache()
     return out
 
@@ -643,6 +699,8 @@ def main():
         out["window"] = bench_window()
     if not args.no_nulls:
         out["nulls"] = bench_nulls(args.rows_log2)
+    if not args.no_encodings:
+        out["encodings"] = bench_encodings(args.rows_log2)
     if not args.no_serve:
         out["serve"] = bench_serve()
     if not args.no_cpu:

@@ -331,6 +331,39 @@ int       voxel_gpu_nullable_reduce(int dtype, const void* data_dev, const uint6
 int       voxel_gpu_nullable_compact(int dtype, const void* data_dev, const uint64_t* valid_dev, size_t n,
                                      void* out_dev, uint64_t* count, void* stream);
 
+/* ---- columnar encodings (data/encoding.hpp:104-527) ---------------------- */
+/* An encoded column on the device, in the reference's member layout:
+ *   Dictionary  values = Dictionary (sorted distinct values), index = Indices (u32 per row), nvalues = dictionary size
+ *   RLE         values = Values, index = RunLengths (u32, none 0), nvalues = runs; rows = sum of the run lengths
+ *   Delta       values = Deltas (i64 / i32), delta_base = BaseValue; rows = Deltas.size()
+ *   BitPacked   values = PackedData (u64 words), nvalues = words, bit_width = BitWidth; values are u32
+ *   FrameOfReference  values = EncodedValues (i32), base = BaseValue, scale = Scale; values are f64
+ * kind takes EncodingKind's values. */
+enum { VOXEL_ENC_DICTIONARY = 1, VOXEL_ENC_RLE = 2, VOXEL_ENC_DELTA = 3, VOXEL_ENC_BITPACKED = 4, VOXEL_ENC_FOR = 6 };
+typedef struct voxel_gpu_encoded_col {
+    int32_t  kind;
+    int32_t  dtype;            /* value type: VOXEL_GPU_F64/F32/I64/I32 (Dictionary, RLE), I64/I32 (Delta);
+                                  ignored for BitPacked (u32) and FrameOfReference (f64) */
+    uint64_t rows;
+    const void* values;
+    const uint32_t* index;
+    uint64_t nvalues;
+    uint32_t bit_width;
+    uint32_t pad;
+    double   base, scale;
+    int64_t  delta_base;
+} voxel_gpu_encoded_col;
+/* IEncoding::DecodeBatch(out, offset, count) into device memory: returns the
+ * rows written (RLE writes count rows, 0 past the last run, as the reference
+ * does; the others stop at the last row), -1 on error.  Waits for the stream. */
+int64_t   voxel_gpu_decode(const voxel_gpu_encoded_col* col, uint64_t offset, uint64_t count, void* out_dev,
+                           void* stream);
+/* Filter op(value, t) + COUNT/SUM/MIN/MAX over the decoded rows, reading the
+ * encoded bytes (Delta: decoded first).  t holds the value type's bits
+ * (BitPacked: a u32; results as VOXEL_GPU_I64). */
+int       voxel_gpu_scan_encoded(const voxel_gpu_encoded_col* col, int op, uint64_t t, void* stream,
+                                 voxel_gpu_agg* out);
+
 /* ---- window family (Engine<f64>::WindowDelta .. WindowQuantile and
  * TimeWindowSum, exec/engine.hpp:183-300; voxel_py EngineF64.window_*,
  * bindings/voxel_bindings.cpp:127-180) ------------------------------------- */

@@ -67,6 +67,22 @@ _SIGS = {
     "vo_tier0_expr_col_i64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P]),
     "vo_window_f64": (I64, [INT, P, SZ, P, SZ, INT, F64]),
     "vo_nullable_f64": (None, [P, SZ, P, P, P, P]),
+    "vo_dict_build_f64": (SZ, [P, SZ, P, P]),
+    "vo_dict_build_i64": (SZ, [P, SZ, P, P]),
+    "vo_dict_decode_f64": (None, [P, P, SZ, SZ, SZ, P]),
+    "vo_dict_decode_i64": (None, [P, P, SZ, SZ, SZ, P]),
+    "vo_rle_build_f64": (SZ, [P, SZ, P, P]),
+    "vo_rle_build_i64": (SZ, [P, SZ, P, P]),
+    "vo_rle_decode_f64": (None, [P, P, SZ, SZ, SZ, P]),
+    "vo_rle_decode_i64": (None, [P, P, SZ, SZ, SZ, P]),
+    "vo_delta_build_i64": (None, [P, SZ, P]),
+    "vo_delta_build_i32": (None, [P, SZ, P]),
+    "vo_delta_decode_i64": (None, [I64, P, SZ, SZ, SZ, P]),
+    "vo_delta_decode_i32": (None, [I32, P, SZ, SZ, SZ, P]),
+    "vo_bitpack_build": (SZ, [P, SZ, P, P]),
+    "vo_bitpack_decode": (None, [P, SZ, U32, SZ, SZ, SZ, P]),
+    "vo_for_build": (None, [P, SZ, P, P, P]),
+    "vo_for_decode": (None, [F64, F64, P, SZ, SZ, SZ, P]),
     "vo_nullable_i64": (None, [P, SZ, P, P, P, P]),
     "vo_time_window_sum_f64": (I64, [P, P, SZ, I64, I64, P, SZ]),
     "vo_agg_create": (P, [INT]),
@@ -125,6 +141,22 @@ _RSIGS = {
     "vr_encode": (U32, [U8, U8, U8, U8, C.c_uint16]),
     "vr_window_f64": (I64, [INT, P, SZ, P, SZ, INT, F64]),
     "vr_nullable_f64": (None, [P, SZ, P, P, P, P]),
+    "vr_dict_build_f64": (SZ, [P, SZ, P, P]),
+    "vr_dict_build_i64": (SZ, [P, SZ, P, P]),
+    "vr_dict_decode_f64": (None, [P, SZ, P, SZ, SZ, SZ, P]),
+    "vr_dict_decode_i64": (None, [P, SZ, P, SZ, SZ, SZ, P]),
+    "vr_rle_build_f64": (SZ, [P, SZ, P, P]),
+    "vr_rle_build_i64": (SZ, [P, SZ, P, P]),
+    "vr_rle_decode_f64": (None, [P, P, SZ, SZ, SZ, P]),
+    "vr_rle_decode_i64": (None, [P, P, SZ, SZ, SZ, P]),
+    "vr_delta_build_i64": (None, [P, SZ, P, P]),
+    "vr_delta_build_i32": (None, [P, SZ, P, P]),
+    "vr_delta_decode_i64": (None, [I64, P, SZ, SZ, SZ, P]),
+    "vr_delta_decode_i32": (None, [I32, P, SZ, SZ, SZ, P]),
+    "vr_bitpack_build": (SZ, [P, SZ, P, P]),
+    "vr_bitpack_decode": (None, [P, SZ, U32, SZ, SZ, SZ, P]),
+    "vr_for_build": (None, [P, SZ, P, P, P]),
+    "vr_for_decode": (None, [F64, F64, P, SZ, SZ, SZ, P]),
     "vr_nullable_i64": (None, [P, SZ, P, P, P, P]),
     "vr_time_window_sum_f64": (I64, [P, P, SZ, I64, I64, P, SZ]),
 }

@@ -7,6 +7,7 @@
 // Used to pin the oracle restatement (tests/) and as the timed CPU baseline
 // (bench.py cpu_baseline, kind "reference").  The product never loads it.
 #include "voxel/voxel.hpp"
+#include <algorithm>
 #include <bit>
 #include <cstring>
 #include <random>
@@ -42,6 +43,52 @@ void nullable_run(T* data, size_t n, const uint64_t* words, T* res, uint64_t* co
     s.CompactValid(arena);
     counts[2] = s.Data.Count;
     if (s.Data.Count) std::memcpy(compact, s.Data.Data, s.Data.Count * sizeof(T));
+}
+// Encodings (data/encoding.hpp:104-527): Build from raw rows, and
+// DecodeBatch over member arrays set by the caller.
+template <typename T>
+size_t dict_build(const T* x, size_t n, T* dict, uint32_t* idx) {
+    encoding::DictionaryEncoding<T> e;
+    e.Build(x, n);
+    std::copy(e.Dictionary.begin(), e.Dictionary.end(), dict);
+    std::copy(e.Indices.begin(), e.Indices.end(), idx);
+    return e.Dictionary.size();
+}
+template <typename T>
+void dict_decode(const T* dict, size_t nd, const uint32_t* idx, size_t rows, size_t off, size_t cnt, T* out) {
+    encoding::DictionaryEncoding<T> e;
+    e.Dictionary.assign(dict, dict + nd);
+    e.Indices.assign(idx, idx + rows);
+    e.DecodeBatch(out, off, cnt);
+}
+template <typename T>
+size_t rle_build(const T* x, size_t n, T* vals, uint32_t* lens) {
+    encoding::RLEEncoding<T> e;
+    e.Build(x, n);
+    std::copy(e.Values.begin(), e.Values.end(), vals);
+    std::copy(e.RunLengths.begin(), e.RunLengths.end(), lens);
+    return e.Values.size();
+}
+template <typename T>
+void rle_decode(const T* vals, const uint32_t* lens, size_t nr, size_t off, size_t cnt, T* out) {
+    encoding::RLEEncoding<T> e;
+    e.Values.assign(vals, vals + nr);
+    e.RunLengths.assign(lens, lens + nr);
+    e.DecodeBatch(out, off, cnt);
+}
+template <typename T>
+void delta_build(const T* x, size_t n, T* deltas, T* base) {
+    encoding::DeltaEncoding<T> e;
+    e.Build(x, n);
+    std::copy(e.Deltas.begin(), e.Deltas.end(), deltas);
+    *base = e.BaseValue;
+}
+template <typename T>
+void delta_decode(T base, const T* deltas, size_t rows, size_t off, size_t cnt, T* out) {
+    encoding::DeltaEncoding<T> e;
+    e.BaseValue = base;
+    e.Deltas.assign(deltas, deltas + rows);
+    e.DecodeBatch(out, off, cnt);
 }
 }  // namespace
 
@@ -218,6 +265,59 @@ void vr_nullable_f64(double* d, size_t n, const uint64_t* w, double* res, uint64
 }
 void vr_nullable_i64(int64_t* d, size_t n, const uint64_t* w, int64_t* res, uint64_t* counts, int64_t* compact) {
     nullable_run<i64>(d, n, w, res, counts, compact);
+}
+
+size_t vr_dict_build_f64(const double* x, size_t n, double* d, uint32_t* i) { return dict_build<f64>(x, n, d, i); }
+size_t vr_dict_build_i64(const int64_t* x, size_t n, int64_t* d, uint32_t* i) { return dict_build<i64>(x, n, d, i); }
+void vr_dict_decode_f64(const double* d, size_t nd, const uint32_t* i, size_t rows, size_t off, size_t cnt, double* o) {
+    dict_decode<f64>(d, nd, i, rows, off, cnt, o);
+}
+void vr_dict_decode_i64(const int64_t* d, size_t nd, const uint32_t* i, size_t rows, size_t off, size_t cnt, int64_t* o) {
+    dict_decode<i64>(d, nd, i, rows, off, cnt, o);
+}
+size_t vr_rle_build_f64(const double* x, size_t n, double* v, uint32_t* l) { return rle_build<f64>(x, n, v, l); }
+size_t vr_rle_build_i64(const int64_t* x, size_t n, int64_t* v, uint32_t* l) { return rle_build<i64>(x, n, v, l); }
+void vr_rle_decode_f64(const double* v, const uint32_t* l, size_t nr, size_t off, size_t cnt, double* o) {
+    rle_decode<f64>(v, l, nr, off, cnt, o);
+}
+void vr_rle_decode_i64(const int64_t* v, const uint32_t* l, size_t nr, size_t off, size_t cnt, int64_t* o) {
+    rle_decode<i64>(v, l, nr, off, cnt, o);
+}
+void vr_delta_build_i64(const int64_t* x, size_t n, int64_t* d, int64_t* b) { delta_build<i64>(x, n, d, b); }
+void vr_delta_build_i32(const int32_t* x, size_t n, int32_t* d, int32_t* b) { delta_build<i32>(x, n, d, b); }
+void vr_delta_decode_i64(int64_t b, const int64_t* d, size_t rows, size_t off, size_t cnt, int64_t* o) {
+    delta_decode<i64>(b, d, rows, off, cnt, o);
+}
+void vr_delta_decode_i32(int32_t b, const int32_t* d, size_t rows, size_t off, size_t cnt, int32_t* o) {
+    delta_decode<i32>(b, d, rows, off, cnt, o);
+}
+size_t vr_bitpack_build(const uint32_t* x, size_t n, uint64_t* words, uint32_t* bw) {
+    encoding::BitPackedEncoding e;
+    e.Build(x, n);
+    std::copy(e.PackedData.begin(), e.PackedData.end(), words);
+    *bw = e.BitWidth;
+    return e.PackedData.size();
+}
+void vr_bitpack_decode(const uint64_t* words, size_t nw, uint32_t bw, size_t rows, size_t off, size_t cnt, uint32_t* o) {
+    encoding::BitPackedEncoding e;
+    e.PackedData.assign(words, words + nw);
+    e.BitWidth = bw;
+    e.ValueCount = rows;
+    e.DecodeBatch(o, off, cnt);
+}
+void vr_for_build(const double* x, size_t n, double* base, double* scale, int32_t* enc) {
+    encoding::FrameOfReferenceEncoding e;
+    e.Build(x, n);
+    *base = e.BaseValue;
+    *scale = e.Scale;
+    std::copy(e.EncodedValues.begin(), e.EncodedValues.end(), enc);
+}
+void vr_for_decode(double base, double scale, const int32_t* enc, size_t rows, size_t off, size_t cnt, double* o) {
+    encoding::FrameOfReferenceEncoding e;
+    e.BaseValue = base;
+    e.Scale = scale;
+    e.EncodedValues.assign(enc, enc + rows);
+    e.DecodeBatch(o, off, cnt);
 }
 
 // Window family, static methods of Engine<f64> (exec/engine.hpp:183-300).

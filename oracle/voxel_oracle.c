@@ -1086,3 +1086,138 @@ void NAME(const T* d, size_t n, const uint64_t* w, T* res, uint64_t* counts, T* 
 }
 VO_DEF_NULLABLE(vo_nullable_f64, double, (double)NAN)
 VO_DEF_NULLABLE(vo_nullable_i64, int64_t, INT64_MIN)
+
+/* ------------------------------------------------------------------------ */
+/* Columnar encodings (data/encoding.hpp:104-527): Build and DecodeBatch.   */
+/* Dictionary: the distinct values (== equality, so -0 and +0 are one entry */
+/* holding the first one seen; NaN-free input) sorted ascending, one u32     */
+/* index per row.  RLE: runs of == equal neighbours, u32 run lengths;        */
+/* DecodeBatch writes T{} for rows past the last run and counts rows in a    */
+/* u32 as the reference does.  Delta (integers): deltas from the previous    */
+/* row (the first from 0), wrapping; decode = BaseValue + running sum.       */
+/* BitPacked: u32 values in BitWidth-bit fields (width of the maximum, at    */
+/* least 1), little-endian across u64 words.  FrameOfReference: Base = min,  */
+/* Scale = (max - min) / INT32_MAX (1 when max <= min), i32 codes =          */
+/* llround((x - Base) / Scale); decode = Base + (f64)code * Scale.           */
+/* ------------------------------------------------------------------------ */
+typedef struct { double v; size_t row; } vo_vr_f64;
+typedef struct { int64_t v; size_t row; } vo_vr_i64;
+static int vo_cmp_vr_f64(const void* a, const void* b) {
+    const vo_vr_f64 *x = (const vo_vr_f64*)a, *y = (const vo_vr_f64*)b;
+    if (x->v < y->v) return -1;
+    if (x->v > y->v) return 1;
+    return x->row < y->row ? -1 : x->row > y->row;
+}
+static int vo_cmp_vr_i64(const void* a, const void* b) {
+    const vo_vr_i64 *x = (const vo_vr_i64*)a, *y = (const vo_vr_i64*)b;
+    if (x->v < y->v) return -1;
+    if (x->v > y->v) return 1;
+    return x->row < y->row ? -1 : x->row > y->row;
+}
+#define VO_DEF_DICT(SUF, T, VR, CMP)                                                         \
+size_t vo_dict_build_##SUF(const T* x, size_t n, T* dict, uint32_t* idx) {                 \
+    VR* p = (VR*)malloc((n ? n : 1) * sizeof(VR));                                           \
+    size_t i, g = 0;                                                                         \
+    for (i = 0; i < n; ++i) { p[i].v = x[i]; p[i].row = i; }                                 \
+    qsort(p, n, sizeof(VR), CMP);                                                            \
+    for (i = 0; i < n; ++i) {                                                                \
+        if (i > 0 && p[i].v == p[i - 1].v) { idx[p[i].row] = (uint32_t)(g - 1); continue; } \
+        dict[g] = p[i].v;                          /* smallest row of an equal group */     \
+        idx[p[i].row] = (uint32_t)g++;                                                       \
+    }                                                                                        \
+    free(p);                                                                                 \
+    return g;                                                                                \
+}                                                                                            \
+void vo_dict_decode_##SUF(const T* dict, const uint32_t* idx, size_t rows, size_t off,      \
+                          size_t cnt, T* out) {                                              \
+    size_t i;                                                                                \
+    for (i = 0; i < cnt && off + i < rows; ++i) out[i] = dict[idx[off + i]];                 \
+}                                                                                            \
+size_t vo_rle_build_##SUF(const T* x, size_t n, T* vals, uint32_t* lens) {                 \
+    size_t i, r = 0;                                                                         \
+    if (n == 0) return 0;                                                                    \
+    vals[0] = x[0]; lens[0] = 1;                                                             \
+    for (i = 1; i < n; ++i) {                                                                \
+        if (x[i] == vals[r]) { lens[r]++; continue; }                                        \
+        ++r; vals[r] = x[i]; lens[r] = 1;                                                    \
+    }                                                                                        \
+    return r + 1;                                                                            \
+}                                                                                            \
+void vo_rle_decode_##SUF(const T* vals, const uint32_t* lens, size_t nr, size_t off,        \
+                         size_t cnt, T* out) {                                               \
+    uint32_t cum = 0;                                                                        \
+    size_t run = 0, row = off, i;                                                            \
+    while (run < nr) { cum += lens[run]; if (off < (size_t)cum) break; ++run; }              \
+    for (i = 0; i < cnt; ++i) {                                                              \
+        if (run >= nr) { out[i] = (T)0; continue; }                                          \
+        out[i] = vals[run];                                                                  \
+        ++row;                                                                               \
+        if (row >= (size_t)cum) { ++run; if (run < nr) cum += lens[run]; }                   \
+    }                                                                                        \
+}
+VO_DEF_DICT(f64, double, vo_vr_f64, vo_cmp_vr_f64)
+VO_DEF_DICT(i64, int64_t, vo_vr_i64, vo_cmp_vr_i64)
+
+#define VO_DEF_DELTA(SUF, T, U)                                                              \
+void vo_delta_build_##SUF(const T* x, size_t n, T* deltas) {                                \
+    U prev = 0;                                                                              \
+    size_t i;                                                                                \
+    for (i = 0; i < n; ++i) { deltas[i] = (T)((U)x[i] - prev); prev = (U)x[i]; }             \
+}                                                                                            \
+void vo_delta_decode_##SUF(T base, const T* deltas, size_t rows, size_t off, size_t cnt,     \
+                           T* out) {                                                         \
+    U run = (U)base;                                                                         \
+    size_t i;                                                                                \
+    for (i = 0; i < off && i < rows; ++i) run += (U)deltas[i];                               \
+    for (i = 0; i < cnt && off + i < rows; ++i) { run += (U)deltas[off + i]; out[i] = (T)run; } \
+}
+VO_DEF_DELTA(i64, int64_t, uint64_t)
+VO_DEF_DELTA(i32, int32_t, uint32_t)
+
+size_t vo_bitpack_build(const uint32_t* x, size_t n, uint64_t* words, uint32_t* bw_out) {
+    uint32_t mx = 0, bw = 0;
+    size_t i, nw;
+    if (n == 0) { *bw_out = 0; return 0; }
+    for (i = 0; i < n; ++i) if (x[i] > mx) mx = x[i];
+    while (mx) { ++bw; mx >>= 1; }
+    if (bw == 0) bw = 1;
+    nw = (n * bw + 63) / 64;
+    memset(words, 0, nw * sizeof(uint64_t));
+    for (i = 0; i < n; ++i) {
+        uint64_t pos = (uint64_t)i * bw, v = (uint64_t)x[i] & ((1ull << bw) - 1);
+        size_t w = pos / 64;
+        unsigned sh = pos % 64;
+        words[w] |= v << sh;
+        if (sh + bw > 64 && w + 1 < nw) words[w + 1] |= v >> (64 - sh);
+    }
+    *bw_out = bw;
+    return nw;
+}
+void vo_bitpack_decode(const uint64_t* words, size_t nw, uint32_t bw, size_t rows, size_t off, size_t cnt,
+                       uint32_t* out) {
+    size_t i;
+    for (i = 0; i < cnt && off + i < rows; ++i) {
+        uint64_t pos = (uint64_t)(off + i) * bw, r;
+        unsigned sh = pos % 64;
+        r = words[pos / 64] >> sh;
+        if (sh + bw > 64 && pos / 64 + 1 < nw) r |= words[pos / 64 + 1] << (64 - sh);
+        out[i] = (uint32_t)(r & ((1ull << bw) - 1));
+    }
+}
+
+void vo_for_build(const double* x, size_t n, double* base, double* scale, int32_t* enc) {
+    double mn, mx, range;
+    size_t i;
+    *base = 0.0; *scale = 1.0;
+    if (n == 0) return;
+    mn = mx = x[0];
+    for (i = 1; i < n; ++i) { if (x[i] < mn) mn = x[i]; if (x[i] > mx) mx = x[i]; }
+    *base = mn;
+    range = mx - mn;
+    *scale = range <= 0.0 ? 1.0 : range / (double)INT32_MAX;
+    for (i = 0; i < n; ++i) enc[i] = (int32_t)llround((x[i] - *base) / *scale);
+}
+void vo_for_decode(double base, double scale, const int32_t* enc, size_t rows, size_t off, size_t cnt, double* out) {
+    size_t i;
+    for (i = 0; i < cnt && off + i < rows; ++i) out[i] = base + (double)enc[off + i] * scale;
+}

@@ -641,3 +641,116 @@ def test_nullable_restatement_against_reference(is_f64):
         # closed form: the valid rows are exactly the set bits below n
         bits = np.unpackbits(w.view(np.uint8), bitorder="little")[:n].astype(bool)
         assert int(c1[0]) == int(bits.sum()) and np.array_equal(k1[:m].view(np.uint64), x[bits].view(np.uint64))
+
+
+def _enc_input(rng, kind, n, trial):
+    if kind == "f64":
+        x = np.round(rng.standard_normal(n) * 10, int(rng.integers(0, 3)))
+        if trial % 3 == 0:
+            x = np.repeat(x[: (n + 7) // 8], 8)[:n].copy()                 # runs
+        if n > 3 and trial % 4 == 1:
+            x[1] = -0.0
+            x[2] = 0.0
+        return x
+    x = rng.integers(-50, 50, n).astype(np.int64) * (1 if trial % 2 else 2**40)
+    if trial % 3 == 0:
+        x = np.repeat(x[: (n + 7) // 8], 8)[:n].copy()
+    return x
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("kind", ["f64", "i64"])
+def test_dictionary_and_rle_restatement_against_reference(kind):
+    """vo_dict_* / vo_rle_* against DictionaryEncoding<T> / RLEEncoding<T>:
+    built arrays and DecodeBatch outputs (any offset and count, including
+    past the end) bit-identical; decode(build(x)) == x."""
+    rng = np.random.default_rng(12 if kind == "f64" else 13)
+    dt = np.float64 if kind == "f64" else np.int64
+    for trial in range(30):
+        n = int(rng.choice([0, 1, 7, 100, 3000]))
+        x = _enc_input(rng, kind, n, trial)
+        m = max(n, 1)
+        d1, d2, i1, i2 = np.zeros(m, dt), np.zeros(m, dt), np.zeros(m, np.uint32), np.zeros(m, np.uint32)
+        g1 = getattr(o.O, "vo_dict_build_" + kind)(o.ptr(x), n, o.ptr(d1), o.ptr(i1))
+        g2 = getattr(o.R, "vr_dict_build_" + kind)(o.ptr(x), n, o.ptr(d2), o.ptr(i2))
+        assert g1 == g2 and np.array_equal(d1[:g1].view(np.uint64), d2[:g2].view(np.uint64)), (trial, n, d1[:g1], d2[:g2], x)
+        assert np.array_equal(i1[:n], i2[:n])
+        v1, v2, l1, l2 = np.zeros(m, dt), np.zeros(m, dt), np.zeros(m, np.uint32), np.zeros(m, np.uint32)
+        r1 = getattr(o.O, "vo_rle_build_" + kind)(o.ptr(x), n, o.ptr(v1), o.ptr(l1))
+        r2 = getattr(o.R, "vr_rle_build_" + kind)(o.ptr(x), n, o.ptr(v2), o.ptr(l2))
+        assert r1 == r2 and np.array_equal(v1[:r1].view(np.uint64), v2[:r2].view(np.uint64))
+        assert np.array_equal(l1[:r1], l2[:r2])
+        for off, cnt in ((0, n), (n // 3, n // 2 + 5), (max(n - 2, 0), 9), (n + 4, 3)):
+            o1, o2 = np.full(cnt, 7, dt), np.full(cnt, 7, dt)
+            getattr(o.O, "vo_dict_decode_" + kind)(o.ptr(d1), o.ptr(i1), n, off, cnt, o.ptr(o1))
+            getattr(o.R, "vr_dict_decode_" + kind)(o.ptr(d2), g2, o.ptr(i2), n, off, cnt, o.ptr(o2))
+            assert np.array_equal(o1.view(np.uint64), o2.view(np.uint64)), (trial, off, cnt)
+            o1, o2 = np.full(cnt, 7, dt), np.full(cnt, 7, dt)
+            getattr(o.O, "vo_rle_decode_" + kind)(o.ptr(v1), o.ptr(l1), r1, off, cnt, o.ptr(o1))
+            getattr(o.R, "vr_rle_decode_" + kind)(o.ptr(v2), o.ptr(l2), r2, off, cnt, o.ptr(o2))
+            assert np.array_equal(o1.view(np.uint64), o2.view(np.uint64)), (trial, off, cnt)
+        full = np.zeros(m, dt)
+        getattr(o.O, "vo_dict_decode_" + kind)(o.ptr(d1), o.ptr(i1), n, 0, n, o.ptr(full))
+        assert np.array_equal(full[:n], x)                       # value equality: -0 may decode as +0
+        getattr(o.O, "vo_rle_decode_" + kind)(o.ptr(v1), o.ptr(l1), r1, 0, n, o.ptr(full))
+        assert np.array_equal(full[:n], x)
+        if kind == "f64" and g1:
+            assert np.all(np.diff(d1[:g1]) > 0)                  # sorted, distinct
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_delta_bitpacked_for_restatement_against_reference():
+    """vo_delta_* / vo_bitpack_* / vo_for_* against DeltaEncoding<T>,
+    BitPackedEncoding, FrameOfReferenceEncoding: built arrays and DecodeBatch
+    outputs bit-identical; delta and bit packing lossless; FOR within half a
+    scale step."""
+    rng = np.random.default_rng(14)
+    for trial in range(30):
+        n = int(rng.choice([0, 1, 33, 1000, 5000]))
+        m = max(n, 1)
+        for suf, dt, lo, hi in (("i64", np.int64, -2**62, 2**62), ("i32", np.int32, -2**31, 2**31 - 1)):
+            x = rng.integers(lo, hi, n).astype(dt)
+            a1, a2, b2 = np.zeros(m, dt), np.zeros(m, dt), np.zeros(1, dt)
+            getattr(o.O, "vo_delta_build_" + suf)(o.ptr(x), n, o.ptr(a1))
+            getattr(o.R, "vr_delta_build_" + suf)(o.ptr(x), n, o.ptr(a2), o.ptr(b2))
+            assert b2[0] == 0 and np.array_equal(a1[:n], a2[:n])
+            base = int(rng.integers(-1000, 1000))
+            for off, cnt in ((0, n), (n // 2, n), (n + 3, 2)):
+                o1, o2 = np.full(m + cnt, 5, dt), np.full(m + cnt, 5, dt)
+                getattr(o.O, "vo_delta_decode_" + suf)(base, o.ptr(a1), n, off, cnt, o.ptr(o1))
+                getattr(o.R, "vr_delta_decode_" + suf)(base, o.ptr(a2), n, off, cnt, o.ptr(o2))
+                assert np.array_equal(o1, o2), (suf, trial, off)
+            full = np.zeros(m, dt)
+            getattr(o.O, "vo_delta_decode_" + suf)(0, o.ptr(a1), n, 0, n, o.ptr(full))
+            assert np.array_equal(full[:n], x)
+        u = rng.integers(0, 2 ** int(rng.integers(0, 33)), n, dtype=np.uint64).astype(np.uint32)
+        w1, w2 = np.zeros(m, np.uint64), np.zeros(m, np.uint64)
+        bw1, bw2 = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+        k1 = o.O.vo_bitpack_build(o.ptr(u), n, o.ptr(w1), o.ptr(bw1))
+        k2 = o.R.vr_bitpack_build(o.ptr(u), n, o.ptr(w2), o.ptr(bw2))
+        assert k1 == k2 and bw1[0] == bw2[0] and np.array_equal(w1[:k1], w2[:k2])
+        for off, cnt in ((0, n), (n // 3, n), (n + 1, 4)):
+            o1, o2 = np.full(m + cnt, 9, np.uint32), np.full(m + cnt, 9, np.uint32)
+            o.O.vo_bitpack_decode(o.ptr(w1), k1, int(bw1[0]), n, off, cnt, o.ptr(o1))
+            o.R.vr_bitpack_decode(o.ptr(w2), k2, int(bw2[0]), n, off, cnt, o.ptr(o2))
+            assert np.array_equal(o1, o2)
+        full = np.zeros(m, np.uint32)
+        o.O.vo_bitpack_decode(o.ptr(w1), k1, int(bw1[0]), n, 0, n, o.ptr(full))
+        assert np.array_equal(full[:n], u)
+        f = rng.standard_normal(n) * 1000 + float(rng.choice([0.0, 5e6]))
+        if trial % 5 == 0 and n:
+            f[:] = 3.25                                         # range 0: Scale 1
+        e1, e2 = np.zeros(m, np.int32), np.zeros(m, np.int32)
+        s1, s2 = np.zeros(2), np.zeros(2)
+        o.O.vo_for_build(o.ptr(f), n, o.ptr(s1[0:]), o.ptr(s1[1:]), o.ptr(e1))
+        o.R.vr_for_build(o.ptr(f), n, o.ptr(s2[0:]), o.ptr(s2[1:]), o.ptr(e2))
+        assert np.array_equal(s1.view(np.uint64), s2.view(np.uint64)) and np.array_equal(e1[:n], e2[:n])
+        for off, cnt in ((0, n), (n // 4, n), (n + 2, 3)):
+            o1, o2 = np.full(m + cnt, 1.5), np.full(m + cnt, 1.5)
+            o.O.vo_for_decode(s1[0], s1[1], o.ptr(e1), n, off, cnt, o.ptr(o1))
+            o.R.vr_for_decode(s2[0], s2[1], o.ptr(e2), n, off, cnt, o.ptr(o2))
+            assert np.array_equal(o1.view(np.uint64), o2.view(np.uint64))
+        if n:
+            dec = np.zeros(n)
+            o.O.vo_for_decode(s1[0], s1[1], o.ptr(e1), n, 0, n, o.ptr(dec))
+            assert np.all(np.abs(dec - f) <= s1[1] * 0.5 + 1e-9 * np.abs(f).max())

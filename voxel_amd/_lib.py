@@ -57,6 +57,13 @@ class LearnedStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class EncodedCol(C.Structure):
+    """voxel_gpu_encoded_col."""
+    _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("rows", U64), ("values", P), ("index", P),
+                ("nvalues", U64), ("bit_width", U32), ("pad", U32), ("base", F64), ("scale", F64),
+                ("delta_base", I64)]
+
+
 class NullableStats(C.Structure):
     """voxel_gpu_nullable_stats."""
     _fields_ = [("valid_count", U64), ("null_count", U64), ("sum", F64), ("min", F64), ("max", F64),
@@ -190,6 +197,8 @@ SIGS = {
     "voxel_gpu_join_set_probe_partitioning": (INT, [P, INT]),
     "voxel_gpu_groupby_f64": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
     "voxel_gpu_groupby_f64_general": (INT, [INT, P, P, SZ, SZ, P, P, P, P, P, P, P, P]),
+    "voxel_gpu_decode": (I64, [P, U64, U64, P, P]),
+    "voxel_gpu_scan_encoded": (INT, [P, INT, U64, P, P]),
     "voxel_gpu_scan_valid": (INT, [P, P, P, P, P]),
     "voxel_gpu_nullable_reduce": (INT, [INT, P, P, SZ, P, P]),
     "voxel_gpu_nullable_compact": (INT, [INT, P, P, SZ, P, P, P]),

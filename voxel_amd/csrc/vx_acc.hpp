@@ -109,5 +109,20 @@ __device__ __forceinline__ V ld16(const V* p) {
     }
 }
 
+// One workgroup: per-workgroup partials of a scan, merged in workgroup order
+// (the same order as the last-workgroup combine of vx_scan.hip).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_final(const ScanOut* __restrict__ parts, uint32_t nblocks,
+                                                       ScanOut* __restrict__ out) {
+    __shared__ Acc<T> lds[kBlock / 64];
+    Acc<T> r; r.init();
+    for (uint32_t i = threadIdx.x; i < nblocks; i += kBlock) {
+        Acc<T> o; o.load(&parts[i]);
+        r.merge(o);
+    }
+    block_reduce(r, lds);
+    if (threadIdx.x == 0) r.store(out);
+}
+
 }  // namespace
 }  // namespace vx

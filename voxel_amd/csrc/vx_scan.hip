@@ -136,21 +136,6 @@ void k_scan(const T* __restrict__ a, const T* __restrict__ b, uint64_t begin, ui
     }
 }
 
-// One workgroup: the partials of a k_scan<..., FIN = false> launch, merged in
-// the same order as the last-workgroup combine.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scan_final(const ScanOut* __restrict__ parts, uint32_t nblocks,
-                                                       ScanOut* __restrict__ out) {
-    __shared__ Acc<T> lds[kBlock / 64];
-    Acc<T> r; r.init();
-    for (uint32_t i = threadIdx.x; i < nblocks; i += kBlock) {
-        Acc<T> o; o.load(&parts[i]);
-        r.merge(o);
-    }
-    block_reduce(r, lds);
-    if (threadIdx.x == 0) r.store(out);
-}
-
 // ---------------------------------------------------------------------------
 // Scans over nullable columns.  A row takes part only when its validity bit
 // is 1 (NullBitmap layout, data/nulls.hpp:16-47: bit r & 63 of word r >> 6,
