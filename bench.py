@@ -21,7 +21,7 @@ floor((1 - u(1 - N^(1-s)))^(1/(1-s)))), lognormal keys floor(exp(20 + 2Z)).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-breakdown]
                     [--no-sort] [--no-join] [--no-groupby] [--no-serve] [--no-window] [--no-nulls]
-                    [--no-encodings]
+                    [--no-encodings] [--no-sql]
 
 value = % of the 8 TB/s HBM peak achieved by the f64 compound scan, from the
 whole timed region (K launches bracketed by device synchronisation);
@@ -72,6 +72,7 @@ def parse():
     p.add_argument("--no-window", action="store_true")
     p.add_argument("--no-nulls", action="store_true")
     p.add_argument("--no-encodings", action="store_true")
+    p.add_argument("--no-sql", action="store_true")
     p.add_argument("--sort-log2", type=int, default=29)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a device-to-device copy of one 2^rows-log2 f64 column (the known-size "
@@ -408,6 +409,34 @@ def bench_encodings(rows_log2: int = 28) -> dict:
     return out
 
 
+def bench_sql(rows_log2: int = 28) -> dict:
+    """SQL over HBM-resident columns (2 x 2^28 f64): a compound WHERE on two
+    columns lowered to one fused filter + aggregate pass, and a three-column
+    condition through bitmaps + the validity-word scan; wall time per
+    statement including parsing and planning."""
+    import torch
+    from voxel_amd import voxel_sql as vsql
+    from voxel_amd._lib import lib
+    n = 1 << rows_log2
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    b = torch.empty(n, dtype=torch.float64, device="cuda")
+    c = torch.empty(n, dtype=torch.float64, device="cuda")
+    for i, col in enumerate((a, b, c)):
+        assert lib.voxel_gpu_fill_uniform(0, col.data_ptr(), n, i + 1, 0.0, 1000.0, None) == 0
+    data = {"t": {"a": a, "b": b, "c": c}}
+    out = {}
+    for name, sql, nbytes in (
+            ("two_column_fused", "SELECT SUM(a), COUNT(*) FROM t WHERE a > 292.8932188 AND b < 707.1067812", n * 16),
+            ("three_column_bitmaps", "SELECT SUM(a) FROM t WHERE a > 292.89 AND b < 707.1 AND c >= 500", n * 24)):
+        res = [None]
+        t = _sync_time(lambda: res.__setitem__(0, vsql.execute(sql, data)), reps=5)
+        out[name] = {"ms": round(t * 1e3, 4), "gb_per_s": round(nbytes / t / 1e9, 1), "sql": sql,
+                     "result": res[0] if not isinstance(res[0], tuple) else list(res[0])}
+    del a, b, c, data
+    torch.cuda.empty_cache()
+    return out
+
+
 # The reference's own figures for its window family (docs/Interpreter.md:58-65:
 # 1M f64 rows, window 20; quantile window 100), milliseconds on its CPU.
 WINDOW_DOC_MS = {"sum": 12.0, "mean": 15.0, "delta": 2.0, "stddev": 39.0, "variance": 37.0, "quantile": 1000.0}
@@ -701,6 +730,8 @@ def main():
         out["nulls"] = bench_nulls(args.rows_log2)
     if not args.no_encodings:
         out["encodings"] = bench_encodings(args.rows_log2)
+    if not args.no_sql:
+        out["sql"] = bench_sql(args.rows_log2)
     if not args.no_serve:
         out["serve"] = bench_serve()
     if not args.no_cpu:

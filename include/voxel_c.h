@@ -208,7 +208,7 @@ int       voxel_gpu_argsort_f64(const double* x_dev, size_t n, int descending, u
  * rows with op_a(a[i], t_a) AND op_b(b[i], t_b); COUNT, SUM(a), MIN(a),
  * MAX(a).  Thresholds are bit patterns of the column type (f32/i32 in the
  * low 32 bits).  Float sums are compensated in the column's own type; MIN/MAX
- * skip NaN. */
+ * skip NaN.  b == a applies both predicates to column a (read once). */
 typedef struct voxel_gpu_scan_desc {
     int32_t  dtype;
     int32_t  op_a, op_b;       /* op_b = VOXEL_CMP_TRUE when b is NULL */

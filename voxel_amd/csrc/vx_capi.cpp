@@ -606,7 +606,7 @@ bool make_scan_args(const voxel_gpu_scan_desc* d, ScanArgs& g) {
     if (g.unroll != 0 && g.unroll != 2 && g.unroll != 4 && g.unroll != 8) { set_error("scan: unroll must be 2, 4 or 8"); return false; }
     g.pa = make_pred(d->dtype, d->op_a, d->t_a);
     if (d->b && d->op_b != CMP_TRUE) {
-        g.mode = 2;
+        g.mode = d->b == d->a ? 1 : 2;                    // both predicates on one column: read it once
         g.pb = make_pred(d->dtype, d->op_b, d->t_b);
     } else {
         g.mode = 0;
