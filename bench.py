@@ -481,6 +481,92 @@ def bench_window() -> dict:
     return out
 
 
+def cpu_reference_paths() -> dict:
+    """The reference's other hot-path CPU implementations (built from its
+    sources into oracle/_ref), each on a bounded sample of its workload on
+    this host: the interpreter fast path (one core), RunParallel (all the
+    host cores given to this process, at most 16), RadixSort<u64> (one core),
+    HashJoin<i64> Build + Probe (one core) and HashAggregator<i32,f64> (one
+    core); median of 3 after a warm-up."""
+    import numpy as np
+    from oracle import oracle as o
+    if o.R is None:
+        return None
+
+    def med(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    out = {}
+    rng = np.random.default_rng(5)
+    code = np.array([0x00001070, 0x000301C5, 0x000015D0, 0x0005002A, 0x00041120, 0x00021040, 0xFFA00052,
+                     0x00000001], np.uint32)
+    n = 1 << 25
+    x = rng.random(n) * 1000.0
+
+    def fast():
+        r = np.zeros(64, np.uint64)
+        r[2] = n
+        r[3] = o.f64_bits(500.0)
+        r[4] = 4
+        o.R.vr_engine_run_f64(o.ptr(x), n, o.ptr(code), code.size, o.ptr(r))
+    t = med(fast)
+    out["fast_path_f64"] = {"rows_per_s": round(n / t, 1), "gb_per_s": round(n * 8 / t / 1e9, 3), "cores": 1,
+                            "sample": "Engine<f64>::Run fast path, 2^25 rows, x > 500"}
+    threads = min(os.cpu_count() or 1, 16)
+
+    def par():
+        r = np.zeros(64, np.uint64)
+        r[2] = n
+        r[3] = o.f64_bits(500.0)
+        r[4] = 4
+        o.R.vr_engine_run_parallel_f64(o.ptr(x), n, o.ptr(code), code.size, o.ptr(r), threads, 0)
+    t = med(par)
+    out["run_parallel_f64"] = {"rows_per_s": round(n / t, 1), "gb_per_s": round(n * 8 / t / 1e9, 3),
+                               "cores": threads, "sample": "Engine<f64>::RunParallel, 2^25 rows, x > 500"}
+    del x
+    ns = 1 << 24
+    keys0 = rng.integers(0, 2**63, ns, dtype=np.int64).astype(np.uint64) * np.uint64(2)
+
+    def rsort():
+        k = keys0.copy()
+        idx = np.arange(ns, dtype=np.uint32)
+        o.R.vr_radix_sort_u64(o.ptr(k), o.ptr(idx), ns)
+    t = med(rsort)
+    out["radix_sort_u64"] = {"gkeys_per_s": round(ns / t / 1e9, 4), "cores": 1,
+                             "sample": "RadixSort<u64>::Sort with u32 row ids, 2^24 uniform keys (array copy included)"}
+    del keys0
+    nb, npr = 1 << 19, 1 << 24          # the reference's arena fails above 2^19 build rows (16 MiB blocks)
+    build = rng.integers(0, 2**62, nb, dtype=np.int64)
+    pay = rng.integers(-2**40, 2**40, nb, dtype=np.int64)
+    probe = np.where(rng.random(npr) < 0.9, build[rng.integers(0, nb, npr)],
+                     rng.integers(0, 2**62, npr, dtype=np.int64) | np.int64(2**62)).astype(np.int64)
+    left = np.zeros(npr, np.uint32)
+    right = np.zeros(npr, np.uint32)
+    ps = np.zeros(1, np.int64)
+    t = med(lambda: o.R.vr_hash_join_i64(o.ptr(build), nb, o.ptr(pay), o.ptr(probe), npr, o.ptr(left), o.ptr(right),
+                                         o.ptr(ps)), reps=2)
+    out["hash_join_i64"] = {"gprobes_per_s": round(npr / t / 1e9, 4), "cores": 1,
+                            "sample": "HashJoin<i64> Build(2^19) + Probe(2^24, 90 % hits) + payload sum; larger builds "
+                                      "crash the reference (its Arena allocates at most 16 MiB per block)"}
+    del build, pay, probe, left, right
+    m = 3_000_000
+    zone = rng.integers(1, 266, m).astype(np.int32)
+    fare = np.exp(2.5 + 0.6 * rng.standard_normal(m))
+    G = 1024
+    bufs = [np.zeros(G, t_) for t_ in (np.int32, np.uint64, np.float64, np.float64, np.float64, np.float64,
+                                       np.float64)]
+    t = med(lambda: o.R.vr_hash_agg_i32_f64(o.ptr(zone), o.ptr(fare), m, 512, G, *(o.ptr(b_) for b_ in bufs)))
+    out["hash_agg_i32_f64"] = {"rows_per_s": round(m / t, 1), "gb_per_s": round(m * 12 / t / 1e9, 3), "cores": 1,
+                               "sample": "HashAggregator<i32,f64> Init + Accumulate + Finalize, 3e6 rows, 265 zones"}
+    return out
+
+
 def cpu_window_baseline() -> dict:
     """The reference's window methods (Engine<f64>::Window*, built from its
     sources into oracle/_ref) on one host core: 1M rows, window 20 (quantile:
@@ -738,6 +824,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
         if not args.no_window and out["cpu_baseline"] is not None:
             out["cpu_baseline"]["window"] = cpu_window_baseline()
+        if out["cpu_baseline"] is not None:
+            out["cpu_baseline"]["paths"] = cpu_reference_paths()
     print(json.dumps(out))
 
 
