@@ -130,6 +130,7 @@ struct BitmapSrc {
 
 template <typename T>
 struct PredSrc {
+    using V = T;
     const T* x;
     PredDev pd;
     T* vals;
@@ -139,8 +140,9 @@ struct PredSrc {
         p.load(pd);
         return p(x[row]);
     }
-    __device__ __forceinline__ void emit(uint64_t row, uint64_t k) const {
-        if (vals) vals[k] = x[row];
+    __device__ __forceinline__ T fetch(uint64_t row) const { return x[row]; }
+    __device__ __forceinline__ void put(uint64_t row, uint64_t k, T v) const {
+        if (vals) vals[k] = v;
         if (idx) idx[k] = (uint32_t)row;
     }
 };

@@ -54,11 +54,13 @@ __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v;
 // CompactValid: rows whose validity bit is set, in row order.
 template <typename U>
 struct ValidSrc {
+    using V = U;
     const U* data;
     const uint64_t* valid;
     U* out;
     __device__ __forceinline__ bool pass(uint64_t i) const { return (valid[i >> 6] >> (i & 63)) & 1ull; }
-    __device__ __forceinline__ void emit(uint64_t i, uint64_t j) const { out[j] = data[i]; }
+    __device__ __forceinline__ U fetch(uint64_t i) const { return data[i]; }
+    __device__ __forceinline__ void put(uint64_t, uint64_t j, U v) const { out[j] = v; }
 };
 
 struct NullBufs {
