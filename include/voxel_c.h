@@ -414,7 +414,10 @@ int       voxel_gpu_mlp_train(float* params_dev, const float* x_dev, const float
 typedef struct voxel_gpu_learned_stats {
     int32_t  model;             /* 0 MLP CDF, 1 equal-width (linear in the key) */
     int32_t  log_grid;          /* model input: 1 = float bits of key - min */
-    int32_t  fell_back;         /* 1 = whole input sorted by the plain radix sort */
+    int32_t  fell_back;         /* 1 = whole input sorted by the plain radix sort (small input, or the
+                                   model did not split the keys); 2 = the plain radix sort chosen up
+                                   front: the sampled keys differ in at most 5 bytes, so it skips the
+                                   other passes and costs less than the learned partition */
     int32_t  partition_passes;  /* digit passes over the bucket id */
     int32_t  train_steps;
     float    final_loss;        /* mean squared error of the logit CDF fit */

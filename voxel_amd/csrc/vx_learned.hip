@@ -819,8 +819,14 @@ static bool train_graph(LearnedBufs& b, uint64_t T, int steps, hipStream_t st) {
 // pick the grid, train the MLP (model 0) and build the table that maps a key
 // to one of nfine buckets, non-decreasing in the key.  Records b.ev[1..3]
 // (b.ev[0] is the caller's); the table lives in b.cell.
+// The plain radix sort skips every byte on which all keys agree; when the
+// sample says at most this many bytes differ, it costs less than the
+// learned partition (three passes plus the bucket sort, measured as about
+// five and a half plain passes at 2^29 keys) and the sort takes it.
+constexpr int kPlainPassesMax = 5;
+
 static bool learned_table(LearnedBufs& b, const uint64_t* kin, uint64_t n, uint64_t flip, int model, uint32_t nfine,
-                          voxel_gpu_learned_stats& S, TableDigit& dig, StreamCtx* c) {
+                          voxel_gpu_learned_stats& S, TableDigit& dig, StreamCtx* c, bool* prefer_plain = nullptr) {
     hipStream_t st = c->stream;
     uint64_t SS = 1ull << g_tune.sample_log2;                        // sample size
     if (SS > n / 4) SS = n / 4;
@@ -845,8 +851,16 @@ static bool learned_table(LearnedBufs& b, const uint64_t* kin, uint64_t n, uint6
         !hip_ok(hipMemcpyAsync(b.qhost, b.qdev, 1025 * 8, hipMemcpyDeviceToHost, st), "read quantiles") ||
         !hip_ok(hipStreamSynchronize(st), "sync"))
         return false;
-    // 2. grid: linear or logarithmic, whichever the sample CDF is straighter on
     const uint64_t* q = b.qhost;
+    if (prefer_plain) {                                // bytes on which the sampled keys differ
+        uint64_t diff = 0;
+        for (int i = 1; i <= 1024; ++i) diff |= q[i] ^ q[0];
+        int bytes = 0;
+        for (int j = 0; j < 8; ++j) bytes += ((diff >> (8 * j)) & 0xFF) != 0;
+        *prefer_plain = bytes <= kPlainPassesMax;
+        if (*prefer_plain) return true;
+    }
+    // 2. grid: linear or logarithmic, whichever the sample CDF is straighter on
     const uint64_t kmin = q[0], kmax = q[1024];
     const uint64_t span = kmax - kmin;
     auto logg = [&](uint64_t k) { return (double)TableDigit::grid(k - kmin, 1); };
@@ -1072,7 +1086,17 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
 
     hipEventRecord(b.ev[0], st);
     TableDigit dig;
-    if (!learned_table(b, kin, n, 0, model, nfine, S, dig, c)) return -1;
+    bool plain = false;
+    if (!learned_table(b, kin, n, 0, model, nfine, S, dig, c, &plain)) return -1;
+    if (plain) {
+        // few distinct key bytes: the plain radix sort's skipped passes win
+        S.fell_back = 2;
+        const uint32_t* vin = payload();
+        if (!vin || sort_pairs_u64(kin, vin, kout, vout, n, c) < 0) return -1;
+        if (!hip_ok(hipStreamSynchronize(st), "sync")) return -1;
+        if (stats) *stats = S;
+        return 0;
+    }
     // 5. stable partition by bucket id (LSD over its bytes); pass 0 computes
     //    the id once and every pass carries it next to the rows
     const uint32_t* vin = payload();
