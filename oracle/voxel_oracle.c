@@ -1221,3 +1221,61 @@ void vo_for_decode(double base, double scale, const int32_t* enc, size_t rows, s
     size_t i;
     for (i = 0; i < cnt && off + i < rows; ++i) out[i] = base + (double)enc[off + i] * scale;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Tier-0 loops whose load is a window word on Engine<f64> (handlers        */
+/* engine.hpp:2589-2640 WSTD / WVARIANCE_W, 3167-3226 WDELTA / WINDOW_SUM / */
+/* WINDOW_MEAN): wkind 1 WINDOW_SUM, 2 WINDOW_MEAN (sum * (1 / w)), 3 WSTD,  */
+/* 4 WVARIANCE_W, 5 WDELTA.  Lane j of a group at offset o is the window    */
+/* value at row o + j, 0 when the window passes Count; the word itself      */
+/* advances the offset register (min(4, Count - o); WDELTA: the lanes it    */
+/* wrote), and the loop ends when the offset equals total.  The body's      */
+/* vector words and reductions are those of vo_tier0_expr_f64.  Returns 1,  */
+/* or 0 when the reference would not terminate (a no-op window word, or an  */
+/* offset that steps past total or stops short of it).                      */
+/* ------------------------------------------------------------------------ */
+static double vo_window_lane(int wkind, const double* d, uint64_t count, uint64_t start, int w) {
+    double s = 0.0, mean, q = 0.0;
+    int k;
+    if (start + (uint64_t)w > count) return 0.0;
+    for (k = 0; k < w; ++k) s += d[start + k];
+    if (wkind == 1) return s;
+    if (wkind == 2) return s * (1.0 / (double)w);
+    mean = s / (double)w;
+    for (k = 0; k < w; ++k) { double e = d[start + k] - mean; q += e * e; }
+    return wkind == 3 ? sqrt(q / (double)w) : q / (double)w;
+}
+
+int vo_tier0_window_f64(const double* data, uint64_t count, uint64_t off, uint64_t total, int wkind, int window,
+                        int vload_dst, int nops, const int* code, const double* sval, int nb, const int* bvreg,
+                        const int* red, const int* accop, uint64_t* acc, uint64_t* part) {
+    double v[16][4];
+    int i, k, b;
+    memset(v, 0, sizeof(v));
+    if (wkind != 5 && (window == 0 || ((wkind == 3 || wkind == 4) && window < 2))) return 0;
+    do {
+        uint64_t adv;
+        if (wkind == 5) {
+            uint64_t cnt = 4;
+            if (off + 4 >= count) cnt = count > off + 1 ? count - off - 1 : 0;
+            for (i = 0; i < 4; ++i) v[vload_dst][i] = (uint64_t)i < cnt ? data[off + i + 1] - data[off + i] : 0.0;
+            adv = cnt;
+        } else {
+            for (i = 0; i < 4; ++i) v[vload_dst][i] = vo_window_lane(wkind, data, count, off + (uint64_t)i, window);
+            adv = count > off ? (count - off < 4 ? count - off : 4) : 0;
+        }
+        for (k = 0; k < nops; ++k) {
+            const int* c = code + 5 * k;
+            double r[4];
+            for (i = 0; i < 4; ++i) r[i] = vo_expr_lane_f64(c[0], c[4], v[c[2]][i], v[c[3]][i], sval[k]);
+            for (i = 0; i < 4; ++i) v[c[1]][i] = r[i];
+        }
+        for (b = 0; b < nb; ++b) {
+            part[b] = vo_red_f64(v[bvreg[b]], red[b]);
+            acc[b] = vo_tier0_accum(acc[b], part[b], accop[b]);
+        }
+        if (adv == 0 || off + adv > total) return 0;
+        off += adv;
+    } while (off != total);
+    return 1;
+}

@@ -227,3 +227,20 @@ def test_no_gpu_means_loud_failure():
     with pytest.raises(RuntimeError, match="no usable HIP device"):
         e.run()
     assert e.get_scalar(0) == 0 and e.get_scalar(1) == 0            # registers untouched
+
+
+def test_lowering_accepts_window_word_loops():
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(7)
+    for wkind in range(1, 6):
+        code, spec = tg.make_window_program(rng, wkind, 8 if wkind != 5 else 0, nops=2)
+        assert lowered(list(code)) == 2, (wkind, spec)
+    W = tg.w
+    body = [W(0xD0, rd=5, ra=0), W(0x2A, rd=0, ra=0, rb=5)]
+    def loop(load, extra=()):
+        words = [load] + body + list(extra) + [W(0x40, ra=1, rb=2)]
+        return words + [W(0x52, imm=(0 - len(words)) & 0xFFF), W(0x01)]
+    assert lowered(loop(W(0xDE, rd=0, ra=1, imm=20))) == 2
+    assert lowered(loop(W(0xDE, rd=0, ra=1, imm=0))) == 0              # window 0: the word is a no-op
+    assert lowered(loop(W(0x97, rd=0, ra=1, imm=1))) == 0              # WSTD needs a window of 2 or more
+    assert lowered(loop(W(0x99, rd=0, ra=1, imm=5))) == 0              # WQUANTILE is not lowered

@@ -754,3 +754,34 @@ def test_delta_bitpacked_for_restatement_against_reference():
             dec = np.zeros(n)
             o.O.vo_for_decode(s1[0], s1[1], o.ptr(e1), n, 0, n, o.ptr(dec))
             assert np.all(np.abs(dec - f) <= s1[1] * 0.5 + 1e-9 * np.abs(f).max())
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_tier0_window_restatement_against_reference_engine():
+    """vo_tier0_window_f64 (loops loading WINDOW_SUM / WINDOW_MEAN / WSTD /
+    WVARIANCE_W / WDELTA, vector words, up to three reductions) against the
+    reference engine running the same words: every accumulator and partial
+    bit-identical."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(93)
+    for trial in range(80):
+        wkind = 1 + trial % 5
+        n = int(rng.integers(6, 400))
+        window = int(rng.choice([1, 2, 3, 7, 20, 64])) if wkind != 5 else 0
+        if wkind in (3, 4) and window < 2:
+            window = 2
+        x = np.round(rng.standard_normal(n) * 100, 2)
+        regs = np.zeros(64, np.uint64)
+        for r in tg.SCAL_REGS:
+            regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+        total = tg.window_total(rng, n, wkind)
+        regs[2] = total
+        code, spec = tg.make_window_program(rng, wkind, window, nops=int(rng.integers(0, 4)))
+        ok, acc, part = tg.oracle_window_loop(o, x, total, spec, regs, wkind, window)
+        assert ok, (trial, wkind, n, total)
+        ref = regs.copy()
+        o.R.vr_engine_run_f64(o.ptr(x), n, o.ptr(code), code.size, o.ptr(ref))
+        assert int(ref[1]) == total
+        for b in range(len(spec["red"])):
+            assert int(ref[tg.ACCS[b]]) == acc[b], (trial, wkind, b, spec)
+            assert int(ref[tg.PARTS[b]]) == part[b], (trial, wkind, b, spec)

@@ -339,6 +339,81 @@ def test_tier0_vector_arithmetic(gpu, is_f64):
             e.run()
 
 
+def test_tier0_window_word_loops(gpu):
+    """Loops loading WINDOW_SUM / WINDOW_MEAN / WSTD / WVARIANCE_W / WDELTA
+    (the word advancing the offset), vector words and up to three
+    reductions: partials, counts, min and max equal the oracle's bits
+    (vo_tier0_window_f64, pinned against the reference engine), NaN results
+    compared as NaN; sums within 2 ulp of the exact sum of the reduced lanes
+    (computed by the oracle's vector evaluation on the window values).
+    Loop bounds the reference never reaches are refused."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(73)
+
+    def same(got, want):
+        if math.isnan(o.bits_f64(want)):
+            return math.isnan(o.bits_f64(got))
+        return got == want
+
+    for trial in range(40):
+        wkind = 1 + trial % 5
+        n = int(rng.integers(8, 20000))
+        window = int(rng.choice([1, 2, 5, 20, 255])) if wkind != 5 else 0
+        if wkind in (3, 4) and window < 2:
+            window = 2
+        x = np.round(rng.standard_normal(n) * 100, 2)
+        regs = np.zeros(64, np.uint64)
+        for r in tg.SCAL_REGS:
+            regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+        total = tg.window_total(rng, n, wkind)
+        regs[2] = total
+        code, spec = tg.make_window_program(rng, wkind, window, nops=int(rng.integers(0, 4)))
+        ok, acc, part = tg.oracle_window_loop(o, x, total, spec, regs, wkind, window)
+        assert ok
+        e = vx.EngineF64()
+        e.add_segment(x)
+        for r in (2,) + tg.SCAL_REGS:
+            e.set_scalar(r, int(regs[r]))
+        e.load_program([int(c) for c in code])
+        e.run()
+        assert e.get_scalar(1) == total
+        for b, red in enumerate(spec["red"]):
+            assert same(e.get_scalar(tg.PARTS[b]), part[b]), (trial, wkind, b, spec)
+            got = e.get_scalar(tg.ACCS[b])
+            if red == 0:
+                g, want = o.bits_f64(got), o.bits_f64(acc[b])
+                if math.isnan(want):
+                    assert math.isnan(g)
+                else:
+                    # the reduced lanes: window values row by row, then the body's words
+                    wv = np.zeros(max(n, 4 * ((total + 3) // 4)) + 4)
+                    kind = {1: "sum", 2: None, 3: "stddev", 4: "variance", 5: "delta"}[wkind]
+                    if wkind == 2:                        # WINDOW_MEAN multiplies by 1 / w
+                        m = max(n - window + 1, 0)
+                        sums = np.zeros(m)
+                        o.O.vo_window_f64(1, o.ptr(x), n, o.ptr(sums), m, window, 0.0)
+                        wv[:m] = sums * (1.0 / window)
+                    else:
+                        m = o.O.vo_window_f64(("delta", "sum", "mean", "stddev", "variance").index(kind),
+                                              o.ptr(x), n, o.ptr(wv), wv.size, window, 0.0)
+                        wv[m:] = 0.0
+                    lanes = tg.oracle_expr_col(o, wv[:n], 4 * ((total + 3) // 4), spec, regs, spec["bvreg"][b], True)
+                    assert close_f64(g, o.exact_sum(lanes[: 4 * ((total + 3) // 4)])), (trial, wkind, b)
+            else:
+                assert same(got, acc[b]), (trial, wkind, b, spec)
+    # a bound the window word never steps onto: refused, registers untouched
+    e = vx.EngineF64()
+    x = np.arange(10, dtype=np.float64)
+    e.add_segment(x)
+    e.set_scalar(2, 7)
+    code, _ = tg.make_window_program(np.random.default_rng(1), 1, 3, nops=0)
+    e.load_program([int(c) for c in code])
+    with pytest.raises(RuntimeError, match="would not terminate"):
+        e.run()
+    assert e.get_scalar(1) == 0
+
+
 @pytest.mark.parametrize("is_f64", [True, False])
 def test_tier0_scalar_prefix_and_epilogue(gpu, is_f64):
     """Scalar words (integer, f64 and bitwise arithmetic, DIV/MOD/DIVF by

@@ -247,3 +247,43 @@ def oracle_expr(o, x, total, spec, regs, is_f64=True, acc0=None):
     fn(o.ptr(x), x.size, 0, total, 0, len(ops), o.ptr(code), o.ptr(sval), nb, o.ptr(bvreg), o.ptr(red),
        o.ptr(accop), o.ptr(acc), o.ptr(part))
     return [int(a) for a in acc], [int(p) for p in part]
+
+
+WINDOW_OPS = {1: 0xDE, 2: 0xDF, 3: 0x97, 4: 0x98, 5: 0xDD}   # WINDOW_SUM WINDOW_MEAN WSTD WVARIANCE_W WDELTA
+
+
+def make_window_program(rng, wkind, window, nops=3, reds=(0, 1, 2, 3)):
+    """A loop whose load is a window word (segment 0, offset R1, the word
+    itself advancing R1), then the vector words and reductions of
+    make_expr_program, CMP R1, R2; JNZ; HALT (no ADD of the offset)."""
+    code, spec = make_expr_program(rng, True, nops=nops, reds=reds)
+    code = [int(c) for c in code]
+    imm = (window & 0xFF) if wkind != 5 else 0
+    code[1] = w(WINDOW_OPS[wkind], rd=0, ra=1, imm=imm)
+    del code[-4]                                           # the ADD R1, R1, R4 of the VLOAD loop
+    jnz = len(code) - 2
+    code[jnz] = w(0x52, imm=(1 - jnz) & 0xFFF)
+    return np.array(code, np.uint32), spec
+
+
+def oracle_window_loop(o, x, total, spec, regs, wkind, window, acc0=None):
+    """vo_tier0_window_f64 on the spec: (terminates, accumulators, partials)."""
+    code, sval = _expr_words(spec, regs, True)
+    nb = len(spec["red"])
+    bvreg = np.array(spec["bvreg"], np.int32)
+    red = np.array(spec["red"], np.int32)
+    accop = np.array(spec["accop"], np.int32)
+    acc = np.array(acc0 if acc0 is not None else [0] * nb, np.uint64)
+    part = np.zeros(nb, np.uint64)
+    ok = o.O.vo_tier0_window_f64(o.ptr(x), x.size, 0, total, wkind, window, 0, len(spec["ops"]), o.ptr(code),
+                                 o.ptr(sval), nb, o.ptr(bvreg), o.ptr(red), o.ptr(accop), o.ptr(acc), o.ptr(part))
+    return bool(ok), [int(a) for a in acc], [int(p) for p in part]
+
+
+def window_total(rng, n, wkind):
+    """A loop bound the reference reaches: Count (WDELTA: Count - 1), or a
+    multiple of 4 short of it."""
+    end = n - 1 if wkind == 5 else n
+    if rng.random() < 0.5 or end < 8:
+        return end
+    return 4 * int(rng.integers(1, (end - 1) // 4 + 1))

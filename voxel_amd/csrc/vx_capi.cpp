@@ -61,6 +61,7 @@ struct Engine {
     bool agg_ready = false;
     SegaggBufs segagg;
     DevColumn expr;                  // a reduced vector of a loop with vector arithmetic
+    DevColumn wcol;                  // window values of a loop loading a window word
 };
 
 size_t lanes_of(int dt) { return 32 / dtype_size(dt); }
@@ -209,23 +210,16 @@ bool run_tier0_block(const Plan::Block& b, const PredDev& pred, int dtype, const
 // (lanes are independent) into the reduced register's values over every row
 // the loop visits, [off, total), rows past Count loading 0; each block then
 // reduces that column as an unfiltered loop would.
-bool run_tier0_expr(Engine& e, uint64_t* regs, uint64_t off, uint64_t total, StreamCtx* c) {
+// The body's vector words over rows [off, off + nrows) of column col (rows at
+// or past count load 0), one evaluation per reduced register, each reduced
+// as an unfiltered loop would; then the loop's register effects.
+bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count, uint64_t off, uint64_t nrows,
+                     uint64_t total, StreamCtx* c) {
     const Plan& p = e.plan;
     const uint64_t kl = 4;
-    if (e.dtype == DT_I64)
-        for (const Plan::VOp& v : p.vops)
-            if (v.op == 0x83 || v.op == 0x93) {
-                set_error("integer vector division is not lowered");
-                return false;
-            }
-    const uint64_t n = total - off;
-    if (n > (1ull << 31)) { set_error("a loop with vector arithmetic over %llu rows (more than 2^31) is not lowered",
-                                      (unsigned long long)n); return false; }
-    Seg& s = *e.segs[p.seg];
-    const uint64_t dend = total < s.count ? total : s.count;
-    const void* d = seg_rows(s, e.dtype, off < dend ? off : dend, dend, c);
-    if (!d) return false;
-    if (!e.expr.ensure(n * 8)) return false;
+    if (nrows > (1ull << 31)) { set_error("a loop with vector arithmetic over %llu rows (more than 2^31) is not lowered",
+                                          (unsigned long long)nrows); return false; }
+    if (!e.expr.ensure(nrows * 8)) return false;
     ExprProg prog{};
     prog.nops = (int32_t)p.vops.size();
     prog.vload = p.vload_dst;
@@ -236,13 +230,66 @@ bool run_tier0_expr(Engine& e, uint64_t* regs, uint64_t off, uint64_t total, Str
     const PredDev all = make_pred(e.dtype, CMP_TRUE, 0);
     for (const Plan::Block& b : p.blocks) {
         prog.target = b.vreg;
-        if (!expr_eval(e.dtype, d, s.count, off, n, prog, e.expr.dev, c)) return false;
-        if (!run_tier0_block(b, all, e.dtype, e.expr.dev, n, 0, n, n / kl, regs, c)) return false;
+        if (!expr_eval(e.dtype, col, count, off, nrows, prog, e.expr.dev, c)) return false;
+        if (!run_tier0_block(b, all, e.dtype, e.expr.dev, nrows, 0, nrows, nrows / kl, regs, c)) return false;
     }
     regs[p.off] = total;
     if (run_scalar(p.epilogue, regs) < 0) return false;
     std::memcpy(e.regs, regs, sizeof(uint64_t) * 64);
     return true;
+}
+
+bool run_tier0_expr(Engine& e, uint64_t* regs, uint64_t off, uint64_t total, StreamCtx* c) {
+    const Plan& p = e.plan;
+    if (e.dtype == DT_I64)
+        for (const Plan::VOp& v : p.vops)
+            if (v.op == 0x83 || v.op == 0x93) {
+                set_error("integer vector division is not lowered");
+                return false;
+            }
+    Seg& s = *e.segs[p.seg];
+    const uint64_t dend = total < s.count ? total : s.count;
+    const void* d = seg_rows(s, e.dtype, off < dend ? off : dend, dend, c);
+    if (!d) return false;
+    return run_expr_blocks(e, regs, d, s.count, off, total - off, total, c);
+}
+
+// Loops loading a window word: the word writes lane j = the window value at
+// row offset + j (0 once the window passes Count) and advances the offset by
+// min(4, Count - offset) (WDELTA: by the lanes it wrote), so the loop ends
+// only at a total it steps onto; the window values are computed once as a
+// column (the window family's kernels, WINDOW_MEAN as sum * (1 / w)) and the
+// body runs on it row by row.
+bool run_tier0_window(Engine& e, uint64_t* regs, StreamCtx* c) {
+    const Plan& p = e.plan;
+    if (e.dtype != DT_F64) { set_error("window words are lowered for f64 engines only"); return false; }
+    if (p.seg >= e.segs.size()) { set_error("program reads segment %u, engine has %zu", p.seg, e.segs.size()); return false; }
+    Seg& s = *e.segs[p.seg];
+    const uint64_t off = regs[p.off], total = regs[p.tot], count = s.count;
+    const uint64_t last = p.wkind == 5 ? (count ? count - 1 : 0) : count;   // where the partial step lands
+    uint64_t groups = 0;
+    bool ok = false;
+    if (off < last && total > off) {
+        if (total == last) { groups = (total - off + 3) / 4; ok = true; }
+        else if (total < last && (total - off) % 4 == 0) {  // full steps of 4 all the way
+            ok = true;
+            groups = (total - off) / 4;
+        }
+    }
+    if (!ok) {
+        set_error("the reference loop would not terminate: the window word steps the offset R%u from %llu to %llu, "
+                  "never onto R%u = %llu", p.off, (unsigned long long)off, (unsigned long long)last, p.tot,
+                  (unsigned long long)total);
+        return false;
+    }
+    const void* d = seg_rows(s, e.dtype, 0, count, c);
+    if (!d) return false;
+    if (!e.wcol.ensure(count * 8)) return false;
+    static const int kind_of[6] = {0, 1, 6, 3, 4, 0};    // window_launch kinds: sum, mean by 1/w, std, var, delta
+    const int64_t nvalid = window_launch(kind_of[p.wkind], static_cast<const double*>(d), count, count, p.window,
+                                         static_cast<double*>(e.wcol.dev), c);
+    if (nvalid < 0) return false;
+    return run_expr_blocks(e, regs, e.wcol.dev, (uint64_t)nvalid, off, groups * 4, total, c);
 }
 
 bool run_tier0(Engine& e, StreamCtx* c) {
@@ -256,6 +303,7 @@ bool run_tier0(Engine& e, StreamCtx* c) {
         std::memcpy(e.regs, regs, sizeof(regs));
         return true;
     }
+    if (p.wkind) return run_tier0_window(e, regs, c);
     const uint64_t kl = 4;
     if (regs[p.step] != kl) { set_error("loop step register R%u holds %llu, not the lane count %llu", p.step,
                                         (unsigned long long)regs[p.step], (unsigned long long)kl); return false; }

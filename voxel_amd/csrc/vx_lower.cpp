@@ -141,12 +141,30 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     size_t i = 0;
     while (i < len && host_op(op_of(c[i]))) p.prefix.push_back(c[i++]);
     const size_t L = i;
-    if (i >= len || op_of(c[i]) != OP_VLOAD) return none("no VLOAD loop: program is not a lowerable scan loop");
+    auto window_kind = [](uint8_t op) -> uint8_t {
+        switch (op) {
+        case 0xDE: return 1;                               // WINDOW_SUM
+        case 0xDF: return 2;                               // WINDOW_MEAN
+        case 0x97: return 3;                               // WSTD
+        case 0x98: return 4;                               // WVARIANCE_W
+        case 0xDD: return 5;                               // WDELTA
+        default: return 0;
+        }
+    };
+    if (i >= len || (op_of(c[i]) != OP_VLOAD && !window_kind(op_of(c[i]))))
+        return none("no VLOAD loop: program is not a lowerable scan loop");
     uint32_t vl = c[i++];
     const uint8_t vA = rd_of(vl), rOff = ra_of(vl);
     p.seg = (imm_of(vl) >> 8) & 0xF;
-    p.vcount = imm_of(vl) & 0xFF;
+    p.vcount = op_of(vl) == OP_VLOAD ? imm_of(vl) & 0xFF : 0;
     p.off = rOff;
+    p.wkind = window_kind(op_of(vl));
+    if (p.wkind) {
+        p.window = p.wkind == 5 ? 0 : imm_of(vl) & 0xFF;
+        if (p.wkind != 5 && (p.window == 0 || (p.wkind >= 3 && p.window < 2)))
+            return none("the window word is a no-op for this window (the reference loop would not end)");
+        p.has_arith = true;                                // lanes evaluated row by row
+    }
 
     // vector register -> filters applied to it (-1: not defined in the loop,
     // -2: computed by vector arithmetic)
@@ -246,17 +264,22 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     }
     if (!p.has_arith && !chain_ok) return none("more than two chained VFILTERs");
 
-    if (i + 2 >= len) return none("loop control missing");
-    uint32_t adw = c[i], cw = c[i + 1], jw = c[i + 2];
-    if (op_of(adw) != OP_ADD || rd_of(adw) != rOff) return none("loop does not advance the offset register");
-    uint8_t rStep;
-    if (ra_of(adw) == rOff) rStep = rb_of(adw);
-    else if (rb_of(adw) == rOff) rStep = ra_of(adw);
-    else return none("loop does not advance the offset register");
+    uint8_t rStep = 0xFF;
+    if (!p.wkind) {
+        if (i >= len) return none("loop control missing");
+        const uint32_t adw = c[i];
+        if (op_of(adw) != OP_ADD || rd_of(adw) != rOff) return none("loop does not advance the offset register");
+        if (ra_of(adw) == rOff) rStep = rb_of(adw);
+        else if (rb_of(adw) == rOff) rStep = ra_of(adw);
+        else return none("loop does not advance the offset register");
+        ++i;
+    }
+    if (i + 1 >= len) return none("loop control missing");
+    uint32_t cw = c[i], jw = c[i + 1];
     if (op_of(cw) != OP_CMP || ra_of(cw) != rOff) return none("loop does not compare the offset");
     uint8_t rTot = rb_of(cw);
     if (op_of(jw) != OP_JNZ) return none("loop does not end in JNZ");
-    long target = (long)(i + 2) + simm_of(jw);
+    long target = (long)(i + 1) + simm_of(jw);
     if (target < 0 || target > (long)L) return none("loop branch does not return to VLOAD");
     // registers the loop writes: every block's partial and accumulator, the offset
     bool wr[16] = {};
@@ -284,14 +307,14 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
                 if (op_of(c[k]) == OP_MOVR && op_of(c[q]) != OP_NOP && rd_of(c[q]) == ra_of(c[k]))
                     return none("loop re-runs a register move that is not idempotent");
     }
-    i += 3;
+    i += 2;
     while (i < len && host_op(op_of(c[i]))) p.epilogue.push_back(c[i++]);
     if (i < len && op_of(c[i]) != OP_HALT) return none("instructions after the loop are not lowerable");
     p.step = rStep;
     p.tot = rTot;
 
     // Registers written by the loop must not be inputs of the loop.
-    if (wr[rStep] || wr[rTot]) return none("loop overwrites its own bound, step or threshold");
+    if ((rStep != 0xFF && wr[rStep]) || wr[rTot]) return none("loop overwrites its own bound, step or threshold");
     for (const Plan::Filt& f : p.filts)
         if (f.cmp != 7 && wr[f.thr]) return none("loop overwrites its own bound, step or threshold");
     for (const Plan::VOp& v : p.vops)

@@ -77,6 +77,11 @@ struct Plan {
     bool has_arith = false;
     uint8_t vload_dst = 0;
     std::vector<VOp> vops;
+    // Loops whose load is a window word (engine.hpp:2589-2640, 3167-3226):
+    // 1 WINDOW_SUM, 2 WINDOW_MEAN, 3 WSTD, 4 WVARIANCE_W, 5 WDELTA (0: VLOAD).
+    // The word advances the offset itself; the loop has no ADD of it.
+    uint8_t wkind = 0;
+    uint8_t window = 0;
 };
 
 // Interpreter view: fast path first (exactly DetectFastPath), then the

@@ -54,6 +54,13 @@ struct ExprProg {
 bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,
                StreamCtx* c);
 
+// The window family's kernels on a device f64 column (vx_window.hip; the
+// caller holds the library lock): kind 0 delta, 1 sum, 2 mean (sum / w),
+// 3 stddev, 4 variance, 5 quantile q, 6 mean as sum * (1 / w).  Writes
+// out[0, N) and returns N (the reference's output count), -1 on error.
+int64_t window_launch(int kind, const double* d, uint64_t count, uint64_t out_len, int window, double* out,
+                      StreamCtx* c, double q = 0.0);
+
 // Order key <-> value bits (host).
 uint64_t order_key(uint64_t bits, int dtype);
 uint64_t order_unkey(uint64_t key, int dtype);

@@ -27,6 +27,7 @@
 #include "vx_groupby.hpp"
 #include "vx_kernels.hpp"
 #include "vx_runtime.hpp"
+#include "vx_segagg.hpp"
 
 #pragma clang fp contract(off)
 
@@ -39,7 +40,7 @@ constexpr int kWTile = kWThreads * kWRows;        // outputs per workgroup
 constexpr int kMaxWindow = 255;                   // the window is a u8 (engine.hpp:197)
 constexpr int kQuantMax = 128;                    // WindowQuantile's buffer (engine.hpp:284)
 
-enum WinKind { K_DELTA = 0, K_SUM = 1, K_MEAN = 2, K_STD = 3, K_VAR = 4, K_QUANT = 5 };
+enum WinKind { K_DELTA = 0, K_SUM = 1, K_MEAN = 2, K_STD = 3, K_VAR = 4, K_QUANT = 5, K_MEAN_MUL = 6 };
 
 // Thread t owns the kWRows consecutive outputs o = t*kWRows + r.  The window
 // rows are walked in chunks of kWRows: each chunk reads kWRows new values
@@ -90,6 +91,10 @@ __global__ __launch_bounds__(kWThreads) void k_window(const double* __restrict__
     } else if constexpr (KIND == K_MEAN) {
 #pragma unroll
         for (int r = 0; r < R; ++r) res[r] = s[r] / wd;          // WindowMean divides (engine.hpp:221)
+    } else if constexpr (KIND == K_MEAN_MUL) {
+        const double winv = 1.0 / wd;                             // WINDOW_MEAN multiplies (engine.hpp:3214)
+#pragma unroll
+        for (int r = 0; r < R; ++r) res[r] = s[r] * winv;
     } else {
         double mean[R], q[R];
 #pragma unroll
@@ -227,15 +232,12 @@ int64_t window_count(int kind, uint64_t count, uint64_t out_len, int w, double q
     return (int64_t)std::min<uint64_t>(count - (uint64_t)w + 1, out_len);
 }
 
-int64_t window_run(int kind, const double* d, size_t count, double* out, size_t out_len, int window, double q,
-                   void* stream) {
-    set_error("%s", "");
-    if (window < 0 || window > kMaxWindow) { set_error("window %d is not a u8 (0..255)", window); return -1; }
-    if ((count && !d) || (out_len && !out)) { set_error("window: null device pointer"); return -1; }
-    std::lock_guard<std::mutex> lk(api_mutex());
-    StreamCtx* c = ctx_for(stream);
-    if (!c) return -1;
-    const int64_t N = window_count(kind, count, out_len, window, q);
+}  // namespace
+
+// The window family's launches (caller holds the library lock): returns N.
+int64_t window_launch(int kind, const double* d, uint64_t count, uint64_t out_len, int window, double* out,
+                      StreamCtx* c, double q) {
+    const int64_t N = window_count(kind == K_MEAN_MUL ? K_MEAN : kind, count, out_len, window, q);
     if (N <= 0) return 0;
     const uint64_t nout = (uint64_t)N;
     switch (kind) {
@@ -257,6 +259,10 @@ int64_t window_run(int kind, const double* d, size_t count, double* out, size_t 
         break;
     case K_VAR:
         hipLaunchKernelGGL(k_window<K_VAR>, dim3(blocks_for(nout, kWTile)), dim3(kWThreads), 0, c->stream, d,
+                           (uint64_t)count, out, nout, window);
+        break;
+    case K_MEAN_MUL:
+        hipLaunchKernelGGL(k_window<K_MEAN_MUL>, dim3(blocks_for(nout, kWTile)), dim3(kWThreads), 0, c->stream, d,
                            (uint64_t)count, out, nout, window);
         break;
     default: {
@@ -288,6 +294,17 @@ int64_t window_run(int kind, const double* d, size_t count, double* out, size_t 
     return N;
 }
 
+namespace {
+int64_t window_run(int kind, const double* d, size_t count, double* out, size_t out_len, int window, double q,
+                   void* stream) {
+    set_error("%s", "");
+    if (window < 0 || window > kMaxWindow) { set_error("window %d is not a u8 (0..255)", window); return -1; }
+    if ((count && !d) || (out_len && !out)) { set_error("window: null device pointer"); return -1; }
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(stream);
+    if (!c) return -1;
+    return window_launch(kind, d, count, out_len, window, out, c, q);
+}
 }  // namespace
 }  // namespace vx
 
