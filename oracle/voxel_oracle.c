@@ -377,13 +377,17 @@ VO_DEF_TIER0_MULTI(vo_tier0_multi_i64, int64_t, vo_red_i64)
 
 /* ------------------------------------------------------------------------ */
 /* Tier-0 loops with vector arithmetic (engine.hpp:2221-2350 VADD VSUB VMUL */
-/* VDIV VNEG VABS VMIN VMAX, 2473-2590 VSADD VSSUB VSMUL VSDIV VSMIN VSMAX)  */
+/* VDIV VMOD VNEG VABS VMIN VMAX, 2473-2590 VSADD VSSUB VSMUL VSDIV VSMOD    */
+/* VSMIN VSMAX)                                                             */
 /* and filters, per kL = 4 group: VLOAD into register vload_dst (zero past  */
 /* Count), then ops in program order on 16 vector registers; op word        */
 /* k: code[5k..5k+4] = opcode, dst, a, b, filter kCmp; sval[k] = the scalar */
 /* operand / threshold (register value as T).  Reduction block b reduces    */
 /* register bvreg[b] and folds into acc[b] as in vo_tier0_*.  f64 division  */
-/* by zero gives a quiet NaN (VDIV per lane, VSDIV for every lane).         */
+/* by zero gives a quiet NaN (VDIV per lane, VSDIV for every lane), f64 MOD  */
+/* is fmod.  Integers: VDIV by a zero lane gives INT64_MAX, VSDIV by zero 0, */
+/* VMOD / VSMOD by zero 0 (engine.hpp:2263-2291, 2516-2549); x / -1 is the   */
+/* wrapping negation here (the reference's x86 build traps on INT64_MIN/-1). */
 /* ------------------------------------------------------------------------ */
 #define VO_DEF_EXPR_LANE(NAME, T, ISF)                                                      \
 static T NAME(int op, int cmp, T x, T y, T s) {                                              \
@@ -391,7 +395,10 @@ static T NAME(int op, int cmp, T x, T y, T s) {                                 
     case 0x80: return ISF ? x + y : (T)((uint64_t)x + (uint64_t)y);                          \
     case 0x81: return ISF ? x - y : (T)((uint64_t)x - (uint64_t)y);                          \
     case 0x82: return ISF ? x * y : (T)((uint64_t)x * (uint64_t)y);                          \
-    case 0x83: return (y == (T)0) ? (T)NAN : x / y;                                          \
+    case 0x83: return ISF ? ((y == (T)0) ? (T)NAN : x / y)                                    \
+                          : (y == (T)0 ? (T)INT64_MAX : y == (T)-1 ? (T)(0 - (uint64_t)x) : (T)((int64_t)x / (int64_t)y)); \
+    case 0x84: return ISF ? (T)fmod((double)x, (double)y)                                    \
+                          : (y == (T)0 || y == (T)-1 ? (T)0 : (T)((int64_t)x % (int64_t)y));  \
     case 0x85: return ISF ? -x : (T)(0 - (uint64_t)x);                                       \
     case 0x86: return ISF ? (T)fabs((double)x) : (x < (T)0 ? (T)(0 - (uint64_t)x) : x);      \
     case 0x87: return ISF ? ((x < y || x != x) ? x : y) : (x < y ? x : y);                   \
@@ -399,7 +406,10 @@ static T NAME(int op, int cmp, T x, T y, T s) {                                 
     case 0x90: return ISF ? x + s : (T)((uint64_t)x + (uint64_t)s);                          \
     case 0x91: return ISF ? x - s : (T)((uint64_t)x - (uint64_t)s);                          \
     case 0x92: return ISF ? x * s : (T)((uint64_t)x * (uint64_t)s);                          \
-    case 0x93: return (s == (T)0) ? (T)NAN : x / s;                                          \
+    case 0x93: return ISF ? ((s == (T)0) ? (T)NAN : x / s)                                    \
+                          : (s == (T)0 ? (T)0 : s == (T)-1 ? (T)(0 - (uint64_t)x) : (T)((int64_t)x / (int64_t)s)); \
+    case 0x94: return ISF ? (T)fmod((double)x, (double)s)                                    \
+                          : (s == (T)0 || s == (T)-1 ? (T)0 : (T)((int64_t)x % (int64_t)s));  \
     case 0x95: return ISF ? ((x < s || x != x) ? x : s) : (x < s ? x : s);                   \
     case 0x96: return ISF ? ((x > s || s != s) ? x : s) : (x > s ? x : s);                   \
     default: return (cmp != 7 && VO_PRED(cmp, x, s)) ? x : (T)0;                            \

@@ -278,7 +278,8 @@ def test_tier0_vector_arithmetic(gpu, is_f64):
     counts, min and max equal the oracle's bits (vo_tier0_expr_*, pinned
     against the reference engine), NaN results compared as NaN; f64 sums
     within 2 ulp of the exact sum of the reduced register's lanes; i64 sums
-    exact.  Integer vector division is refused."""
+    exact (division and modulo with the reference's zero-divisor results).
+    An INT64_MIN / -1 lane, a trap in the reference, is refused."""
     from tests import tier0_gen as tg
     from voxel_amd import voxel_py as vx
     rng = np.random.default_rng(71 if is_f64 else 72)
@@ -327,16 +328,19 @@ def test_tier0_vector_arithmetic(gpu, is_f64):
             else:
                 assert same(got, acc[b]), (trial, b, spec)
     if not is_f64:
+        # INT64_MIN / -1 traps in the reference's x86 build: refused, registers untouched
         e = vx.EngineI64()
         x = np.arange(1, 65, dtype=np.int64)
+        x[17] = np.iinfo(np.int64).min
         e.add_segment(x)
         e.set_scalar(2, 64)
-        e.set_scalar(3, 7)
+        e.set_scalar(3, (1 << 64) - 1)                       # R3 = -1
         e.load_program([tg.w(0x10, rd=4, imm=4), tg.w(0x70, rd=0, ra=1), tg.w(0x93, rd=1, ra=0, rb=3),
                         tg.w(0xD0, rd=5, ra=1), tg.w(0x20, rd=8, ra=8, rb=5), tg.w(0x20, rd=1, ra=1, rb=4),
                         tg.w(0x40, ra=1, rb=2), tg.w(0x52, imm=(1 - 8) & 0xFFF), tg.w(0x01)])
-        with pytest.raises(RuntimeError, match="integer vector division"):
+        with pytest.raises(RuntimeError, match="INT64_MIN"):
             e.run()
+        assert e.get_scalar(8) == 0 and e.get_scalar(1) == 0
 
 
 def test_tier0_window_word_loops(gpu):

@@ -146,16 +146,15 @@ def last_agg_writers(words):
     return out
 
 
-VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x87, 0x88]             # VADD VSUB VMUL VDIV VMIN VMAX
+VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x84, 0x87, 0x88]       # VADD VSUB VMUL VDIV VMOD VMIN VMAX
 VEC_UN = [0x85, 0x86]                                       # VNEG VABS
-VEC_SCAL = [0x90, 0x91, 0x92, 0x93, 0x95, 0x96]            # VSADD VSSUB VSMUL VSDIV VSMIN VSMAX
+VEC_SCAL = [0x90, 0x91, 0x92, 0x93, 0x94, 0x95, 0x96]      # VSADD VSSUB VSMUL VSDIV VSMOD VSMIN VSMAX
 SCAL_REGS = (3, 6, 13)                                      # scalar operands / thresholds
 
 
 def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
     """Loop with vector arithmetic and filters on V0 = VLOAD and up to three
-    reductions.  Returns (words, spec) for vo_tier0_expr_* (division only on
-    f64 engines)."""
+    reductions.  Returns (words, spec) for vo_tier0_expr_*."""
     code = [w(0x10, rd=4, imm=4)]
     vload = len(code)
     code.append(w(0x70, rd=0, ra=1, imm=0))
@@ -168,7 +167,7 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
         b = int(rng.choice(defined))
         sreg = int(rng.choice(SCAL_REGS))
         if kind < 0.3:
-            op = int(rng.choice([o for o in VEC_BIN if is_f64 or o != 0x83]))
+            op = int(rng.choice(VEC_BIN))
             code.append(w(op, rd=dst, ra=a, rb=b))
             ops.append((op, dst, a, b, 0, -1))
         elif kind < 0.4:
@@ -176,7 +175,7 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
             code.append(w(op, rd=dst, ra=a))
             ops.append((op, dst, a, 0, 0, -1))
         elif kind < 0.75:
-            op = int(rng.choice([o for o in VEC_SCAL if is_f64 or o != 0x93]))
+            op = int(rng.choice(VEC_SCAL))
             code.append(w(op, rd=dst, ra=a, rb=sreg))
             ops.append((op, dst, a, 0, 0, sreg))
         else:

@@ -62,6 +62,7 @@ struct Engine {
     SegaggBufs segagg;
     DevColumn expr;                  // a reduced vector of a loop with vector arithmetic
     DevColumn wcol;                  // window values of a loop loading a window word
+    DevColumn trap;                  // set by an INT64_MIN / -1 lane
 };
 
 size_t lanes_of(int dt) { return 32 / dtype_size(dt); }
@@ -227,11 +228,19 @@ bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count,
         const Plan::VOp& v = p.vops[k];
         prog.ops[k] = ExprOp{v.op, v.dst, v.a, v.b, v.cmp, v.sreg != 0xFF ? regs[v.sreg] : 0};
     }
+    if (!e.trap.ensure(sizeof(int))) return false;
+    if (!hip_ok(launch_zero32(static_cast<uint32_t*>(e.trap.dev), 1, c->stream), "clear")) return false;
     const PredDev all = make_pred(e.dtype, CMP_TRUE, 0);
     for (const Plan::Block& b : p.blocks) {
         prog.target = b.vreg;
-        if (!expr_eval(e.dtype, col, count, off, nrows, prog, e.expr.dev, c)) return false;
+        if (!expr_eval(e.dtype, col, count, off, nrows, prog, e.expr.dev, static_cast<int*>(e.trap.dev), c)) return false;
         if (!run_tier0_block(b, all, e.dtype, e.expr.dev, nrows, 0, nrows, nrows / kl, regs, c)) return false;
+    }
+    int trapped = 0;
+    if (!sync_read(c, &trapped, e.trap.dev, sizeof(int))) return false;
+    if (trapped) {
+        set_error("a vector division or modulo of INT64_MIN by -1 (the reference's x86 build traps there)");
+        return false;
     }
     regs[p.off] = total;
     if (run_scalar(p.epilogue, regs) < 0) return false;
@@ -241,12 +250,6 @@ bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count,
 
 bool run_tier0_expr(Engine& e, uint64_t* regs, uint64_t off, uint64_t total, StreamCtx* c) {
     const Plan& p = e.plan;
-    if (e.dtype == DT_I64)
-        for (const Plan::VOp& v : p.vops)
-            if (v.op == 0x83 || v.op == 0x93) {
-                set_error("integer vector division is not lowered");
-                return false;
-            }
     Seg& s = *e.segs[p.seg];
     const uint64_t dend = total < s.count ? total : s.count;
     const void* d = seg_rows(s, e.dtype, off < dend ? off : dend, dend, c);
@@ -849,9 +852,6 @@ int voxel_gpu_jit_lowerable(const uint32_t* code, size_t len) {
 
 int voxel_gpu_engine_lowered(const void* h) {
     const Engine& e = *static_cast<const Engine*>(h);
-    if (e.plan.kind == Plan::TIER0 && e.plan.has_arith && e.dtype == DT_I64)
-        for (const Plan::VOp& v : e.plan.vops)
-            if (v.op == 0x83 || v.op == 0x93) return 0;      // integer vector division is not lowered
     return e.plan.kind == Plan::FAST ? 1 : e.plan.kind == Plan::TIER0 ? 2 : e.plan.kind == Plan::STRAIGHT ? 3 : 0;
 }
 

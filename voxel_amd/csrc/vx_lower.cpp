@@ -176,8 +176,7 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     bool chain_ok = true;
     auto defined = [&](uint8_t v) { return chain_len[v] != -1; };
     auto vec_arith = [](uint8_t op) {          // lowered vector arithmetic (engine.hpp:2221-2350, 2473-2590)
-        return (op >= 0x80 && op <= 0x83) || op == 0x85 || op == 0x86 || op == 0x87 || op == 0x88 ||
-               (op >= 0x90 && op <= 0x93) || op == 0x95 || op == 0x96;
+        return (op >= 0x80 && op <= 0x88) || (op >= 0x90 && op <= 0x96);
     };
     while (i < len) {
         const uint32_t w = c[i];

@@ -101,9 +101,11 @@ __device__ __forceinline__ bool expr_pass(int cmp, T x, T t) {
 // of its operands only.
 template <typename T>
 __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, uint64_t count, uint64_t off,
-                                                   uint64_t n, ExprProg prog, T* __restrict__ out) {
+                                                   uint64_t n, ExprProg prog, T* __restrict__ out,
+                                                   int* __restrict__ trap) {
     constexpr bool F = sizeof(T) == 8 && (T)0.5 != (T)0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    bool trapped = false;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         T v[16];
 #pragma unroll
@@ -119,7 +121,20 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
             case 0x80: r = F ? x + y : (T)((uint64_t)x + (uint64_t)y); break;                   // VADD
             case 0x81: r = F ? x - y : (T)((uint64_t)x - (uint64_t)y); break;                   // VSUB
             case 0x82: r = F ? x * y : (T)((uint64_t)x * (uint64_t)y); break;                   // VMUL
-            case 0x83: r = y == T(0) ? (T)__builtin_nan("") : x / y; break;                      // VDIV (f64)
+            case 0x83:                                                                          // VDIV
+                if constexpr (F) r = y == T(0) ? (T)__builtin_nan("") : x / y;
+                else {
+                    trapped |= y == T(-1) && x == (T)INT64_MIN;                                   // x86 idiv traps
+                    r = y == T(0) ? (T)INT64_MAX : y == T(-1) ? (T)(0 - (uint64_t)x) : x / y;
+                }
+                break;
+            case 0x84:                                                                          // VMOD
+                if constexpr (F) r = __builtin_fmod(x, y);
+                else {
+                    trapped |= y == T(-1) && x == (T)INT64_MIN;
+                    r = (y == T(0) || y == T(-1)) ? T(0) : x % y;
+                }
+                break;
             case 0x85: r = F ? -x : (T)(0 - (uint64_t)x); break;                                 // VNEG
             case 0x86:                                                                          // VABS
                 if constexpr (F) r = __builtin_fabs(x);
@@ -130,7 +145,20 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
             case 0x90: r = F ? x + s : (T)((uint64_t)x + (uint64_t)s); break;                   // VSADD
             case 0x91: r = F ? x - s : (T)((uint64_t)x - (uint64_t)s); break;                   // VSSUB
             case 0x92: r = F ? x * s : (T)((uint64_t)x * (uint64_t)s); break;                   // VSMUL
-            case 0x93: r = s == T(0) ? (T)__builtin_nan("") : x / s; break;                      // VSDIV (f64)
+            case 0x93:                                                                          // VSDIV
+                if constexpr (F) r = s == T(0) ? (T)__builtin_nan("") : x / s;
+                else {
+                    trapped |= s == T(-1) && x == (T)INT64_MIN;
+                    r = s == T(0) ? T(0) : s == T(-1) ? (T)(0 - (uint64_t)x) : x / s;
+                }
+                break;
+            case 0x94:                                                                          // VSMOD
+                if constexpr (F) r = __builtin_fmod(x, s);
+                else {
+                    trapped |= s == T(-1) && x == (T)INT64_MIN;
+                    r = (s == T(0) || s == T(-1)) ? T(0) : x % s;
+                }
+                break;
             case 0x95: r = F ? ((x < s || x != x) ? x : s) : (x < s ? x : s); break;             // VSMIN
             case 0x96: r = F ? ((x > s || s != s) ? x : s) : (x > s ? x : s); break;             // VSMAX
             default: r = expr_pass<T>(o.cmp, x, s) ? x : T(0); break;                            // VFILTER_*
@@ -139,19 +167,20 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
         }
         out[i] = v[prog.target];
     }
+    if (trapped) *trap = 1;
 }
 
 }  // namespace
 
 bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,
-               StreamCtx* c) {
+               int* trap, StreamCtx* c) {
     if (n == 0) return true;
     if (dtype == DT_F64)
         hipLaunchKernelGGL(k_expr_eval<double>, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, (const double*)col, count,
-                           off, n, prog, (double*)out);
+                           off, n, prog, (double*)out, trap);
     else
         hipLaunchKernelGGL(k_expr_eval<int64_t>, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, (const int64_t*)col,
-                           count, off, n, prog, (int64_t*)out);
+                           count, off, n, prog, (int64_t*)out, trap);
     return hip_ok(hipGetLastError(), "vector expression launch");
 }
 

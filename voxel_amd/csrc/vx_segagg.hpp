@@ -51,8 +51,9 @@ struct ExprProg {
 
 // out[i] = register `prog.target` after the words, for rows off + i, i < n
 // (the loaded value is 0 for rows at or past count).  dtype DT_F64 / DT_I64.
+// *trap (device) is set to 1 when an i64 lane divides INT64_MIN by -1.
 bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,
-               StreamCtx* c);
+               int* trap, StreamCtx* c);
 
 // The window family's kernels on a device f64 column (vx_window.hip; the
 // caller holds the library lock): kind 0 delta, 1 sum, 2 mean (sum / w),
