@@ -376,9 +376,10 @@ VO_DEF_TIER0_MULTI(vo_tier0_multi_f64, double, vo_red_f64)
 VO_DEF_TIER0_MULTI(vo_tier0_multi_i64, int64_t, vo_red_i64)
 
 /* ------------------------------------------------------------------------ */
-/* Tier-0 loops with vector arithmetic (engine.hpp:2221-2350 VADD VSUB VMUL */
-/* VDIV VMOD VNEG VABS VMIN VMAX, 2473-2590 VSADD VSSUB VSMUL VSDIV VSMOD    */
-/* VSMIN VSMAX)                                                             */
+/* Tier-0 loops with vector arithmetic (engine.hpp:2221-2450 VADD VSUB VMUL */
+/* VDIV VMOD VNEG VABS VMIN VMAX VAVG VFMA VFMS VSQRT VRSQRT VRCP,          */
+/* 2473-2590 VSADD VSSUB VSMUL VSDIV VSMOD VSMIN VSMAX; VFMA/VFMS take the  */
+/* scalar register imm12 & 0xF as c)                                        */
 /* and filters, per kL = 4 group: VLOAD into register vload_dst (zero past  */
 /* Count), then ops in program order on 16 vector registers; op word        */
 /* k: code[5k..5k+4] = opcode, dst, a, b, filter kCmp; sval[k] = the scalar */
@@ -400,6 +401,15 @@ static T NAME(int op, int cmp, T x, T y, T s) {                                 
     case 0x84: return ISF ? (T)fmod((double)x, (double)y)                                    \
                           : (y == (T)0 || y == (T)-1 ? (T)0 : (T)((int64_t)x % (int64_t)y));  \
     case 0x85: return ISF ? -x : (T)(0 - (uint64_t)x);                                       \
+    case 0x89: return ISF ? (x + y) * (T)0.5                                                 \
+                          : (T)(((int64_t)x >> 1) + ((int64_t)y >> 1) + (((int64_t)x & 1) + ((int64_t)y & 1)) / 2); \
+    case 0x8A: return ISF ? (T)fma((double)x, (double)y, (double)s)                          \
+                          : (T)((uint64_t)x * (uint64_t)y + (uint64_t)s);                    \
+    case 0x8B: return ISF ? x * y - s : (T)((uint64_t)x * (uint64_t)y - (uint64_t)s);         \
+    case 0x8C: return ISF ? (T)sqrt((double)x) : ((double)x < 0 ? (T)0 : (T)sqrt((double)x)); \
+    case 0x8D: return ISF ? (T)((T)1 / (T)sqrt((double)x))                                   \
+                          : ((double)x <= 0 ? (T)0 : (T)(1.0 / sqrt((double)x)));            \
+    case 0x8E: return ISF ? (T)1 / x : (x == (T)0 ? (T)0 : (T)(1.0 / (double)x));            \
     case 0x86: return ISF ? (T)fabs((double)x) : (x < (T)0 ? (T)(0 - (uint64_t)x) : x);      \
     case 0x87: return ISF ? ((x < y || x != x) ? x : y) : (x < y ? x : y);                   \
     case 0x88: return ISF ? ((x > y || y != y) ? x : y) : (x > y ? x : y);                   \

@@ -142,7 +142,7 @@ def test_lowering_accepts_vector_arithmetic():
     assert lowered(loop([W(0x80, rd=2, ra=0, rb=7)] + red)) == 0                   # V7 undefined
     assert lowered(loop([mul, W(0x80, rd=1, ra=0, rb=0)] + red)) == 0             # V1 redefined
     assert lowered(loop([W(0x92, rd=2, ra=0, rb=5)] + red)) == 0                  # scalar operand written by the loop
-    assert lowered(loop([W(0x89, rd=2, ra=0, rb=0)] + red)) == 0                  # VAVG is not lowered
+    assert lowered(loop([W(0x8F, rd=2, ra=0, rb=0)] + red)) == 0                  # VPOW is not lowered
     for op in (0x83, 0x84, 0x93, 0x94):                                           # division and modulo
         div = loop([W(op, rd=2, ra=0, rb=3 if op >= 0x90 else 0)] + red)
         assert lowered(div) == 2 and lowered(div, i64=True) == 2

@@ -146,8 +146,9 @@ def last_agg_writers(words):
     return out
 
 
-VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x84, 0x87, 0x88]       # VADD VSUB VMUL VDIV VMOD VMIN VMAX
-VEC_UN = [0x85, 0x86]                                       # VNEG VABS
+VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x84, 0x87, 0x88, 0x89]  # VADD VSUB VMUL VDIV VMOD VMIN VMAX VAVG
+VEC_FMA = [0x8A, 0x8B]                                      # VFMA VFMS: a * b +/- R[imm12 & 0xF]
+VEC_UN = [0x85, 0x86, 0x8C, 0x8D, 0x8E]                     # VNEG VABS VSQRT VRSQRT VRCP
 VEC_SCAL = [0x90, 0x91, 0x92, 0x93, 0x94, 0x95, 0x96]      # VSADD VSSUB VSMUL VSDIV VSMOD VSMIN VSMAX
 SCAL_REGS = (3, 6, 13)                                      # scalar operands / thresholds
 
@@ -166,10 +167,14 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
         a = int(rng.choice(defined))
         b = int(rng.choice(defined))
         sreg = int(rng.choice(SCAL_REGS))
-        if kind < 0.3:
+        if kind < 0.25:
             op = int(rng.choice(VEC_BIN))
             code.append(w(op, rd=dst, ra=a, rb=b))
             ops.append((op, dst, a, b, 0, -1))
+        elif kind < 0.3:
+            op = int(rng.choice(VEC_FMA))
+            code.append(w(op, rd=dst, ra=a, rb=b, imm=sreg))
+            ops.append((op, dst, a, b, 0, sreg))
         elif kind < 0.4:
             op = int(rng.choice(VEC_UN))
             code.append(w(op, rd=dst, ra=a))

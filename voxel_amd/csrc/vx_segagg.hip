@@ -96,7 +96,7 @@ __device__ __forceinline__ bool expr_pass(int cmp, T x, T t) {
     }
 }
 
-// Row-wise evaluation of the body's vector words (engine.hpp:2221-2350,
+// Row-wise evaluation of the body's vector words (engine.hpp:2221-2450,
 // 2473-2590, 1090-1110): each lane of a vector word depends on the same lane
 // of its operands only.
 template <typename T>
@@ -136,6 +136,30 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
                 }
                 break;
             case 0x85: r = F ? -x : (T)(0 - (uint64_t)x); break;                                 // VNEG
+            case 0x89:                                                                          // VAVG
+                if constexpr (F) r = (x + y) * T(0.5);
+                else r = (x >> 1) + (y >> 1) + ((x & 1) + (y & 1)) / 2;
+                break;
+            case 0x8A:                                                                          // VFMA, c = s
+                if constexpr (F) r = __builtin_fma(x, y, s);
+                else r = (T)((uint64_t)x * (uint64_t)y + (uint64_t)s);
+                break;
+            case 0x8B:                                                                          // VFMS
+                if constexpr (F) r = x * y - s;
+                else r = (T)((uint64_t)x * (uint64_t)y - (uint64_t)s);
+                break;
+            case 0x8C:                                                                          // VSQRT
+                if constexpr (F) r = __builtin_sqrt(x);
+                else r = (double)x < 0 ? T(0) : (T)__builtin_sqrt((double)x);
+                break;
+            case 0x8D:                                                                          // VRSQRT
+                if constexpr (F) r = T(1) / __builtin_sqrt(x);
+                else r = (double)x <= 0 ? T(0) : (T)(1.0 / __builtin_sqrt((double)x));
+                break;
+            case 0x8E:                                                                          // VRCP
+                if constexpr (F) r = T(1) / x;
+                else r = x == T(0) ? T(0) : (T)(1.0 / (double)x);
+                break;
             case 0x86:                                                                          // VABS
                 if constexpr (F) r = __builtin_fabs(x);
                 else r = x < T(0) ? (T)(0 - (uint64_t)x) : x;
