@@ -379,7 +379,9 @@ VO_DEF_TIER0_MULTI(vo_tier0_multi_i64, int64_t, vo_red_i64)
 /* Tier-0 loops with vector arithmetic (engine.hpp:2221-2450 VADD VSUB VMUL */
 /* VDIV VMOD VNEG VABS VMIN VMAX VAVG VFMA VFMS VSQRT VRSQRT VRCP,          */
 /* 2473-2590 VSADD VSSUB VSMUL VSDIV VSMOD VSMIN VSMAX; VFMA/VFMS take the  */
-/* scalar register imm12 & 0xF as c)                                        */
+/* scalar register imm12 & 0xF as c; 2660-2866 VCMPxx / VCMP(NOT)NULL       */
+/* (all-ones lanes), VAND VOR VXOR VNOT VANDN on lane bits, VSHL VSHR VSAR  */
+/* (f64: ldexp by the lane's (int) value))                                  */
 /* and filters, per kL = 4 group: VLOAD into register vload_dst (zero past  */
 /* Count), then ops in program order on 16 vector registers; op word        */
 /* k: code[5k..5k+4] = opcode, dst, a, b, filter kCmp; sval[k] = the scalar */
@@ -390,7 +392,18 @@ VO_DEF_TIER0_MULTI(vo_tier0_multi_i64, int64_t, vo_red_i64)
 /* VMOD / VSMOD by zero 0 (engine.hpp:2263-2291, 2516-2549); x / -1 is the   */
 /* wrapping negation here (the reference's x86 build traps on INT64_MIN/-1). */
 /* ------------------------------------------------------------------------ */
-#define VO_DEF_EXPR_LANE(NAME, T, ISF)                                                      \
+/* lane bits of a vector register and x86's cvttsd2si (NaN and out-of-range
+   give INT_MIN), as the reference's build computes (int) of a double */
+static uint64_t vo_lb_f64(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static double vo_bl_f64(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+static uint64_t vo_lb_i64(int64_t x) { return (uint64_t)x; }
+static int64_t vo_bl_i64(uint64_t u) { return (int64_t)u; }
+static int vo_cvtt(double b) {
+    return (b != b || b >= 2147483648.0 || b <= -2147483649.0) ? INT32_MIN : (int)b;
+}
+static int vo_neg32(int k) { return k == INT32_MIN ? INT32_MIN : -k; }
+
+#define VO_DEF_EXPR_LANE(NAME, T, ISF, LB, BL)                                              \
 static T NAME(int op, int cmp, T x, T y, T s) {                                              \
     switch (op) {                                                                            \
     case 0x80: return ISF ? x + y : (T)((uint64_t)x + (uint64_t)y);                          \
@@ -410,6 +423,21 @@ static T NAME(int op, int cmp, T x, T y, T s) {                                 
     case 0x8D: return ISF ? (T)((T)1 / (T)sqrt((double)x))                                   \
                           : ((double)x <= 0 ? (T)0 : (T)(1.0 / sqrt((double)x)));            \
     case 0x8E: return ISF ? (T)1 / x : (x == (T)0 ? (T)0 : (T)(1.0 / (double)x));            \
+    case 0xA0: case 0xA1: case 0xA2: case 0xA3: case 0xA4: case 0xA5:                        \
+        return VO_PRED(op - 0xA0, x, y) ? BL(~0ull) : (T)0;       /* VCMPxx: all-ones lane */ \
+    case 0xA6: return (ISF ? x != x : x == (T)0) ? BL(~0ull) : (T)0;   /* VCMPNULL */        \
+    case 0xA7: return (ISF ? x == x : x != (T)0) ? BL(~0ull) : (T)0;   /* VCMPNOTNULL */     \
+    case 0xB0: return BL(LB(x) & LB(y));                                                     \
+    case 0xB1: return BL(LB(x) | LB(y));                                                     \
+    case 0xB2: return BL(LB(x) ^ LB(y));                                                     \
+    case 0xB3: return BL(~LB(x));                                                            \
+    case 0xB4: return BL(~LB(x) & LB(y));                                                    \
+    case 0xB5: return ISF ? (T)ldexp((double)x, vo_cvtt((double)y))                          \
+                          : (T)(LB(x) << (LB(y) & 63));                                      \
+    case 0xB6: return ISF ? (T)ldexp((double)x, vo_neg32(vo_cvtt((double)y)))                \
+                          : (T)(LB(x) >> (LB(y) & 63));                                      \
+    case 0xB7: return ISF ? (T)ldexp((double)x, vo_neg32(vo_cvtt((double)y)))                \
+                          : (T)((int64_t)LB(x) >> (LB(y) & 63));                             \
     case 0x86: return ISF ? (T)fabs((double)x) : (x < (T)0 ? (T)(0 - (uint64_t)x) : x);      \
     case 0x87: return ISF ? ((x < y || x != x) ? x : y) : (x < y ? x : y);                   \
     case 0x88: return ISF ? ((x > y || y != y) ? x : y) : (x > y ? x : y);                   \
@@ -425,8 +453,8 @@ static T NAME(int op, int cmp, T x, T y, T s) {                                 
     default: return (cmp != 7 && VO_PRED(cmp, x, s)) ? x : (T)0;                            \
     }                                                                                        \
 }
-VO_DEF_EXPR_LANE(vo_expr_lane_f64, double, 1)
-VO_DEF_EXPR_LANE(vo_expr_lane_i64, int64_t, 0)
+VO_DEF_EXPR_LANE(vo_expr_lane_f64, double, 1, vo_lb_f64, vo_bl_f64)
+VO_DEF_EXPR_LANE(vo_expr_lane_i64, int64_t, 0, vo_lb_i64, vo_bl_i64)
 
 /* one kL = 4 group of the body's vector words on the 16 vector registers */
 #define VO_DEF_EXPR_GROUP(NAME, T, LANE)                                                     \

@@ -273,8 +273,9 @@ def test_tier0_filter_chains_and_several_reductions(gpu, is_f64):
 
 @pytest.mark.parametrize("is_f64", [True, False])
 def test_tier0_vector_arithmetic(gpu, is_f64):
-    """Loops with vector arithmetic (VADD..VMAX, VSADD..VSMAX) and filters on
-    computed vectors, up to three reductions (generated words): partials,
+    """Loops with vector words (VADD..VRCP, VSADD..VSMAX, VCMPxx, VCMPNULL,
+    VCMPNOTNULL, VAND..VANDN, VSHL/VSHR/VSAR) and filters on computed
+    vectors, up to three reductions (generated words): partials,
     counts, min and max equal the oracle's bits (vo_tier0_expr_*, pinned
     against the reference engine), NaN results compared as NaN; f64 sums
     within 2 ulp of the exact sum of the reduced register's lanes; i64 sums
@@ -289,7 +290,9 @@ def test_tier0_vector_arithmetic(gpu, is_f64):
             return math.isnan(o.bits_f64(got))
         return got == want
 
-    for trial in range(30):
+    forced = tg.VEC_BIN + tg.VEC_UN                 # each binary and unary word ends two programs
+    for trial in range(30 + 2 * len(forced)):
+        last = forced[(trial - 30) // 2] if trial >= 30 else None
         n = int(rng.integers(1, 3000)) * 4
         regs = np.zeros(64, np.uint64)
         if is_f64:
@@ -307,11 +310,15 @@ def test_tier0_vector_arithmetic(gpu, is_f64):
                 regs[r] = int(rng.integers(-2**20, 2**20)) & (2**64 - 1)
             e = vx.EngineI64()
         regs[2] = n
-        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 9)))
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 9)), last=last)
         e.add_segment(x)
         for r in (2,) + tg.SCAL_REGS:
             e.set_scalar(r, int(regs[r]))
         e.load_program([int(c) for c in code])
+        if not is_f64 and tg.expr_traps(o, x, n, spec, regs):
+            with pytest.raises(RuntimeError, match="INT64_MIN"):
+                e.run()
+            continue
         e.run()
         acc, part = tg.oracle_expr(o, x, n, spec, regs, is_f64)
         assert e.get_scalar(1) == n

@@ -146,16 +146,20 @@ def last_agg_writers(words):
     return out
 
 
-VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x84, 0x87, 0x88, 0x89]  # VADD VSUB VMUL VDIV VMOD VMIN VMAX VAVG
+VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x84, 0x87, 0x88, 0x89,  # VADD VSUB VMUL VDIV VMOD VMIN VMAX VAVG
+           0xA0, 0xA1, 0xA2, 0xA3, 0xA4, 0xA5,              # VCMPEQ .. VCMPGE
+           0xB0, 0xB1, 0xB2, 0xB4, 0xB5, 0xB6, 0xB7]        # VAND VOR VXOR VANDN VSHL VSHR VSAR
 VEC_FMA = [0x8A, 0x8B]                                      # VFMA VFMS: a * b +/- R[imm12 & 0xF]
-VEC_UN = [0x85, 0x86, 0x8C, 0x8D, 0x8E]                     # VNEG VABS VSQRT VRSQRT VRCP
+VEC_UN = [0x85, 0x86, 0x8C, 0x8D, 0x8E, 0xA6, 0xA7, 0xB3]   # VNEG VABS VSQRT VRSQRT VRCP VCMPNULL VCMPNOTNULL VNOT
 VEC_SCAL = [0x90, 0x91, 0x92, 0x93, 0x94, 0x95, 0x96]      # VSADD VSSUB VSMUL VSDIV VSMOD VSMIN VSMAX
 SCAL_REGS = (3, 6, 13)                                      # scalar operands / thresholds
 
 
-def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
+def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3), last=None):
     """Loop with vector arithmetic and filters on V0 = VLOAD and up to three
-    reductions.  Returns (words, spec) for vo_tier0_expr_*."""
+    reductions.  Returns (words, spec) for vo_tier0_expr_*.  `last`, when
+    given, is the opcode of the final vector word, whose register the first
+    reduction reads."""
     code = [w(0x10, rd=4, imm=4)]
     vload = len(code)
     code.append(w(0x70, rd=0, ra=1, imm=0))
@@ -167,8 +171,10 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
         a = int(rng.choice(defined))
         b = int(rng.choice(defined))
         sreg = int(rng.choice(SCAL_REGS))
+        if last is not None and k == nops - 1:
+            kind = 0.35 if last in VEC_UN else 0.0
         if kind < 0.25:
-            op = int(rng.choice(VEC_BIN))
+            op = int(rng.choice(VEC_BIN)) if last is None or k < nops - 1 else last
             code.append(w(op, rd=dst, ra=a, rb=b))
             ops.append((op, dst, a, b, 0, -1))
         elif kind < 0.3:
@@ -176,7 +182,7 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
             code.append(w(op, rd=dst, ra=a, rb=b, imm=sreg))
             ops.append((op, dst, a, b, 0, sreg))
         elif kind < 0.4:
-            op = int(rng.choice(VEC_UN))
+            op = int(rng.choice(VEC_UN)) if last is None or k < nops - 1 else last
             code.append(w(op, rd=dst, ra=a))
             ops.append((op, dst, a, 0, 0, -1))
         elif kind < 0.75:
@@ -192,6 +198,8 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3)):
     bvreg, reds_used, accops = [], [], []
     for b in range(nb):
         vreg = int(rng.choice(defined[1:] if len(defined) > 1 else defined))
+        if last is not None and b == 0:
+            vreg = defined[-1]
         red = int(rng.choice(reds))
         code.append(w(RED_OPS[red], rd=PARTS[b], ra=vreg))
         if red == 0:
@@ -235,6 +243,23 @@ def oracle_expr_col(o, x, total, spec, regs, target, is_f64=True):
     fn = o.O.vo_tier0_expr_col_f64 if is_f64 else o.O.vo_tier0_expr_col_i64
     fn(o.ptr(x), x.size, 0, total, 0, len(spec["ops"]), o.ptr(code), o.ptr(sval), target, o.ptr(out))
     return out
+
+
+def expr_traps(o, x, total, spec, regs):
+    """True when an i64 VDIV/VMOD/VSDIV/VSMOD lane divides INT64_MIN by -1
+    (a trap in the reference's x86 build; the product refuses such a run)."""
+    lo = np.iinfo(np.int64).min
+    for op, dst, a, b, cmp, sreg in spec["ops"]:
+        if op not in (0x83, 0x84, 0x93, 0x94):
+            continue
+        xa = oracle_expr_col(o, x, total, spec, regs, a, False)
+        if op in (0x83, 0x84):
+            xb = oracle_expr_col(o, x, total, spec, regs, b, False)
+        else:
+            xb = np.full(total, np.uint64(regs[sreg]).view(np.int64))
+        if np.any((xa == lo) & (xb == -1)):
+            return True
+    return False
 
 
 def oracle_expr(o, x, total, spec, regs, is_f64=True, acc0=None):

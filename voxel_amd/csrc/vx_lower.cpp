@@ -175,8 +175,9 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     p.vload_dst = vA;
     bool chain_ok = true;
     auto defined = [&](uint8_t v) { return chain_len[v] != -1; };
-    auto vec_arith = [](uint8_t op) {          // lowered vector arithmetic (engine.hpp:2221-2450, 2473-2590)
-        return (op >= 0x80 && op <= 0x8E) || (op >= 0x90 && op <= 0x96);
+    auto vec_arith = [](uint8_t op) {          // lowered vector words (engine.hpp:2221-2450, 2473-2590, 2671-2865)
+        return (op >= 0x80 && op <= 0x8E) || (op >= 0x90 && op <= 0x96) || (op >= 0xA0 && op <= 0xA7) ||
+               (op >= 0xB0 && op <= 0xB7);
     };
     while (i < len) {
         const uint32_t w = c[i];
@@ -209,7 +210,9 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
         }
         if (vec_arith(op)) {
             const uint8_t dst = rd_of(w), a = ra_of(w), b = rb_of(w);
-            const bool scal = op >= 0x90, unary = op == 0x85 || op == 0x86 || (op >= 0x8C && op <= 0x8E);
+            const bool scal = op >= 0x90 && op <= 0x96;
+            const bool unary = op == 0x85 || op == 0x86 || (op >= 0x8C && op <= 0x8E) || op == 0xA6 || op == 0xA7 ||
+                               op == 0xB3;
             const bool fma = op == 0x8A || op == 0x8B;         // c = R[imm12 & 0xF]
             if (!defined(a) || (!scal && !unary && !defined(b))) return none("vector arithmetic reads a vector the loop does not define");
             if (defined(dst)) return none("vector arithmetic overwrites a vector the loop already defines");

@@ -96,9 +96,43 @@ __device__ __forceinline__ bool expr_pass(int cmp, T x, T t) {
     }
 }
 
+// A lane's 64 bits, and a lane from 64 bits (the logical words act on the
+// bit pattern of either lane type).
+template <typename T>
+__device__ __forceinline__ uint64_t bits_of(T x) {
+    uint64_t u;
+    __builtin_memcpy(&u, &x, 8);
+    return u;
+}
+template <typename T>
+__device__ __forceinline__ T lane_of(uint64_t u) {
+    T x;
+    __builtin_memcpy(&x, &u, 8);
+    return x;
+}
+// (int) of a double as the reference's x86-64 build converts it (cvttsd2si):
+// NaN and values outside int32's range give INT32_MIN; the f64 shift words
+// scale by 2^(int)b and 2^-(int)b, and negating INT32_MIN keeps INT32_MIN.
+__device__ __forceinline__ int cvtt_i32(double b) {
+    return (b != b || b >= 2147483648.0 || b <= -2147483649.0) ? INT32_MIN : (int)b;
+}
+__device__ __forceinline__ int neg_i32(int k) { return k == INT32_MIN ? INT32_MIN : -k; }
+
+// The NaN an f64 operation of the reference's x86-64 build leaves in a lane:
+// the first NaN operand, quieted, or the default NaN (sign bit set) when the
+// operation itself is invalid (inf - inf, 0 * inf, fmod(x, 0), sqrt(-1)).
+// The GPU's default NaN has the sign bit clear, and the lane's bits reach
+// the CMP-select accumulators, which order registers as signed integers.
+__device__ __forceinline__ double x86_nan(double r, double x, double y, double z) {
+    if (r == r) return r;
+    const uint64_t b = x != x ? bits_of(x) : y != y ? bits_of(y) : z != z ? bits_of(z) : 0xFFF8000000000000ull;
+    return lane_of<double>(b | 0x0008000000000000ull);
+}
+__device__ __forceinline__ double x86_nan(double r, double x, double y) { return x86_nan(r, x, y, y); }
+
 // Row-wise evaluation of the body's vector words (engine.hpp:2221-2450,
-// 2473-2590, 1090-1110): each lane of a vector word depends on the same lane
-// of its operands only.
+// 2473-2590, 2671-2865, 1090-1110): each lane of a vector word depends on the
+// same lane of its operands only.
 template <typename T>
 __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, uint64_t count, uint64_t off,
                                                    uint64_t n, ExprProg prog, T* __restrict__ out,
@@ -118,66 +152,84 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
             __builtin_memcpy(&s, &o.s, sizeof(T));
             T r;
             switch (o.op) {
-            case 0x80: r = F ? x + y : (T)((uint64_t)x + (uint64_t)y); break;                   // VADD
-            case 0x81: r = F ? x - y : (T)((uint64_t)x - (uint64_t)y); break;                   // VSUB
-            case 0x82: r = F ? x * y : (T)((uint64_t)x * (uint64_t)y); break;                   // VMUL
+            case 0x80:                                                                          // VADD
+                if constexpr (F) r = x86_nan(x + y, x, y);
+                else r = (T)((uint64_t)x + (uint64_t)y);
+                break;
+            case 0x81:                                                                          // VSUB
+                if constexpr (F) r = x86_nan(x - y, x, y);
+                else r = (T)((uint64_t)x - (uint64_t)y);
+                break;
+            case 0x82:                                                                          // VMUL
+                if constexpr (F) r = x86_nan(x * y, x, y);
+                else r = (T)((uint64_t)x * (uint64_t)y);
+                break;
             case 0x83:                                                                          // VDIV
-                if constexpr (F) r = y == T(0) ? (T)__builtin_nan("") : x / y;
+                if constexpr (F) r = y == T(0) ? (T)__builtin_nan("") : x86_nan(x / y, x, y);
                 else {
                     trapped |= y == T(-1) && x == (T)INT64_MIN;                                   // x86 idiv traps
                     r = y == T(0) ? (T)INT64_MAX : y == T(-1) ? (T)(0 - (uint64_t)x) : x / y;
                 }
                 break;
             case 0x84:                                                                          // VMOD
-                if constexpr (F) r = __builtin_fmod(x, y);
+                if constexpr (F) r = x86_nan(__builtin_fmod(x, y), x, y);
                 else {
                     trapped |= y == T(-1) && x == (T)INT64_MIN;
                     r = (y == T(0) || y == T(-1)) ? T(0) : x % y;
                 }
                 break;
-            case 0x85: r = F ? -x : (T)(0 - (uint64_t)x); break;                                 // VNEG
+            case 0x85: r = lane_of<T>(F ? bits_of(x) ^ (1ull << 63) : 0 - bits_of(x)); break;   // VNEG
             case 0x89:                                                                          // VAVG
-                if constexpr (F) r = (x + y) * T(0.5);
+                if constexpr (F) r = x86_nan((x + y) * T(0.5), x, y);
                 else r = (x >> 1) + (y >> 1) + ((x & 1) + (y & 1)) / 2;
                 break;
             case 0x8A:                                                                          // VFMA, c = s
-                if constexpr (F) r = __builtin_fma(x, y, s);
+                if constexpr (F) r = x86_nan(__builtin_fma(x, y, s), x, y, s);
                 else r = (T)((uint64_t)x * (uint64_t)y + (uint64_t)s);
                 break;
             case 0x8B:                                                                          // VFMS
-                if constexpr (F) r = x * y - s;
+                if constexpr (F) r = x86_nan(x86_nan(x * y, x, y) - s, x86_nan(x * y, x, y), s);
                 else r = (T)((uint64_t)x * (uint64_t)y - (uint64_t)s);
                 break;
             case 0x8C:                                                                          // VSQRT
-                if constexpr (F) r = __builtin_sqrt(x);
+                if constexpr (F) r = x86_nan(__builtin_sqrt(x), x, x);
                 else r = (double)x < 0 ? T(0) : (T)__builtin_sqrt((double)x);
                 break;
             case 0x8D:                                                                          // VRSQRT
-                if constexpr (F) r = T(1) / __builtin_sqrt(x);
+                if constexpr (F) r = x86_nan(T(1) / __builtin_sqrt(x), x, x);
                 else r = (double)x <= 0 ? T(0) : (T)(1.0 / __builtin_sqrt((double)x));
                 break;
             case 0x8E:                                                                          // VRCP
-                if constexpr (F) r = T(1) / x;
+                if constexpr (F) r = x86_nan(T(1) / x, x, x);
                 else r = x == T(0) ? T(0) : (T)(1.0 / (double)x);
                 break;
             case 0x86:                                                                          // VABS
-                if constexpr (F) r = __builtin_fabs(x);
+                if constexpr (F) r = lane_of<T>(bits_of(x) & ~(1ull << 63));
                 else r = x < T(0) ? (T)(0 - (uint64_t)x) : x;
                 break;
             case 0x87: r = F ? ((x < y || x != x) ? x : y) : (x < y ? x : y); break;             // VMIN
             case 0x88: r = F ? ((x > y || y != y) ? x : y) : (x > y ? x : y); break;             // VMAX
-            case 0x90: r = F ? x + s : (T)((uint64_t)x + (uint64_t)s); break;                   // VSADD
-            case 0x91: r = F ? x - s : (T)((uint64_t)x - (uint64_t)s); break;                   // VSSUB
-            case 0x92: r = F ? x * s : (T)((uint64_t)x * (uint64_t)s); break;                   // VSMUL
+            case 0x90:                                                                          // VSADD
+                if constexpr (F) r = x86_nan(x + s, x, s);
+                else r = (T)((uint64_t)x + (uint64_t)s);
+                break;
+            case 0x91:                                                                          // VSSUB
+                if constexpr (F) r = x86_nan(x - s, x, s);
+                else r = (T)((uint64_t)x - (uint64_t)s);
+                break;
+            case 0x92:                                                                          // VSMUL
+                if constexpr (F) r = x86_nan(x * s, x, s);
+                else r = (T)((uint64_t)x * (uint64_t)s);
+                break;
             case 0x93:                                                                          // VSDIV
-                if constexpr (F) r = s == T(0) ? (T)__builtin_nan("") : x / s;
+                if constexpr (F) r = s == T(0) ? (T)__builtin_nan("") : x86_nan(x / s, x, s);
                 else {
                     trapped |= s == T(-1) && x == (T)INT64_MIN;
                     r = s == T(0) ? T(0) : s == T(-1) ? (T)(0 - (uint64_t)x) : x / s;
                 }
                 break;
             case 0x94:                                                                          // VSMOD
-                if constexpr (F) r = __builtin_fmod(x, s);
+                if constexpr (F) r = x86_nan(__builtin_fmod(x, s), x, s);
                 else {
                     trapped |= s == T(-1) && x == (T)INT64_MIN;
                     r = (s == T(0) || s == T(-1)) ? T(0) : x % s;
@@ -185,6 +237,25 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
                 break;
             case 0x95: r = F ? ((x < s || x != x) ? x : s) : (x < s ? x : s); break;             // VSMIN
             case 0x96: r = F ? ((x > s || s != s) ? x : s) : (x > s ? x : s); break;             // VSMAX
+            case 0xA0: case 0xA1: case 0xA2: case 0xA3: case 0xA4: case 0xA5:                     // VCMPxx
+                r = expr_pass<T>(o.op - 0xA0, x, y) ? lane_of<T>(~0ull) : T(0);
+                break;
+            case 0xA6: r = (F ? x != x : x == T(0)) ? lane_of<T>(~0ull) : T(0); break;            // VCMPNULL
+            case 0xA7: r = (F ? x == x : x != T(0)) ? lane_of<T>(~0ull) : T(0); break;            // VCMPNOTNULL
+            case 0xB0: r = lane_of<T>(bits_of(x) & bits_of(y)); break;                            // VAND
+            case 0xB1: r = lane_of<T>(bits_of(x) | bits_of(y)); break;                            // VOR
+            case 0xB2: r = lane_of<T>(bits_of(x) ^ bits_of(y)); break;                            // VXOR
+            case 0xB3: r = lane_of<T>(~bits_of(x)); break;                                        // VNOT
+            case 0xB4: r = lane_of<T>(~bits_of(x) & bits_of(y)); break;                           // VANDN
+            case 0xB5:                                                                          // VSHL
+                if constexpr (F) r = x86_nan(__builtin_ldexp(x, cvtt_i32(y)), x, x);
+                else r = (T)(bits_of(x) << (bits_of(y) & 63));
+                break;
+            case 0xB6:                                                                          // VSHR
+            case 0xB7:                                                                          // VSAR
+                if constexpr (F) r = x86_nan(__builtin_ldexp(x, neg_i32(cvtt_i32(y))), x, x);
+                else r = o.op == 0xB6 ? (T)(bits_of(x) >> (bits_of(y) & 63)) : (T)(x >> (bits_of(y) & 63));
+                break;
             default: r = expr_pass<T>(o.cmp, x, s) ? x : T(0); break;                            // VFILTER_*
             }
             v[o.dst] = r;
