@@ -427,7 +427,8 @@ def bench_sql(rows_log2: int = 28) -> dict:
     out = {}
     for name, sql, nbytes in (
             ("two_column_fused", "SELECT SUM(a), COUNT(*) FROM t WHERE a > 292.8932188 AND b < 707.1067812", n * 16),
-            ("three_column_bitmaps", "SELECT SUM(a) FROM t WHERE a > 292.89 AND b < 707.1 AND c >= 500", n * 24)):
+            ("three_column_fused", "SELECT SUM(a) FROM t WHERE a > 292.89 AND b < 707.1 AND c >= 500", n * 24),
+            ("three_column_or", "SELECT SUM(a) FROM t WHERE a > 292.89 AND (b < 707.1 OR c >= 500)", n * 24)):
         res = [None]
         t = _sync_time(lambda: res.__setitem__(0, vsql.execute(sql, data)), reps=5)
         out[name] = {"ms": round(t * 1e3, 4), "gb_per_s": round(nbytes / t / 1e9, 1), "sql": sql,

@@ -55,6 +55,10 @@ def test_lowered_plan_over_named_columns(gpu):
         ("fare < 5 OR dist > 20", (fare < 5) | (dist > 20)),
         ("NOT (fare >= 8 AND zone <> 7)", ~((fare >= 8) & (zone != 7))),
         ("30 < fare", fare > 30),
+        ("fare > 10 AND (dist < 5 OR zone = 7)", (fare > 10) & ((dist < 5) | (zone == 7))),   # fused + OR group words
+        ("dist < 3 AND zone > 100 AND fare < 40 AND fare > 5",
+         (dist < 3) & (zone > 100) & (fare < 40) & (fare > 5)),                  # fused + two comparisons' words
+        ("NOT (zone >= 100) AND dist BETWEEN 1 AND 9", (zone < 100) & (dist >= 1) & (dist <= 9)),
     ]
     for where, sel in cases:
         v = fare[sel]

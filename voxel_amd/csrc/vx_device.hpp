@@ -24,6 +24,17 @@ template <> __device__ __forceinline__ int64_t from_bits<int64_t>(uint64_t b) { 
 template <> __device__ __forceinline__ int32_t from_bits<int32_t>(uint64_t b) { return (int32_t)(uint32_t)b; }
 template <> __device__ __forceinline__ uint64_t from_bits<uint64_t>(uint64_t b) { return b; }
 
+// Sum of v over a 256-thread workgroup, returned to every thread (one
+// atomic per workgroup instead of per wave keeps a persistent grid's
+// closing atomics on one address from queueing at the L2).
+__device__ __forceinline__ uint64_t block_sum_256(uint64_t v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    __shared__ uint64_t part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return part[0] + part[1] + part[2] + part[3];
+}
+
 __device__ __forceinline__ uint64_t to_bits(double v) { return (uint64_t)__double_as_longlong(v); }
 __device__ __forceinline__ uint64_t to_bits(float v) { return (uint64_t)__float_as_uint(v); }
 __device__ __forceinline__ uint64_t to_bits(int64_t v) { return (uint64_t)v; }

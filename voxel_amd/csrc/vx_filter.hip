@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/voxel_c.h"
+#include "vx_acc.hpp"
 #include "vx_compact.hpp"
 #include "vx_device.hpp"
 #include "vx_kernels.hpp"
@@ -43,12 +44,18 @@ __device__ __forceinline__ uint64_t spread4(uint64_t x) {
     return x;
 }
 
+// Four 16-byte loads per column in flight per lane, eight workgroups of four
+// waves per CU (measured on 2^28 f64 rows: 0.39 ms; eight loads with two
+// workgroups per CU 0.41 ms; one load per step with sixteen 0.44 ms).
+template <bool TWO> constexpr int kBitmapUnroll = 4;
+
 template <typename T, bool TWO>
 __global__ __launch_bounds__(256) void k_bitmap(const T* __restrict__ a, const T* __restrict__ b, uint64_t n,
                                                 PredDev pda, PredDev pdb, uint64_t* __restrict__ words,
                                                 unsigned long long* __restrict__ count) {
     using V = typename TyInfo<T>::V;
     constexpr int E = TyInfo<T>::E;
+    constexpr int U = kBitmapUnroll<TWO>;          // chunks per wave step, their loads issued together
     Pred<T, false> pa; pa.load(pda);
     Pred<T, false> pb; pb.load(pdb);
     const int lane = threadIdx.x & 63;
@@ -57,52 +64,65 @@ __global__ __launch_bounds__(256) void k_bitmap(const T* __restrict__ a, const T
     const uint64_t rows_per = 64ull * E;
     const uint64_t nw = (n + 63) / 64;
     uint32_t cnt = 0;
-    for (uint64_t blk = wave; blk * rows_per < n; blk += nwaves) {
-        const uint64_t r0 = blk * rows_per + (uint64_t)lane * E;
-        T xa[E], xb[E];
-        if (r0 + E <= n) {
-            V va = *reinterpret_cast<const V*>(a + r0);
-            const T* pva = reinterpret_cast<const T*>(&va);
+    for (uint64_t g = wave; g * rows_per * U < n; g += nwaves) {
+        const uint64_t base = g * rows_per * U;
+        T xa[U][E], xb[U][E];
+        if (base + rows_per * U <= n) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) xa[e] = pva[e];
-            if constexpr (TWO) {
-                V vb = *reinterpret_cast<const V*>(b + r0);
-                const T* pvb = reinterpret_cast<const T*>(&vb);
+            for (int u = 0; u < U; ++u) {
+                const uint64_t r0 = base + u * rows_per + (uint64_t)lane * E;
+                V va = ld16<V, true>(reinterpret_cast<const V*>(a + r0));
+                const T* pva = reinterpret_cast<const T*>(&va);
 #pragma unroll
-                for (int e = 0; e < E; ++e) xb[e] = pvb[e];
+                for (int e = 0; e < E; ++e) xa[u][e] = pva[e];
+                if constexpr (TWO) {
+                    V vb = ld16<V, true>(reinterpret_cast<const V*>(b + r0));
+                    const T* pvb = reinterpret_cast<const T*>(&vb);
+#pragma unroll
+                    for (int e = 0; e < E; ++e) xb[u][e] = pvb[e];
+                }
             }
         } else {
 #pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t r0 = base + u * rows_per + (uint64_t)lane * E;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    xa[u][e] = r0 + e < n ? a[r0 + e] : T(0);
+                    if constexpr (TWO) xb[u][e] = r0 + e < n ? b[r0 + e] : T(0);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t blk = g * U + u;
+            const uint64_t r0 = blk * rows_per + (uint64_t)lane * E;
+            uint64_t bal[E];
+#pragma unroll
             for (int e = 0; e < E; ++e) {
-                xa[e] = r0 + e < n ? a[r0 + e] : T(0);
-                if constexpr (TWO) xb[e] = r0 + e < n ? b[r0 + e] : T(0);
+                bool p = (r0 + e < n) && pa(xa[u][e]);
+                if constexpr (TWO) p = p && pb(xb[u][e]);
+                cnt += p ? 1u : 0u;
+                bal[e] = __ballot(p);
             }
-        }
-        uint64_t bal[E];
+            // word k covers rows blk*rows_per + 64k .. +63: lanes (64/E)k .. (64/E)(k+1)-1
+            if (lane < E) {
+                const int k = lane;
+                uint64_t wd = 0;
+                if constexpr (E == 2) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            bool p = (r0 + e < n) && pa(xa[e]);
-            if constexpr (TWO) p = p && pb(xb[e]);
-            cnt += p ? 1u : 0u;
-            bal[e] = __ballot(p);
-        }
-        // word k covers rows blk*rows_per + 64k .. +63: lanes (64/E)k .. (64/E)(k+1)-1
-        if (lane < E) {
-            const int k = lane;
-            uint64_t wd = 0;
-            if constexpr (E == 2) {
+                    for (int e = 0; e < 2; ++e) wd |= spread2(bal[e] >> (32 * k)) << e;
+                } else {
 #pragma unroll
-                for (int e = 0; e < 2; ++e) wd |= spread2(bal[e] >> (32 * k)) << e;
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) wd |= spread4(bal[e] >> (16 * k)) << e;
+                    for (int e = 0; e < 4; ++e) wd |= spread4(bal[e] >> (16 * k)) << e;
+                }
+                const uint64_t wi = blk * E + k;
+                if (wi < nw) words[wi] = wd;
             }
-            uint64_t wi = blk * E + k;
-            if (wi < nw) words[wi] = wd;
         }
     }
-    for (int m = 32; m > 0; m >>= 1) cnt += __shfl_xor(cnt, m, 64);
-    if (lane == 0 && cnt) atomicAdd(count, (unsigned long long)cnt);
+    const uint64_t total = block_sum_256(cnt);
+    if (threadIdx.x == 0 && total) atomicAdd(count, (unsigned long long)total);
 }
 
 // ColumnFilter combine: 0 AND, 1 OR, 2 AND-NOT; tail bits past `rows` cleared.
@@ -117,8 +137,8 @@ __global__ __launch_bounds__(256) void k_combine(uint64_t* __restrict__ act, con
         act[i] = v;
         cnt += (uint32_t)__popcll(v);
     }
-    for (int m = 32; m > 0; m >>= 1) cnt += __shfl_xor(cnt, m, 64);
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(count, (unsigned long long)cnt);
+    const uint64_t total = block_sum_256(cnt);
+    if (threadIdx.x == 0 && total) atomicAdd(count, (unsigned long long)total);
 }
 
 struct BitmapSrc {
@@ -195,8 +215,8 @@ int bitmap_t(const void* a, const void* b, size_t n, PredDev pa, PredDev pb, uin
     unsigned long long* cnt = reinterpret_cast<unsigned long long*>(&c->dev->last);
     if (!hip_ok(launch_zero32((uint32_t*)cnt, 2, c->stream), "zero")) return -1;
     if (n) {
-        uint64_t rows_per = 64ull * TyInfo<T>::E;
-        unsigned blocks = grid_cap(((n + rows_per - 1) / rows_per + 3) / 4, (unsigned)device_cu_count() * 16);
+        uint64_t rows_per = 64ull * TyInfo<T>::E * (b ? kBitmapUnroll<true> : kBitmapUnroll<false>);
+        unsigned blocks = grid_cap(((n + rows_per - 1) / rows_per + 3) / 4, (unsigned)device_cu_count() * 8);
         bool aligned = (((uintptr_t)a) & 15u) == 0 && (!b || (((uintptr_t)b) & 15u) == 0);
         if (!aligned) { set_error("bitmap filter: columns must be 16-byte aligned"); return -1; }
         if (b)
@@ -256,7 +276,8 @@ int voxel_gpu_bitmap_combine(uint64_t* active_dev, const uint64_t* other_dev, si
     if (!hip_ok(launch_zero32((uint32_t*)cnt, 2, c->stream), "zero")) return -1;
     uint64_t nw = (rows + 63) / 64;
     if (nw) {
-        hipLaunchKernelGGL(k_combine, dim3(grid_cap((nw + 255) / 256, 4096)), dim3(256), 0, c->stream, active_dev,
+        hipLaunchKernelGGL(k_combine, dim3(grid_cap((nw + 255) / 256, (unsigned)device_cu_count() * 4)), dim3(256), 0,
+                           c->stream, active_dev,
                            other_dev, nw, (uint64_t)rows, combine, cnt);
         if (!hip_ok(hipGetLastError(), "combine launch")) return -1;
     }

@@ -167,11 +167,10 @@ __global__ __launch_bounds__(256) void k_join_probe_sum(const int64_t* __restric
         uint64_t at;
         if (join_find(slots, L, k, at)) { ++cnt; sum += (uint64_t)slots[at].payload; }
     }
-    for (int m = 32; m > 0; m >>= 1) {
-        cnt += __shfl_xor(cnt, m, 64);
-        sum += __shfl_xor(sum, m, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
+    cnt = block_sum_256(cnt);
+    __syncthreads();                                // the partial slots are reused by the second sum
+    sum = block_sum_256(sum);
+    if (threadIdx.x == 0) {
         if (cnt) atomicAdd(&out[0], (unsigned long long)cnt);
         if (sum) atomicAdd(&out[1], (unsigned long long)sum);
     }

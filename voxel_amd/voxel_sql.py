@@ -20,9 +20,12 @@ Two plans:
   meaning.  A conjunction touching the aggregated column and at most one
   other column (at most two comparisons) runs as one fused filter +
   aggregate pass (voxel_gpu_scan, both predicates on the aggregated column
-  when they share it); anything else builds per-comparison bitmaps on the
-  GPU, combines them (AND / OR / AND-NOT) and aggregates through the
-  validity-word scan (voxel_gpu_scan_valid).  Row queries (``SELECT col``)
+  when they share it).  A longer conjunction keeps that fused pass over the
+  aggregated column and one other, its remaining conjuncts (comparisons,
+  OR / NOT groups) becoming selection words the same pass reads
+  (voxel_gpu_scan_valid); anything else builds per-comparison bitmaps on
+  the GPU, combines them (AND / OR / AND-NOT) and aggregates through the
+  validity-word scan.  Row queries (``SELECT col``)
   compact the selected rows, ORDER BY sorts them on the GPU, LIMIT keeps the
   first n; GROUP BY runs the GPU group-by on the selected rows.
 """
@@ -422,6 +425,16 @@ def _comparisons(e):
     return None
 
 
+def _conjuncts(e):
+    """The items of a top-level AND (nested ANDs flattened), or [e]."""
+    if e[0] != "and":
+        return [e]
+    out = []
+    for x in e[1]:
+        out += _conjuncts(x)
+    return out
+
+
 def _bits(v: float) -> int:
     return int(np.array([v], np.float64).view(np.uint64)[0])
 
@@ -432,7 +445,8 @@ def _bitmap(cols, e):
     torch = cols.torch
     n = cols.rows()
     nw = max((n + 63) // 64, 1)
-    w = torch.zeros(nw, dtype=torch.int64, device="cuda")
+    alloc = torch.empty if n else torch.zeros              # the bitmap kernel writes every word
+    w = alloc(nw, dtype=torch.int64, device="cuda")
     cnt = C.c_uint64(0)
     kind = e[0]
     if kind in ("cmp", "between"):
@@ -442,7 +456,7 @@ def _bitmap(cols, e):
         first = True
         for col, op, v in comps:
             d = cols.get(col)
-            t = w if first else torch.zeros(nw, dtype=torch.int64, device="cuda")
+            t = w if first else alloc(nw, dtype=torch.int64, device="cuda")
             if lib.voxel_gpu_filter_bitmap(0, C.c_void_p(d.data_ptr()), n, _CMP[op], _bits(v), 0,
                                            C.c_void_p(t.data_ptr()), C.byref(cnt), None) != 0:
                 check("voxel_gpu_filter_bitmap")
@@ -492,6 +506,27 @@ def _aggregate(cols, col, where):
             op_a, t_a = (_CMP[on_a[0][1]], on_a[0][2]) if on_a else (vs.TRUE, 0.0)
             b = cols.get(others[0][0])
             return vs.scan(a, op_a, t_a, b, _CMP[others[0][1]], others[0][2])
+    # a conjunction: a fused pass over the aggregated column and one other
+    # (a comparison on each), its rows also limited by the selection words of
+    # the remaining conjuncts (comparisons, OR / NOT groups)
+    comps, groups = [], []
+    for item in _conjuncts(where):
+        c = _comparisons(item)
+        if c is None:
+            groups.append(item)
+        else:
+            comps += c
+    if comps:
+        name = cols.name(agg_col)
+        on_a = [c for c in comps if cols.name(c[0]) == name]
+        others = [c for c in comps if cols.name(c[0]) != name]
+        op_a, t_a = (_CMP[on_a[0][1]], on_a[0][2]) if on_a else (vs.TRUE, 0.0)
+        rest = [("cmp", ("col", c), op, ("const", v)) for c, op, v in on_a[1:] + others[1:]] + groups
+        w = _bitmap(cols, ("and", rest)) if rest else None
+        if others:
+            b = cols.get(others[0][0])
+            return vs.scan(a, op_a, t_a, b, _CMP[others[0][1]], others[0][2], valid_a=w)
+        return vs.scan(a, op_a, t_a, valid_a=w)
     w = _bitmap(cols, where)
     return vs.scan(a, vs.TRUE, 0, valid_a=w)
 
