@@ -423,6 +423,7 @@ static T NAME(int op, int cmp, T x, T y, T s) {                                 
     case 0x8D: return ISF ? (T)((T)1 / (T)sqrt((double)x))                                   \
                           : ((double)x <= 0 ? (T)0 : (T)(1.0 / sqrt((double)x)));            \
     case 0x8E: return ISF ? (T)1 / x : (x == (T)0 ? (T)0 : (T)(1.0 / (double)x));            \
+    case 0x78: return s;                                   /* VSPLAT: register bits as T */  \
     case 0xA0: case 0xA1: case 0xA2: case 0xA3: case 0xA4: case 0xA5:                        \
         return VO_PRED(op - 0xA0, x, y) ? BL(~0ull) : (T)0;       /* VCMPxx: all-ones lane */ \
     case 0xA6: return (ISF ? x != x : x == (T)0) ? BL(~0ull) : (T)0;   /* VCMPNULL */        \

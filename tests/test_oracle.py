@@ -316,7 +316,7 @@ def test_tier0_expr_restatement_against_reference_engine(is_f64):
     binary and unary word is the last word of two programs, reduced first."""
     from tests import tier0_gen as tg
     rng = np.random.default_rng(91 if is_f64 else 92)
-    forced = tg.VEC_BIN + tg.VEC_UN
+    forced = tg.VEC_BIN + tg.VEC_UN + [tg.VSPLAT]
     for trial in range(60 + 2 * len(forced)):
         last = forced[(trial - 60) // 2] if trial >= 60 else None
         n = int(rng.integers(1, 300)) * 4

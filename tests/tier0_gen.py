@@ -149,6 +149,7 @@ def last_agg_writers(words):
 VEC_BIN = [0x80, 0x81, 0x82, 0x83, 0x84, 0x87, 0x88, 0x89,  # VADD VSUB VMUL VDIV VMOD VMIN VMAX VAVG
            0xA0, 0xA1, 0xA2, 0xA3, 0xA4, 0xA5,              # VCMPEQ .. VCMPGE
            0xB0, 0xB1, 0xB2, 0xB4, 0xB5, 0xB6, 0xB7]        # VAND VOR VXOR VANDN VSHL VSHR VSAR
+VSPLAT = 0x78
 VEC_FMA = [0x8A, 0x8B]                                      # VFMA VFMS: a * b +/- R[imm12 & 0xF]
 VEC_UN = [0x85, 0x86, 0x8C, 0x8D, 0x8E, 0xA6, 0xA7, 0xB3]   # VNEG VABS VSQRT VRSQRT VRCP VCMPNULL VCMPNOTNULL VNOT
 VEC_SCAL = [0x90, 0x91, 0x92, 0x93, 0x94, 0x95, 0x96]      # VSADD VSSUB VSMUL VSDIV VSMOD VSMIN VSMAX
@@ -172,7 +173,7 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3), last=None):
         b = int(rng.choice(defined))
         sreg = int(rng.choice(SCAL_REGS))
         if last is not None and k == nops - 1:
-            kind = 0.35 if last in VEC_UN else 0.0
+            kind = 0.35 if last in VEC_UN else 0.41 if last == VSPLAT else 0.0
         if kind < 0.25:
             op = int(rng.choice(VEC_BIN)) if last is None or k < nops - 1 else last
             code.append(w(op, rd=dst, ra=a, rb=b))
@@ -185,6 +186,10 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3), last=None):
             op = int(rng.choice(VEC_UN)) if last is None or k < nops - 1 else last
             code.append(w(op, rd=dst, ra=a))
             ops.append((op, dst, a, 0, 0, -1))
+        elif kind < 0.43:
+            op = VSPLAT                                       # every lane = R[ra]'s bits
+            code.append(w(op, rd=dst, ra=sreg))
+            ops.append((op, dst, 0, 0, 0, sreg))
         elif kind < 0.75:
             op = int(rng.choice(VEC_SCAL))
             code.append(w(op, rd=dst, ra=a, rb=sreg))

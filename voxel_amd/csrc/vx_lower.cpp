@@ -176,8 +176,8 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     bool chain_ok = true;
     auto defined = [&](uint8_t v) { return chain_len[v] != -1; };
     auto vec_arith = [](uint8_t op) {          // lowered vector words (engine.hpp:2221-2450, 2473-2590, 2671-2865)
-        return (op >= 0x80 && op <= 0x8E) || (op >= 0x90 && op <= 0x96) || (op >= 0xA0 && op <= 0xA7) ||
-               (op >= 0xB0 && op <= 0xB7);
+        return op == 0x78 || (op >= 0x80 && op <= 0x8E) || (op >= 0x90 && op <= 0x96) ||
+               (op >= 0xA0 && op <= 0xA7) || (op >= 0xB0 && op <= 0xB7);
     };
     while (i < len) {
         const uint32_t w = c[i];
@@ -214,11 +214,13 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
             const bool unary = op == 0x85 || op == 0x86 || (op >= 0x8C && op <= 0x8E) || op == 0xA6 || op == 0xA7 ||
                                op == 0xB3;
             const bool fma = op == 0x8A || op == 0x8B;         // c = R[imm12 & 0xF]
-            if (!defined(a) || (!scal && !unary && !defined(b))) return none("vector arithmetic reads a vector the loop does not define");
+            const bool splat = op == 0x78;                      // VSPLAT: every lane = R[ra] (engine.hpp:2106-2112)
+            if ((!splat && !defined(a)) || (!scal && !unary && !splat && !defined(b)))
+                return none("vector arithmetic reads a vector the loop does not define");
             if (defined(dst)) return none("vector arithmetic overwrites a vector the loop already defines");
             if (p.vops.size() >= 16) return none("more than sixteen vector words in the loop");
-            const uint8_t sreg = scal ? b : fma ? (uint8_t)(imm_of(w) & 0xF) : (uint8_t)0xFF;
-            p.vops.push_back(Plan::VOp{op, dst, a, (uint8_t)(scal || unary ? 0 : b), sreg, 0});
+            const uint8_t sreg = scal ? b : fma ? (uint8_t)(imm_of(w) & 0xF) : splat ? a : (uint8_t)0xFF;
+            p.vops.push_back(Plan::VOp{op, dst, (uint8_t)(splat ? 0 : a), (uint8_t)(scal || unary || splat ? 0 : b), sreg, 0});
             chain_len[dst] = -2;
             p.has_arith = true;
             ++i;

@@ -237,6 +237,7 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
                 break;
             case 0x95: r = F ? ((x < s || x != x) ? x : s) : (x < s ? x : s); break;             // VSMIN
             case 0x96: r = F ? ((x > s || s != s) ? x : s) : (x > s ? x : s); break;             // VSMAX
+            case 0x78: r = s; break;                                                             // VSPLAT
             case 0xA0: case 0xA1: case 0xA2: case 0xA3: case 0xA4: case 0xA5:                     // VCMPxx
                 r = expr_pass<T>(o.op - 0xA0, x, y) ? lane_of<T>(~0ull) : T(0);
                 break;
