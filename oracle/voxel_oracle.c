@@ -558,6 +558,24 @@ int vo_scalar_run(const uint32_t* w, size_t n, uint64_t* r) {
         case 0x34: r[rd] = a << (b & 0x3F); break;
         case 0x35: r[rd] = a >> (b & 0x3F); break;
         case 0x36: r[rd] = (uint64_t)(sa >> (b & 0x3F)); break;
+        /* conversions, engine.hpp:1822-1937 */
+        case 0x60: r[rd] = (uint64_t)(int64_t)(int8_t)(uint8_t)a; break;
+        case 0x61: r[rd] = (uint64_t)(int64_t)(int16_t)(uint16_t)a; break;
+        case 0x62: r[rd] = (uint64_t)(int64_t)(int32_t)(uint32_t)a; break;
+        case 0x64: r[rd] = vo_f64bits((double)(float)fa); break;
+        case 0x66: r[rd] = a & 0xFF; break;
+        case 0x67: r[rd] = a & 0xFFFF; break;
+        case 0x68: r[rd] = a & 0xFFFFFFFFull; break;
+        case 0x63: case 0x65: case 0x69: case 0x6A: case 0x6B: r[rd] = a; break;
+        case 0x6C: {           /* (i64)trunc(x): cvttsd2si, INT64_MIN for NaN / out of range */
+            double t = trunc(fa);
+            r[rd] = (t != t || t >= 9223372036854775808.0 || t < -9223372036854775808.0)
+                        ? 0x8000000000000000ull : (uint64_t)(int64_t)t;
+            break;
+        }
+        case 0x6D: r[rd] = vo_f64bits(round(fa)); break;
+        case 0x6E: r[rd] = vo_f64bits(ceil(fa)); break;
+        case 0x6F: r[rd] = vo_f64bits(floor(fa)); break;
         default: return -1;
         }
     }

@@ -22,13 +22,13 @@ def w(op, rd=0, ra=0, rb=0, imm=0):
     return (op & 0xFF) | (rd & 0xF) << 8 | (ra & 0xF) << 12 | (rb & 0xF) << 16 | (imm & 0xFFF) << 20
 
 
-SCALAR_OPS = [0x00, 0x10, 0x11] + list(range(0x20, 0x30)) + list(range(0x30, 0x37))
+SCALAR_OPS = [0x00, 0x10, 0x11] + list(range(0x20, 0x30)) + list(range(0x30, 0x37)) + list(range(0x60, 0x70))
 SCRATCH = (13, 14, 15)
 
 
 def scalar_words(rng, k, reads, writes):
     """k random scalar words (NOP MOV MOVR, integer, f64 and bitwise
-    arithmetic) reading registers `reads` and writing `writes`."""
+    arithmetic, conversions) reading registers `reads` and writing `writes`."""
     out = []
     for _ in range(k):
         op = int(rng.choice(SCALAR_OPS))

@@ -20,7 +20,8 @@ Plan none(const char* why) {
 }  // namespace
 
 bool host_op(uint8_t op) {
-    return op == OP_NOP || op == OP_MOV || op == OP_MOVR || (op >= 0x20 && op <= 0x2F) || (op >= 0x30 && op <= 0x36);
+    return op == OP_NOP || op == OP_MOV || op == OP_MOVR || (op >= 0x20 && op <= 0x2F) || (op >= 0x30 && op <= 0x36) ||
+           (op >= 0x60 && op <= 0x6F);
 }
 
 int run_scalar(const std::vector<uint32_t>& words, uint64_t* regs) {
@@ -72,6 +73,28 @@ int run_scalar(const std::vector<uint32_t>& words, uint64_t* regs) {
         case 0x34: regs[rd] = a << (b & 0x3F); break;                        // SHL
         case 0x35: regs[rd] = a >> (b & 0x3F); break;                        // SHR
         case 0x36: regs[rd] = (uint64_t)(sa >> (b & 0x3F)); break;           // SAR
+        // conversions (engine.hpp:1822-1937): sign-extending / masking
+        // integer widths, f32 rounding, bit copies, and trunc / round /
+        // ceil / floor of the register read as f64
+        case 0x60: regs[rd] = (uint64_t)(int64_t)(int8_t)(uint8_t)a; break;  // CVT_I8
+        case 0x61: regs[rd] = (uint64_t)(int64_t)(int16_t)(uint16_t)a; break;// CVT_I16
+        case 0x62: regs[rd] = (uint64_t)(int64_t)(int32_t)(uint32_t)a; break;// CVT_I32
+        case 0x64: regs[rd] = bits((double)(float)f64(a)); break;            // CVT_F32
+        case 0x66: regs[rd] = a & 0xFF; break;                               // CVT_U8
+        case 0x67: regs[rd] = a & 0xFFFF; break;                             // CVT_U16
+        case 0x68: regs[rd] = a & 0xFFFFFFFFull; break;                      // CVT_U32
+        case 0x63: case 0x65: case 0x69: case 0x6A: case 0x6B:               // CVT_I64 CVT_F64 CVT_U64
+            regs[rd] = a;                                                    // BITCAST REINTERPRET
+            break;
+        case 0x6C: {                                                         // TRUNC: x86 cvttsd2si
+            const double t = std::trunc(f64(a));
+            regs[rd] = (t != t || t >= 9223372036854775808.0 || t < -9223372036854775808.0)
+                           ? 0x8000000000000000ull : (uint64_t)(int64_t)t;
+            break;
+        }
+        case 0x6D: regs[rd] = bits(std::round(f64(a))); break;               // ROUND
+        case 0x6E: regs[rd] = bits(std::ceil(f64(a))); break;                // CEIL
+        case 0x6F: regs[rd] = bits(std::floor(f64(a))); break;               // FLOOR
         default: set_error("%s", "word is not a scalar instruction"); return -1;
         }
     }
