@@ -240,8 +240,9 @@ def test_tier0_multi_restatement_against_reference_engine(is_f64):
 @pytest.mark.skipif(o.R is None, reason="reference build absent")
 def test_scalar_words_restatement_against_reference_engine():
     """vo_scalar_run against the reference engine on straight-line programs
-    of integer, f64 and bitwise words (including DIV/MOD/DIVF by zero, which
-    stop the program): all 16 scalar registers bit-identical."""
+    of integer, f64, bitwise and conversion words (including DIV/MOD/DIVF by
+    zero, which stop the program, and TRUNC of NaN or out-of-range values):
+    all 16 scalar registers bit-identical."""
     from tests import tier0_gen as tg
     rng = np.random.default_rng(41)
     x = np.zeros(4, np.float64)
