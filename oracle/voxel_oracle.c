@@ -1313,13 +1313,36 @@ static double vo_window_lane(int wkind, const double* d, uint64_t count, uint64_
     return wkind == 3 ? sqrt(q / (double)w) : q / (double)w;
 }
 
+/* WQUANTILE (engine.hpp:2642-2664): element idx of the sorted window; 0 past Count */
+static double vo_nth_lane(const double* d, uint64_t count, uint64_t start, int w, int idx) {
+    double buf[32];
+    int k, j;
+    if (start + (uint64_t)w > count) return 0.0;
+    for (k = 0; k < w; ++k) {
+        double x = d[start + k];
+        for (j = k; j > 0 && buf[j - 1] > x; --j) buf[j] = buf[j - 1];
+        buf[j] = x;
+    }
+    return buf[idx];
+}
+
+/* wkind 6 (WQUANTILE): `window` is the word's imm12 & 0xFF (5-bit window,
+   0 meaning 16; 3-bit quantile q selecting element (q * w) / 7). */
 int vo_tier0_window_f64(const double* data, uint64_t count, uint64_t off, uint64_t total, int wkind, int window,
                         int vload_dst, int nops, const int* code, const double* sval, int nb, const int* bvreg,
                         const int* red, const int* accop, uint64_t* acc, uint64_t* part) {
     double v[16][4];
     int i, k, b;
+    int qidx = 0;
     memset(v, 0, sizeof(v));
-    if (wkind != 5 && (window == 0 || ((wkind == 3 || wkind == 4) && window < 2))) return 0;
+    if (wkind == 6) {
+        int imm = window;
+        window = imm & 0x1F;
+        if (window == 0) window = 16;
+        qidx = (((imm >> 5) & 7) * window) / 7;
+        if (qidx >= window) qidx = window - 1;
+    }
+    if (wkind != 5 && (window == 0 || ((wkind == 3 || wkind == 4 || wkind == 6) && window < 2))) return 0;
     do {
         uint64_t adv;
         if (wkind == 5) {
@@ -1328,7 +1351,9 @@ int vo_tier0_window_f64(const double* data, uint64_t count, uint64_t off, uint64
             for (i = 0; i < 4; ++i) v[vload_dst][i] = (uint64_t)i < cnt ? data[off + i + 1] - data[off + i] : 0.0;
             adv = cnt;
         } else {
-            for (i = 0; i < 4; ++i) v[vload_dst][i] = vo_window_lane(wkind, data, count, off + (uint64_t)i, window);
+            for (i = 0; i < 4; ++i)
+                v[vload_dst][i] = wkind == 6 ? vo_nth_lane(data, count, off + (uint64_t)i, window, qidx)
+                                             : vo_window_lane(wkind, data, count, off + (uint64_t)i, window);
             adv = count > off ? (count - off < 4 ? count - off : 4) : 0;
         }
         for (k = 0; k < nops; ++k) {

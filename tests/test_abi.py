@@ -244,4 +244,5 @@ def test_lowering_accepts_window_word_loops():
     assert lowered(loop(W(0xDE, rd=0, ra=1, imm=20))) == 2
     assert lowered(loop(W(0xDE, rd=0, ra=1, imm=0))) == 0              # window 0: the word is a no-op
     assert lowered(loop(W(0x97, rd=0, ra=1, imm=1))) == 0              # WSTD needs a window of 2 or more
-    assert lowered(loop(W(0x99, rd=0, ra=1, imm=5))) == 0              # WQUANTILE is not lowered
+    assert lowered(loop(W(0x99, rd=0, ra=1, imm=5 | (3 << 5)))) == 2   # WQUANTILE, window 5, q = 3
+    assert lowered(loop(W(0x99, rd=0, ra=1, imm=1))) == 0              # WQUANTILE window 1: a no-op

@@ -766,18 +766,20 @@ def test_delta_bitpacked_for_restatement_against_reference():
 @pytest.mark.skipif(o.R is None, reason="reference build absent")
 def test_tier0_window_restatement_against_reference_engine():
     """vo_tier0_window_f64 (loops loading WINDOW_SUM / WINDOW_MEAN / WSTD /
-    WVARIANCE_W / WDELTA, vector words, up to three reductions) against the
+    WVARIANCE_W / WDELTA / WQUANTILE, vector words, up to three reductions) against the
     reference engine running the same words: every accumulator and partial
     bit-identical."""
     from tests import tier0_gen as tg
     rng = np.random.default_rng(93)
-    for trial in range(80):
-        wkind = 1 + trial % 5
+    for trial in range(96):
+        wkind = 1 + trial % 6
         n = int(rng.integers(6, 400))
         window = int(rng.choice([1, 2, 3, 7, 20, 64])) if wkind != 5 else 0
         if wkind in (3, 4) and window < 2:
             window = 2
-        x = np.round(rng.standard_normal(n) * 100, 2)
+        if wkind == 6:
+            window = tg.wquantile_imm(rng)
+        x = np.round(rng.standard_normal(n) * 100, 2) + 0.0      # no -0.0: a +-0 tie's order is unspecified
         regs = np.zeros(64, np.uint64)
         for r in tg.SCAL_REGS:
             regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))

@@ -354,8 +354,8 @@ def test_tier0_vector_arithmetic(gpu, is_f64):
 
 
 def test_tier0_window_word_loops(gpu):
-    """Loops loading WINDOW_SUM / WINDOW_MEAN / WSTD / WVARIANCE_W / WDELTA
-    (the word advancing the offset), vector words and up to three
+    """Loops loading WINDOW_SUM / WINDOW_MEAN / WSTD / WVARIANCE_W / WDELTA /
+    WQUANTILE (the word advancing the offset), vector words and up to three
     reductions: partials, counts, min and max equal the oracle's bits
     (vo_tier0_window_f64, pinned against the reference engine), NaN results
     compared as NaN; sums within 2 ulp of the exact sum of the reduced lanes
@@ -370,13 +370,15 @@ def test_tier0_window_word_loops(gpu):
             return math.isnan(o.bits_f64(got))
         return got == want
 
-    for trial in range(40):
-        wkind = 1 + trial % 5
+    for trial in range(48):
+        wkind = 1 + trial % 6
         n = int(rng.integers(8, 20000))
         window = int(rng.choice([1, 2, 5, 20, 255])) if wkind != 5 else 0
         if wkind in (3, 4) and window < 2:
             window = 2
-        x = np.round(rng.standard_normal(n) * 100, 2)
+        if wkind == 6:
+            window = tg.wquantile_imm(rng)                 # the WQUANTILE word's imm
+        x = np.round(rng.standard_normal(n) * 100, 2) + 0.0      # no -0.0: a +-0 tie's order is unspecified
         regs = np.zeros(64, np.uint64)
         for r in tg.SCAL_REGS:
             regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
@@ -402,8 +404,11 @@ def test_tier0_window_word_loops(gpu):
                 else:
                     # the reduced lanes: window values row by row, then the body's words
                     wv = np.zeros(max(n, 4 * ((total + 3) // 4)) + 4)
-                    kind = {1: "sum", 2: None, 3: "stddev", 4: "variance", 5: "delta"}[wkind]
-                    if wkind == 2:                        # WINDOW_MEAN multiplies by 1 / w
+                    kind = {1: "sum", 2: None, 3: "stddev", 4: "variance", 5: "delta", 6: None}[wkind]
+                    if wkind == 6:                        # WQUANTILE: one order statistic per window
+                        q = tg.wquantile_values(x, window)
+                        wv[:q.size] = q
+                    elif wkind == 2:                      # WINDOW_MEAN multiplies by 1 / w
                         m = max(n - window + 1, 0)
                         sums = np.zeros(m)
                         o.O.vo_window_f64(1, o.ptr(x), n, o.ptr(sums), m, window, 0.0)

@@ -283,7 +283,23 @@ def oracle_expr(o, x, total, spec, regs, is_f64=True, acc0=None):
     return [int(a) for a in acc], [int(p) for p in part]
 
 
-WINDOW_OPS = {1: 0xDE, 2: 0xDF, 3: 0x97, 4: 0x98, 5: 0xDD}   # WINDOW_SUM WINDOW_MEAN WSTD WVARIANCE_W WDELTA
+WINDOW_OPS = {1: 0xDE, 2: 0xDF, 3: 0x97, 4: 0x98, 5: 0xDD,   # WINDOW_SUM WINDOW_MEAN WSTD WVARIANCE_W WDELTA
+              6: 0x99}                                     # WQUANTILE: window = imm (5-bit w | 3-bit q << 5)
+
+
+def wquantile_imm(rng):
+    """A WQUANTILE imm: window 2..31 or 0 (meaning 16), quantile 0..7."""
+    wq = int(rng.choice([0, 2, 3, 5, 8, 16, 31]))
+    return wq | (int(rng.integers(0, 8)) << 5)
+
+
+def wquantile_values(x, imm):
+    """Element (q * w) / 7 of each sorted window x[s:s + w] (numpy)."""
+    wn = imm & 0x1F or 16
+    idx = min(((imm >> 5) & 7) * wn // 7, wn - 1)
+    if x.size < wn:
+        return np.zeros(0)
+    return np.sort(np.lib.stride_tricks.sliding_window_view(x, wn), axis=1)[:, idx]
 
 
 def make_window_program(rng, wkind, window, nops=3, reds=(0, 1, 2, 3)):

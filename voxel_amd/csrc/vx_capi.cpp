@@ -288,9 +288,10 @@ bool run_tier0_window(Engine& e, uint64_t* regs, StreamCtx* c) {
     const void* d = seg_rows(s, e.dtype, 0, count, c);
     if (!d) return false;
     if (!e.wcol.ensure(count * 8)) return false;
-    static const int kind_of[6] = {0, 1, 6, 3, 4, 0};    // window_launch kinds: sum, mean by 1/w, std, var, delta
+    // window_launch kinds: sum, mean by 1/w, std, var, delta, order statistic
+    static const int kind_of[7] = {0, 1, 6, 3, 4, 0, 7};
     const int64_t nvalid = window_launch(kind_of[p.wkind], static_cast<const double*>(d), count, count, p.window,
-                                         static_cast<double*>(e.wcol.dev), c);
+                                         static_cast<double*>(e.wcol.dev), c, (double)p.wq);
     if (nvalid < 0) return false;
     return run_expr_blocks(e, regs, e.wcol.dev, (uint64_t)nvalid, off, groups * 4, total, c);
 }

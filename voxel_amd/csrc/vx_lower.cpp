@@ -171,6 +171,7 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
         case 0x97: return 3;                               // WSTD
         case 0x98: return 4;                               // WVARIANCE_W
         case 0xDD: return 5;                               // WDELTA
+        case 0x99: return 6;                               // WQUANTILE
         default: return 0;
         }
     };
@@ -184,6 +185,12 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     p.wkind = window_kind(op_of(vl));
     if (p.wkind) {
         p.window = p.wkind == 5 ? 0 : imm_of(vl) & 0xFF;
+        if (p.wkind == 6) {                                // engine.hpp:2642-2664: 5-bit window (0 -> 16),
+            p.window = imm_of(vl) & 0x1F;                  // 3-bit quantile q, element (q * w) / 7
+            if (p.window == 0) p.window = 16;
+            const int q = (imm_of(vl) >> 5) & 0x7, idx = (q * p.window) / 7;
+            p.wq = (uint8_t)(idx < p.window ? idx : p.window - 1);
+        }
         if (p.wkind != 5 && (p.window == 0 || (p.wkind >= 3 && p.window < 2)))
             return none("the window word is a no-op for this window (the reference loop would not end)");
         p.has_arith = true;                                // lanes evaluated row by row

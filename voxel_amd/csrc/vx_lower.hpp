@@ -81,6 +81,7 @@ struct Plan {
     // 1 WINDOW_SUM, 2 WINDOW_MEAN, 3 WSTD, 4 WVARIANCE_W, 5 WDELTA (0: VLOAD).
     // The word advances the offset itself; the loop has no ADD of it.
     uint8_t wkind = 0;
+    uint8_t wq = 0;                 // WQUANTILE: order-statistic index in its window
     uint8_t window = 0;
 };
 

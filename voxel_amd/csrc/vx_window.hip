@@ -40,7 +40,9 @@ constexpr int kWTile = kWThreads * kWRows;        // outputs per workgroup
 constexpr int kMaxWindow = 255;                   // the window is a u8 (engine.hpp:197)
 constexpr int kQuantMax = 128;                    // WindowQuantile's buffer (engine.hpp:284)
 
-enum WinKind { K_DELTA = 0, K_SUM = 1, K_MEAN = 2, K_STD = 3, K_VAR = 4, K_QUANT = 5, K_MEAN_MUL = 6 };
+// K_NTH: the q-th smallest value of each window (q an index, as the
+// WQUANTILE word selects it).
+enum WinKind { K_DELTA = 0, K_SUM = 1, K_MEAN = 2, K_STD = 3, K_VAR = 4, K_QUANT = 5, K_MEAN_MUL = 6, K_NTH = 7 };
 
 // Thread t owns the kWRows consecutive outputs o = t*kWRows + r.  The window
 // rows are walked in chunks of kWRows: each chunk reads kWRows new values
@@ -226,6 +228,8 @@ int64_t window_count(int kind, uint64_t count, uint64_t out_len, int w, double q
         if (w == 0 || count < (uint64_t)w) return 0;
     } else if (kind == K_STD || kind == K_VAR) {
         if (w < 2 || count < (uint64_t)w) return 0;
+    } else if (kind == K_NTH) {
+        if (w < 2 || w > kQuantMax || count < (uint64_t)w || !(q >= 0.0 && q < (double)w)) return 0;
     } else if (w < 2 || count < (uint64_t)w || !(q >= 0.0 && q <= 1.0)) {
         return 0;
     }
@@ -269,11 +273,18 @@ int64_t window_launch(int kind, const double* d, uint64_t count, uint64_t out_le
         WindowBufs& b = window_bufs();
         if (!b.ensure_flag()) return -1;
         const int ww = std::min(window, kQuantMax);
-        const double pos = q * (double)(ww - 1);          // engine.hpp:286-289
-        int lo = (int)pos;
-        if (lo >= ww) lo = ww - 1;
-        const int hi = lo + 1 < ww ? lo + 1 : lo;
-        const double frac = pos - (double)lo;
+        int lo, hi;
+        double frac;
+        if (kind == K_NTH) {                              // one order statistic, no interpolation
+            lo = hi = (int)q;
+            frac = 0.0;
+        } else {
+            const double pos = q * (double)(ww - 1);      // engine.hpp:286-289
+            lo = (int)pos;
+            if (lo >= ww) lo = ww - 1;
+            hi = lo + 1 < ww ? lo + 1 : lo;
+            frac = pos - (double)lo;
+        }
         hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, b.flag_dev);
         hipLaunchKernelGGL(k_window_quantile, dim3(blocks_for(nout, kWThreads)), dim3(kWThreads), 0, c->stream, d,
                            (uint64_t)count, out, nout, ww, lo, hi, frac, b.flag_dev);
