@@ -243,6 +243,24 @@ int       voxel_gpu_scan(const voxel_gpu_scan_desc* d, void* stream, voxel_gpu_a
 int       voxel_gpu_scan_launch(const voxel_gpu_scan_desc* d, void* stream);
 int       voxel_gpu_scan_result(void* stream, voxel_gpu_agg* out);
 
+/* Allocation-free voxel_gpu_scan_launch.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * scan kernels are enqueued on `stream` (NULL = the default stream) with
+ * their per-workgroup partials in a fixed static device buffer of
+ * VOXEL_GPU_NOALLOC_MAX_BLOCKS x 64 bytes per GPU, held until the process
+ * ends, instead of library-allocated scratch; a last one-thread kernel writes
+ * the result as a voxel_gpu_agg (the same fields voxel_gpu_scan returns) into
+ * out_dev, caller device memory of sizeof(voxel_gpu_agg) bytes.  Nothing is
+ * allocated and nothing waits: read out_dev after the stream's work is done.
+ * Fails (-1, voxel_last_error) for d->blocks above VOXEL_GPU_NOALLOC_MAX_BLOCKS
+ * and for layouts 1 and 4 (their last-workgroup combine needs a ticket).
+ * Every call on one GPU shares that GPU's buffer: calls that overlap on one
+ * GPU (on two streams, or from two host threads) race; calls in order on one
+ * stream do not. */
+#define VOXEL_GPU_NOALLOC_MAX_BLOCKS 4096
+int       voxel_gpu_scan_noalloc(const voxel_gpu_scan_desc* d, voxel_gpu_agg* out_dev, void* stream);
+
 /* ---- bitmap filters, ColumnFilter algebra, compaction ------------------ */
 /* Predicate code 7 = Between: t <= x <= t2 (VectorFilter::ApplyBetween). */
 enum { VOXEL_CMP_BETWEEN = 7 };

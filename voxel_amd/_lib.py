@@ -234,6 +234,7 @@ SIGS = {
     "voxel_gpu_scan": (INT, [C.POINTER(ScanDesc), P, C.POINTER(Agg)]),
     "voxel_gpu_scan_launch": (INT, [C.POINTER(ScanDesc), P]),
     "voxel_gpu_scan_result": (INT, [P, C.POINTER(Agg)]),
+    "voxel_gpu_scan_noalloc": (INT, [C.POINTER(ScanDesc), P, P]),
 }
 
 for _name, (_res, _args) in SIGS.items():

@@ -885,4 +885,24 @@ int voxel_gpu_scan(const voxel_gpu_scan_desc* d, void* stream, voxel_gpu_agg* ou
     return voxel_gpu_scan_result(stream, out);
 }
 
+// No runtime_ok()/ctx_for(): those allocate the per-stream scratch on first
+// use.  The device is the caller's current one.
+int voxel_gpu_scan_noalloc(const voxel_gpu_scan_desc* d, voxel_gpu_agg* out_dev, void* stream) {
+    set_error("%s", "");
+    ScanArgs g;
+    if (!make_scan_args(d, g)) return -1;
+    if (!out_dev) { set_error("scan_noalloc: out_dev is null"); return -1; }
+    if (d->blocks < 0 || (size_t)d->blocks > kNoAllocMaxParts) {
+        set_error("scan_noalloc: %d workgroups requested, the static partial buffer holds at most %zu",
+                  d->blocks, kNoAllocMaxParts);
+        return -1;
+    }
+    if (!(g.layout == 0 || g.layout == 2 || g.layout == 3)) {
+        set_error("scan_noalloc: layout %d combines in the last workgroup (needs a ticket); use layout 0, 2 or 3",
+                  g.layout);
+        return -1;
+    }
+    return hip_ok(launch_scan_noalloc(g, out_dev, static_cast<hipStream_t>(stream)), "scan_noalloc launch") ? 0 : -1;
+}
+
 }  // extern "C"

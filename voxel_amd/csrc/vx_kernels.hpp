@@ -80,6 +80,12 @@ hipError_t launch_scan(const ScanArgs& args, const Scratch& scr, ScanOut* out_de
 // The same over nullable columns: rows whose validity bit (va for a, vb for
 // b; NULL = all valid; words indexed from row 0, args.begin a multiple of 64)
 // is 0 are skipped.  Range predicates only (no table form).
+// launch_scan with the partials in the file-scope static device buffer of
+// vx_scan.hip (max_parts workgroups) instead of runtime scratch, then a
+// one-thread kernel writing the result as a voxel_gpu_agg (80 bytes) to
+// out_dev.  Allocates nothing; split layouts (0, 2, 3) only.
+constexpr size_t kNoAllocMaxParts = 4096;
+hipError_t launch_scan_noalloc(const ScanArgs& args, void* out_dev, hipStream_t stream);
 hipError_t launch_scan_valid(const ScanArgs& args, const uint64_t* va, const uint64_t* vb, const Scratch& scr,
                              ScanOut* out_dev, hipStream_t stream);
 

@@ -12,10 +12,12 @@
 //   * predicates as a closed range test (two compares -> SGPR lane masks);
 //   * floats: per-lane TwoSum-compensated (s, c) pairs kept in T;
 //     ints: wrapping 64-bit sums (exact);
-//   * wave64 butterfly reduction, one partial per workgroup, and a
-//     deterministic final reduction by the last workgroup to finish (ticket),
-//     so one launch produces the result and the order of the final combine
-//     depends only on the grid size.
+//   * wave64 butterfly reduction and one partial per workgroup; by default a
+//     second one-workgroup kernel (k_scan_final) merges the partials in
+//     workgroup order after the kernel boundary, so the order of the final
+//     combine depends only on the grid size.  Layouts 1 and 4 instead merge
+//     in the last workgroup to finish (a self-resetting ticket), which costs
+//     an agent-scope release/acquire per workgroup on this multi-XCD part.
 #include "vx_acc.hpp"
 #include "vx_device.hpp"
 #include "vx_kernels.hpp"
@@ -23,7 +25,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <type_traits>
+
+#include "../../include/voxel_c.h"
 
 #pragma clang fp contract(off)
 
@@ -498,7 +503,114 @@ hipError_t launch_type(const ScanArgs& g, const Scratch& scr, ScanOut* out, hipS
     return launch_t<T, 2, false, false>(g, scr, out, st);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free scans (voxel_gpu_scan_noalloc).  The partials live in a
+// file-scope static device buffer: kNoAllocMaxParts 64-byte partials, then
+// the 64-byte ScanOut the combine writes.  Every partial k_scan_final reads
+// was written by the same launch's k_scan and the result slot is written
+// before it is read, so nothing has to start at zero.  Each GPU has its own
+// copy at its own address, looked up once per device with
+// hipGetSymbolAddress; host code uses only that address.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[(kNoAllocMaxParts + 1) * 64];
+static_assert(sizeof(voxel_gpu_agg) == 80, "voxel_gpu_agg layout");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_of_current_device(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+// ScanOut -> voxel_gpu_agg, the same conversion as the host's fill_agg
+// (finish of the compensated pair with a zero initial value, NaN-free
+// min <= max test for floats).  empty: no rows, every field zero.
+template <typename T>
+__global__ __launch_bounds__(64)
+void k_agg_out(const ScanOut* __restrict__ r, int empty, voxel_gpu_agg* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    voxel_gpu_agg g;
+    g.count = 0; g.sum = 0.0; g.sum_hi = 0.0; g.sum_lo = 0.0; g.isum = 0;
+    g.fmin = 0.0; g.fmax = 0.0; g.imin = 0; g.imax = 0; g.has_minmax = 0; g.pad = 0;
+    if (!empty) {
+        const ScanOut o = *r;
+        g.count = o.cnt;
+        if constexpr (std::is_same<T, double>::value) {
+            const double s = __longlong_as_double((long long)o.s), c = __longlong_as_double((long long)o.c);
+            g.sum_hi = s; g.sum_lo = c;
+            if (!__builtin_isfinite(s)) {
+                g.sum = 0.0 + s;
+            } else {
+                const double t = 0.0 + s, bp = t - 0.0, e = (0.0 - (t - bp)) + (s - bp);
+                g.sum = t + (e + c);
+            }
+            g.fmin = __longlong_as_double((long long)o.mn); g.fmax = __longlong_as_double((long long)o.mx);
+            g.has_minmax = o.cnt > 0 && g.fmin <= g.fmax;
+        } else if constexpr (std::is_same<T, float>::value) {
+            const float s = __uint_as_float((uint32_t)o.s), c = __uint_as_float((uint32_t)o.c);
+            g.sum_hi = s; g.sum_lo = c;
+            if (!__builtin_isfinite(s)) {
+                g.sum = 0.0f + s;
+            } else {
+                const float t = 0.0f + s, bp = t - 0.0f, e = (0.0f - (t - bp)) + (s - bp);
+                g.sum = t + (e + c);
+            }
+            g.fmin = __uint_as_float((uint32_t)o.mn); g.fmax = __uint_as_float((uint32_t)o.mx);
+            g.has_minmax = o.cnt > 0 && g.fmin <= g.fmax;
+        } else if constexpr (std::is_same<T, int64_t>::value) {
+            g.isum = (int64_t)o.s; g.imin = (int64_t)o.mn; g.imax = (int64_t)o.mx;
+            g.has_minmax = o.cnt > 0;
+        } else {
+            g.isum = (int64_t)o.s; g.imin = (int32_t)(uint32_t)o.mn; g.imax = (int32_t)(uint32_t)o.mx;
+            g.has_minmax = o.cnt > 0;
+        }
+    }
+    *out = g;
+}
+
 }  // namespace
+
+hipError_t launch_scan_noalloc(const ScanArgs& g, void* out_dev, hipStream_t st) {
+    if (g.end < g.begin || !out_dev) return hipErrorInvalidValue;
+    if (!(g.layout == 0 || g.layout == 2 || g.layout == 3)) return hipErrorInvalidValue;
+    if (g.blocks < 0 || (size_t)g.blocks > kNoAllocMaxParts) return hipErrorInvalidValue;
+    void* base = nullptr;
+    hipError_t e = scratch_of_current_device(&base);
+    if (e != hipSuccess) return e;
+    Scratch scr{};
+    scr.partials = base;
+    scr.ticket = nullptr;                                   // split layouts never touch it
+    scr.max_blocks = kNoAllocMaxParts;
+    ScanOut* res = reinterpret_cast<ScanOut*>(static_cast<char*>(base) + kNoAllocMaxParts * 64);
+    const int empty = g.end == g.begin;
+    if (!empty) {
+        e = launch_scan(g, scr, res, st);
+        if (e != hipSuccess) return e;
+    }
+    voxel_gpu_agg* out = static_cast<voxel_gpu_agg*>(out_dev);
+    switch (g.dtype) {
+    case DT_F64: hipLaunchKernelGGL((k_agg_out<double>), dim3(1), dim3(64), 0, st, res, empty, out); break;
+    case DT_F32: hipLaunchKernelGGL((k_agg_out<float>), dim3(1), dim3(64), 0, st, res, empty, out); break;
+    case DT_I64: hipLaunchKernelGGL((k_agg_out<int64_t>), dim3(1), dim3(64), 0, st, res, empty, out); break;
+    case DT_I32: hipLaunchKernelGGL((k_agg_out<int32_t>), dim3(1), dim3(64), 0, st, res, empty, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 int device_cu_count() {
     static int cus = 0;
