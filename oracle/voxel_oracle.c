@@ -1366,7 +1366,10 @@ int vo_tier0_window_f64(const double* data, uint64_t count, uint64_t off, uint64
             part[b] = vo_red_f64(v[bvreg[b]], red[b]);
             acc[b] = vo_tier0_accum(acc[b], part[b], accop[b]);
         }
-        if (adv == 0 || off + adv > total) return 0;
+        /* the offset stays when the word is past the column's end (advance 0):
+           CMP then falls through only if it already equals the total */
+        if (adv == 0) return off == total;
+        if (off + adv > total) return 0;
         off += adv;
     } while (off != total);
     return 1;

@@ -86,6 +86,33 @@ def test_dictionary_and_rle_decode_and_scan(gpu, kind):
                 check_scan(got, expect(x, op, t), kind == "f64")
 
 
+@pytest.mark.parametrize("nd", [10, 5000])                      # dictionary in LDS / in HBM
+def test_dictionary_code_out_of_range_is_refused(gpu, nd):
+    """An index at or past the dictionary's size (a caller-built column; the
+    reference's Build never writes one) makes DecodeBatch and the fused scan
+    fail with an error; the kernels read no entry past the dictionary (such
+    a code is taken as 0 and flagged).  Valid columns still decode after."""
+    import torch
+    from voxel_amd import encoding as enc
+    rng = np.random.default_rng(64)
+    d = np.sort(rng.standard_normal(nd))
+    for bad in (nd, 2**32 - 1):
+        ix = rng.integers(0, nd, 4099).astype(np.uint32)
+        ix[int(rng.integers(0, ix.size))] = bad
+        col = enc.dictionary(dev(d), dev(ix.view(np.int32)))
+        out = torch.zeros(ix.size, dtype=torch.float64, device="cuda")
+        with pytest.raises(RuntimeError, match="not below the dictionary"):
+            col.decode(0, ix.size, out)
+        with pytest.raises(RuntimeError, match="not below the dictionary"):
+            col.scan("gt", 0.0)
+    ix = rng.integers(0, nd, 4099).astype(np.uint32)
+    col = enc.dictionary(dev(d), dev(ix.view(np.int32)))
+    out = torch.zeros(ix.size, dtype=torch.float64, device="cuda")
+    assert col.decode(0, ix.size, out) == ix.size
+    assert np.array_equal(out.cpu().numpy(), d[ix])
+    check_scan(col.scan("gt", 0.0), expect(d[ix], 4, 0.0), True)
+
+
 def test_delta_decode_and_scan(gpu):
     import torch
     from voxel_amd import encoding as enc

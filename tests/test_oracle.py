@@ -794,3 +794,25 @@ def test_tier0_window_restatement_against_reference_engine():
         for b in range(len(spec["red"])):
             assert int(ref[tg.ACCS[b]]) == acc[b], (trial, wkind, b, spec)
             assert int(ref[tg.PARTS[b]]) == part[b], (trial, wkind, b, spec)
+    # the offset starts at (or past) the end the word steps to: it writes zero
+    # lanes and advances by 0, so one pass runs and CMP falls through
+    for trial in range(24):
+        wkind = 1 + trial % 6
+        n = int(rng.integers(6, 100))
+        window = {5: 0, 6: tg.wquantile_imm(rng)}.get(wkind, int(rng.choice([2, 3, 7])))
+        x = np.round(rng.standard_normal(n) * 100, 2) + 0.0
+        regs = np.zeros(64, np.uint64)
+        for r in tg.SCAL_REGS:
+            regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+        start = (n - 1 if wkind == 5 else n) + int(rng.integers(0, 3))
+        regs[1] = regs[2] = start
+        code, spec = tg.make_window_program(rng, wkind, window, nops=int(rng.integers(0, 4)))
+        acc0 = [int(regs[a]) for a in tg.ACCS[:len(spec["red"])]]
+        ok, acc, part = tg.oracle_window_loop(o, x, start, spec, regs, wkind, window, acc0=acc0, off=start)
+        assert ok, (trial, wkind, n, start)
+        ref = regs.copy()
+        o.R.vr_engine_run_f64(o.ptr(x), n, o.ptr(code), code.size, o.ptr(ref))
+        assert int(ref[1]) == start
+        for b in range(len(spec["red"])):
+            assert int(ref[tg.ACCS[b]]) == acc[b], (trial, wkind, b, spec)
+            assert int(ref[tg.PARTS[b]]) == part[b], (trial, wkind, b, spec)

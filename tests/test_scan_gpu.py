@@ -425,6 +425,33 @@ def test_tier0_window_word_loops(gpu):
                         assert g == want, (trial, wkind, b)
             else:
                 assert same(got, acc[b]), (trial, wkind, b, spec)
+    # the offset starts at (or past) the end the word steps to: one pass over
+    # zero lanes, then CMP falls through (the oracle's restatement of it is
+    # pinned against the reference engine)
+    for trial in range(12):
+        wkind = 1 + trial % 6
+        n = int(rng.integers(8, 300))
+        window = {5: 0, 6: tg.wquantile_imm(rng)}.get(wkind, int(rng.choice([2, 3, 7])))
+        x = np.round(rng.standard_normal(n) * 100, 2) + 0.0
+        regs = np.zeros(64, np.uint64)
+        for r in tg.SCAL_REGS:
+            regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+        start = (n - 1 if wkind == 5 else n) + int(rng.integers(0, 3))
+        code, spec = tg.make_window_program(rng, wkind, window, nops=int(rng.integers(1, 4)))
+        ok, acc, part = tg.oracle_window_loop(o, x, start, spec, regs, wkind, window, off=start)
+        assert ok
+        e = vx.EngineF64()
+        e.add_segment(x)
+        for r in tg.SCAL_REGS:
+            e.set_scalar(r, int(regs[r]))
+        e.set_scalar(1, start)
+        e.set_scalar(2, start)
+        e.load_program([int(c) for c in code])
+        e.run()
+        assert e.get_scalar(1) == start
+        for b in range(len(spec["red"])):
+            assert same(e.get_scalar(tg.PARTS[b]), part[b]), (trial, wkind, b, spec)
+            assert same(e.get_scalar(tg.ACCS[b]), acc[b]), (trial, wkind, b, spec)
     # a bound the window word never steps onto: refused, registers untouched
     e = vx.EngineF64()
     x = np.arange(10, dtype=np.float64)
@@ -576,6 +603,50 @@ def test_segment_aggregate_programs_are_lowered_or_refused(gpu):
     e.load_program([tg.w(0xE0, rd=4, ra=0, rb=1), tg.w(0xE3, rd=5, ra=0, rb=1), 1])
     e.run()
     assert e.get_scalar(4) == 4 and o.bits_f64(e.get_scalar(5)) == 1.0     # NaN counts as != 0; MIN skips it
+
+
+def test_failed_aggregate_program_leaves_slots_untouched(gpu):
+    """A straight-line program that fails part-way leaves the registers and
+    the persistent AGG slots as they were: after AGG_SUM, AGG_COUNT_DISTINCT
+    and AGG_COUNT updated their slots, an AGG_MEDIAN over a NaN segment (or
+    an AGG_COUNT_DISTINCT on a slot AGG_SUM started) fails; the same words
+    run again without the failing one then give bit for bit what one clean
+    run on a fresh engine gives (no row counted twice)."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(23)
+    clean = np.round(rng.standard_normal(5000) * 100, 1)
+    bad = clean.copy()
+    bad[17] = np.nan
+    good = [tg.w(0xE1, rd=4, ra=0, rb=1), tg.w(0xE9, rd=5, ra=0, rb=1), tg.w(0xE0, rd=6, ra=0, rb=1)]
+    failing = [good + [tg.w(0xEB, rd=7, ra=1, rb=1), 1],            # MEDIAN over NaN
+               good + [tg.w(0xE9, rd=4, ra=0, rb=1), 1]]            # COUNT_DISTINCT on SUM's slot
+    fresh = vx.EngineF64()
+    fresh.add_segment(clean)
+    fresh.add_segment(bad)
+    fresh.load_program(good + [1])
+    fresh.run()
+    want = [fresh.get_scalar(r) for r in (4, 5, 6)]
+    assert want[1] == np.unique(clean).size and want[2] == np.count_nonzero(clean)
+    for prog in failing:
+        e = vx.EngineF64()
+        e.add_segment(clean)
+        e.add_segment(bad)
+        e.load_program(prog)
+        with pytest.raises(RuntimeError):
+            e.run()
+        assert [e.get_scalar(r) for r in (4, 5, 6, 7)] == [0, 0, 0, 0]
+        e.load_program(good + [1])
+        e.run()
+        assert [e.get_scalar(r) for r in (4, 5, 6)] == want
+        # a second clean run accumulates on the committed slots only
+        e.run()
+        fresh2 = vx.EngineF64()
+        fresh2.add_segment(clean)
+        fresh2.load_program(good + [1])
+        fresh2.run()
+        fresh2.run()
+        assert [e.get_scalar(r) for r in (4, 5, 6)] == [fresh2.get_scalar(r) for r in (4, 5, 6)]
 
 
 def test_tier0_minmax_mixed_signs_uses_exact_fold(gpu, golden):

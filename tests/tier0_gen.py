@@ -316,8 +316,9 @@ def make_window_program(rng, wkind, window, nops=3, reds=(0, 1, 2, 3)):
     return np.array(code, np.uint32), spec
 
 
-def oracle_window_loop(o, x, total, spec, regs, wkind, window, acc0=None):
-    """vo_tier0_window_f64 on the spec: (terminates, accumulators, partials)."""
+def oracle_window_loop(o, x, total, spec, regs, wkind, window, acc0=None, off=0):
+    """vo_tier0_window_f64 on the spec from offset `off`: (terminates,
+    accumulators, partials)."""
     code, sval = _expr_words(spec, regs, True)
     nb = len(spec["red"])
     bvreg = np.array(spec["bvreg"], np.int32)
@@ -325,7 +326,7 @@ def oracle_window_loop(o, x, total, spec, regs, wkind, window, acc0=None):
     accop = np.array(spec["accop"], np.int32)
     acc = np.array(acc0 if acc0 is not None else [0] * nb, np.uint64)
     part = np.zeros(nb, np.uint64)
-    ok = o.O.vo_tier0_window_f64(o.ptr(x), x.size, 0, total, wkind, window, 0, len(spec["ops"]), o.ptr(code),
+    ok = o.O.vo_tier0_window_f64(o.ptr(x), x.size, off, total, wkind, window, 0, len(spec["ops"]), o.ptr(code),
                                  o.ptr(sval), nb, o.ptr(bvreg), o.ptr(red), o.ptr(accop), o.ptr(acc), o.ptr(part))
     return bool(ok), [int(a) for a in acc], [int(p) for p in part]
 

@@ -3,6 +3,7 @@
 // lowered to the GPU scan kernels of vx_scan.hip.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -37,7 +38,10 @@ struct Seg {
 // Engine<T>: 64 scalar registers (core/registers.hpp:20) + segments + program.
 // One AggState slot (engine.hpp:347-372): the fields the aggregates share.
 // The f64 sum is kept as a compensated pair; the distinct set as sorted
-// order keys in device memory.
+// order keys in one of the slot's two device buffers (`cur`).  A run works
+// on a copy of the slots and writes a changed distinct set into the buffer
+// that is not committed, so a program that fails part-way leaves every slot
+// as it was.
 struct AggSlot {
     uint64_t count = 0;
     double sh = 0.0, sl = 0.0;       // SumF64
@@ -46,9 +50,14 @@ struct AggSlot {
     uint64_t firstv = 0, lastv = 0;
     double mean = 0.0, m2 = 0.0;
     bool init = false, first_set = false, has_set = false;
-    uint64_t* set = nullptr;         // distinct order keys, ascending
-    uint64_t nset = 0, setcap = 0;
-    ~AggSlot() { if (set) (void)hipFree(set); }
+    uint64_t nset = 0;               // distinct order keys, ascending, in set buffer `cur`
+    int cur = 0;
+};
+
+struct SetBuf {
+    uint64_t* p = nullptr;
+    uint64_t cap = 0;
+    ~SetBuf() { if (p) (void)hipFree(p); }
 };
 
 struct Engine {
@@ -57,7 +66,8 @@ struct Engine {
     std::vector<std::unique_ptr<Seg>> segs;
     std::vector<uint32_t> code;
     Plan plan;
-    AggSlot agg[16];
+    AggSlot agg[16];                 // committed slot state
+    SetBuf sets[16][2];
     bool agg_ready = false;
     SegaggBufs segagg;
     DevColumn expr;                  // a reduced vector of a loop with vector arithmetic
@@ -270,6 +280,12 @@ bool run_tier0_window(Engine& e, uint64_t* regs, StreamCtx* c) {
     Seg& s = *e.segs[p.seg];
     const uint64_t off = regs[p.off], total = regs[p.tot], count = s.count;
     const uint64_t last = p.wkind == 5 ? (count ? count - 1 : 0) : count;   // where the partial step lands
+    if (off >= last && total == off) {
+        // the word writes four zero lanes and advances by 0: the body runs
+        // once on them and CMP falls through
+        if (!e.wcol.ensure(32)) return false;
+        return run_expr_blocks(e, regs, e.wcol.dev, 0, off, 4, total, c);
+    }
     uint64_t groups = 0;
     bool ok = false;
     if (off < last && total > off) {
@@ -378,7 +394,7 @@ bool read_bits(StreamCtx* c, const void* dev, uint64_t row, int dtype, uint64_t*
     return true;
 }
 
-bool run_agg(Engine& e, uint32_t w, uint64_t* regs, StreamCtx* c) {
+bool run_agg(Engine& e, uint32_t w, uint64_t* regs, AggSlot* work, StreamCtx* c) {
     const uint8_t op = op_of(w), rd = rd_of(w), seg = ra_of(w), rb = rb_of(w);
     const uint16_t imm = imm_of(w);
     if (seg >= e.segs.size()) return true;                       // missing segment: PC++ only
@@ -389,7 +405,8 @@ bool run_agg(Engine& e, uint32_t w, uint64_t* regs, StreamCtx* c) {
     uint64_t end = start + cnt;
     if (end > s.count) end = s.count;
     const uint64_t n = end > start ? end - start : 0;
-    AggSlot& a = e.agg[rd & 0xF];
+    const int slot = rd & 0xF;
+    AggSlot& a = work[slot];
     const size_t es = dtype_size(e.dtype);
     const char* d = nullptr;
     if (n) {
@@ -501,20 +518,27 @@ bool run_agg(Engine& e, uint32_t w, uint64_t* regs, StreamCtx* c) {
             if (!segagg_unique(b.k2, n, b.k1, &nu, c)) return false;          // new distinct keys in k1
             uint64_t total = nu + a.nset;
             if (a.nset) {                                                      // merge: sort old + new, unique again
-                if (!hip_ok(hipMemcpyAsync(b.k1 + nu, a.set, a.nset * 8, hipMemcpyDeviceToDevice, c->stream), "copy set") ||
+                if (!hip_ok(hipMemcpyAsync(b.k1 + nu, e.sets[slot][a.cur].p, a.nset * 8, hipMemcpyDeviceToDevice,
+                                           c->stream), "copy set") ||
                     !hip_ok(launch_iota(b.v1, total, c->stream), "iota"))
                     return false;
                 if (sort_pairs_u64(b.k1, b.v1, b.k2, b.v2, total, c) < 0) return false;
                 if (!segagg_unique(b.k2, total, b.k1, &nu, c)) return false;
             }
-            if (nu > a.setcap) {
-                if (a.set) (void)hipFree(a.set);
-                a.set = nullptr; a.setcap = 0;
-                if (!hip_ok(hipMalloc((void**)&a.set, nu * 8), "hipMalloc(distinct set)")) return false;
-                a.setcap = nu;
+            // the new set goes to the buffer that is not committed (it may
+            // be this run's earlier set, already read into k1 above)
+            const int out = 1 - e.agg[slot].cur;
+            SetBuf& dst = e.sets[slot][out];
+            if (nu > dst.cap) {
+                if (!hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return false;
+                if (dst.p) (void)hipFree(dst.p);
+                dst.p = nullptr; dst.cap = 0;
+                if (!hip_ok(hipMalloc((void**)&dst.p, nu * 8), "hipMalloc(distinct set)")) return false;
+                dst.cap = nu;
             }
-            if (nu && !hip_ok(hipMemcpyAsync(a.set, b.k1, nu * 8, hipMemcpyDeviceToDevice, c->stream), "copy set")) return false;
+            if (nu && !hip_ok(hipMemcpyAsync(dst.p, b.k1, nu * 8, hipMemcpyDeviceToDevice, c->stream), "copy set")) return false;
             a.nset = nu;
+            a.cur = out;
         }
         regs[rd] = a.nset;
         return true;
@@ -570,17 +594,21 @@ bool run_straight(Engine& e, StreamCtx* c) {
         }
         e.agg_ready = true;
     }
+    // registers and slots change only when the whole program succeeds
     uint64_t regs[64];
     std::memcpy(regs, e.regs, sizeof(regs));
+    AggSlot work[16];
+    std::copy(std::begin(e.agg), std::end(e.agg), std::begin(work));
     for (uint32_t w : e.plan.words) {
         if (agg_op(op_of(w))) {
-            if (!run_agg(e, w, regs, c)) return false;
+            if (!run_agg(e, w, regs, work, c)) return false;
             continue;
         }
         const int rc = run_scalar(std::vector<uint32_t>{w}, regs);
         if (rc < 0) return false;
         if (rc == 0) break;                                      // division by zero stops the program
     }
+    std::copy(std::begin(work), std::end(work), std::begin(e.agg));
     std::memcpy(e.regs, regs, sizeof(regs));
     return true;
 }

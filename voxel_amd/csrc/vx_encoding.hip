@@ -44,11 +44,23 @@ unsigned grid_for(uint64_t n, unsigned per, unsigned cap) {
 
 // ---- decoders ---------------------------------------------------------------
 
+// A code at or past the dictionary's size is never read through: its row
+// decodes as 0 and *bad is set, so the call fails instead of reading past
+// the dictionary.
 template <typename U>
-__global__ __launch_bounds__(kEncThreads) void k_dict_decode(const U* __restrict__ dict, const uint32_t* __restrict__ idx,
-                                                             uint64_t off, uint64_t cnt, U* __restrict__ out) {
+__global__ __launch_bounds__(kEncThreads) void k_dict_decode(const U* __restrict__ dict, uint32_t nd,
+                                                             const uint32_t* __restrict__ idx, uint64_t off,
+                                                             uint64_t cnt, U* __restrict__ out, uint32_t* __restrict__ bad) {
     const uint64_t stride = (uint64_t)gridDim.x * kEncThreads;
-    for (uint64_t i = (uint64_t)blockIdx.x * kEncThreads + threadIdx.x; i < cnt; i += stride) out[i] = dict[idx[off + i]];
+    for (uint64_t i = (uint64_t)blockIdx.x * kEncThreads + threadIdx.x; i < cnt; i += stride) {
+        const uint32_t k = idx[off + i];
+        if (k < nd) {
+            out[i] = dict[k];
+        } else {
+            out[i] = U(0);
+            *bad = 1u;
+        }
+    }
 }
 
 // RLE: run r covers rows [start[r], start[r] + len[r]); rows past the last
@@ -217,10 +229,13 @@ struct DeltaOut {                                         // out[row - off] = ba
 
 // ---- fused scans -------------------------------------------------------------
 
+// Codes at or past nd are read as code 0 (in bounds: nd > kDictLds puts at
+// least one entry in global memory, and a smaller dictionary is read from
+// LDS) and set *bad, which fails the call.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scan_dict(const T* __restrict__ dict, uint32_t nd,
                                                       const uint32_t* __restrict__ idx, uint64_t n, PredDev pd,
-                                                      ScanOut* __restrict__ parts) {
+                                                      ScanOut* __restrict__ parts, uint32_t* __restrict__ bad) {
     __shared__ T ld[kDictLds];
     __shared__ Acc<T> red[kBlock / 64];
     const bool in_lds = nd <= kDictLds;
@@ -236,9 +251,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_dict(const T* __restrict__ dict
     const uint64_t nq = (n - head) / 4;                   // 16-byte loads of four indices
     const uint64_t tail0 = head + nq * 4;
     const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool oob = false;
+    auto code = [&](uint32_t k) -> uint32_t {
+        oob |= k >= nd;
+        return k < nd ? k : 0u;
+    };
     if (g < head + (n - tail0)) {
         const uint64_t r = g < head ? g : tail0 + (g - head);
-        const T x = dv[idx[r]];
+        const T x = dv[code(idx[r])];
         acc.add(p(x), x);
     }
     const uint4* q = reinterpret_cast<const uint4*>(idx + head);
@@ -253,11 +273,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_dict(const T* __restrict__ dict
 #pragma unroll
         for (int u = 0; u < kEncUnroll; ++u) {
             if (base + u * kBlock + threadIdx.x < nq) {
-                const T x0 = dv[k[u].x], x1 = dv[k[u].y], x2 = dv[k[u].z], x3 = dv[k[u].w];
+                const T x0 = dv[code(k[u].x)], x1 = dv[code(k[u].y)], x2 = dv[code(k[u].z)], x3 = dv[code(k[u].w)];
                 acc.add(p(x0), x0); acc.add(p(x1), x1); acc.add(p(x2), x2); acc.add(p(x3), x3);
             }
         }
     }
+    if (oob) *bad = 1u;
     block_reduce(acc, red);
     if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
 }
@@ -476,9 +497,28 @@ bool check_col(const voxel_gpu_encoded_col* e) {
 }
 
 template <typename U>
-void launch_dict_decode(const voxel_gpu_encoded_col* e, uint64_t off, uint64_t cnt, void* out, StreamCtx* c) {
+void launch_dict_decode(const voxel_gpu_encoded_col* e, uint64_t off, uint64_t cnt, void* out, uint32_t* bad,
+                        StreamCtx* c) {
     hipLaunchKernelGGL((k_dict_decode<U>), dim3(grid_for(cnt, kEncThreads, 65536)), dim3(kEncThreads), 0, c->stream,
-                       static_cast<const U*>(e->values), e->index, off, cnt, static_cast<U*>(out));
+                       static_cast<const U*>(e->values), (uint32_t)e->nvalues, e->index, off, cnt,
+                       static_cast<U*>(out), bad);
+}
+
+// Clears the code-range flag before a dictionary launch.
+bool dict_flag(StreamCtx* c) {
+    EncBufs& b = enc_bufs();
+    if (!b.flag()) return false;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(1), 0, c->stream, b.flag_dev);
+    return hip_ok(hipGetLastError(), "clear");
+}
+// After the stream has finished: false (error set) when a code was out of range.
+bool dict_codes_ok(const voxel_gpu_encoded_col* e) {
+    if (*enc_bufs().flag_host) {
+        set_error("Dictionary column: an index is not below the dictionary's %llu values",
+                  (unsigned long long)e->nvalues);
+        return false;
+    }
+    return true;
 }
 template <typename U>
 void launch_rle_decode(const voxel_gpu_encoded_col* e, uint64_t total, uint64_t off, uint64_t cnt, void* out,
@@ -508,10 +548,17 @@ int64_t decode(const voxel_gpu_encoded_col* e, uint64_t off, uint64_t cnt, void*
     if (n == 0) return 0;
     if (!out) { set_error("decode: null output"); return -1; }
     switch (e->kind) {
-    case VOXEL_ENC_DICTIONARY:
-        if (es == 8) launch_dict_decode<uint64_t>(e, off, n, out, c);
-        else launch_dict_decode<uint32_t>(e, off, n, out, c);
-        break;
+    case VOXEL_ENC_DICTIONARY: {
+        if (!dict_flag(c)) return -1;
+        uint32_t* bad = enc_bufs().flag_dev;
+        if (es == 8) launch_dict_decode<uint64_t>(e, off, n, out, bad, c);
+        else launch_dict_decode<uint32_t>(e, off, n, out, bad, c);
+        if (!hip_ok(hipGetLastError(), "decode launch") ||
+            !hip_ok(hipMemcpyAsync(enc_bufs().flag_host, bad, 4, hipMemcpyDeviceToHost, c->stream), "read back") ||
+            !wait(c))
+            return -1;
+        return dict_codes_ok(e) ? (int64_t)n : -1;
+    }
     case VOXEL_ENC_DELTA:
         if (e->dtype == VOXEL_GPU_I64) {
             if (!delta_decode<int64_t, uint64_t>(static_cast<const int64_t*>(e->values), e->rows,
@@ -540,7 +587,7 @@ int64_t decode(const voxel_gpu_encoded_col* e, uint64_t off, uint64_t cnt, void*
 template <typename T>
 void launch_dict_scan(const voxel_gpu_encoded_col* e, PredDev pd, ScanOut* parts, unsigned blocks, StreamCtx* c) {
     hipLaunchKernelGGL((k_scan_dict<T>), dim3(blocks), dim3(kBlock), 0, c->stream, static_cast<const T*>(e->values),
-                       (uint32_t)e->nvalues, e->index, e->rows, pd, parts);
+                       (uint32_t)e->nvalues, e->index, e->rows, pd, parts, enc_bufs().flag_dev);
 }
 template <typename T>
 void launch_rle_scan(const voxel_gpu_encoded_col* e, PredDev pd, ScanOut* parts, unsigned blocks, StreamCtx* c) {
@@ -597,6 +644,7 @@ int voxel_gpu_scan_encoded(const voxel_gpu_encoded_col* col, int op, uint64_t t,
         unsigned blocks = 1;
         switch (col->kind) {
         case VOXEL_ENC_DICTIONARY:
+            if (!dict_flag(c)) return -1;
             blocks = grid_for(col->rows / 4 + 1, kBlock * kEncUnroll, cap);
             if (col->dtype == VOXEL_GPU_F64) launch_dict_scan<double>(col, pd, parts, blocks, c);
             else if (col->dtype == VOXEL_GPU_F32) launch_dict_scan<float>(col, pd, parts, blocks, c);
@@ -629,8 +677,14 @@ int voxel_gpu_scan_encoded(const voxel_gpu_encoded_col* col, int op, uint64_t t,
         if (!hip_ok(hipGetLastError(), "encoded scan launch")) return -1;
     }
     if (!hip_ok(hipMemcpyAsync(&c->host->scan, &c->dev->scan, sizeof(ScanOut), hipMemcpyDeviceToHost, c->stream),
-                "read back result") || !wait(c))
+                "read back result"))
         return -1;
+    if (col->kind == VOXEL_ENC_DICTIONARY &&
+        !hip_ok(hipMemcpyAsync(enc_bufs().flag_host, enc_bufs().flag_dev, 4, hipMemcpyDeviceToHost, c->stream),
+                "read back"))
+        return -1;
+    if (!wait(c)) return -1;
+    if (col->kind == VOXEL_ENC_DICTIONARY && !dict_codes_ok(col)) return -1;
     fill_agg(adt, c->host->scan, out);
     return 0;
 }
