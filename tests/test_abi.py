@@ -246,3 +246,30 @@ def test_lowering_accepts_window_word_loops():
     assert lowered(loop(W(0x97, rd=0, ra=1, imm=1))) == 0              # WSTD needs a window of 2 or more
     assert lowered(loop(W(0x99, rd=0, ra=1, imm=5 | (3 << 5)))) == 2   # WQUANTILE, window 5, q = 3
     assert lowered(loop(W(0x99, rd=0, ra=1, imm=1))) == 0              # WQUANTILE window 1: a no-op
+
+
+def test_import_voxel_py_resolves_to_the_backend():
+    """`import voxel_py` (the reference's module name, bindings/voxel_bindings.cpp:25,
+    imported as `_vx` by bindings/voxel_query.py:8 and voxel_market.py:13)
+    yields voxel_amd.voxel_py, with every name those callers use and the
+    pybind module's enums and constants."""
+    import voxel_py as _vx
+    from voxel_amd import voxel_py as impl
+    assert _vx is impl
+    for name in ("Instruction", "EngineF64", "EngineI64", "CMP_EQ", "CMP_NE", "CMP_LT", "CMP_LE", "CMP_GT",
+                 "CMP_GE", "HashAggregator", "Arena", "sort_ascending", "topk_select", "HashJoin",
+                 "VectorFilterF64", "jit_run", "jit_compile", "NullBitmap", "SegmentF64", "cpu_info"):
+        assert hasattr(_vx, name), name
+    assert (_vx.CMP_EQ, _vx.CMP_NE, _vx.CMP_LT, _vx.CMP_LE, _vx.CMP_GT, _vx.CMP_GE) == (0, 1, 2, 3, 4, 5)
+    assert _vx.Float64 == _vx.DataType.Float64 == 11 and _vx.Timestamp == 32 and _vx.VSUM == 0xD0
+    assert _vx.EngineF64.k_lanes == 4 and _vx.EngineI64.k_lanes == 4
+    seg = _vx.SegmentF64(np.arange(5.0))
+    assert len(seg) == seg.size == 5 and seg[3] == 3.0
+    nb = _vx.NullBitmap(100)
+    nb.set_null(7)
+    assert nb.is_null(7) and not nb.is_null(8) and nb.null_count() == 1 and nb.valid_count() == 99
+    info = _vx.cpu_info()
+    assert set(info) == {"logical_cores", "physical_cores", "l1_cache", "l2_cache", "l3_cache", "has_avx2",
+                         "has_avx512", "has_fma"}
+    with pytest.raises(AttributeError, match="StreamingEngine"):
+        _vx.StreamingEngine

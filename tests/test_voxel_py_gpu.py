@@ -85,3 +85,32 @@ def test_voxel_query_group_sort_topk(gpu):
     assert np.array_equal(idx, np.argsort(x, kind="stable").astype(np.uint32))
     top = vq.from_numpy(x).topk(5).run()
     assert np.array_equal(top, np.sort(x)[::-1][:5])
+
+
+def test_import_voxel_py_as_the_reference_callers_do(gpu, golden, x42):
+    """`import voxel_py as _vx` (bindings/voxel_query.py:8) resolves to this
+    backend: a program built with _vx.Instruction (the words voxel_query's
+    sum() emits, bindings/voxel_query.py:74-110) runs on _vx.EngineF64 and
+    gives the reference's bits; _vx.jit_run gives the JIT's golden result
+    within the reference's own 1e-12 criterion (validate.cpp:68)."""
+    import voxel_py as _vx
+    I = _vx.Instruction
+    code = [I.mov(4, 4).raw, I.vload(0, 1, 0, 0).raw, I.vfilter_gt(1, 0, 3).raw, I.vsum(5, 1).raw,
+            I.addf(0, 0, 5).raw, I.add(1, 1, 4).raw, I.cmp(1, 2).raw, I.jnz(-6).raw, I.halt().raw]
+    assert code == [int(w, 16) for w in golden["programs"]["vq_sum"]]
+    e = _vx.EngineF64()
+    e.add_segment(x42)
+    e.set_scalar(2, x42.size)
+    e.set_scalar_f64(3, 500.0)
+    e.load_program(code)
+    e.run()
+    exact = float(o.relational(x42, 4, 500.0)["sum"])
+    got = e.get_scalar_f64(0)
+    assert abs(got - exact) <= 2 * o.ulp(exact)
+    assert e.get_scalar(1) == x42.size
+    ref = o.bits_f64(int(golden["seed42"]["vq_sum"]["R0"], 16))
+    assert abs(got - ref) <= 1e-12 * abs(ref)
+    fast = [int(w, 16) for w in golden["programs"]["fast"]]
+    j = _vx.jit_run(fast, x42, 500.0, x42.size)
+    ref = o.bits_f64(int(golden["seed42"]["jit"]["result"], 16))
+    assert abs(j - ref) <= 1e-12 * abs(ref) and abs(j - exact) <= 2 * o.ulp(exact)

@@ -16,6 +16,8 @@ Differences from the pybind module (documented per member):
 from __future__ import annotations
 
 import ctypes as C
+import enum
+import os
 
 import numpy as np
 
@@ -27,11 +29,24 @@ __version__ = "1.0.0"
 CMP_EQ, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE = 0, 1, 2, 3, 4, 5
 
 
-class Opcode:
-    """The opcode values the pybind module exports (voxel_bindings.cpp:39-47)."""
+class DataType(enum.IntEnum):
+    """Column types the pybind module exports (voxel_bindings.cpp:29-39;
+    values of core/types.hpp:32-53)."""
+    Null, Bool, Int8, Int16, Int32, Int64 = 0, 1, 2, 3, 4, 5
+    Uint8, Uint16, Uint32, Uint64, Float32, Float64 = 6, 7, 8, 9, 10, 11
+    Decimal128, String, Binary, Date, Timestamp = 14, 20, 21, 30, 32
+
+
+class Opcode(enum.IntEnum):
+    """The opcode values the pybind module exports (voxel_bindings.cpp:42-50)."""
     NOP, HALT, MOV, ADD, SUB, MUL = 0x00, 0x01, 0x10, 0x20, 0x21, 0x22
     CMP, JMP, JZ, JNZ = 0x40, 0x50, 0x51, 0x52
     VLOAD, VSTORE, VADD, VFILTER_GT, VSUM, ADDF = 0x70, 0x71, 0x80, 0xC5, 0xD0, 0x2A
+
+
+# py::enum_::export_values(): the members are module attributes too
+globals().update({m.name: m for m in DataType})
+globals().update({m.name: m for m in Opcode})
 
 
 def _word(op: int, rd: int = 0, ra: int = 0, rb: int = 0, imm: int = 0) -> int:
@@ -440,3 +455,127 @@ class HashJoin:
     def contains(self, key: int) -> bool:
         _, _, n = self.probe(np.array([key], np.uint32))
         return n > 0
+
+
+class SegmentF64:
+    """Segment<f64> (data/segment.hpp:25-60; voxel_bindings.cpp:97-102): a
+    borrowed view of f64 rows.  This one keeps its array alive."""
+
+    def __init__(self, data=None, size=None):
+        if data is None:
+            self._a = np.zeros(0, np.float64)
+        else:
+            a = _as_column(data, np.float64)
+            self._a = a if size is None else a[: int(size)]
+
+    def __getitem__(self, i: int) -> float:
+        return float(self._a[i])
+
+    def __len__(self) -> int:
+        return int(self._a.size)
+
+    @property
+    def size(self) -> int:
+        return int(self._a.size)
+
+
+class NullBitmap:
+    """voxel::NullBitmap through the C ABI's voxel_null_* (bit 1 = valid;
+    voxel_bindings.cpp:192-200)."""
+
+    def __init__(self, count: int):
+        self._n = int(count)
+        self._h = lib.voxel_null_create(self._n)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.voxel_null_destroy(h)
+            self._h = None
+
+    def is_null(self, row: int) -> bool:
+        return bool(lib.voxel_null_is_null(self._h, row))
+
+    def set_null(self, row: int) -> None:
+        lib.voxel_null_set_null(self._h, row)
+
+    def set_valid(self, row: int) -> None:
+        lib.voxel_null_set_valid(self._h, row)
+
+    def null_count(self) -> int:
+        return int(lib.voxel_null_null_count(self._h))
+
+    def valid_count(self) -> int:
+        return int(lib.voxel_null_valid_count(self._h))
+
+    def set_all_valid(self) -> None:
+        for r in range(self._n):
+            lib.voxel_null_set_valid(self._h, r)
+
+    def set_all_null(self) -> None:
+        for r in range(self._n):
+            lib.voxel_null_set_null(self._h, r)
+
+
+def cpu_info() -> dict:
+    """platform::QueryCpuInfo's fields (voxel_bindings.cpp:313-324) for the
+    host this process runs on (Linux /proc and /sys)."""
+    flags, phys = set(), set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = core = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "flags" and not flags:
+                    flags = set(v.split())
+                elif k == "physical id":
+                    pid = v.strip()
+                elif k == "core id":
+                    core = v.strip()
+                    phys.add((pid, core))
+    except OSError:
+        pass
+
+    def cache(level: int) -> int:
+        base = "/sys/devices/system/cpu/cpu0/cache"
+        try:
+            for d in sorted(os.listdir(base)):
+                p = os.path.join(base, d)
+                with open(os.path.join(p, "level")) as f:
+                    if int(f.read()) != level:
+                        continue
+                with open(os.path.join(p, "type")) as f:
+                    if f.read().strip() == "Instruction":
+                        continue
+                with open(os.path.join(p, "size")) as f:
+                    t = f.read().strip()
+                return int(t[:-1]) * 1024 if t.endswith("K") else int(t)
+        except (OSError, ValueError):
+            pass
+        return 0
+
+    logical = os.cpu_count() or 0
+    return {"logical_cores": logical, "physical_cores": len(phys) or logical,
+            "l1_cache": cache(1), "l2_cache": cache(2), "l3_cache": cache(3),
+            "has_avx2": "avx2" in flags, "has_avx512": "avx512f" in flags, "has_fma": "fma" in flags}
+
+
+# Members of the pybind module that are outside this backend's path: host
+# data-format, I/O and profiling classes.  Asking for one names the reason.
+_ABSENT = {
+    "DictionaryEncodingF64": "host-side column encodings are in voxel_amd.encoding (GPU DecodeBatch and scans)",
+    "RLEEncodingF64": "host-side column encodings are in voxel_amd.encoding (GPU DecodeBatch and scans)",
+    "ThreadPool": "host thread pools run no query here (the C ABI's voxel_pool_* remain)",
+    "InstructionProfiler": "the per-opcode cycle profiler belongs to the CPU interpreter",
+    "MmapSegment": "file I/O (mmap segments) is outside the GPU execution path",
+    "ChunkedReader": "file I/O (chunked readers) is outside the GPU execution path",
+    "StreamingEngine": "the reference's StreamingEngine never resets its offset registers (streaming.hpp:49-55); "
+                       "batches of rows go through EngineF64.run",
+}
+
+
+def __getattr__(name):
+    if name in _ABSENT:
+        raise AttributeError("voxel_py.%s is not provided by the GPU backend: %s" % (name, _ABSENT[name]))
+    raise AttributeError("module 'voxel_py' has no attribute %r" % name)
