@@ -107,6 +107,14 @@ _SIGS = {
     "vo_hashjoin_i64": (U64, [P, SZ, P, P, SZ, P, P, P]),
     "vo_groupby_f64": (SZ, [P, P, SZ, INT, SZ, P, P, P, P, P, P, P, P]),
     "vo_splitmix64": (U64, [U64]),
+    "vo_hash_u64": (U64, [U64]),
+    "vo_hash_element": (U64, [U64, INT]),
+    "vo_value_key": (U64, [U64, INT, INT]),
+    "vo_partial_sort": (None, [P, U64, U64, INT, INT]),
+    "vo_join_count": (I64, [INT, P, U64, P, U64, INT]),
+    "vo_relw_create": (P, []),
+    "vo_relw_destroy": (None, [P]),
+    "vo_relw_run": (INT, [P, P, P, INT, INT, P, SZ, P]),
 }
 for _n, (_r, _a) in _SIGS.items():
     _f = getattr(O, _n)
@@ -160,6 +168,8 @@ _RSIGS = {
     "vr_for_decode": (None, [F64, F64, P, SZ, SZ, SZ, P]),
     "vr_nullable_i64": (None, [P, SZ, P, P, P, P]),
     "vr_time_window_sum_f64": (I64, [P, P, SZ, I64, I64, P, SZ]),
+    "vr_engine_run_segs_f64": (INT, [P, P, INT, P, SZ, P]),
+    "vr_engine_run_segs_i64": (INT, [P, P, INT, P, SZ, P]),
 }
 if R is not None:
     for _n, (_r, _a) in _RSIGS.items():
@@ -244,3 +254,11 @@ def bits_f64(b: int) -> float:
 
 def ulp(x: float) -> float:
     return float(np.spacing(np.float64(abs(x)))) if x != 0 else 5e-324
+
+
+def seg_ptrs(segs):
+    """(array of segment pointers, array of row counts) for vo_relw_run /
+    vr_engine_run_segs_*; the arrays must stay alive for the call."""
+    p = np.array([a.ctypes.data for a in segs], np.uint64)
+    n = np.array([a.size for a in segs], np.uint64)
+    return p, n

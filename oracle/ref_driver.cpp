@@ -17,6 +17,17 @@ using namespace voxel;
 
 namespace {
 template <typename T>
+int run_engine_segs(T* const* data, const size_t* n, int nseg, const uint32_t* code, size_t len, uint64_t* regs) {
+    Engine<T> e;
+    for (int s = 0; s < nseg; ++s) e.AddSegment(data[s], n[s]);
+    for (size_t r = 0; r < 64; ++r) e.ScalarReg(r) = regs[r];
+    std::vector<u32> v(code, code + len);
+    e.LoadProgram(v);
+    e.Run();
+    for (size_t r = 0; r < 64; ++r) regs[r] = e.ScalarReg(r);
+    return 0;
+}
+template <typename T>
 int run_engine(T* data, size_t n, const uint32_t* code, size_t len, uint64_t* regs) {
     Engine<T> e;
     e.AddSegment(data, n);
@@ -115,6 +126,14 @@ void vr_mt_uniform_i32(uint64_t seed, size_t n, int32_t lo, int32_t hi, int32_t*
 int vr_engine_run_f64(double* d, size_t n, const uint32_t* c, size_t l, uint64_t* r) { return run_engine<f64>(d, n, c, l, r); }
 int vr_engine_run_i64(int64_t* d, size_t n, const uint32_t* c, size_t l, uint64_t* r) { return run_engine<i64>(d, n, c, l, r); }
 int vr_engine_run_i32(int32_t* d, size_t n, const uint32_t* c, size_t l, uint64_t* r) { return run_engine<i32>(d, n, c, l, r); }
+// The same over several segments (segment s = data[s], n[s] rows); the
+// engine writes the segments in place as its sort words do.
+int vr_engine_run_segs_f64(double* const* d, const size_t* n, int ns, const uint32_t* c, size_t l, uint64_t* r) {
+    return run_engine_segs<f64>(d, n, ns, c, l, r);
+}
+int vr_engine_run_segs_i64(int64_t* const* d, const size_t* n, int ns, const uint32_t* c, size_t l, uint64_t* r) {
+    return run_engine_segs<i64>(d, n, ns, c, l, r);
+}
 
 // Engine<f64>::RunParallel, registers in/out.
 void vr_engine_run_parallel_f64(double* d, size_t n, const uint32_t* c, size_t l, uint64_t* regs,

@@ -1374,3 +1374,300 @@ int vo_tier0_window_f64(const double* data, uint64_t count, uint64_t off, uint64
     } while (off != total);
     return 1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Hash / sort / join words of straight-line programs (Engine<T>,           */
+/* engine.hpp:704-842 handlers, 1051-1071 HashElement/HashU64; dispatch of  */
+/* 0xEE-0xFA, 0xFD at engine.hpp:3286-3355).  Segments are u64 bit arrays   */
+/* of the engine type (f64 or i64) and are written in place by the sort     */
+/* words, as the reference writes its borrowed segments.                    */
+/*   HASH_INIT/HASH_BUILD (0xEE/0xF0): slot rd & 15 := the distinct          */
+/*     HashElement values of segment ra with the first row of each;         */
+/*     R[rd] = their number (unordered_map size).                           */
+/*   HASH_PROBE/HASH_LOOKUP (0xEF/0xF1): R[rd] = first row whose hash        */
+/*     equals HashElement(R[rb] as T) in slot ra, else 2^64 - 1.            */
+/*   SORT_ASC/SORT_DESC (0xF2/0xF3): std::sort / std::sort(greater) of the   */
+/*     segment; R[rd] = Count.  Values are sorted by the total order of     */
+/*     their bits (-0.0 before +0.0 ascending; the reference leaves equal   */
+/*     elements in unspecified order); NaN is refused (-1).                 */
+/*   SORT_TOPK/SORT_BOTTOMK (0xF4/0xF5): std::partial_sort with greater /    */
+/*     less over k = min((size_t)R[rb], imm & 0xFFF) rows, restated step by */
+/*     step from libstdc++ (GCC 11, bits/stl_heap.h __make_heap,            */
+/*     __adjust_heap, __push_heap, __pop_heap, __sort_heap; bits/stl_algo.h */
+/*     __heap_select, __partial_sort), so rows past k keep libstdc++'s      */
+/*     order; k == 0 does nothing; k > Count is refused (-1: the reference  */
+/*     reads past the segment there).  R[rd] = k.                           */
+/*   JOIN_HASH/JOIN_NESTED (0xF6/0xF8): number of (left, right) pairs with   */
+/*     equal values (== : NaN never equal, -0.0 == +0.0).                   */
+/*   JOIN_MERGE (0xF7): sum over values of min(left count, right count) --  */
+/*     the merge walk over sorted copies; NaN refused (-1).                 */
+/*   JOIN_SEMI/JOIN_ANTI (0xFA/0xF9): left rows whose value is / is not      */
+/*     equal to some right value.                                           */
+/*   PARTITION_HASH (0xFD): R[rd] = Count.                                  */
+/* A word naming a missing segment does nothing (PC++ only).                */
+/* ------------------------------------------------------------------------ */
+uint64_t vo_hash_u64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xFF51AFD7ED558CCDull;
+    x ^= x >> 33;
+    x *= 0xC4CEB9FE1A85EC53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+static int vo_bits_nan(uint64_t b) { return ((b >> 52) & 0x7FF) == 0x7FF && (b & 0xFFFFFFFFFFFFFull) != 0; }
+
+uint64_t vo_hash_element(uint64_t bits, int is_f64) {
+    if (is_f64 && vo_bits_nan(bits)) return 0;
+    return vo_hash_u64(bits);
+}
+
+typedef struct {
+    uint64_t* h;      /* distinct hashes, ascending */
+    uint64_t* row;    /* first row of each */
+    uint64_t n;
+    int valid;
+} vo_htab;
+
+typedef struct {
+    vo_htab t[16];
+} vo_relstate;
+
+void* vo_relw_create(void) { return calloc(1, sizeof(vo_relstate)); }
+
+void vo_relw_destroy(void* p) {
+    vo_relstate* s = (vo_relstate*)p;
+    int i;
+    if (!s) return;
+    for (i = 0; i < 16; ++i) { free(s->t[i].h); free(s->t[i].row); }
+    free(s);
+}
+
+typedef struct { uint64_t h, row; } vo_hrow;
+static int vo_hrow_cmp(const void* a, const void* b) {
+    const vo_hrow* x = (const vo_hrow*)a; const vo_hrow* y = (const vo_hrow*)b;
+    if (x->h != y->h) return x->h < y->h ? -1 : 1;
+    return x->row < y->row ? -1 : (x->row > y->row);
+}
+
+/* Value order of T as an unsigned key: f64 total order of the bits (-0.0
+ * below +0.0, NaN at the ends), i64 signed order.  canon: -0.0 -> +0.0. */
+uint64_t vo_value_key(uint64_t b, int is_f64, int canon) {
+    if (!is_f64) return b ^ 0x8000000000000000ull;
+    if (canon && b == 0x8000000000000000ull) b = 0;
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+static uint64_t vo_value_unkey(uint64_t k, int is_f64) {
+    if (!is_f64) return k ^ 0x8000000000000000ull;
+    return (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+}
+static int vo_u64_cmp(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : (x > y);
+}
+
+/* comparator of the partial sort: desc = std::greater<T>, else std::less<T> */
+static int vo_cmp_t(uint64_t a, uint64_t b, int is_f64, int desc) {
+    if (is_f64) {
+        double x, y;
+        memcpy(&x, &a, 8); memcpy(&y, &b, 8);
+        return desc ? x > y : x < y;
+    }
+    return desc ? (int64_t)a > (int64_t)b : (int64_t)a < (int64_t)b;
+}
+
+static void vo_push_heap(uint64_t* f, long hole, long top, uint64_t v, int F, int desc) {
+    long parent = (hole - 1) / 2;
+    while (hole > top && vo_cmp_t(f[parent], v, F, desc)) {
+        f[hole] = f[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    f[hole] = v;
+}
+
+static void vo_adjust_heap(uint64_t* f, long hole, long len, uint64_t v, int F, int desc) {
+    const long top = hole;
+    long child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (vo_cmp_t(f[child], f[child - 1], F, desc)) child--;
+        f[hole] = f[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        f[hole] = f[child - 1];
+        hole = child - 1;
+    }
+    vo_push_heap(f, hole, top, v, F, desc);
+}
+
+void vo_partial_sort(uint64_t* f, uint64_t k, uint64_t n, int is_f64, int desc) {
+    long len = (long)k, parent;
+    uint64_t i;
+    if (len >= 2) {                                   /* __make_heap */
+        parent = (len - 2) / 2;
+        for (;;) {
+            vo_adjust_heap(f, parent, len, f[parent], is_f64, desc);
+            if (parent == 0) break;
+            parent--;
+        }
+    }
+    for (i = k; i < n; ++i)                           /* __heap_select */
+        if (vo_cmp_t(f[i], f[0], is_f64, desc)) {     /* __pop_heap(first, middle, i) */
+            uint64_t v = f[i];
+            f[i] = f[0];
+            vo_adjust_heap(f, 0, len, v, is_f64, desc);
+        }
+    while (len > 1) {                                 /* __sort_heap */
+        uint64_t v;
+        --len;
+        v = f[len];
+        f[len] = f[0];
+        vo_adjust_heap(f, 0, len, v, is_f64, desc);
+    }
+}
+
+/* number of rows of sorted keys s[0, n) equal to key: [lo, hi) */
+static void vo_equal_range(const uint64_t* s, uint64_t n, uint64_t key, uint64_t* lo, uint64_t* hi) {
+    uint64_t a = 0, b = n;
+    while (a < b) { uint64_t m = a + (b - a) / 2; if (s[m] < key) a = m + 1; else b = m; }
+    *lo = a;
+    b = n;
+    while (a < b) { uint64_t m = a + (b - a) / 2; if (s[m] <= key) a = m + 1; else b = m; }
+    *hi = a;
+}
+
+/* kind: 0 pairs (JOIN_HASH / JOIN_NESTED), 1 merge, 2 semi, 3 anti.
+ * Returns -1 for JOIN_MERGE over NaN. */
+int64_t vo_join_count(int kind, const uint64_t* l, uint64_t nl, const uint64_t* r, uint64_t nr, int is_f64) {
+    uint64_t* rs;
+    uint64_t* ls = NULL;
+    uint64_t i, m = 0, total = 0;
+    if (nl == 0 || nr == 0) return kind == 3 ? (int64_t)nl : 0;   /* no pair to test: defined even with NaN */
+    rs = (uint64_t*)malloc(nr * 8);
+    for (i = 0; i < nr; ++i) {
+        if (is_f64 && vo_bits_nan(r[i])) { if (kind == 1) { free(rs); return -1; } continue; }
+        rs[m++] = vo_value_key(r[i], is_f64, 1);
+    }
+    qsort(rs, m, 8, vo_u64_cmp);
+    if (kind == 1) {
+        uint64_t nls = 0;
+        ls = (uint64_t*)malloc((nl ? nl : 1) * 8);
+        for (i = 0; i < nl; ++i) {
+            if (is_f64 && vo_bits_nan(l[i])) { free(rs); free(ls); return -1; }
+            ls[nls++] = vo_value_key(l[i], is_f64, 1);
+        }
+        qsort(ls, nls, 8, vo_u64_cmp);
+        for (i = 0; i < nls; ++i) {             /* the i-th of its run matches while the right run lasts */
+            uint64_t llo, lhi, rlo, rhi;
+            vo_equal_range(ls, nls, ls[i], &llo, &lhi);
+            vo_equal_range(rs, m, ls[i], &rlo, &rhi);
+            if (i - llo < rhi - rlo) total++;
+        }
+        free(ls);
+        free(rs);
+        return (int64_t)total;
+    }
+    for (i = 0; i < nl; ++i) {
+        uint64_t lo, hi;
+        if (is_f64 && vo_bits_nan(l[i])) { if (kind == 3) total++; continue; }
+        vo_equal_range(rs, m, vo_value_key(l[i], is_f64, 1), &lo, &hi);
+        if (kind == 0) total += hi - lo;
+        else if (kind == 2) total += hi > lo;
+        else total += hi == lo;
+    }
+    free(rs);
+    return (int64_t)total;
+}
+
+/* Straight-line program of scalar words and the words above over nseg
+ * segments; returns 1 at HALT / end, 0 when a DIV/MOD by zero stopped it,
+ * -1 for a refused word (segments may already be sorted by earlier words). */
+int vo_relw_run(void* state, uint64_t* const* segs, const uint64_t* counts, int nseg, int is_f64, const uint32_t* w,
+                size_t n, uint64_t* r) {
+    vo_relstate* st = (vo_relstate*)state;
+    size_t k;
+    for (k = 0; k < n; ++k) {
+        const uint32_t x = w[k];
+        const unsigned op = x & 0xFF, rd = (x >> 8) & 0xF, ra = (x >> 12) & 0xF, rb = (x >> 16) & 0xF;
+        const unsigned imm = (x >> 20) & 0xFFF;
+        if (op == 0x01) return 1;
+        switch (op) {
+        case 0xEE: case 0xF0: {                                  /* HASH_INIT / HASH_BUILD */
+            vo_htab* t;
+            vo_hrow* hr;
+            uint64_t i, c, m = 0;
+            if ((int)ra >= nseg) break;
+            t = &st->t[rd & 0xF];
+            c = counts[ra];
+            hr = (vo_hrow*)malloc((c ? c : 1) * sizeof(vo_hrow));
+            for (i = 0; i < c; ++i) { hr[i].h = vo_hash_element(segs[ra][i], is_f64); hr[i].row = i; }
+            qsort(hr, c, sizeof(vo_hrow), vo_hrow_cmp);
+            free(t->h); free(t->row);
+            t->h = (uint64_t*)malloc((c ? c : 1) * 8);
+            t->row = (uint64_t*)malloc((c ? c : 1) * 8);
+            for (i = 0; i < c; ++i)
+                if (i == 0 || hr[i].h != hr[i - 1].h) { t->h[m] = hr[i].h; t->row[m] = hr[i].row; m++; }
+            t->n = m;
+            t->valid = 1;
+            free(hr);
+            r[rd] = m;
+            break;
+        }
+        case 0xEF: case 0xF1: {                                  /* HASH_PROBE / HASH_LOOKUP */
+            const vo_htab* t = &st->t[ra & 0xF];
+            uint64_t res = ~0ull;
+            if (t->valid) {
+                const uint64_t h = vo_hash_element(r[rb], is_f64);
+                uint64_t lo, hi;
+                vo_equal_range(t->h, t->n, h, &lo, &hi);
+                if (hi > lo) res = t->row[lo];
+            }
+            r[rd] = res;
+            break;
+        }
+        case 0xF2: case 0xF3: {                                  /* SORT_ASC / SORT_DESC */
+            uint64_t i, c;
+            if ((int)ra >= nseg) break;
+            c = counts[ra];
+            for (i = 0; i < c; ++i) {
+                if (is_f64 && vo_bits_nan(segs[ra][i])) return -1;
+                segs[ra][i] = vo_value_key(segs[ra][i], is_f64, 0);
+            }
+            qsort(segs[ra], c, 8, vo_u64_cmp);
+            for (i = 0; i < c; ++i) segs[ra][i] = vo_value_unkey(segs[ra][i], is_f64);
+            if (op == 0xF3)
+                for (i = 0; i < c / 2; ++i) { uint64_t tmp = segs[ra][i]; segs[ra][i] = segs[ra][c - 1 - i]; segs[ra][c - 1 - i] = tmp; }
+            r[rd] = c;
+            break;
+        }
+        case 0xF4: case 0xF5: {                                  /* SORT_TOPK (greater) / SORT_BOTTOMK (less) */
+            uint64_t kk = r[rb] < (uint64_t)imm ? r[rb] : (uint64_t)imm;
+            if ((int)ra >= nseg || kk == 0) break;
+            if (kk > counts[ra]) return -1;
+            vo_partial_sort(segs[ra], kk, counts[ra], is_f64, op == 0xF4);
+            r[rd] = kk;
+            break;
+        }
+        case 0xF6: case 0xF7: case 0xF8: case 0xF9: case 0xFA: { /* JOIN_* */
+            static const int kind_of[5] = {0, 1, 0, 3, 2};
+            int64_t m;
+            if ((int)ra >= nseg || (int)rb >= nseg) break;
+            m = vo_join_count(kind_of[op - 0xF6], segs[ra], counts[ra], segs[rb], counts[rb], is_f64);
+            if (m < 0) return -1;
+            r[rd] = (uint64_t)m;
+            break;
+        }
+        case 0xFD:                                               /* PARTITION_HASH */
+            if ((int)ra < nseg) r[rd] = counts[ra];
+            break;
+        default: {
+            int rc = vo_scalar_run(&w[k], 1, r);
+            if (rc != 1) return rc;
+        }
+        }
+    }
+    return 1;
+}

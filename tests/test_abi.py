@@ -93,6 +93,18 @@ def test_lowering_classifies_reference_programs(golden):
     assert lowered(bad) == 0
 
 
+def test_lowering_accepts_hash_sort_join_words():
+    """Straight-line programs of HASH_*, SORT_*, JOIN_* and PARTITION_HASH
+    words (with scalar and AGG_* words) lower to the straight-line plan;
+    SERIALIZE / DESERIALIZE / WINDOW_ROW / WINDOW_RANGE do not."""
+    from tests import tier0_gen as tg
+    for op in tg.REL_OPS:
+        assert lowered([tg.w(0x10, rd=13, imm=5), tg.w(op, rd=2, ra=0, rb=1), tg.w(0xE1, rd=4, ra=0, rb=1), 1]) == 3
+        assert lowered([tg.w(op, rd=2, ra=0, rb=13, imm=7), 1], i64=True) == 3
+    for op in (0xFB, 0xFC, 0xFE, 0xFF):
+        assert lowered([tg.w(op, rd=2, ra=0, rb=1), 1]) == 0
+
+
 def test_lowering_accepts_filter_chains_and_several_reductions():
     from tests import tier0_gen as tg
     rng = np.random.default_rng(5)

@@ -816,3 +816,113 @@ def test_tier0_window_restatement_against_reference_engine():
         for b in range(len(spec["red"])):
             assert int(ref[tg.ACCS[b]]) == acc[b], (trial, wkind, b, spec)
             assert int(ref[tg.PARTS[b]]) == part[b], (trial, wkind, b, spec)
+
+
+def _brute_join(kind, l, r):
+    """Counts by Python == over floats / ints (NaN never equal, -0.0 == 0.0)."""
+    if kind == 0:
+        return sum(sum(1 for b in r if a == b) for a in l)
+    if kind == 1:
+        from collections import Counter
+        cl = Counter(float(a) if isinstance(a, float) else a for a in l if a == a)
+        cr = Counter(float(b) if isinstance(b, float) else b for b in r if b == b)
+        return sum(min(c, cr.get(v, 0)) for v, c in cl.items())
+    found = [any(a == b for b in r) for a in l]
+    return sum(found) if kind == 2 else len(l) - sum(found)
+
+
+def test_join_count_and_partial_sort_closed_forms():
+    """vo_join_count against brute-force == loops (pairs, merge, semi, anti;
+    NaN never equal, -0.0 == +0.0); vo_partial_sort: the first k rows are
+    the k largest (smallest) in order, the whole stays a permutation."""
+    rng = np.random.default_rng(31)
+    for trial in range(60):
+        is_f64 = trial % 2 == 0
+        nl, nr = int(rng.integers(0, 60)), int(rng.integers(0, 60))
+        if is_f64:
+            l = np.round(rng.standard_normal(nl) * 3)
+            r = np.round(rng.standard_normal(nr) * 3)
+            if trial % 4 == 0 and nl and nr:
+                l[0], r[-1] = np.nan, -0.0
+        else:
+            l = rng.integers(-5, 6, nl).astype(np.int64)
+            r = rng.integers(-5, 6, nr).astype(np.int64)
+        lb, rb = l.view(np.uint64), r.view(np.uint64)
+        for kind in range(4):
+            got = o.O.vo_join_count(kind, o.ptr(lb), nl, o.ptr(rb), nr, int(is_f64))
+            if kind == 1 and is_f64 and (np.isnan(l).any() or np.isnan(r).any()):
+                assert got == -1
+                continue
+            assert got == _brute_join(kind, l.tolist(), r.tolist()), (trial, kind)
+        if nl:
+            k = int(rng.integers(1, nl + 1))
+            for desc in (1, 0):
+                a = l.copy()
+                if is_f64:
+                    a = a[~np.isnan(a)] + 0.0
+                if a.size < k:
+                    continue
+                b = a.view(np.uint64).copy()
+                o.O.vo_partial_sort(o.ptr(b), k, b.size, int(is_f64), desc)
+                v = b.view(a.dtype)
+                want = np.sort(a)[::-1][:k] if desc else np.sort(a)[:k]
+                assert np.array_equal(v[:k], want) and np.array_equal(np.sort(v), np.sort(a))
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_hash_sort_join_words_restatement_against_reference_engine(is_f64):
+    """vo_relw_run (HASH_INIT/BUILD/PROBE/LOOKUP, SORT_ASC/DESC/TOPK/BOTTOMK,
+    JOIN_HASH/MERGE/NESTED/ANTI/SEMI, PARTITION_HASH in straight-line
+    programs over up to three segments) against the reference engine
+    running the same words: every register and every segment row after the
+    run bit-identical.  Inputs hold ties, NaN and -0.0 except where the
+    reference's result is unspecified (std::sort's order of -0.0 and +0.0,
+    and any order with NaN, in SORT_ASC/DESC and JOIN_MERGE); partial sorts
+    and hash tables see them all."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(57 if is_f64 else 58)
+    runs = 0
+    for trial in range(120):
+        nseg = int(rng.integers(1, 4))
+        full_order = trial % 3 != 0            # programs with SORT_ASC/DESC / JOIN_MERGE
+        segs = []
+        for s in range(nseg):
+            n = int(rng.integers(1, 300))
+            if is_f64:
+                x = np.round(rng.standard_normal(n) * 4, int(rng.integers(0, 2)))
+                if full_order:
+                    x = x + 0.0                      # no -0.0
+                else:
+                    x[rng.integers(0, n, 2)] = np.nan
+            else:
+                x = rng.integers(-20, 20, n).astype(np.int64)
+                x[rng.integers(0, n, 2)] = rng.integers(-2**62, 2**62, 2)
+            segs.append(x)
+        ops = tg.REL_OPS if full_order else [op for op in tg.REL_OPS if op not in (0xF2, 0xF3, 0xF7)]
+        code = tg.rel_program(rng, nseg, k=int(rng.integers(1, 12)), ops=ops,
+                              kmax=min(a.size for a in segs))
+        regs = np.zeros(64, np.uint64)
+        for r in tg.KEY_REGS:
+            src = segs[int(rng.integers(0, nseg))]
+            regs[r] = src.view(np.uint64)[int(rng.integers(0, src.size))] if rng.random() < 0.8 else \
+                np.uint64(rng.integers(0, 2**63))
+        for r in tg.K_REGS:
+            regs[r] = int(rng.integers(0, 5000)) if rng.random() < 0.8 else 2**64 - 1
+        mine = [a.view(np.uint64).copy() for a in segs]
+        p, n = o.seg_ptrs(mine)
+        st = o.O.vo_relw_create()
+        ro = regs.copy()
+        rc = o.O.vo_relw_run(st, o.ptr(p), o.ptr(n), nseg, int(is_f64), o.ptr(code), code.size, o.ptr(ro))
+        o.O.vo_relw_destroy(st)
+        assert rc == 1, (trial, [hex(int(c)) for c in code])
+        theirs = [a.copy() for a in segs]
+        pr, nr = o.seg_ptrs(theirs)
+        rr = regs.copy()
+        (o.R.vr_engine_run_segs_f64 if is_f64 else o.R.vr_engine_run_segs_i64)(
+            o.ptr(pr), o.ptr(nr), nseg, o.ptr(code), code.size, o.ptr(rr))
+        assert np.array_equal(ro, rr), (trial, [hex(int(c)) for c in code], np.nonzero(ro != rr))
+        for a, b in zip(mine, theirs):
+            assert np.array_equal(a, b.view(np.uint64)), (trial, [hex(int(c)) for c in code])
+        runs += 1
+    assert runs == 120

@@ -338,3 +338,36 @@ def window_total(rng, n, wkind):
     if rng.random() < 0.5 or end < 8:
         return end
     return 4 * int(rng.integers(1, (end - 1) // 4 + 1))
+
+
+REL_OPS = [0xEE, 0xEF, 0xF0, 0xF1, 0xF2, 0xF3, 0xF4, 0xF5, 0xF6, 0xF7, 0xF8, 0xF9, 0xFA, 0xFD]
+KEY_REGS = (8, 9, 10, 11, 12)          # probe keys, set by the caller
+K_REGS = (13, 14)                      # row counts of the partial sorts
+
+
+def rel_program(rng, nseg, k=10, ops=REL_OPS, kmax=None):
+    """Straight-line program of hash / sort / join words over nseg
+    segments: hash slots 0..3 (HASH_INIT/BUILD write R0..R3), probes into
+    R4..R7 with keys from KEY_REGS, sorts and partial sorts (k from
+    K_REGS, capped by imm), joins and PARTITION_HASH into R15..R20; now and
+    then a segment id past the last one (the word does nothing).  Ends with
+    HALT."""
+    code = []
+    built = []
+    for _ in range(k):
+        op = int(rng.choice(ops))
+        seg = lambda: int(rng.integers(0, nseg + 1)) if rng.random() < 0.1 else int(rng.integers(0, nseg))  # noqa: E731
+        if op in (0xEE, 0xF0):
+            slot = int(rng.integers(0, 4))
+            built.append(slot)
+            code.append(w(op, rd=slot, ra=seg()))
+        elif op in (0xEF, 0xF1):
+            slot = int(rng.choice(built)) if built and rng.random() < 0.85 else int(rng.integers(0, 6))
+            code.append(w(op, rd=int(rng.integers(4, 8)), ra=slot, rb=int(rng.choice(KEY_REGS))))
+        elif op in (0xF4, 0xF5):
+            imm = int(rng.integers(0, 4096)) if kmax is None else int(rng.integers(0, kmax + 1))
+            code.append(w(op, rd=int(rng.integers(15, 16)), ra=seg(), rb=int(rng.choice(K_REGS)), imm=imm))
+        else:
+            code.append(w(op, rd=int(rng.integers(15, 16)), ra=seg(), rb=seg()))
+    code.append(w(0x01))
+    return np.array(code, np.uint32)

@@ -15,6 +15,7 @@
 #include "vx_lower.hpp"
 #include "vx_runtime.hpp"
 #include "vx_radix.hpp"
+#include "vx_relwords.hpp"
 #include "vx_segagg.hpp"
 
 namespace vx {
@@ -60,6 +61,15 @@ struct SetBuf {
     ~SetBuf() { if (p) (void)hipFree(p); }
 };
 
+// One hash-table slot (HashTables_[slot], engine.hpp:704-744): the table in
+// device buffer `cur` of the slot's two, `n` distinct hashes.  Like the
+// AGG slots, a run works on a copy and builds into the other buffer.
+struct TabSlot {
+    uint64_t n = 0;
+    int cur = 0;
+    bool valid = false;
+};
+
 struct Engine {
     int dtype = DT_F64;
     uint64_t regs[64] = {};
@@ -68,6 +78,9 @@ struct Engine {
     Plan plan;
     AggSlot agg[16];                 // committed slot state
     SetBuf sets[16][2];
+    TabSlot tab[16];                 // committed hash-table slots
+    HashTab tabs[16][2];
+    RelBufs rel;
     bool agg_ready = false;
     SegaggBufs segagg;
     DevColumn expr;                  // a reduced vector of a loop with vector arithmetic
@@ -584,8 +597,108 @@ bool run_agg(Engine& e, uint32_t w, uint64_t* regs, AggSlot* work, StreamCtx* c)
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Hash / sort / join words (engine.hpp:704-842, dispatch 3286-3355) in
+// straight-line programs.  The sort words reorder the segment's rows where
+// the caller keeps them, as the reference's sorts write its borrowed
+// Segment<T>::Data: device segments in place, host segments in HBM and then
+// back to the caller's memory (and the resident mirror stays current).
+// ---------------------------------------------------------------------------
+namespace {
+// Device rows [0, Count) of a segment that a word will reorder in place.
+void* seg_rows_mut(Seg& s, int dtype, StreamCtx* c) {
+    return const_cast<void*>(seg_rows(s, dtype, 0, s.count, c));
+}
+bool seg_write_back(Seg& s, int dtype, const void* dev, StreamCtx* c) {
+    if (s.devext || s.count == 0) return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    void* host = const_cast<void*>(s.host);               // the reference sorts its borrowed rows
+    return sync_read(c, host, dev, s.count * dtype_size(dtype));
+}
+
+bool run_relword(Engine& e, uint32_t w, uint64_t* regs, TabSlot* twork, StreamCtx* c) {
+    const uint8_t op = op_of(w), rd = rd_of(w), ra = ra_of(w), rb = rb_of(w);
+    const uint16_t imm = imm_of(w);
+    const size_t nseg = e.segs.size();
+    switch (op) {
+    case 0xEE:
+    case 0xF0: {                                            // HASH_INIT / HASH_BUILD
+        if (ra >= nseg) return true;
+        Seg& s = *e.segs[ra];
+        const int slot = rd & 0xF;
+        const int out = 1 - e.tab[slot].cur;                // never the committed table
+        const void* d = s.count ? seg_rows(s, e.dtype, 0, s.count, c) : nullptr;
+        if (s.count && !d) return false;
+        uint64_t distinct = 0;
+        if (!relw_hash_build(e.dtype, d, s.count, e.tabs[slot][out], &distinct, e.rel, c)) return false;
+        twork[slot] = TabSlot{distinct, out, true};
+        regs[rd] = distinct;
+        return true;
+    }
+    case 0xEF:
+    case 0xF1: {                                            // HASH_PROBE / HASH_LOOKUP
+        const TabSlot& t = twork[ra & 0xF];
+        uint64_t row = ~0ull;
+        if (t.valid && !relw_hash_probe(e.tabs[ra & 0xF][t.cur], t.n, relw_hash_element(regs[rb], e.dtype), &row,
+                                        e.rel, c))
+            return false;
+        regs[rd] = row;
+        return true;
+    }
+    case 0xF2:
+    case 0xF3: {                                            // SORT_ASC / SORT_DESC
+        if (ra >= nseg) return true;
+        Seg& s = *e.segs[ra];
+        if (s.count >= 2) {
+            void* d = seg_rows_mut(s, e.dtype, c);
+            if (!d || !relw_sort(e.dtype, d, s.count, op == 0xF3, e.rel, c) || !seg_write_back(s, e.dtype, d, c))
+                return false;
+        }
+        regs[rd] = s.count;
+        return true;
+    }
+    case 0xF4:
+    case 0xF5: {                                            // SORT_TOPK (greater) / SORT_BOTTOMK (less)
+        const uint64_t k = regs[rb] < (uint64_t)imm ? regs[rb] : (uint64_t)imm;
+        if (ra >= nseg || k == 0) return true;
+        Seg& s = *e.segs[ra];
+        if (k > s.count) {
+            set_error("SORT_TOPK / SORT_BOTTOMK of %llu rows from a segment of %zu (the reference's partial_sort "
+                      "reads past the segment there)", (unsigned long long)k, s.count);
+            return false;
+        }
+        void* d = seg_rows_mut(s, e.dtype, c);
+        if (!d || !relw_partial_sort(e.dtype, d, s.count, (uint32_t)k, op == 0xF4, c) ||
+            !seg_write_back(s, e.dtype, d, c))
+            return false;
+        regs[rd] = k;
+        return true;
+    }
+    case 0xFD:                                              // PARTITION_HASH
+        if (ra < nseg) regs[rd] = e.segs[ra]->count;
+        return true;
+    default: {                                              // JOIN_HASH .. JOIN_SEMI
+        if (ra >= nseg || rb >= nseg) return true;
+        static const int kind_of[5] = {0, 1, 0, 3, 2};
+        Seg& l = *e.segs[ra];
+        Seg& r = *e.segs[rb];
+        const void* dl = l.count ? seg_rows(l, e.dtype, 0, l.count, c) : nullptr;
+        if (l.count && !dl) return false;
+        const void* dr = r.count ? seg_rows(r, e.dtype, 0, r.count, c) : nullptr;
+        if (r.count && !dr) return false;
+        uint64_t m = 0;
+        if (!relw_join_count(kind_of[op - 0xF6], e.dtype, dl, l.count, dr, r.count, &m, e.rel, c)) return false;
+        regs[rd] = m;
+        return true;
+    }
+    }
+}
+}  // namespace
+
 bool run_straight(Engine& e, StreamCtx* c) {
-    if (e.dtype != DT_F64 && e.dtype != DT_I64) { set_error("segment aggregates are lowered for f64 and i64 engines only"); return false; }
+    if (e.dtype != DT_F64 && e.dtype != DT_I64) {
+        set_error("segment aggregate, hash, sort and join words are lowered for f64 and i64 engines only");
+        return false;
+    }
     if (!e.agg_ready) {                                          // AggState defaults (engine.hpp:347-360)
         const bool F = e.dtype == DT_F64;
         for (AggSlot& a : e.agg) {
@@ -599,9 +712,15 @@ bool run_straight(Engine& e, StreamCtx* c) {
     std::memcpy(regs, e.regs, sizeof(regs));
     AggSlot work[16];
     std::copy(std::begin(e.agg), std::end(e.agg), std::begin(work));
+    TabSlot twork[16];
+    std::copy(std::begin(e.tab), std::end(e.tab), std::begin(twork));
     for (uint32_t w : e.plan.words) {
         if (agg_op(op_of(w))) {
             if (!run_agg(e, w, regs, work, c)) return false;
+            continue;
+        }
+        if (rel_word_op(op_of(w))) {
+            if (!run_relword(e, w, regs, twork, c)) return false;
             continue;
         }
         const int rc = run_scalar(std::vector<uint32_t>{w}, regs);
@@ -609,6 +728,7 @@ bool run_straight(Engine& e, StreamCtx* c) {
         if (rc == 0) break;                                      // division by zero stops the program
     }
     std::copy(std::begin(work), std::end(work), std::begin(e.agg));
+    std::copy(std::begin(twork), std::end(twork), std::begin(e.tab));
     std::memcpy(e.regs, regs, sizeof(regs));
     return true;
 }

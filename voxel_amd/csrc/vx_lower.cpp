@@ -135,8 +135,9 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     Plan p;
     if (detect_fast_path(c, len, p)) return p;
 
-    // Straight-line program of scalar words and segment aggregates: the
-    // interpreter runs it word by word to HALT (engine.hpp:76-80).
+    // Straight-line program of scalar words, segment aggregates and the
+    // hash / sort / join words: the interpreter runs it word by word to HALT
+    // (engine.hpp:76-80).
     {
         size_t k = 0;
         bool any_agg = false;
@@ -145,6 +146,10 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
             if (agg_op(op)) {
                 if (op == 0xEA || op == 0xEC)
                     return none("AGG_SUM_DISTINCT and AGG_MODE are not lowered (their results follow hash-map order)");
+                any_agg = true;
+                continue;
+            }
+            if (rel_word_op(op)) {
                 any_agg = true;
                 continue;
             }

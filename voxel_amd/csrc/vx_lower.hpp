@@ -39,7 +39,7 @@ inline int cmp_of_filter_op(uint8_t op) {
 
 struct Plan {
     enum Kind { NONE, FAST, TIER0, STRAIGHT } kind = NONE;
-    std::vector<uint32_t> words;     // STRAIGHT: scalar and AGG_* words up to HALT
+    std::vector<uint32_t> words;     // STRAIGHT: scalar, AGG_* and hash / sort / join words up to HALT
     std::string why;                 // reason when NONE
 
     // FAST: Engine<T>::DetectFastPath (engine.hpp:84-107)
@@ -113,6 +113,12 @@ bool host_op(uint8_t op);
 
 // Segment aggregate words AGG_COUNT..AGG_PERCENTILE (opcodes.hpp:228-241).
 inline bool agg_op(uint8_t op) { return op >= 0xE0 && op <= 0xED; }
+
+// Hash / sort / join words of straight-line programs: HASH_INIT, HASH_PROBE,
+// HASH_BUILD, HASH_LOOKUP, SORT_ASC, SORT_DESC, SORT_TOPK, SORT_BOTTOMK,
+// JOIN_HASH, JOIN_MERGE, JOIN_NESTED, JOIN_ANTI, JOIN_SEMI, PARTITION_HASH
+// (opcodes.hpp:242-259).
+inline bool rel_word_op(uint8_t op) { return op == 0xEE || op == 0xEF || (op >= 0xF0 && op <= 0xFA) || op == 0xFD; }
 
 // Runs host_op words in order on a register file.  Returns 1 when all ran,
 // 0 when a DIV/MOD/DIVF by zero stopped the program (the reference clears

@@ -81,8 +81,9 @@ bool ensure_stage(SortBufs& b, uint64_t n) {
 }
 }  // namespace
 
-// Stable sort of (key, payload) pairs by key; output in kout/vout.  The
-// input is never written.  Returns the number of digit passes executed.
+// Stable sort of (key, payload) pairs by key; output in kout/vout (vin and
+// vout both NULL: keys only).  The input is never written.  Returns the
+// number of digit passes executed.
 int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
                    StreamCtx* c) {
     if (n >= (1ull << 30)) { set_error("sort: %llu keys exceed the 2^30 limit of the look-back words",
@@ -108,7 +109,7 @@ int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uin
     for (int i = 0; i < np; ++i) {
         bool to_out = ((np - 1 - i) % 2) == 0;
         uint64_t* kd = to_out ? kout : b.ktmp;
-        uint32_t* vd = to_out ? vout : b.vtmp;
+        uint32_t* vd = vin ? (to_out ? vout : b.vtmp) : nullptr;     // keys only without a payload
         if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, passes[i], b.w, st, nullptr, nullptr), "radix pass")) return -1;
         ks = kd;
         vs = vd;
