@@ -615,7 +615,10 @@ def bench_serve() -> dict:
     srv = Server(max_batch=nq, memory_budget=256 << 20, inflight=2)
     tune = srv.autotune(qs, reps=1)
     res = {}
-    for mode, name in ((0, "batched"), (1, "unbatched")):
+    # batched: the live request loop fed at the arrivals; unbatched: one
+    # synchronous launch per query; unbatched_async: one launch per query
+    # queued back to back without waiting, results read through events
+    for mode, name in ((0, "batched"), (1, "unbatched"), (2, "unbatched_async")):
         best = None
         for _ in range(2):                       # saturated: all queries at t = 0; best of two replays
             out, lat, st = srv.replay_raw(qs, np.zeros(nq), mode=mode)
@@ -628,6 +631,8 @@ def bench_serve() -> dict:
                                           "mean_batch": round(st["mean_batch"], 2)}
     res["throughput_ratio_saturated"] = round(res["batched_saturated"]["queries_per_s"] /
                                               res["unbatched_saturated"]["queries_per_s"], 3)
+    res["throughput_ratio_saturated_vs_async"] = round(res["batched_saturated"]["queries_per_s"] /
+                                                       res["unbatched_async_saturated"]["queries_per_s"], 3)
     res["autotune"] = tune
     # a saturated batch holds all 4096 queries: queries on one column share a
     # scan, up to 4 per job, so the rows read are fewer than the rows queried

@@ -489,13 +489,16 @@ int       voxel_gpu_join_set_probe_partitioning(void* join, int on);
  * per Engine::Run, exec/engine.hpp:109-141) ------------------------------- */
 /* A batch engine runs up to max_queries scan descriptors (voxel_gpu_scan's,
  * blocks/unroll/layout fields ignored) in one persistent kernel launched
- * through a HIP graph on the engine's own stream.  Each query is cut into
- * tiles of tile_rows rows; a query's result depends only on its rows and
- * tile_rows (bit-identical alone or in any mix).  One launch in flight per
+ * through a HIP graph on the engine's own stream.  Each query is reduced in
+ * units of 8192 rows (one compensated partial per unit, merged in unit
+ * order), so its result depends only on its rows: bit-identical alone, in
+ * any mix and under any configuration.  A launch cuts the queries into
+ * tiles of whole units, at most tile_rows rows, fewer while the batch would
+ * give under two tiles per resident workgroup.  One launch in flight per
  * engine; an engine is not thread-safe. */
 void*     voxel_gpu_batch_create(size_t max_queries);
 void      voxel_gpu_batch_destroy(void* batch);
-/* unroll 2/4/8, tile_rows multiple of 4096, blocks_per_cu 1..8; 0 keeps. */
+/* unroll 2/4/8, tile_rows multiple of 8192, blocks_per_cu 1..8; 0 keeps. */
 int       voxel_gpu_batch_config(void* batch, int unroll, uint32_t tile_rows, int blocks_per_cu);
 int       voxel_gpu_batch_get_config(void* batch, int* unroll, uint32_t* tile_rows, int* blocks_per_cu);
 int       voxel_gpu_batch_launch(void* batch, const voxel_gpu_scan_desc* q, size_t nq);
@@ -506,8 +509,7 @@ int       voxel_gpu_batch_run(void* batch, const voxel_gpu_scan_desc* q, size_t 
                               float* ms);
 /* Times every configuration of unroll {2,4,8} x tile_rows {16K..256K} x
  * blocks_per_cu {1,2,4} on these queries (median of reps), checks that each
- * gives the same per-query results (counts/ints/min/max identical, float sums
- * within 4 ulp), and keeps the fastest. */
+ * gives the same per-query results bit for bit, and keeps the fastest. */
 int       voxel_gpu_batch_autotune(void* batch, const voxel_gpu_scan_desc* q, size_t nq, int reps,
                                    double* best_ms, int* configs_tried);
 
@@ -518,16 +520,43 @@ typedef struct voxel_gpu_serve_stats {
     double   p50_s, p99_s, mean_s, max_s;   /* arrival -> result on the host */
 } voxel_gpu_serve_stats;
 /* inflight batch engines (1..8); a query is admitted while the scratch of the
- * queries on the device (64 B per tile + result + descriptor) stays within
- * memory_budget_bytes (a query larger than the budget runs alone). */
+ * queries on the device (64 B per 8192-row unit + result + descriptor)
+ * stays within memory_budget_bytes (a query larger than the budget runs
+ * alone). */
 void*     voxel_gpu_server_create(size_t max_batch, uint64_t memory_budget_bytes, int inflight);
+/* Stops the request loop if it runs, then frees the server. */
 void      voxel_gpu_server_destroy(void* server);
 int       voxel_gpu_server_config(void* server, int unroll, uint32_t tile_rows, int blocks_per_cu);
 int       voxel_gpu_server_autotune(void* server, const voxel_gpu_scan_desc* q, size_t nq, int reps,
                                     double* best_ms, int* configs_tried);
+
+/* Live serving.  start() runs the request loop on a thread of its own: it
+ * takes submitted queries in arrival order, admits them into the next free
+ * batch engine while the memory budget allows, launches, and completes
+ * earlier batches as they finish, so queries that arrive while others are
+ * on the GPU join the next batch.  submit() may be called from any thread
+ * and returns a ticket (>= 0; -1 with voxel_last_error set for a bad
+ * descriptor or a server that is not running); the columns must stay
+ * resident until the ticket completes.  poll() returns 1 and the result when
+ * the ticket has completed, 0 while it is pending, -1 for an unknown ticket
+ * or a failed query; wait() blocks until it completes (latency_s may be
+ * NULL: seconds from submit to completion).  stop() lets every submitted
+ * query finish, ends the loop and reports the queries served since start()
+ * (stats may be NULL); results stay readable until the next start(). */
+int       voxel_gpu_server_start(void* server);
+int64_t   voxel_gpu_server_submit(void* server, const voxel_gpu_scan_desc* q);
+int       voxel_gpu_server_poll(void* server, int64_t ticket, voxel_gpu_agg* out);
+int       voxel_gpu_server_wait(void* server, int64_t ticket, voxel_gpu_agg* out, double* latency_s);
+int       voxel_gpu_server_stop(void* server, voxel_gpu_serve_stats* stats);
+
 /* Replays n queries arriving at arrival_s[i] seconds (non-decreasing) after
- * the call starts.  mode 0: continuous batching; mode 1: one synchronous
- * voxel_gpu_scan per query.  out[i], latency_s[i] (may be NULL), stats. */
+ * the call starts; latency is measured from the scheduled arrival.
+ * mode 0: continuous batching (start, one submit per arrival from the
+ * calling thread, the live request loop serving them, stop); mode 1: one
+ * synchronous voxel_gpu_scan per query; mode 2: one launch per query,
+ * queued back to back on a stream of the server's own without waiting,
+ * each result read back through its own event as it completes.  out[i],
+ * latency_s[i] (may be NULL), stats.  Not while the live loop runs. */
 int       voxel_gpu_server_replay(void* server, const voxel_gpu_scan_desc* q, const double* arrival_s, size_t n,
                                   int mode, voxel_gpu_agg* out, double* latency_s, voxel_gpu_serve_stats* stats);
 

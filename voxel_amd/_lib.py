@@ -231,6 +231,11 @@ SIGS = {
     "voxel_gpu_server_config": (INT, [P, INT, U32, INT]),
     "voxel_gpu_server_autotune": (INT, [P, P, SZ, INT, P, P]),
     "voxel_gpu_server_replay": (INT, [P, P, P, SZ, INT, P, P, P]),
+    "voxel_gpu_server_start": (INT, [P]),
+    "voxel_gpu_server_submit": (I64, [P, P]),
+    "voxel_gpu_server_poll": (INT, [P, I64, P]),
+    "voxel_gpu_server_wait": (INT, [P, I64, P, P]),
+    "voxel_gpu_server_stop": (INT, [P, P]),
     "voxel_gpu_scan": (INT, [C.POINTER(ScanDesc), P, C.POINTER(Agg)]),
     "voxel_gpu_scan_launch": (INT, [C.POINTER(ScanDesc), P]),
     "voxel_gpu_scan_result": (INT, [P, C.POINTER(Agg)]),

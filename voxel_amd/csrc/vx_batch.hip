@@ -6,20 +6,24 @@
 // over columns of any length.  Queries that read the same rows (same
 // column(s), length and element type) are grouped into jobs of up to
 // kBatchShare queries: a job loads each row once and feeds every member's
-// predicate and accumulator (a shared scan).  Every job is cut into tiles of
-// tile_rows rows; the tiles of all jobs form one global list.  Resident
-// workgroups take tiles from an atomic counter, reduce a tile with the
-// scan's compensated COUNT/SUM/MIN/MAX accumulator (vx_acc.hpp), one per
-// member, and store one partial per member and tile; a per-query pass after
-// the kernel boundary (k_batch_finish) combines each query's partials in
-// tile order.  A query's accumulator sees the same rows in the same order
-// whether it shares its job or runs alone, so its result depends only on
-// its own rows and tile_rows: it is the same bits alone or in any mix.
+// predicate and accumulator (a shared scan).  Every job is cut into chunks
+// of tile_rows rows (whole units of kUnitRows); the chunks of all jobs form
+// one global list.  Resident waves take chunks round-robin (wave g: chunks
+// g, g + waves, ...) and reduce each unit of a chunk with the scan's
+// compensated COUNT/SUM/MIN/MAX accumulator (vx_acc.hpp), one per member,
+// then a wave reduction, storing one partial per member and unit; a
+// per-query pass after the kernel boundary (k_batch_finish) combines each
+// query's partials in unit order.  A query's accumulator sees the same rows
+// of a unit in the same lane order whatever the chunk size, the unroll, the
+// grid or the other members of its job, so its result depends only on its
+// own rows: it is the same bits alone, in any mix and under any
+// configuration, and each launch may pick its own chunk size.
 //
 // Host side: BatchRunner keeps pinned descriptor/result buffers and a graph
 // of four nodes -- descriptor upload, tile kernel, finish kernel, result
-// download -- whose copy sizes are updated per launch; the finish kernel
-// resets the tile counter, so the graph holds no memset node.
+// download -- whose copy sizes are updated per launch (no memset node); a
+// batch of at most kSmallQ queries instead runs as one kernel launch
+// (k_batch_small).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -42,128 +46,163 @@
 namespace vx {
 namespace {
 
-// Rows [s, e) of a job, tile t of it: every member's predicate(s) and
-// accumulator on each row loaded; member m's partial goes to its own slot
-// tile0 + t.
-template <typename T, int U, bool NT>
-__device__ __forceinline__ void batch_tile(const BatchJob& job, const BatchQuery* __restrict__ qs, uint64_t s,
-                                           uint64_t e, uint32_t t, ScanOut* __restrict__ parts) {
+// Rows [s, e) of a job, one unit (local unit index lu = s / kUnitRows), run
+// by one wave: each of the job's NM members' predicate(s) (TWO: a predicate
+// on column b too) and accumulator on every row loaded, lane l taking the
+// 16-byte vectors l, l + 64, ... of the unit in order; then a wave
+// reduction (shuffles, no barrier) and lane 0 stores member m's partial in
+// its slot unit0 + lu.  NM and TWO are template parameters, so the row loop
+// carries no runtime branch.
+template <typename T, int U, bool NT, int NM, bool TWO>
+__device__ __forceinline__ void wave_unit(const BatchJob& job, const BatchQuery* __restrict__ qs, uint64_t s,
+                                          uint64_t e, uint64_t lu, ScanOut* __restrict__ parts) {
     using V = typename TyInfo<T>::V;
     constexpr int E = TyInfo<T>::E;
-    constexpr int G = kBatchShare;
-    __shared__ Acc<T> lds[kBlock / 64];
-    const int nm = (int)job.nmem;
-    Pred<T, false> pa[G], pb[G];
-    Acc<T> acc[G];
+    const uint32_t lane = threadIdx.x & 63;
+    Pred<T, false> pa[NM], pb[NM];
+    Acc<T> acc[NM];
 #pragma unroll
-    for (int m = 0; m < G; ++m) {
+    for (int m = 0; m < NM; ++m) {
         acc[m].init();
-        if (m < nm) {
-            pa[m].load(qs[job.mem[m]].pa);
-            pb[m].load(qs[job.mem[m]].pb);
-        }
+        pa[m].load(qs[job.mem[m]].pa);
+        if constexpr (TWO) pb[m].load(qs[job.mem[m]].pb);
     }
-    const bool two = job.mode == 2;
     const T* __restrict__ a = reinterpret_cast<const T*>(job.a);
-    const T* __restrict__ b = reinterpret_cast<const T*>(two ? job.b : job.a);
+    const T* __restrict__ b = reinterpret_cast<const T*>(TWO ? job.b : job.a);
     auto one = [&](T x, T y) {
 #pragma unroll
-        for (int m = 0; m < G; ++m)
-            if (m < nm) {
-                bool p = pa[m](x);
-                if (two) p = p & pb[m](y);
-                acc[m].add(p, x);
-            }
+        for (int m = 0; m < NM; ++m) {
+            bool p = pa[m](x);
+            if constexpr (TWO) p = p & pb[m](y);
+            acc[m].add(p, x);
+        }
     };
     const bool aligned = ((reinterpret_cast<uintptr_t>(a + s) | reinterpret_cast<uintptr_t>(b + s)) & 15u) == 0;
     const uint64_t nvec = aligned ? (e - s) / E : 0;
     const V* __restrict__ av = reinterpret_cast<const V*>(a + s);
     const V* __restrict__ bv = reinterpret_cast<const V*>(b + s);
-    for (uint64_t base = 0; base < nvec; base += (uint64_t)kBlock * U) {
+    for (uint64_t base = 0; base < nvec; base += 64ull * U) {
         V xa[U], xb[U];
-        if (base + (uint64_t)kBlock * U <= nvec) {
+        if (base + 64ull * U <= nvec) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) xa[u] = ld16<V, NT>(&av[base + u * kBlock + threadIdx.x]);
-            if (two) {
+            for (int k = 0; k < U; ++k) xa[k] = ld16<V, NT>(&av[base + k * 64 + lane]);
+            if constexpr (TWO) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) xb[u] = ld16<V, NT>(&bv[base + u * kBlock + threadIdx.x]);
+                for (int k = 0; k < U; ++k) xb[k] = ld16<V, NT>(&bv[base + k * 64 + lane]);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const T* pxa = reinterpret_cast<const T*>(&xa[u]);
-                const T* pxb = reinterpret_cast<const T*>(&xb[u]);
+            for (int k = 0; k < U; ++k) {
+                const T* pxa = reinterpret_cast<const T*>(&xa[k]);
+                const T* pxb = reinterpret_cast<const T*>(&xb[k]);
 #pragma unroll
-                for (int k = 0; k < E; ++k) one(pxa[k], two ? pxb[k] : T(0));
+                for (int r = 0; r < E; ++r) one(pxa[r], TWO ? pxb[r] : T(0));
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint64_t vi = base + u * kBlock + threadIdx.x;
+            for (int k = 0; k < U; ++k) {
+                const uint64_t vi = base + k * 64 + lane;
                 if (vi < nvec) {
-                    xa[u] = av[vi];
-                    if (two) xb[u] = bv[vi];
-                    const T* pxa = reinterpret_cast<const T*>(&xa[u]);
-                    const T* pxb = reinterpret_cast<const T*>(&xb[u]);
+                    xa[k] = av[vi];
+                    if constexpr (TWO) xb[k] = bv[vi];
+                    const T* pxa = reinterpret_cast<const T*>(&xa[k]);
+                    const T* pxb = reinterpret_cast<const T*>(&xb[k]);
 #pragma unroll
-                    for (int k = 0; k < E; ++k) one(pxa[k], two ? pxb[k] : T(0));
+                    for (int r = 0; r < E; ++r) one(pxa[r], TWO ? pxb[r] : T(0));
                 }
             }
         }
     }
-    for (uint64_t i = s + nvec * E + threadIdx.x; i < e; i += kBlock) one(a[i], two ? b[i] : T(0));
+    for (uint64_t i = s + nvec * E + lane; i < e; i += 64) one(a[i], TWO ? b[i] : T(0));
 #pragma unroll
-    for (int m = 0; m < G; ++m)
-        if (m < nm) {
-            block_reduce(acc[m], lds);
-            if (threadIdx.x == 0) acc[m].store(&parts[qs[job.mem[m]].tile0 + t]);
-            __syncthreads();
-        }
+    for (int m = 0; m < NM; ++m) {
+        acc[m].fold();
+        for (int mask = 32; mask > 0; mask >>= 1) acc[m].shfl_merge(mask);
+        if (lane == 0) acc[m].store(&parts[qs[job.mem[m]].unit0 + lu]);
+    }
 }
 
-// Tiles of all jobs: resident workgroups take tile ids from ctr[0] and
-// store one partial per member and tile.  No workgroup waits on or reads
-// another's partial here (that would need an agent-scope release and
-// acquire per tile: an L2 write-back and invalidate on this multi-XCD part);
-// k_batch_finish combines them after the kernel boundary.
+template <typename T, int U, bool NT, int NM, bool TWO>
+__device__ __forceinline__ void wave_units(const BatchJob& job, const BatchQuery* __restrict__ qs, uint64_t s,
+                                           uint64_t e, ScanOut* __restrict__ parts) {
+    uint64_t us = s;
+    do {
+        const uint64_t ue = us + kUnitRows < e ? us + kUnitRows : e;
+        wave_unit<T, U, NT, NM, TWO>(job, qs, us, ue, us / kUnitRows, parts);
+        us = ue;
+    } while (us < e);
+}
+
+// One chunk of a job (rows [s, e), whole units; an empty query has one
+// empty unit) run by one wave, with the row loop specialised on the job's
+// member count and mode.
+template <typename T, int U, bool NT>
+__device__ __forceinline__ void wave_chunk(const BatchJob& job, const BatchQuery* __restrict__ qs, uint64_t s,
+                                           uint64_t e, ScanOut* __restrict__ parts) {
+    const bool two = job.mode == 2;
+    // one member on one column: twice the loads in flight (its accumulator
+    // leaves the registers for them)
+    constexpr int U1 = U * 2 < 16 ? U * 2 : 16;
+    switch (job.nmem * 2 + (two ? 1 : 0)) {
+        case 2: wave_units<T, U1, NT, 1, false>(job, qs, s, e, parts); break;
+        case 3: wave_units<T, U, NT, 1, true>(job, qs, s, e, parts); break;
+        case 4: wave_units<T, U, NT, 2, false>(job, qs, s, e, parts); break;
+        case 5: wave_units<T, U, NT, 2, true>(job, qs, s, e, parts); break;
+        case 6: wave_units<T, U, NT, 3, false>(job, qs, s, e, parts); break;
+        case 7: wave_units<T, U, NT, 3, true>(job, qs, s, e, parts); break;
+        case 8: wave_units<T, U, NT, 4, false>(job, qs, s, e, parts); break;
+        default: wave_units<T, U, NT, 4, true>(job, qs, s, e, parts); break;
+    }
+}
+
+// Chunks of all jobs (tile_rows rows each, whole units; the list runs largest
+// job first): wave g of the grid takes chunks g, g + waves, ..., where g
+// counts the first wave of every workgroup before any second wave (so a
+// short list spreads over all CUs) -- no shared counter, whose atomics
+// serialise on one address, and no barrier: each wave reduces its own
+// units.  Job of a chunk: the last job with tile0 <= chunk;
+// a wave's chunks increase, so it steps forward from the previous one and
+// jumps by binary search when far.
 template <int U, bool NT>
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchHeader* __restrict__ hdr, const BatchQuery* __restrict__ qs,
-                                                  uint32_t* __restrict__ ctr) {
-    __shared__ uint32_t s_tile;
-    const uint32_t total = hdr->total_tiles, nj = hdr->njobs, tile_rows = hdr->tile_rows;
-    const BatchJob* __restrict__ jobs = hdr->jobs;
-    ScanOut* parts = hdr->parts;
-    uint32_t lo = 0;                                             // job of the block's previous tile
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(&ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        __syncthreads();
-        if (tile >= total) break;                               // block-uniform
-        // last job with tile0 <= tile: a block's tiles increase, so step
-        // forward from the previous one, and jump by binary search when far
-        if (lo + 1 < nj && jobs[lo + 1].tile0 <= tile) {
-            if (lo + 2 < nj && jobs[lo + 2].tile0 <= tile) {
+__device__ __forceinline__ void run_chunks(const BatchJob* __restrict__ jobs, uint32_t nj, uint32_t total,
+                                           uint32_t tile_rows, const BatchQuery* __restrict__ qs,
+                                           ScanOut* __restrict__ parts) {
+    constexpr uint32_t kWaves = kBlock / 64;
+    const uint32_t waves = gridDim.x * kWaves;
+    uint32_t lo = 0;
+    for (uint32_t c = (threadIdx.x >> 6) * gridDim.x + blockIdx.x; c < total; c += waves) {
+        if (lo + 1 < nj && jobs[lo + 1].tile0 <= c) {
+            if (lo + 2 < nj && jobs[lo + 2].tile0 <= c) {
                 uint32_t a = lo + 2, z = nj;
                 while (z - a > 1) {
                     const uint32_t mid = (a + z) >> 1;
-                    if (jobs[mid].tile0 <= tile) a = mid; else z = mid;
+                    if (jobs[mid].tile0 <= c) a = mid; else z = mid;
                 }
                 lo = a;
             } else {
                 lo = lo + 1;
             }
         }
-        const BatchJob job = jobs[lo];
-        const uint32_t t = (uint32_t)(tile - job.tile0);
-        const uint64_t s = (uint64_t)t * tile_rows;
+        const BatchJob& job = jobs[lo];
+        const uint64_t s = (uint64_t)(c - job.tile0) * tile_rows;
         const uint64_t e = s + tile_rows < job.n ? s + tile_rows : job.n;
         switch (job.dtype) {
-            case DT_F64: batch_tile<double, U, NT>(job, qs, s, e, t, parts); break;
-            case DT_F32: batch_tile<float, U, NT>(job, qs, s, e, t, parts); break;
-            case DT_I64: batch_tile<int64_t, U, NT>(job, qs, s, e, t, parts); break;
-            default: batch_tile<int32_t, U, NT>(job, qs, s, e, t, parts); break;
+            case DT_F64: wave_chunk<double, U, NT>(job, qs, s, e, parts); break;
+            case DT_F32: wave_chunk<float, U, NT>(job, qs, s, e, parts); break;
+            case DT_I64: wave_chunk<int64_t, U, NT>(job, qs, s, e, parts); break;
+            default: wave_chunk<int32_t, U, NT>(job, qs, s, e, parts); break;
         }
     }
+}
+
+// The chunk pass of a batch.  No wave waits on or reads another's partial
+// here (that would need an agent-scope release and acquire per chunk: an L2
+// write-back and invalidate on this multi-XCD part); k_batch_finish combines
+// them after the kernel boundary.
+template <int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchHeader* __restrict__ hdr, const BatchQuery* __restrict__ qs,
+                                                  uint32_t* __restrict__ ctr) {
+    (void)ctr;
+    run_chunks<U, NT>(hdr->jobs, hdr->njobs, hdr->total_tiles, hdr->tile_rows, qs, hdr->parts);
 }
 
 template <typename T>
@@ -171,8 +210,8 @@ __device__ __forceinline__ void finish_query(const BatchQuery& q, const ScanOut*
                                              ScanOut* __restrict__ out) {
     __shared__ Acc<T> lds[kBlock / 64];
     Acc<T> r; r.init();
-    for (uint32_t i = threadIdx.x; i < q.ntiles; i += kBlock) {
-        Acc<T> o; o.load(&parts[q.tile0 + i]);
+    for (uint32_t i = threadIdx.x; i < q.nunits; i += kBlock) {
+        Acc<T> o; o.load(&parts[q.unit0 + i]);
         r.merge(o);
     }
     block_reduce(r, lds);
@@ -180,11 +219,11 @@ __device__ __forceinline__ void finish_query(const BatchQuery& q, const ScanOut*
     __syncthreads();
 }
 
-// One workgroup per query (grid-stride): the query's tile partials in a fixed
-// order -> its result, in the caller's order; resets the tile counter.
+// One workgroup per query (grid-stride): the query's unit partials in a
+// fixed order -> its result, in the caller's order.
 __global__ __launch_bounds__(kBlock) void k_batch_finish(const BatchHeader* __restrict__ hdr,
                                                          const BatchQuery* __restrict__ qs, uint32_t* __restrict__ ctr) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = 0;
+    (void)ctr;
     const uint32_t nq = hdr->nq;
     for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
         const BatchQuery q = qs[qi];
@@ -196,6 +235,56 @@ __global__ __launch_bounds__(kBlock) void k_batch_finish(const BatchHeader* __re
             default: finish_query<int32_t>(q, hdr->parts, out); break;
         }
     }
+}
+
+// Batches of at most kSmallQ queries: one kernel, no graph.  The header,
+// query and job records travel as the kernel's first argument (the kernarg
+// segment, read through its segment pointer and copied to LDS by each
+// workgroup); waves take chunks like k_batch, and the last workgroup to
+// finish (a ticket with an agent-scope release/acquire, once per workgroup)
+// merges each query's unit partials with the same code as k_batch_finish --
+// the same bits -- and writes the results straight into pinned host memory.
+constexpr uint32_t kSmallQ = 8;
+
+struct SmallArgs {
+    BatchHeader h;
+    BatchQuery q[kSmallQ];
+    BatchJob j[kSmallQ];
+};
+static_assert(sizeof(SmallArgs) == 64 + kSmallQ * (128 + 64), "SmallArgs layout");
+static_assert(sizeof(SmallArgs) <= 2048, "SmallArgs must stay a small kernel argument");
+
+template <int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_batch_small(const SmallArgs args, ScanOut* __restrict__ parts,
+                                                        uint32_t* __restrict__ ctr, ScanOut* __restrict__ hres) {
+    __shared__ SmallArgs sa;
+    __shared__ bool s_last;
+    {
+        // args is the first explicit argument: offset 0 of the kernarg segment
+        const uint4* src = (const uint4*)__builtin_amdgcn_kernarg_segment_ptr();
+        uint4* dst = reinterpret_cast<uint4*>(&sa);
+        const uint32_t nq = src[0].x, nj = src[0].w;            // BatchHeader.nq, .njobs
+        const uint32_t wq = 4 + nq * 8;
+        for (uint32_t i = threadIdx.x; i < 4 + (uint32_t)kSmallQ * 8 + nj * 4; i += kBlock) {
+            if (i >= wq && i < 4 + (uint32_t)kSmallQ * 8) continue;     // unused query records
+            dst[i] = src[i];
+        }
+    }
+    __syncthreads();
+    const uint32_t nq = sa.h.nq;
+    run_chunks<U, NT>(sa.j, sa.h.njobs, sa.h.total_tiles, sa.h.tile_rows, sa.q, parts);
+    __syncthreads();                                            // every wave of the workgroup is done
+    if (!publish_and_check_last(&ctr[1], &s_last)) return;
+    for (uint32_t qi = 0; qi < nq; ++qi) {
+        const BatchQuery& q = sa.q[qi];
+        switch (q.dtype) {
+            case DT_F64: finish_query<double>(q, parts, &hres[q.orig]); break;
+            case DT_F32: finish_query<float>(q, parts, &hres[q.orig]); break;
+            case DT_I64: finish_query<int64_t>(q, parts, &hres[q.orig]); break;
+            default: finish_query<int32_t>(q, parts, &hres[q.orig]); break;
+        }
+    }
+    if (threadIdx.x == 0) ctr[1] = 0;                            // every other workgroup has exited
 }
 
 void* batch_kernel(int unroll) {
@@ -240,7 +329,10 @@ BatchRunner::~BatchRunner() {
 
 bool BatchRunner::set_config(const BatchConfig& c) {
     if (c.unroll != 2 && c.unroll != 4 && c.unroll != 8) { set_error("batch: unroll must be 2, 4 or 8"); return false; }
-    if (c.tile_rows < 4096 || c.tile_rows % 4096 != 0) { set_error("batch: tile_rows must be a multiple of 4096"); return false; }
+    if (c.tile_rows < kUnitRows || c.tile_rows % kUnitRows != 0) {
+        set_error("batch: tile_rows must be a multiple of %u", kUnitRows);
+        return false;
+    }
     if (c.blocks_per_cu < 1 || c.blocks_per_cu > 8) { set_error("batch: blocks_per_cu must be 1..8"); return false; }
     if (inflight_) { set_error("batch: a launch is in flight"); return false; }
     if (c.unroll != cfg_.unroll || c.blocks_per_cu != cfg_.blocks_per_cu) graph_valid_ = false;
@@ -248,14 +340,18 @@ bool BatchRunner::set_config(const BatchConfig& c) {
     return true;
 }
 
-uint64_t BatchRunner::tiles_of(uint64_t n) const {
-    uint64_t t = (n + cfg_.tile_rows - 1) / cfg_.tile_rows;
-    return t ? t : 1;
+uint32_t BatchRunner::tile_rows_for(uint64_t units, uint64_t grid) const {
+    const uint64_t waves = grid * (kBlock / 64);
+    const uint64_t max_units = cfg_.tile_rows / kUnitRows;
+    uint64_t tu = units / (2 * waves);
+    if (tu < 1) tu = 1;
+    if (tu > max_units) tu = max_units;
+    return (uint32_t)(tu * kUnitRows);
 }
 
-bool BatchRunner::ensure_parts(uint64_t tiles) {
-    if (tiles <= parts_cap_) return true;
-    uint64_t cap = std::max<uint64_t>(tiles, parts_cap_ * 2);
+bool BatchRunner::ensure_parts(uint64_t units) {
+    if (units <= parts_cap_) return true;
+    uint64_t cap = std::max<uint64_t>(units, parts_cap_ * 2);
     if (parts_) (void)hipFree(parts_);
     parts_ = nullptr;
     parts_cap_ = 0;
@@ -306,10 +402,10 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
     BatchHeader* h = reinterpret_cast<BatchHeader*>(hq_);
     BatchQuery* qs = reinterpret_cast<BatchQuery*>(hq_ + sizeof(BatchHeader));
     // Jobs: queries reading the same rows (column(s), length, element type)
-    // share a scan, up to kBatchShare per job, in the caller's order.  Tile
-    // ids are handed out largest job first, so the last tiles the workgroups
-    // take are the small ones and no long tile runs alone at the end; results
-    // go back to the caller's order (descriptor field orig).
+    // share a scan, up to kBatchShare per job, in the caller's order.  Tiles
+    // are listed largest job first, so the last tiles of every workgroup are
+    // the small ones and no long tile runs alone at the end; results go back
+    // to the caller's order (descriptor field orig).
     std::vector<ScanArgs> args(nq);
     for (size_t i = 0; i < nq; ++i)
         if (!make_scan_args(&q[i], args[i])) { set_error("batch query %zu: %s", i, last_error()); return false; }
@@ -331,17 +427,20 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
         std::unordered_map<Key, size_t, KeyHash> open;      // key -> job still taking members
         for (size_t i = 0; i < nq; ++i) {
             const ScanArgs& g = args[i];
-            Key k{g.a, g.mode == 2 ? g.b : nullptr, q[i].n, g.dtype, g.mode};
+            // mode 1 (both predicates on column a) runs as a two-column job
+            // whose second column is a itself
+            const int jm = g.mode == 0 ? 0 : 2;
+            const void* jb = g.mode == 2 ? g.b : g.a;
+            Key k{g.a, jm ? jb : nullptr, q[i].n, g.dtype, jm};
             auto it = open.find(k);
             if (it == open.end() || members[it->second].size() == (size_t)kBatchShare) {
                 BatchJob j;
                 std::memset(&j, 0, sizeof(j));
                 j.a = g.a;
-                j.b = g.mode == 2 ? g.b : g.a;
+                j.b = jb;
                 j.n = q[i].n;
                 j.dtype = g.dtype;
-                j.mode = g.mode;
-                j.ntiles = (uint32_t)tiles_of(j.n);
+                j.mode = jm;
                 jobs_.push_back(j);
                 members.emplace_back();
                 open[k] = jobs_.size() - 1;
@@ -351,6 +450,17 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
         }
     }
     const size_t nj = jobs_.size();
+    uint64_t all_units = 0;
+    for (const BatchJob& j : jobs_) all_units += units_of(j.n);
+    // a small batch runs as one kernel on at most one workgroup per CU (its
+    // chunks spread one wave per CU first); a large one on the configured grid
+    const bool small = nq <= kSmallQ;
+    const uint64_t grid_max = (uint64_t)device_cu_count() * (small ? 1 : cfg_.blocks_per_cu);
+    const uint32_t tile_rows = tile_rows_for(all_units, grid_max);
+    for (BatchJob& j : jobs_) {
+        const uint64_t t = (j.n + tile_rows - 1) / tile_rows;
+        j.ntiles = (uint32_t)(t ? t : 1);
+    }
     order_.resize(nj);
     for (size_t i = 0; i < nj; ++i) order_[i] = (uint32_t)i;
     std::stable_sort(order_.begin(), order_.end(), [&](uint32_t x, uint32_t y) { return jobs_[x].n > jobs_[y].n; });
@@ -369,16 +479,16 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
             BatchQuery& d = qs[qn];
             std::memset(&d, 0, sizeof(d));
             d.a = g.a;
-            d.b = g.mode == 2 ? g.b : g.a;
+            d.b = g.mode == 2 ? g.b : g.a;               // mode 1: column a again
             d.n = q[i].n;
             d.pa = g.pa;
-            d.pb = g.mode == 2 ? g.pb : g.pa;
+            d.pb = g.mode != 0 ? g.pb : g.pa;
             d.dtype = g.dtype;
-            d.mode = g.mode;
-            d.tile0 = parts;
-            d.ntiles = j.ntiles;
+            d.mode = g.mode == 0 ? 0 : 2;
+            d.unit0 = parts;
+            d.nunits = (uint32_t)units_of(d.n);
             d.orig = i;
-            parts += j.ntiles;
+            parts += d.nunits;
             dtypes_[i] = g.dtype;
             j.mem[m] = qn++;
         }
@@ -389,11 +499,34 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
     std::memset(h, 0, sizeof(*h));
     h->nq = (uint32_t)nq;
     h->total_tiles = (uint32_t)tiles;
-    h->tile_rows = cfg_.tile_rows;
+    h->tile_rows = tile_rows;
+    last_tile_rows_ = tile_rows;
     h->njobs = (uint32_t)nj;
     h->parts = parts_;
     h->results = dres_;
     h->jobs = reinterpret_cast<const BatchJob*>(dq_ + sizeof(BatchHeader) + nq * sizeof(BatchQuery));
+    if (small) {                                        // one kernel, descriptors as its argument
+        const unsigned grid = (unsigned)std::min<uint64_t>(tiles, grid_max);
+        SmallArgs sa;
+        std::memset(&sa, 0, sizeof(sa));
+        std::memcpy(&sa.h, h, sizeof(BatchHeader));
+        std::memcpy(sa.q, qs, nq * sizeof(BatchQuery));
+        std::memcpy(sa.j, jd, nj * sizeof(BatchJob));
+        if (!hip_ok(hipEventRecord(ev0_, st_), "hipEventRecord")) return false;
+        switch (cfg_.unroll) {
+            case 2: hipLaunchKernelGGL((k_batch_small<2, true>), dim3(grid), dim3(kBlock), 0, st_, sa, parts_, ctr_,
+                                       hres_); break;
+            case 8: hipLaunchKernelGGL((k_batch_small<8, true>), dim3(grid), dim3(kBlock), 0, st_, sa, parts_, ctr_,
+                                       hres_); break;
+            default: hipLaunchKernelGGL((k_batch_small<4, true>), dim3(grid), dim3(kBlock), 0, st_, sa, parts_, ctr_,
+                                        hres_); break;
+        }
+        if (!hip_ok(hipGetLastError(), "small batch launch") || !hip_ok(hipEventRecord(ev1_, st_), "hipEventRecord"))
+            return false;
+        nq_ = nq;
+        inflight_ = true;
+        return true;
+    }
     if (!graph_valid_ && !build_graph()) return false;
     const size_t up = sizeof(BatchHeader) + nq * sizeof(BatchQuery) + nj * sizeof(BatchJob);
     if (!hip_ok(hipGraphExecMemcpyNodeSetParams1D(exec_, n_up_, dq_, hq_, up, hipMemcpyHostToDevice), "graph upload size") ||
@@ -484,8 +617,8 @@ int voxel_gpu_batch_run(void* h, const voxel_gpu_scan_desc* q, size_t nq, voxel_
 // Autotuner: times every (unroll, tile_rows, blocks_per_cu) of the search
 // space on the given queries (median of reps launches each) and keeps the
 // fastest.  Per-query results of every configuration are checked against
-// the first one: counts, integer sums and min/max identical, float sums
-// within 4 ulp of the compensated value (only the tile split differs).
+// the first one bit for bit (the reduction units do not change with the
+// configuration).
 int voxel_gpu_batch_autotune(void* h, const voxel_gpu_scan_desc* q, size_t nq, int reps, double* best_ms,
                              int* configs_tried) {
     set_error("%s", "");
@@ -512,15 +645,7 @@ int voxel_gpu_batch_autotune(void* h, const voxel_gpu_scan_desc* q, size_t nq, i
                 ++tried;
                 if (!have_ref) { ref = cur; have_ref = true; }
                 for (size_t i = 0; i < nq; ++i) {
-                    const voxel_gpu_agg& x = cur[i];
-                    const voxel_gpu_agg& y = ref[i];
-                    bool same = x.count == y.count && x.isum == y.isum && x.imin == y.imin && x.imax == y.imax &&
-                                x.has_minmax == y.has_minmax &&
-                                (!x.has_minmax || (x.fmin == y.fmin && x.fmax == y.fmax) ||
-                                 (std::isnan(x.fmin) && std::isnan(y.fmin)));
-                    const double tol = 4.0 * std::fabs(std::nextafter(y.sum, INFINITY) - y.sum) + 1e-300;
-                    same = same && (x.sum == y.sum || std::fabs(x.sum - y.sum) <= tol);
-                    if (!same) {
+                    if (std::memcmp(&cur[i], &ref[i], sizeof(voxel_gpu_agg)) != 0) {
                         set_error("autotune: query %zu differs under unroll %d tile %u blocks/CU %d", i, u, t, b);
                         return -1;
                     }

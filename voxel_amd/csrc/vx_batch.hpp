@@ -20,15 +20,27 @@ namespace vx {
 // kBatchShare queries' predicates and accumulators.
 constexpr int kBatchShare = 4;
 
+// A query's rows are reduced in units of kUnitRows rows counted from its
+// first row, one wave per unit: one compensated partial per unit, merged in
+// unit order.  Chunks (the rows a wave takes at a time, tile_rows) are whole
+// units, so a query's bits depend on its rows only -- not on the chunk size,
+// the batch it runs in or the configuration.
+constexpr uint32_t kUnitRows = 8192;
+
+inline uint64_t units_of(uint64_t n) {
+    const uint64_t u = (n + kUnitRows - 1) / kUnitRows;
+    return u ? u : 1;
+}
+
 // Device descriptor of one query (128 bytes).
 struct BatchQuery {
     const void* a;
     const void* b;
     uint64_t n;
-    uint64_t tile0;          // first tile partial of the query
+    uint64_t unit0;          // first unit partial of the query
     PredDev pa, pb;
     int32_t dtype, mode;     // mode 0: pa on a; 2: pa on a and pb on b
-    uint32_t ntiles, orig;   // tiles; index of the query in the caller's list
+    uint32_t nunits, orig;   // unit partials; index of the query in the caller's list
     uint64_t pad2[2];
 };
 static_assert(sizeof(BatchQuery) == 128, "BatchQuery layout");
@@ -58,7 +70,7 @@ static_assert(sizeof(BatchHeader) == 64, "BatchHeader layout");
 
 struct BatchConfig {
     int unroll = 4;              // 16-byte loads in flight per lane and column: 2, 4, 8
-    uint32_t tile_rows = 65536;  // rows per tile (multiple of 4096)
+    uint32_t tile_rows = 65536;  // largest rows per chunk (multiple of kUnitRows)
     int blocks_per_cu = 2;       // resident workgroups per CU
 };
 
@@ -72,8 +84,12 @@ class BatchRunner {
     size_t max_queries() const { return max_q_; }
     const BatchConfig& config() const { return cfg_; }
     bool set_config(const BatchConfig& c);
-    // Tiles a query of n rows takes under the current configuration.
-    uint64_t tiles_of(uint64_t n) const;
+    // Rows of a chunk (the rows one wave takes at a time) in a launch of
+    // `units` unit partials on `grid` workgroups: the configured tile_rows,
+    // made smaller (down to one unit) while the batch would give fewer than
+    // two chunks per wave.
+    uint32_t tile_rows_for(uint64_t units, uint64_t grid) const;
+    uint32_t last_tile_rows() const { return last_tile_rows_; }
     // Validates and enqueues nq queries; false (error set) on bad input.
     bool launch(const voxel_gpu_scan_desc* q, size_t nq);
     bool done();                                   // non-blocking
@@ -82,7 +98,7 @@ class BatchRunner {
     hipStream_t stream() const { return st_; }
 
   private:
-    bool ensure_parts(uint64_t tiles);
+    bool ensure_parts(uint64_t units);
     bool build_graph();
     size_t max_q_;
     bool ok_ = false;
@@ -103,6 +119,7 @@ class BatchRunner {
     std::vector<uint32_t> order_;
     std::vector<BatchJob> jobs_;
     size_t nq_ = 0;
+    uint32_t last_tile_rows_ = 0;
     bool inflight_ = false;
     bool graph_valid_ = false;
 };

@@ -4,7 +4,9 @@ voxel_gpu_server_* in include/voxel_c.h).
 Queries are scan descriptors (``voxel_amd.scan.make_desc``) over resident
 device columns.  ``Batch.run`` executes a list of them in one persistent
 kernel launch; ``Server.replay`` feeds an arrival schedule through the
-continuous-batching loop (mode 0) or one launch per query (mode 1).
+continuous-batching loop (mode 0), one synchronous launch per query (mode 1)
+or one launch per query queued without waiting (mode 2); ``Server.start`` /
+``submit`` / ``poll`` / ``wait`` / ``stop`` serve queries as they arrive.
 """
 from __future__ import annotations
 
@@ -73,7 +75,13 @@ class Batch:
 
 
 class Server:
-    """Continuous-batching request loop over resident columns."""
+    """Continuous-batching request loop over resident columns.
+
+    Live use: ``start()``, then ``submit(desc)`` from any thread (a ticket),
+    ``poll(ticket)`` / ``wait(ticket)`` for its result, ``stop()`` for the
+    statistics.  ``replay`` feeds a recorded arrival schedule through the
+    same loop (mode 0), one synchronous launch per query (mode 1) or one
+    launch per query queued without waiting (mode 2)."""
 
     def __init__(self, max_batch: int = 1024, memory_budget: int = 64 << 20, inflight: int = 2):
         self.h = lib.voxel_gpu_server_create(max_batch, memory_budget, inflight)
@@ -99,6 +107,37 @@ class Server:
                                          C.byref(tried)) != 0:
             check("voxel_gpu_server_autotune")
         return {"best_ms": best.value, "configs_tried": tried.value}
+
+    def start(self) -> None:
+        if lib.voxel_gpu_server_start(self.h) != 0:
+            check("voxel_gpu_server_start")
+
+    def submit(self, desc) -> int:
+        t = int(lib.voxel_gpu_server_submit(self.h, C.byref(desc)))
+        if t < 0:
+            check("voxel_gpu_server_submit")
+        return t
+
+    def poll_raw(self, ticket: int):
+        """The result (Agg) once the ticket has completed, else None."""
+        r = Agg()
+        rc = lib.voxel_gpu_server_poll(self.h, ticket, C.byref(r))
+        if rc < 0:
+            check("voxel_gpu_server_poll")
+        return r if rc == 1 else None
+
+    def wait_raw(self, ticket: int):
+        """(Agg, seconds from submit to completion)."""
+        r, lat = Agg(), C.c_double()
+        if lib.voxel_gpu_server_wait(self.h, ticket, C.byref(r), C.byref(lat)) != 0:
+            check("voxel_gpu_server_wait")
+        return r, lat.value
+
+    def stop(self) -> dict:
+        st = ServeStats()
+        if lib.voxel_gpu_server_stop(self.h, C.byref(st)) != 0:
+            check("voxel_gpu_server_stop")
+        return st.as_dict()
 
     def replay_raw(self, queries, arrivals, mode: int = 0):
         n = len(queries)
