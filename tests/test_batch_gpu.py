@@ -304,5 +304,20 @@ def test_live_server_submit_poll_wait_from_threads(gpu):
     for i in range(0, 300, 11):
         alone, _ = b.run_raw([qs[i]])
         assert bytes(alone[0]) == got[i], i
+    from oracle import oracle as o
+    from voxel_amd._lib import Agg
+    from voxel_amd.scan import _result
+    for i in range(0, 300, 29):                                # served results against the oracle
+        q = qs[i]
+        r = _result(q.dtype, Agg.from_buffer_copy(got[i]))
+        a = _host(q.a, q.n, q.dtype)
+        bb = _host(q.b, q.n, q.dtype) if q.b and q.op_b != 6 else None
+        ref = o.relational(a, q.op_a, _thr(q.t_a, q.dtype), bb, q.op_b if bb is not None else 6, _thr(q.t_b, q.dtype))
+        assert r["count"] == ref["count"], i
+        if q.dtype in (0, 1) and ref["count"]:
+            ex = float(ref["sum"])
+            assert abs(r["sum"] - ex) <= (2 if q.dtype == 0 else 4) * _ulp(ex, q.dtype) + 1e-30, i
+        elif ref["count"]:
+            assert r["sum"] == ref["sum"], i
     print("live server: %d batches (mean %.2f), p50 %.1f us p99 %.1f us" % (
         st["batches"], st["mean_batch"], st["p50_s"] * 1e6, st["p99_s"] * 1e6))
