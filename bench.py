@@ -74,15 +74,33 @@ def parse():
     p.add_argument("--no-encodings", action="store_true")
     p.add_argument("--no-sql", action="store_true")
     p.add_argument("--sort-log2", type=int, default=29)
+    p.add_argument("--settle", type=float, default=1.0,
+                   help="seconds of untimed headline launches before the warm-up (clocks at steady state)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a device-to-device copy of one 2^rows-log2 f64 column (the known-size "
                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes)")
     return p.parse_args()
 
 
-def time_scan(desc, steps, warmup, stream_handle, settle_s=0.0):
+_MARK = {}
+
+
+def mark(stream_handle):
+    """A one-row i32 fill (k_fill<int> with a one-workgroup grid) on the launch
+    stream: in a rocprofv3 kernel trace the timed scans are the ones between
+    two such dispatches.  Launched outside the timed region."""
+    import torch
+    from voxel_amd._lib import lib
+    if "t" not in _MARK:
+        _MARK["t"] = torch.empty(1, dtype=torch.int32, device="cuda")
+    if lib.voxel_gpu_fill_uniform(3, _MARK["t"].data_ptr(), 1, 0, 0.0, 1.0, stream_handle) != 0:
+        raise RuntimeError("marker fill failed")
+
+
+def time_scan(desc, steps, warmup, stream_handle, settle_s=0.0, marks=False):
     """settle_s: untimed launches for that long first (clocks and caches at
-    steady state), then the W warm-up launches, then K timed launches."""
+    steady state), then the W warm-up launches, then K timed launches
+    (between two marker dispatches when marks is set)."""
     import torch
     from voxel_amd import scan
     t_end = time.perf_counter() + settle_s
@@ -92,6 +110,8 @@ def time_scan(desc, steps, warmup, stream_handle, settle_s=0.0):
         torch.cuda.synchronize()
     for _ in range(warmup):
         scan.launch(desc, stream_handle)
+    if marks:
+        mark(stream_handle)
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -103,6 +123,9 @@ def time_scan(desc, steps, warmup, stream_handle, settle_s=0.0):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = ev0.elapsed_time(ev1) / steps
+    if marks:
+        mark(stream_handle)
+        torch.cuda.synchronize()
     return wall, kernel_ms
 
 
@@ -752,7 +775,7 @@ def main():
 
     # headline: f64 compound at 50 %
     desc, es = desc_for("f64", 0.50)
-    wall, kernel_ms = time_scan(desc, args.steps, args.warmup, stream, settle_s=0.5)
+    wall, kernel_ms = time_scan(desc, args.steps, args.warmup, stream, settle_s=args.settle, marks=True)
     res = scan.result(0, stream)
     algo_bytes = 2 * n * es
     ms_per_step = wall * 1e3 / args.steps
