@@ -185,3 +185,24 @@ def test_groupby_full_size_trip_table(gpu):
     assert abs(s - tot["sum"]) <= 1e-12 * tot["sum"]
     cnt_ref = torch.bincount(zone.to(torch.int64), minlength=266)[1:266]
     assert torch.equal(cnt_ref.to(torch.int64), out[1][:g].to(torch.int64))
+
+
+@pytest.mark.parametrize("span", [512, 513])
+def test_groupby_window_limit_and_max_groups(gpu, span):
+    """Key windows of exactly kMaxDense = 512 keys stay on the dense path
+    decided on the device, 513 fall back to the sort path; both match the
+    oracle.  max_groups below the group count fails with the count set."""
+    import torch
+    from voxel_amd._lib import lib
+    rng = np.random.default_rng(span)
+    n = 2_000_000
+    keys = rng.integers(-100, -100 + span, n).astype(np.int32)
+    keys[:span] = np.arange(-100, -100 + span, dtype=np.int32)        # every key present
+    vals = rng.standard_normal(n) * 100
+    check(gpu_groupby(keys, vals), keys, vals)
+    kd, vd = torch.from_numpy(keys).cuda(), torch.from_numpy(vals).cuda()
+    out = [torch.zeros(16, dtype=torch.float64, device="cuda") for _ in range(6)]
+    ng = np.zeros(1, np.uint64)
+    rc = lib.voxel_gpu_groupby_f64(3, kd.data_ptr(), vd.data_ptr(), n, 16, *(t.data_ptr() for t in out),
+                                   ng.ctypes.data, None)
+    assert rc == -1 and int(ng[0]) == span and "max_groups" in lib.voxel_last_error().decode()

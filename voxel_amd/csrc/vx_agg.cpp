@@ -134,6 +134,20 @@ int voxel_gpu_groupby_f64(int key_dtype, const void* keys_dev, const double* val
     std::lock_guard<std::mutex> lk(api_mutex());
     StreamCtx* c = ctx_for(stream);
     if (!c) return -1;
+    // 32-bit keys: the residue-binned dense path decides on the device and
+    // writes the caller's arrays (one host synchronisation); a key range
+    // wider than 512 values falls back to groupby_f64 (sampled window, then
+    // the sort-based path)
+    if ((key_dtype == DT_I32 || key_dtype == DT_U32) && n > 0) {
+        const int64_t g = groupby_dense_device(key_dtype, keys_dev, vals_dev, n, max_groups, out_keys, out_count,
+                                               out_sum, out_avg, out_min, out_max, c);
+        if (g == -1) return -1;
+        if (g >= 0) {
+            if (ngroups) *ngroups = (uint64_t)g;
+            if ((uint64_t)g > max_groups) { set_error("group-by: %lld groups > max_groups %zu", (long long)g, max_groups); return -1; }
+            return 0;
+        }
+    }
     int64_t* k;
     GBin* r;
     int64_t g = groupby_f64(key_dtype, keys_dev, vals_dev, n, &k, &r, c, 0);

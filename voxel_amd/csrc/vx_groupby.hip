@@ -16,6 +16,13 @@
 // workgroups write partials that one workgroup per bin merges in workgroup
 // order.
 //
+// Residue path (32-bit keys, voxel_gpu_groupby_f64): the same per-wave LDS
+// bins indexed by key mod kMaxDense, so no key window is needed before the
+// pass; each workgroup records its exact key range and the result stands
+// when the whole range spans at most kMaxDense values (distinct residues).
+// Bin merge and output kernels write the caller's arrays and the group
+// count to pinned memory: one host synchronisation per call.
+//
 // General path (any key domain): keys become order-preserving u64 and are
 // sorted with their row ids by the radix sort (vx_radix.hip); group starts
 // are compacted (vx_compact.hpp); one wave per group reduces its rows
@@ -46,6 +53,7 @@ constexpr int kGWaves = kGThreads / 64;
 constexpr int kGChunk = 256;          // rows per wave step: 4 per lane
 constexpr int kGUnroll = 2;           // wave steps in flight
 constexpr size_t kGBinBytes = 4 * sizeof(double) + sizeof(uint32_t);
+constexpr int kResidSteps = 2;       // k_gb_resid: wave steps loaded before the LDS updates
 
 __device__ __forceinline__ void bin_init(GBin& b) {
     b.s = 0.0; b.c = 0.0; b.cnt = 0;
@@ -72,8 +80,13 @@ __device__ __forceinline__ void bin_merge(GBin& a, const GBin& o) {
 // counts, each indexed by wave copy * bins + bin.  A row costs one
 // ds_add_rtn_f64, one ds_read_b128 of its (min, max) pair and one ds_add_u32;
 // the compensation add is skipped when the rounding error is zero, and the
-// min / max atomics are issued only when the row beats the pair just read
-// (only this wave writes the copy, so the pair read is current).
+// min / max atomics are issued only when the row beats the pair read.
+// add_n issues the sum, count and pair reads of N rows back to back and
+// only then the dependent compensation and min / max atomics, so a wave
+// keeps N LDS round trips in flight instead of one per row.  That is exact
+// whatever the order: TwoSum of the value an atomic add returned recovers
+// that add's rounding error, and min / max only move one way, so a pair
+// read before another row's update is a conservative test.
 struct WaveBins {
     double* s;
     double* c;
@@ -90,6 +103,32 @@ struct WaveBins {
         if (v > p.y) __hip_atomic_fetch_max(&mm[i].y, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         __hip_atomic_fetch_add(&cnt[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    // rows whose bin is `skip` are not added (no LDS access)
+    template <int N>
+    __device__ __forceinline__ void add_n(const uint32_t (&b)[N], const double (&v)[N], uint32_t skip) const {
+        double old[N];
+        double2 p[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (b[j] != skip)
+                old[j] = __hip_atomic_fetch_add(&s[b[j]], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (b[j] != skip) __hip_atomic_fetch_add(&cnt[b[j]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (b[j] != skip) p[j] = mm[b[j]];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (b[j] == skip) continue;
+            const double t = old[j] + v[j];
+            const double bp = t - old[j];
+            const double e = (old[j] - (t - bp)) + (v[j] - bp);
+            if (e != 0.0) __hip_atomic_fetch_add(&c[b[j]], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (v[j] < p[j].x) __hip_atomic_fetch_min(&mm[b[j]].x, v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (v[j] > p[j].y) __hip_atomic_fetch_max(&mm[b[j]].y, v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
     __device__ __forceinline__ GBin get(uint32_t i) const {
         GBin b;
         const double2 p = mm[i];
@@ -98,18 +137,89 @@ struct WaveBins {
     }
 };
 
+// Key range of a sample of the key column: rows i * n / S for i < S (every
+// row when n <= S); workgroup b of kSampBlocks writes the (min, max) of its
+// rows to samp[2b], samp[2b + 1] (INT64_MAX, INT64_MIN when it saw none).
+constexpr uint32_t kSample = 65536;            // 2^16
+static_assert(kSample == (1u << 16), "the sample row index divides by 2^16 with a shift");
+constexpr int kSampBlocks = 32, kSampThreads = 512;
+constexpr int kSampPer = (int)(kSample / (kSampBlocks * kSampThreads));
+
+__device__ __forceinline__ void wave_minmax(int64_t& lo, int64_t& hi) {
+    for (int m = 32; m > 0; m >>= 1) {
+        const int64_t x = __shfl_xor(lo, m, 64), y = __shfl_xor(hi, m, 64);
+        lo = x < lo ? x : lo;
+        hi = y > hi ? y : hi;
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSampThreads) void k_key_sample(const K* __restrict__ keys, uint64_t n,
+                                                             long long* __restrict__ samp) {
+    __shared__ int64_t red[2][kSampThreads / 64];
+    const uint64_t S = n < kSample ? n : kSample;
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+#pragma unroll
+    for (int j = 0; j < kSampPer; ++j) {
+        const uint64_t i = (uint64_t)j * (kSampBlocks * kSampThreads) + (uint64_t)blockIdx.x * kSampThreads + threadIdx.x;
+        if (i < S) {
+            // S == n, or S == kSample = 2^16: row floor(i * n / 2^16) from the 128-bit product
+            const uint64_t r = S == n ? i : ((__umul64hi(i, n) << 48) | ((i * n) >> 16));
+            const int64_t k = (int64_t)keys[r];
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
+    }
+    wave_minmax(lo, hi);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = lo; red[1][w] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int ww = 1; ww < kSampThreads / 64; ++ww) {
+            lo = red[0][ww] < lo ? red[0][ww] : lo;
+            hi = red[1][ww] > hi ? red[1][ww] : hi;
+        }
+        samp[2 * blockIdx.x] = lo;
+        samp[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+// Exact key range and rows outside the window over the checked dense
+// workgroups' (min, max, outside) records, by one whole wave; lane 0
+// writes out[0..2].
+__device__ __forceinline__ void fold_ranges(const long long* __restrict__ rngp, uint32_t nblocks,
+                                            long long* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    unsigned long long miss = 0;
+    for (uint32_t j = lane; j < nblocks; j += 64) {
+        lo = rngp[3 * j] < lo ? rngp[3 * j] : lo;
+        hi = rngp[3 * j + 1] > hi ? rngp[3 * j + 1] : hi;
+        miss += (unsigned long long)rngp[3 * j + 2];
+    }
+    wave_minmax(lo, hi);
+    for (int m = 32; m > 0; m >>= 1) miss += __shfl_xor(miss, m, 64);
+    if (lane == 0) { out[0] = lo; out[1] = hi; out[2] = (long long)miss; }
+}
+
+__device__ __forceinline__ bool window_ok(int64_t lo, int64_t hi) {
+    return hi >= lo && (uint64_t)(hi - lo) < (uint64_t)kMaxDense;
+}
+
 // Rows [0, n): wave steps of kGChunk rows go to waves round-robin (wave
 // wid takes steps wid, wid + waves, ...).  VEC: keys and values 16-byte
 // aligned, so a lane reads its 4 consecutive rows with non-temporal 16-byte
 // loads.  CHECK: the key window [kmin, kmin + nbins) came from a sample;
-// rows outside it go to a spare bin (index nbins) that is never read and
-// are counted in range[2], and the exact key range is folded into range[0]
-// (min) and range[1] (max).  Each wave copy holds nbins + 1 bins.
+// rows outside it go to a spare bin (index nbins) that is never read, and
+// each workgroup writes (min key, max key, rows outside) to rngp[3b..3b+2]
+// (plain stores: no contended atomics).  Each wave copy holds nbins + 1
+// bins.
 template <typename K, bool VEC, bool CHECK>
 __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ keys, const double* __restrict__ vals,
                                                         uint64_t n, int64_t kmin, uint32_t nbins,
-                                                        GBin* __restrict__ parts, long long* __restrict__ range) {
+                                                        GBin* __restrict__ parts, long long* __restrict__ rngp) {
     extern __shared__ double gsm[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t nb1 = nbins + 1;
     const uint32_t tot = kGWaves * nb1;
     WaveBins all{gsm, gsm + tot, reinterpret_cast<double2*>(gsm + 2 * tot), reinterpret_cast<uint32_t*>(gsm + 4 * tot)};
@@ -118,12 +228,11 @@ __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ ke
         all.mm[i] = make_double2(__builtin_huge_val(), -__builtin_huge_val());
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t off = (uint32_t)w * nb1;
     WaveBins mine{all.s + off, all.c + off, all.mm + off, all.cnt + off};
     K klo = std::numeric_limits<K>::max(), khi = std::numeric_limits<K>::lowest();
     uint32_t miss = 0;
-    auto row = [&](K k, double v) {
+    auto bin_of = [&](K k) {
         uint32_t b = (uint32_t)((int64_t)k - kmin);
         if constexpr (CHECK) {
             klo = k < klo ? k : klo;
@@ -131,8 +240,9 @@ __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ ke
             miss += b >= nbins ? 1u : 0u;
             b = b < nbins ? b : nbins;
         }
-        mine.add(b, v);
+        return b;
     };
+    auto row = [&](K k, double v) { mine.add(bin_of(k), v); };
     const uint64_t waves = (uint64_t)gridDim.x * kGWaves;
     const uint64_t wid = (uint64_t)blockIdx.x * kGWaves + w;
     const uint64_t steps = n / kGChunk;
@@ -157,10 +267,16 @@ __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ ke
                 }
             }
         }
+        uint32_t bb[kGUnroll * 4];
+        double vv[kGUnroll * 4];
 #pragma unroll
         for (int u = 0; u < kGUnroll; ++u)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) row(kk[u][j], v[u][j]);
+            for (int j = 0; j < 4; ++j) {
+                bb[u * 4 + j] = bin_of(kk[u][j]);
+                vv[u * 4 + j] = v[u][j];
+            }
+        mine.add_n(bb, vv, nbins);                   // rows outside a checked window are dropped
     }
     for (; st < steps; st += waves) {
         const uint64_t base = st * kGChunk;
@@ -169,23 +285,28 @@ __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ ke
     }
     if (wid == waves - 1)                                   // rows past the last whole step
         for (uint64_t r = steps * kGChunk + lane; r < n; r += 64) row(keys[r], vals[r]);
+    __shared__ int64_t s_rng[3][kGWaves];
     if constexpr (CHECK) {
         int64_t l = (int64_t)klo, h = (int64_t)khi;
-        for (int m = 32; m > 0; m >>= 1) {
-            const int64_t x = __shfl_xor(l, m, 64), y = __shfl_xor(h, m, 64);
-            l = x < l ? x : l;
-            h = y > h ? y : h;
-        }
+        wave_minmax(l, h);
         uint32_t msum = miss;
-        if (__ballot(miss != 0))
-            for (int m = 32; m > 0; m >>= 1) msum += __shfl_xor(msum, m, 64);
-        if (lane == 0) {
-            atomicMin(&range[0], (long long)l);
-            atomicMax(&range[1], (long long)h);
-            if (msum) atomicAdd((unsigned long long*)&range[2], (unsigned long long)msum);
-        }
+        for (int m = 32; m > 0; m >>= 1) msum += __shfl_xor(msum, m, 64);
+        if (lane == 0) { s_rng[0][w] = l; s_rng[1][w] = h; s_rng[2][w] = msum; }
     }
     __syncthreads();
+    if constexpr (CHECK) {
+        if (threadIdx.x == 0) {
+            int64_t l = s_rng[0][0], h = s_rng[1][0], m = s_rng[2][0];
+            for (int ww = 1; ww < kGWaves; ++ww) {
+                l = s_rng[0][ww] < l ? s_rng[0][ww] : l;
+                h = s_rng[1][ww] > h ? s_rng[1][ww] : h;
+                m += s_rng[2][ww];
+            }
+            rngp[3 * blockIdx.x] = l;
+            rngp[3 * blockIdx.x + 1] = h;
+            rngp[3 * blockIdx.x + 2] = m;
+        }
+    }
     GBin* out = parts + (size_t)blockIdx.x * nbins;
     for (uint32_t i = threadIdx.x; i < nbins; i += kGThreads) {
         GBin acc = all.get(i);
@@ -194,11 +315,135 @@ __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ ke
     }
 }
 
-// one workgroup per bin: partials of all workgroups merged in a fixed order
+// Dense group-by without a key window: a row goes to bin key mod kMaxDense
+// (the low bits of the key's two's-complement value), so any key range of
+// at most kMaxDense consecutive values lands in distinct bins and the range
+// itself need not be known up front.  Each workgroup writes its exact
+// (min key, max key) to rngp[2b..2b+1]; the merge kernel accepts the result
+// only when max - min < kMaxDense.  Rows [0, n) in wave steps of kGChunk
+// rows, round-robin over the waves, U steps (4U rows per lane) loaded
+// before the LDS updates; the rows after the last whole step are read row
+// by row with bounds checks by the wave whose turn that step is.  Steps
+// past the end and rows past n go to the spare bin kMaxDense, which add_n
+// skips.  VEC as in k_gb_dense.
+template <typename K, bool VEC, int U>
+__global__ __launch_bounds__(kGThreads) void k_gb_resid(const K* __restrict__ keys, const double* __restrict__ vals,
+                                                        uint64_t n, GBin* __restrict__ parts,
+                                                        long long* __restrict__ rngp) {
+    static_assert(U % 2 == 0, "rows go to add_n eight at a time");
+    extern __shared__ double gsm[];
+    constexpr uint32_t nb1 = kMaxDense + 1, tot = kGWaves * nb1, spare = kMaxDense;
+    constexpr uint32_t rmask = kMaxDense - 1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    WaveBins all{gsm, gsm + tot, reinterpret_cast<double2*>(gsm + 2 * tot), reinterpret_cast<uint32_t*>(gsm + 4 * tot)};
+    for (uint32_t i = threadIdx.x; i < tot; i += kGThreads) {
+        all.s[i] = 0.0; all.c[i] = 0.0; all.cnt[i] = 0;
+        all.mm[i] = make_double2(__builtin_huge_val(), -__builtin_huge_val());
+    }
+    __syncthreads();
+    const uint32_t off = (uint32_t)w * nb1;
+    WaveBins mine{all.s + off, all.c + off, all.mm + off, all.cnt + off};
+    K klo = std::numeric_limits<K>::max(), khi = std::numeric_limits<K>::lowest();
+    const uint64_t waves = (uint64_t)gridDim.x * kGWaves;
+    const uint64_t wid = (uint64_t)blockIdx.x * kGWaves + w;
+    const uint64_t full = n / kGChunk;
+    // whole steps; a step past `full` (the last iteration's) goes to the spare bin
+    for (uint64_t st = wid; st < full; st += (uint64_t)U * waves) {
+        K kk[U][4];
+        double v[U][4];
+        bool live[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t sx = st + (uint64_t)u * waves;
+            const uint64_t base = sx * kGChunk;
+            live[u] = sx < full;
+            if (live[u]) {
+                if constexpr (VEC) {
+                    uint4 k4 = ld16<uint4, true>(reinterpret_cast<const uint4*>(keys + base + 4 * lane));
+                    double2 v0 = ld16<double2, true>(reinterpret_cast<const double2*>(vals + base + 4 * lane));
+                    double2 v1 = ld16<double2, true>(reinterpret_cast<const double2*>(vals + base + 4 * lane + 2));
+                    kk[u][0] = (K)k4.x; kk[u][1] = (K)k4.y; kk[u][2] = (K)k4.z; kk[u][3] = (K)k4.w;
+                    v[u][0] = v0.x; v[u][1] = v0.y; v[u][2] = v1.x; v[u][3] = v1.y;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        kk[u][j] = keys[base + 64 * j + lane];
+                        v[u][j] = vals[base + 64 * j + lane];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) { kk[u][j] = 0; v[u][j] = 0.0; }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < U; h += 2) {
+            uint32_t bb[8];
+            double vv[8];
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const K k = kk[h + u][j];
+                    // after every load of the batch is issued: a use of a key next
+                    // to its load would wait for it before the next step's loads
+                    klo = live[h + u] && k < klo ? k : klo;
+                    khi = live[h + u] && k > khi ? k : khi;
+                    bb[u * 4 + j] = live[h + u] ? ((uint32_t)k & rmask) : spare;
+                    vv[u * 4 + j] = v[h + u][j];
+                }
+            mine.add_n(bb, vv, spare);
+        }
+    }
+    // rows past the last whole step: the wave whose turn step `full` is
+    if (full * kGChunk < n && wid == full % waves) {
+        uint32_t bb[4];
+        double vv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t r = full * kGChunk + 64 * j + lane;
+            bb[j] = spare;
+            vv[j] = 0.0;
+            if (r < n) {
+                const K k = keys[r];
+                klo = k < klo ? k : klo;
+                khi = k > khi ? k : khi;
+                bb[j] = (uint32_t)k & rmask;
+                vv[j] = vals[r];
+            }
+        }
+        mine.add_n(bb, vv, spare);
+    }
+    __shared__ int64_t s_rng[2][kGWaves];
+    int64_t l = (int64_t)klo, hh = (int64_t)khi;
+    wave_minmax(l, hh);
+    if (lane == 0) { s_rng[0][w] = l; s_rng[1][w] = hh; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int ww = 1; ww < kGWaves; ++ww) {
+            l = s_rng[0][ww] < l ? s_rng[0][ww] : l;
+            hh = s_rng[1][ww] > hh ? s_rng[1][ww] : hh;
+        }
+        rngp[2 * blockIdx.x] = l;
+        rngp[2 * blockIdx.x + 1] = hh;
+    }
+    GBin* out = parts + (size_t)blockIdx.x * kMaxDense;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxDense; i += kGThreads) {
+        GBin acc = all.get(i);
+        for (int ww = 1; ww < kGWaves; ++ww) bin_merge(acc, all.get((uint32_t)ww * nb1 + i));
+        out[i] = acc;
+    }
+}
+
+// one workgroup per bin: partials of all workgroups merged in a fixed order;
+// with rngp, the first wave of workgroup 0 also folds the checked dense
+// workgroups' range records into mm[2..4]
 __global__ __launch_bounds__(256) void k_gb_final(const GBin* __restrict__ parts, uint32_t nblocks, uint32_t nbins,
-                                                  GBin* __restrict__ out) {
+                                                  GBin* __restrict__ out, const long long* __restrict__ rngp,
+                                                  long long* __restrict__ mm) {
     __shared__ GBin red[256];
     const uint32_t bin = blockIdx.x;
+    if (rngp && bin == 0 && threadIdx.x < 64) fold_ranges(rngp, nblocks, mm + 2);
     GBin acc; bin_init(acc);
     for (uint32_t j = threadIdx.x; j < nblocks; j += 256) bin_merge(acc, parts[(size_t)j * nbins + bin]);
     red[threadIdx.x] = acc;
@@ -210,32 +455,81 @@ __global__ __launch_bounds__(256) void k_gb_final(const GBin* __restrict__ parts
     if (threadIdx.x == 0) out[bin] = red[0];
 }
 
-// range[0..1] = key range of a sample: rows i * n / S for i < S (every row
-// when n <= S), one row per thread; range[0..4] start as (max, min, max,
-// min, 0) from k_range_init, and the checked dense kernel folds the exact
-// range and the rows outside the window into range[2..4].
-constexpr uint32_t kSample = 65536;
-__global__ void k_range_init(long long* range) {
-    range[0] = INT64_MAX; range[1] = INT64_MIN; range[2] = INT64_MAX; range[3] = INT64_MIN; range[4] = 0;
+__device__ __forceinline__ void bin_shfl_merge(GBin& a, int m) {
+    GBin o;
+    o.s = __shfl_xor(a.s, m, 64);
+    o.c = __shfl_xor(a.c, m, 64);
+    o.mn = __shfl_xor(a.mn, m, 64);
+    o.mx = __shfl_xor(a.mx, m, 64);
+    o.cnt = __shfl_xor(a.cnt, m, 64);
+    bin_merge(a, o);
 }
-template <typename K>
-__global__ __launch_bounds__(256) void k_key_sample(const K* __restrict__ keys, uint64_t n,
-                                                    long long* __restrict__ range) {
-    const uint64_t S = n < kSample ? n : kSample;
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    int64_t lo = INT64_MAX, hi = INT64_MIN;
-    if (i < S) {
-        const uint64_t r = S == n ? i : (uint64_t)(((unsigned __int128)i * n) / S);
-        lo = hi = (int64_t)keys[r];
+
+// Bin merge of k_gb_resid's partials: one wave per bin; lane l merges the
+// partials of workgroups l, l + 64, ... in order, then a fixed butterfly;
+// lane 0 stores the bin.
+constexpr int kMergeThreads = 256;
+__global__ __launch_bounds__(kMergeThreads) void k_gb_merge_resid(const GBin* __restrict__ parts, uint32_t nblocks,
+                                                                   GBin* __restrict__ bins) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t bin = blockIdx.x * (kMergeThreads / 64) + (threadIdx.x >> 6);
+    if (bin >= (uint32_t)kMaxDense) return;
+    GBin acc; bin_init(acc);
+    for (uint32_t j = lane; j < nblocks; j += 64) bin_merge(acc, parts[(size_t)j * kMaxDense + bin]);
+    for (int m = 1; m < 64; m <<= 1) bin_shfl_merge(acc, m);
+    if (lane == 0) bins[bin] = acc;
+}
+
+// Output of the residue path, one workgroup of kMaxDense threads: folds the
+// workgroups' key ranges and, when max - min < kMaxDense, writes the
+// non-empty bins in key order (key lo + t is bin (lo + t) mod kMaxDense)
+// straight into the caller's arrays (the k_gb_finish formulas; rows past
+// max_groups are not written).  Result in pinned host memory: hres[0] =
+// group count, or -1 when the key range is wider than kMaxDense;
+// hres[2..3] = the key range.
+__global__ __launch_bounds__(kMaxDense) void k_gb_emit_resid(
+    const GBin* __restrict__ bins, uint32_t nblocks, const long long* __restrict__ rngp, uint64_t max_groups,
+    int64_t* __restrict__ ok, uint64_t* __restrict__ oc, double* __restrict__ os, double* __restrict__ oa,
+    double* __restrict__ omn, double* __restrict__ omx, volatile long long* __restrict__ hres) {
+    __shared__ uint32_t wc[kMaxDense / 64];
+    __shared__ int64_t s_win[2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w == 0) {
+        int64_t lo = INT64_MAX, hi = INT64_MIN;
+        for (uint32_t j = lane; j < nblocks; j += 64) {
+            lo = rngp[2 * j] < lo ? rngp[2 * j] : lo;
+            hi = rngp[2 * j + 1] > hi ? rngp[2 * j + 1] : hi;
+        }
+        wave_minmax(lo, hi);
+        if (lane == 0) { s_win[0] = lo; s_win[1] = hi; }
     }
-    for (int m = 32; m > 0; m >>= 1) {
-        const int64_t x = __shfl_xor(lo, m, 64), y = __shfl_xor(hi, m, 64);
-        lo = x < lo ? x : lo;
-        hi = y > hi ? y : hi;
+    __syncthreads();
+    const int64_t lo = s_win[0], hi = s_win[1];
+    if (!window_ok(lo, hi)) {
+        if (threadIdx.x == 0) { hres[0] = -1; hres[2] = lo; hres[3] = hi; }
+        return;
     }
-    if ((threadIdx.x & 63) == 0 && lo <= hi) {
-        atomicMin(&range[0], (long long)lo);
-        atomicMax(&range[1], (long long)hi);
+    const uint32_t nbins = (uint32_t)(hi - lo) + 1, t = threadIdx.x;
+    GBin b;
+    const bool live = t < nbins && (b = bins[(uint32_t)(lo + (int64_t)t) & (uint32_t)(kMaxDense - 1)], b.cnt > 0);
+    const unsigned long long msk = __ballot(live);
+    if (lane == 0) wc[w] = (uint32_t)__popcll(msk);
+    __syncthreads();
+    uint32_t pos = (uint32_t)__popcll(msk & (lane ? (~0ull >> (64 - lane)) : 0ull));
+    for (int ww = 0; ww < w; ++ww) pos += wc[ww];
+    if (live && pos < max_groups) {
+        const double sum = b.s + b.c;
+        if (ok) ok[pos] = lo + (int64_t)t;
+        if (oc) oc[pos] = b.cnt;
+        if (os) os[pos] = sum;
+        if (oa) oa[pos] = b.cnt ? sum / (double)b.cnt : 0.0;     // AggregateState::Avg (aggregate.hpp:136-139)
+        if (omn) omn[pos] = b.mn;
+        if (omx) omx[pos] = b.mx;
+    }
+    if (t == 0) {
+        uint32_t c = 0;
+        for (int ww = 0; ww < kMaxDense / 64; ++ww) c += wc[ww];
+        hres[0] = c; hres[2] = lo; hres[3] = hi;
     }
 }
 
@@ -350,8 +644,11 @@ struct GroupbyWork {
     GBin* parts = nullptr;
     uint64_t parts_cap = 0;      // GBin records
     GBin* bins = nullptr;        // kMaxDense
-    long long* mm = nullptr;
-    long long* mm_host = nullptr;
+    long long* mm = nullptr;     // device: [2..4] folded ranges of a checked dense pass
+    long long* mm_host = nullptr;  // pinned: [0..4] key window and ranges, [8..11] residue-path result, [16..] sample records
+    long long* samp = nullptr;   // kSampBlocks (min, max) sample records
+    long long* rngp = nullptr;   // (min, max, outside) per checked dense workgroup
+    uint32_t rng_cap = 0;
     // general path
     uint64_t* k1 = nullptr; uint64_t* k2 = nullptr; uint32_t* r1 = nullptr; uint32_t* r2 = nullptr;
     uint64_t* starts = nullptr; GBin* rec = nullptr;
@@ -361,6 +658,7 @@ struct GroupbyWork {
     GBin* orec[2] = {nullptr, nullptr};
     uint64_t ocap[2] = {0, 0};
     int cur = 0;
+    int dev_occ = 0;             // workgroups per CU of the device-window dense kernel
 };
 
 static GroupbyWork& gwork(StreamCtx* c) {
@@ -368,6 +666,34 @@ static GroupbyWork& gwork(StreamCtx* c) {
     for (auto& p : all) if (p.first == c) return *p.second;
     all.emplace_back(c, new GroupbyWork());
     return *all.back().second;
+}
+
+static bool ensure_small(GroupbyWork& w, hipStream_t st) {
+    if (w.mm) return true;
+    if (!hip_ok(hipMalloc((void**)&w.mm, 64), "hipMalloc") || !hip_ok(hipMemsetAsync(w.mm, 0, 64, st), "hipMemsetAsync") ||
+        !hip_ok(hipHostMalloc((void**)&w.mm_host, 1024, hipHostMallocDefault), "hipHostMalloc") ||
+        !hip_ok(hipMalloc((void**)&w.bins, kMaxDense * sizeof(GBin)), "hipMalloc") ||
+        !hip_ok(hipMalloc((void**)&w.samp, 2 * kSampBlocks * sizeof(long long)), "hipMalloc"))
+        return false;
+    return true;
+}
+
+static bool ensure_rng(GroupbyWork& w, uint32_t blocks) {
+    if (blocks <= w.rng_cap) return true;
+    if (w.rngp) (void)hipFree(w.rngp);
+    w.rngp = nullptr; w.rng_cap = 0;
+    if (!hip_ok(hipMalloc((void**)&w.rngp, (size_t)blocks * 3 * sizeof(long long)), "hipMalloc(ranges)")) return false;
+    w.rng_cap = blocks;
+    return true;
+}
+
+// Launches the key sample into w.samp.
+static bool launch_sample(GroupbyWork& w, int key_dtype, const void* keys, uint64_t n, hipStream_t st) {
+    if (key_dtype == DT_I32)
+        hipLaunchKernelGGL(k_key_sample<int32_t>, dim3(kSampBlocks), dim3(kSampThreads), 0, st, (const int32_t*)keys, n, w.samp);
+    else
+        hipLaunchKernelGGL(k_key_sample<uint32_t>, dim3(kSampBlocks), dim3(kSampThreads), 0, st, (const uint32_t*)keys, n, w.samp);
+    return hip_ok(hipGetLastError(), "sample launch");
 }
 
 static bool ensure_out(GroupbyWork& w, int which, uint64_t n) {
@@ -446,10 +772,15 @@ static int64_t dense_pass(GroupbyWork& w, int key_dtype, const void* keys, const
         if (!hip_ok(hipMalloc((void**)&w.parts, recs * sizeof(GBin)), "hipMalloc(partials)")) return -1;
         w.parts_cap = recs;
     }
-    long long* range = w.mm + 2;                  // exact min, max, rows outside the window
-    void* args[] = {(void*)&keys, (void*)&vals, (void*)&n, (void*)&kmin, (void*)&nbins, (void*)&w.parts, (void*)&range};
+    if (check && !ensure_rng(w, blocks)) return -1;
+    long long* rngp = check ? w.rngp : nullptr;   // per-workgroup exact min, max, rows outside the window
+    const long long* no_samp = nullptr;
+    long long* no_mm = nullptr;
+    void* args[] = {(void*)&keys, (void*)&vals, (void*)&n, (void*)&kmin, (void*)&nbins, (void*)&w.parts,
+                    (void*)&rngp, (void*)&no_samp, (void*)&no_mm};
     if (!hip_ok(hipLaunchKernel(fn, dim3(blocks), dim3(kGThreads), args, lds, st), "group-by launch")) return -1;
-    hipLaunchKernelGGL(k_gb_final, dim3(nbins), dim3(256), 0, st, w.parts, blocks, nbins, w.bins);
+    hipLaunchKernelGGL(k_gb_final, dim3(nbins), dim3(256), 0, st, w.parts, blocks, nbins, w.bins,
+                       (const long long*)rngp, w.mm);
     if (!hip_ok(hipGetLastError(), "group-by final launch")) return -1;
     if (check && !hip_ok(hipMemcpyAsync(w.mm_host + 2, w.mm + 2, 24, hipMemcpyDeviceToHost, st), "read key range"))
         return -1;
@@ -457,6 +788,59 @@ static int64_t dense_pass(GroupbyWork& w, int key_dtype, const void* keys, const
     uint64_t ng = 0;
     if (!run_compact(src, nbins, c, &ng)) return -1;      // synchronises the stream
     return (int64_t)ng;
+}
+
+// The dense path of a 32-bit key column decided on the device: residue
+// bins (k_gb_resid), the bin merge (k_gb_merge_resid) and the output
+// (k_gb_emit_resid) into the caller's arrays, one host synchronisation.  Returns the group count; -2
+// when the key range is wider than kMaxDense (the caller runs groupby_f64).
+int64_t groupby_dense_device(int key_dtype, const void* keys, const double* vals, uint64_t n, uint64_t max_groups,
+                             int64_t* ok, uint64_t* oc, double* os, double* oa, double* omn, double* omx,
+                             StreamCtx* c) {
+    GroupbyWork& w = gwork(c);
+    hipStream_t st = c->stream;
+    if (!ensure_small(w, st)) return -1;
+    const bool i32 = key_dtype == DT_I32;
+    const bool vec = ((uintptr_t)keys & 15) == 0 && ((uintptr_t)vals & 15) == 0;
+    constexpr int U = kResidSteps;
+    const void* fns[2][2] = {{(const void*)k_gb_resid<uint32_t, false, U>, (const void*)k_gb_resid<uint32_t, true, U>},
+                             {(const void*)k_gb_resid<int32_t, false, U>, (const void*)k_gb_resid<int32_t, true, U>}};
+    const void* fn = fns[i32][vec];
+    const size_t lds = (size_t)kGWaves * (kMaxDense + 1) * kGBinBytes;
+    if (w.dev_occ == 0) {
+        int occ = 0;
+        if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kGThreads, lds), "occupancy")) return -1;
+        w.dev_occ = occ < 1 ? 1 : occ;
+    }
+    unsigned blocks = (unsigned)device_cu_count() * (unsigned)w.dev_occ;
+    const uint64_t per_block = (uint64_t)kGChunk * U * kGWaves;       // rows one workgroup takes per iteration
+    const uint64_t need = (n + per_block - 1) / per_block;
+    if (need < blocks) blocks = (unsigned)(need ? need : 1);
+    // a wave's bin counts are u32: it sees at most ceil(steps / waves) steps
+    const uint64_t steps = (n + kGChunk - 1) / kGChunk;
+    if ((steps + (uint64_t)blocks * kGWaves - 1) / ((uint64_t)blocks * kGWaves) * kGChunk >= (1ull << 32)) {
+        set_error("group-by: %llu rows exceed the dense path's per-wave count", (unsigned long long)n);
+        return -1;
+    }
+    const uint64_t recs = (uint64_t)blocks * kMaxDense;
+    if (recs > w.parts_cap) {
+        if (w.parts) (void)hipFree(w.parts);
+        w.parts = nullptr; w.parts_cap = 0;
+        if (!hip_ok(hipMalloc((void**)&w.parts, recs * sizeof(GBin)), "hipMalloc(partials)")) return -1;
+        w.parts_cap = recs;
+    }
+    if (!ensure_rng(w, blocks)) return -1;
+    void* args[] = {(void*)&keys, (void*)&vals, (void*)&n, (void*)&w.parts, (void*)&w.rngp};
+    volatile long long* hres = w.mm_host + 8;
+    if (!hip_ok(hipLaunchKernel(fn, dim3(blocks), dim3(kGThreads), args, lds, st), "group-by launch")) return -1;
+    hipLaunchKernelGGL(k_gb_merge_resid, dim3(kMaxDense / (kMergeThreads / 64)), dim3(kMergeThreads), 0, st,
+                       (const GBin*)w.parts, blocks, w.bins);
+    hipLaunchKernelGGL(k_gb_emit_resid, dim3(1), dim3(kMaxDense), 0, st, (const GBin*)w.bins, blocks,
+                       (const long long*)w.rngp, max_groups, ok, oc, os, oa, omn, omx, hres);
+    if (!hip_ok(hipGetLastError(), "group-by launch") || !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+        return -1;
+    if (hres[0] < 0) return -2;                            // key range wider than kMaxDense
+    return hres[0];
 }
 
 // Group-by over device columns; group records in ascending key order.
@@ -473,25 +857,25 @@ int64_t groupby_f64(int key_dtype, const void* keys, const double* vals, uint64_
     if (n == 0) return 0;
     const bool k32 = key_dtype == DT_I32 || key_dtype == DT_U32;
     const int is_signed = (key_dtype == DT_I32 || key_dtype == DT_I64) ? 1 : 0;
-    if (!w.mm) {
-        if (!hip_ok(hipMalloc((void**)&w.mm, 64), "hipMalloc")) return -1;
-        if (!hip_ok(hipHostMalloc((void**)&w.mm_host, 64, hipHostMallocDefault), "hipHostMalloc")) return -1;
-        if (!hip_ok(hipMalloc((void**)&w.bins, kMaxDense * sizeof(GBin)), "hipMalloc")) return -1;
-    }
+    if (!ensure_small(w, st)) return -1;
     if (key_dtype != DT_I32 && key_dtype != DT_U32 && key_dtype != DT_I64) {
         set_error("group-by: key dtype %d not supported (i32, u32, i64)", key_dtype);
         return -1;
     }
     if (!force_general && k32) {
         // key window from a sample of kSample rows (all rows when n <= kSample)
-        const unsigned sblocks = (unsigned)(((n < kSample ? n : kSample) + 255) / 256);
-        hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, st, w.mm);
-        if (key_dtype == DT_I32) hipLaunchKernelGGL(k_key_sample<int32_t>, dim3(sblocks), dim3(256), 0, st, (const int32_t*)keys, n, w.mm);
-        else hipLaunchKernelGGL(k_key_sample<uint32_t>, dim3(sblocks), dim3(256), 0, st, (const uint32_t*)keys, n, w.mm);
-        if (!hip_ok(hipGetLastError(), "sample launch") ||
-            !hip_ok(hipMemcpyAsync(w.mm_host, w.mm, 16, hipMemcpyDeviceToHost, st), "read key range") ||
+        long long* hs = w.mm_host + 16;
+        if (!launch_sample(w, key_dtype, keys, n, st) ||
+            !hip_ok(hipMemcpyAsync(hs, w.samp, 2 * kSampBlocks * sizeof(long long), hipMemcpyDeviceToHost, st),
+                    "read key sample") ||
             !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
             return -1;
+        for (int b = 1; b < kSampBlocks; ++b) {
+            hs[0] = hs[2 * b] < hs[0] ? hs[2 * b] : hs[0];
+            hs[1] = hs[2 * b + 1] > hs[1] ? hs[2 * b + 1] : hs[1];
+        }
+        w.mm_host[0] = hs[0];
+        w.mm_host[1] = hs[1];
         int64_t kmin = w.mm_host[0];
         uint64_t span = (uint64_t)(w.mm_host[1] - kmin) + 1;
         bool exact = n <= kSample;
