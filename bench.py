@@ -312,9 +312,10 @@ def bench_groupby() -> dict:
                                          ng.ctypes.data, None) != 0:
                 raise RuntimeError("groupby failed")
 
-        t = _sync_time(run)
+        t = _sync_time(run, reps=15, warmup=5)
         out["rows_%d" % rows] = {"ms": round(t * 1e3, 3), "groups": int(ng[0]),
                                  "gb_per_s": round(rows * 12 / t / 1e9, 1),
+                                 "frac_of_8tbs": round(rows * 12 / t / 8e12, 4),
                                  "count_total_ok": int(res[1][: int(ng[0])].sum().item()) == rows}
         del zone, fare, res
         torch.cuda.empty_cache()

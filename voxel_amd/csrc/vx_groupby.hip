@@ -315,6 +315,71 @@ __global__ __launch_bounds__(kGThreads) void k_gb_dense(const K* __restrict__ ke
     }
 }
 
+// f32 guards of an exact f64 minimum / maximum: guard_up(v) >= v and
+// guard_dn(v) <= v, never an f32 subnormal (LDS float atomics may flush
+// those to zero, which would break the bound).
+__device__ __forceinline__ float guard_up(double v) {
+    const float f = __double2float_ru(v);
+    return (f > 0.0f && f < __FLT_MIN__) ? __FLT_MIN__ : f;
+}
+__device__ __forceinline__ float guard_dn(double v) {
+    const float f = __double2float_rd(v);
+    return (f < 0.0f && f > -__FLT_MIN__) ? -__FLT_MIN__ : f;
+}
+
+// Bins of the residue path.  Sum and compensation are per wave copy (f64
+// adds round differently in another order, so only one wave writes a
+// copy); count, exact (min, max) and the f32 guard pair are shared by the
+// workgroup: integer adds and min / max give the same bits in any order.
+// A row costs ds_add_rtn_f64 (sum), ds_add_f64 (compensation, skipped when
+// the rounding error is zero), ds_add_u32 (count) and one ds_read_b64 of
+// its guard pair; the exact min / max atomics (and the guard's) are issued
+// only when the row is below the min guard or above the max guard.  The
+// guards bound the exact values (gd.x >= mm.x, gd.y <= mm.y): a row's
+// guard update is issued together with its exact update, so a row at or
+// above a guard it reads is at or above the final minimum.
+struct ResidBins {
+    double* s;           // this wave's copy
+    double* c;
+    double2* mm;         // shared by the workgroup
+    float2* gd;
+    uint32_t* cnt;
+    template <int N>
+    __device__ __forceinline__ void add_n(const uint32_t (&b)[N], const double (&v)[N], uint32_t skip) const {
+        double old[N];
+        float2 g[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (b[j] != skip)
+                old[j] = __hip_atomic_fetch_add(&s[b[j]], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (b[j] != skip) __hip_atomic_fetch_add(&cnt[b[j]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (b[j] != skip) g[j] = gd[b[j]];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (b[j] == skip) continue;
+            const double t = old[j] + v[j];
+            const double bp = t - old[j];
+            const double e = (old[j] - (t - bp)) + (v[j] - bp);
+            if (e != 0.0) __hip_atomic_fetch_add(&c[b[j]], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (v[j] < (double)g[j].x) {
+                __hip_atomic_fetch_min(&mm[b[j]].x, v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_min(&gd[b[j]].x, guard_up(v[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            if (v[j] > (double)g[j].y) {
+                __hip_atomic_fetch_max(&mm[b[j]].y, v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_max(&gd[b[j]].y, guard_dn(v[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+};
+// LDS bytes of the residue path's bins: per wave copy (s, c), per workgroup
+// (mm, gd, cnt), kMaxDense + 1 bins each (the last one the spare bin).
+constexpr size_t kResidLds = ((size_t)kGWaves * 16 + 16 + 8 + 4) * (kMaxDense + 1);
+
 // Dense group-by without a key window: a row goes to bin key mod kMaxDense
 // (the low bits of the key's two's-complement value), so any key range of
 // at most kMaxDense consecutive values lands in distinct bins and the range
@@ -334,15 +399,23 @@ __global__ __launch_bounds__(kGThreads) void k_gb_resid(const K* __restrict__ ke
     extern __shared__ double gsm[];
     constexpr uint32_t nb1 = kMaxDense + 1, tot = kGWaves * nb1, spare = kMaxDense;
     constexpr uint32_t rmask = kMaxDense - 1;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    WaveBins all{gsm, gsm + tot, reinterpret_cast<double2*>(gsm + 2 * tot), reinterpret_cast<uint32_t*>(gsm + 4 * tot)};
-    for (uint32_t i = threadIdx.x; i < tot; i += kGThreads) {
-        all.s[i] = 0.0; all.c[i] = 0.0; all.cnt[i] = 0;
-        all.mm[i] = make_double2(__builtin_huge_val(), -__builtin_huge_val());
+    // the wave index through readfirstlane: step addresses become scalar, so
+    // the loads take a scalar base and a 32-bit lane offset
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double* s_all = gsm;
+    double* c_all = gsm + tot;
+    double2* mm = reinterpret_cast<double2*>(gsm + 2 * tot);
+    float2* gd = reinterpret_cast<float2*>(mm + nb1);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(gd + nb1);
+    for (uint32_t i = threadIdx.x; i < tot; i += kGThreads) { s_all[i] = 0.0; c_all[i] = 0.0; }
+    for (uint32_t i = threadIdx.x; i < nb1; i += kGThreads) {
+        mm[i] = make_double2(__builtin_huge_val(), -__builtin_huge_val());
+        gd[i] = make_float2(__builtin_huge_valf(), -__builtin_huge_valf());
+        cnt[i] = 0;
     }
     __syncthreads();
     const uint32_t off = (uint32_t)w * nb1;
-    WaveBins mine{all.s + off, all.c + off, all.mm + off, all.cnt + off};
+    ResidBins mine{s_all + off, c_all + off, mm, gd, cnt};
     K klo = std::numeric_limits<K>::max(), khi = std::numeric_limits<K>::lowest();
     const uint64_t waves = (uint64_t)gridDim.x * kGWaves;
     const uint64_t wid = (uint64_t)blockIdx.x * kGWaves + w;
@@ -429,8 +502,15 @@ __global__ __launch_bounds__(kGThreads) void k_gb_resid(const K* __restrict__ ke
     }
     GBin* out = parts + (size_t)blockIdx.x * kMaxDense;
     for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxDense; i += kGThreads) {
-        GBin acc = all.get(i);
-        for (int ww = 1; ww < kGWaves; ++ww) bin_merge(acc, all.get((uint32_t)ww * nb1 + i));
+        GBin acc;
+        acc.s = s_all[i]; acc.c = c_all[i];
+        for (int ww = 1; ww < kGWaves; ++ww) {
+            double cc = acc.c + c_all[(uint32_t)ww * nb1 + i];
+            two_sum<double>(acc.s, cc, s_all[(uint32_t)ww * nb1 + i]);
+            acc.c = cc;
+        }
+        const double2 p = mm[i];
+        acc.mn = p.x; acc.mx = p.y; acc.cnt = cnt[i];
         out[i] = acc;
     }
 }
@@ -806,19 +886,25 @@ int64_t groupby_dense_device(int key_dtype, const void* keys, const double* vals
     const void* fns[2][2] = {{(const void*)k_gb_resid<uint32_t, false, U>, (const void*)k_gb_resid<uint32_t, true, U>},
                              {(const void*)k_gb_resid<int32_t, false, U>, (const void*)k_gb_resid<int32_t, true, U>}};
     const void* fn = fns[i32][vec];
-    const size_t lds = (size_t)kGWaves * (kMaxDense + 1) * kGBinBytes;
+    const size_t lds = kResidLds;
     if (w.dev_occ == 0) {
         int occ = 0;
         if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kGThreads, lds), "occupancy")) return -1;
         w.dev_occ = occ < 1 ? 1 : occ;
     }
-    unsigned blocks = (unsigned)device_cu_count() * (unsigned)w.dev_occ;
+    // every resident workgroup for large inputs; one per CU (half the
+    // partials to merge) until each wave would see 16 steps
+    const uint64_t cus = (uint64_t)device_cu_count();
+    const uint64_t steps16 = (n + (uint64_t)kGChunk * kGWaves * 16 - 1) / ((uint64_t)kGChunk * kGWaves * 16);
+    uint64_t nb = steps16 > cus ? steps16 : cus;
+    if (nb > cus * (uint64_t)w.dev_occ) nb = cus * (uint64_t)w.dev_occ;
     const uint64_t per_block = (uint64_t)kGChunk * U * kGWaves;       // rows one workgroup takes per iteration
     const uint64_t need = (n + per_block - 1) / per_block;
-    if (need < blocks) blocks = (unsigned)(need ? need : 1);
-    // a wave's bin counts are u32: it sees at most ceil(steps / waves) steps
+    if (need < nb) nb = need ? need : 1;
+    const unsigned blocks = (unsigned)nb;
+    // a workgroup's bin counts are u32: it sees at most ceil(steps / waves) steps per wave
     const uint64_t steps = (n + kGChunk - 1) / kGChunk;
-    if ((steps + (uint64_t)blocks * kGWaves - 1) / ((uint64_t)blocks * kGWaves) * kGChunk >= (1ull << 32)) {
+    if ((steps + (uint64_t)blocks * kGWaves - 1) / ((uint64_t)blocks * kGWaves) * kGChunk * kGWaves >= (1ull << 32)) {
         set_error("group-by: %llu rows exceed the dense path's per-wave count", (unsigned long long)n);
         return -1;
     }
