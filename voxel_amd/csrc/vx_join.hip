@@ -23,11 +23,11 @@
 // partition) rebuilds the table as one table.
 //
 // Measured on MI355X (2^24 build keys, 2^28 Zipf(1.0) probes, 90 % hits):
-// one table 5.1 ms per probe batch; hash sub-tables 2.3 ms for the partition
-// pass + 4.0 ms for the probe (L2 hit rate 52 % against 44 % unpartitioned:
-// the probes in flight span several sub-tables, more than an XCD's 4 MB L2
-// holds).  Mode 0 therefore builds one table; the partitioned layouts are
-// kept for callers that partition anyway (the learned range partition gives
+// one table 4.2 ms per probe batch, 83 B of HBM traffic per probe (one
+// 128-byte line per L2 miss, 0.59 misses per probe); hash sub-tables 2.9 ms
+// for the probe partition pass + 3.1 ms for the probe in partition order.
+// Mode 0 therefore builds one table; the partitioned layouts are kept for
+// callers that partition anyway (the learned range partition gives
 // key-ordered sub-tables, e.g. as splitters for a multi-GPU join).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -151,21 +151,70 @@ __device__ __forceinline__ bool join_find(const JoinSlot* __restrict__ slots, co
     return false;
 }
 
+// One 16-byte load of a slot (key and payload from one request).
+__device__ __forceinline__ JoinSlot load_slot(const JoinSlot* __restrict__ slots, uint64_t at) {
+    const uint4 q = *reinterpret_cast<const uint4*>(slots + at);
+    JoinSlot r;
+    r.key = (int64_t)(((uint64_t)q.y << 32) | q.x);
+    r.payload = (int64_t)(((uint64_t)q.w << 32) | q.z);
+    return r;
+}
+
+// Probes in input order, P per lane per iteration: the keys are loaded
+// non-temporally (the probe stream is read once and should not evict table
+// lines), then the first slot of each probe, before any is resolved; the
+// rare continuation of a probe sequence is walked one slot at a time.  A
+// lane visits at most a sub-table's worth of slots per probe.  P = 1: on
+// the Zipf(1.0) benchmark probes, two or four probes in flight per lane ran
+// 6-9 % slower (more table lines in flight evict the hot ones from L2).
 __global__ __launch_bounds__(256) void k_join_probe_sum(const int64_t* __restrict__ probe, uint64_t np,
                                                         const JoinSlot* __restrict__ slots, JoinLayout L,
                                                         const unsigned long long* __restrict__ side,
                                                         unsigned long long* __restrict__ out) {
+    constexpr int P = 1;
     uint64_t cnt = 0, sum = 0;
     const unsigned long long side_row = side[0], side_pay = side[1];
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += stride) {
-        int64_t k = probe[i];
-        if (k == kEmpty) {
-            if (side_row) { ++cnt; sum += side_pay; }
-            continue;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, smask = L.smask();
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < np; i0 += P * stride) {
+        int64_t k[P];
+        bool v[P];
+        uint64_t base[P], j[P];
+        JoinSlot x[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const uint64_t i = i0 + (uint64_t)p * stride;
+            v[p] = i < np;
+            k[p] = v[p] ? __builtin_nontemporal_load(probe + i) : kEmpty;
         }
-        uint64_t at;
-        if (join_find(slots, L, k, at)) { ++cnt; sum += (uint64_t)slots[at].payload; }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const uint64_t h = join_hash(k[p]);
+            base[p] = L.base(k[p], h);
+            j[p] = h & smask;
+            x[p] = JoinSlot{kEmpty, 0};
+            if (v[p] && k[p] != kEmpty) x[p] = load_slot(slots, base[p] | j[p]);
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            if (!v[p]) continue;
+            if (k[p] == kEmpty) {
+                if (side_row) { ++cnt; sum += side_pay; }
+                continue;
+            }
+            // the first slot's payload is used whether or not it matches, so the
+            // compiler keeps one 16-byte load instead of a key load and a reload
+            const bool m = x[p].key == k[p];
+            cnt += m ? 1 : 0;
+            sum += m ? (uint64_t)x[p].payload : 0;
+            if (m || x[p].key == kEmpty) continue;
+            uint64_t jj = j[p];
+            for (uint64_t step = 1; step <= smask; ++step) {
+                jj = (jj + 1) & smask;
+                const JoinSlot sl = load_slot(slots, base[p] | jj);
+                if (sl.key == k[p]) { ++cnt; sum += (uint64_t)sl.payload; break; }
+                if (sl.key == kEmpty) break;
+            }
+        }
     }
     cnt = block_sum_256(cnt);
     __syncthreads();                                // the partial slots are reused by the second sum
@@ -406,9 +455,9 @@ int voxel_gpu_join_probe_sum_launch(void* j, const int64_t* probe_dev, size_t np
             return -1;
         keys = t->pk;
     }
-    unsigned blocks = (unsigned)device_cu_count() * 16;
+    unsigned blocks = (unsigned)device_cu_count() * 8;
     uint64_t need = (np + 255) / 256;
-    if (need < blocks) blocks = (unsigned)need;
+    if (need < blocks) blocks = (unsigned)(need ? need : 1);
     hipLaunchKernelGGL(k_join_probe_sum, dim3(blocks), dim3(256), 0, st, keys, (uint64_t)np, t->slots, t->L, t->side,
                        t->side + 2);
     return hip_ok(hipGetLastError(), "join probe launch") ? 0 : -1;
