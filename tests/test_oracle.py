@@ -365,6 +365,35 @@ def test_run_parallel_restatement(golden, x42):
         assert o.f64_bits(r) == B(g["run_parallel_%d" % t]["R0"])
 
 
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_run_parallel_tier0_restatement_against_reference():
+    """The per-chunk restatement above equals the reference engine's own
+    RunParallel (oracle/_ref, threads included) bit for bit on generated
+    Tier-0 loops: sums, counts, minima and maxima, 1 to 9 workers."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(303)
+    checked = 0
+    for trial in range(40):
+        n = int(rng.integers(1, 600)) * 4
+        x = rng.standard_normal(n) * 100
+        x[rng.integers(0, n, 5)] = 0.0
+        thr = [float(rng.standard_normal() * 60) for _ in range(2)]
+        code, spec = tg.make_program(rng, True)
+        if not tg.mergeable(spec):
+            continue
+        threads = int(rng.integers(1, 10))
+        b = int(rng.integers(0, len(spec["red"])))
+        use_acc = bool(rng.integers(0, 2))
+        rr = tg.ACCS[b] if use_acc else tg.PARTS[b]
+        regs = np.zeros(64, np.uint64)
+        regs[2], regs[3], regs[6] = n, o.f64_bits(thr[0]), o.f64_bits(thr[1])
+        c = np.array(code, np.uint32)
+        o.R.vr_engine_run_parallel_f64(o.ptr(x), n, o.ptr(c), c.size, o.ptr(regs), threads, rr)
+        assert int(regs[rr]) == tg.run_parallel_expected(o, x, n, spec, thr, threads, b, use_acc), (trial, spec)
+        checked += 1
+    assert checked >= 20
+
+
 def test_bitmap_and_selection(golden):
     g = golden["bitmap_gt4"]
     x = np.array(g["data"], np.float64)

@@ -753,6 +753,113 @@ def gen(dt, n, rng):
     return rng.integers(0, 2**31, n).astype(dt)
 
 
+def test_run_parallel_tier0_programs(gpu):
+    """RunParallel over generated Tier-0 loops (engine.hpp:143-175; the
+    reference exposes it on Engine<f64> only) through the engine ABI, against
+    the per-chunk restatement tier0_gen.run_parallel_expected (pinned bit for
+    bit against the reference's own RunParallel in test_oracle): counts,
+    minima and maxima bit-exact; sums within 1e-12 of the sum of |x| (the
+    reference adds lanes in row order, the GPU a compensated sum).  1 to 16
+    workers, chunks of ragged length."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    is_f64 = True
+    rng = np.random.default_rng(91)
+    checked = 0
+    for trial in range(24):
+        n = int(rng.integers(1, 1500)) * 4
+        if is_f64:
+            x = rng.standard_normal(n) * 100
+            x[rng.integers(0, n, 5)] = 0.0
+            thr = [float(rng.standard_normal() * 60) for _ in range(2)]
+            tb = [o.f64_bits(t) for t in thr]
+            e = vx.EngineF64()
+        else:
+            x = rng.integers(-2**40, 2**40, n).astype(np.int64)
+            thr = [int(rng.integers(-2**39, 2**39)) for _ in range(2)]
+            tb = [t & (2**64 - 1) for t in thr]
+            e = vx.EngineI64()
+        code, spec = tg.make_program(rng, is_f64)
+        if not tg.mergeable(spec):
+            continue
+        threads = int(rng.choice([1, 2, 3, 7, 8, 16]))
+        b = int(rng.integers(0, len(spec["red"])))
+        use_acc = bool(rng.integers(0, 2))
+        rr = tg.ACCS[b] if use_acc else tg.PARTS[b]
+        e.add_segment(x)
+        for r, v in ((2, n), (3, tb[0]), (6, tb[1])):
+            e.set_scalar(r, v)
+        e.load_program([int(c) for c in code])
+        e.run_parallel(0, threads, rr)
+        want = tg.run_parallel_expected(o, x, n, spec, thr, threads, b, use_acc, is_f64)
+        got = e.get_scalar(rr)
+        if is_f64 and use_acc and spec["red"][b] == 0:
+            assert abs(o.bits_f64(got) - o.bits_f64(want)) <= 1e-12 * float(np.abs(x).sum()), (trial, spec)
+        else:
+            assert got == want, (trial, threads, spec)
+        assert e.get_scalar(2) == n                       # the caller's registers other than R[result] stay
+        checked += 1
+    assert checked >= 12
+
+
+def test_run_parallel_straight_programs(gpu):
+    """RunParallel of straight-line AGG_* programs: each worker is a fresh
+    engine (fresh aggregate slots) over its chunk; R[result] = the sum in f64
+    of the workers' R[result] (register bits read as f64, as ScalarAs<T>
+    does), against vo_straight_run per chunk (pinned against the reference
+    engine), on registers whose last writer is exact (not SUM/AVG/STDDEV/
+    VARIANCE).  A sort word sorts each chunk of the caller's rows in place,
+    as the reference's workers do."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(93)
+    checked = 0
+    for trial in range(14):
+        n = int(rng.integers(1, 5000))
+        x = np.round(rng.standard_normal(n) * 1e3, 3)
+        code = tg.straight_program(rng, n, ops=tg.AGG_OPS, scalar_reads=(1, 2, 3, 13, 14, 15))
+        writers = tg.last_agg_writers(code)
+        exact = [r for r, op in writers.items() if op not in (0xE1, 0xE2, 0xE7, 0xE8)]
+        if not exact:
+            continue
+        rr = int(rng.choice(exact))
+        threads = int(rng.choice([1, 2, 5, 8]))
+        e = vx.EngineF64()
+        e.add_segment(x)
+        e.load_program([int(c) for c in code])
+        e.run_parallel(0, threads, rr)
+        chunk = -(-n // threads)
+        want = 0.0
+        for t in range(threads):
+            s0 = t * chunk
+            if s0 >= n:
+                continue
+            xc = np.ascontiguousarray(x[s0:s0 + min(chunk, n - s0)])
+            st = o.O.vo_agg_create(1)
+            try:
+                regs = np.zeros(64, np.uint64)
+                assert o.O.vo_straight_run(st, o.ptr(xc.view(np.uint64)), xc.size, o.ptr(code), code.size, o.ptr(regs)) >= 0
+            finally:
+                o.O.vo_agg_destroy(st)
+            want = want + o.bits_f64(int(regs[rr]))
+        assert e.get_scalar(rr) == o.f64_bits(want), (trial, threads, rr, hex(writers[rr]))
+        checked += 1
+    assert checked >= 8
+    # SORT_ASC over each worker's chunk: the caller's rows come back sorted chunk by chunk
+    x = np.round(rng.standard_normal(103) * 1e3, 3)
+    keep = x.copy()
+    e = vx.EngineF64()
+    e.add_segment(x)
+    e.load_program([tg.w(0xF2, rd=4, ra=0), 1])
+    e.run_parallel(0, 4, 4)
+    chunk = -(-x.size // 4)
+    want = np.concatenate([np.sort(keep[i:i + chunk]) for i in range(0, x.size, chunk)])
+    assert np.array_equal(x, want)
+    # each worker's R4 = its chunk's row count (integer bits), summed as f64
+    sizes = [min(chunk, x.size - i) for i in range(0, x.size, chunk)]
+    assert e.get_scalar(4) == o.f64_bits(sum((o.bits_f64(c) for c in sizes), 0.0))
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32, np.int64, np.int32])
 @pytest.mark.parametrize("n", [0, 1, 5, 1023, 65_537, 2_000_003])
 def test_relational_compound_scan(gpu, dt, n):

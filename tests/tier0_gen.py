@@ -101,6 +101,29 @@ def oracle_run(o, x, total, spec, thr, is_f64=True, acc0=None):
     return [int(a) for a in acc], [int(p) for p in part]
 
 
+def run_parallel_expected(o, x, total, spec, thr, threads, rr_block, use_acc, is_f64=True):
+    """RunParallel of a Tier-0 loop restated (engine.hpp:143-175): worker t
+    runs the loop on chunk t of ceil(N / threads) rows as its only segment
+    (R[total] copied, rows past the chunk read as zeros), and R[result] is
+    the sum in T of the workers' R[result] in worker order from zero; the
+    per-chunk registers come from vo_tier0_multi_* (oracle_run)."""
+    n = x.size
+    chunk = -(-n // threads)
+    fs, isum = 0.0, 0
+    for t in range(threads):
+        s0 = t * chunk
+        if s0 >= n:
+            continue
+        xc = np.ascontiguousarray(x[s0:s0 + min(chunk, n - s0)])
+        acc, part = oracle_run(o, xc, total, spec, thr, is_f64)
+        v = acc[rr_block] if use_acc else part[rr_block]
+        if is_f64:
+            fs = fs + o.bits_f64(v)
+        else:
+            isum = (isum + v) % 2**64
+    return o.f64_bits(fs) if is_f64 else isum
+
+
 def mergeable(spec):
     """True when every block's filter chain is one range (no != in a chain of two)."""
     for b, src in enumerate(spec["bsrc"]):

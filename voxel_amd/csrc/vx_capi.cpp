@@ -826,10 +826,63 @@ size_t voxel_engine_add_segment(void* h, const double* data, size_t count) { ret
 void voxel_engine_load_program(void* h, const uint32_t* code, size_t len) { engine_load(h, code, len); }
 void voxel_engine_run(void* h) { engine_run(h); }
 
+}  // extern "C"
+
+namespace vx {
+namespace {
+// RunParallel of a Tier-0 or straight-line program (engine.hpp:143-175):
+// worker t is a fresh engine whose only segment (id 0) is chunk t of
+// segment seg_id (rows [t * ceil(N / threads), ...), in HBM), holding the
+// program and a copy of the scalar registers; the workers run one after
+// another on the GPU, and R[result] = the sum in T of their R[result] in
+// worker order, starting from zero.  Workers whose chunk starts at or past N
+// are skipped (the reference's worker reads out of bounds there).
+void run_parallel_program(Engine& e, size_t seg_id, uint32_t threads, uint8_t rr, StreamCtx* c) {
+    Seg& s = *e.segs[seg_id];
+    const uint64_t n = s.count, es = dtype_size(e.dtype);
+    const uint64_t chunk = (n + threads - 1) / threads;
+    const char* d = nullptr;
+    if (n) {
+        d = static_cast<const char*>(seg_rows(s, e.dtype, 0, n, c));
+        if (!d) return;
+    }
+    double fsum = 0.0;
+    uint64_t isum = 0;
+    for (uint32_t t = 0; t < threads; ++t) {
+        const uint64_t start = (uint64_t)t * chunk;
+        if (start >= n) continue;
+        const uint64_t cnt = start + chunk > n ? n - start : chunk;
+        Engine w;
+        w.dtype = e.dtype;
+        auto sg = std::make_unique<Seg>();
+        sg->devext = d + start * es;
+        sg->count = cnt;
+        w.segs.push_back(std::move(sg));
+        w.code = e.code;
+        w.plan = e.plan;
+        std::memcpy(w.regs, e.regs, sizeof(w.regs));
+        const bool ok = e.plan.kind == Plan::TIER0 ? run_tier0(w, c) : run_straight(w, c);
+        if (!ok) return;
+        if (e.dtype == DT_F64) fsum = fsum + bits_f64(w.regs[rr]);
+        else isum += w.regs[rr];
+    }
+    // sort words sort each worker's chunk in place: the reference's workers
+    // hold the caller's rows, so the sorted chunks go back to them
+    bool sorts = false;
+    for (uint32_t wd : e.code) sorts |= op_of(wd) >= 0xF2 && op_of(wd) <= 0xF5;
+    if (sorts && n && !s.devext && s.host && !sync_read(c, const_cast<void*>(s.host), d, n * es)) return;
+    e.regs[rr] = e.dtype == DT_F64 ? f64_bits(fsum) : isum;
+}
+}  // namespace
+}  // namespace vx
+
+extern "C" {
+
 // Engine<f64>::RunParallel (engine.hpp:143-175): row chunks of ceil(N/threads),
 // each runs the fast path with copied registers (dropping its own tail), and
 // R[result] = sum over workers of their R[result] in T.  Workers whose chunk
 // starts at or past N are skipped (the reference reads out of bounds there).
+// Tier-0 and straight-line programs go through run_parallel_program.
 void voxel_engine_run_parallel(void* h, size_t seg_id, uint32_t threads, uint8_t result_reg) {
     set_error("%s", "");
     if (!h) { set_error("null engine"); return; }
@@ -837,8 +890,14 @@ void voxel_engine_run_parallel(void* h, size_t seg_id, uint32_t threads, uint8_t
     if (e.segs.empty() || seg_id >= e.segs.size()) return;          // engine.hpp:146
     if (threads == 0) threads = std::thread::hardware_concurrency();
     if (threads < 1) threads = 1;
-    if (e.plan.kind != Plan::FAST) { set_error("run_parallel is lowered for fast-path programs only: %s",
-                                               e.plan.kind == Plan::NONE ? e.plan.why.c_str() : "Tier-0 loop"); return; }
+    if (e.plan.kind == Plan::TIER0 || e.plan.kind == Plan::STRAIGHT) {
+        std::lock_guard<std::mutex> lk(api_mutex());
+        StreamCtx* c = ctx_for(nullptr);
+        if (!c) return;
+        run_parallel_program(e, seg_id, threads, result_reg & 0x3F, c);
+        return;
+    }
+    if (e.plan.kind != Plan::FAST) { set_error("program not lowered to the GPU: %s", e.plan.why.c_str()); return; }
     if (e.plan.seg != 0) { set_error("run_parallel workers hold one segment; the program reads segment %u", e.plan.seg); return; }
     std::lock_guard<std::mutex> lk(api_mutex());
     StreamCtx* c = ctx_for(nullptr);
