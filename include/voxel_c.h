@@ -41,11 +41,16 @@ extern "C" {
  * add_segment borrows the caller's pointer exactly as the reference does
  * (voxel_c.cpp:18-20); the rows a run needs are copied to HBM at each run
  * (the caller may have changed them).  voxel_gpu_engine_set_resident() opts
- * out of the re-copy.  run: fast-path programs (engine.hpp:84-141) and the
- * Tier-0 loop programs of voxel_query / the MOV-prefixed tests run on the
- * GPU with the reference's register effects; any other program, and loops
- * that would not terminate in the reference, set voxel_last_error() and leave
- * the registers untouched. */
+ * out of the re-copy.  run: fast-path programs (engine.hpp:84-141), Tier-0
+ * loops (filters, reductions, vector arithmetic, window words, scalar words
+ * around the loop) and straight-line programs of AGG_*, HASH_*, SORT_*,
+ * TOPK / BOTTOMK, JOIN_* and scalar words run on the GPU with the
+ * reference's register effects (sort words sort the caller's rows in place,
+ * as the reference does); any other program, and loops that would not
+ * terminate in the reference, set voxel_last_error() and leave the
+ * registers untouched.  run_parallel (engine.hpp:143-175): fast-path
+ * programs as chunked scans, other lowered programs as one fresh engine per
+ * chunk with copied registers, R[result] summed in T in worker order. */
 void*     voxel_engine_create_f64(void);
 void      voxel_engine_destroy(void* engine);
 size_t    voxel_engine_add_segment(void* engine, const double* data, size_t count);
