@@ -37,7 +37,7 @@ def test_radix_golden(gpu, golden):
     assert k.tolist() == g["keys"] and i.tolist() == g["idx"]
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 63, 4095, 4096, 4097, 100_003, 1_000_000])
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 4095, 4096, 4097, 8191, 8192, 8193, 65537, 100_003, 1_000_000])
 def test_radix_random_vs_oracle(gpu, n):
     rng = np.random.default_rng(n)
     keys = rng.integers(0, 2**64, n, dtype=np.uint64)

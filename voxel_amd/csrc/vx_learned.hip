@@ -47,6 +47,7 @@
 namespace vx {
 
 int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n, StreamCtx* c);
+int sort_pairs_u64_rowidx(const uint64_t* kin, uint64_t* kout, uint32_t* vout, uint64_t n, StreamCtx* c);
 
 constexpr int kH = 32;                       // hidden width
 constexpr int kParams = 32 + 32 + 32 * 32 + 32 + 32 + 1;   // 1153
@@ -672,7 +673,7 @@ static bool ensure_model(LearnedBufs& b) {
               hip_ok(hipHostMalloc((void**)&b.phost, kParams * 4, hipHostMallocDefault), "hipHostMalloc") &&
               galloc((void**)&b.qdev, 1025 * 8) && galloc((void**)&b.biglist, kBigCap * 4) &&
               hip_ok(hipHostMalloc((void**)&b.biglist_host, kBigCap * 4, hipHostMallocDefault), "hipHostMalloc") &&
-              galloc((void**)&b.w.hist, 8 * 256 * 8) && galloc((void**)&b.w.offsets, 8 * 256 * 8) &&
+              galloc((void**)&b.w.hist, 8 * 256 * 8) &&
               galloc((void**)&b.w.skip, 64);
     for (int i = 0; ok && i < 8; ++i) ok = hip_ok(hipEventCreate(&b.ev[i]), "hipEventCreate");
     return ok;
@@ -1041,16 +1042,15 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
             return -1;
         b.cap = n;
     }
-    // payload: the caller's, or the row index (kept in v2 until the partition overwrites it)
-    auto payload = [&]() -> const uint32_t* {
-        if (vin_user) return vin_user;
-        return hip_ok(launch_iota(b.v2, n, st), "iota") ? b.v2 : nullptr;
+    // the plain radix sort of the input, with the caller's payload or the row index
+    auto plain_sort = [&]() -> bool {
+        if (vin_user) return sort_pairs_u64(kin, vin_user, kout, vout, n, c) >= 0;
+        return sort_pairs_u64_rowidx(kin, kout, vout, n, c) >= 0;
     };
     if (n < (1u << 16)) {
         // small inputs: the plain radix sort (sampling would dominate)
         S.fell_back = 1;
-        const uint32_t* vin = n ? payload() : vin_user;
-        if (n && (!vin || sort_pairs_u64(kin, vin, kout, vout, n, c) < 0)) return -1;
+        if (n && !plain_sort()) return -1;
         if (!hip_ok(hipStreamSynchronize(st), "sync")) return -1;
         if (stats) *stats = S;
         return 0;
@@ -1091,25 +1091,24 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     if (plain) {
         // few distinct key bytes: the plain radix sort's skipped passes win
         S.fell_back = 2;
-        const uint32_t* vin = payload();
-        if (!vin || sort_pairs_u64(kin, vin, kout, vout, n, c) < 0) return -1;
+        if (!plain_sort()) return -1;
         if (!hip_ok(hipStreamSynchronize(st), "sync")) return -1;
         if (stats) *stats = S;
         return 0;
     }
     // 5. stable partition by bucket id (LSD over its bytes); pass 0 computes
-    //    the id once and every pass carries it next to the rows
-    const uint32_t* vin = payload();
-    if (!vin) return -1;
+    //    the id once and every pass carries it next to the rows; without a
+    //    payload, pass 0 writes the row index
     const uint64_t* ks = kin;
-    const uint32_t* vs = vin;
+    const uint32_t* vs = vin_user;
     const uint32_t* fs = nullptr;
     int executed = 0;
     for (int p = 0; p < parts; ++p) {
         uint64_t* kd = (ks == b.k1) ? b.k2 : b.k1;
-        uint32_t* vd = (vs == b.v1) ? b.v2 : b.v1;          // v2 may hold the row index: read once, then free
+        uint32_t* vd = (vs == b.v1) ? b.v2 : b.v1;
         uint32_t* fd = (fs == b.f1) ? b.f2 : b.f1;
-        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, p, b.w, st, fs, fd), "model pass")) return -1;
+        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, p, b.w, st, fs, fd, p == 0 && !vin_user), "model pass"))
+            return -1;
         ks = kd;
         vs = vd;
         fs = fd;
@@ -1207,8 +1206,7 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
     if (overflow || unsorted_rows > n / 2) {
         // the model did not split the keys: plain radix sort of the input
         S.fell_back = 1;
-        const uint32_t* v0 = payload();
-        if (!v0 || sort_pairs_u64(kin, v0, kout, vout, n, c) < 0) return -1;
+        if (!plain_sort()) return -1;
     } else if (!rng.empty()) {
         // moves (source row, destination row, rows) in kChunk pieces:
         // single-key buckets straight to kout, the others packed into the free

@@ -13,28 +13,6 @@ namespace vx {
 
 constexpr int kThreads = 256;           // = number of digit values
 constexpr int kWaves = kThreads / 64;
-constexpr int kRounds = 16;             // keys per lane per tile
-constexpr int kTile = kThreads * kRounds;   // 4096 keys per tile
-
-template <class Digit>
-__global__ __launch_bounds__(kThreads) void k_hist(const uint64_t* __restrict__ keys, uint64_t n, Digit dig,
-                                                   int passes, unsigned long long* __restrict__ ghist) {
-    __shared__ uint32_t h[kWaves][8][256];
-    const int w = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < kWaves * 8 * 256; i += kThreads) (&h[0][0][0])[i] = 0;
-    __syncthreads();
-    uint64_t stride = (uint64_t)gridDim.x * kThreads;
-    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-        const uint64_t b = dig.bits(keys[i]);
-        for (int p = 0; p < passes; ++p) atomicAdd(&h[w][p][dig.digit_of_bits(b, p)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < passes * 256; i += kThreads) {
-        uint32_t s = 0;
-        for (int ww = 0; ww < kWaves; ++ww) s += (&h[ww][0][0])[i];
-        if (s) atomicAdd(&ghist[i], (unsigned long long)s);
-    }
-}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = threadIdx.x & 63;
@@ -52,24 +30,34 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
     return valid ? peers : 0ull;
 }
 
-// Rows of the key array are cut into ranges of whole 4096-key tiles, at
-// most kMaxRanges of them.  Per digit pass: k_upsweep counts each range's
-// digits, k_range_scan turns the counts into each range's first output
-// position per digit, and k_downsweep (one workgroup per range, tiles in
-// order) scatters stably.  No workgroup waits on another.
-constexpr uint64_t kMaxRanges = 2048;     // and words of RadixWork::status: 2 x ranges x 256 + 256
+// Tiles of kTile rows are the unit of every digit pass.  Per pass:
+// k_tile_counts writes each tile's 256 digit counts (tile-major, 1 KB per
+// tile); k_chunk_sums, k_chunk_scan and k_tile_offsets turn them in place
+// into each tile's first output position per digit (digits in order, then
+// tiles in order inside a digit: the stable order); k_scatter_tile (one
+// workgroup per tile) ranks the tile's rows by digit and writes them through
+// LDS so that each digit run is written by consecutive lanes.  No workgroup
+// waits on another and no tile depends on another tile's progress, so tiles
+// are handed to the eight XCDs in contiguous blocks (workgroups are
+// dispatched to the XCDs round-robin): the runs that neighbouring tiles
+// write next to each other are then merged in one XCD's L2 before they
+// reach HBM.
+constexpr int kDsThreads = 512;                 // scatter workgroup
+constexpr int kDsWaves = kDsThreads / 64;
+constexpr int kRounds = 16;                     // rows per lane per tile
+constexpr int kTile = kDsThreads * kRounds;     // 8192 rows per tile
+constexpr uint64_t kChunkTiles = 256;           // tiles per chunk of the offset scan
+constexpr uint64_t kXcds = 8;
 
-__host__ __device__ inline uint64_t radix_tiles_per_range(uint64_t n) {
-    uint64_t tiles = (n + kTile - 1) / kTile;
-    uint64_t r = (tiles + kMaxRanges - 1) / kMaxRanges;
-    return r ? r : 1;
-}
+__host__ __device__ inline uint64_t radix_tiles(uint64_t n) { return (n + kTile - 1) / kTile; }
+__host__ __device__ inline uint64_t radix_chunks(uint64_t n) { return (radix_tiles(n) + kChunkTiles - 1) / kChunkTiles; }
 
-__host__ __device__ inline uint64_t radix_ranges(uint64_t n) {
-    uint64_t tiles = (n + kTile - 1) / kTile;
-    uint64_t per = radix_tiles_per_range(n);
-    uint64_t r = (tiles + per - 1) / per;
-    return r ? r : 1;
+// Tile of workgroup b: XCD x = b mod 8 takes tiles [x * per, (x + 1) * per)
+// in order; the last ntiles mod 8 tiles map to themselves.  A bijection of
+// [0, ntiles).
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t ntiles) {
+    const uint64_t per = ntiles / kXcds;
+    return b < per * kXcds ? (b % kXcds) * per + b / kXcds : b;
 }
 
 // Digit source of a pass: the key through the digit function, or (CIN) a
@@ -80,18 +68,18 @@ __device__ __forceinline__ uint32_t pass_digit(const Digit& dig, uint64_t key, u
     else return dig.digit(key, pass);
 }
 
-// cnt[d * nr + range] = keys of the range with digit d
+// cnt[t * 256 + d] = rows of tile t with digit d
 template <class Digit, bool CIN>
-__global__ __launch_bounds__(kThreads) void k_upsweep(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ fin,
-                                                      uint64_t n, Digit dig, int pass, uint64_t range_len, uint32_t nr,
-                                                      uint32_t* __restrict__ cnt) {
+__global__ __launch_bounds__(kThreads) void k_tile_counts(const uint64_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ fin, uint64_t n, Digit dig,
+                                                          int pass, uint32_t* __restrict__ cnt) {
     __shared__ uint32_t h[kWaves][256];
     const int w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&h[0][0])[i] = 0;
     __syncthreads();
-    const uint64_t s = (uint64_t)blockIdx.x * range_len;
-    const uint64_t e = s + range_len < n ? s + range_len : n;
-    constexpr int U = 8;                                    // keys per lane in flight
+    const uint64_t s = (uint64_t)blockIdx.x * kTile;
+    const uint64_t e = s + kTile < n ? s + kTile : n;
+    constexpr int U = 8;                                    // rows per lane in flight
     for (uint64_t i0 = s; i0 < e; i0 += kThreads * U) {      // block-uniform trip count
         uint64_t kk[U];
         uint32_t dd[U];
@@ -112,237 +100,354 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint64_t* __restrict
     }
     __syncthreads();
     const int d = threadIdx.x;
-    cnt[(uint64_t)d * nr + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+    cnt[(uint64_t)blockIdx.x * 256 + d] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
-// One workgroup per digit: off[d * nr + r] = keys with digit d in ranges
-// before r; tot[d] = keys with digit d.  (Internal linkage: every file that
-// instantiates the passes gets its own copy.)
-static __global__ __launch_bounds__(kThreads) void k_range_scan(const uint32_t* __restrict__ cnt, uint32_t nr,
-                                                                uint32_t* __restrict__ off, uint32_t* __restrict__ tot) {
-    __shared__ uint32_t sc[kThreads];
-    const uint32_t d = blockIdx.x, t = threadIdx.x;
-    const uint32_t per = (nr + kThreads - 1) / kThreads;
-    const uint32_t lo = t * per < nr ? t * per : nr, hi = lo + per < nr ? lo + per : nr;
-    const uint32_t* c = cnt + (uint64_t)d * nr;
-    uint32_t sum = 0;
-    for (uint32_t r = lo; r < hi; ++r) sum += c[r];
-    sc[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < kThreads; o <<= 1) {
-        uint32_t x = t >= (uint32_t)o ? sc[t - o] : 0;
-        __syncthreads();
-        sc[t] += x;
-        __syncthreads();
+// csum[j * 256 + d] = rows with digit d in the tiles of chunk j.  (Internal
+// linkage for the scan kernels: every file that instantiates the passes gets
+// its own copy.)
+static __global__ __launch_bounds__(256) void k_chunk_sums(const uint32_t* __restrict__ cnt, uint64_t ntiles,
+                                                           uint32_t* __restrict__ csum) {
+    const uint32_t d = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kChunkTiles;
+    const uint64_t t1 = t0 + kChunkTiles < ntiles ? t0 + kChunkTiles : ntiles;
+    uint32_t s = 0;
+    uint64_t t = t0;
+    for (; t + 8 <= t1; t += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = cnt[(t + u) * 256 + d];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += c[u];
     }
-    uint32_t run = sc[t] - sum;
-    uint32_t* o = off + (uint64_t)d * nr;
-    for (uint32_t r = lo; r < hi; ++r) {
-        o[r] = run;
-        run += c[r];
-    }
-    if (t == kThreads - 1) tot[d] = sc[t];
+    for (; t < t1; ++t) s += cnt[t * 256 + d];
+    csum[(uint64_t)blockIdx.x * 256 + d] = s;
 }
 
-// Stable scatter of one range's tiles.  Keys, payloads (and with CIN the
-// carried digit words) of a tile are loaded together, ranked by digit, placed
-// in LDS in digit order and written out so that each digit run is written
-// by consecutive lanes.  COUT writes the digit word of every row (carried
-// from fin, or dig.bits(key)) next to it.
-template <class Digit, bool CIN, bool COUT, bool PAY = true>
-__global__ __launch_bounds__(kThreads) void k_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                        const uint32_t* __restrict__ fin, uint64_t* __restrict__ kout,
-                                                        uint32_t* __restrict__ vout, uint32_t* __restrict__ fout,
-                                                        uint64_t n, Digit dig, int pass, uint64_t tiles_per_range,
-                                                        uint32_t nr, const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ tot) {
-    __shared__ uint64_t kb[kTile];
-    __shared__ uint32_t vb[PAY ? kTile : 1];
-    __shared__ uint32_t fb[(CIN || COUT) ? kTile : 1];   // digit words of the LDS slots
-    __shared__ uint32_t wcnt[kWaves][256];          // per-wave digit counters -> wave prefixes
-    __shared__ uint32_t lstart[256];                // tile-local start of each digit
-    __shared__ uint32_t run[256];                   // output position of the digit's next key
-    __shared__ uint32_t scan_s[256];
-
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, d = threadIdx.x;
-    {   // digit base: exclusive scan of the digit totals
-        const uint32_t t0 = tot[d];
-        scan_s[d] = t0;
-        __syncthreads();
-        for (int o = 1; o < 256; o <<= 1) {
-            uint32_t x = d >= o ? scan_s[d - o] : 0;
-            __syncthreads();
-            scan_s[d] += x;
-            __syncthreads();
+// One workgroup: csum[j * 256 + d] becomes the rows with digit d in chunks
+// before j; dbase[d] = rows with a digit below d.
+static __global__ __launch_bounds__(256) void k_chunk_scan(uint32_t* __restrict__ csum, uint64_t nch,
+                                                           uint32_t* __restrict__ dbase) {
+    __shared__ uint32_t ws[4];
+    const uint32_t d = threadIdx.x, lane = d & 63, w = d >> 6;
+    uint32_t run = 0;
+    uint64_t j = 0;
+    for (; j + 8 <= nch; j += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = csum[(j + u) * 256 + d];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            csum[(j + u) * 256 + d] = run;
+            run += c[u];
         }
-        run[d] = scan_s[d] - t0 + off[(uint64_t)d * nr + blockIdx.x];
     }
-    // rows of a tile: k/v/f registers; the next tile's rows are loaded into
-    // them as soon as the current tile sits in LDS, so the loads overlap the
-    // scatter of the current tile
+    for (; j < nch; ++j) {
+        const uint32_t c = csum[j * 256 + d];
+        csum[j * 256 + d] = run;
+        run += c;
+    }
+    uint32_t x = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (uint32_t ww = 0; ww < 4; ++ww) add += ww < w ? ws[ww] : 0u;
+    dbase[d] = x - run + add;
+}
+
+// cnt[t * 256 + d] becomes tile t's first output position for digit d.
+static __global__ __launch_bounds__(256) void k_tile_offsets(uint32_t* __restrict__ cnt, uint64_t ntiles,
+                                                             const uint32_t* __restrict__ csum,
+                                                             const uint32_t* __restrict__ dbase) {
+    const uint32_t d = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kChunkTiles;
+    const uint64_t t1 = t0 + kChunkTiles < ntiles ? t0 + kChunkTiles : ntiles;
+    uint32_t run = csum[(uint64_t)blockIdx.x * 256 + d] + dbase[d];
+    uint64_t t = t0;
+    for (; t + 8 <= t1; t += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = cnt[(t + u) * 256 + d];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            cnt[(t + u) * 256 + d] = run;
+            run += c[u];
+        }
+    }
+    for (; t < t1; ++t) {
+        const uint32_t c = cnt[t * 256 + d];
+        cnt[t * 256 + d] = run;
+        run += c;
+    }
+}
+
+// LDS of one scatter workgroup (75.8 KB: two workgroups per CU).
+struct ScatterLds {
+    uint64_t kb[kTile];                 // keys (load order, then digit order); u32 view for payloads / digit words
+    union {
+        uint32_t wcnt[kDsWaves][256];   // per-wave digit counters -> wave prefixes
+        uint8_t sdig[kTile];            // then: digit of each LDS slot
+    };
+    uint32_t lstart[256];               // tile-local start of each digit
+    uint32_t base[256];                 // output row of the tile's first row of each digit, minus its slot
+    uint32_t scan_s[4];
+};
+
+// Stable scatter of one tile (FULL: all kTile rows present, no per-row
+// checks).  Rows are loaded and their digits taken (from the key, or the
+// carried digit word); the keys wait in LDS in load order while the rows
+// are ranked by digit (wave ballots of the digit bits, per-wave LDS
+// counters), which gives every row its LDS slot in digit order and every
+// slot its digit and output row.  Then each array in turn (digit words,
+// keys, payloads) is placed in LDS in digit order and written out from
+// there, so that consecutive lanes write consecutive rows of a digit run.
+// One 64 KB LDS buffer serves the three arrays.  COUT writes the digit word
+// of every row (carried from fin, or dig.bits(key)) next to it.  Output rows
+// are clamped to n - 1, so that inconsistent counts could only misplace
+// rows, never write outside the arrays.
+template <class Digit, bool CIN, bool COUT, bool PAY, bool IDX, bool FULL>
+__device__ __forceinline__ void scatter_tile(ScatterLds& L, const uint64_t* __restrict__ kin,
+                                             const uint32_t* __restrict__ vin, const uint32_t* __restrict__ fin,
+                                             uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                             uint32_t* __restrict__ fout, uint64_t n, const Digit& dig, int pass,
+                                             uint64_t tile, const uint32_t* __restrict__ off) {
+    uint32_t* const ub = reinterpret_cast<uint32_t*>(L.kb);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint64_t tbase = tile * kTile;
+    const uint32_t tn = FULL ? (uint32_t)kTile : (uint32_t)(n - tbase);
+    const uint32_t last = (uint32_t)(n - 1);                    // n <= 2^32
+    // wave w reads rows [b0, b0 + 64 kRounds) of the tile: round r, lane l ->
+    // row b0 + 64 r + l, present while 64 r + l < rem
+    const uint64_t b0 = tbase + (uint64_t)w * (64 * kRounds);
+    const int rem = FULL ? 64 * kRounds
+                         : (b0 < n ? (n - b0 < (uint64_t)(64 * kRounds) ? (int)(n - b0) : 64 * kRounds) : 0);
+    auto present = [&](int r) { return FULL || r * 64 + lane < rem; };
     uint64_t k[kRounds];
     uint32_t v[kRounds], f[kRounds];
-    auto load_tile = [&](uint64_t tb) {
-        const uint64_t b0 = tb + (uint64_t)w * (64 * kRounds);
+    {
+        const uint64_t* const kw = kin + b0 + lane;
+        const uint32_t* const fw = fin + b0 + lane;
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
-            const uint64_t idx = b0 + r * 64 + lane;
-            const bool ok = idx < n;
-            k[r] = ok ? kin[idx] : 0;
-            if constexpr (PAY) v[r] = ok ? vin[idx] : 0;
-            if constexpr (CIN) f[r] = ok ? fin[idx] : 0u;
+            if constexpr (FULL) {
+                k[r] = kw[r * 64];
+                if constexpr (CIN) f[r] = fw[r * 64];
+            } else {
+                k[r] = present(r) ? kw[r * 64] : 0;
+                if constexpr (CIN) f[r] = present(r) ? fw[r * 64] : 0u;
+            }
         }
-    };
-    load_tile((uint64_t)blockIdx.x * tiles_per_range * kTile);
-    for (uint64_t t = 0; t < tiles_per_range; ++t) {
-        const uint64_t tbase = ((uint64_t)blockIdx.x * tiles_per_range + t) * kTile;
-        if (tbase >= n) break;                                          // block-uniform
-        for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
-        const uint64_t base = tbase + (uint64_t)w * (64 * kRounds);
-        __syncthreads();                                                // wcnt cleared
-        uint32_t rank[kRounds];
+    }
+    for (int i = tid; i < kDsWaves * 256; i += kDsThreads) (&L.wcnt[0][0])[i] = 0;
+    if (tid < 256) L.base[tid] = off[tile * 256 + tid];
+    uint32_t rank[kRounds];
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            if constexpr (COUT && !CIN) f[r] = (uint32_t)dig.bits(k[r]);
-            if constexpr (COUT || CIN) rank[r] = (f[r] >> (8 * pass)) & 0xFFu;
-            else rank[r] = dig.digit(k[r], pass);                       // all digits first: loads in flight together
-        }
-        // rank inside the wave: the lowest lane of each group of equal digits
-        // adds the group's size to the wave's counter (LDS atomics of one wave
-        // are performed in issue order, so round r + 1 sees round r's adds)
-        // and hands the counter's old value to its group
-        uint32_t leader[kRounds];
+    for (int r = 0; r < kRounds; ++r) {
+        if constexpr (COUT && !CIN) f[r] = (uint32_t)dig.bits(k[r]);
+        if constexpr (COUT || CIN) rank[r] = (f[r] >> (8 * pass)) & 0xFFu;
+        else rank[r] = dig.digit(k[r], pass);
+    }
+    // the keys wait in LDS in load order while the rows are ranked (the
+    // ranking's ballots need the registers); row (w, r, lane) sits at
+    // w * 64 kRounds + 64 r + lane
+    const int nat = w * (64 * kRounds) + lane;
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            const uint64_t idx = base + r * 64 + lane;
-            const bool valid = idx < n;
-            const uint32_t dd = rank[r];
-            const uint64_t peers = match_digit(dd, valid);
-            const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
-            uint32_t old = 0;
-            if (valid && below == 0) old = atomicAdd(&wcnt[w][dd], (uint32_t)__popcll(peers));
-            leader[r] = old;
-            rank[r] = below | (dd << 16) | (valid ? ((uint32_t)(__ffsll((unsigned long long)peers) - 1) << 24) : 0u);
-        }
+    for (int r = 0; r < kRounds; ++r) L.kb[nat + r * 64] = k[r];
+    __syncthreads();                                                // wcnt cleared
+    // rank inside the wave: the lowest lane of each group of equal digits
+    // adds the group's size to the wave's counter (LDS atomics of one wave
+    // are performed in issue order, so round r + 1 sees round r's adds) and
+    // hands the counter's old value to its group
+    uint32_t leader[kRounds];
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            const uint32_t before = (uint32_t)__shfl((int)leader[r], (int)(rank[r] >> 24), 64);
-            rank[r] = ((rank[r] & 0xFFu) + before) | (rank[r] & 0x00FF0000u);   // rank < 4096, digit in bits 16..23
-        }
-        __syncthreads();
-        // per digit: wave prefixes, tile count, tile-local digit start
-        uint32_t tcount = 0;
+    for (int r = 0; r < kRounds; ++r) {
+        const bool valid = present(r);
+        const uint32_t dd = rank[r];
+        const uint64_t peers = match_digit(dd, valid);
+        const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+        uint32_t old = 0;
+        if (valid && below == 0) old = atomicAdd(&L.wcnt[w][dd], (uint32_t)__popcll(peers));
+        leader[r] = old;
+        rank[r] = below | (dd << 16) | (valid ? ((uint32_t)(__ffsll((unsigned long long)peers) - 1) << 24) : 0u);
+    }
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) {
-            uint32_t c = wcnt[ww][d];
-            wcnt[ww][d] = tcount;
+    for (int r = 0; r < kRounds; ++r) {
+        const uint32_t before = (uint32_t)__shfl((int)leader[r], (int)(rank[r] >> 24), 64);
+        rank[r] = ((rank[r] & 0xFFu) + before) | (rank[r] & 0x00FF0000u);   // rank < 1024, digit in bits 16..23
+    }
+    __syncthreads();
+    // per digit (threads 0..255): wave prefixes, tile count, tile-local start
+    uint32_t tcount = 0, x = 0;
+    if (tid < 256) {
+#pragma unroll
+        for (int ww = 0; ww < kDsWaves; ++ww) {
+            const uint32_t c = L.wcnt[ww][tid];
+            L.wcnt[ww][tid] = tcount;
             tcount += c;
         }
-        {   // exclusive scan of the 256 tile counts: wave scans, then the wave totals
-            uint32_t x = tcount;
+        x = tcount;
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o, 64);
-                if (lane >= o) x += y;
-            }
-            if (lane == 63) scan_s[w] = x;
-            __syncthreads();
-            uint32_t add = 0;
-#pragma unroll
-            for (int ww = 0; ww < kWaves; ++ww) add += ww < w ? scan_s[ww] : 0u;
-            lstart[d] = x - tcount + add;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
         }
-        __syncthreads();
-        // rows into LDS in digit order (stable: wave prefix + rank)
+        if (lane == 63) L.scan_s[w] = x;
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t add = 0;
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            const uint64_t idx = base + r * 64 + lane;
-            if (idx < n) {
-                const uint32_t dd = rank[r] >> 16;
-                const uint32_t lp = lstart[dd] + wcnt[w][dd] + (rank[r] & 0xFFFFu);
-                if (lp < kTile) {
-                    kb[lp] = k[r];
-                    if constexpr (PAY) vb[lp] = v[r];
-                    if constexpr (CIN || COUT) fb[lp] = f[r];
-                }
+        for (int ww = 0; ww < 4; ++ww) add += ww < w ? L.scan_s[ww] : 0u;
+        const uint32_t ls = x - tcount + add;
+        L.lstart[tid] = ls;
+        L.base[tid] -= ls;                          // slot i of digit d goes to row base[d] + i (mod 2^32)
+    }
+    __syncthreads();
+    // LDS slot of each row (stable: digit start + wave prefix + rank in the
+    // wave); rows past the end get slot kTile and are not placed
+    uint32_t lp[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const uint32_t dd = rank[r] >> 16;
+        lp[r] = present(r) ? L.lstart[dd] + L.wcnt[w][dd] + (rank[r] & 0xFFFFu) : (uint32_t)kTile;
+    }
+    auto place = [&](int r) { return FULL || lp[r] < (uint32_t)kTile; };
+    auto slot_ok = [&](int r) { return FULL || (uint32_t)(r * kDsThreads + tid) < tn; };
+    __syncthreads();                                                // wcnt read: its space becomes sdig
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r)
+        if (place(r)) L.sdig[lp[r]] = (uint8_t)(rank[r] >> 16);
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) k[r] = L.kb[nat + r * 64];
+    auto load_payload = [&]() {
+        if constexpr (PAY && IDX) {
+#pragma unroll
+            for (int r = 0; r < kRounds; ++r) v[r] = (uint32_t)(b0 + lane + r * 64);   // the row index
+        } else if constexpr (PAY) {
+            const uint32_t* const vw = vin + b0 + lane;
+#pragma unroll
+            for (int r = 0; r < kRounds; ++r) {
+                if constexpr (FULL) v[r] = vw[r * 64];
+                else v[r] = present(r) ? vw[r * 64] : 0u;
             }
         }
-        if (t + 1 < tiles_per_range && tbase + kTile < n) load_tile(tbase + kTile);
-        __syncthreads();
-        const uint32_t tn = tbase + kTile <= n ? kTile : (uint32_t)(n - tbase);
+    };
+    // payloads are loaded only once the digit words are out (fewer live
+    // registers); they arrive while the keys go out
+    if constexpr (!COUT) load_payload();
+    __syncthreads();                                                // sdig written, load-order keys read
+    // output row of slot i = r * kDsThreads + tid (looked up in each phase:
+    // fewer live registers than keeping it)
+    auto g = [&](int r) {
+        const uint32_t i = r * kDsThreads + tid;
+        return min(L.base[L.sdig[i]] + i, last);
+    };
+    if constexpr (COUT) {
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            const uint32_t i = r * kThreads + threadIdx.x;
-            if (i < tn) {
-                uint32_t dd;
-                if constexpr (CIN || COUT) dd = (fb[i] >> (8 * pass)) & 0xFFu;
-                else dd = dig.digit(kb[i], pass);
-                const uint64_t g = (uint64_t)run[dd] + (i - lstart[dd]);
-                if (g < n) {                                  // always true when the counts match
-                    kout[g] = kb[i];
-                    if constexpr (PAY) vout[g] = vb[i];
-                    if constexpr (COUT) fout[g] = fb[i];
-                }
-            }
-        }
+        for (int r = 0; r < kRounds; ++r)
+            if (place(r)) ub[lp[r]] = f[r];
         __syncthreads();
-        run[d] += tcount;
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r)
+            if (slot_ok(r)) fout[g(r)] = ub[r * kDsThreads + tid];
+        load_payload();
+        __syncthreads();                                            // digit words read
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r)
+        if (place(r)) L.kb[lp[r]] = k[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r)
+        if (slot_ok(r)) kout[g(r)] = L.kb[r * kDsThreads + tid];
+    if constexpr (PAY) {
+        __syncthreads();                                            // keys read
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r)
+            if (place(r)) ub[lp[r]] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r)
+            if (slot_ok(r)) vout[g(r)] = ub[r * kDsThreads + tid];
     }
 }
 
-hipError_t launch_scan_hist(const unsigned long long* ghist, int passes, uint64_t n, unsigned long long* offsets,
-                            uint32_t* skip, hipStream_t st);
-
-template <class Digit>
-hipError_t radix_histogram(const uint64_t* keys, uint64_t n, const Digit& dig, int passes, RadixWork& w,
-                           hipStream_t st) {
-    hipError_t e = launch_zero32((uint32_t*)w.hist, (uint64_t)passes * 256 * 2, st);
-    if (e != hipSuccess) return e;
-    if ((e = launch_zero32(w.skip, 8, st)) != hipSuccess) return e;
-    unsigned blocks = (unsigned)(device_cu_count() * 4);
-    uint64_t need = (n + kThreads * 16 - 1) / (kThreads * 16);
-    if (need < blocks) blocks = (unsigned)(need ? need : 1);
-    hipLaunchKernelGGL((k_hist<Digit>), dim3(blocks), dim3(kThreads), 0, st, keys, n, dig, passes, w.hist);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    return launch_scan_hist(w.hist, passes, n, w.offsets, w.skip, st);
+template <class Digit, bool CIN, bool COUT, bool PAY = true, bool IDX = false>
+__global__ __launch_bounds__(kDsThreads, 4) void k_scatter_tile(const uint64_t* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin,
+                                                                const uint32_t* __restrict__ fin,
+                                                                uint64_t* __restrict__ kout,
+                                                                uint32_t* __restrict__ vout,
+                                                                uint32_t* __restrict__ fout, uint64_t n, Digit dig,
+                                                                int pass, uint64_t ntiles,
+                                                                const uint32_t* __restrict__ off) {
+    __shared__ ScatterLds L;
+    const uint64_t tile = xcd_tile(blockIdx.x, ntiles);
+    if ((tile + 1) * kTile <= n)
+        scatter_tile<Digit, CIN, COUT, PAY, IDX, true>(L, kin, vin, fin, kout, vout, fout, n, dig, pass, tile, off);
+    else
+        scatter_tile<Digit, CIN, COUT, PAY, IDX, false>(L, kin, vin, fin, kout, vout, fout, n, dig, pass, tile, off);
 }
+
+// skip[p] = 1 when every key has the same byte p (a stable pass over that
+// digit is the identity permutation); w.hist holds the per-workgroup
+// partials.  All eight flags are set for n = 0.
+hipError_t launch_constant_bytes(const uint64_t* keys, uint64_t n, RadixWork& w, hipStream_t st);
 
 // One stable digit pass; needs w.status of at least radix_work_words(n)
 // words.  fin: carried digit words of kin's rows (byte `pass` is the digit)
 // instead of dig; fout: write the rows' digit words (carried, or
 // dig.bits(key)) next to kout.  vin and vout null: keys only (no carried
-// digit words then).
+// digit words then).  row_index: the payload of row i is i (vin unused,
+// vout required, no fin).
 template <class Digit>
 hipError_t radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
                       const Digit& dig, int pass, RadixWork& w, hipStream_t st, const uint32_t* fin,
-                      uint32_t* fout) {
+                      uint32_t* fout, bool row_index) {
     if (n == 0) return hipSuccess;
-    if ((!vin || !vout) && (vin || vout || fin || fout)) return hipErrorInvalidValue;
-    const uint64_t per = radix_tiles_per_range(n), nr = radix_ranges(n);
+    if (row_index ? (!vout || fin) : ((!vin || !vout) && (vin || vout || fin || fout))) return hipErrorInvalidValue;
+    const uint64_t nt = radix_tiles(n), nch = radix_chunks(n);
+    if (nt > 0xFFFFFFFFull || w.status_cap < nt * 256 + nch * 256 + 256) return hipErrorInvalidValue;
     uint32_t* cnt = w.status;
-    uint32_t* off = w.status + nr * 256;
-    uint32_t* tot = w.status + nr * 512;
-    if (fin) hipLaunchKernelGGL((k_upsweep<Digit, true>), dim3((unsigned)nr), dim3(kThreads), 0, st, kin, fin, n, dig,
-                                pass, per * kTile, (uint32_t)nr, cnt);
-    else hipLaunchKernelGGL((k_upsweep<Digit, false>), dim3((unsigned)nr), dim3(kThreads), 0, st, kin, fin, n, dig,
-                            pass, per * kTile, (uint32_t)nr, cnt);
-    hipLaunchKernelGGL(k_range_scan, dim3(256), dim3(kThreads), 0, st, cnt, (uint32_t)nr, off, tot);
-    const dim3 g((unsigned)nr), b(kThreads);
-    if (!vin || !vout) {
-        hipLaunchKernelGGL((k_downsweep<Digit, false, false, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n,
-                           dig, pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
-    } else if (fin && fout)
-        hipLaunchKernelGGL((k_downsweep<Digit, true, true>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig, pass,
-                           per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+    uint32_t* csum = w.status + nt * 256;
+    uint32_t* dbase = csum + nch * 256;
+    if (fin) hipLaunchKernelGGL((k_tile_counts<Digit, true>), dim3((unsigned)nt), dim3(kThreads), 0, st, kin, fin, n,
+                                dig, pass, cnt);
+    else hipLaunchKernelGGL((k_tile_counts<Digit, false>), dim3((unsigned)nt), dim3(kThreads), 0, st, kin, fin, n, dig,
+                            pass, cnt);
+    hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)nch), dim3(256), 0, st, (const uint32_t*)cnt, nt, csum);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(256), 0, st, csum, nch, dbase);
+    hipLaunchKernelGGL(k_tile_offsets, dim3((unsigned)nch), dim3(256), 0, st, cnt, nt, (const uint32_t*)csum,
+                       (const uint32_t*)dbase);
+    const dim3 g((unsigned)nt), b(kDsThreads);
+    const uint32_t* off = cnt;
+    if (row_index && fout)
+        hipLaunchKernelGGL((k_scatter_tile<Digit, false, true, true, true>), g, b, 0, st, kin, vin, fin, kout, vout,
+                           fout, n, dig, pass, nt, off);
+    else if (row_index)
+        hipLaunchKernelGGL((k_scatter_tile<Digit, false, false, true, true>), g, b, 0, st, kin, vin, fin, kout, vout,
+                           fout, n, dig, pass, nt, off);
+    else if (!vin || !vout)
+        hipLaunchKernelGGL((k_scatter_tile<Digit, false, false, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout,
+                           n, dig, pass, nt, off);
+    else if (fin && fout)
+        hipLaunchKernelGGL((k_scatter_tile<Digit, true, true>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
+                           pass, nt, off);
     else if (fin)
-        hipLaunchKernelGGL((k_downsweep<Digit, true, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
-                           pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+        hipLaunchKernelGGL((k_scatter_tile<Digit, true, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
+                           pass, nt, off);
     else if (fout)
-        hipLaunchKernelGGL((k_downsweep<Digit, false, true>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
-                           pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+        hipLaunchKernelGGL((k_scatter_tile<Digit, false, true>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
+                           pass, nt, off);
     else
-        hipLaunchKernelGGL((k_downsweep<Digit, false, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
-                           pass, per, (uint32_t)nr, (const uint32_t*)off, (const uint32_t*)tot);
+        hipLaunchKernelGGL((k_scatter_tile<Digit, false, false>), g, b, 0, st, kin, vin, fin, kout, vout, fout, n, dig,
+                           pass, nt, off);
     return hipGetLastError();
 }
 

@@ -4,24 +4,23 @@
 // MergeSort<f64>/SortOperator argsorts (ops/sort.hpp:196-213, 506-541).
 //
 // Structure:
-//   1. k_hist: one read of the keys builds all eight 8-bit digit histograms
-//      (per-wave LDS counters, then global 64-bit counters);
-//   2. k_scan_hist: exclusive digit offsets per pass; a pass whose digit is
-//      the same for every key is skipped (a stable pass over one digit is
-//      the identity permutation);
+//   1. k_key_bits: one read of the keys gives the OR and the AND of all
+//      keys; a pass whose byte is the same in every key is skipped (a
+//      stable pass over one digit is the identity permutation);
+//   2. (the digit offsets of each pass come from its own tile counts);
 //   3. per executed pass (reduce-then-scan, no inter-workgroup waits): the
-//      keys are cut into at most 2048 ranges of whole 4096-key tiles;
-//      k_upsweep counts each range's digits, k_range_scan gives every range
-//      its first output position per digit, and k_downsweep (one workgroup
-//      per range, tiles in order) ranks each tile's keys stably (wave
-//      ballots of the digit bits + per-wave LDS counters) and writes keys and
-//      payloads through LDS so that each digit run is written by consecutive
-//      lanes.
-// A decoupled look-back over 4096-key tiles (one status word per tile and
-// digit) was measured at about 0.1 TB/s here: the status loads are coherent
-// across the eight XCDs and the look-back walks them one at a time; the
-// upsweep costs one extra read of the keys (8 of 32 bytes per key and pass)
-// and leaves every workgroup independent.
+//      keys are cut into 8192-key tiles; k_tile_counts counts each tile's
+//      digits, three small kernels turn the counts into every tile's first
+//      output position per digit, and k_scatter_tile (one workgroup per
+//      tile, tiles handed to the XCDs in contiguous blocks) ranks the tile's
+//      keys stably (wave ballots of the digit bits + per-wave LDS counters)
+//      and writes keys and payloads through LDS so that each digit run is
+//      written by consecutive lanes.
+// A decoupled look-back over tiles (one status word per tile and digit) was
+// measured at about 0.1 TB/s here: the status loads are coherent across the
+// eight XCDs and the look-back walks them one at a time; the count pass
+// costs one extra read of the keys (8 of 32 bytes per key and pass) and
+// leaves every workgroup independent.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -37,24 +36,56 @@ __global__ __launch_bounds__(256) void k_zero32(uint32_t* __restrict__ p, uint64
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
 }
 
-// offsets[p][d] = sum of hist[p][d'] for d' < d; skip[p] = one digit holds all keys
-__global__ __launch_bounds__(256) void k_scan_hist(const unsigned long long* __restrict__ ghist, int passes,
-                                                   uint64_t n, unsigned long long* __restrict__ offsets,
-                                                   uint32_t* __restrict__ skip) {
-    __shared__ unsigned long long s[256];
-    for (int p = 0; p < passes; ++p) {
-        unsigned long long v = ghist[p * 256 + threadIdx.x];
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            unsigned long long t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
-            __syncthreads();
-            s[threadIdx.x] += t;
-            __syncthreads();
-        }
-        offsets[p * 256 + threadIdx.x] = s[threadIdx.x] - v;
-        if (v == n) skip[p] = 1;
-        __syncthreads();
+// part[2 b] = OR, part[2 b + 1] = AND of the keys workgroup b reads.
+__global__ __launch_bounds__(256) void k_key_bits(const uint64_t* __restrict__ keys, uint64_t n,
+                                                  unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long so[4], sa[4];
+    uint64_t o = 0, a = ~0ull;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint64_t k0 = keys[i], k1 = keys[i + stride], k2 = keys[i + 2 * stride], k3 = keys[i + 3 * stride];
+        o |= (k0 | k1) | (k2 | k3);
+        a &= (k0 & k1) & (k2 & k3);
+    }
+    for (; i < n; i += stride) {
+        o |= keys[i];
+        a &= keys[i];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        o |= (uint64_t)__shfl_xor((unsigned long long)o, off, 64);
+        a &= (uint64_t)__shfl_xor((unsigned long long)a, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { so[w] = o; sa[w] = a; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = so[0] | so[1] | so[2] | so[3];
+        part[2 * blockIdx.x + 1] = sa[0] & sa[1] & sa[2] & sa[3];
+    }
+}
+
+// skip[p] = byte p of (OR ^ AND) is zero: every key has the same byte p
+__global__ __launch_bounds__(256) void k_key_bits_final(const unsigned long long* __restrict__ part, unsigned nb,
+                                                        uint64_t n, uint32_t* __restrict__ skip) {
+    __shared__ unsigned long long so[4], sa[4];
+    uint64_t o = 0, a = ~0ull;
+    for (unsigned b = threadIdx.x; b < nb; b += 256) {
+        o |= part[2 * b];
+        a &= part[2 * b + 1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        o |= (uint64_t)__shfl_xor((unsigned long long)o, off, 64);
+        a &= (uint64_t)__shfl_xor((unsigned long long)a, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { so[w] = o; sa[w] = a; }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const uint64_t d = (so[0] | so[1] | so[2] | so[3]) ^ (sa[0] & sa[1] & sa[2] & sa[3]);
+        skip[threadIdx.x] = (n == 0 || ((d >> (8 * threadIdx.x)) & 0xFFu) == 0) ? 1u : 0u;
     }
 }
 
@@ -114,16 +145,19 @@ hipError_t launch_iota(uint32_t* idx, uint64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
-uint64_t radix_work_words(uint64_t n) { return radix_ranges(n) * 256 * 2 + 256; }
+uint64_t radix_work_words(uint64_t n) { return (radix_tiles(n) + radix_chunks(n)) * 256 + 256; }
 
-hipError_t launch_scan_hist(const unsigned long long* ghist, int passes, uint64_t n, unsigned long long* offsets,
-                            uint32_t* skip, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(256), 0, st, ghist, passes, n, offsets, skip);
+hipError_t launch_constant_bytes(const uint64_t* keys, uint64_t n, RadixWork& w, hipStream_t st) {
+    unsigned nb = (unsigned)(n / (256 * 16));               // >= 16 keys per thread
+    const unsigned cap = (unsigned)(device_cu_count() * 4) < 1024u ? (unsigned)(device_cu_count() * 4) : 1024u;
+    nb = nb < 1 ? 1 : (nb > cap ? cap : nb);
+    hipLaunchKernelGGL(k_key_bits, dim3(nb), dim3(256), 0, st, keys, n, w.hist);
+    hipLaunchKernelGGL(k_key_bits_final, dim3(1), dim3(256), 0, st, (const unsigned long long*)w.hist, nb, n, w.skip);
     return hipGetLastError();
 }
 
-template hipError_t radix_histogram<LsdDigit>(const uint64_t*, uint64_t, const LsdDigit&, int, RadixWork&, hipStream_t);
 template hipError_t radix_pass<LsdDigit>(const uint64_t*, const uint32_t*, uint64_t*, uint32_t*, uint64_t,
-                                         const LsdDigit&, int, RadixWork&, hipStream_t, const uint32_t*, uint32_t*);
+                                         const LsdDigit&, int, RadixWork&, hipStream_t, const uint32_t*, uint32_t*,
+                                         bool);
 
 }  // namespace vx

@@ -64,8 +64,7 @@ struct TableDigit {
 
 // Device work areas of one sort.
 struct RadixWork {
-    unsigned long long* hist = nullptr;     // 8 x 256 digit counts
-    unsigned long long* offsets = nullptr;  // 8 x 256 exclusive digit offsets
+    unsigned long long* hist = nullptr;     // 2048 words: (or, and) of the keys per workgroup
     uint32_t* skip = nullptr;               // 8 flags: pass moves nothing
     uint32_t* status = nullptr;             // per-range digit counts and offsets (radix_work_words)
     uint64_t status_cap = 0;                // words
@@ -77,12 +76,10 @@ hipError_t launch_f64_keys(const double* x, uint64_t n, int descending, uint64_t
 hipError_t launch_iota(uint32_t* idx, uint64_t n, hipStream_t st);
 hipError_t launch_gather_f64(const double* x, const uint32_t* idx, uint64_t k, double* out, hipStream_t st);
 
-template <class Digit>
-hipError_t radix_histogram(const uint64_t* keys, uint64_t n, const Digit& dig, int passes, RadixWork& w,
-                           hipStream_t st);
+hipError_t launch_constant_bytes(const uint64_t* keys, uint64_t n, RadixWork& w, hipStream_t st);
 template <class Digit>
 hipError_t radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
                       const Digit& dig, int pass, RadixWork& w, hipStream_t st, const uint32_t* fin,
-                      uint32_t* fout);
+                      uint32_t* fout, bool row_index = false);
 
 }  // namespace vx

@@ -45,7 +45,6 @@ bool ensure_work(SortBufs& b, uint64_t n) {
     RadixWork& w = b.w;
     if (!w.hist) {
         if (!hip_ok(hipMalloc((void**)&w.hist, 8 * 256 * 8), "hipMalloc(hist)")) return false;
-        if (!hip_ok(hipMalloc((void**)&w.offsets, 8 * 256 * 8), "hipMalloc(offsets)")) return false;
         if (!hip_ok(hipMalloc((void**)&w.skip, 64), "hipMalloc(skip)")) return false;
         if (!hip_ok(hipHostMalloc((void**)&b.skip_host, 64, hipHostMallocDefault), "hipHostMalloc(skip)")) return false;
     }
@@ -82,17 +81,18 @@ bool ensure_stage(SortBufs& b, uint64_t n) {
 }  // namespace
 
 // Stable sort of (key, payload) pairs by key; output in kout/vout (vin and
-// vout both NULL: keys only).  The input is never written.  Returns the
-// number of digit passes executed.
-int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
-                   StreamCtx* c) {
+// vout both NULL: keys only; row_index: the payload of row i is i, made by
+// the first pass).  The input is never written.  Returns the number of digit
+// passes executed.
+static int sort_pairs_impl(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
+                           StreamCtx* c, bool row_index) {
     if (n >= (1ull << 30)) { set_error("sort: %llu keys exceed the 2^30 limit of the look-back words",
                                        (unsigned long long)n); return -1; }
     SortBufs& b = bufs_for(c);
     if (!ensure_work(b, n)) return -1;
     hipStream_t st = c->stream;
     LsdDigit dig;
-    if (!hip_ok(radix_histogram(kin, n, dig, 8, b.w, st), "radix histogram")) return -1;
+    if (!hip_ok(launch_constant_bytes(kin, n, b.w, st), "key bytes")) return -1;
     if (!hip_ok(hipMemcpyAsync(b.skip_host, b.w.skip, 32, hipMemcpyDeviceToHost, st), "read skip flags")) return -1;
     if (!hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return -1;
     int passes[8], np = 0;
@@ -100,6 +100,7 @@ int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uin
         if (!b.skip_host[p]) passes[np++] = p;
     if (np == 0) {
         if (kout != kin && !hip_ok(hipMemcpyAsync(kout, kin, n * 8, hipMemcpyDeviceToDevice, st), "copy")) return -1;
+        if (row_index) return hip_ok(launch_iota(vout, n, st), "iota") ? 0 : -1;
         if (vout != vin && !hip_ok(hipMemcpyAsync(vout, vin, n * 4, hipMemcpyDeviceToDevice, st), "copy")) return -1;
         return 0;
     }
@@ -109,12 +110,24 @@ int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uin
     for (int i = 0; i < np; ++i) {
         bool to_out = ((np - 1 - i) % 2) == 0;
         uint64_t* kd = to_out ? kout : b.ktmp;
-        uint32_t* vd = vin ? (to_out ? vout : b.vtmp) : nullptr;     // keys only without a payload
-        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, passes[i], b.w, st, nullptr, nullptr), "radix pass")) return -1;
+        uint32_t* vd = (vin || row_index) ? (to_out ? vout : b.vtmp) : nullptr;    // keys only without a payload
+        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, passes[i], b.w, st, nullptr, nullptr, row_index && i == 0),
+                    "radix pass"))
+            return -1;
         ks = kd;
         vs = vd;
     }
     return np;
+}
+
+int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
+                   StreamCtx* c) {
+    return sort_pairs_impl(kin, vin, kout, vout, n, c, false);
+}
+
+// The same with the row index as payload (vout required).
+int sort_pairs_u64_rowidx(const uint64_t* kin, uint64_t* kout, uint32_t* vout, uint64_t n, StreamCtx* c) {
+    return sort_pairs_impl(kin, nullptr, kout, vout, n, c, true);
 }
 
 }  // namespace vx
@@ -179,15 +192,9 @@ int voxel_gpu_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, u
     StreamCtx* c = ctx_for(stream);
     if (!c) return -1;
     if (n == 0) return 0;
-    const uint32_t* vin = vals_in;
-    if (!vin) {
-        // payload = row index (RadixSort's usual indices argument)
-        SortBufs& b = bufs_for(c);
-        if (!ensure_stage(b, n)) return -1;
-        if (!hip_ok(launch_iota(b.v, n, c->stream), "iota")) return -1;
-        vin = b.v;
-    }
-    return sort_pairs_u64(keys_in, vin, keys_out, vals_out, n, c) < 0 ? -1 : 0;
+    if (!vals_in)       // payload = row index (RadixSort's usual indices argument)
+        return sort_pairs_u64_rowidx(keys_in, keys_out, vals_out, n, c) < 0 ? -1 : 0;
+    return sort_pairs_u64(keys_in, vals_in, keys_out, vals_out, n, c) < 0 ? -1 : 0;
 }
 
 int voxel_gpu_argsort_f64(const double* x_dev, size_t n, int descending, uint32_t* idx_out_dev, void* stream) {
