@@ -217,6 +217,15 @@ def bench_sort(log2n: int) -> dict:
     out["speedup_geomean"] = round(math.exp(sum(math.log(x) for x in speedups) / len(speedups)), 3)
     del ko_l, vo_l, ko_r, vo_r
     torch.cuda.empty_cache()
+    # read + write rate of a plain device copy of the same n x 12 bytes: the
+    # practical ceiling of a digit pass, which reads and writes every row
+    src = torch.empty(n * 12, dtype=torch.uint8, device="cuda")
+    src.fill_(1)
+    dst = torch.empty_like(src)
+    t_copy = _sync_time(lambda: dst.copy_(src), reps=5)
+    out["device_copy_rw_gb_per_s"] = round(2 * n * 12 / t_copy / 1e9, 1)
+    del src, dst
+    torch.cuda.empty_cache()
     return out
 
 
