@@ -591,8 +591,10 @@ int vo_scalar_run(const uint32_t* w, size_t n, uint64_t* r) {
 /* row, imm12 = row count (0: all; AGG_PERCENTILE also reads the percent    */
 /* from imm12 & 0xFF).  Sums and Welford run in row order as in the         */
 /* reference.  The distinct set is a sorted array (its size is what the     */
-/* reference returns).  AGG_SUM_DISTINCT and AGG_MODE are not restated:     */
-/* their results depend on unordered_map iteration order.                   */
+/* reference returns).  AGG_SUM_DISTINCT sums the set in ascending order  */
+/* (the reference: unordered_map order; integer sums are exact, f64 sums    */
+/* agree within rounding); AGG_MODE is restated where the largest count is  */
+/* unique (with ties the reference's pick follows unordered_map order).     */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     uint64_t count; double sumf; int64_t sumi;
@@ -628,6 +630,14 @@ void vo_agg_destroy(void* p) {
 static int vo_cmp_u64(const void* a, const void* b) {
     uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
     return x < y ? -1 : x > y;
+}
+
+/* (key, row) pairs: by key, then row */
+static int vo_cmp_pair(const void* a, const void* b) {
+    const uint64_t* x = (const uint64_t*)a;
+    const uint64_t* y = (const uint64_t*)b;
+    if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+    return x[1] < y[1] ? -1 : x[1] > y[1];
 }
 
 /* order key of a T value (f64: -0 as +0) */
@@ -718,7 +728,7 @@ static int vo_agg_word(vo_aggstate* st, const uint64_t* data, uint64_t count, ui
         r[rd] = F ? vo_f64bits(res) : (uint64_t)(int64_t)res;
         return 1;
     }
-    case 0xE9: {                                                         /* COUNT_DISTINCT */
+    case 0xE9: case 0xEA: {                                              /* COUNT_DISTINCT, SUM_DISTINCT */
         if (!s->init) { s->has_set = 1; s->nset = 0; s->init = 1; }
         if (!s->has_set) return -1;                                      /* the reference dereferences a null set */
         for (uint64_t i = start; i < end; ++i) {
@@ -735,7 +745,47 @@ static int vo_agg_word(vo_aggstate* st, const uint64_t* data, uint64_t count, ui
             s->set[lo] = k;
             s->nset++;
         }
-        r[rd] = s->nset;
+        if (op == 0xE9) { r[rd] = s->nset; return 1; }
+        /* SUM_DISTINCT (engine.hpp:629-647): the set's values summed in T's
+           accumulator (f64 / wrapping i64); the reference adds them in
+           unordered_map order, restated here in ascending order (f64 sums
+           agree within rounding, integer sums exactly) */
+        if (F) {
+            double sum = 0;
+            for (size_t i = 0; i < s->nset; ++i) sum += vo_bitsf64(vo_unkey(s->set[i], F));
+            r[rd] = vo_f64bits(sum);
+        } else {
+            uint64_t sum = 0;
+            for (size_t i = 0; i < s->nset; ++i) sum += vo_unkey(s->set[i], F);
+            r[rd] = sum;
+        }
+        return 1;
+    }
+    case 0xEC: {                                                         /* MODE (engine.hpp:669-681) */
+        uint64_t n = end > start ? end - start : 0;
+        if (n == 0) { r[rd] = 0; return 1; }
+        /* (key, row) pairs sorted by key then row: each run of equal keys is
+           one map entry whose stored key is its first row's value; a NaN row
+           is an entry of its own (count 1).  The reference returns the
+           entry with the largest count met first in unordered_map order,
+           which is restated only when that largest count is unique (-1
+           otherwise). */
+        uint64_t* kr = (uint64_t*)malloc(2 * n * 8);
+        for (uint64_t i = 0; i < n; ++i) { kr[2 * i] = vo_okey(data[start + i], F); kr[2 * i + 1] = i; }
+        qsort(kr, n, 16, vo_cmp_pair);
+        uint64_t best = 0, best_row = 0, ties = 0;
+        for (uint64_t i = 0; i < n;) {
+            uint64_t j = i + 1;
+            const int nan = F && isnan(vo_bitsf64(data[start + kr[2 * i + 1]]));
+            if (!nan) while (j < n && kr[2 * j] == kr[2 * i]) ++j;
+            const uint64_t len = j - i;
+            if (len > best) { best = len; best_row = kr[2 * i + 1]; ties = 1; }
+            else if (len == best) ties++;
+            i = j;
+        }
+        free(kr);
+        if (ties != 1) return -1;
+        r[rd] = data[start + best_row];
         return 1;
     }
     case 0xEB: case 0xED: {                                              /* MEDIAN, PERCENTILE */

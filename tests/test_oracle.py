@@ -955,3 +955,71 @@ def test_hash_sort_join_words_restatement_against_reference_engine(is_f64):
             assert np.array_equal(a, b.view(np.uint64)), (trial, [hex(int(c)) for c in code])
         runs += 1
     assert runs == 120
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_sum_distinct_and_mode_restatement_against_reference_engine(is_f64):
+    """vo_straight_run with AGG_SUM_DISTINCT (sharing the distinct set of
+    AGG_COUNT_DISTINCT's slot) and AGG_MODE, against the reference engine on
+    the same words: every register bit-identical except an f64
+    SUM_DISTINCT, which agrees within 1e-12 of the sum of the distinct |x|
+    (the reference adds the set in unordered_map order, the restatement in
+    ascending order).  Programs whose AGG_MODE has a tied largest count are
+    skipped (the restatement refuses them); data with repeated values keeps
+    most counts unique."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(53 if is_f64 else 54)
+    done = modes = sums = 0
+    for trial in range(60):
+        n = int(rng.integers(1, 3000))
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 4, 1)          # repeated values
+            x[rng.integers(0, n, 3)] = -0.0
+            x[rng.integers(0, n, 3)] = 0.0
+        else:
+            x = rng.integers(-40, 40, n).astype(np.int64) * 1_000_003
+        st = o.O.vo_agg_create(1 if is_f64 else 0)
+        regs = np.zeros(64, np.uint64)
+        ref_regs = np.zeros(64, np.uint64)
+        code = tg.straight_program(rng, n, k=10, ops=[0xE9, 0xEA, 0xEA, 0xEC, 0xEC, 0xE1],
+                                   scalar_reads=(1, 2, 3, 13, 14, 15))
+        rc = o.O.vo_straight_run(st, o.ptr(x.view(np.uint64)), n, o.ptr(code), code.size, o.ptr(regs))
+        o.O.vo_agg_destroy(st)
+        if rc < 0:
+            continue
+        run = o.R.vr_engine_run_f64 if is_f64 else o.R.vr_engine_run_i64
+        run(o.ptr(x), n, o.ptr(code), code.size, o.ptr(ref_regs))
+        writers = tg.last_agg_writers(code)
+        scale = float(np.abs(np.unique(x)).sum()) if is_f64 else 0.0
+        for r in range(16):
+            if regs[r] == ref_regs[r]:
+                continue
+            assert is_f64 and writers.get(r) in (0xEA, 0xE1), (trial, r, [hex(int(v)) for v in code])
+            assert abs(o.bits_f64(int(regs[r])) - o.bits_f64(int(ref_regs[r]))) <= 1e-12 * max(scale, 1.0)
+        modes += sum(1 for c in code if int(c) & 0xFF == 0xEC)
+        sums += sum(1 for c in code if int(c) & 0xFF == 0xEA)
+        done += 1
+    assert done >= 30 and modes >= 30 and sums >= 30, (done, modes, sums)
+
+
+def test_mode_restatement_closed_forms():
+    """AGG_MODE on hand-made columns: the unique most frequent value; -0.0 and
+    +0.0 are one key whose value is the first row's; NaN rows are entries of
+    their own; a tied largest count is refused; an empty range gives 0."""
+    def mode(x, start=0, cnt=0):
+        x = np.asarray(x, np.float64)
+        st = o.O.vo_agg_create(1)
+        regs = np.zeros(64, np.uint64)
+        regs[1] = start
+        code = np.array([0xEC | (4 << 8) | (0 << 12) | (1 << 16) | (cnt << 20), 1], np.uint32)
+        rc = o.O.vo_straight_run(st, o.ptr(x.view(np.uint64)), x.size, o.ptr(code), code.size, o.ptr(regs))
+        o.O.vo_agg_destroy(st)
+        return rc, int(regs[4])
+    assert mode([3.0, 1.0, 3.0, 2.0]) == (1, o.f64_bits(3.0))
+    assert mode([-0.0, 5.0, 0.0]) == (1, o.f64_bits(-0.0))
+    assert mode([0.0, -0.0, 5.0]) == (1, o.f64_bits(0.0))
+    assert mode([np.nan, np.nan, 7.0, 7.0])[1] == o.f64_bits(7.0)
+    assert mode([np.nan, 7.0])[0] == -1
+    assert mode([1.0, 2.0])[0] == -1
+    assert mode([1.0, 2.0, 2.0], start=3) == (1, 0)

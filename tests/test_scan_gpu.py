@@ -585,8 +585,10 @@ def test_segment_aggregates_against_oracle(gpu, is_f64):
 
 
 def test_segment_aggregate_programs_are_lowered_or_refused(gpu):
-    """AGG_SUM_DISTINCT / AGG_MODE are refused at load; NaN keys in
-    AGG_COUNT_DISTINCT are refused at run time with the registers untouched."""
+    """AGG_MODE runs where the largest count is unique (NaN rows are entries
+    of their own) and is refused at run time on a tie, with the registers
+    untouched; NaN keys in AGG_COUNT_DISTINCT / AGG_SUM_DISTINCT are refused
+    at run time with the registers untouched."""
     from tests import tier0_gen as tg
     from voxel_amd import voxel_py as vx
     from voxel_amd._lib import lib
@@ -594,15 +596,83 @@ def test_segment_aggregate_programs_are_lowered_or_refused(gpu):
     e = vx.EngineF64()
     e.add_segment(x)
     e.load_program([tg.w(0xEC, rd=4, ra=0, rb=1), 1])
+    e.run()
+    assert o.bits_f64(e.get_scalar(4)) == 2.0
+    e.load_program([tg.w(0xEC, rd=5, ra=0, rb=1, imm=2), 1])             # 1.0 and NaN: one row each
     with pytest.raises(RuntimeError, match="AGG_MODE"):
         e.run()
+    assert e.get_scalar(5) == 0
+    e.load_program([tg.w(0xEA, rd=9, ra=0, rb=1), 1])
+    with pytest.raises(RuntimeError, match="NaN"):
+        e.run()
+    assert e.get_scalar(9) == 0
     e.load_program([tg.w(0xE9, rd=4, ra=0, rb=1), 1])
     with pytest.raises(RuntimeError, match="NaN"):
         e.run()
-    assert e.get_scalar(4) == 0
+    assert o.bits_f64(e.get_scalar(4)) == 2.0                             # the MODE run's result, untouched
     e.load_program([tg.w(0xE0, rd=4, ra=0, rb=1), tg.w(0xE3, rd=5, ra=0, rb=1), 1])
     e.run()
     assert e.get_scalar(4) == 4 and o.bits_f64(e.get_scalar(5)) == 1.0     # NaN counts as != 0; MIN skips it
+
+
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_sum_distinct_and_mode_against_oracle(gpu, is_f64):
+    """Straight-line programs with AGG_SUM_DISTINCT (sharing the distinct set
+    of AGG_COUNT_DISTINCT's slot across three runs of one engine) and
+    AGG_MODE, against the oracle (vo_straight_run, pinned against the
+    reference engine): registers bit-identical, except an f64 SUM_DISTINCT
+    or SUM within 1e-12 of the sum of |x| over the distinct values / rows.
+    A program whose AGG_MODE has a tied largest count fails on the GPU with
+    the registers untouched (the oracle refuses it too)."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(83 if is_f64 else 84)
+    runs = refused = 0
+    for trial in range(10):
+        n = int(rng.integers(1, 30_000))
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 8, 1)
+            x[rng.integers(0, n, 3)] = -0.0
+        else:
+            x = rng.integers(-60, 60, n).astype(np.int64) * 1_000_003
+        scale = float(np.abs(x).sum()) if is_f64 else 0.0
+        e = vx.EngineF64() if is_f64 else vx.EngineI64()
+        e.add_segment(x)
+        st = o.O.vo_agg_create(1 if is_f64 else 0)
+        regs = np.zeros(64, np.uint64)
+        writers = {}
+        try:
+            for k in range(3):
+                code = tg.straight_program(rng, n, k=8, ops=[0xE9, 0xEA, 0xEA, 0xEC, 0xE1],
+                                           scalar_reads=(1, 2, 3, 13, 14, 15))
+                rc = o.O.vo_straight_run(st, o.ptr(x.view(np.uint64)), n, o.ptr(code), code.size, o.ptr(regs))
+                e.load_program([int(c) for c in code])
+                if rc < 0:
+                    # the oracle refused a tied AGG_MODE: so does the GPU, and the
+                    # engine is left as it was (the oracle's slots may have moved:
+                    # start a fresh pair)
+                    snap = [e.get_scalar(r) for r in range(16)]
+                    with pytest.raises(RuntimeError, match="AGG_MODE"):
+                        e.run()
+                    assert [e.get_scalar(r) for r in range(16)] == snap
+                    refused += 1
+                    break
+                e.run()
+                # every AGG word that writes a register (a program may stop at a
+                # division by zero before its last writer runs)
+                for x_ in code:
+                    if 0xE0 <= (int(x_) & 0xFF) <= 0xED and ((int(x_) >> 12) & 0xF) == 0:
+                        writers.setdefault((int(x_) >> 8) & 0xF, set()).add(int(x_) & 0xFF)
+                for r in range(16):
+                    got, want = e.get_scalar(r), int(regs[r])
+                    if got == want:
+                        continue
+                    assert is_f64 and writers.get(r, set()) & {0xEA, 0xE1}, (trial, k, r, writers.get(r))
+                    assert abs(o.bits_f64(got) - o.bits_f64(want)) <= 1e-12 * max(scale, 1.0), (trial, k, r)
+                runs += 1
+        finally:
+            o.O.vo_agg_destroy(st)
+    assert runs >= 15, (runs, refused)
 
 
 def test_failed_aggregate_program_leaves_slots_untouched(gpu):

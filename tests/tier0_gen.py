@@ -138,6 +138,7 @@ def mergeable(spec):
 
 
 AGG_OPS = list(range(0xE0, 0xEA)) + [0xEB, 0xED]          # AGG_COUNT..COUNT_DISTINCT, MEDIAN, PERCENTILE
+AGG_OPS_ALL = list(range(0xE0, 0xEE))                     # ... with SUM_DISTINCT and MODE
 
 
 def straight_program(rng, n, k=8, ops=AGG_OPS, scalar_reads=tuple(range(16))):
@@ -151,7 +152,9 @@ def straight_program(rng, n, k=8, ops=AGG_OPS, scalar_reads=tuple(range(16))):
             op = int(rng.choice(ops))
             imm = 0 if rng.random() < 0.3 else int(rng.integers(1, 4096))
             seg = 0 if rng.random() < 0.95 else 1
-            slot = 9 if op == 0xE9 else int(rng.integers(4, 9))      # the distinct set keeps slot 9 to itself
+            # the distinct set keeps slot 9 to itself (COUNT_DISTINCT and
+            # SUM_DISTINCT share it, as they share the reference's set)
+            slot = 9 if op in (0xE9, 0xEA) else int(rng.integers(4, 9))
             code.append(w(op, rd=slot, ra=seg, rb=int(rng.integers(1, 4)), imm=imm))
         else:
             code += scalar_words(rng, 1, scalar_reads, SCRATCH)

@@ -515,14 +515,16 @@ bool run_agg(Engine& e, uint32_t w, uint64_t* regs, AggSlot* work, StreamCtx* c)
         regs[rd] = F ? f64_bits(res) : (uint64_t)(int64_t)res;
         return true;
     }
-    case 0xE9: {                                                 // COUNT_DISTINCT
+    case 0xE9:                                                   // COUNT_DISTINCT
+    case 0xEA: {                                                 // SUM_DISTINCT (the same slot set, engine.hpp:629-647)
+        const char* name = op == 0xE9 ? "AGG_COUNT_DISTINCT" : "AGG_SUM_DISTINCT";
         if (!a.init) { a.has_set = true; a.nset = 0; a.init = true; }
-        if (!a.has_set) { set_error("AGG_COUNT_DISTINCT on a slot another aggregate started: the reference dereferences a null set"); return false; }
+        if (!a.has_set) { set_error("%s on a slot another aggregate started: the reference dereferences a null set", name); return false; }
         if (n) {
             if (F) {
                 ScanOut r{};
                 if (!scan(CMP_LE, f64_bits(INFINITY), &r)) return false;     // non-NaN rows
-                if (r.cnt != n) { set_error("AGG_COUNT_DISTINCT over NaN values is not lowered (each NaN is its own key in the reference)"); return false; }
+                if (r.cnt != n) { set_error("%s over NaN values is not lowered (each NaN is its own key in the reference)", name); return false; }
             }
             SegaggBufs& b = e.segagg;
             if (!b.ensure(n + a.nset)) return false;                           // no reallocation below
@@ -553,8 +555,39 @@ bool run_agg(Engine& e, uint32_t w, uint64_t* regs, AggSlot* work, StreamCtx* c)
             a.nset = nu;
             a.cur = out;
         }
-        regs[rd] = a.nset;
+        if (op == 0xE9) { regs[rd] = a.nset; return true; }
+        // the set's values summed: i64 wrapping (order-free, exact); f64
+        // compensated (the reference adds them in unordered_map order)
+        if (a.nset == 0) { regs[rd] = 0; return true; }
+        const uint64_t* set = e.sets[slot][a.cur].p;
+        ScanArgs g{};
+        g.begin = 0; g.end = a.nset; g.mode = 0;
+        g.pa = make_pred(e.dtype, CMP_TRUE, 0);
+        ScanOut r{};
+        if (F) {
+            SegaggBufs& b = e.segagg;
+            if (!b.ensure(a.nset)) return false;
+            if (!segagg_unkey_f64(set, a.nset, b.k2, c)) return false;
+            g.dtype = DT_F64; g.a = b.k2;
+            if (!scan_once(c, g, &r)) return false;
+            regs[rd] = f64_bits(bits_f64(r.s) + bits_f64(r.c));
+        } else {
+            g.dtype = DT_I64; g.a = set;                                        // key = value + 2^63 (mod 2^64)
+            if (!scan_once(c, g, &r)) return false;
+            regs[rd] = r.s + (a.nset << 63);
+        }
         return true;
+    }
+    case 0xEC: {                                                 // MODE (no slot state, engine.hpp:669-681)
+        if (n == 0) { regs[rd] = 0; return true; }
+        uint64_t row = 0, ties = 0;
+        if (!segagg_mode(e.dtype, d, n, e.segagg, &row, &ties, c)) return false;
+        if (ties != 1) {
+            set_error("AGG_MODE: %llu values share the largest count; the reference picks one by unordered_map "
+                      "iteration order, which is not lowered", (unsigned long long)ties);
+            return false;
+        }
+        return read_bits(c, d, row, e.dtype, &regs[rd]);
     }
     case 0xEB:
     case 0xED: {                                                 // MEDIAN, PERCENTILE (no slot state)

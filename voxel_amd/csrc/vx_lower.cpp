@@ -144,8 +144,6 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
         for (; k < len && op_of(c[k]) != OP_HALT; ++k) {
             const uint8_t op = op_of(c[k]);
             if (agg_op(op)) {
-                if (op == 0xEA || op == 0xEC)
-                    return none("AGG_SUM_DISTINCT and AGG_MODE are not lowered (their results follow hash-map order)");
                 any_agg = true;
                 continue;
             }

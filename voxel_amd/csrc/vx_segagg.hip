@@ -78,6 +78,60 @@ struct UniqueSrc {
     __device__ __forceinline__ void emit(uint64_t i, uint64_t j) const { out[j] = k[i]; }
 };
 
+// f64 order keys -> the values' bits
+__global__ __launch_bounds__(256) void k_unkey_f64(const uint64_t* __restrict__ k, uint64_t n, uint64_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t key = k[i];
+        out[i] = (key >> 63) ? (key & 0x7FFFFFFFFFFFFFFFull) : ~key;
+    }
+}
+
+// An f64 order key of a NaN: above the key of +inf or below that of -inf.
+__device__ __forceinline__ bool nan_key(uint64_t k) {
+    return k > 0xFFF0000000000000ull || k < 0x000FFFFFFFFFFFFFull;
+}
+
+// Map entries of sorted keys (AGG_MODE): a run of equal keys is one entry,
+// each NaN row an entry of its own; emits each entry's first position.
+struct EntrySrc {
+    const uint64_t* k;
+    uint32_t* out;
+    int f64;
+    __device__ __forceinline__ bool pass(uint64_t i) const {
+        return i == 0 || k[i] != k[i - 1] || (f64 && nan_key(k[i]));
+    }
+    __device__ __forceinline__ void emit(uint64_t i, uint64_t j) const { out[j] = (uint32_t)i; }
+};
+
+// Largest entry size, how many entries have it, and the first position of
+// the earliest of them.
+struct ModePart {
+    uint64_t best, ties, first;
+};
+__device__ __forceinline__ ModePart mode_merge(ModePart a, ModePart b) {
+    if (a.best != b.best) return a.best > b.best ? a : b;
+    return ModePart{a.best, a.ties + b.ties, a.first < b.first ? a.first : b.first};
+}
+
+__global__ __launch_bounds__(256) void k_mode_entries(const uint32_t* __restrict__ starts, uint64_t ne, uint64_t n,
+                                                      ModePart* __restrict__ parts) {
+    __shared__ ModePart red[256];
+    ModePart m{0, 0, ~0ull};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ne; j += stride) {
+        const uint64_t s = starts[j], e = j + 1 < ne ? starts[j + 1] : n;
+        m = mode_merge(m, ModePart{e - s, 1, s});
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = mode_merge(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = red[0];
+}
+
 unsigned gridn(uint64_t n, unsigned cap) {
     const uint64_t b = (n + 255) / 256;
     return (unsigned)(b ? (b < cap ? b : cap) : 1);
@@ -341,6 +395,46 @@ bool segagg_sorted_keys(int dtype, const void* x, uint64_t n, SegaggBufs& b, Str
         if (!hip_ok(hipGetLastError(), "key launch")) return false;
     }
     return sort_pairs_u64(b.k1, b.v1, b.k2, b.v2, n, c) >= 0;
+}
+
+bool segagg_unkey_f64(const uint64_t* keys, uint64_t n, uint64_t* out, StreamCtx* c) {
+    if (n == 0) return true;
+    hipLaunchKernelGGL(k_unkey_f64, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, keys, n, out);
+    return hip_ok(hipGetLastError(), "unkey launch");
+}
+
+bool segagg_mode(int dtype, const void* x, uint64_t n, SegaggBufs& b, uint64_t* row, uint64_t* ties, StreamCtx* c) {
+    *row = 0;
+    *ties = 0;
+    if (n == 0) return true;
+    if (!segagg_sorted_keys(dtype, x, n, b, c)) return false;   // k2: keys, v2: rows (stable); k1 / v1 free
+    EntrySrc src{b.k2, b.v1, dtype == DT_F64 ? 1 : 0};
+    uint64_t ne = 0;
+    if (!run_compact(src, n, c, &ne)) return false;
+    unsigned blocks = gridn(ne, kMomBlocksMax);
+    static_assert(sizeof(ModePart) == sizeof(Moments), "ModePart reuses the moments partials");
+    ModePart* parts = reinterpret_cast<ModePart*>(b.parts);
+    hipLaunchKernelGGL(k_mode_entries, dim3(blocks), dim3(256), 0, c->stream, (const uint32_t*)b.v1, ne, n, parts);
+    if (!hip_ok(hipGetLastError(), "mode launch") ||
+        !hip_ok(hipMemcpyAsync(b.parts_host, parts, blocks * sizeof(ModePart), hipMemcpyDeviceToHost, c->stream),
+                "read mode") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+        return false;
+    const ModePart* ph = reinterpret_cast<const ModePart*>(b.parts_host);
+    ModePart m{0, 0, ~0ull};
+    for (unsigned i = 0; i < blocks; ++i) {
+        const ModePart p = ph[i];
+        if (p.best != m.best) m = p.best > m.best ? p : m;
+        else m = ModePart{m.best, m.ties + p.ties, m.first < p.first ? m.first : p.first};
+    }
+    *ties = m.ties;
+    uint32_t r32 = 0;                                           // the entry's first row in row order
+    if (!hip_ok(hipMemcpyAsync(b.parts_host, b.v2 + m.first, 4, hipMemcpyDeviceToHost, c->stream), "read row") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+        return false;
+    __builtin_memcpy(&r32, b.parts_host, 4);
+    *row = r32;
+    return true;
 }
 
 bool segagg_unique(const uint64_t* k, uint64_t n, uint64_t* out, uint64_t* nout, StreamCtx* c) {

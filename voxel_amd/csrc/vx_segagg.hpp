@@ -37,6 +37,14 @@ bool segagg_sorted_keys(int dtype, const void* x, uint64_t n, SegaggBufs& b, Str
 // Ascending keys k[0, n) -> the distinct ones in out (device), *nout.
 bool segagg_unique(const uint64_t* k, uint64_t n, uint64_t* out, uint64_t* nout, StreamCtx* c);
 
+// f64 order keys k[0, n) -> the values' bits in out (device).
+bool segagg_unkey_f64(const uint64_t* k, uint64_t n, uint64_t* out, StreamCtx* c);
+
+// AGG_MODE over x[0, n): entries of equal values (-0.0 with +0.0; each NaN
+// row its own entry); *ties = entries with the largest row count, *row =
+// first row (row order) of the earliest-sorting of them.  n < 2^30.
+bool segagg_mode(int dtype, const void* x, uint64_t n, SegaggBufs& b, uint64_t* row, uint64_t* ties, StreamCtx* c);
+
 // Vector words of a Tier-0 loop body evaluated per row (the lanes of the
 // reference's vector registers are independent for these words).
 struct ExprOp {
