@@ -65,6 +65,9 @@ _SIGS = {
     "vo_tier0_expr_i64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P, P, P, P, P]),
     "vo_tier0_expr_col_f64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P]),
     "vo_tier0_expr_col_i64": (None, [P, U64, U64, U64, INT, INT, P, P, INT, P]),
+    "vo_expr_reset_masks": (None, []),
+    "vo_expr_set_gather": (None, [INT, P, U64]),
+    "vo_expr_set_store": (None, [INT, P, U64]),
     "vo_window_f64": (I64, [INT, P, SZ, P, SZ, INT, F64]),
     "vo_tier0_window_f64": (INT, [P, U64, U64, U64, INT, INT, INT, INT, P, P, INT, P, P, P, P, P]),
     "vo_nullable_f64": (None, [P, SZ, P, P, P, P]),

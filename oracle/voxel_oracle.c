@@ -451,14 +451,62 @@ static T NAME(int op, int cmp, T x, T y, T s) {                                 
                           : (s == (T)0 || s == (T)-1 ? (T)0 : (T)((int64_t)x % (int64_t)s));  \
     case 0x95: return ISF ? ((x < s || x != x) ? x : s) : (x < s ? x : s);                   \
     case 0x96: return ISF ? ((x > s || s != s) ? x : s) : (x > s ? x : s);                   \
-    default: return (cmp != 7 && VO_PRED(cmp, x, s)) ? x : (T)0;                            \
+    case 0x8F: return ISF ? (T)pow((double)x, (double)y) : (T)vo_ipow((int64_t)x, (int64_t)y); \
+    default: return ((cmp & 0xF) != 7 && VO_PRED(cmp & 0xF, x, s)) ? x : (T)0;              \
     }                                                                                        \
+}
+/* Engine<T>::IntPow (engine.hpp:443-454): square-and-multiply in wrapping
+   i64, 0 for a negative exponent */
+static int64_t vo_ipow(int64_t b, int64_t e) {
+    if (e < 0) return 0;
+    uint64_t r = 1, bb = (uint64_t)b;
+    while (e) {
+        if (e & 1) r *= bb;
+        bb *= bb;
+        e >>= 1;
+    }
+    return (int64_t)r;
 }
 VO_DEF_EXPR_LANE(vo_expr_lane_f64, double, 1, vo_lb_f64, vo_bl_f64)
 VO_DEF_EXPR_LANE(vo_expr_lane_i64, int64_t, 0, vo_lb_i64, vo_bl_i64)
 
-/* one kL = 4 group of the body's vector words on the 16 vector registers */
-#define VO_DEF_EXPR_GROUP(NAME, T, LANE)                                                     \
+/* The engine's eight mask registers (RegFile masks): the filters, VCMPxx
+   (mask 0) and VCMPNULL / VCMPNOTNULL (mask imm12 & 7) write lane bits that
+   VBLEND reads (engine.hpp:1090-1110, 480-493, 2700-2750, 2926-2936). */
+static uint32_t vo_masks[8];
+void vo_expr_reset_masks(void) { memset(vo_masks, 0, sizeof(vo_masks)); }
+
+/* The segment VGATHER reads (engine.hpp:1987-2006), one segment id at a
+   time (test aid): other ids are segments the engine does not have. */
+static const void* vo_gdata;
+static uint64_t vo_gcount;
+static int vo_gseg = -1;
+void vo_expr_set_gather(int seg, const void* data, uint64_t count) {
+    vo_gseg = seg;
+    vo_gdata = data;
+    vo_gcount = count;
+}
+
+/* The segment VSTORE writes (engine.hpp:1971-1985), one segment id at a
+   time (test aid; it may be the loaded column itself): lanes [0, count) to
+   rows off + lane below the segment's count.  Other ids: nothing written. */
+static void* vo_sdata;
+static uint64_t vo_scount;
+static int vo_sseg = -1;
+void vo_expr_set_store(int seg, void* data, uint64_t count) {
+    vo_sseg = seg;
+    vo_sdata = data;
+    vo_scount = count;
+}
+
+/* one kL = 4 group of the body's vector words on the 16 vector registers.
+   Op word k: code[5k..5k+4] = opcode, dst, a, b, x; x = the filter's kCmp
+   | mask register << 4 for VFILTER*, imm12 for the lane words (VROTATE
+   shift, VSHUFFLE mode, VINSERT lane, VBLEND and VCMP(NOT)NULL mask,
+   VGATHER segment << 8 | count).  Lane words (engine.hpp:2124-2217):
+   VINSERT lane = R[rb], VPERMUTE by the bytes of rb's first lane, VSHUFFLE
+   halves, VREVERSE, VROTATE, VSLIDE by R[rb] (size_t arithmetic). */
+#define VO_DEF_EXPR_GROUP(NAME, T, ISF, LANE, LB, BL)                                        \
 static void NAME(T v[16][4], const T* data, uint64_t count, uint64_t off, int vload_dst,    \
                  int nops, const int* code, const T* sval) {                                 \
     for (int i = 0; i < 4; ++i) {                                                            \
@@ -467,13 +515,70 @@ static void NAME(T v[16][4], const T* data, uint64_t count, uint64_t off, int vl
     }                                                                                        \
     for (int k = 0; k < nops; ++k) {                                                         \
         const int* c = code + 5 * k;                                                         \
+        const int op = c[0], a = c[2], b = c[3], x = c[4];                                   \
         T r[4];                                                                              \
-        for (int i = 0; i < 4; ++i) r[i] = LANE(c[0], c[4], v[c[2]][i], v[c[3]][i], sval[k]); \
+        switch (op) {                                                                        \
+        case 0x71:                                                              /* VSTORE */  \
+            if (((x >> 8) & 0xF) == vo_sseg) {                                               \
+                const int cnt = (x & 0xFF) ? (x & 0xFF) : 4;                                 \
+                for (int i = 0; i < cnt && off + (uint64_t)i < vo_scount; ++i)               \
+                    ((T*)vo_sdata)[off + (uint64_t)i] = v[a][i];                             \
+            }                                                                                \
+            continue;                                                   /* no register written */ \
+        case 0x7D: for (int i = 0; i < 4; ++i) r[i] = v[a][3 - i]; break;      /* VREVERSE */ \
+        case 0x7E: for (int i = 0; i < 4; ++i) r[(i + (x & 3)) & 3] = v[a][i]; break;  /* VROTATE */ \
+        case 0x7F: {                                                            /* VSLIDE */  \
+            const uint64_t sh = LB(sval[k]);                                                 \
+            for (int i = 0; i < 4; ++i) { uint64_t si = (uint64_t)i + sh; r[i] = si < 4 ? v[a][si] : (T)0; } \
+            break;                                                                           \
+        }                                                                                    \
+        case 0x7C:                                                              /* VSHUFFLE */ \
+            for (int i = 0; i < 2; ++i) {                                                    \
+                const int j = (x & 3) == 0 ? 2 * i : (x & 3) == 1 ? 2 * i + 1 : (x & 3) == 2 ? i : i + 2; \
+                r[i] = v[a][j];                                                              \
+                r[i + 2] = v[b][j];                                                          \
+            }                                                                                \
+            break;                                                                           \
+        case 0x7B: {                                                            /* VPERMUTE */ \
+            const uint64_t ib = LB(v[b][0]);                                                 \
+            for (int i = 0; i < 4; ++i) r[i] = v[a][((ib >> (8 * i)) & 0xFF) % 4];           \
+            break;                                                                           \
+        }                                                                                    \
+        case 0x7A: for (int i = 0; i < 4; ++i) r[i] = i == (x & 3) ? sval[k] : v[a][i]; break; /* VINSERT */ \
+        case 0xC7:                                                              /* VBLEND */  \
+            for (int i = 0; i < 4; ++i) r[i] = ((vo_masks[x & 7] >> i) & 1) ? v[b][i] : v[a][i]; \
+            break;                                                                           \
+        case 0x72: {                                                            /* VGATHER */ \
+            const int cnt = (x & 0xFF) ? (x & 0xFF) : 4;                                     \
+            if (((x >> 8) & 0xF) != vo_gseg) { for (int i = 0; i < 4; ++i) r[i] = (T)0; break; } \
+            for (int i = 0; i < 4; ++i) r[i] = v[c[1]][i];                                   \
+            for (int i = 0; i < 4 && i < cnt; ++i) {                                         \
+                const int32_t i32 = (int32_t)(uint32_t)(LB(v[a][i >> 1]) >> (32 * (i & 1))); \
+                const uint64_t gi = (uint64_t)(int64_t)i32;                                  \
+                r[i] = gi < vo_gcount ? ((const T*)vo_gdata)[gi] : (T)0;                     \
+            }                                                                                \
+            break;                                                                           \
+        }                                                                                    \
+        default: {                                                                           \
+            uint32_t m = 0;                                                                  \
+            for (int i = 0; i < 4; ++i) {                                                    \
+                r[i] = LANE(op, x, v[a][i], v[b][i], sval[k]);                               \
+                if (op >= 0xC0 && op <= 0xC6)                                                \
+                    m |= ((x & 0xF) != 7 && VO_PRED(x & 0xF, v[a][i], sval[k])) ? 1u << i : 0u; \
+                else                                                                         \
+                    m |= LB(r[i]) ? 1u << i : 0u;                                            \
+            }                                                                                \
+            if (op >= 0xC0 && op <= 0xC6) vo_masks[(x >> 4) & 7] = m;                        \
+            else if (op >= 0xA0 && op <= 0xA5) vo_masks[0] = m;                              \
+            else if (op == 0xA6 || op == 0xA7) vo_masks[x & 7] = m;                          \
+            break;                                                                           \
+        }                                                                                    \
+        }                                                                                    \
         for (int i = 0; i < 4; ++i) v[c[1]][i] = r[i];                                       \
     }                                                                                        \
 }
-VO_DEF_EXPR_GROUP(vo_expr_group_f64, double, vo_expr_lane_f64)
-VO_DEF_EXPR_GROUP(vo_expr_group_i64, int64_t, vo_expr_lane_i64)
+VO_DEF_EXPR_GROUP(vo_expr_group_f64, double, 1, vo_expr_lane_f64, vo_lb_f64, vo_bl_f64)
+VO_DEF_EXPR_GROUP(vo_expr_group_i64, int64_t, 0, vo_expr_lane_i64, vo_lb_i64, vo_bl_i64)
 
 #define VO_DEF_TIER0_EXPR(NAME, COLNAME, T, GROUP, RED)                                      \
 void NAME(const T* data, uint64_t count, uint64_t off, uint64_t total, int vload_dst,       \

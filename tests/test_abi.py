@@ -154,7 +154,27 @@ def test_lowering_accepts_vector_arithmetic():
     assert lowered(loop([W(0x80, rd=2, ra=0, rb=7)] + red)) == 0                   # V7 undefined
     assert lowered(loop([mul, W(0x80, rd=1, ra=0, rb=0)] + red)) == 0             # V1 redefined
     assert lowered(loop([W(0x92, rd=2, ra=0, rb=5)] + red)) == 0                  # scalar operand written by the loop
-    assert lowered(loop([W(0x8F, rd=2, ra=0, rb=0)] + red)) == 0                  # VPOW is not lowered
+    assert lowered(loop([W(0x8F, rd=2, ra=0, rb=0)] + red)) == 2                  # VPOW
+    # lane words: VREVERSE, VROTATE, VSLIDE by R6, VSHUFFLE, VPERMUTE, VINSERT R3 into lane 2, VGATHER
+    for lw in (W(0x7D, rd=2, ra=0), W(0x7E, rd=2, ra=0, imm=1), W(0x7F, rd=2, ra=0, rb=6),
+               W(0x7C, rd=2, ra=0, rb=0, imm=3), W(0x7B, rd=2, ra=0, rb=0), W(0x7A, rd=2, ra=0, rb=3, imm=2),
+               W(0x72, rd=2, ra=0, imm=(1 << 8) | 0)):
+        assert lowered(loop([lw] + red)) == 2, hex(lw)
+    # VBLEND needs a mask written earlier in the iteration; VGATHER needs every lane
+    blend = W(0xC7, rd=2, ra=0, rb=1, imm=3)
+    assert lowered(loop([W(0xC5, rd=1, ra=0, rb=6, imm=3), blend] + red)) == 2
+    assert lowered(loop([W(0xC5, rd=1, ra=0, rb=6, imm=2), blend] + red)) == 0
+    assert lowered(loop([W(0x72, rd=2, ra=0, imm=(1 << 8) | 3)] + red)) == 0
+    # VSTORE at the loop offset, lanes 1..4, with or without a reduction
+    store = W(0x71, rd=0, ra=1, imm=(2 << 8) | 3)
+    v2 = W(0x92, rd=2, ra=0, rb=3)
+    assert lowered(loop([v2, store] + red)) == 2 and lowered(loop([v2, store])) == 2
+    assert lowered(loop([v2, W(0x71, rd=0, ra=1, imm=(2 << 8) | 5)] + red)) == 0    # past the register
+    assert lowered(loop([v2, W(0x71, rd=0, ra=3, imm=2 << 8)] + red)) == 0          # another offset register
+    assert lowered(loop([W(0x72, rd=2, ra=0, imm=2 << 8), W(0x71, rd=0, ra=1, imm=2 << 8)] + red)) == 0
+    segs = [W(0x72, rd=2, ra=0, imm=1 << 8), W(0x72, rd=3, ra=0, imm=2 << 8), W(0x72, rd=4, ra=0, imm=3 << 8)]
+    assert lowered(loop(segs[:2] + [W(0xD0, rd=5, ra=3), W(0x2A, rd=0, ra=0, rb=5)])) == 2
+    assert lowered(loop(segs + [W(0xD0, rd=5, ra=4), W(0x2A, rd=0, ra=0, rb=5)])) == 0
     for op in (0x83, 0x84, 0x93, 0x94):                                           # division and modulo
         div = loop([W(op, rd=2, ra=0, rb=3 if op >= 0x90 else 0)] + red)
         assert lowered(div) == 2 and lowered(div, i64=True) == 2

@@ -9,6 +9,8 @@ CPU only.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import math
 import random
 from fractions import Fraction
@@ -1023,3 +1025,117 @@ def test_mode_restatement_closed_forms():
     assert mode([np.nan, 7.0])[0] == -1
     assert mode([1.0, 2.0])[0] == -1
     assert mode([1.0, 2.0, 2.0], start=3) == (1, 0)
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_lane_words_restatement_against_reference_engine(is_f64):
+    """vo_tier0_expr_* with the lane words (VINSERT, VPERMUTE, VSHUFFLE,
+    VREVERSE, VROTATE, VSLIDE, VBLEND on masks the body's filters and
+    compares wrote, VPOW, VGATHER from a second segment or a missing one)
+    against the reference engine (two segments) running the same words:
+    every accumulator and partial bit-identical.  After the random
+    programs, each lane word is the last word of three programs, reduced
+    first."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(95 if is_f64 else 96)
+    forced = tg.LANE_OPS
+    done = 0
+    for trial in range(80 + 3 * len(forced)):
+        last = forced[(trial - 80) // 3] if trial >= 80 else None
+        n = int(rng.integers(1, 200)) * 4
+        m = int(rng.integers(1, 400))
+        regs = np.zeros(64, np.uint64)
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 2)
+            x[rng.integers(0, n, 6)] = 0.0
+            g = np.round(rng.standard_normal(m) * 10, 3)
+            for r in tg.SCAL_REGS:
+                regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+        else:
+            x = rng.integers(-20, 600, n).astype(np.int64)          # small values: in-range gather indices
+            g = rng.integers(-1000, 1000, m).astype(np.int64)
+            for r in tg.SCAL_REGS:
+                regs[r] = int(rng.integers(-5, 50)) & (2**64 - 1)
+        regs[tg.SLIDE_REG] = int(rng.choice([0, 1, 2, 3, 5, -1, -2])) & (2**64 - 1)
+        regs[2] = n
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 7)), last=last, lane_words=True)
+        if not is_f64 and tg.expr_traps(o, x, n, spec, regs):
+            continue
+        o.O.vo_expr_reset_masks()
+        o.O.vo_expr_set_gather(1, o.ptr(g), m)
+        acc, part = tg.oracle_expr(o, x, n, spec, regs, is_f64)
+        ref = regs.copy()
+        ptrs = (C.c_void_p * 2)(x.ctypes.data, g.ctypes.data)
+        ns = (C.c_size_t * 2)(n, m)
+        run = o.R.vr_engine_run_segs_f64 if is_f64 else o.R.vr_engine_run_segs_i64
+        run(ptrs, ns, 2, o.ptr(code), code.size, o.ptr(ref))
+        for b in range(len(spec["red"])):
+            # VPOW runs the host's libm pow on both sides: bit-identical too
+            assert int(ref[tg.ACCS[b]]) == acc[b], (trial, b, spec)
+            assert int(ref[tg.PARTS[b]]) == part[b], (trial, b, spec)
+        done += 1
+    assert done >= 60
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_vstore_restatement_against_reference_engine(is_f64):
+    """vo_tier0_expr_* with VSTOREs at the loop offset (into a third segment,
+    or into the loaded column itself, lane counts 1..4 and more) after
+    vector arithmetic, filters and lane words, with or without reductions,
+    against the reference engine on three segments: the stored segments
+    and every accumulator and partial bit-identical."""
+    from tests import tier0_gen as tg
+    rng = np.random.default_rng(97 if is_f64 else 98)
+    done = 0
+    for trial in range(80):
+        n = int(rng.integers(1, 150)) * 4
+        m = int(rng.integers(1, 300))
+        ny = int(rng.integers(1, n + 9))
+        regs = np.zeros(64, np.uint64)
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 2)
+            g = np.round(rng.standard_normal(m) * 10, 3)
+            y = np.round(rng.standard_normal(ny), 1)
+            for r in tg.SCAL_REGS:
+                regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+        else:
+            x = rng.integers(-20, 600, n).astype(np.int64)
+            g = rng.integers(-1000, 1000, m).astype(np.int64)
+            y = rng.integers(-9, 9, ny).astype(np.int64)
+            for r in tg.SCAL_REGS:
+                regs[r] = int(rng.integers(-5, 50)) & (2**64 - 1)
+        regs[tg.SLIDE_REG] = int(rng.choice([0, 1, 2, -1])) & (2**64 - 1)
+        regs[2] = n
+        segs = (2,) if trial % 3 else (2, 0)
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 6)), lane_words=trial % 2 == 0,
+                                          store_segs=segs, lane_ops=[op for op in tg.LANE_OPS if op != 0x8F])
+        if not is_f64 and tg.expr_traps(o, x, n, spec, regs):
+            continue
+        xo, yo = x.copy(), y.copy()
+        o.O.vo_expr_reset_masks()
+        o.O.vo_expr_set_gather(1, o.ptr(g), m)
+        acc = part = []
+        store_ids = {op[4] >> 8 for op in spec["ops"] if op[0] == 0x71}
+        if len(store_ids) > 1:
+            continue                                           # the restatement holds one store segment
+        sid = store_ids.pop()
+        o.O.vo_expr_set_store(sid, o.ptr(xo if sid == 0 else yo), n if sid == 0 else ny)
+        if spec["red"]:
+            acc, part = tg.oracle_expr(o, xo, n, spec, regs, is_f64)
+        else:
+            tg.oracle_expr(o, xo, n, spec, regs, is_f64)
+        xr, yr = x.copy(), y.copy()
+        ref = regs.copy()
+        ptrs = (C.c_void_p * 3)(xr.ctypes.data, g.ctypes.data, yr.ctypes.data)
+        ns = (C.c_size_t * 3)(n, m, ny)
+        run = o.R.vr_engine_run_segs_f64 if is_f64 else o.R.vr_engine_run_segs_i64
+        run(ptrs, ns, 3, o.ptr(code), code.size, o.ptr(ref))
+        assert np.array_equal(yo.view(np.uint64), yr.view(np.uint64)), (trial, spec)
+        assert np.array_equal(xo.view(np.uint64), xr.view(np.uint64)), (trial, spec)
+        for b in range(len(spec["red"])):
+            assert int(ref[tg.ACCS[b]]) == acc[b], (trial, b, spec)
+            assert int(ref[tg.PARTS[b]]) == part[b], (trial, b, spec)
+        done += 1
+    assert done >= 50, done

@@ -526,6 +526,174 @@ def test_tier0_scalar_prefix_and_epilogue(gpu, is_f64):
 
 
 @pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_vstore(gpu, is_f64):
+    """Loops that VSTORE computed vectors at the loop offset (into a third
+    segment over caller memory, or into the loaded column itself; lane
+    counts 1..4 and more; rows past the segment's end dropped), with or
+    without reductions, after vector arithmetic, filters and lane words:
+    the caller's arrays hold exactly what the oracle (vo_tier0_expr_* with
+    its store segment, pinned against the reference engine) wrote, and the
+    registers match as in the arithmetic tests."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(77 if is_f64 else 78)
+    checked = 0
+    for trial in range(40):
+        n = int(rng.integers(1, 2000)) * 4
+        m = int(rng.integers(1, 3000))
+        ny = int(rng.integers(1, n + 9))
+        regs = np.zeros(64, np.uint64)
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 2)
+            g = np.round(rng.standard_normal(m) * 10, 3)
+            y = np.round(rng.standard_normal(ny), 1)
+            for r in tg.SCAL_REGS:
+                regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+            e = vx.EngineF64()
+        else:
+            x = rng.integers(-20, 6000, n).astype(np.int64)
+            g = rng.integers(-1000, 1000, m).astype(np.int64)
+            y = rng.integers(-9, 9, ny).astype(np.int64)
+            for r in tg.SCAL_REGS:
+                regs[r] = int(rng.integers(-5, 50)) & (2**64 - 1)
+            e = vx.EngineI64()
+        regs[tg.SLIDE_REG] = int(rng.choice([0, 1, 2, -1])) & (2**64 - 1)
+        regs[2] = n
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 7)), lane_words=trial % 2 == 0,
+                                          store_segs=(2,) if trial % 3 else (0,),
+                                          lane_ops=[op for op in tg.LANE_OPS if op != 0x8F])
+        if not is_f64 and tg.expr_traps(o, x, n, spec, regs):
+            continue
+        sid = 2 if trial % 3 else 0
+        xo, yo = x.copy(), y.copy()
+        o.O.vo_expr_reset_masks()
+        o.O.vo_expr_set_gather(1, o.ptr(g), m)
+        o.O.vo_expr_set_store(sid, o.ptr(xo if sid == 0 else yo), n if sid == 0 else ny)
+        acc, part = tg.oracle_expr(o, xo, n, spec, regs, is_f64) if spec["red"] else ([], [])
+        if not spec["red"]:
+            tg.oracle_expr(o, xo, n, spec, regs, is_f64)
+        for a in (x, g, y):
+            e.add_segment(a)
+        for r in (2, tg.SLIDE_REG) + tg.SCAL_REGS:
+            e.set_scalar(r, int(regs[r]))
+        e.load_program([int(c) for c in code])
+        e.run()
+        assert np.array_equal(x.view(np.uint64), xo.view(np.uint64)), (trial, spec)
+        assert np.array_equal(y.view(np.uint64), yo.view(np.uint64)), (trial, spec)
+        assert e.get_scalar(1) == n
+        for b, red in enumerate(spec["red"]):
+            if red == 0 and is_f64:
+                continue                              # f64 sums: the arithmetic tests' tolerance; here the stores
+            got_p, got = e.get_scalar(tg.PARTS[b]), e.get_scalar(tg.ACCS[b])
+            if is_f64 and math.isnan(o.bits_f64(part[b])):
+                assert math.isnan(o.bits_f64(got_p))
+            else:
+                assert got_p == part[b], (trial, b, spec)
+        checked += 1
+    assert checked >= 30
+
+
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_lane_words(gpu, is_f64):
+    """Loops with the lane words (VINSERT, VPERMUTE, VSHUFFLE, VREVERSE,
+    VROTATE, VSLIDE by a register, VBLEND on masks that filters and compares
+    of the same iteration wrote, VPOW, VGATHER from a second segment or a
+    missing one) mixed with vector arithmetic and filters (generated words):
+    partials, counts, min and max equal the oracle's bits (vo_tier0_expr_*,
+    pinned against the reference engine), NaN compared as NaN; sums of
+    reduced registers within 2 ulp of the exact sum of the oracle's lanes
+    (i64 exact).  An f64 VPOW lane is the device pow: programs with one are
+    compared within 1e-12 of the sum of |lanes| (the reference's pow is the
+    host libm's; both are faithfully rounded)."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    rng = np.random.default_rng(75 if is_f64 else 76)
+
+    def same(got, want):
+        if is_f64 and math.isnan(o.bits_f64(want)):
+            return math.isnan(o.bits_f64(got))
+        return got == want
+
+    forced = tg.LANE_OPS
+    checked = 0
+    for trial in range(40 + 3 * len(forced)):
+        last = forced[(trial - 40) // 3] if trial >= 40 else None
+        n = int(rng.integers(1, 3000)) * 4
+        m = int(rng.integers(1, 5000))
+        regs = np.zeros(64, np.uint64)
+        if is_f64:
+            x = np.round(rng.standard_normal(n) * 100, 2)
+            x[rng.integers(0, n, 6)] = 0.0
+            g = np.round(rng.standard_normal(m) * 10, 3)
+            for r in tg.SCAL_REGS:
+                regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)))
+            e = vx.EngineF64()
+        else:
+            x = rng.integers(-20, 6000, n).astype(np.int64)
+            g = rng.integers(-1000, 1000, m).astype(np.int64)
+            for r in tg.SCAL_REGS:
+                regs[r] = int(rng.integers(-5, 50)) & (2**64 - 1)
+            e = vx.EngineI64()
+        regs[tg.SLIDE_REG] = int(rng.choice([0, 1, 2, 3, 5, -1, -2])) & (2**64 - 1)
+        regs[2] = n
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 9)), last=last, lane_words=True)
+        e.add_segment(x)
+        e.add_segment(g)
+        for r in (2, tg.SLIDE_REG) + tg.SCAL_REGS:
+            e.set_scalar(r, int(regs[r]))
+        e.load_program([int(c) for c in code])
+        if not is_f64 and tg.expr_traps(o, x, n, spec, regs):
+            continue
+        e.run()
+        o.O.vo_expr_reset_masks()
+        o.O.vo_expr_set_gather(1, o.ptr(g), m)
+        acc, part = tg.oracle_expr(o, x, n, spec, regs, is_f64)
+        has_pow = is_f64 and any(op[0] == 0x8F for op in spec["ops"])
+        assert e.get_scalar(1) == n
+        for b, red in enumerate(spec["red"]):
+            got_p, got = e.get_scalar(tg.PARTS[b]), e.get_scalar(tg.ACCS[b])
+            col = tg.oracle_expr_col(o, x, n, spec, regs, spec["bvreg"][b], is_f64)
+            if has_pow and red != 1:
+                for gv, wv, lanes in ((got_p, part[b], col[-4:]), (got, acc[b], col)):
+                    gf, wf = o.bits_f64(gv), o.bits_f64(wv)
+                    if math.isnan(wf) or math.isinf(wf):
+                        assert math.isnan(gf) == math.isnan(wf) and (math.isnan(wf) or gf == wf), (trial, b)
+                    else:
+                        fin = np.abs(lanes[np.isfinite(lanes)])
+                        scale = max(float(fin.sum()) if fin.size else 0.0, abs(wf))
+                        assert abs(gf - wf) <= 1e-12 * scale + 1e-300, (trial, b, gf, wf)
+                continue
+            assert same(got_p, part[b]), (trial, b, spec)
+            if red == 0 and is_f64:
+                gf = o.bits_f64(got)
+                if math.isnan(o.bits_f64(acc[b])):
+                    assert math.isnan(gf)
+                elif np.all(np.isfinite(col)):
+                    assert close_f64(gf, o.exact_sum(col)), (trial, b, gf)
+                else:
+                    assert gf == o.bits_f64(acc[b]), (trial, b, gf)
+            else:
+                assert same(got, acc[b]), (trial, b, spec)
+        checked += 1
+    assert checked >= 50
+    # refused shapes: VBLEND on a mask the iteration has not written, a partial VGATHER
+    e = vx.EngineF64()
+    e.add_segment(np.arange(64, dtype=np.float64))
+    e.set_scalar(2, 64)
+    body_blend = [tg.w(0x10, rd=4, imm=4), tg.w(0x70, rd=0, ra=1), tg.w(0xC7, rd=1, ra=0, rb=0, imm=3),
+                  tg.w(0xD0, rd=5, ra=1), tg.w(0x2A, rd=8, ra=8, rb=5), tg.w(0x20, rd=1, ra=1, rb=4),
+                  tg.w(0x40, ra=1, rb=2), tg.w(0x52, imm=(1 - 8) & 0xFFF), tg.w(0x01)]
+    e.load_program(body_blend)
+    with pytest.raises(RuntimeError, match="VBLEND"):
+        e.run()
+    body_gather = list(body_blend)
+    body_gather[2] = tg.w(0x72, rd=1, ra=0, imm=(0 << 8) | 2)
+    e.load_program(body_gather)
+    with pytest.raises(RuntimeError, match="VGATHER"):
+        e.run()
+
+
+@pytest.mark.parametrize("is_f64", [True, False])
 def test_segment_aggregates_against_oracle(gpu, is_f64):
     """Straight-line programs of AGG_* words (persistent slots across three
     runs of one engine) and scalar words: counts, MIN/MAX, FIRST/LAST,

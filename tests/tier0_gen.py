@@ -182,18 +182,72 @@ VEC_SCAL = [0x90, 0x91, 0x92, 0x93, 0x94, 0x95, 0x96]      # VSADD VSSUB VSMUL V
 SCAL_REGS = (3, 6, 13)                                      # scalar operands / thresholds
 
 
-def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3), last=None):
+# Lane words (engine.hpp:1987-2006, 2124-2217, 2453-2466, 2926-2936): each
+# entry (opcode, kind); the ops tuple's fifth field carries imm12.
+LANE_OPS = [0x7A, 0x7B, 0x7C, 0x7D, 0x7E, 0x7F, 0xC7, 0x8F, 0x72]   # VINSERT VPERMUTE VSHUFFLE VREVERSE
+                                                                    # VROTATE VSLIDE VBLEND VPOW VGATHER
+SLIDE_REG = 14                                                      # VSLIDE's shift register (small integers)
+
+
+def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3), last=None, lane_words=False, gather_seg=1,
+                      store_segs=(), lane_ops=None):
     """Loop with vector arithmetic and filters on V0 = VLOAD and up to three
     reductions.  Returns (words, spec) for vo_tier0_expr_*.  `last`, when
     given, is the opcode of the final vector word, whose register the first
-    reduction reads."""
+    reduction reads.  lane_words: the body also draws the lane words of
+    LANE_OPS (or lane_ops; VGATHER reading segment gather_seg), and filters
+    / compares write random mask registers for VBLEND to read.  store_segs:
+    one or two VSTOREs at the loop offset into these segment ids after the
+    vector words (then sometimes no reduction at all)."""
     code = [w(0x10, rd=4, imm=4)]
     vload = len(code)
     code.append(w(0x70, rd=0, ra=1, imm=0))
     defined = [0]
     ops = []
+    masks = set()
     for k in range(nops):
         dst = k + 1
+        force_lane = last in LANE_OPS and k == nops - 1
+        force_other = last is not None and last not in LANE_OPS and k == nops - 1
+        if lane_words and not force_other and (force_lane or rng.random() < 0.45):
+            op = last if force_lane else int(rng.choice(lane_ops or LANE_OPS))
+            if op == 0xC7 and not masks:
+                op = 0x7D
+            a = int(rng.choice(defined))
+            b = int(rng.choice(defined))
+            sreg = int(rng.choice(SCAL_REGS))
+            if op == 0x7A:
+                lane = int(rng.integers(0, 4))
+                code.append(w(op, rd=dst, ra=a, rb=sreg, imm=lane))
+                ops.append((op, dst, a, 0, lane, sreg))
+            elif op in (0x7B, 0x8F):
+                code.append(w(op, rd=dst, ra=a, rb=b))
+                ops.append((op, dst, a, b, 0, -1))
+            elif op == 0x7C:
+                mode = int(rng.integers(0, 4))
+                code.append(w(op, rd=dst, ra=a, rb=b, imm=mode))
+                ops.append((op, dst, a, b, mode, -1))
+            elif op == 0x7D:
+                code.append(w(op, rd=dst, ra=a))
+                ops.append((op, dst, a, 0, 0, -1))
+            elif op == 0x7E:
+                sh = int(rng.integers(0, 8))
+                code.append(w(op, rd=dst, ra=a, imm=sh))
+                ops.append((op, dst, a, 0, sh, -1))
+            elif op == 0x7F:
+                code.append(w(op, rd=dst, ra=a, rb=SLIDE_REG))
+                ops.append((op, dst, a, 0, 0, SLIDE_REG))
+            elif op == 0xC7:
+                m = int(rng.choice(sorted(masks)))
+                code.append(w(op, rd=dst, ra=a, rb=b, imm=m))
+                ops.append((op, dst, a, b, m, -1))
+            else:                                                   # VGATHER: all lanes, or a missing segment
+                seg = gather_seg if rng.random() < 0.85 else 7
+                imm = (seg << 8) | int(rng.choice([0, 4, 9]))
+                code.append(w(op, rd=dst, ra=a, imm=imm))
+                ops.append((op, dst, a, 0, imm, -1))
+            defined.append(dst)
+            continue
         kind = rng.random()
         a = int(rng.choice(defined))
         b = int(rng.choice(defined))
@@ -222,10 +276,20 @@ def make_expr_program(rng, is_f64=True, nops=4, reds=(0, 1, 2, 3), last=None):
             ops.append((op, dst, a, 0, 0, sreg))
         else:
             cmp = int(rng.choice([1, 2, 3, 4, 5]))
-            code.append(w(FILTER_OPS[cmp], rd=dst, ra=a, rb=sreg))
-            ops.append((0xC0 | (cmp + 1), dst, a, 0, cmp, sreg))
+            m = int(rng.integers(0, 8)) if lane_words else 0       # the mask register it writes
+            code.append(w(FILTER_OPS[cmp], rd=dst, ra=a, rb=sreg, imm=m))
+            ops.append((0xC0 | (cmp + 1), dst, a, 0, cmp | (m << 4), sreg))
+            masks.add(m)
+        if lane_words and 0xA0 <= ops[-1][0] <= 0xA7:
+            masks.add(0)                                            # VCMPxx, VCMP(NOT)NULL imm 0: mask 0
         defined.append(dst)
-    nb = int(rng.integers(1, 4))
+    for _ in range(int(rng.integers(1, 3)) if store_segs else 0):
+        src = int(rng.choice(defined))
+        seg = int(rng.choice(store_segs))
+        imm = (seg << 8) | int(rng.choice([0, 4, 1, 2, 3]))       # (more than 4 reads past the register)
+        code.append(w(0x71, rd=src, ra=1, imm=imm))
+        ops.append((0x71, 0, src, 0, imm, -1))
+    nb = int(rng.integers(0 if store_segs else 1, 4))
     bvreg, reds_used, accops = [], [], []
     for b in range(nb):
         vreg = int(rng.choice(defined[1:] if len(defined) > 1 else defined))

@@ -86,6 +86,7 @@ struct Engine {
     DevColumn expr;                  // a reduced vector of a loop with vector arithmetic
     DevColumn wcol;                  // window values of a loop loading a window word
     DevColumn trap;                  // set by an INT64_MIN / -1 lane
+    DevColumn stores;                // the stored vectors of a loop's VSTOREs, until the run commits them
 };
 
 size_t lanes_of(int dt) { return 32 / dtype_size(dt); }
@@ -249,7 +250,13 @@ bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count,
     prog.vload = p.vload_dst;
     for (size_t k = 0; k < p.vops.size(); ++k) {
         const Plan::VOp& v = p.vops[k];
-        prog.ops[k] = ExprOp{v.op, v.dst, v.a, v.b, v.cmp, v.sreg != 0xFF ? regs[v.sreg] : 0};
+        prog.ops[k] = ExprOp{v.op, v.dst, v.a, v.b, (int16_t)v.cmp, v.imm, v.sreg != 0xFF ? regs[v.sreg] : 0};
+    }
+    for (int q = 0; q < 2; ++q) {                                     // VGATHER's segments, whole
+        if (p.gsegs[q] == 0xFF || p.gsegs[q] >= e.segs.size()) continue;
+        Seg& g = *e.segs[p.gsegs[q]];
+        prog.gcount[q] = g.count;
+        if (g.count && !(prog.gcol[q] = seg_rows(g, e.dtype, 0, g.count, c))) return false;
     }
     if (!e.trap.ensure(sizeof(int))) return false;
     if (!hip_ok(launch_zero32(static_cast<uint32_t*>(e.trap.dev), 1, c->stream), "clear")) return false;
@@ -259,11 +266,40 @@ bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count,
         if (!expr_eval(e.dtype, col, count, off, nrows, prog, e.expr.dev, static_cast<int*>(e.trap.dev), c)) return false;
         if (!run_tier0_block(b, all, e.dtype, e.expr.dev, nrows, 0, nrows, nrows / kl, regs, c)) return false;
     }
+    // the stored vectors, evaluated before anything is written
+    const size_t es = dtype_size(e.dtype);
+    if (!p.stores.empty() && !e.stores.ensure(nrows * es * p.stores.size())) return false;
+    for (size_t q = 0; q < p.stores.size(); ++q) {
+        prog.target = p.stores[q].vreg;
+        void* dst = static_cast<char*>(e.stores.dev) + q * nrows * es;
+        if (!expr_eval(e.dtype, col, count, off, nrows, prog, dst, static_cast<int*>(e.trap.dev), c)) return false;
+    }
     int trapped = 0;
     if (!sync_read(c, &trapped, e.trap.dev, sizeof(int))) return false;
     if (trapped) {
         set_error("a vector division or modulo of INT64_MIN by -1 (the reference's x86 build traps there)");
         return false;
+    }
+    // VSTOREs in program order (a later one wins where two write a row):
+    // lanes [0, count) of each group go to rows off + 4g + lane below the
+    // segment's Count, in HBM and, for a segment over caller memory, back in
+    // the caller's rows (the reference writes its borrowed Segment<T>::Data)
+    for (size_t q = 0; q < p.stores.size(); ++q) {
+        const Plan::Store& st = p.stores[q];
+        if (st.seg >= e.segs.size()) continue;                 // HandleVSTORE: no such segment, nothing written
+        Seg& sg = *e.segs[st.seg];
+        if (off >= sg.count) continue;
+        void* d = const_cast<void*>(seg_rows(sg, e.dtype, 0, sg.count, c));
+        if (!d) return false;
+        const void* src = static_cast<const char*>(e.stores.dev) + q * nrows * es;
+        if (!expr_store(e.dtype, src, nrows, st.count, d, off, sg.count, c)) return false;
+        if (!sg.devext) {
+            const uint64_t end = off + nrows < sg.count ? off + nrows : sg.count;
+            void* host = const_cast<void*>(sg.host);
+            if (!sync_read(c, static_cast<char*>(host) + off * es, static_cast<const char*>(d) + off * es,
+                           (end - off) * es))
+                return false;
+        }
     }
     regs[p.off] = total;
     if (run_scalar(p.epilogue, regs) < 0) return false;

@@ -207,11 +207,16 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
     chain_len[vA] = 0;
     p.vload_dst = vA;
     bool chain_ok = true;
+    bool mask_set[8] = {};                     // mask registers written earlier in the body
     auto defined = [&](uint8_t v) { return chain_len[v] != -1; };
     auto vec_arith = [](uint8_t op) {          // lowered vector words (engine.hpp:2221-2450, 2473-2590, 2671-2865)
-        return op == 0x78 || (op >= 0x80 && op <= 0x8E) || (op >= 0x90 && op <= 0x96) ||
+        return op == 0x78 || (op >= 0x80 && op <= 0x8F) || (op >= 0x90 && op <= 0x96) ||
                (op >= 0xA0 && op <= 0xA7) || (op >= 0xB0 && op <= 0xB7);
     };
+    // lane words (engine.hpp:1987-2006, 2124-2217, 2926-2936): they move
+    // lanes inside one kL group, so the per-row evaluation reads the group's
+    // other rows
+    auto lane_word = [](uint8_t op) { return op == 0x72 || (op >= 0x7A && op <= 0x7F) || op == 0xC7; };
     while (i < len) {
         const uint32_t w = c[i];
         const uint8_t op = op_of(w);
@@ -237,7 +242,48 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
                 chain_len[dst] = -2;
             }
             p.filts.push_back(f);
-            p.vops.push_back(Plan::VOp{op, dst, src, 0, f.thr, f.cmp});
+            p.vops.push_back(Plan::VOp{op, dst, src, 0, f.thr, f.cmp, (uint16_t)(imm_of(w) & 0x7)});
+            mask_set[imm_of(w) & 0x7] = true;                  // VFilterImpl / HandleVFILTER: Mask(imm12 & 7)
+            ++i;
+            continue;
+        }
+        if (op == 0x71) {                                                   // VSTORE
+            const uint8_t src = rd_of(w), ro = ra_of(w);
+            const uint16_t imm = imm_of(w);
+            if (!defined(src)) return none("VSTORE stores a vector the loop does not define");
+            if (ro != rOff) return none("VSTORE at an offset other than the loop's offset register is not lowered");
+            const uint8_t cnt = (imm & 0xFF) ? (uint8_t)(imm & 0xFF) : (uint8_t)4;
+            if (cnt > 4)
+                return none("VSTORE of more than kL lanes (the reference reads past the vector register into the "
+                            "next ones) is not lowered");
+            p.stores.push_back(Plan::Store{src, (uint8_t)((imm >> 8) & 0xF), cnt});
+            p.has_arith = true;
+            ++i;
+            continue;
+        }
+        if (lane_word(op)) {
+            const uint8_t dst = rd_of(w), a = ra_of(w), b = rb_of(w);
+            const uint16_t imm = imm_of(w);
+            const bool two = op == 0x7B || op == 0x7C || op == 0xC7;          // VPERMUTE VSHUFFLE VBLEND
+            if (!defined(a) || (two && !defined(b))) return none("a lane word reads a vector the loop does not define");
+            if (defined(dst)) return none("a lane word overwrites a vector the loop already defines");
+            if (p.vops.size() >= 16) return none("more than sixteen vector words in the loop");
+            if (op == 0xC7 && !mask_set[imm & 0x7])
+                return none("VBLEND reads a mask register no earlier word of the iteration writes (the value would "
+                            "come from the previous iteration)");
+            int gslot = 0;
+            if (op == 0x72) {
+                const uint8_t cnt = imm & 0xFF, sg = (imm >> 8) & 0xF;
+                if (cnt != 0 && cnt < 4)
+                    return none("VGATHER of fewer than kL lanes keeps the other lanes from the previous iteration");
+                if (p.gsegs[0] == 0xFF || p.gsegs[0] == sg) p.gsegs[0] = sg;
+                else if (p.gsegs[1] == 0xFF || p.gsegs[1] == sg) { p.gsegs[1] = sg; gslot = 1; }
+                else return none("VGATHER words of one loop read more than two segments");
+            }
+            const uint8_t sreg = (op == 0x7F || op == 0x7A) ? b : (uint8_t)0xFF;   // VSLIDE shift, VINSERT value
+            p.vops.push_back(Plan::VOp{op, dst, a, (uint8_t)(two ? b : 0), sreg, gslot, imm});
+            chain_len[dst] = -2;
+            p.has_arith = true;
             ++i;
             continue;
         }
@@ -253,7 +299,10 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
             if (defined(dst)) return none("vector arithmetic overwrites a vector the loop already defines");
             if (p.vops.size() >= 16) return none("more than sixteen vector words in the loop");
             const uint8_t sreg = scal ? b : fma ? (uint8_t)(imm_of(w) & 0xF) : splat ? a : (uint8_t)0xFF;
-            p.vops.push_back(Plan::VOp{op, dst, (uint8_t)(splat ? 0 : a), (uint8_t)(scal || unary || splat ? 0 : b), sreg, 0});
+            p.vops.push_back(Plan::VOp{op, dst, (uint8_t)(splat ? 0 : a), (uint8_t)(scal || unary || splat ? 0 : b), sreg, 0,
+                                       (uint16_t)(imm_of(w) & 0x7)});
+            if (op >= 0xA0 && op <= 0xA5) mask_set[0] = true;                 // VCMPxx: Mask(0)
+            if (op == 0xA6 || op == 0xA7) mask_set[imm_of(w) & 0x7] = true;   // VCMP(NOT)NULL: Mask(imm12 & 7)
             chain_len[dst] = -2;
             p.has_arith = true;
             ++i;
@@ -297,10 +346,13 @@ Plan lower_engine_program(const uint32_t* c, size_t len) {
         }
         p.blocks.push_back(b);
     }
-    if (p.blocks.empty()) {
+    if (p.blocks.empty() && p.stores.empty()) {
         if (i >= len) return none("loop body ends early");
         return none("loop reduction is not VSUM/VCOUNT/VRED_MIN/VRED_MAX");
     }
+    for (const Plan::Store& st : p.stores)
+        if (st.seg == p.gsegs[0] || st.seg == p.gsegs[1])
+            return none("VGATHER reads a segment the loop's VSTORE writes (later iterations would read stored rows)");
     if (!p.has_arith && !chain_ok) return none("more than two chained VFILTERs");
 
     uint8_t rStep = 0xFF;

@@ -73,10 +73,21 @@ struct Plan {
         uint8_t op, dst, a, b;       // opcode, vector registers
         uint8_t sreg;                // scalar operand / threshold register (0xFF none)
         int cmp;                     // filters: kCmp code, 7 never
+        uint16_t imm = 0;            // imm12: mask register of filters / compares, VROTATE shift,
+                                     // VSHUFFLE mode, VINSERT lane, VBLEND mask, VGATHER segment << 8 | count
     };
     bool has_arith = false;
     uint8_t vload_dst = 0;
     std::vector<VOp> vops;
+    uint8_t gsegs[2] = {0xFF, 0xFF}; // segments VGATHER words read (a VGATHER's cmp = its index here)
+    // VSTOREs of the body (engine.hpp:1971-1985), in program order: each
+    // iteration writes lanes [0, count) of `vreg` to rows offset + lane of
+    // segment `seg`; the lowering admits them at the loop's own offset only
+    // (no iteration reads rows another one wrote).
+    struct Store {
+        uint8_t vreg, seg, count;
+    };
+    std::vector<Store> stores;
     // Loops whose load is a window word (engine.hpp:2589-2640, 3167-3226):
     // 1 WINDOW_SUM, 2 WINDOW_MEAN, 3 WSTD, 4 WVARIANCE_W, 5 WDELTA (0: VLOAD).
     // The word advances the offset itself; the loop has no ADD of it.

@@ -184,21 +184,53 @@ __device__ __forceinline__ double x86_nan(double r, double x, double y, double z
 }
 __device__ __forceinline__ double x86_nan(double r, double x, double y) { return x86_nan(r, x, y, y); }
 
-// Row-wise evaluation of the body's vector words (engine.hpp:2221-2450,
-// 2473-2590, 2671-2865, 1090-1110): each lane of a vector word depends on the
-// same lane of its operands only.
+// A lane of the row's group: the value the group's thread `src` (0..3)
+// holds (the four rows of a group sit in four consecutive threads of one
+// wave, all of them active).
+template <typename T>
+__device__ __forceinline__ T quad_lane(T v, int src) {
+    const int l = (int)(threadIdx.x & 63);
+    const int from = (l & ~3) + src;
+    const uint64_t u = bits_of(v);
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, from, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), from, 64);
+    return lane_of<T>(((uint64_t)hi << 32) | lo);
+}
+
+// Engine<T>::IntPow (engine.hpp:443-454): square-and-multiply in wrapping
+// i64, 0 for a negative exponent.
+__device__ __forceinline__ int64_t int_pow(int64_t b, int64_t e) {
+    if (e < 0) return 0;
+    uint64_t r = 1, bb = (uint64_t)b;
+    while (e) {
+        if (e & 1) r *= bb;
+        bb *= bb;
+        e >>= 1;
+    }
+    return (int64_t)r;
+}
+
+// Row-wise evaluation of the body's vector words (engine.hpp:2221-2466,
+// 2473-2590, 2671-2865, 1090-1110): an arithmetic lane depends on the same
+// lane of its operands only; the lane words (1987-2006, 2124-2217,
+// 2926-2936) read lanes of the same group through quad_lane.  Each row
+// keeps its lane's bit of the eight mask registers the filters and
+// compares write and VBLEND reads (the lowering admits only masks written
+// earlier in the same iteration).
 template <typename T>
 __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, uint64_t count, uint64_t off,
                                                    uint64_t n, ExprProg prog, T* __restrict__ out,
                                                    int* __restrict__ trap) {
     constexpr bool F = sizeof(T) == 8 && (T)0.5 != (T)0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;     // a multiple of 4: rows keep their lane
+    const int lane = (int)(threadIdx.x & 3);                       // row off + i is lane i mod 4 of its group
     bool trapped = false;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         T v[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) v[r] = T(0);
         v[prog.vload] = off + i < count ? col[off + i] : T(0);
+        uint32_t msk = 0;                                          // this lane's bit of masks 0..7
         for (int k = 0; k < prog.nops; ++k) {
             const ExprOp& o = prog.ops[k];
             const T x = v[o.a], y = v[o.b];
@@ -206,6 +238,40 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
             __builtin_memcpy(&s, &o.s, sizeof(T));
             T r;
             switch (o.op) {
+            case 0x7D: r = quad_lane(x, lane ^ 3); break;                                       // VREVERSE
+            case 0x7E: r = quad_lane(x, (lane - (int)(o.imm & 3)) & 3); break;                  // VROTATE
+            case 0x7F: {                                                                        // VSLIDE
+                const uint64_t si = (uint64_t)lane + o.s;                 // size_t arithmetic (wraps)
+                const T t = quad_lane(x, (int)(si & 3));
+                r = si < 4 ? t : T(0);
+                break;
+            }
+            case 0x7C: {                                                                        // VSHUFFLE
+                const int m = o.imm & 3, h = lane & 1;
+                const int j = m == 0 ? 2 * h : m == 1 ? 2 * h + 1 : m == 2 ? h : h + 2;
+                const T ta = quad_lane(x, j), tb = quad_lane(y, j);
+                r = lane < 2 ? ta : tb;
+                break;
+            }
+            case 0x7B: {                                                                        // VPERMUTE
+                const uint64_t ib = bits_of(quad_lane(y, 0));             // index bytes: the first lane's
+                r = quad_lane(x, (int)((ib >> (8 * lane)) & 0xFF) & 3);
+                break;
+            }
+            case 0x7A: r = lane == (int)(o.imm & 3) ? s : x; break;                             // VINSERT
+            case 0xC7: r = ((msk >> (o.imm & 7)) & 1) ? y : x; break;                           // VBLEND
+            case 0x72: {                                                                        // VGATHER
+                // i32 index `lane` of the register's bytes: half (lane & 1) of lane (lane >> 1)
+                const uint64_t ib = bits_of(quad_lane(x, lane >> 1));
+                const uint64_t gi = (uint64_t)(int64_t)(int32_t)(uint32_t)(ib >> (32 * (lane & 1)));
+                const T* const gcol = static_cast<const T*>(prog.gcol[o.cmp & 1]);
+                r = (gcol && gi < prog.gcount[o.cmp & 1]) ? gcol[gi] : T(0);
+                break;
+            }
+            case 0x8F:                                                                          // VPOW
+                if constexpr (F) r = x86_nan(::pow((double)x, (double)y), x, y);   // the device libm pow
+                else r = (T)int_pow((int64_t)x, (int64_t)y);
+                break;
             case 0x80:                                                                          // VADD
                 if constexpr (F) r = x86_nan(x + y, x, y);
                 else r = (T)((uint64_t)x + (uint64_t)y);
@@ -292,11 +358,20 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
             case 0x95: r = F ? ((x < s || x != x) ? x : s) : (x < s ? x : s); break;             // VSMIN
             case 0x96: r = F ? ((x > s || s != s) ? x : s) : (x > s ? x : s); break;             // VSMAX
             case 0x78: r = s; break;                                                             // VSPLAT
-            case 0xA0: case 0xA1: case 0xA2: case 0xA3: case 0xA4: case 0xA5:                     // VCMPxx
-                r = expr_pass<T>(o.op - 0xA0, x, y) ? lane_of<T>(~0ull) : T(0);
+            case 0xA0: case 0xA1: case 0xA2: case 0xA3: case 0xA4: case 0xA5: {                   // VCMPxx: mask 0
+                const bool p = expr_pass<T>(o.op - 0xA0, x, y);
+                r = p ? lane_of<T>(~0ull) : T(0);
+                msk = (msk & ~1u) | (p ? 1u : 0u);
                 break;
-            case 0xA6: r = (F ? x != x : x == T(0)) ? lane_of<T>(~0ull) : T(0); break;            // VCMPNULL
-            case 0xA7: r = (F ? x == x : x != T(0)) ? lane_of<T>(~0ull) : T(0); break;            // VCMPNOTNULL
+            }
+            case 0xA6:                                                                          // VCMPNULL
+            case 0xA7: {                                                                        // VCMPNOTNULL
+                const bool nul = F ? x != x : x == T(0);
+                const bool p = o.op == 0xA6 ? nul : !nul;
+                r = p ? lane_of<T>(~0ull) : T(0);
+                msk = (msk & ~(1u << (o.imm & 7))) | ((p ? 1u : 0u) << (o.imm & 7));
+                break;
+            }
             case 0xB0: r = lane_of<T>(bits_of(x) & bits_of(y)); break;                            // VAND
             case 0xB1: r = lane_of<T>(bits_of(x) | bits_of(y)); break;                            // VOR
             case 0xB2: r = lane_of<T>(bits_of(x) ^ bits_of(y)); break;                            // VXOR
@@ -311,7 +386,12 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
                 if constexpr (F) r = x86_nan(__builtin_ldexp(x, neg_i32(cvtt_i32(y))), x, x);
                 else r = o.op == 0xB6 ? (T)(bits_of(x) >> (bits_of(y) & 63)) : (T)(x >> (bits_of(y) & 63));
                 break;
-            default: r = expr_pass<T>(o.cmp, x, s) ? x : T(0); break;                            // VFILTER_*
+            default: {                                                                          // VFILTER_*
+                const bool p = expr_pass<T>(o.cmp, x, s);
+                r = p ? x : T(0);
+                msk = (msk & ~(1u << (o.imm & 7))) | ((p ? 1u : 0u) << (o.imm & 7));
+                break;
+            }
             }
             v[o.dst] = r;
         }
@@ -320,7 +400,23 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
     if (trapped) *trap = 1;
 }
 
+__global__ __launch_bounds__(256) void k_expr_store(const uint64_t* __restrict__ src, uint64_t n, uint32_t count,
+                                                    uint64_t* __restrict__ dst, uint64_t off, uint64_t seg_count) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        if ((i & 3) < count && off + i < seg_count) dst[off + i] = src[i];
+}
+
 }  // namespace
+
+bool expr_store(int dtype, const void* src, uint64_t n, uint32_t count, void* dst, uint64_t off, uint64_t seg_count,
+                StreamCtx* c) {
+    (void)dtype;                                       // 8-byte lanes either way
+    if (n == 0) return true;
+    hipLaunchKernelGGL(k_expr_store, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, (const uint64_t*)src, n, count,
+                       (uint64_t*)dst, off, seg_count);
+    return hip_ok(hipGetLastError(), "store launch");
+}
 
 bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,
                int* trap, StreamCtx* c) {

@@ -45,23 +45,33 @@ bool segagg_unkey_f64(const uint64_t* k, uint64_t n, uint64_t* out, StreamCtx* c
 // first row (row order) of the earliest-sorting of them.  n < 2^30.
 bool segagg_mode(int dtype, const void* x, uint64_t n, SegaggBufs& b, uint64_t* row, uint64_t* ties, StreamCtx* c);
 
-// Vector words of a Tier-0 loop body evaluated per row (the lanes of the
-// reference's vector registers are independent for these words).
+// Vector words of a Tier-0 loop body evaluated per row; the lane words
+// (VINSERT..VSLIDE, VBLEND, VGATHER) read the other rows of the row's kL = 4
+// group, which sit in the neighbouring threads.
 struct ExprOp {
-    uint8_t op, dst, a, b;           // opcode (VADD..VMAX, VSADD..VSMAX, VFILTER_*), vector registers
-    int32_t cmp;                     // filters: kCmp code, 7 never
+    uint8_t op, dst, a, b;           // opcode, vector registers
+    int16_t cmp;                     // filters: kCmp code, 7 never
+    uint16_t imm;                    // mask register of filters / compares, lane words' imm12
     uint64_t s;                      // scalar operand / threshold bits
 };
 struct ExprProg {
     ExprOp ops[16];
     int32_t nops, vload, target, pad;
+    const void* gcol[2];             // rows VGATHER reads (device; the op's cmp picks one); null: a segment
+    uint64_t gcount[2];              // the engine does not have
 };
 
 // out[i] = register `prog.target` after the words, for rows off + i, i < n
-// (the loaded value is 0 for rows at or past count).  dtype DT_F64 / DT_I64.
+// (the loaded value is 0 for rows at or past count; n a multiple of 4, the
+// rows off + 4g .. off + 4g + 3 one group).  dtype DT_F64 / DT_I64.
 // *trap (device) is set to 1 when an i64 lane divides INT64_MIN by -1.
 bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,
                int* trap, StreamCtx* c);
+
+// VSTORE of an evaluated vector: src[i] (i < n, lane i mod 4 below count)
+// -> dst[off + i] for off + i < seg_count.  dtype DT_F64 / DT_I64.
+bool expr_store(int dtype, const void* src, uint64_t n, uint32_t count, void* dst, uint64_t off, uint64_t seg_count,
+                StreamCtx* c);
 
 // The window family's kernels on a device f64 column (vx_window.hip; the
 // caller holds the library lock): kind 0 delta, 1 sum, 2 mean (sum / w),
