@@ -563,7 +563,18 @@ def cpu_reference_paths() -> dict:
     t = med(par)
     out["run_parallel_f64"] = {"rows_per_s": round(n / t, 1), "gb_per_s": round(n * 8 / t / 1e9, 3),
                                "cores": threads, "sample": "Engine<f64>::RunParallel, 2^25 rows, x > 500"}
-    del x
+    # config 2's compound predicate the reference's operator way, one core
+    b = rng.random(n) * 1000.0
+    res = np.zeros(4)
+    t = med(lambda: o.R.vr_compound_f64(o.ptr(x), o.ptr(b), n, 292.8932188, 707.1067812, o.ptr(res)))
+    out["compound_filter_f64"] = {
+        "rows_per_s": round(n / t, 1), "gb_per_s": round(2 * n * 8 / t / 1e9, 3), "cores": 1,
+        "pct_of_8tbs": round(100 * 2 * n * 8 / t / 1e9 / HBM_PEAK_GBS, 4),
+        "selectivity_measured": round(float(res[0]) / n, 5),
+        "sample": "VectorFilter<f64>::ApplyGT(a) + ApplyLT(b), ColumnFilter::Init + AndWith x2, "
+                  "SelectionVector::BuildFromBitmap, then COUNT/SUM/MIN/MAX of a over the selected rows; "
+                  "2^25 rows per column, a > 292.8932188 AND b < 707.1067812 (50 %)"}
+    del x, b
     ns = 1 << 24
     keys0 = rng.integers(0, 2**63, ns, dtype=np.int64).astype(np.uint64) * np.uint64(2)
 

@@ -150,6 +150,7 @@ _RSIGS = {
     "vr_hash_agg_i32_f64": (SZ, [P, P, SZ, SZ, SZ, P, P, P, P, P, P, P]),
     "vr_hash_agg_u32_f64": (SZ, [P, P, SZ, SZ, SZ, P, P, P, P, P, P, P]),
     "vr_topk_f64": (None, [P, SZ, SZ, P, INT]),
+    "vr_compound_f64": (None, [P, P, SZ, F64, F64, P]),
     "vr_encode": (U32, [U8, U8, U8, U8, C.c_uint16]),
     "vr_window_f64": (I64, [INT, P, SZ, P, SZ, INT, F64]),
     "vr_nullable_f64": (None, [P, SZ, P, P, P, P]),

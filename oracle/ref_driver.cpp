@@ -274,6 +274,34 @@ size_t NAME(const K* keys, const double* vals, size_t n, size_t expected, size_t
 VR_AGG(vr_hash_agg_i32_f64, int32_t)
 VR_AGG(vr_hash_agg_u32_f64, uint32_t)
 
+// Compound predicate a > ta AND b < tb the reference's way (the path its
+// operators offer for config 2: VectorFilter<f64>::ApplyGT / ApplyLT,
+// ColumnFilter::Init + AndWith twice, SelectionVector::BuildFromBitmap), then
+// a row-order loop over the selected rows for COUNT / SUM / MIN / MAX of a.
+// out = {count, sum, min, max}.
+void vr_compound_f64(const double* a, const double* b, size_t n, double ta, double tb, double* out) {
+    ops::VectorFilter<f64> fa, fb;
+    auto ra = fa.ApplyGT(a, n, ta);
+    auto rb = fb.ApplyLT(b, n, tb);
+    ops::ColumnFilter<f64> cf;
+    cf.Init(n);
+    cf.AndWith(ra.Bitmap.data(), ra.Bitmap.size());
+    cf.AndWith(rb.Bitmap.data(), rb.Bitmap.size());
+    ops::SelectionVector<f64> sel;
+    cf.BuildSelectionVector(sel);
+    double s = 0.0, mn = INFINITY, mx = -INFINITY;
+    for (size_t i = 0; i < sel.Count; ++i) {
+        const double v = a[sel.Indices[i]];
+        s += v;
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    out[0] = (double)sel.Count;
+    out[1] = s;
+    out[2] = mn;
+    out[3] = mx;
+}
+
 // TopKSelect (voxel_topk).
 void vr_topk_f64(const double* data, size_t n, size_t k, double* out, int largest) {
     ops::TopKSelect(data, n, k, out, largest != 0);

@@ -1139,3 +1139,23 @@ def test_tier0_vstore_restatement_against_reference_engine(is_f64):
             assert int(ref[tg.PARTS[b]]) == part[b], (trial, b, spec)
         done += 1
     assert done >= 50, done
+
+
+@pytest.mark.skipif(o.R is None, reason="reference build absent")
+def test_reference_compound_path_matches_relational_restatement():
+    """The reference's compound-predicate operators (VectorFilter ApplyGT /
+    ApplyLT, ColumnFilter AndWith, SelectionVector, then a row-order loop;
+    the CPU baseline of the two-column scan) give the oracle's relational
+    count, min and max exactly and its exact sum within the row-order sum's
+    rounding."""
+    rng = np.random.default_rng(12)
+    for n in (0, 1, 63, 64, 65, 10_007):
+        a = rng.random(n) * 1000.0
+        b = rng.random(n) * 1000.0
+        out = np.zeros(4)
+        o.R.vr_compound_f64(o.ptr(a), o.ptr(b), n, 292.8932188, 707.1067812, o.ptr(out))
+        r = o.relational(a, 4, 292.8932188, b, 2, 707.1067812)
+        assert int(out[0]) == r["count"]
+        if r["count"]:
+            assert out[2] == r["min"] and out[3] == r["max"]
+            assert abs(out[1] - float(r["sum"])) <= 1e-12 * float(r["sum"])
