@@ -26,8 +26,9 @@ floor((1 - u(1 - N^(1-s)))^(1/(1-s)))), lognormal keys floor(exp(20 + 2Z)).
 value = % of the 8 TB/s HBM peak achieved by the f64 compound scan, from the
 whole timed region (K launches bracketed by device synchronisation);
 roofline.achieved = algorithmic bytes / average kernel duration from HIP
-events recorded on the launch stream; cpu_baseline = the reference's own JIT
-fused filter+sum (oracle/_ref, single core) on a bounded sample.
+events recorded on the launch stream; cpu_baseline = the reference's own
+compound-predicate operator path (oracle/_ref, single core) on a bounded
+sample, with its JIT single-column filter+sum beside it.
 """
 from __future__ import annotations
 
@@ -695,38 +696,65 @@ def bench_serve() -> dict:
     return res
 
 
-def cpu_baseline(n_rows_log2: int = 27):
-    """The reference's fastest CPU path for filter+sum: its x86-64 JIT fused
-    loop (codegen/x86_64.hpp:1632-1716), built from the reference sources into
-    oracle/_ref, on one core; 2 warmups + median of 3 (validate.cpp:20)."""
+def cpu_baseline(n_rows_log2: int = 26):
+    """The reference's CPU path for the headline workload (config 2's
+    compound predicate): its operators VectorFilter<f64>::ApplyGT / ApplyLT,
+    ColumnFilter AndWith and SelectionVector, then COUNT/SUM/MIN/MAX of a
+    over the selected rows, built from the reference sources into
+    oracle/_ref, on one core, 2^n_rows_log2 rows per column; median of 3
+    after a warm-up.  Beside it, the reference's fastest single-column path,
+    its x86-64 JIT fused filter+sum loop (codegen/x86_64.hpp:1632-1716), on
+    2^27 rows (2 warmups + median of 3, validate.cpp:20)."""
     import numpy as np
     from oracle import oracle as o
     if o.R is None:
         return None
     n = 1 << n_rows_log2
-    x = (np.random.default_rng(1).random(n) * 1000.0)
+    rng = np.random.default_rng(1)
+    a = rng.random(n) * 1000.0
+    b = rng.random(n) * 1000.0
+    res = np.zeros(4)
+    ts = []
+    for i in range(4):
+        t0 = time.perf_counter()
+        o.R.vr_compound_f64(o.ptr(a), o.ptr(b), n, 292.8932188, 707.1067812, o.ptr(res))
+        ts.append(time.perf_counter() - t0)
+    med = statistics.median(ts[1:])
+    gbs = 2 * n * 8 / med / 1e9
+    ref = o.relational(a, 4, 292.8932188, b, 2, 707.1067812)
+    out = {"value": round(100.0 * gbs / HBM_PEAK_GBS, 4), "unit": "% of 8 TB/s HBM peak",
+           "gb_per_s": round(gbs, 3), "rows_per_s": round(n / med, 1), "cores": 1, "kind": "reference",
+           "sample": "reference operators for a > 292.8932188 AND b < 707.1067812 (VectorFilter<f64> ApplyGT / "
+                     "ApplyLT, ColumnFilter Init + AndWith x2, SelectionVector::BuildFromBitmap, then COUNT/SUM/"
+                     "MIN/MAX of a over the selected rows in row order), 2^%d f64 rows per column uniform "
+                     "[0,1000) (16 B/row), median of 3 after a warm-up, one core" % n_rows_log2,
+           "host_cpu": _cpu_model(), "nproc": os.cpu_count(),
+           "result_check": {"count": int(res[0]), "count_exact": ref["count"],
+                            "sum_rel_diff": abs(res[1] - float(ref["sum"])) / max(float(ref["sum"]), 1.0)}}
+    del a, b
+    # the fastest single-column reference path, for scale
+    m = 1 << 27
+    x = (np.random.default_rng(1).random(m) * 1000.0)
     code = np.array([0x00001070, 0x000301C5, 0x000015D0, 0x0005002A, 0x00041120, 0x00021040, 0xFFA00052,
                      0x00000001], np.uint32)
     fn = o.R.vr_jit_compile(o.ptr(code), code.size, None)
-    if not fn:
-        return None
-    ts = []
-    got = 0.0
-    for i in range(5):
-        t0 = time.perf_counter()
-        got = o.R.vr_jit_entry(fn, o.ptr(x), n, n, 500.0)
-        ts.append(time.perf_counter() - t0)
-    o.R.vr_jit_release(fn)
-    med = statistics.median(ts[2:])
-    want = float(x[x > 500.0].sum())
-    gbs = n * 8 / med / 1e9
-    return {"value": round(100.0 * gbs / HBM_PEAK_GBS, 4), "unit": "% of 8 TB/s HBM peak",
-            "gb_per_s": round(gbs, 3), "rows_per_s": round(n / med, 1), "cores": 1, "kind": "reference",
-            "sample": "reference JIT fused filter+sum (oracle/_ref vr_jit_entry = voxel_jit_run loop), "
-                      "2^%d f64 rows uniform [0,1000), x > 500, single column (8 B/row), "
-                      "median of 3 after 2 warmups, one core" % n_rows_log2,
-            "host_cpu": _cpu_model(), "nproc": os.cpu_count(),
+    if fn:
+        ts = []
+        got = 0.0
+        for i in range(5):
+            t0 = time.perf_counter()
+            got = o.R.vr_jit_entry(fn, o.ptr(x), m, m, 500.0)
+            ts.append(time.perf_counter() - t0)
+        o.R.vr_jit_release(fn)
+        jm = statistics.median(ts[2:])
+        want = float(x[x > 500.0].sum())
+        out["jit_single_column"] = {
+            "pct_of_8tbs": round(100.0 * m * 8 / jm / 1e9 / HBM_PEAK_GBS, 4), "gb_per_s": round(m * 8 / jm / 1e9, 3),
+            "rows_per_s": round(m / jm, 1), "cores": 1,
+            "sample": "reference JIT fused filter+sum (oracle/_ref vr_jit_entry = voxel_jit_run loop), 2^27 f64 "
+                      "rows, x > 500, one column (8 B/row), median of 3 after 2 warmups, one core",
             "result_check": {"jit_sum": got, "numpy_sum": want, "rel_diff": abs(got - want) / abs(want)}}
+    return out
 
 
 def _traffic(rows_log2: int):
