@@ -98,17 +98,33 @@ def mark(stream_handle):
         raise RuntimeError("marker fill failed")
 
 
-def time_scan(desc, steps, warmup, stream_handle, settle_s=0.0, marks=False):
-    """settle_s: untimed launches for that long first (clocks and caches at
-    steady state), then the W warm-up launches, then K timed launches
-    (between two marker dispatches when marks is set)."""
+def time_scan(desc, steps, warmup, stream_handle, settle_s=0.0, marks=False, settle_max_s=10.0, settle_chunks=80):
+    """settle_s: untimed launches for at least that long first, and then on
+    until the rate stops improving (40-launch chunks, settle_chunks in a row
+    not 0.5 % faster than the best, at most settle_max_s in all): right after another
+    process freed a lot of device memory, the first seconds run about 5 %
+    slower (the driver clears the freed memory in the background), and the
+    timed region is meant to see the kernel at steady state.  Then the W
+    warm-up launches, then K timed launches (between two marker dispatches
+    when marks is set)."""
     import torch
     from voxel_amd import scan
-    t_end = time.perf_counter() + settle_s
-    while time.perf_counter() < t_end:
-        for _ in range(20):
-            scan.launch(desc, stream_handle)
-        torch.cuda.synchronize()
+    if settle_s > 0:
+        t_start = time.perf_counter()
+        best, stable = float("inf"), 0
+        while True:
+            t0 = time.perf_counter()
+            for _ in range(40):
+                scan.launch(desc, stream_handle)
+            torch.cuda.synchronize()
+            now = time.perf_counter()
+            dt = now - t0
+            if dt < best * 0.995:
+                best, stable = dt, 0
+            else:
+                stable += 1
+            if now - t_start >= settle_max_s or (now - t_start >= settle_s and stable >= settle_chunks):
+                break
     for _ in range(warmup):
         scan.launch(desc, stream_handle)
     if marks:
@@ -844,7 +860,7 @@ def main():
         for dt_name in ("f64", "f32", "i64", "i32"):
             for sel in (0.01, 0.50, 0.99):
                 d, e = desc_for(dt_name, sel)
-                _, kms = time_scan(d, max(5, args.steps // 2), 2, stream, settle_s=0.2)
+                _, kms = time_scan(d, max(5, args.steps // 2), 2, stream, settle_s=0.2, settle_chunks=20)
                 r = scan.result(d.dtype, stream)
                 gb = 2 * n * e / (kms * 1e-3) / 1e9
                 breakdown["%s_sel%02d" % (dt_name, int(sel * 100))] = {
