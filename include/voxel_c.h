@@ -205,6 +205,14 @@ int       voxel_gpu_engine_lowered(const void* engine);
  * means payload = row index.  Inputs are not modified.  n < 2^30. */
 int       voxel_gpu_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
                                    uint32_t* vals_out, size_t n, void* stream);
+/* out[i] = x[idx[i]] for i < k (device arrays): the rows an argsort
+ * ordered (no reference counterpart; the SQL front end's ORDER BY). */
+int       voxel_gpu_gather_f64(const double* x_dev, const uint32_t* idx_dev, size_t k, double* out_dev,
+                               void* stream);
+/* out[i] = (int64) x[i], toward zero; NaN and values outside int64 give
+ * INT64_MIN (x86-64's cvttsd2si, as the reference's casts compile).  The
+ * SQL front end's GROUP BY of an f64 key column. */
+int       voxel_gpu_f64_to_i64(const double* x_dev, size_t n, int64_t* out_dev, void* stream);
 /* Stable f64 argsort over a device column (SortOperator order). */
 int       voxel_gpu_argsort_f64(const double* x_dev, size_t n, int descending, uint32_t* idx_out_dev,
                                 void* stream);

@@ -97,3 +97,31 @@ def test_lowered_rows_order_limit_and_group_by(gpu):
     x = fare
     s = vs.execute("SELECT SUM(p) FROM t WHERE p > 10 AND p < 50", x, compat=False)
     assert close(s, x[(x > 10) & (x < 50)])
+
+
+def test_lowered_plan_on_library_buffers(gpu):
+    """Columns held in the library's own device arrays, NaN rows under NOT,
+    negative and fractional GROUP BY keys (truncated toward zero), ORDER BY
+    with LIMIT past the selected rows, an empty selection."""
+    from voxel_amd import voxel_sql as vs
+    from voxel_amd.devarray import DeviceArray
+    rng = np.random.default_rng(11)
+    n = 70_003
+    v = rng.normal(0.0, 50.0, n)
+    v[rng.integers(0, n, 300)] = np.nan
+    k = np.round(rng.uniform(-6.0, 6.0, n), 1)
+    data = {"t": {"v": DeviceArray.from_numpy(v), "k": k}}
+    got = vs.execute("SELECT v FROM t WHERE NOT (v > -10 AND v < 10)", data)
+    want = v[~((v > -10) & (v < 10))]                     # NaN rows fail both comparisons, so NOT keeps them
+    assert np.array_equal(got, want, equal_nan=True)
+    got = vs.execute("SELECT v FROM t WHERE v > 140 ORDER BY v ASC LIMIT 100000", data)
+    assert np.array_equal(got, np.sort(v[v > 140]))
+    assert vs.execute("SELECT v FROM t WHERE v > 1e9", data).size == 0
+    r = vs.execute("SELECT k, COUNT(v), MIN(v) FROM t WHERE v > 0 GROUP BY k", data)
+    sel = v > 0
+    ik = np.trunc(k[sel]).astype(np.int64)
+    keys = np.unique(ik)
+    assert np.array_equal(r["k"], keys)
+    for i, g in enumerate(keys):
+        x = v[sel][ik == g]
+        assert r["count(v)"][i] == x.size and r["min(v)"][i] == x.min()

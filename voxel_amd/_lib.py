@@ -182,6 +182,8 @@ SIGS = {
     "voxel_gpu_engine_lowered": (INT, [P]),
     "voxel_gpu_sort_pairs_u64": (INT, [P, P, P, P, SZ, P]),
     "voxel_gpu_argsort_f64": (INT, [P, SZ, INT, P, P]),
+    "voxel_gpu_gather_f64": (INT, [P, P, SZ, P, P]),
+    "voxel_gpu_f64_to_i64": (INT, [P, SZ, P, P]),
     "voxel_gpu_filter_bitmap": (INT, [INT, P, SZ, INT, U64, U64, P, P, P]),
     "voxel_gpu_filter_bitmap2": (INT, [INT, P, P, SZ, INT, U64, INT, U64, P, P, P]),
     "voxel_gpu_bitmap_combine": (INT, [P, P, SZ, INT, P, P]),

@@ -109,6 +109,17 @@ __global__ __launch_bounds__(256) void k_gather_f64(const double* __restrict__ x
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) out[i] = x[idx[i]];
 }
 
+// (int64_t) of a double as x86-64's cvttsd2si converts it: toward zero,
+// INT64_MIN for NaN and values outside int64's range.
+__global__ __launch_bounds__(256) void k_f64_to_i64(const double* __restrict__ x, uint64_t n,
+                                                    int64_t* __restrict__ out) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double v = x[i];
+        out[i] = (v != v || v >= 9223372036854775808.0 || v < -9223372036854775808.0) ? INT64_MIN : (int64_t)v;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ idx, uint64_t n) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) idx[i] = (uint32_t)i;
@@ -136,6 +147,12 @@ hipError_t launch_f64_keys(const double* x, uint64_t n, int descending, uint64_t
 hipError_t launch_gather_f64(const double* x, const uint32_t* idx, uint64_t k, double* out, hipStream_t st) {
     if (k == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_f64, dim3(grid_for(k, 16384)), dim3(256), 0, st, x, idx, k, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_f64_to_i64(const double* x, uint64_t n, int64_t* out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_f64_to_i64, dim3(grid_for(n, 16384)), dim3(256), 0, st, x, n, out);
     return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ hipError_t launch_zero32(uint32_t* p, uint64_t n, hipStream_t st);
 hipError_t launch_f64_keys(const double* x, uint64_t n, int descending, uint64_t* keys, uint32_t* idx, hipStream_t st);
 hipError_t launch_iota(uint32_t* idx, uint64_t n, hipStream_t st);
 hipError_t launch_gather_f64(const double* x, const uint32_t* idx, uint64_t k, double* out, hipStream_t st);
+hipError_t launch_f64_to_i64(const double* x, uint64_t n, int64_t* out, hipStream_t st);
 
 hipError_t launch_constant_bytes(const uint64_t* keys, uint64_t n, RadixWork& w, hipStream_t st);
 template <class Digit>

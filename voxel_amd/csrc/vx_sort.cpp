@@ -197,6 +197,24 @@ int voxel_gpu_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, u
     return sort_pairs_u64(keys_in, vals_in, keys_out, vals_out, n, c) < 0 ? -1 : 0;
 }
 
+int voxel_gpu_gather_f64(const double* x_dev, const uint32_t* idx_dev, size_t k, double* out_dev, void* stream) {
+    set_error("%s", "");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(stream);
+    if (!c) return -1;
+    if (k && (!x_dev || !idx_dev || !out_dev)) { set_error("gather: null pointer"); return -1; }
+    return hip_ok(launch_gather_f64(x_dev, idx_dev, k, out_dev, c->stream), "gather launch") ? 0 : -1;
+}
+
+int voxel_gpu_f64_to_i64(const double* x_dev, size_t n, int64_t* out_dev, void* stream) {
+    set_error("%s", "");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    StreamCtx* c = ctx_for(stream);
+    if (!c) return -1;
+    if (n && (!x_dev || !out_dev)) { set_error("f64_to_i64: null pointer"); return -1; }
+    return hip_ok(launch_f64_to_i64(x_dev, n, out_dev, c->stream), "convert launch") ? 0 : -1;
+}
+
 int voxel_gpu_argsort_f64(const double* x_dev, size_t n, int descending, uint32_t* idx_out_dev, void* stream) {
     set_error("%s", "");
     std::lock_guard<std::mutex> lk(api_mutex());

@@ -33,8 +33,17 @@ def threshold_bits(dtype: int, value) -> int:
     return int(value) & 0xFFFFFFFF
 
 
+_NP_DT = {"float64": F64, "float32": F32, "int64": I64, "int32": I32}
+
+
 def _dev(col):
-    """(pointer, rows, dtype) of a torch CUDA tensor."""
+    """(pointer, rows, dtype) of a DeviceArray or a torch CUDA tensor."""
+    from .devarray import DeviceArray
+    if isinstance(col, DeviceArray):
+        dt = _NP_DT.get(col.dtype.name)
+        if dt is None:
+            raise TypeError("column dtype %s not supported" % col.dtype)
+        return col.ptr, col.n, dt
     dt = _TORCH_DT.get(str(col.dtype))
     if dt is None:
         raise TypeError("column dtype %s not supported" % col.dtype)
@@ -76,6 +85,11 @@ def _words(valid):
     """Device pointer of a validity-word tensor (int64/uint64, NullBitmap layout) or None."""
     if valid is None:
         return None
+    from .devarray import DeviceArray
+    if isinstance(valid, DeviceArray):
+        if valid.dtype.itemsize != 8:
+            raise TypeError("validity words must be 64-bit")
+        return C.c_void_p(valid.ptr)
     if not valid.is_cuda or not valid.is_contiguous() or valid.element_size() != 8:
         raise TypeError("validity words must be a contiguous 64-bit device tensor")
     return C.c_void_p(valid.data_ptr())

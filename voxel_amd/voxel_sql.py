@@ -367,14 +367,29 @@ _CMP = {"=": 0, "<>": 1, "<": 2, "<=": 3, ">": 4, ">=": 5}
 _FLIP = {"<": ">", "<=": ">=", ">": "<", ">=": "<=", "=": "=", "<>": "<>"}
 
 
+def _device_column(v):
+    """A caller's HBM-resident f64 column (a DeviceArray, or any object with
+    data_ptr() / numel() on a contiguous f64 device buffer, such as a torch
+    CUDA tensor) wrapped in place; None for host data."""
+    from .devarray import DeviceArray
+    if isinstance(v, DeviceArray):
+        if v.dtype != np.float64:
+            raise TypeError("device columns must be f64")
+        return v
+    if hasattr(v, "data_ptr") and getattr(v, "is_cuda", False):
+        if str(getattr(v, "dtype", "")) not in ("torch.float64", "float64") or not v.is_contiguous():
+            raise TypeError("device columns must be contiguous f64")
+        return DeviceArray.borrow(v.data_ptr(), v.numel(), np.float64, owner=v)
+    return None
+
+
 class _Cols:
-    """Device copies of the table's columns, made on first use."""
+    """Device copies of the table's columns (the library's allocator), made
+    on first use; HBM-resident f64 columns are used in place."""
 
     def __init__(self, table):
-        import torch
         self.host = table if isinstance(table, dict) else {"": table}
         self.dev = {}
-        self.torch = torch
 
     def name(self, col):
         if col in self.host:
@@ -384,19 +399,19 @@ class _Cols:
         raise ValueError("unknown column %s" % col)
 
     def get(self, col):
+        from .devarray import DeviceArray
         c = self.name(col)
         if c not in self.dev:
             v = self.host[c]
-            if isinstance(v, self.torch.Tensor) and v.is_cuda:     # HBM-resident column: used in place
-                self.dev[c] = v.to(self.torch.float64).contiguous()
-            else:
-                a = np.ascontiguousarray(np.asarray(v), dtype=np.float64)
-                self.dev[c] = self.torch.from_numpy(a).cuda()
+            d = _device_column(v)
+            if d is None:
+                d = DeviceArray.from_numpy(np.ascontiguousarray(np.asarray(v), dtype=np.float64))
+            self.dev[c] = d
         return self.dev[c]
 
     def rows(self):
         v = next(iter(self.host.values()))
-        return v.numel() if isinstance(v, self.torch.Tensor) else len(v)
+        return v.numel() if hasattr(v, "numel") else len(v)
 
 
 def _comparisons(e):
@@ -440,13 +455,16 @@ def _bits(v: float) -> int:
 
 
 def _bitmap(cols, e):
-    """Selection words of condition e (device tensor of (n + 63) // 64 u64)."""
+    """Selection words of condition e (DeviceArray of (n + 63) // 64 u64)."""
     from ._lib import check, lib
-    torch = cols.torch
+    from .devarray import DeviceArray
     n = cols.rows()
     nw = max((n + 63) // 64, 1)
-    alloc = torch.empty if n else torch.zeros              # the bitmap kernel writes every word
-    w = alloc(nw, dtype=torch.int64, device="cuda")
+
+    def alloc():
+        # the bitmap kernel writes every word; with no rows the one word is zero
+        return DeviceArray(nw, np.uint64) if n else DeviceArray.zeros(nw, np.uint64)
+    w = alloc()
     cnt = C.c_uint64(0)
     kind = e[0]
     if kind in ("cmp", "between"):
@@ -456,7 +474,7 @@ def _bitmap(cols, e):
         first = True
         for col, op, v in comps:
             d = cols.get(col)
-            t = w if first else alloc(nw, dtype=torch.int64, device="cuda")
+            t = w if first else alloc()
             if lib.voxel_gpu_filter_bitmap(0, C.c_void_p(d.data_ptr()), n, _CMP[op], _bits(v), 0,
                                            C.c_void_p(t.data_ptr()), C.byref(cnt), None) != 0:
                 check("voxel_gpu_filter_bitmap")
@@ -475,9 +493,10 @@ def _bitmap(cols, e):
         return w
     if kind == "not":
         inner = _bitmap(cols, e[1])
-        ones = torch.full((nw,), -1, dtype=torch.int64, device="cuda")
-        if n % 64:
-            ones[-1] = (1 << (n % 64)) - 1
+        ones = alloc()                                     # every row: the always-true predicate's words
+        if n and lib.voxel_gpu_filter_bitmap(0, C.c_void_p(cols.get(next(iter(cols.host))).data_ptr()), n, 6, 0, 0,
+                                             C.c_void_p(ones.data_ptr()), C.byref(cnt), None) != 0:
+            check("voxel_gpu_filter_bitmap")
         if lib.voxel_gpu_bitmap_combine(C.c_void_p(ones.data_ptr()), C.c_void_p(inner.data_ptr()), n, 2,
                                         C.byref(cnt), None) != 0:
             check("voxel_gpu_bitmap_combine")
@@ -571,17 +590,19 @@ def _run_lowered(q, table, data):
 def _selected(cols, col, where):
     """Values of ``col`` in the selected rows, row order (device tensor)."""
     from . import scan as vs
+    from .devarray import DeviceArray
     a = cols.get(col)
     if where is None:
         return a
     w = _bitmap(cols, where)
-    out = cols.torch.empty_like(a)
+    out = DeviceArray(a.n, np.float64)
     m = vs.nullable_compact(a, w, out)
-    return out[:m]
+    return out.view(m)
 
 
 def _rows_lowered(cols, q):
     from ._lib import check, lib
+    from .devarray import DeviceArray
     t = q["targets"][0]
     col = next(iter(cols.host)) if t[0] == "star" else t[1]
     if t[0] == "star" and len(cols.host) > 1:
@@ -592,19 +613,24 @@ def _rows_lowered(cols, q):
         if cols.name(ocol) != cols.name(col):
             raise ValueError("ORDER BY another column than the selected one is not supported")
         n = v.numel()
-        idx = cols.torch.empty(max(n, 1), dtype=cols.torch.int32, device="cuda")
-        if n and lib.voxel_gpu_argsort_f64(C.c_void_p(v.data_ptr()), n, 0 if asc else 1, C.c_void_p(idx.data_ptr()),
+        idx = DeviceArray(n, np.uint32)
+        if n and lib.voxel_gpu_argsort_f64(C.c_void_p(v.data_ptr()), n, 0 if asc else 1, C.c_void_p(idx.ptr),
                                            None) != 0:
             check("voxel_gpu_argsort_f64")
-        v = v[idx[:n].long()]
+        k = n if q["limit"] is None else min(n, q["limit"])
+        out = DeviceArray(k, np.float64)
+        if k and lib.voxel_gpu_gather_f64(C.c_void_p(v.data_ptr()), C.c_void_p(idx.ptr), k, C.c_void_p(out.ptr),
+                                          None) != 0:
+            check("voxel_gpu_gather_f64")
+        return out.numpy()
     if q["limit"] is not None:
-        v = v[: q["limit"]]
-    return v.cpu().numpy()
+        v = v.view(min(v.numel(), q["limit"]))
+    return v.numpy()
 
 
 def _group_by_lowered(cols, q):
     from ._lib import check, lib
-    torch = cols.torch
+    from .devarray import DeviceArray
     key = q["group_by"]
     funcs = [t for t in q["targets"] if t[0] == "func"]
     if not funcs:
@@ -615,19 +641,21 @@ def _group_by_lowered(cols, q):
     vcol = val_cols.pop() if val_cols else key
     keys_f = _selected(cols, key, q["where"])
     vals = _selected(cols, vcol, q["where"])
-    k64 = keys_f.to(torch.int64)
-    n = k64.numel()
+    n = keys_f.numel()
+    k64 = DeviceArray(n, np.int64)                         # keys as integers, toward zero
+    if n and lib.voxel_gpu_f64_to_i64(C.c_void_p(keys_f.data_ptr()), n, C.c_void_p(k64.ptr), None) != 0:
+        check("voxel_gpu_f64_to_i64")
     G = max(n, 1)
-    res = [torch.zeros(G, dtype=torch.int64 if i < 2 else torch.float64, device="cuda") for i in range(6)]
+    res = [DeviceArray(G, np.int64 if i < 2 else np.float64) for i in range(6)]
     ng = C.c_uint64(0)
     if lib.voxel_gpu_groupby_f64(2, C.c_void_p(k64.data_ptr()), C.c_void_p(vals.data_ptr()), n, G,
                                  *(C.c_void_p(r.data_ptr()) for r in res), C.byref(ng), None) != 0:
         check("voxel_gpu_groupby_f64")
     g = int(ng.value)
-    out = {key: res[0][:g].cpu().numpy()}
+    out = {key: res[0].view(g).numpy()}
     names = {"count": 1, "sum": 2, "avg": 3, "min": 4, "max": 5}
     for _, fn, arg in funcs:
-        out["%s(%s)" % (fn, arg)] = res[names[fn]][:g].cpu().numpy()
+        out["%s(%s)" % (fn, arg)] = res[names[fn]].view(g).numpy()
     return out
 
 
