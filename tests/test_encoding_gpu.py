@@ -200,3 +200,57 @@ def test_encoded_columns_at_scale(gpu):
     check_scan(col.scan(2, 1000), expect(u.astype(np.int64), 2, 1000), False)
     del col
     torch.cuda.empty_cache()
+
+
+def test_bitpacked_scan_every_width(gpu):
+    """The packed scan at every bit width 1..32: row counts around whole
+    32-row groups, the words 16-byte aligned and 8 bytes past alignment, and
+    thresholds at the ends of the u32 range (empty, full and wrapped
+    complement ranges of the predicate)."""
+    import torch
+    from voxel_amd import encoding as enc
+    rng = np.random.default_rng(66)
+    for bits in range(1, 33):
+        n = int(rng.choice([31, 32, 33, 4095, 70_001]))
+        top = (1 << bits) - 1
+        u = rng.integers(0, top + 1, n, dtype=np.uint64).astype(np.uint32)
+        u[: min(n, 3)] = [top, 0, top][: min(n, 3)]
+        w = np.zeros(max(n, 1), np.uint64)
+        bw = np.zeros(1, np.uint32)
+        k = o.O.vo_bitpack_build(o.ptr(u), n, o.ptr(w), o.ptr(bw))
+        assert bw[0] == bits
+        width = bits
+        buf = torch.zeros(k + 2, dtype=torch.int64, device="cuda")
+        ui = u.astype(np.int64)
+        for shift in (0, 1):                                  # aligned words, then 8 bytes past
+            buf[shift:shift + k] = dev(w[:k].view(np.int64))
+            col = enc.bitpacked(buf[shift:shift + k], width, n)
+            cases = [(op, int(u[int(rng.integers(0, n))])) for op in range(6)]
+            cases += [(2, 0), (5, 0), (4, 0xFFFFFFFF), (3, 0xFFFFFFFF), (1, 0), (1, 0xFFFFFFFF), (0, top), (6, 0)]
+            for op, t in cases:
+                check_scan(col.scan(op, t), expect(ui, op, t), False)
+
+
+def test_delta_scan_across_workgroup_spans(gpu):
+    """Delta scans long enough that each workgroup walks several 4096-row
+    tiles, with the deltas 16-byte aligned and 8 (4) bytes past alignment."""
+    import torch
+    from voxel_amd import encoding as enc
+    rng = np.random.default_rng(67)
+    n = (1 << 25) + 12_345
+    for suf, dt, tdt in (("i64", np.int64, torch.int64), ("i32", np.int32, torch.int32)):
+        x = np.cumsum(rng.integers(-1000, 1000, n)).astype(dt)
+        dl = np.zeros(n, dt)
+        getattr(o.O, "vo_delta_build_" + suf)(o.ptr(x), n, o.ptr(dl))
+        base = int(rng.integers(-5, 5))
+        full = np.zeros(n, dt)
+        getattr(o.O, "vo_delta_decode_" + suf)(base, o.ptr(dl), n, 0, n, o.ptr(full))
+        buf = torch.zeros(n + 1, dtype=tdt, device="cuda")
+        for shift in (0, 1):
+            buf[shift:shift + n] = dev(dl)
+            col = enc.delta(buf[shift:shift + n], base)
+            for op in (1, 2, 4, 6):
+                t = full[int(rng.integers(0, n))] if op != 6 else 0
+                check_scan(col.scan(op, int(t)), expect(full, op, t), False)
+        del buf, col
+        torch.cuda.empty_cache()

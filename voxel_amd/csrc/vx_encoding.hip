@@ -16,8 +16,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
+#include <utility>
 
 #include "../../include/voxel_c.h"
 #include "vx_acc.hpp"
@@ -227,6 +230,111 @@ struct DeltaOut {                                         // out[row - off] = ba
     }
 };
 
+// ---- Delta: filter + aggregate without the decoded column -------------------
+// Workgroup b owns rows [b * span, min(n, (b + 1) * span)), span a multiple of
+// kScanChunk.  k_delta_sum sums each workgroup's deltas (wrapping in U),
+// k_parts_scan turns the sums into each workgroup's starting value, and
+// k_delta_scan walks the workgroup's rows in kScanChunk tiles from that value:
+// each thread owns 16 consecutive rows of a tile (staged through LDS so the
+// loads stay coalesced), a wave shuffle scan and the four wave totals give
+// each thread the value before its rows, and the rows enter the filter +
+// aggregate in registers.  The deltas are read twice and nothing is written
+// but one partial per workgroup.
+
+// VEC: 16-byte loads (x 16-byte aligned).
+template <typename T, typename U, bool VEC>
+__global__ __launch_bounds__(kEncThreads) void k_delta_sum(const T* __restrict__ x, uint64_t n, uint64_t span,
+                                                           U* __restrict__ part) {
+    __shared__ U red[kEncThreads / 64];
+    const uint64_t b0 = (uint64_t)blockIdx.x * span;
+    const uint64_t b1 = b0 + span < n ? b0 + span : n;
+    U s = 0;
+    uint64_t i0 = b0;
+    if constexpr (VEC) {
+        using V = typename TyInfo<T>::V;
+        constexpr int E = TyInfo<T>::E;
+        constexpr int kU = 8;                             // vector loads in flight per lane
+        const V* q = reinterpret_cast<const V*>(x);
+        const uint64_t v0 = b0 / E, v1 = b1 / E;
+        uint64_t v = v0 + threadIdx.x;
+        for (; v + (kU - 1) * kEncThreads < v1; v += kU * kEncThreads) {
+            V r[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) r[u] = ld16<V, true>(&q[v + u * kEncThreads]);
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const T* e = reinterpret_cast<const T*>(&r[u]);
+#pragma unroll
+                for (int k = 0; k < E; ++k) s += (U)e[k];
+            }
+        }
+        for (; v < v1; v += kEncThreads) {
+            const V r = ld16<V, true>(&q[v]);
+            const T* e = reinterpret_cast<const T*>(&r);
+#pragma unroll
+            for (int k = 0; k < E; ++k) s += (U)e[k];
+        }
+        i0 = v1 * E;
+    }
+    for (uint64_t i = i0 + threadIdx.x; i < b1; i += kEncThreads) s += (U)x[i];
+    for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+template <typename T, typename U>
+__global__ __launch_bounds__(kEncThreads) void k_delta_scan(const T* __restrict__ x, uint64_t n, uint64_t span,
+                                                            const U* __restrict__ start, U base, PredDev pd,
+                                                            ScanOut* __restrict__ parts) {
+    __shared__ U tile[kScanChunk + kScanChunk / 16];
+    __shared__ U wtot[kEncThreads / 64];
+    __shared__ Acc<T> red[kEncThreads / 64];
+    auto at = [](int i) { return i + (i >> 4); };         // pad: a thread's 16 items avoid one bank
+    Pred<T, false> p; p.load(pd);
+    Acc<T> acc; acc.init();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t b0 = (uint64_t)blockIdx.x * span;
+    const uint64_t b1 = b0 + span < n ? b0 + span : n;
+    U run = base + start[blockIdx.x];                     // value before the tile's first row
+    for (uint64_t t0 = b0; t0 < b1; t0 += kScanChunk) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint64_t i = t0 + (uint64_t)k * kEncThreads + threadIdx.x;
+            tile[at(k * kEncThreads + threadIdx.x)] = i < b1 ? (U)x[i] : U(0);
+        }
+        __syncthreads();
+        U v[kScanItems];
+        U s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = tile[at(threadIdx.x * kScanItems + k)];
+            s += v[k];
+        }
+        U incl = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const U y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wtot[wid] = incl;
+        __syncthreads();
+        U r = run + incl - s;
+        for (int w = 0; w < wid; ++w) r += wtot[w];
+        run += wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        const uint64_t row0 = t0 + (uint64_t)threadIdx.x * kScanItems;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            r += v[k];
+            const T val = (T)r;
+            acc.add(row0 + k < b1 && p(val), val);
+        }
+        __syncthreads();                                  // tile and wtot are rewritten by the next tile
+    }
+    block_reduce(acc, red);
+    if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
+}
+
 // ---- fused scans -------------------------------------------------------------
 
 // Codes at or past nd are read as code 0 (in bounds: nd > kDictLds puts at
@@ -315,34 +423,84 @@ __global__ __launch_bounds__(kBlock) void k_scan_for(const int32_t* __restrict__
     if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
 }
 
-// Lane l of a load step decodes row base + l: neighbouring lanes read the
-// same or adjacent words, so a wave's loads fall in a few cache lines.
-__global__ __launch_bounds__(kBlock) void k_scan_bitpack(const uint64_t* __restrict__ w, uint64_t nw, uint32_t bw,
-                                                         uint64_t n, PredDev pd, ScanOut* __restrict__ parts) {
+// BitPacked scan.  Row r's BW bits start at bit r * BW of the packed words
+// (little-endian u64 words read as a stream of u32 words, which is the same
+// bit stream), so 32 consecutive rows fill exactly BW u32 words.  Each thread
+// takes such groups of 32 rows: it loads the group's BW words (16- or 8-byte
+// vector loads when BW allows), and with BW a template constant every row's
+// word and shift are constants, so a row costs one bit-field extract or one
+// funnel shift.  The predicate is the 64-bit range predicate restricted to
+// u32 values, a wrapped range (x - a) mod 2^32 <= d (en = 0: no row passes).
+// Sums of a group of 32 rows stay in u32 while 32 * (2^BW - 1) fits; MIN/MAX
+// run on the u32 values (their order is the i64 order).  COUNT, the wrapping
+// i64 SUM, MIN and MAX do not depend on the order of the rows, so the result
+// is the same as a row-by-row scan.
+struct U32Range {
+    uint32_t a, d, en;
+};
+
+template <int BW, int VW>
+__global__ __launch_bounds__(kBlock) void k_scan_packed(const uint32_t* __restrict__ w, uint64_t groups,
+                                                        const uint64_t* __restrict__ w64, uint64_t nw, uint64_t n,
+                                                        U32Range pr, ScanOut* __restrict__ parts) {
+    using GS = typename std::conditional<(BW <= 27), uint32_t, uint64_t>::type;
+    constexpr uint32_t kMask = BW == 32 ? 0xFFFFFFFFu : ((1u << BW) - 1u);
     __shared__ Acc<int64_t> red[kBlock / 64];
-    Pred<int64_t, false> p; p.load(pd);
-    Acc<int64_t> acc; acc.init();
-    constexpr int U = 2 * kEncUnroll;
-    const uint64_t tile = (uint64_t)kBlock * U;
-    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
-        uint64_t lo[U], hi[U];
+    uint64_t sum = 0;
+    uint32_t cnt = 0, mn = 0xFFFFFFFFu, mx = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; pr.en && g < groups; g += stride) {
+        uint32_t v[BW];
+        const uint32_t* src = w + g * BW;
+        if constexpr (VW == 4) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint64_t r = base + u * kBlock + threadIdx.x;
-            const uint64_t wi = (r * bw) >> 6;
-            lo[u] = r < n ? w[wi] : 0;
-            hi[u] = r < n && wi + 1 < nw ? w[wi + 1] : 0;
+            for (int i = 0; i < BW / 4; ++i) {
+                const uint4 q = ld16<uint4, true>(reinterpret_cast<const uint4*>(src) + i);
+                v[4 * i] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
+            }
+        } else if constexpr (VW == 2) {
+#pragma unroll
+            for (int i = 0; i < BW / 2; ++i) {
+                const uint2 q = reinterpret_cast<const uint2*>(src)[i];
+                v[2 * i] = q.x; v[2 * i + 1] = q.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < BW; ++i) v[i] = src[i];
         }
+        GS gs = 0;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint64_t r = base + u * kBlock + threadIdx.x;
-            const uint32_t sh = (uint32_t)((r * bw) & 63);
-            uint64_t v = lo[u] >> sh;
-            if (sh + bw > 64) v |= hi[u] << (64 - sh);
-            const int64_t x = (int64_t)(v & ((1ull << bw) - 1));
-            acc.add(r < n && p(x), x);
+        for (int j = 0; j < 32; ++j) {
+            const int o = j * BW, q = o >> 5, sh = o & 31;
+            uint32_t x;
+            if (sh + BW <= 32) x = (v[q] >> sh) & kMask;
+            else x = __builtin_amdgcn_alignbit(v[q + 1], v[q], (uint32_t)sh) & kMask;
+            const bool p = x - pr.a <= pr.d;
+            const uint32_t sel = p ? x : 0u;
+            gs += sel;
+            cnt += p ? 1u : 0u;
+            mx = sel > mx ? sel : mx;
+            const uint32_t lo = p ? x : 0xFFFFFFFFu;
+            mn = lo < mn ? lo : mn;
+        }
+        sum += gs;
+    }
+    // rows past the last whole group (fewer than 32), row by row
+    if (pr.en && blockIdx.x == 0 && threadIdx.x == 0) {
+        for (uint64_t r = groups * 32; r < n; ++r) {
+            const uint32_t x = unpack(w64, nw, BW, r);
+            if (x - pr.a <= pr.d) {
+                sum += x; ++cnt;
+                mx = x > mx ? x : mx;
+                mn = x < mn ? x : mn;
+            }
         }
     }
+    Acc<int64_t> acc;
+    acc.init();
+    acc.s = sum;
+    acc.k = cnt;
+    if (mn <= mx) { acc.mn = (int64_t)mn; acc.mx = (int64_t)mx; }   // mn > mx: no row of this thread passed
     block_reduce(acc, red);
     if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
 }
@@ -594,6 +752,69 @@ void launch_rle_scan(const voxel_gpu_encoded_col* e, PredDev pd, ScanOut* parts,
     hipLaunchKernelGGL((k_scan_rle<T>), dim3(blocks), dim3(kBlock), 0, c->stream, static_cast<const T*>(e->values),
                        e->index, e->nvalues, pd, parts);
 }
+// The i64 range predicate (lo, hi, inv) restricted to u32 values as a wrapped
+// range: (x - a) mod 2^32 <= d.
+U32Range u32_range(const PredDev& pd) {
+    const int64_t lo = (int64_t)pd.lo, hi = (int64_t)pd.hi;
+    const int64_t L = lo > 0 ? lo : 0, H = hi < 0xFFFFFFFFll ? hi : 0xFFFFFFFFll;
+    const bool empty = L > H, full = L == 0 && H == 0xFFFFFFFFll;
+    if (!pd.inv) {
+        if (empty) return {0u, 0u, 0u};
+        return {(uint32_t)L, (uint32_t)(H - L), 1u};
+    }
+    if (empty) return {0u, 0xFFFFFFFFu, 1u};             // the complement of nothing
+    if (full) return {0u, 0u, 0u};
+    return {(uint32_t)(H + 1), (uint32_t)(L - H - 2), 1u}; // [H + 1, L - 1] around the u32 circle
+}
+
+template <int BW>
+void launch_packed_bw(const voxel_gpu_encoded_col* e, U32Range r, ScanOut* parts, unsigned blocks, StreamCtx* c) {
+    const uint64_t* w64 = static_cast<const uint64_t*>(e->values);
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(w64);
+    const uint64_t groups = e->rows / 32;
+    if constexpr (BW % 4 == 0) {
+        if ((((uintptr_t)w) & 15u) == 0) {
+            hipLaunchKernelGGL((k_scan_packed<BW, 4>), dim3(blocks), dim3(kBlock), 0, c->stream, w, groups, w64,
+                               e->nvalues, e->rows, r, parts);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_scan_packed<BW, BW % 2 == 0 ? 2 : 1>), dim3(blocks), dim3(kBlock), 0, c->stream, w, groups,
+                       w64, e->nvalues, e->rows, r, parts);
+}
+using PackedLaunch = void (*)(const voxel_gpu_encoded_col*, U32Range, ScanOut*, unsigned, StreamCtx*);
+template <int... I>
+constexpr std::array<PackedLaunch, sizeof...(I)> packed_table(std::integer_sequence<int, I...>) {
+    return {&launch_packed_bw<I + 1>...};
+}
+constexpr std::array<PackedLaunch, 32> kPackedLaunch = packed_table(std::make_integer_sequence<int, 32>{});
+
+// Delta filter + aggregate: per-workgroup partials into parts, their count
+// into *blocks.
+template <typename T, typename U>
+bool delta_scan(const voxel_gpu_encoded_col* e, U base, PredDev pd, ScanOut* parts, unsigned* blocks, StreamCtx* c) {
+    const T* x = static_cast<const T*>(e->values);
+    const uint64_t n = e->rows;
+    const uint64_t nch = n ? (n + kScanChunk - 1) / kScanChunk : 1;
+    const uint64_t per = (nch + c->scr.max_blocks - 1) / c->scr.max_blocks;
+    const uint64_t nb = (nch + per - 1) / per;
+    const uint64_t span = per * (uint64_t)kScanChunk;
+    EncBufs& b = enc_bufs();
+    if (!b.grow(&b.parts, &b.parts_bytes, nb * sizeof(U))) return false;
+    U* starts = static_cast<U*>(b.parts);
+    if ((((uintptr_t)x) & 15u) == 0)
+        hipLaunchKernelGGL((k_delta_sum<T, U, true>), dim3((unsigned)nb), dim3(kEncThreads), 0, c->stream, x, n, span,
+                           starts);
+    else
+        hipLaunchKernelGGL((k_delta_sum<T, U, false>), dim3((unsigned)nb), dim3(kEncThreads), 0, c->stream, x, n, span,
+                           starts);
+    hipLaunchKernelGGL((k_parts_scan<U>), dim3(1), dim3(kEncThreads), 0, c->stream, starts, nb);
+    hipLaunchKernelGGL((k_delta_scan<T, U>), dim3((unsigned)nb), dim3(kEncThreads), 0, c->stream, x, n, span,
+                       (const U*)starts, base, pd, parts);
+    *blocks = (unsigned)nb;
+    return hip_ok(hipGetLastError(), "delta scan launch");
+}
+
 template <typename T>
 void launch_final(ScanOut* parts, unsigned blocks, ScanOut* out, StreamCtx* c) {
     hipLaunchKernelGGL((k_scan_final<T>), dim3(1), dim3(kBlock), 0, c->stream, (const ScanOut*)parts, blocks, out);
@@ -631,18 +852,16 @@ int voxel_gpu_scan_encoded(const voxel_gpu_encoded_col* col, int op, uint64_t t,
     const PredDev pd = make_pred(adt, op, tb);
     ScanOut* parts = static_cast<ScanOut*>(c->scr.partials);
     const unsigned cap = (unsigned)std::min<uint64_t>((uint64_t)device_cu_count() * 4, c->scr.max_blocks);
-    if (col->kind == VOXEL_ENC_DELTA) {                   // decode, then the plain scan
-        const size_t es = dtype_size(col->dtype);
-        EncBufs& b = enc_bufs();
-        if (!b.grow(&b.tmp, &b.tmp_bytes, col->rows * es)) return -1;
-        if (col->rows && decode(col, 0, col->rows, b.tmp, c) < 0) return -1;
-        ScanArgs g{};
-        g.dtype = col->dtype; g.a = b.tmp; g.begin = 0; g.end = col->rows; g.mode = 0; g.pa = pd;
-        if (g.end == 0) { std::memset(&c->host->scan, 0, sizeof(ScanOut)); fill_agg(adt, c->host->scan, out); return 0; }
-        if (!hip_ok(launch_scan(g, c->scr, &c->dev->scan, c->stream), "scan launch")) return -1;
-    } else {
+    {
         unsigned blocks = 1;
         switch (col->kind) {
+        case VOXEL_ENC_DELTA: {
+            bool ok = col->dtype == VOXEL_GPU_I64
+                          ? delta_scan<int64_t, uint64_t>(col, (uint64_t)col->delta_base, pd, parts, &blocks, c)
+                          : delta_scan<int32_t, uint32_t>(col, (uint32_t)(int32_t)col->delta_base, pd, parts, &blocks, c);
+            if (!ok) return -1;
+            break;
+        }
         case VOXEL_ENC_DICTIONARY:
             if (!dict_flag(c)) return -1;
             blocks = grid_for(col->rows / 4 + 1, kBlock * kEncUnroll, cap);
@@ -659,10 +878,10 @@ int voxel_gpu_scan_encoded(const voxel_gpu_encoded_col* col, int op, uint64_t t,
             else launch_rle_scan<int32_t>(col, pd, parts, blocks, c);
             break;
         case VOXEL_ENC_BITPACKED:
-            blocks = grid_for(col->rows, kBlock * 8, cap);
-            hipLaunchKernelGGL(k_scan_bitpack, dim3(blocks), dim3(kBlock), 0, c->stream,
-                               static_cast<const uint64_t*>(col->values), col->nvalues, col->bit_width, col->rows, pd,
-                               parts);
+            // eight workgroups per CU keep enough 32-row groups in flight
+            blocks = grid_for(col->rows / 32, kBlock,
+                              (unsigned)std::min<uint64_t>((uint64_t)device_cu_count() * 8, c->scr.max_blocks));
+            kPackedLaunch[col->bit_width - 1](col, u32_range(pd), parts, blocks, c);
             break;
         default:
             blocks = grid_for(col->rows / 4 + 1, kBlock * kEncUnroll, cap);
