@@ -21,7 +21,7 @@ floor((1 - u(1 - N^(1-s)))^(1/(1-s)))), lognormal keys floor(exp(20 + 2Z)).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-breakdown]
                     [--no-sort] [--no-join] [--no-groupby] [--no-serve] [--no-window] [--no-nulls]
-                    [--no-encodings] [--no-sql]
+                    [--no-encodings] [--no-sql] [--no-expr]
 
 value = % of the 8 TB/s HBM peak achieved by the f64 compound scan, from the
 whole timed region (K launches bracketed by device synchronisation);
@@ -74,6 +74,7 @@ def parse():
     p.add_argument("--no-nulls", action="store_true")
     p.add_argument("--no-encodings", action="store_true")
     p.add_argument("--no-sql", action="store_true")
+    p.add_argument("--no-expr", action="store_true")
     p.add_argument("--sort-log2", type=int, default=29)
     p.add_argument("--settle", type=float, default=1.0,
                    help="seconds of untimed headline launches before the warm-up (clocks at steady state)")
@@ -532,6 +533,89 @@ def bench_window() -> dict:
     return out
 
 
+def _word(op, rd=0, ra=0, rb=0, imm=0):
+    """One bytecode word [op:8][rd:4][ra:4][rb:4][imm12:12]."""
+    return (op & 0xFF) | (rd & 0xF) << 8 | (ra & 0xF) << 12 | (rb & 0xF) << 16 | (imm & 0xFFF) << 20
+
+
+def expr_programs() -> dict:
+    """Tier-0 loops with vector words over segment 0 (R1 offset, R2 rows,
+    R4 = 4 lanes, R3 a scalar operand): name -> (words, reduced-register
+    accumulator)."""
+    def loop(body, acc_words):
+        n = 2 + len(body) + len(acc_words) + 3
+        return ([_word(0x10, rd=4, imm=4), _word(0x70, rd=0, ra=1)] + body + acc_words +
+                [_word(0x20, rd=1, ra=1, rb=4), _word(0x40, ra=1, rb=2), _word(0x52, imm=(1 - (n - 1)) & 0xFFF),
+                 _word(0x01)])
+    vsum = [_word(0xD0, rd=5, ra=2), _word(0x2A, rd=0, ra=0, rb=5)]
+    return {
+        # sum(x * x + x)
+        "sum_x2_plus_x": (loop([_word(0x82, rd=1, ra=0, rb=0), _word(0x80, rd=2, ra=1, rb=0)], vsum), 0),
+        # count(x * x > R3)
+        "count_x2_gt_t": (loop([_word(0x82, rd=1, ra=0, rb=0), _word(0xC5, rd=2, ra=1, rb=3)],
+                               [_word(0xD7, rd=5, ra=2), _word(0x20, rd=7, ra=7, rb=5)]), 7),
+        # sum(sqrt(|x * R3 - R3|) where x * R3 - R3 >= R3), six words
+        "sum_sqrt_abs_affine_filtered": (loop([_word(0x92, rd=1, ra=0, rb=3), _word(0x91, rd=2, ra=1, rb=3),
+                                               _word(0xC6, rd=3, ra=2, rb=3), _word(0x86, rd=6, ra=3),
+                                               _word(0x8C, rd=2 + 5, ra=6), _word(0x80, rd=2 + 6, ra=7, rb=0)],
+                                              [_word(0xD0, rd=5, ra=8), _word(0x2A, rd=0, ra=0, rb=5)]), 0),
+    }
+
+
+def bench_expr(rows_log2: int = 27) -> dict:
+    """Tier-0 loops with vector words on an HBM-resident f64 column (2^27
+    rows): compiled whole at run time (the default for loops of 2^20 rows or
+    more; compile_ms is the first run's extra time, paid once per program
+    shape and process) and on the interpreter kernels, beside the plain VSUM
+    loop.  Algorithmic bytes: 8 per row (one read of the column)."""
+    import ctypes as C
+    import torch
+    from voxel_amd._lib import check, lib
+    n = 1 << rows_log2
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    assert lib.voxel_gpu_fill_uniform(0, C.c_void_p(x.data_ptr()), n, 41, -2.0, 2.0, None) == 0
+    torch.cuda.synchronize()
+    out = {"rows": n}
+
+    def run_loop(words, acc_reg, mode, reps=7):
+        prev = lib.voxel_gpu_set_expr_jit(mode)
+        e = lib.voxel_gpu_engine_create(0)
+        try:
+            lib.voxel_gpu_engine_add_device_segment(e, C.c_void_p(x.data_ptr()), n)
+            arr = (C.c_uint32 * len(words))(*words)
+            lib.voxel_engine_load_program(e, arr, len(words))
+            ts = []
+            for _ in range(reps):
+                for r in (0, 1, 5, 7):
+                    lib.voxel_engine_set_scalar(e, r, 0)
+                lib.voxel_engine_set_scalar(e, 2, n)
+                lib.voxel_engine_set_scalar_f64(e, 3, 0.5)
+                t0 = time.perf_counter()
+                lib.voxel_engine_run(e)
+                ts.append(time.perf_counter() - t0)
+                check("run")
+            return ts[0], statistics.median(ts[1:]), lib.voxel_engine_get_scalar(e, acc_reg)
+        finally:
+            lib.voxel_engine_destroy(e)
+            lib.voxel_gpu_set_expr_jit(prev)
+
+    plain = [_word(0x10, rd=4, imm=4), _word(0x70, rd=0, ra=1), _word(0xD0, rd=5, ra=0), _word(0x2A, rd=0, ra=0, rb=5),
+             _word(0x20, rd=1, ra=1, rb=4), _word(0x40, ra=1, rb=2), _word(0x52, imm=(1 - 6) & 0xFFF), _word(0x01)]
+    _, t_plain, _ = run_loop(plain, 0, 1)
+    out["plain_vsum"] = {"ms": round(t_plain * 1e3, 4), "gb_per_s": round(n * 8 / t_plain / 1e9, 1)}
+    for name, (words, acc) in expr_programs().items():
+        first, tc, rc = run_loop(words, acc, 1)
+        _, ti, ri = run_loop(words, acc, 0, reps=4)
+        out[name] = {"compiled_ms": round(tc * 1e3, 4), "compiled_gb_per_s": round(n * 8 / tc / 1e9, 1),
+                     "frac_of_peak": round(n * 8 / tc / 1e9 / HBM_PEAK_GBS, 4),
+                     "compile_ms": round((first - tc) * 1e3, 1),
+                     "interpreted_ms": round(ti * 1e3, 4), "speedup_vs_interpreted": round(ti / tc, 2),
+                     "same_result": rc == ri if acc == 7 else None}
+    del x
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_reference_paths() -> dict:
     """The reference's other hot-path CPU implementations (built from its
     sources into oracle/_ref), each on a bounded sample of its workload on
@@ -906,6 +990,8 @@ def main():
         out["groupby"] = bench_groupby()
     if not args.no_window:
         out["window"] = bench_window()
+    if not args.no_expr:
+        out["tier0_expression"] = bench_expr()
     if not args.no_nulls:
         out["nulls"] = bench_nulls(args.rows_log2)
     if not args.no_encodings:

@@ -199,6 +199,15 @@ double    voxel_gpu_jit_run_r2(const uint32_t* code, size_t code_len, const doub
  * loop, 3 straight-line scalar and segment-aggregate (AGG_*) words, 0 not
  * lowerable (run will fail with the reason). */
 int       voxel_gpu_engine_lowered(const void* engine);
+/* Tier-0 loops with vector words whose reductions are all VSUM / VCOUNT run
+ * as one fused pass.  mode 1 (the default): a loop of at least 2^20 rows is
+ * compiled whole at run time (hipRTC, once per shape of its words, kept for
+ * the process); smaller loops, and any loop whose compile fails, run on the
+ * library's interpreter kernels.  mode 2: always compiled (a compile failure
+ * fails the run); mode 0: never.  Returns the previous mode, -1 for a bad
+ * one.  No counterpart in the reference (a tuning switch of this backend,
+ * as jit_compile picks the reference's JIT). */
+int       voxel_gpu_set_expr_jit(int mode);
 
 /* Stable sort of (u64 key, u32 payload) pairs by key (RadixSort<u64>::Sort
  * with indices, ops/sort.hpp:83-160) over device buffers; vals_in NULL

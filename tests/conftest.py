@@ -38,3 +38,12 @@ def gpu():
     from voxel_amd._lib import lib, last_error
     assert lib.voxel_gpu_available() == 1, "libvoxel_c found no usable GPU: " + last_error()
     return lib
+
+
+@pytest.fixture(params=["interpreted", "compiled"])
+def expr_mode(request, gpu):
+    """Tier-0 loops with vector words on the library's interpreter kernels, or
+    compiled whole at run time for every loop size (voxel_gpu_set_expr_jit)."""
+    prev = gpu.voxel_gpu_set_expr_jit(0 if request.param == "interpreted" else 2)
+    yield request.param
+    gpu.voxel_gpu_set_expr_jit(prev)

@@ -272,7 +272,7 @@ def test_tier0_filter_chains_and_several_reductions(gpu, is_f64):
 
 
 @pytest.mark.parametrize("is_f64", [True, False])
-def test_tier0_vector_arithmetic(gpu, is_f64):
+def test_tier0_vector_arithmetic(gpu, is_f64, expr_mode):
     """Loops with vector words (VADD..VRCP, VSADD..VSMAX, VCMPxx, VCMPNULL,
     VCMPNOTNULL, VAND..VANDN, VSHL/VSHR/VSAR) and filters on computed
     vectors, up to three reductions (generated words): partials,
@@ -353,7 +353,67 @@ def test_tier0_vector_arithmetic(gpu, is_f64):
         assert e.get_scalar(8) == 0 and e.get_scalar(1) == 0
 
 
-def test_tier0_window_word_loops(gpu):
+@pytest.mark.parametrize("is_f64", [True, False])
+def test_tier0_expression_loops_compiled_at_size(gpu, is_f64):
+    """Loops of 2^21+ rows with vector words and VSUM / VCOUNT reductions run
+    compiled whole (the default mode compiles loops of 2^20 rows or more):
+    partial registers and counts equal the oracle's bits (vo_tier0_expr_*),
+    i64 sums exact, f64 sums within 2 ulp of the exact sum of the reduced
+    register's lanes; the same shape run again with other scalar operands
+    reuses the compiled kernel and still matches."""
+    from tests import tier0_gen as tg
+    from voxel_amd import voxel_py as vx
+    assert gpu.voxel_gpu_set_expr_jit(1) in (0, 1, 2)
+    rng = np.random.default_rng(75 if is_f64 else 76)
+    for trial in range(6):
+        n = (1 << 21) + 4 * int(rng.integers(0, 1000))
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 7)), reds=(0, 1),
+                                          lane_words=trial % 3 == 2)
+        for rep in range(2):
+            regs = np.zeros(64, np.uint64)
+            if is_f64:
+                x = np.round(rng.standard_normal(n) * 100, 2)
+                for r in tg.SCAL_REGS:
+                    regs[r] = o.f64_bits(float(np.round(rng.standard_normal() * 10, 1)) or 1.0)
+                e = vx.EngineF64()
+            else:
+                x = rng.integers(-2**40, 2**40, n).astype(np.int64)
+                for r in tg.SCAL_REGS:
+                    regs[r] = int(rng.integers(1, 2**20)) & (2**64 - 1)
+                e = vx.EngineI64()
+            regs[2] = n
+            regs[tg.SLIDE_REG] = int(rng.choice([0, 1, 2, -1])) & (2**64 - 1)
+            g = x[: n // 2].copy()                            # VGATHER's segment
+            e.add_segment(x)
+            e.add_segment(g)
+            for r in (2, tg.SLIDE_REG) + tg.SCAL_REGS:
+                e.set_scalar(r, int(regs[r]))
+            e.load_program([int(c) for c in code])
+            o.O.vo_expr_reset_masks()
+            o.O.vo_expr_set_gather(1, o.ptr(g), g.size)
+            if not is_f64 and tg.expr_traps(o, x, n, spec, regs):
+                continue
+            e.run()
+            acc, part = tg.oracle_expr(o, x, n, spec, regs, is_f64)
+            assert e.get_scalar(1) == n
+            for b, red in enumerate(spec["red"]):
+                got_p, got = e.get_scalar(tg.PARTS[b]), e.get_scalar(tg.ACCS[b])
+                if is_f64 and math.isnan(o.bits_f64(part[b])):
+                    assert math.isnan(o.bits_f64(got_p))
+                else:
+                    assert got_p == part[b], (trial, rep, b, spec)
+                if red == 0 and is_f64:
+                    col = tg.oracle_expr_col(o, x, n, spec, regs, spec["bvreg"][b], is_f64)
+                    g = o.bits_f64(got)
+                    if np.all(np.isfinite(col)):
+                        assert close_f64(g, o.exact_sum(col)), (trial, rep, b, g)
+                    else:
+                        assert math.isnan(g) == math.isnan(o.bits_f64(acc[b]))
+                else:
+                    assert got == acc[b], (trial, rep, b, spec)
+
+
+def test_tier0_window_word_loops(gpu, expr_mode):
     """Loops loading WINDOW_SUM / WINDOW_MEAN / WSTD / WVARIANCE_W / WDELTA /
     WQUANTILE (the word advancing the offset), vector words and up to three
     reductions: partials, counts, min and max equal the oracle's bits
@@ -594,7 +654,7 @@ def test_tier0_vstore(gpu, is_f64):
 
 
 @pytest.mark.parametrize("is_f64", [True, False])
-def test_tier0_lane_words(gpu, is_f64):
+def test_tier0_lane_words(gpu, is_f64, expr_mode):
     """Loops with the lane words (VINSERT, VPERMUTE, VSHUFFLE, VREVERSE,
     VROTATE, VSLIDE by a register, VBLEND on masks that filters and compares
     of the same iteration wrote, VPOW, VGATHER from a second segment or a

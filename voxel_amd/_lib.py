@@ -180,6 +180,7 @@ SIGS = {
     "voxel_gpu_jit_lowerable": (INT, [P, SZ]),
     "voxel_gpu_jit_run_r2": (F64, [P, SZ, P, SZ, U64, F64]),
     "voxel_gpu_engine_lowered": (INT, [P]),
+    "voxel_gpu_set_expr_jit": (INT, [INT]),
     "voxel_gpu_sort_pairs_u64": (INT, [P, P, P, P, SZ, P]),
     "voxel_gpu_argsort_f64": (INT, [P, SZ, INT, P, P]),
     "voxel_gpu_gather_f64": (INT, [P, P, SZ, P, P]),

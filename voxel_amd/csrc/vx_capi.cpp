@@ -87,6 +87,7 @@ struct Engine {
     DevColumn wcol;                  // window values of a loop loading a window word
     DevColumn trap;                  // set by an INT64_MIN / -1 lane
     DevColumn stores;                // the stored vectors of a loop's VSTOREs, until the run commits them
+    DevColumn fused;                 // fused expression reductions: 4 results, 4 last-group values, trap flag
 };
 
 size_t lanes_of(int dt) { return 32 / dtype_size(dt); }
@@ -238,10 +239,92 @@ bool run_tier0_block(const Plan::Block& b, const PredDev& pred, int dtype, const
 // The body's vector words over rows [off, off + nrows) of column col (rows at
 // or past count load 0), one evaluation per reduced register, each reduced
 // as an unfiltered loop would; then the loop's register effects.
+// The accumulate word of a VSUM / VCOUNT block must be one the lowering maps
+// (ADDF for f64 sums, ADD for i64 sums and counts).
+bool sum_count_pair_ok(const Plan::Block& b, int dtype) {
+    const bool f = dtype == DT_F64;
+    return (b.red == 0 && b.accop == 0 && f) || (b.red == 0 && b.accop == 1 && !f) || (b.red == 1 && b.accop == 1);
+}
+
+// Loops whose reductions are all VSUM / VCOUNT and that store nothing: the
+// words and the reductions run in one pass (expr_reduce), with no evaluated
+// column in HBM; then the loop's register effects.
+bool run_expr_fused(Engine& e, uint64_t* regs, const ExprProg& prog, const void* col, uint64_t count, uint64_t off,
+                    uint64_t nrows, uint64_t total, StreamCtx* c) {
+    const Plan& p = e.plan;
+    ExprRed rd{};
+    rd.n = (int32_t)p.blocks.size();
+    for (size_t q = 0; q < p.blocks.size(); ++q) {
+        const Plan::Block& b = p.blocks[q];
+        if (!sum_count_pair_ok(b, e.dtype)) {
+            set_error("loop (reduction %u, accumulate %u) on this engine type is not lowered", b.red, b.accop);
+            return false;
+        }
+        rd.vreg[q] = b.vreg;
+        rd.red[q] = b.red;
+    }
+    // layout: ScanOut res[4] | u64 last[4] | int trap
+    const size_t last_at = 4 * sizeof(ScanOut), trap_at = last_at + 32, bytes = trap_at + 8;
+    if (!e.fused.ensure(bytes)) return false;
+    char* fb = static_cast<char*>(e.fused.dev);
+    if (!hip_ok(launch_zero32(reinterpret_cast<uint32_t*>(fb + last_at), (32 + 8) / 4, c->stream), "clear")) return false;
+    ScanOut* res = reinterpret_cast<ScanOut*>(fb);
+    if (!expr_reduce(e.dtype, col, count, off, nrows, prog, rd, res, reinterpret_cast<uint64_t*>(fb + last_at),
+                     reinterpret_cast<int*>(fb + trap_at), c))
+        return false;
+    struct {
+        ScanOut res[4];
+        uint64_t last[4];
+        int trap, pad;
+    } h{};
+    static_assert(sizeof(h) == 4 * sizeof(ScanOut) + 40, "fused result layout");
+    if (!sync_read(c, &h, fb, bytes)) return false;
+    if (h.trap) {
+        set_error("a vector division or modulo of INT64_MIN by -1 (the reference's x86 build traps there)");
+        return false;
+    }
+    for (size_t q = 0; q < p.blocks.size(); ++q) {
+        const Plan::Block& b = p.blocks[q];
+        const ScanOut& r = h.res[q];
+        uint64_t accb = regs[b.acc];
+        if (b.red == 0 && e.dtype == DT_F64) {
+            accb = f64_bits(finish_sum_f64(bits_f64(accb), bits_f64(r.s), bits_f64(r.c)));
+        } else if (b.red == 0) {
+            accb = accb + r.s;
+        } else {
+            accb = accb + r.cnt;
+        }
+        regs[b.acc] = accb;
+        regs[b.part] = h.last[q];
+    }
+    regs[p.off] = total;
+    if (run_scalar(p.epilogue, regs) < 0) return false;
+    std::memcpy(e.regs, regs, sizeof(uint64_t) * 64);
+    return true;
+}
+
 bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count, uint64_t off, uint64_t nrows,
                      uint64_t total, StreamCtx* c) {
     const Plan& p = e.plan;
     const uint64_t kl = 4;
+    bool fusable = p.stores.empty() && !p.blocks.empty() && p.blocks.size() <= 4;
+    for (const Plan::Block& b : p.blocks) fusable = fusable && b.red <= 1;
+    if (fusable) {
+        ExprProg prog{};
+        prog.nops = (int32_t)p.vops.size();
+        prog.vload = p.vload_dst;
+        for (size_t k = 0; k < p.vops.size(); ++k) {
+            const Plan::VOp& v = p.vops[k];
+            prog.ops[k] = ExprOp{v.op, v.dst, v.a, v.b, (int16_t)v.cmp, v.imm, v.sreg != 0xFF ? regs[v.sreg] : 0};
+        }
+        for (int q = 0; q < 2; ++q) {                                 // VGATHER's segments, whole
+            if (p.gsegs[q] == 0xFF || p.gsegs[q] >= e.segs.size()) continue;
+            Seg& g = *e.segs[p.gsegs[q]];
+            prog.gcount[q] = g.count;
+            if (g.count && !(prog.gcol[q] = seg_rows(g, e.dtype, 0, g.count, c))) return false;
+        }
+        return run_expr_fused(e, regs, prog, col, count, off, nrows, total, c);
+    }
     if (nrows > (1ull << 31)) { set_error("a loop with vector arithmetic over %llu rows (more than 2^31) is not lowered",
                                           (unsigned long long)nrows); return false; }
     if (!e.expr.ensure(nrows * 8)) return false;

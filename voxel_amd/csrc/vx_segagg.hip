@@ -9,9 +9,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 
+#include "vx_acc.hpp"
 #include "vx_compact.hpp"
+#include "vx_expr_ops.hpp"
 #include "vx_kernels.hpp"
 #include "vx_radix.hpp"
 #include "vx_runtime.hpp"
@@ -24,6 +27,7 @@ namespace vx {
 int sort_pairs_u64(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n, StreamCtx* c);
 
 namespace {
+using namespace xo;
 
 constexpr int kMomThreads = 256;
 constexpr unsigned kMomBlocksMax = 1024;
@@ -137,267 +141,96 @@ unsigned gridn(uint64_t n, unsigned cap) {
     return (unsigned)(b ? (b < cap ? b : cap) : 1);
 }
 
+// expr_row runs the words on one row (registers v): each word dispatches to
+// apply_op.  v[prog.vload] holds the loaded value on entry; every register
+// the words read was written earlier in the same iteration (the lowering
+// checks it), so nothing else needs a start value.
 template <typename T>
-__device__ __forceinline__ bool expr_pass(int cmp, T x, T t) {
-    switch (cmp) {                            // VFilterImpl predicates (engine.hpp:2896-2924)
-    case 0: return x == t;
-    case 1: return x != t;
-    case 2: return x < t;
-    case 3: return x <= t;
-    case 4: return x > t;
-    case 5: return x >= t;
-    default: return false;
+__device__ __forceinline__ void expr_row(const ExprProg& prog, T (&v)[16], int lane, bool& trapped) {
+    uint32_t msk = 0;                                              // this lane's bit of masks 0..7
+    for (int k = 0; k < prog.nops; ++k) {
+        const ExprOp& o = prog.ops[k];
+        T s;
+        __builtin_memcpy(&s, &o.s, sizeof(T));
+        switch (o.op) {
+#define VX_EXPR_OP(c) \
+    case c: v[o.dst] = apply_op<T, c>(o, prog, v[o.a], v[o.b], s, lane, msk, trapped); break;
+            VX_EXPR_CASES(VX_EXPR_OP)
+#undef VX_EXPR_OP
+        default: v[o.dst] = apply_op<T, 0xC0>(o, prog, v[o.a], v[o.b], s, lane, msk, trapped); break;   // VFILTER_*
+        }
     }
 }
 
-// A lane's 64 bits, and a lane from 64 bits (the logical words act on the
-// bit pattern of either lane type).
-template <typename T>
-__device__ __forceinline__ uint64_t bits_of(T x) {
-    uint64_t u;
-    __builtin_memcpy(&u, &x, 8);
-    return u;
-}
-template <typename T>
-__device__ __forceinline__ T lane_of(uint64_t u) {
-    T x;
-    __builtin_memcpy(&x, &u, 8);
-    return x;
-}
-// (int) of a double as the reference's x86-64 build converts it (cvttsd2si):
-// NaN and values outside int32's range give INT32_MIN; the f64 shift words
-// scale by 2^(int)b and 2^-(int)b, and negating INT32_MIN keeps INT32_MIN.
-__device__ __forceinline__ int cvtt_i32(double b) {
-    return (b != b || b >= 2147483648.0 || b <= -2147483649.0) ? INT32_MIN : (int)b;
-}
-__device__ __forceinline__ int neg_i32(int k) { return k == INT32_MIN ? INT32_MIN : -k; }
-
-// The NaN an f64 operation of the reference's x86-64 build leaves in a lane:
-// the first NaN operand, quieted, or the default NaN (sign bit set) when the
-// operation itself is invalid (inf - inf, 0 * inf, fmod(x, 0), sqrt(-1)).
-// The GPU's default NaN has the sign bit clear, and the lane's bits reach
-// the CMP-select accumulators, which order registers as signed integers.
-__device__ __forceinline__ double x86_nan(double r, double x, double y, double z) {
-    if (r == r) return r;
-    const uint64_t b = x != x ? bits_of(x) : y != y ? bits_of(y) : z != z ? bits_of(z) : 0xFFF8000000000000ull;
-    return lane_of<double>(b | 0x0008000000000000ull);
-}
-__device__ __forceinline__ double x86_nan(double r, double x, double y) { return x86_nan(r, x, y, y); }
-
-// A lane of the row's group: the value the group's thread `src` (0..3)
-// holds (the four rows of a group sit in four consecutive threads of one
-// wave, all of them active).
-template <typename T>
-__device__ __forceinline__ T quad_lane(T v, int src) {
-    const int l = (int)(threadIdx.x & 63);
-    const int from = (l & ~3) + src;
-    const uint64_t u = bits_of(v);
-    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, from, 64);
-    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), from, 64);
-    return lane_of<T>(((uint64_t)hi << 32) | lo);
-}
-
-// Engine<T>::IntPow (engine.hpp:443-454): square-and-multiply in wrapping
-// i64, 0 for a negative exponent.
-__device__ __forceinline__ int64_t int_pow(int64_t b, int64_t e) {
-    if (e < 0) return 0;
-    uint64_t r = 1, bb = (uint64_t)b;
-    while (e) {
-        if (e & 1) r *= bb;
-        bb *= bb;
-        e >>= 1;
-    }
-    return (int64_t)r;
-}
-
-// Row-wise evaluation of the body's vector words (engine.hpp:2221-2466,
-// 2473-2590, 2671-2865, 1090-1110): an arithmetic lane depends on the same
-// lane of its operands only; the lane words (1987-2006, 2124-2217,
-// 2926-2936) read lanes of the same group through quad_lane.  Each row
-// keeps its lane's bit of the eight mask registers the filters and
-// compares write and VBLEND reads (the lowering admits only masks written
-// earlier in the same iteration).
 template <typename T>
 __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, uint64_t count, uint64_t off,
                                                    uint64_t n, ExprProg prog, T* __restrict__ out,
                                                    int* __restrict__ trap) {
-    constexpr bool F = sizeof(T) == 8 && (T)0.5 != (T)0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;     // a multiple of 4: rows keep their lane
     const int lane = (int)(threadIdx.x & 3);                       // row off + i is lane i mod 4 of its group
     bool trapped = false;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         T v[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = T(0);
         v[prog.vload] = off + i < count ? col[off + i] : T(0);
-        uint32_t msk = 0;                                          // this lane's bit of masks 0..7
-        for (int k = 0; k < prog.nops; ++k) {
-            const ExprOp& o = prog.ops[k];
-            const T x = v[o.a], y = v[o.b];
-            T s;
-            __builtin_memcpy(&s, &o.s, sizeof(T));
-            T r;
-            switch (o.op) {
-            case 0x7D: r = quad_lane(x, lane ^ 3); break;                                       // VREVERSE
-            case 0x7E: r = quad_lane(x, (lane - (int)(o.imm & 3)) & 3); break;                  // VROTATE
-            case 0x7F: {                                                                        // VSLIDE
-                const uint64_t si = (uint64_t)lane + o.s;                 // size_t arithmetic (wraps)
-                const T t = quad_lane(x, (int)(si & 3));
-                r = si < 4 ? t : T(0);
-                break;
-            }
-            case 0x7C: {                                                                        // VSHUFFLE
-                const int m = o.imm & 3, h = lane & 1;
-                const int j = m == 0 ? 2 * h : m == 1 ? 2 * h + 1 : m == 2 ? h : h + 2;
-                const T ta = quad_lane(x, j), tb = quad_lane(y, j);
-                r = lane < 2 ? ta : tb;
-                break;
-            }
-            case 0x7B: {                                                                        // VPERMUTE
-                const uint64_t ib = bits_of(quad_lane(y, 0));             // index bytes: the first lane's
-                r = quad_lane(x, (int)((ib >> (8 * lane)) & 0xFF) & 3);
-                break;
-            }
-            case 0x7A: r = lane == (int)(o.imm & 3) ? s : x; break;                             // VINSERT
-            case 0xC7: r = ((msk >> (o.imm & 7)) & 1) ? y : x; break;                           // VBLEND
-            case 0x72: {                                                                        // VGATHER
-                // i32 index `lane` of the register's bytes: half (lane & 1) of lane (lane >> 1)
-                const uint64_t ib = bits_of(quad_lane(x, lane >> 1));
-                const uint64_t gi = (uint64_t)(int64_t)(int32_t)(uint32_t)(ib >> (32 * (lane & 1)));
-                const T* const gcol = static_cast<const T*>(prog.gcol[o.cmp & 1]);
-                r = (gcol && gi < prog.gcount[o.cmp & 1]) ? gcol[gi] : T(0);
-                break;
-            }
-            case 0x8F:                                                                          // VPOW
-                if constexpr (F) r = x86_nan(::pow((double)x, (double)y), x, y);   // the device libm pow
-                else r = (T)int_pow((int64_t)x, (int64_t)y);
-                break;
-            case 0x80:                                                                          // VADD
-                if constexpr (F) r = x86_nan(x + y, x, y);
-                else r = (T)((uint64_t)x + (uint64_t)y);
-                break;
-            case 0x81:                                                                          // VSUB
-                if constexpr (F) r = x86_nan(x - y, x, y);
-                else r = (T)((uint64_t)x - (uint64_t)y);
-                break;
-            case 0x82:                                                                          // VMUL
-                if constexpr (F) r = x86_nan(x * y, x, y);
-                else r = (T)((uint64_t)x * (uint64_t)y);
-                break;
-            case 0x83:                                                                          // VDIV
-                if constexpr (F) r = y == T(0) ? (T)__builtin_nan("") : x86_nan(x / y, x, y);
-                else {
-                    trapped |= y == T(-1) && x == (T)INT64_MIN;                                   // x86 idiv traps
-                    r = y == T(0) ? (T)INT64_MAX : y == T(-1) ? (T)(0 - (uint64_t)x) : x / y;
-                }
-                break;
-            case 0x84:                                                                          // VMOD
-                if constexpr (F) r = x86_nan(__builtin_fmod(x, y), x, y);
-                else {
-                    trapped |= y == T(-1) && x == (T)INT64_MIN;
-                    r = (y == T(0) || y == T(-1)) ? T(0) : x % y;
-                }
-                break;
-            case 0x85: r = lane_of<T>(F ? bits_of(x) ^ (1ull << 63) : 0 - bits_of(x)); break;   // VNEG
-            case 0x89:                                                                          // VAVG
-                if constexpr (F) r = x86_nan((x + y) * T(0.5), x, y);
-                else r = (x >> 1) + (y >> 1) + ((x & 1) + (y & 1)) / 2;
-                break;
-            case 0x8A:                                                                          // VFMA, c = s
-                if constexpr (F) r = x86_nan(__builtin_fma(x, y, s), x, y, s);
-                else r = (T)((uint64_t)x * (uint64_t)y + (uint64_t)s);
-                break;
-            case 0x8B:                                                                          // VFMS
-                if constexpr (F) r = x86_nan(x86_nan(x * y, x, y) - s, x86_nan(x * y, x, y), s);
-                else r = (T)((uint64_t)x * (uint64_t)y - (uint64_t)s);
-                break;
-            case 0x8C:                                                                          // VSQRT
-                if constexpr (F) r = x86_nan(__builtin_sqrt(x), x, x);
-                else r = (double)x < 0 ? T(0) : (T)__builtin_sqrt((double)x);
-                break;
-            case 0x8D:                                                                          // VRSQRT
-                if constexpr (F) r = x86_nan(T(1) / __builtin_sqrt(x), x, x);
-                else r = (double)x <= 0 ? T(0) : (T)(1.0 / __builtin_sqrt((double)x));
-                break;
-            case 0x8E:                                                                          // VRCP
-                if constexpr (F) r = x86_nan(T(1) / x, x, x);
-                else r = x == T(0) ? T(0) : (T)(1.0 / (double)x);
-                break;
-            case 0x86:                                                                          // VABS
-                if constexpr (F) r = lane_of<T>(bits_of(x) & ~(1ull << 63));
-                else r = x < T(0) ? (T)(0 - (uint64_t)x) : x;
-                break;
-            case 0x87: r = F ? ((x < y || x != x) ? x : y) : (x < y ? x : y); break;             // VMIN
-            case 0x88: r = F ? ((x > y || y != y) ? x : y) : (x > y ? x : y); break;             // VMAX
-            case 0x90:                                                                          // VSADD
-                if constexpr (F) r = x86_nan(x + s, x, s);
-                else r = (T)((uint64_t)x + (uint64_t)s);
-                break;
-            case 0x91:                                                                          // VSSUB
-                if constexpr (F) r = x86_nan(x - s, x, s);
-                else r = (T)((uint64_t)x - (uint64_t)s);
-                break;
-            case 0x92:                                                                          // VSMUL
-                if constexpr (F) r = x86_nan(x * s, x, s);
-                else r = (T)((uint64_t)x * (uint64_t)s);
-                break;
-            case 0x93:                                                                          // VSDIV
-                if constexpr (F) r = s == T(0) ? (T)__builtin_nan("") : x86_nan(x / s, x, s);
-                else {
-                    trapped |= s == T(-1) && x == (T)INT64_MIN;
-                    r = s == T(0) ? T(0) : s == T(-1) ? (T)(0 - (uint64_t)x) : x / s;
-                }
-                break;
-            case 0x94:                                                                          // VSMOD
-                if constexpr (F) r = x86_nan(__builtin_fmod(x, s), x, s);
-                else {
-                    trapped |= s == T(-1) && x == (T)INT64_MIN;
-                    r = (s == T(0) || s == T(-1)) ? T(0) : x % s;
-                }
-                break;
-            case 0x95: r = F ? ((x < s || x != x) ? x : s) : (x < s ? x : s); break;             // VSMIN
-            case 0x96: r = F ? ((x > s || s != s) ? x : s) : (x > s ? x : s); break;             // VSMAX
-            case 0x78: r = s; break;                                                             // VSPLAT
-            case 0xA0: case 0xA1: case 0xA2: case 0xA3: case 0xA4: case 0xA5: {                   // VCMPxx: mask 0
-                const bool p = expr_pass<T>(o.op - 0xA0, x, y);
-                r = p ? lane_of<T>(~0ull) : T(0);
-                msk = (msk & ~1u) | (p ? 1u : 0u);
-                break;
-            }
-            case 0xA6:                                                                          // VCMPNULL
-            case 0xA7: {                                                                        // VCMPNOTNULL
-                const bool nul = F ? x != x : x == T(0);
-                const bool p = o.op == 0xA6 ? nul : !nul;
-                r = p ? lane_of<T>(~0ull) : T(0);
-                msk = (msk & ~(1u << (o.imm & 7))) | ((p ? 1u : 0u) << (o.imm & 7));
-                break;
-            }
-            case 0xB0: r = lane_of<T>(bits_of(x) & bits_of(y)); break;                            // VAND
-            case 0xB1: r = lane_of<T>(bits_of(x) | bits_of(y)); break;                            // VOR
-            case 0xB2: r = lane_of<T>(bits_of(x) ^ bits_of(y)); break;                            // VXOR
-            case 0xB3: r = lane_of<T>(~bits_of(x)); break;                                        // VNOT
-            case 0xB4: r = lane_of<T>(~bits_of(x) & bits_of(y)); break;                           // VANDN
-            case 0xB5:                                                                          // VSHL
-                if constexpr (F) r = x86_nan(__builtin_ldexp(x, cvtt_i32(y)), x, x);
-                else r = (T)(bits_of(x) << (bits_of(y) & 63));
-                break;
-            case 0xB6:                                                                          // VSHR
-            case 0xB7:                                                                          // VSAR
-                if constexpr (F) r = x86_nan(__builtin_ldexp(x, neg_i32(cvtt_i32(y))), x, x);
-                else r = o.op == 0xB6 ? (T)(bits_of(x) >> (bits_of(y) & 63)) : (T)(x >> (bits_of(y) & 63));
-                break;
-            default: {                                                                          // VFILTER_*
-                const bool p = expr_pass<T>(o.cmp, x, s);
-                r = p ? x : T(0);
-                msk = (msk & ~(1u << (o.imm & 7))) | ((p ? 1u : 0u) << (o.imm & 7));
-                break;
-            }
-            }
-            v[o.dst] = r;
-        }
+        expr_row<T>(prog, v, lane, trapped);
         out[i] = v[prog.target];
     }
     if (trapped) *trap = 1;
+}
+
+// The words and up to four VSUM / VCOUNT reductions in one pass: no
+// evaluated column is written.  A workgroup takes tiles of 256 * kExprPf rows
+// (row base + j * 256 + t), issuing the tile's loads before evaluating any
+// of its rows.  Reduction q keeps a compensated f64 pair or a wrapping i64 sum
+// (VSUM) or a count of lanes that are non-zero and not NaN (VCOUNT) per
+// thread; the workgroup's totals go to parts[q * gridDim.x + blockIdx.x] as a
+// scan partial, merged afterwards by k_scan_final.  The last group of four
+// rows is reduced as the reference reduces one group into last[q], the
+// loop's partial register.
+constexpr int kExprPf = 4;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_expr_reduce(const T* __restrict__ col, uint64_t count, uint64_t off,
+                                                     uint64_t n, ExprProg prog, ExprRed rd,
+                                                     uint64_t* __restrict__ parts, uint64_t* __restrict__ last,
+                                                     int* __restrict__ trap) {
+    __shared__ XAcc<T> lds[256 / 64];
+    XAcc<T> acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xacc_init(acc[q]);
+    const int lane = (int)(threadIdx.x & 3);
+    bool trapped = false;
+    const uint64_t tile = 256ull * kExprPf;
+    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+        T x[kExprPf];
+#pragma unroll
+        for (int j = 0; j < kExprPf; ++j) {
+            const uint64_t i = base + (uint64_t)j * 256 + threadIdx.x;
+            x[j] = i < n && off + i < count ? col[off + i] : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < kExprPf; ++j) {
+            const uint64_t i = base + (uint64_t)j * 256 + threadIdx.x;
+            if (i >= n) break;                                     // n is whole groups: quads leave together
+            T v[16];
+            v[prog.vload] = x[j];
+            expr_row<T>(prog, v, lane, trapped);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q >= rd.n) break;
+                const T y = v[rd.vreg[q]];
+                if (rd.red[q] == 0) xacc_sum(acc[q], y);
+                else xacc_count(acc[q], y);
+                if (i + 4 >= n) group_partial(y, rd.red[q], lane, &last[q]);   // the last group
+            }
+        }
+    }
+    if (trapped) *trap = 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q >= rd.n) break;
+        xacc_block_store(acc[q], lds, parts + 8 * ((uint64_t)q * gridDim.x + blockIdx.x));
+    }
 }
 
 __global__ __launch_bounds__(256) void k_expr_store(const uint64_t* __restrict__ src, uint64_t n, uint32_t count,
@@ -416,6 +249,39 @@ bool expr_store(int dtype, const void* src, uint64_t n, uint32_t count, void* ds
     hipLaunchKernelGGL(k_expr_store, dim3(gridn(n, 16384)), dim3(256), 0, c->stream, (const uint64_t*)src, n, count,
                        (uint64_t*)dst, off, seg_count);
     return hip_ok(hipGetLastError(), "store launch");
+}
+
+bool expr_reduce(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog,
+                 const ExprRed& rd, ScanOut* res, uint64_t* last, int* trap, StreamCtx* c) {
+    const uint64_t tile = 256ull * kExprPf;
+    uint64_t nb = (n + tile - 1) / tile;
+    // eight workgroups per CU, as many as the partials buffer holds for rd.n reductions
+    const uint64_t cap = std::min<uint64_t>((uint64_t)device_cu_count() * 8,
+                                            c->scr.max_blocks * 64 / ((uint64_t)(rd.n > 0 ? rd.n : 1) * sizeof(ScanOut)));
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    ScanOut* parts = static_cast<ScanOut*>(c->scr.partials);
+    static_assert(sizeof(ScanOut) == 64, "a scan partial is eight words");
+    bool compiled = false;
+    if (!jit_expr_launch(dtype == DT_F64, col, count, off, n, prog, rd, reinterpret_cast<uint64_t*>(parts),
+                         (unsigned)nb, last, trap, c->stream, &compiled))
+        return false;
+    if (compiled) {
+    } else if (dtype == DT_F64)
+        hipLaunchKernelGGL(k_expr_reduce<double>, dim3((unsigned)nb), dim3(256), 0, c->stream, (const double*)col, count,
+                           off, n, prog, rd, reinterpret_cast<uint64_t*>(parts), last, trap);
+    else
+        hipLaunchKernelGGL(k_expr_reduce<int64_t>, dim3((unsigned)nb), dim3(256), 0, c->stream, (const int64_t*)col,
+                           count, off, n, prog, rd, reinterpret_cast<uint64_t*>(parts), last, trap);
+    for (int q = 0; q < rd.n; ++q) {
+        if (dtype == DT_F64)
+            hipLaunchKernelGGL(k_scan_final<double>, dim3(1), dim3(kBlock), 0, c->stream, (const ScanOut*)parts + q * nb,
+                               (uint32_t)nb, res + q);
+        else
+            hipLaunchKernelGGL(k_scan_final<int64_t>, dim3(1), dim3(kBlock), 0, c->stream, (const ScanOut*)parts + q * nb,
+                               (uint32_t)nb, res + q);
+    }
+    return hip_ok(hipGetLastError(), "fused vector expression launch");
 }
 
 bool expr_eval(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog, void* out,

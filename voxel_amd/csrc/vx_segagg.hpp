@@ -5,9 +5,12 @@
 #pragma once
 #include <stdint.h>
 
+#include "vx_expr_abi.hpp"
+
 namespace vx {
 
 struct StreamCtx;
+struct ScanOut;
 
 // Moments of rows as f64: count, mean, sum of squared deviations.
 struct Moments {
@@ -45,21 +48,21 @@ bool segagg_unkey_f64(const uint64_t* k, uint64_t n, uint64_t* out, StreamCtx* c
 // first row (row order) of the earliest-sorting of them.  n < 2^30.
 bool segagg_mode(int dtype, const void* x, uint64_t n, SegaggBufs& b, uint64_t* row, uint64_t* ties, StreamCtx* c);
 
-// Vector words of a Tier-0 loop body evaluated per row; the lane words
-// (VINSERT..VSLIDE, VBLEND, VGATHER) read the other rows of the row's kL = 4
-// group, which sit in the neighbouring threads.
-struct ExprOp {
-    uint8_t op, dst, a, b;           // opcode, vector registers
-    int16_t cmp;                     // filters: kCmp code, 7 never
-    uint16_t imm;                    // mask register of filters / compares, lane words' imm12
-    uint64_t s;                      // scalar operand / threshold bits
-};
-struct ExprProg {
-    ExprOp ops[16];
-    int32_t nops, vload, target, pad;
-    const void* gcol[2];             // rows VGATHER reads (device; the op's cmp picks one); null: a segment
-    uint64_t gcount[2];              // the engine does not have
-};
+// The words over rows off + i, i < n (as expr_eval), reduced without writing
+// the evaluated values: res[q] (device) = scan result of reduction q over
+// every row (VSUM: sum pair / wrapping sum; VCOUNT: cnt), last[q] (device) =
+// the reduction of the last group of four rows (the loop's partial
+// register; not written when n = 0).  Uses the stream's scan partials.
+bool expr_reduce(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog,
+                 const ExprRed& rd, ScanOut* res, uint64_t* last, int* trap, StreamCtx* c);
+
+// The compiled form of a loop body (vx_jit.cpp): launches it over rows off +
+// i, i < n, with nb workgroups writing 8-word scan partials (reduction q's
+// at parts + 8 * q * nb) and the last group's values into last; *launched =
+// false when the loop is left to the interpreter kernels.
+bool jit_expr_launch(bool f64, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog,
+                     const ExprRed& rd, uint64_t* parts, unsigned nb, uint64_t* last, int* trap, void* stream,
+                     bool* launched);
 
 // out[i] = register `prog.target` after the words, for rows off + i, i < n
 // (the loaded value is 0 for rows at or past count; n a multiple of 4, the
