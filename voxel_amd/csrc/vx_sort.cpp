@@ -86,7 +86,7 @@ bool ensure_stage(SortBufs& b, uint64_t n) {
 // passes executed.
 static int sort_pairs_impl(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
                            StreamCtx* c, bool row_index) {
-    if (n >= (1ull << 30)) { set_error("sort: %llu keys exceed the 2^30 limit of the look-back words",
+    if (n >= (1ull << 30)) { set_error("sort: %llu keys exceed the sort's 2^30-row limit",
                                        (unsigned long long)n); return -1; }
     SortBufs& b = bufs_for(c);
     if (!ensure_work(b, n)) return -1;
