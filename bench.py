@@ -779,6 +779,31 @@ def bench_serve() -> dict:
     res["throughput_ratio_saturated_vs_async"] = round(res["batched_saturated"]["queries_per_s"] /
                                                        res["unbatched_async_saturated"]["queries_per_s"], 3)
     res["autotune"] = tune
+    # one query alone, no arrival schedule: a batch of one (the server's
+    # small-batch kernels) against one synchronous voxel_gpu_scan, wall time
+    # of the call, on the columns of about 1M and 8M rows
+    from voxel_amd.serve import Batch
+    single = {}
+    b1 = Batch(16)
+    for want in (1 << 20, 1 << 23):
+        ci = int(np.argmin(np.abs(lengths - want) + (np.arange(ncol) >= 32) * 2**40))   # an f64 column
+        q1 = scan.make_desc(cols[ci], "lt", 500.0)
+
+        def lat(fn, reps=300):
+            for _ in range(20):
+                fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            ts.sort()
+            return round(ts[reps // 2] * 1e6, 1), round(ts[int(reps * 0.99)] * 1e6, 1)
+        pb = lat(lambda: b1.run_raw([q1]))
+        ps = lat(lambda: scan.scan(cols[ci], "lt", 500.0))
+        single["rows_%d" % int(lengths[ci])] = {"batch_of_one_p50_p99_us": pb, "voxel_gpu_scan_p50_p99_us": ps}
+    b1.close()
+    res["single_query_latency"] = single
     # a saturated batch holds all 4096 queries: queries on one column share a
     # scan, up to 4 per job, so the rows read are fewer than the rows queried
     per_col = np.bincount(pick, minlength=ncol)

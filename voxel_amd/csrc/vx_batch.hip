@@ -22,8 +22,8 @@
 // Host side: BatchRunner keeps pinned descriptor/result buffers and a graph
 // of four nodes -- descriptor upload, tile kernel, finish kernel, result
 // download -- whose copy sizes are updated per launch (no memset node); a
-// batch of at most kSmallQ queries instead runs as one kernel launch
-// (k_batch_small).
+// batch of at most kSmallQ queries instead runs as two kernel launches with
+// their descriptors as arguments (k_batch_small, k_batch_small_finish).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -237,13 +237,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_finish(const BatchHeader* __re
     }
 }
 
-// Batches of at most kSmallQ queries: one kernel, no graph.  The header,
-// query and job records travel as the kernel's first argument (the kernarg
-// segment, read through its segment pointer and copied to LDS by each
-// workgroup); waves take chunks like k_batch, and the last workgroup to
-// finish (a ticket with an agent-scope release/acquire, once per workgroup)
-// merges each query's unit partials with the same code as k_batch_finish --
-// the same bits -- and writes the results straight into pinned host memory.
+// Batches of at most kSmallQ queries: no graph, no descriptor upload.  The
+// header, query and job records travel as the kernel's first argument (the
+// kernarg segment, read through its segment pointer and copied to LDS by
+// each workgroup); waves take chunks like k_batch at the deepest unroll, and
+// a second kernel merges each query's unit partials with the same code as
+// k_batch_finish -- the same bits -- and writes the results straight into
+// pinned host memory.  (A last-workgroup ticket in one kernel needs an
+// agent-scope release / acquire in every workgroup, an L2 write-back on this
+// multi-XCD part, which cost more than the second launch.)
 constexpr uint32_t kSmallQ = 8;
 
 struct SmallArgs {
@@ -255,10 +257,8 @@ static_assert(sizeof(SmallArgs) == 64 + kSmallQ * (128 + 64), "SmallArgs layout"
 static_assert(sizeof(SmallArgs) <= 2048, "SmallArgs must stay a small kernel argument");
 
 template <int U, bool NT>
-__global__ __launch_bounds__(kBlock) void k_batch_small(const SmallArgs args, ScanOut* __restrict__ parts,
-                                                        uint32_t* __restrict__ ctr, ScanOut* __restrict__ hres) {
+__global__ __launch_bounds__(kBlock) void k_batch_small(const SmallArgs args, ScanOut* __restrict__ parts) {
     __shared__ SmallArgs sa;
-    __shared__ bool s_last;
     {
         // args is the first explicit argument: offset 0 of the kernarg segment
         const uint4* src = (const uint4*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -271,20 +271,27 @@ __global__ __launch_bounds__(kBlock) void k_batch_small(const SmallArgs args, Sc
         }
     }
     __syncthreads();
-    const uint32_t nq = sa.h.nq;
     run_chunks<U, NT>(sa.j, sa.h.njobs, sa.h.total_tiles, sa.h.tile_rows, sa.q, parts);
-    __syncthreads();                                            // every wave of the workgroup is done
-    if (!publish_and_check_last(&ctr[1], &s_last)) return;
-    for (uint32_t qi = 0; qi < nq; ++qi) {
-        const BatchQuery& q = sa.q[qi];
-        switch (q.dtype) {
-            case DT_F64: finish_query<double>(q, parts, &hres[q.orig]); break;
-            case DT_F32: finish_query<float>(q, parts, &hres[q.orig]); break;
-            case DT_I64: finish_query<int64_t>(q, parts, &hres[q.orig]); break;
-            default: finish_query<int32_t>(q, parts, &hres[q.orig]); break;
-        }
+}
+
+// The small batch's queries (at most kSmallQ), as a kernel argument.
+struct SmallQueries {
+    BatchQuery q[kSmallQ];
+};
+
+// After k_batch_small, at the kernel boundary (no release / acquire per
+// workgroup): workgroup b merges query b's unit partials in unit order and
+// writes its result to the host-visible slot of its caller index.
+__global__ __launch_bounds__(kBlock) void k_batch_small_finish(const SmallQueries sq,
+                                                               const ScanOut* __restrict__ parts,
+                                                               ScanOut* __restrict__ hres) {
+    const BatchQuery& q = sq.q[blockIdx.x];
+    switch (q.dtype) {
+        case DT_F64: finish_query<double>(q, parts, &hres[q.orig]); break;
+        case DT_F32: finish_query<float>(q, parts, &hres[q.orig]); break;
+        case DT_I64: finish_query<int64_t>(q, parts, &hres[q.orig]); break;
+        default: finish_query<int32_t>(q, parts, &hres[q.orig]); break;
     }
-    if (threadIdx.x == 0) ctr[1] = 0;                            // every other workgroup has exited
 }
 
 void* batch_kernel(int unroll) {
@@ -513,14 +520,16 @@ bool BatchRunner::launch(const voxel_gpu_scan_desc* q, size_t nq) {
         std::memcpy(sa.q, qs, nq * sizeof(BatchQuery));
         std::memcpy(sa.j, jd, nj * sizeof(BatchJob));
         if (!hip_ok(hipEventRecord(ev0_, st_), "hipEventRecord")) return false;
-        switch (cfg_.unroll) {
-            case 2: hipLaunchKernelGGL((k_batch_small<2, true>), dim3(grid), dim3(kBlock), 0, st_, sa, parts_, ctr_,
-                                       hres_); break;
-            case 8: hipLaunchKernelGGL((k_batch_small<8, true>), dim3(grid), dim3(kBlock), 0, st_, sa, parts_, ctr_,
-                                       hres_); break;
-            default: hipLaunchKernelGGL((k_batch_small<4, true>), dim3(grid), dim3(kBlock), 0, st_, sa, parts_, ctr_,
-                                        hres_); break;
-        }
+        // a small batch has few units to spread over the waves, so each wave
+        // keeps the deepest unroll in flight whatever the configuration (the
+        // unroll does not change a lane's order of rows, so the bits are the
+        // large path's)
+        hipLaunchKernelGGL((k_batch_small<8, true>), dim3(grid), dim3(kBlock), 0, st_, sa, parts_);
+        SmallQueries sq;
+        std::memset(&sq, 0, sizeof(sq));
+        std::memcpy(sq.q, qs, nq * sizeof(BatchQuery));
+        hipLaunchKernelGGL(k_batch_small_finish, dim3((unsigned)nq), dim3(kBlock), 0, st_, sq, (const ScanOut*)parts_,
+                           hres_);
         if (!hip_ok(hipGetLastError(), "small batch launch") || !hip_ok(hipEventRecord(ev1_, st_), "hipEventRecord"))
             return false;
         nq_ = nq;
