@@ -297,13 +297,26 @@ __global__ __launch_bounds__(kEncThreads) void k_delta_scan(const T* __restrict_
     const uint64_t b0 = (uint64_t)blockIdx.x * span;
     const uint64_t b1 = b0 + span < n ? b0 + span : n;
     U run = base + start[blockIdx.x];                     // value before the tile's first row
+    // the next tile's deltas are loaded into registers while this one is
+    // scanned, so the loads of a workgroup overlap its LDS work
+    U pf[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = b0 + (uint64_t)k * kEncThreads + threadIdx.x;
+        pf[k] = i < b1 ? (U)x[i] : U(0);
+    }
     for (uint64_t t0 = b0; t0 < b1; t0 += kScanChunk) {
 #pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            const uint64_t i = t0 + (uint64_t)k * kEncThreads + threadIdx.x;
-            tile[at(k * kEncThreads + threadIdx.x)] = i < b1 ? (U)x[i] : U(0);
-        }
+        for (int k = 0; k < kScanItems; ++k) tile[at(k * kEncThreads + threadIdx.x)] = pf[k];
         __syncthreads();
+        const uint64_t t1 = t0 + kScanChunk;
+        if (t1 < b1) {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k) {
+                const uint64_t i = t1 + (uint64_t)k * kEncThreads + threadIdx.x;
+                pf[k] = i < b1 ? (U)x[i] : U(0);
+            }
+        }
         U v[kScanItems];
         U s = 0;
 #pragma unroll
