@@ -10,6 +10,8 @@ selected decoded rows.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 
@@ -254,3 +256,52 @@ def test_delta_scan_across_workgroup_spans(gpu):
                 check_scan(col.scan(op, int(t)), expect(full, op, t), False)
         del buf, col
         torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32, np.int64, np.int32])
+def test_dictionary_scan_by_code_counts(gpu, dt):
+    """Dictionaries of at most 4096 values are scanned by per-code row counts
+    (the aggregate taken once per code as count x value): every predicate,
+    dictionaries of 1, 3 and 4096 values, infinite and huge f64 values, i64
+    sums that wrap; COUNT, MIN, MAX exact, integer SUM exact modulo 2^64,
+    f64 SUM within 2 ulp and f32 SUM within 4 f32 ulp of the exact sum."""
+    from voxel_amd import encoding as enc
+    rng = np.random.default_rng(68)
+    for nd in (1, 3, 4096):
+        n = (1 << 22) + int(rng.integers(0, 1000))
+        if dt == np.float64:
+            d = np.sort(rng.standard_normal(nd) * 1e3)
+            if nd >= 3:
+                d[0], d[-1] = -np.inf, np.inf
+                d[nd // 2] = 3e300
+            d = np.sort(d)
+        elif dt == np.float32:
+            d = np.sort(rng.standard_normal(nd).astype(np.float32) * np.float32(100))
+        elif dt == np.int64:
+            d = np.sort(rng.integers(-2**62, 2**62, nd)).astype(np.int64)
+        else:
+            d = np.sort(rng.integers(-2**30, 2**30, nd)).astype(np.int32)
+        ix = rng.integers(0, nd, n).astype(np.uint32)
+        x = d[ix]
+        col = enc.dictionary(dev(d), dev(ix.view(np.int32)))
+        for op in range(7):
+            t = d[int(rng.integers(0, nd))] if op != 6 else dt(0)
+            if dt == np.float64 and not np.isfinite(t):
+                t = dt(0.5)
+            got = col.scan(op, t.item() if hasattr(t, "item") else t)
+            v = expect(x, op, t)
+            assert got["count"] == v.size, (nd, op)
+            if v.size == 0:
+                continue
+            assert got["min"] == v.min() and got["max"] == v.max(), (nd, op)
+            if dt == np.float64:
+                if np.all(np.isfinite(v)):
+                    ex = float(o.exact_sum(v))
+                    assert abs(got["sum"] - ex) <= 2 * o.ulp(ex) + 1e-300, (nd, op, got["sum"], ex)
+                else:
+                    assert math.isnan(got["sum"]) or math.isinf(got["sum"])
+            elif dt == np.float32:
+                ex = float(o.exact_sum(v.astype(np.float64)))
+                assert abs(got["sum"] - ex) <= 4 * float(np.spacing(np.float32(abs(ex)))) + 1e-30, (nd, op)
+            else:
+                assert (got["sum"] - int(v.astype(object).sum())) % 2**64 == 0, (nd, op)

@@ -26,6 +26,7 @@
 #include "vx_acc.hpp"
 #include "vx_device.hpp"
 #include "vx_kernels.hpp"
+#include "vx_radix.hpp"
 #include "vx_runtime.hpp"
 
 #pragma clang fp contract(off)
@@ -404,6 +405,111 @@ __global__ __launch_bounds__(kBlock) void k_scan_dict(const T* __restrict__ dict
     if (threadIdx.x == 0) acc.store(&parts[blockIdx.x]);
 }
 
+// Dictionary scan over a dictionary of at most kDictLds entries: the
+// predicate and the aggregates depend only on a row's code, so each
+// workgroup counts its rows per code in an LDS histogram (one LDS atomic per
+// row, no per-row float work) and adds the nonzero counts into a global
+// per-code count; k_dict_final then filters the codes and aggregates count x
+// value per code.  Codes at or past nd set *bad (the call fails).
+__global__ __launch_bounds__(kBlock) void k_dict_hist(uint32_t nd, const uint32_t* __restrict__ idx, uint64_t n,
+                                                      unsigned long long* __restrict__ counts,
+                                                      uint32_t* __restrict__ bad) {
+    __shared__ uint32_t h[kDictLds];
+    for (uint32_t i = threadIdx.x; i < nd; i += kBlock) h[i] = 0u;
+    __syncthreads();
+    bool oob = false;
+    auto add = [&](uint32_t k) {
+        if (k < nd) atomicAdd(&h[k], 1u);
+        else oob = true;
+    };
+    uint64_t head = ((16u - ((uintptr_t)idx & 15u)) & 15u) / 4;
+    if (head > n) head = n;
+    const uint64_t nq = (n - head) / 4;                   // 16-byte loads of four indices
+    const uint64_t tail0 = head + nq * 4;
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g < head + (n - tail0)) add(idx[g < head ? g : tail0 + (g - head)]);
+    const uint4* q = reinterpret_cast<const uint4*>(idx + head);
+    const uint64_t tile = (uint64_t)kBlock * kEncUnroll;
+    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < nq; base += (uint64_t)gridDim.x * tile) {
+        uint4 k[kEncUnroll];
+#pragma unroll
+        for (int u = 0; u < kEncUnroll; ++u) {            // the tile's loads are in flight together
+            const uint64_t v = base + u * kBlock + threadIdx.x;
+            if (v < nq) k[u] = ld16<uint4, true>(&q[v]);
+        }
+#pragma unroll
+        for (int u = 0; u < kEncUnroll; ++u)
+            if (base + u * kBlock + threadIdx.x < nq) { add(k[u].x); add(k[u].y); add(k[u].z); add(k[u].w); }
+    }
+    if (oob) *bad = 1u;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nd; i += kBlock)
+        if (h[i]) atomicAdd(&counts[i], (unsigned long long)h[i]);
+}
+
+// One workgroup: COUNT / SUM / MIN / MAX over the codes that pass, each code
+// entering with its row count c: float sums as the exact product c * x (its
+// rounded value plus the FMA error) in a compensated double pair, merged in
+// thread order; integer sums wrapping; the result as a scan partial of T.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dict_final(const T* __restrict__ dict, uint32_t nd,
+                                                       const unsigned long long* __restrict__ counts, PredDev pd,
+                                                       ScanOut* __restrict__ out) {
+    constexpr bool F = TyInfo<T>::F;
+    __shared__ double ss[kBlock], sc[kBlock];
+    __shared__ unsigned long long sn[kBlock], si[kBlock];
+    __shared__ T smn[kBlock], smx[kBlock];
+    Pred<T, false> p; p.load(pd);
+    double s = 0.0, c = 0.0;
+    unsigned long long cnt = 0, isum = 0;
+    T mn = Lim<T>::hi(), mx = Lim<T>::lo();
+    for (uint32_t k = threadIdx.x; k < nd; k += kBlock) {
+        const unsigned long long ck = counts[k];
+        const T x = dict[k];
+        if (ck == 0 || !p(x)) continue;
+        cnt += ck;
+        if constexpr (F) {
+            const double cd = (double)ck, xd = (double)x;
+            const double ph = cd * xd;
+            const double pl = __builtin_fma(cd, xd, -ph);
+            two_sum<double>(s, c, ph);
+            c += __builtin_isfinite(ph) ? pl : 0.0;
+        } else {
+            isum += (unsigned long long)ck * (unsigned long long)(long long)x;
+        }
+        mn = x < mn ? x : mn;
+        mx = x > mx ? x : mx;
+    }
+    ss[threadIdx.x] = s; sc[threadIdx.x] = c; sn[threadIdx.x] = cnt; si[threadIdx.x] = isum;
+    smn[threadIdx.x] = mn; smx[threadIdx.x] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int t = 1; t < kBlock; ++t) {                // thread order
+            double cc = c + sc[t];
+            two_sum<double>(s, cc, ss[t]);
+            c = cc;
+            cnt += sn[t];
+            isum += si[t];
+            mn = smn[t] < mn ? smn[t] : mn;
+            mx = smx[t] > mx ? smx[t] : mx;
+        }
+        ScanOut r{};
+        r.cnt = cnt;
+        if constexpr (F) {
+            const T hi = (T)(s + c);                      // the pair in T: rounded total and its remainder
+            const T lo = (T)((s - (double)hi) + c);
+            r.s = to_bits(hi);
+            r.c = to_bits(lo);
+        } else {
+            r.s = isum;
+            r.c = 0;
+        }
+        r.mn = to_bits(mn);
+        r.mx = to_bits(mx);
+        *out = r;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_for(const int32_t* __restrict__ enc, double base, double scale,
                                                      uint64_t n, PredDev pd, ScanOut* __restrict__ parts) {
     __shared__ Acc<double> red[kBlock / 64];
@@ -755,6 +861,22 @@ int64_t decode(const voxel_gpu_encoded_col* e, uint64_t off, uint64_t cnt, void*
     return (int64_t)n;
 }
 
+// Dictionaries of at most kDictLds entries: per-code counts, then the
+// aggregate from the dictionary (result straight into *res).
+template <typename T>
+bool launch_dict_hist(const voxel_gpu_encoded_col* e, PredDev pd, unsigned blocks, ScanOut* res, StreamCtx* c) {
+    EncBufs& b = enc_bufs();
+    if (!b.grow(&b.parts, &b.parts_bytes, (size_t)e->nvalues * 8)) return false;
+    unsigned long long* counts = static_cast<unsigned long long*>(b.parts);
+    if (e->nvalues &&
+        !hip_ok(launch_zero32(reinterpret_cast<uint32_t*>(counts), e->nvalues * 2, c->stream), "clear")) return false;
+    hipLaunchKernelGGL(k_dict_hist, dim3(blocks), dim3(kBlock), 0, c->stream, (uint32_t)e->nvalues, e->index, e->rows,
+                       counts, enc_bufs().flag_dev);
+    hipLaunchKernelGGL((k_dict_final<T>), dim3(1), dim3(kBlock), 0, c->stream, static_cast<const T*>(e->values),
+                       (uint32_t)e->nvalues, (const unsigned long long*)counts, pd, res);
+    return hip_ok(hipGetLastError(), "dictionary scan launch");
+}
+
 template <typename T>
 void launch_dict_scan(const voxel_gpu_encoded_col* e, PredDev pd, ScanOut* parts, unsigned blocks, StreamCtx* c) {
     hipLaunchKernelGGL((k_scan_dict<T>), dim3(blocks), dim3(kBlock), 0, c->stream, static_cast<const T*>(e->values),
@@ -878,6 +1000,16 @@ int voxel_gpu_scan_encoded(const voxel_gpu_encoded_col* col, int op, uint64_t t,
         case VOXEL_ENC_DICTIONARY:
             if (!dict_flag(c)) return -1;
             blocks = grid_for(col->rows / 4 + 1, kBlock * kEncUnroll, cap);
+            if (col->nvalues <= kDictLds) {               // per-code counts; the result is written directly
+                bool ok;
+                if (col->dtype == VOXEL_GPU_F64) ok = launch_dict_hist<double>(col, pd, blocks, &c->dev->scan, c);
+                else if (col->dtype == VOXEL_GPU_F32) ok = launch_dict_hist<float>(col, pd, blocks, &c->dev->scan, c);
+                else if (col->dtype == VOXEL_GPU_I64) ok = launch_dict_hist<int64_t>(col, pd, blocks, &c->dev->scan, c);
+                else ok = launch_dict_hist<int32_t>(col, pd, blocks, &c->dev->scan, c);
+                if (!ok) return -1;
+                blocks = 0;                               // no partials to merge
+                break;
+            }
             if (col->dtype == VOXEL_GPU_F64) launch_dict_scan<double>(col, pd, parts, blocks, c);
             else if (col->dtype == VOXEL_GPU_F32) launch_dict_scan<float>(col, pd, parts, blocks, c);
             else if (col->dtype == VOXEL_GPU_I64) launch_dict_scan<int64_t>(col, pd, parts, blocks, c);
@@ -902,7 +1034,8 @@ int voxel_gpu_scan_encoded(const voxel_gpu_encoded_col* col, int op, uint64_t t,
                                static_cast<const int32_t*>(col->values), col->base, col->scale, col->rows, pd, parts);
             break;
         }
-        if (adt == DT_F64) launch_final<double>(parts, blocks, &c->dev->scan, c);
+        if (blocks == 0) {
+        } else if (adt == DT_F64) launch_final<double>(parts, blocks, &c->dev->scan, c);
         else if (adt == DT_F32) launch_final<float>(parts, blocks, &c->dev->scan, c);
         else if (adt == DT_I64) launch_final<int64_t>(parts, blocks, &c->dev->scan, c);
         else launch_final<int32_t>(parts, blocks, &c->dev->scan, c);
