@@ -70,7 +70,7 @@ class DeviceArray:
     def __len__(self) -> int:
         return self.n
 
-    def __del__(self):
+    def __del__(self, _lib=lib):          # the library outlives module teardown
         if getattr(self, "_own", False) and self.ptr:
-            lib.voxel_gpu_free(C.c_void_p(self.ptr))
+            _lib.voxel_gpu_free(C.c_void_p(self.ptr))
             self.ptr = 0

@@ -39,8 +39,10 @@ class Batch:
             lib.voxel_gpu_batch_destroy(self.h)
             self.h = None
 
-    def __del__(self):
-        self.close()
+    def __del__(self, _lib=lib):          # the library outlives module teardown
+        if getattr(self, "h", None):
+            _lib.voxel_gpu_batch_destroy(self.h)
+            self.h = None
 
     def config(self, unroll: int = 0, tile_rows: int = 0, blocks_per_cu: int = 0) -> None:
         if lib.voxel_gpu_batch_config(self.h, unroll, tile_rows, blocks_per_cu) != 0:
@@ -93,8 +95,10 @@ class Server:
             lib.voxel_gpu_server_destroy(self.h)
             self.h = None
 
-    def __del__(self):
-        self.close()
+    def __del__(self, _lib=lib):          # the library outlives module teardown
+        if getattr(self, "h", None):
+            _lib.voxel_gpu_server_destroy(self.h)
+            self.h = None
 
     def config(self, unroll: int = 0, tile_rows: int = 0, blocks_per_cu: int = 0) -> None:
         if lib.voxel_gpu_server_config(self.h, unroll, tile_rows, blocks_per_cu) != 0:

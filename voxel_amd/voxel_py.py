@@ -157,10 +157,10 @@ class _Engine:
         self._keep = []          # borrowed arrays stay alive as long as the engine
         self._code = None
 
-    def __del__(self):
+    def __del__(self, _lib=lib):          # the library outlives module teardown
         h = getattr(self, "_h", None)
         if h:
-            getattr(lib, self._destroy)(h)
+            getattr(_lib, self._destroy)(h)
             self._h = None
 
     def add_segment(self, arr) -> int:
@@ -313,9 +313,9 @@ class _Dev:
             check("voxel_gpu_memcpy_d2h")
         return out
 
-    def __del__(self):
+    def __del__(self, _lib=lib):          # the library outlives module teardown
         if getattr(self, "ptr", None):
-            lib.voxel_gpu_free(self.ptr)
+            _lib.voxel_gpu_free(self.ptr)
             self.ptr = None
 
 
@@ -378,9 +378,9 @@ class HashAggregator:
         self._arena = arena
         self.h = lib.voxel_agg_create()
 
-    def __del__(self):
+    def __del__(self, _lib=lib):          # the library outlives module teardown
         if getattr(self, "h", None):
-            lib.voxel_agg_destroy(self.h)
+            _lib.voxel_agg_destroy(self.h)
             self.h = None
 
     def init(self, expected_groups: int) -> None:
@@ -422,9 +422,9 @@ class HashJoin:
         self._arena = arena
         self.h = None
 
-    def __del__(self):
+    def __del__(self, _lib=lib):          # the library outlives module teardown
         if getattr(self, "h", None):
-            lib.voxel_gpu_join_destroy(self.h)
+            _lib.voxel_gpu_join_destroy(self.h)
             self.h = None
 
     def build(self, keys) -> None:
@@ -487,10 +487,10 @@ class NullBitmap:
         self._n = int(count)
         self._h = lib.voxel_null_create(self._n)
 
-    def __del__(self):
+    def __del__(self, _lib=lib):          # the library outlives module teardown
         h = getattr(self, "_h", None)
         if h:
-            lib.voxel_null_destroy(h)
+            _lib.voxel_null_destroy(h)
             self._h = None
 
     def is_null(self, row: int) -> bool:
