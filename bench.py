@@ -620,7 +620,8 @@ def cpu_reference_paths() -> dict:
     """The reference's other hot-path CPU implementations (built from its
     sources into oracle/_ref), each on a bounded sample of its workload on
     this host: the interpreter fast path (one core), RunParallel (all the
-    host cores given to this process, at most 16), RadixSort<u64> (one core),
+    host cores given to this process, at most 16), the Tier-0 interpreter on
+    a loop with vector words (one core), RadixSort<u64> (one core),
     HashJoin<i64> Build + Probe (one core) and HashAggregator<i32,f64> (one
     core); median of 3 after a warm-up."""
     import numpy as np
@@ -675,7 +676,22 @@ def cpu_reference_paths() -> dict:
         "sample": "VectorFilter<f64>::ApplyGT(a) + ApplyLT(b), ColumnFilter::Init + AndWith x2, "
                   "SelectionVector::BuildFromBitmap, then COUNT/SUM/MIN/MAX of a over the selected rows; "
                   "2^25 rows per column, a > 292.8932188 AND b < 707.1067812 (50 %)"}
-    del x, b
+    del b
+    # a Tier-0 loop with vector words (sum(x * x + x), the bench's
+    # tier0_expression loop) in the reference's interpreter, one core
+    words = np.array(expr_programs()["sum_x2_plus_x"][0], np.uint32)
+    ne = 1 << 24
+    xe = x[:ne].copy()
+
+    def expr():
+        r = np.zeros(64, np.uint64)
+        r[2] = ne
+        o.R.vr_engine_run_f64(o.ptr(xe), ne, o.ptr(words), words.size, o.ptr(r))
+    t = med(expr)
+    out["tier0_expression_f64"] = {"rows_per_s": round(ne / t, 1), "gb_per_s": round(ne * 8 / t / 1e9, 3), "cores": 1,
+                                   "sample": "Engine<f64>::Run of the loop V1 = V0 * V0, V2 = V1 + V0, VSUM, ADDF "
+                                             "(Tier-0 interpreter), 2^24 rows"}
+    del x, xe
     ns = 1 << 24
     keys0 = rng.integers(0, 2**63, ns, dtype=np.int64).astype(np.uint64) * np.uint64(2)
 
