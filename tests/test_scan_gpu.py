@@ -355,20 +355,21 @@ def test_tier0_vector_arithmetic(gpu, is_f64, expr_mode):
 
 @pytest.mark.parametrize("is_f64", [True, False])
 def test_tier0_expression_loops_compiled_at_size(gpu, is_f64):
-    """Loops of 2^21+ rows with vector words and VSUM / VCOUNT reductions run
-    compiled whole (the default mode compiles loops of 2^20 rows or more):
-    partial registers and counts equal the oracle's bits (vo_tier0_expr_*),
-    i64 sums exact, f64 sums within 2 ulp of the exact sum of the reduced
-    register's lanes; the same shape run again with other scalar operands
-    reuses the compiled kernel and still matches."""
+    """Loops of 2^21+ rows with vector words and VSUM / VCOUNT / VRED_MIN /
+    VRED_MAX reductions run compiled whole (the default mode compiles loops
+    of 2^20 rows or more): partial registers, counts, min and max equal the
+    oracle's bits (vo_tier0_expr_*; f64 min / max over both signs take the
+    exact in-order fold), i64 sums exact, f64 sums within 2 ulp of the exact
+    sum of the reduced register's lanes; the same shape run again with other
+    scalar operands reuses the compiled kernel and still matches."""
     from tests import tier0_gen as tg
     from voxel_amd import voxel_py as vx
     assert gpu.voxel_gpu_set_expr_jit(1) in (0, 1, 2)
     rng = np.random.default_rng(75 if is_f64 else 76)
     for trial in range(6):
         n = (1 << 21) + 4 * int(rng.integers(0, 1000))
-        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 7)), reds=(0, 1),
-                                          lane_words=trial % 3 == 2)
+        code, spec = tg.make_expr_program(rng, is_f64, nops=int(rng.integers(1, 7)),
+                                          reds=(0, 1) if trial < 2 else (0, 1, 2, 3), lane_words=trial % 3 == 2)
         for rep in range(2):
             regs = np.zeros(64, np.uint64)
             if is_f64:

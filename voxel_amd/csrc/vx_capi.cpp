@@ -239,55 +239,78 @@ bool run_tier0_block(const Plan::Block& b, const PredDev& pred, int dtype, const
 // The body's vector words over rows [off, off + nrows) of column col (rows at
 // or past count load 0), one evaluation per reduced register, each reduced
 // as an unfiltered loop would; then the loop's register effects.
-// The accumulate word of a VSUM / VCOUNT block must be one the lowering maps
-// (ADDF for f64 sums, ADD for i64 sums and counts).
-bool sum_count_pair_ok(const Plan::Block& b, int dtype) {
+// The accumulate word of a block must be one the lowering maps (ADDF for
+// f64 sums, ADD for i64 sums and counts, a CMP-select fold for VRED_*).
+bool block_pair_ok(const Plan::Block& b, int dtype) {
     const bool f = dtype == DT_F64;
-    return (b.red == 0 && b.accop == 0 && f) || (b.red == 0 && b.accop == 1 && !f) || (b.red == 1 && b.accop == 1);
+    return (b.red == 0 && b.accop == 0 && f) || (b.red == 0 && b.accop == 1 && !f) || (b.red == 1 && b.accop == 1) ||
+           (b.red >= 2 && b.accop >= 2);
 }
 
-// Loops whose reductions are all VSUM / VCOUNT and that store nothing: the
-// words and the reductions run in one pass (expr_reduce), with no evaluated
-// column in HBM; then the loop's register effects.
-bool run_expr_fused(Engine& e, uint64_t* regs, const ExprProg& prog, const void* col, uint64_t count, uint64_t off,
-                    uint64_t nrows, uint64_t total, StreamCtx* c) {
+// Loops that store nothing: the words and the reductions run in one pass
+// (expr_reduce), with no evaluated column in HBM; then the loop's register
+// effects.  Returns 1 when done, 0 when a VRED_MIN / VRED_MAX fold needs the
+// exact sequential fold (its group values span more than 2^63 in both the
+// signed and the unsigned order; nothing committed), -1 on error.
+int run_expr_fused(Engine& e, uint64_t* regs, const ExprProg& prog, const void* col, uint64_t count, uint64_t off,
+                   uint64_t nrows, uint64_t total, StreamCtx* c) {
     const Plan& p = e.plan;
     ExprRed rd{};
     rd.n = (int32_t)p.blocks.size();
     for (size_t q = 0; q < p.blocks.size(); ++q) {
         const Plan::Block& b = p.blocks[q];
-        if (!sum_count_pair_ok(b, e.dtype)) {
+        if (!block_pair_ok(b, e.dtype)) {
             set_error("loop (reduction %u, accumulate %u) on this engine type is not lowered", b.red, b.accop);
-            return false;
+            return -1;
         }
         rd.vreg[q] = b.vreg;
         rd.red[q] = b.red;
     }
     // layout: ScanOut res[4] | u64 last[4] | int trap
     const size_t last_at = 4 * sizeof(ScanOut), trap_at = last_at + 32, bytes = trap_at + 8;
-    if (!e.fused.ensure(bytes)) return false;
+    if (!e.fused.ensure(bytes)) return -1;
     char* fb = static_cast<char*>(e.fused.dev);
-    if (!hip_ok(launch_zero32(reinterpret_cast<uint32_t*>(fb + last_at), (32 + 8) / 4, c->stream), "clear")) return false;
+    if (!hip_ok(launch_zero32(reinterpret_cast<uint32_t*>(fb + last_at), (32 + 8) / 4, c->stream), "clear")) return -1;
     ScanOut* res = reinterpret_cast<ScanOut*>(fb);
     if (!expr_reduce(e.dtype, col, count, off, nrows, prog, rd, res, reinterpret_cast<uint64_t*>(fb + last_at),
                      reinterpret_cast<int*>(fb + trap_at), c))
-        return false;
+        return -1;
     struct {
         ScanOut res[4];
         uint64_t last[4];
         int trap, pad;
     } h{};
     static_assert(sizeof(h) == 4 * sizeof(ScanOut) + 40, "fused result layout");
-    if (!sync_read(c, &h, fb, bytes)) return false;
+    if (!sync_read(c, &h, fb, bytes)) return -1;
     if (h.trap) {
         set_error("a vector division or modulo of INT64_MIN by -1 (the reference's x86 build traps there)");
-        return false;
+        return -1;
+    }
+    uint64_t accs[4];
+    for (size_t q = 0; q < p.blocks.size(); ++q) {              // the folds first: a fallback commits nothing
+        const Plan::Block& b = p.blocks[q];
+        if (b.red < 2) continue;
+        const ScanOut& r = h.res[q];
+        const uint64_t a0 = regs[b.acc], half = 1ull << 63;
+        int64_t smin = (int64_t)r.s, smax = (int64_t)r.c;
+        uint64_t umin = r.cnt, umax = r.mn;
+        smin = (int64_t)a0 < smin ? (int64_t)a0 : smin;          // the accumulator's own value takes part
+        smax = (int64_t)a0 > smax ? (int64_t)a0 : smax;
+        umin = a0 < umin ? a0 : umin;
+        umax = a0 > umax ? a0 : umax;
+        // every value in a window narrower than 2^63: the wrapping CMP
+        // difference is that order, so the fold's result is its extreme
+        if ((uint64_t)smax - (uint64_t)smin < half) accs[q] = b.accop == 2 ? (uint64_t)smin : (uint64_t)smax;
+        else if (umax - umin < half) accs[q] = b.accop == 2 ? umin : umax;
+        else return 0;
     }
     for (size_t q = 0; q < p.blocks.size(); ++q) {
         const Plan::Block& b = p.blocks[q];
         const ScanOut& r = h.res[q];
         uint64_t accb = regs[b.acc];
-        if (b.red == 0 && e.dtype == DT_F64) {
+        if (b.red >= 2) {
+            accb = accs[q];
+        } else if (b.red == 0 && e.dtype == DT_F64) {
             accb = f64_bits(finish_sum_f64(bits_f64(accb), bits_f64(r.s), bits_f64(r.c)));
         } else if (b.red == 0) {
             accb = accb + r.s;
@@ -298,17 +321,16 @@ bool run_expr_fused(Engine& e, uint64_t* regs, const ExprProg& prog, const void*
         regs[b.part] = h.last[q];
     }
     regs[p.off] = total;
-    if (run_scalar(p.epilogue, regs) < 0) return false;
+    if (run_scalar(p.epilogue, regs) < 0) return -1;
     std::memcpy(e.regs, regs, sizeof(uint64_t) * 64);
-    return true;
+    return 1;
 }
 
 bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count, uint64_t off, uint64_t nrows,
                      uint64_t total, StreamCtx* c) {
     const Plan& p = e.plan;
     const uint64_t kl = 4;
-    bool fusable = p.stores.empty() && !p.blocks.empty() && p.blocks.size() <= 4;
-    for (const Plan::Block& b : p.blocks) fusable = fusable && b.red <= 1;
+    const bool fusable = p.stores.empty() && !p.blocks.empty() && p.blocks.size() <= 4;
     if (fusable) {
         ExprProg prog{};
         prog.nops = (int32_t)p.vops.size();
@@ -323,7 +345,9 @@ bool run_expr_blocks(Engine& e, uint64_t* regs, const void* col, uint64_t count,
             prog.gcount[q] = g.count;
             if (g.count && !(prog.gcol[q] = seg_rows(g, e.dtype, 0, g.count, c))) return false;
         }
-        return run_expr_fused(e, regs, prog, col, count, off, nrows, total, c);
+        const int r = run_expr_fused(e, regs, prog, col, count, off, nrows, total, c);
+        if (r != 0) return r > 0;
+        // a VRED_* fold over values spanning both orders: the column path's exact fold
     }
     if (nrows > (1ull << 31)) { set_error("a loop with vector arithmetic over %llu rows (more than 2^31) is not lowered",
                                           (unsigned long long)nrows); return false; }

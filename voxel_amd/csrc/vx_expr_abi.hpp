@@ -24,7 +24,8 @@ struct ExprProg {
 };
 
 // Reductions fused into one evaluation pass (expr_reduce): n of them, the
-// q-th reading vector register vreg[q] with red[q] 0 = VSUM, 1 = VCOUNT.
+// q-th reading vector register vreg[q] with red[q] 0 = VSUM, 1 = VCOUNT,
+// 2 = VRED_MIN, 3 = VRED_MAX.
 struct ExprRed {
     int32_t n;
     uint8_t vreg[4], red[4];

@@ -334,12 +334,79 @@ __device__ __forceinline__ void xacc_block_store(XAcc<T> a, XAcc<T>* lds, uint64
     }
     __syncthreads();
 }
+// VRED_MIN (red 2) / VRED_MAX (red 3) of one group's four lanes in lane
+// order, as the reference reduces them: a lane replaces the running value
+// when it is below (above) it, a NaN lane always replaces the running MIN, a
+// NaN running MAX is always replaced.  All four threads of the group call it
+// and get the group's value bits.
+template <typename T>
+__device__ __forceinline__ uint64_t group_minmax(T y, int red) {
+    constexpr bool F = sizeof(T) == 8 && (T)0.5 != (T)0;
+    const T l[4] = {quad_lane(y, 0), quad_lane(y, 1), quad_lane(y, 2), quad_lane(y, 3)};
+    T m = l[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+        bool take;
+        if (red == 2) take = l[i] < m || (F && l[i] != l[i]);
+        else take = l[i] > m || (F && m != m);
+        m = take ? l[i] : m;
+    }
+    return bits_of(m);
+}
+
+// Extremes of the group values of a VRED_MIN / VRED_MAX reduction in the
+// signed and the unsigned order of their bits (the CMP-select accumulator
+// compares register bits as signed integers; whether the wrapping difference
+// is the signed order is decided on the host from these four).
+struct GAccX {
+    int64_t smin, smax;
+    uint64_t umin, umax;
+};
+__device__ __forceinline__ void gacc_init(GAccX& a) {
+    a.smin = kI64Max; a.smax = kI64Min; a.umin = ~0ull; a.umax = 0ull;
+}
+__device__ __forceinline__ void gacc_add(GAccX& a, uint64_t v) {
+    const int64_t s = (int64_t)v;
+    a.smin = s < a.smin ? s : a.smin; a.smax = s > a.smax ? s : a.smax;
+    a.umin = v < a.umin ? v : a.umin; a.umax = v > a.umax ? v : a.umax;
+}
+__device__ __forceinline__ void gacc_merge(GAccX& a, const GAccX& b) {
+    a.smin = b.smin < a.smin ? b.smin : a.smin; a.smax = b.smax > a.smax ? b.smax : a.smax;
+    a.umin = b.umin < a.umin ? b.umin : a.umin; a.umax = b.umax > a.umax ? b.umax : a.umax;
+}
+// The workgroup's extremes written by thread 0 into a scan-partial slot
+// (eight words: smin, smax, umin, umax, then unused).
+__device__ __forceinline__ void gacc_block_store(GAccX a, GAccX* lds, uint64_t* part) {
+    for (int m = 32; m > 0; m >>= 1) {
+        GAccX o;
+        o.smin = __shfl_xor(a.smin, m, 64); o.smax = __shfl_xor(a.smax, m, 64);
+        o.umin = __shfl_xor(a.umin, m, 64); o.umax = __shfl_xor(a.umax, m, 64);
+        gacc_merge(a, o);
+    }
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) gacc_merge(a, lds[w]);
+        part[0] = (uint64_t)a.smin;
+        part[1] = (uint64_t)a.smax;
+        part[2] = a.umin;
+        part[3] = a.umax;
+    }
+    __syncthreads();
+}
+
 // The loop's partial register from the last group's four lanes (one
 // reduction, as the reference reduces one group): VSUM 0 + l0 + l1 + l2 + l3
-// in lane order, VCOUNT lanes non-zero and not NaN.  All four threads of the
-// group call it; the group's lane 0 writes *out.
+// in lane order, VCOUNT lanes non-zero and not NaN, VRED_MIN / VRED_MAX as
+// group_minmax.  All four threads of the group call it; the group's lane 0
+// writes *out.
 template <typename T>
 __device__ __forceinline__ void group_partial(T y, int red, int lane, uint64_t* out) {
+    if (red >= 2) {
+        const uint64_t g = group_minmax(y, red);
+        if (lane == 0) *out = g;
+        return;
+    }
     const T l0 = quad_lane(y, 0), l1 = quad_lane(y, 1), l2 = quad_lane(y, 2), l3 = quad_lane(y, 3);
     if (lane != 0) return;
     constexpr bool F = sizeof(T) == 8 && (T)0.5 != (T)0;

@@ -1,6 +1,7 @@
 // vx_jit.cpp -- run-time compilation of Tier-0 loop bodies (the GPU side of
 // what the reference's JIT does for its fused loops, jit/compiler.hpp): a
-// loop whose reductions are all VSUM / VCOUNT is turned into HIP source in
+// loop whose reductions are VSUM / VCOUNT / VRED_MIN / VRED_MAX is turned
+// into HIP source in
 // which every vector word is one apply_op<T, OP> call on named registers
 // (straight-line code, no per-row dispatch), compiled once per program shape
 // with hipRTC for the device's architecture, and kept for the process.  The
@@ -76,8 +77,13 @@ std::string kernel_source(bool f64, const ExprProg& prog, const ExprRed& rd) {
          "    uint64_t off, uint64_t n, vx::ExprProg prog, uint64_t* __restrict__ parts, uint64_t* __restrict__ last,\n"
          "    int* __restrict__ trap) {\n"
          "  using namespace vx::xo;\n"
-         "  __shared__ XAcc<VxT> lds[4];\n";
-    for (int q = 0; q < rd.n; ++q) s += "  XAcc<VxT> a" + std::to_string(q) + "; xacc_init(a" + std::to_string(q) + ");\n";
+         "  __shared__ XAcc<VxT> lds[4];\n"
+         "  __shared__ GAccX glds[4];\n";
+    for (int q = 0; q < rd.n; ++q) {
+        const std::string Q = std::to_string(q);
+        s += rd.red[q] <= 1 ? "  XAcc<VxT> a" + Q + "; xacc_init(a" + Q + ");\n"
+                            : "  GAccX a" + Q + "; gacc_init(a" + Q + ");\n";
+    }
     for (int k = 0; k < prog.nops; ++k)
         s += "  const VxT s" + std::to_string(k) + " = lane_of<VxT>(prog.ops[" + std::to_string(k) + "].s);\n";
     s += "  const int lane = (int)(threadIdx.x & 3);\n"
@@ -106,7 +112,11 @@ std::string kernel_source(bool f64, const ExprProg& prog, const ExprRed& rd) {
         s += "      if (i" + J + " < n) {\n        tr = tr || t;\n";
         for (int q = 0; q < rd.n; ++q) {
             const std::string Q = std::to_string(q), R = "v" + std::to_string(rd.vreg[q] & 15);
-            s += rd.red[q] == 0 ? "        xacc_sum(a" + Q + ", " + R + ");\n" : "        xacc_count(a" + Q + ", " + R + ");\n";
+            if (rd.red[q] == 0) s += "        xacc_sum(a" + Q + ", " + R + ");\n";
+            else if (rd.red[q] == 1) s += "        xacc_count(a" + Q + ", " + R + ");\n";
+            else
+                s += "        { const uint64_t g = group_minmax(" + R + ", " + std::to_string(rd.red[q]) +
+                     "); if (lane == 0) gacc_add(a" + Q + ", g); }\n";
         }
         s += "        if (i" + J + " + 4 >= n) {\n";
         for (int q = 0; q < rd.n; ++q)
@@ -116,7 +126,8 @@ std::string kernel_source(bool f64, const ExprProg& prog, const ExprRed& rd) {
     }
     s += "  }\n  if (tr) *trap = 1;\n";
     for (int q = 0; q < rd.n; ++q)
-        s += "  xacc_block_store(a" + std::to_string(q) + ", lds, parts + 8 * ((uint64_t)" + std::to_string(q) +
+        s += std::string(rd.red[q] <= 1 ? "  xacc_block_store(a" : "  gacc_block_store(a") + std::to_string(q) +
+             (rd.red[q] <= 1 ? ", lds" : ", glds") + ", parts + 8 * ((uint64_t)" + std::to_string(q) +
              " * gridDim.x + blockIdx.x));\n";
     s += "}\n";
     return s;

@@ -178,15 +178,18 @@ __global__ __launch_bounds__(256) void k_expr_eval(const T* __restrict__ col, ui
     if (trapped) *trap = 1;
 }
 
-// The words and up to four VSUM / VCOUNT reductions in one pass: no
-// evaluated column is written.  A workgroup takes tiles of 256 * kExprPf rows
-// (row base + j * 256 + t), issuing the tile's loads before evaluating any
-// of its rows.  Reduction q keeps a compensated f64 pair or a wrapping i64 sum
-// (VSUM) or a count of lanes that are non-zero and not NaN (VCOUNT) per
-// thread; the workgroup's totals go to parts[q * gridDim.x + blockIdx.x] as a
-// scan partial, merged afterwards by k_scan_final.  The last group of four
-// rows is reduced as the reference reduces one group into last[q], the
-// loop's partial register.
+// The words and up to four VSUM / VCOUNT / VRED_MIN / VRED_MAX reductions
+// in one pass: no evaluated column is written.  A workgroup takes tiles of
+// 256 * kExprPf rows (row base + j * 256 + t), issuing the tile's loads
+// before evaluating any of its rows.  Reduction q keeps a compensated f64
+// pair or a wrapping i64 sum (VSUM) or a count of lanes that are non-zero and
+// not NaN (VCOUNT) per thread, merged afterwards by k_scan_final, or the
+// signed and unsigned extremes of its group values (VRED_*: each group
+// reduced in lane order, its value's bits kept by the group's lane 0),
+// merged by k_gacc_final; the workgroup's record goes to
+// parts[q * gridDim.x + blockIdx.x].  The last group of four rows is reduced
+// as the reference reduces one group into last[q], the loop's partial
+// register.
 constexpr int kExprPf = 4;
 
 template <typename T>
@@ -195,9 +198,11 @@ __global__ __launch_bounds__(256) void k_expr_reduce(const T* __restrict__ col, 
                                                      uint64_t* __restrict__ parts, uint64_t* __restrict__ last,
                                                      int* __restrict__ trap) {
     __shared__ XAcc<T> lds[256 / 64];
+    __shared__ GAccX glds[256 / 64];
     XAcc<T> acc[4];
+    GAccX gacc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) xacc_init(acc[q]);
+    for (int q = 0; q < 4; ++q) { xacc_init(acc[q]); gacc_init(gacc[q]); }
     const int lane = (int)(threadIdx.x & 3);
     bool trapped = false;
     const uint64_t tile = 256ull * kExprPf;
@@ -219,8 +224,14 @@ __global__ __launch_bounds__(256) void k_expr_reduce(const T* __restrict__ col, 
             for (int q = 0; q < 4; ++q) {
                 if (q >= rd.n) break;
                 const T y = v[rd.vreg[q]];
-                if (rd.red[q] == 0) xacc_sum(acc[q], y);
-                else xacc_count(acc[q], y);
+                if (rd.red[q] == 0) {
+                    xacc_sum(acc[q], y);
+                } else if (rd.red[q] == 1) {
+                    xacc_count(acc[q], y);
+                } else {
+                    const uint64_t g = group_minmax(y, rd.red[q]);        // the quad's group value
+                    if (lane == 0) gacc_add(gacc[q], g);
+                }
                 if (i + 4 >= n) group_partial(y, rd.red[q], lane, &last[q]);   // the last group
             }
         }
@@ -229,8 +240,26 @@ __global__ __launch_bounds__(256) void k_expr_reduce(const T* __restrict__ col, 
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (q >= rd.n) break;
-        xacc_block_store(acc[q], lds, parts + 8 * ((uint64_t)q * gridDim.x + blockIdx.x));
+        uint64_t* part = parts + 8 * ((uint64_t)q * gridDim.x + blockIdx.x);
+        if (rd.red[q] <= 1) xacc_block_store(acc[q], lds, part);
+        else gacc_block_store(gacc[q], glds, part);
     }
+}
+
+// One workgroup: the signed / unsigned extremes of nb workgroup records
+// (VRED_MIN / VRED_MAX reductions) -> out (the same four words).
+__global__ __launch_bounds__(256) void k_gacc_final(const uint64_t* __restrict__ parts, uint32_t nb,
+                                                    uint64_t* __restrict__ out) {
+    __shared__ GAccX lds[256 / 64];
+    GAccX a;
+    gacc_init(a);
+    for (uint32_t i = threadIdx.x; i < nb; i += 256) {
+        GAccX o;
+        o.smin = (int64_t)parts[8 * (uint64_t)i]; o.smax = (int64_t)parts[8 * (uint64_t)i + 1];
+        o.umin = parts[8 * (uint64_t)i + 2]; o.umax = parts[8 * (uint64_t)i + 3];
+        gacc_merge(a, o);
+    }
+    gacc_block_store(a, lds, out);
 }
 
 __global__ __launch_bounds__(256) void k_expr_store(const uint64_t* __restrict__ src, uint64_t n, uint32_t count,
@@ -274,7 +303,11 @@ bool expr_reduce(int dtype, const void* col, uint64_t count, uint64_t off, uint6
         hipLaunchKernelGGL(k_expr_reduce<int64_t>, dim3((unsigned)nb), dim3(256), 0, c->stream, (const int64_t*)col,
                            count, off, n, prog, rd, reinterpret_cast<uint64_t*>(parts), last, trap);
     for (int q = 0; q < rd.n; ++q) {
-        if (dtype == DT_F64)
+        if (rd.red[q] >= 2)
+            hipLaunchKernelGGL(k_gacc_final, dim3(1), dim3(256), 0, c->stream,
+                               reinterpret_cast<const uint64_t*>((const ScanOut*)parts + q * nb), (uint32_t)nb,
+                               reinterpret_cast<uint64_t*>(res + q));
+        else if (dtype == DT_F64)
             hipLaunchKernelGGL(k_scan_final<double>, dim3(1), dim3(kBlock), 0, c->stream, (const ScanOut*)parts + q * nb,
                                (uint32_t)nb, res + q);
         else

@@ -49,10 +49,12 @@ bool segagg_unkey_f64(const uint64_t* k, uint64_t n, uint64_t* out, StreamCtx* c
 bool segagg_mode(int dtype, const void* x, uint64_t n, SegaggBufs& b, uint64_t* row, uint64_t* ties, StreamCtx* c);
 
 // The words over rows off + i, i < n (as expr_eval), reduced without writing
-// the evaluated values: res[q] (device) = scan result of reduction q over
-// every row (VSUM: sum pair / wrapping sum; VCOUNT: cnt), last[q] (device) =
-// the reduction of the last group of four rows (the loop's partial
-// register; not written when n = 0).  Uses the stream's scan partials.
+// the evaluated values: res[q] (device) = result of reduction q over every
+// row (VSUM: sum pair / wrapping sum; VCOUNT: cnt; VRED_MIN / VRED_MAX: the
+// signed min and max, then the unsigned min and max of the group values'
+// bits in the first four words), last[q] (device) = the reduction of the
+// last group of four rows (the loop's partial register; not written when n
+// = 0).  Uses the stream's scan partials.
 bool expr_reduce(int dtype, const void* col, uint64_t count, uint64_t off, uint64_t n, const ExprProg& prog,
                  const ExprRed& rd, ScanOut* res, uint64_t* last, int* trap, StreamCtx* c);
 
