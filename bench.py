@@ -147,11 +147,18 @@ def time_scan(desc, steps, warmup, stream_handle, settle_s=0.0, marks=False, set
     return wall, kernel_ms
 
 
-def _sync_time(fn, reps=3, warmup=1):
+def _sync_time(fn, reps=3, warmup=1, settle_s=0.0):
+    """Median wall time of reps calls of fn, each followed by a device
+    synchronisation; warmup calls first, then further calls until settle_s
+    seconds have passed (the clocks reach their steady state under load)."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    t_end = time.perf_counter() + settle_s
+    while time.perf_counter() < t_end:
+        fn()
+        torch.cuda.synchronize()
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -339,7 +346,7 @@ def bench_groupby() -> dict:
                                          ng.ctypes.data, None) != 0:
                 raise RuntimeError("groupby failed")
 
-        t = _sync_time(run, reps=15, warmup=5)
+        t = _sync_time(run, reps=30, warmup=5, settle_s=0.25)
         out["rows_%d" % rows] = {"ms": round(t * 1e3, 3), "groups": int(ng[0]),
                                  "gb_per_s": round(rows * 12 / t / 1e9, 1),
                                  "frac_of_8tbs": round(rows * 12 / t / 8e12, 4),

@@ -56,6 +56,8 @@ constexpr int kGradThreads = 64;
 constexpr uint32_t kLeaves = 4096;           // linear models under the MLP
 constexpr int kMinBucketLog2 = 5;            // average bucket rows are kept at or above 32
 constexpr uint32_t kBigCap = 65536;          // groups over a wave's capacity that are fixed up
+constexpr int kTrainSteps = 60;              // Adam steps of the root MLP (fewer leave its logit-CDF fit
+                                             // of lognormal keys above 0.02 mean squared error)
 
 namespace {
 
@@ -646,7 +648,7 @@ struct LearnedBufs {
 };
 
 struct LearnedTune {
-    int sample_log2 = 20, train_points = 4096, train_steps = 60, bucket_log2 = 8;
+    int sample_log2 = 20, train_points = 4096, train_steps = kTrainSteps, bucket_log2 = 8;
 };
 static LearnedTune g_tune;
 
@@ -877,7 +879,8 @@ static bool learned_table(LearnedBufs& b, const uint64_t* kin, uint64_t n, uint6
     const uint64_t gmax_i = logmode ? (uint64_t)glog : span;
     int gbits = 0;
     while (gbits < 64 && (gmax_i >> gbits) != 0) ++gbits;
-    const int cshift = gbits > 16 ? gbits - 16 : 0;
+    // at most 2^14 cells: the table (8 B a cell) fits in LDS for pass 0
+    const int cshift = gbits > 14 ? gbits - 14 : 0;
     const uint32_t ncells = (uint32_t)(gmax_i >> cshift) + 1;
     S.cells = ncells;
     hipEventRecord(b.ev[1], st);
@@ -913,7 +916,7 @@ static bool learned_table(LearnedBufs& b, const uint64_t* kin, uint64_t n, uint6
     hipLaunchKernelGGL(k_prefix_max<double>, dim3(1), dim3(1024), 0, st, b.yd, ncells);
     hipLaunchKernelGGL(k_table, dim3((ncells + 255) / 256), dim3(256), 0, st, b.yd, ncells, nfine, b.cell);
     if (!hip_ok(hipGetLastError(), "table")) return false;
-    dig = TableDigit{b.cell, kmin, kmax, ncells, cshift, logmode, nfine, std::ldexp(1.0f, -cshift)};
+    dig = TableDigit{b.cell, kmin, kmax, ncells, cshift, logmode, nfine};
     hipEventRecord(b.ev[3], st);
     return true;
 }
@@ -1012,7 +1015,7 @@ int voxel_gpu_learned_tune(int sample_log2, int train_points, int train_steps, i
     std::lock_guard<std::mutex> lk(api_mutex());
     g_tune.sample_log2 = sample_log2 > 0 ? sample_log2 : 20;
     g_tune.train_points = train_points > 0 ? train_points : 4096;
-    g_tune.train_steps = train_steps >= 0 ? (train_steps ? train_steps : 60) : 0;
+    g_tune.train_steps = train_steps >= 0 ? (train_steps ? train_steps : kTrainSteps) : 0;
     g_tune.bucket_log2 = rows_per_bucket_log2 > 0 ? rows_per_bucket_log2 : 8;
     if (g_tune.bucket_log2 > 8) g_tune.bucket_log2 = 8;       // a bucket group must fit one wave's LDS on average
     if (g_tune.bucket_log2 < kMinBucketLog2) g_tune.bucket_log2 = kMinBucketLog2;
@@ -1096,9 +1099,11 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         if (stats) *stats = S;
         return 0;
     }
-    // 5. stable partition by bucket id (LSD over its bytes); pass 0 computes
-    //    the id once and every pass carries it next to the rows; without a
-    //    payload, pass 0 writes the row index
+    // 5. stable partition by bucket id (LSD over its bytes); pass 0's
+    //    counting kernel computes the id once per row (into the carried-id
+    //    buffer that pass 0 does not write), its scatter reads it, and every
+    //    pass carries it next to the rows; without a payload, pass 0 writes
+    //    the row index
     const uint64_t* ks = kin;
     const uint32_t* vs = vin_user;
     const uint32_t* fs = nullptr;
@@ -1107,7 +1112,8 @@ int voxel_gpu_sort_pairs_u64_learned(const uint64_t* kin, const uint32_t* vin_us
         uint64_t* kd = (ks == b.k1) ? b.k2 : b.k1;
         uint32_t* vd = (vs == b.v1) ? b.v2 : b.v1;
         uint32_t* fd = (fs == b.f1) ? b.f2 : b.f1;
-        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, p, b.w, st, fs, fd, p == 0 && !vin_user), "model pass"))
+        uint32_t* pre = p == 0 ? (fd == b.f1 ? b.f2 : b.f1) : nullptr;     // free during pass 0
+        if (!hip_ok(radix_pass(ks, vs, kd, vd, n, dig, p, b.w, st, fs, fd, p == 0 && !vin_user, pre), "model pass"))
             return -1;
         ks = kd;
         vs = vd;

@@ -4,6 +4,7 @@
 // for the algorithm.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 #include "vx_kernels.hpp"
@@ -68,11 +69,30 @@ __device__ __forceinline__ uint32_t pass_digit(const Digit& dig, uint64_t key, u
     else return dig.digit(key, pass);
 }
 
-// cnt[t * 256 + d] = rows of tile t with digit d
-template <class Digit, bool CIN>
+// Adds the wave's rows (digit d where valid) to the wave's counters hw:
+// one LDS atomic per row, or one for the whole wave when every valid row has
+// the same digit (sorted or nearly sorted keys, where per-row atomics on one
+// counter would serialise).  Called by all lanes of the wave together.
+__device__ __forceinline__ void count_digits(uint32_t* hw, uint32_t d, bool valid) {
+    const uint64_t act = __ballot(valid);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(valid && d == d0) == act) {
+        if (act && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)act) - 1)
+            atomicAdd(&hw[d0], (uint32_t)__popcll(act));
+    } else if (valid) {
+        atomicAdd(&hw[d], 1u);
+    }
+}
+
+// cnt[t * 256 + d] = rows of tile t with digit d.  FOUT (keys through the
+// digit function only): also fout[i] = dig.bits(key i), the row's digit
+// word, so the scatter can read it instead of evaluating the function again.
+template <class Digit, bool CIN, bool FOUT = false>
 __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint64_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ fin, uint64_t n, Digit dig,
-                                                          int pass, uint32_t* __restrict__ cnt) {
+                                                          int pass, uint32_t* __restrict__ cnt,
+                                                          uint32_t* __restrict__ fout = nullptr) {
+    static_assert(!(CIN && FOUT), "a carried digit word is not written again");
     __shared__ uint32_t h[kWaves][256];
     const int w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&h[0][0])[i] = 0;
@@ -80,26 +100,97 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint64_t* __rest
     const uint64_t s = (uint64_t)blockIdx.x * kTile;
     const uint64_t e = s + kTile < n ? s + kTile : n;
     constexpr int U = 8;                                    // rows per lane in flight
-    for (uint64_t i0 = s; i0 < e; i0 += kThreads * U) {      // block-uniform trip count
+    // rows i0 + u * kThreads + tid; ALL: every one of them is below e (no
+    // per-row bounds test)
+    auto step = [&](uint64_t i0, auto all_c) {
+        constexpr bool ALL = decltype(all_c)::value;
+        auto in = [&](int u) { return ALL || i0 + (uint64_t)u * kThreads + threadIdx.x < e; };
         uint64_t kk[U];
         uint32_t dd[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint64_t i = i0 + (uint64_t)u * kThreads + threadIdx.x;
-            if constexpr (CIN) dd[u] = i < e ? fin[i] : 0u;
-            else kk[u] = i < e ? keys[i] : 0;
+            if constexpr (CIN) dd[u] = in(u) ? fin[i] : 0u;
+            else kk[u] = in(u) ? keys[i] : 0;
         }
+        if constexpr (FOUT) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) dd[u] = pass_digit<Digit, CIN>(dig, CIN ? 0 : kk[u], dd[u], pass);
-        // counting needs no ranks: one LDS atomic per row (the LDS serialises
-        // equal digits of a wave; far cheaper than matching the digit bits)
+            for (int u = 0; u < U; ++u) {
+                const uint32_t f = (uint32_t)dig.bits(kk[u]);
+                if (in(u)) fout[i0 + (uint64_t)u * kThreads + threadIdx.x] = f;
+                dd[u] = (f >> (8 * pass)) & 0xFFu;
+            }
+        } else {
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (i0 + (uint64_t)u * kThreads + threadIdx.x < e) atomicAdd(&h[w][dd[u]], 1u);
-    }
+            for (int u = 0; u < U; ++u) dd[u] = pass_digit<Digit, CIN>(dig, CIN ? 0 : kk[u], dd[u], pass);
+        }
+        // counting needs no ranks: one LDS atomic per row, or per wave when
+        // its rows share the digit
+#pragma unroll
+        for (int u = 0; u < U; ++u) count_digits(h[w], dd[u], in(u));
+    };
+    uint64_t i0 = s;
+    for (; i0 + (uint64_t)kThreads * U <= e; i0 += kThreads * U) step(i0, std::true_type{});   // block-uniform
+    if (i0 < e) step(i0, std::false_type{});
     __syncthreads();
     const int d = threadIdx.x;
     cnt[(uint64_t)blockIdx.x * 256 + d] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+}
+
+// k_tile_counts<TableDigit, false, true> with the cell table in LDS, for
+// tables of at most kLdsCells cells: one persistent workgroup of
+// kTcThreads threads per CU loads the table once and takes tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ... (eight rows per thread per tile), issuing the
+// next tile's key loads before it evaluates the current one.  Uniform keys
+// read the table at random cells: from LDS that is one ds_read per row
+// instead of one L2 line request per row.  Same outputs as k_tile_counts.
+constexpr uint32_t kLdsCells = 16384;
+constexpr int kTcThreads = 1024;
+static_assert(kTile == 8 * kTcThreads, "eight rows per thread per tile");
+static __global__ __launch_bounds__(kTcThreads) void k_table_counts(const uint64_t* __restrict__ keys, uint64_t n,
+                                                             TableDigit dig, int pass, uint32_t* __restrict__ cnt,
+                                                             uint32_t* __restrict__ fout, uint64_t ntiles) {
+    constexpr int U = 8, NW = kTcThreads / 64;
+    __shared__ uint2 tab[kLdsCells];
+    __shared__ uint32_t h[NW][256];
+    const int w = threadIdx.x >> 6;
+    const uint32_t nc = dig.ncells < kLdsCells ? dig.ncells : kLdsCells;     // the host checks ncells
+    for (uint32_t i = threadIdx.x; i < nc; i += kTcThreads) tab[i] = dig.cell[i];
+    for (int i = threadIdx.x; i < NW * 256; i += kTcThreads) (&h[0][0])[i] = 0;
+    auto load = [&](uint64_t t, uint64_t (&kk)[U]) {
+        const uint64_t s = t * kTile;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = s + (uint64_t)u * kTcThreads + threadIdx.x;
+            kk[u] = i < n ? keys[i] : 0;
+        }
+    };
+    uint64_t kk[U];
+    if (blockIdx.x < ntiles) load(blockIdx.x, kk);
+    __syncthreads();                                                // table and counters ready
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint64_t nx[U];
+        if (t + gridDim.x < ntiles) load(t + gridDim.x, nx);          // block-uniform
+        const uint64_t s = t * kTile;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = s + (uint64_t)u * kTcThreads + threadIdx.x;
+            const uint32_t f = dig.fine_in(tab, kk[u]);
+            if (i < n) fout[i] = f;
+            count_digits(h[w], (f >> (8 * pass)) & 0xFFu, i < n);
+        }
+        __syncthreads();
+        if (threadIdx.x < 256) {
+            const int d = threadIdx.x;
+            uint32_t c = 0;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) { c += h[ww][d]; h[ww][d] = 0; }
+            cnt[t * 256 + d] = c;
+        }
+        __syncthreads();                                            // counters cleared
+#pragma unroll
+        for (int u = 0; u < U; ++u) kk[u] = nx[u];
+    }
 }
 
 // csum[j * 256 + d] = rows with digit d in the tiles of chunk j.  (Internal
@@ -438,29 +529,51 @@ hipError_t launch_constant_bytes(const uint64_t* keys, uint64_t n, RadixWork& w,
 // instead of dig; fout: write the rows' digit words (carried, or
 // dig.bits(key)) next to kout.  vin and vout null: keys only (no carried
 // digit words then).  row_index: the payload of row i is i (vin unused,
-// vout required, no fin).
+// vout required, no fin).  fpre (with fout, without fin): n words the
+// counting kernel fills with dig.bits(key) in kin's row order, which the
+// scatter then reads as carried words (the digit function runs once per
+// row instead of twice).
 template <class Digit>
 hipError_t radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint64_t n,
                       const Digit& dig, int pass, RadixWork& w, hipStream_t st, const uint32_t* fin,
-                      uint32_t* fout, bool row_index) {
+                      uint32_t* fout, bool row_index, uint32_t* fpre) {
     if (n == 0) return hipSuccess;
     if (row_index ? (!vout || fin) : ((!vin || !vout) && (vin || vout || fin || fout))) return hipErrorInvalidValue;
+    if (fpre && (fin || !fout)) return hipErrorInvalidValue;
     const uint64_t nt = radix_tiles(n), nch = radix_chunks(n);
     if (nt > 0xFFFFFFFFull || w.status_cap < nt * 256 + nch * 256 + 256) return hipErrorInvalidValue;
     uint32_t* cnt = w.status;
     uint32_t* csum = w.status + nt * 256;
     uint32_t* dbase = csum + nch * 256;
     if (fin) hipLaunchKernelGGL((k_tile_counts<Digit, true>), dim3((unsigned)nt), dim3(kThreads), 0, st, kin, fin, n,
-                                dig, pass, cnt);
+                                dig, pass, cnt, nullptr);
+    else if (fpre) {
+        bool lds = false;
+        if constexpr (std::is_same<Digit, TableDigit>::value) {
+            if (dig.ncells <= kLdsCells) {
+                const uint64_t cus = (uint64_t)device_cu_count();
+                hipLaunchKernelGGL(k_table_counts, dim3((unsigned)(nt < cus ? nt : cus)), dim3(kTcThreads), 0, st, kin,
+                                   n, dig, pass, cnt, fpre, nt);
+                lds = true;
+            }
+        }
+        if (!lds)
+            hipLaunchKernelGGL((k_tile_counts<Digit, false, true>), dim3((unsigned)nt), dim3(kThreads), 0, st, kin, fin,
+                               n, dig, pass, cnt, fpre);
+    }
     else hipLaunchKernelGGL((k_tile_counts<Digit, false>), dim3((unsigned)nt), dim3(kThreads), 0, st, kin, fin, n, dig,
-                            pass, cnt);
+                            pass, cnt, nullptr);
+    if (fpre) fin = fpre;
     hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)nch), dim3(256), 0, st, (const uint32_t*)cnt, nt, csum);
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(256), 0, st, csum, nch, dbase);
     hipLaunchKernelGGL(k_tile_offsets, dim3((unsigned)nch), dim3(256), 0, st, cnt, nt, (const uint32_t*)csum,
                        (const uint32_t*)dbase);
     const dim3 g((unsigned)nt), b(kDsThreads);
     const uint32_t* off = cnt;
-    if (row_index && fout)
+    if (row_index && fin)
+        hipLaunchKernelGGL((k_scatter_tile<Digit, true, true, true, true>), g, b, 0, st, kin, vin, fin, kout, vout,
+                           fout, n, dig, pass, nt, off);
+    else if (row_index && fout)
         hipLaunchKernelGGL((k_scatter_tile<Digit, false, true, true, true>), g, b, 0, st, kin, vin, fin, kout, vout,
                            fout, n, dig, pass, nt, off);
     else if (row_index)

@@ -175,6 +175,6 @@ hipError_t launch_constant_bytes(const uint64_t* keys, uint64_t n, RadixWork& w,
 
 template hipError_t radix_pass<LsdDigit>(const uint64_t*, const uint32_t*, uint64_t*, uint32_t*, uint64_t,
                                          const LsdDigit&, int, RadixWork&, hipStream_t, const uint32_t*, uint32_t*,
-                                         bool);
+                                         bool, uint32_t*);
 
 }  // namespace vx
