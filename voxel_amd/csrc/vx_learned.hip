@@ -462,6 +462,23 @@ __global__ __launch_bounds__(256) void k_min_apply(uint32_t* __restrict__ a, uin
     }
 }
 
+// rank += 1 when (k, row j) orders before (key, row i): the borrow out of
+// the 96-bit subtraction (k hi : k lo : jl) - (key hi : key lo : il), whose
+// low words carry the row order (jl < il exactly when j < i; both offset so
+// that neither wraps).  Four VALU instructions instead of two 64-bit
+// compares, a row compare and their combination.
+__device__ __forceinline__ void rank_step(uint32_t& rank, uint32_t jl, uint32_t il, uint64_t k, uint32_t klo,
+                                          uint32_t khi) {
+    uint32_t t0, t1, t2;
+    asm("v_sub_co_u32 %0, vcc, %4, %5\n\t"
+        "v_subb_co_u32 %1, vcc, %6, %7, vcc\n\t"
+        "v_subb_co_u32 %2, vcc, %8, %9, vcc\n\t"
+        "v_addc_co_u32 %3, vcc, 0, %3, vcc"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "+v"(rank)
+        : "v"(jl), "v"(il), "v"((uint32_t)k), "v"(klo), "v"((uint32_t)(k >> 32)), "v"(khi)
+        : "vcc");
+}
+
 constexpr int kWaveRows = 512;
 constexpr uint32_t kMaxGroup = 63;           // buckets per wave group (their starts sit in LDS)
 
@@ -487,19 +504,21 @@ __device__ __forceinline__ void wave_rank_sort(const uint64_t* __restrict__ kin,
         }
         const uint32_t lo = sb[a], hi = sb[a + 1];
         const uint64_t key = kb[i];
+        const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
+        // rank = rows j of the bucket with (kb[j], j) < (key, i): row j + u
+        // compares j + 4 with i + 4 - u
+        const uint32_t i4 = i + 4;
         uint32_t rank = 0;
         uint32_t j = lo;
         for (; j + 4 <= hi; j += 4) {
             const uint64_t k0 = kb[j], k1 = kb[j + 1], k2 = kb[j + 2], k3 = kb[j + 3];
-            rank += (k0 < key) || (k0 == key && j < i);
-            rank += (k1 < key) || (k1 == key && j + 1 < i);
-            rank += (k2 < key) || (k2 == key && j + 2 < i);
-            rank += (k3 < key) || (k3 == key && j + 3 < i);
+            const uint32_t j4 = j + 4;
+            rank_step(rank, j4, i4, k0, klo, khi);
+            rank_step(rank, j4, i4 - 1, k1, klo, khi);
+            rank_step(rank, j4, i4 - 2, k2, klo, khi);
+            rank_step(rank, j4, i4 - 3, k3, klo, khi);
         }
-        for (; j < hi; ++j) {
-            const uint64_t kj = kb[j];
-            rank += (kj < key) || (kj == key && j < i);
-        }
+        for (; j < hi; ++j) rank_step(rank, j + 4, i4, kb[j], klo, khi);
         if (hi - lo > longest) longest = hi - lo;
         kout[s + lo + rank] = key;
         vout[s + lo + rank] = vb[i];
