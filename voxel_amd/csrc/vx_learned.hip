@@ -495,13 +495,12 @@ __device__ __forceinline__ void wave_rank_sort(const uint64_t* __restrict__ kin,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t a = 0;                     // bucket of the lane's previous row (a lane's rows increase)
     for (uint32_t i = lane; i < m; i += 64) {
-        // bucket of row i: last q with sb[q] <= i (binary search in LDS)
-        uint32_t a = 0, z = nbk;
-        while (z - a > 1) {
-            const uint32_t mid = (a + z) >> 1;
-            if (sb[mid] <= i) a = mid; else z = mid;
-        }
+        // bucket of row i: last q with sb[q] <= i, stepping on from the
+        // previous row's (64 rows on: about two buckets); sb[nbk] = m > i
+        // ends the walk
+        while (sb[a + 1] <= i) ++a;
         const uint32_t lo = sb[a], hi = sb[a + 1];
         const uint64_t key = kb[i];
         const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
