@@ -37,6 +37,7 @@ import json
 import math
 import os
 import statistics
+import struct
 import sys
 import time
 
@@ -617,7 +618,10 @@ def bench_expr(rows_log2: int = 27) -> dict:
                      "frac_of_peak": round(n * 8 / tc / 1e9 / HBM_PEAK_GBS, 4),
                      "compile_ms": round((first - tc) * 1e3, 1),
                      "interpreted_ms": round(ti * 1e3, 4), "speedup_vs_interpreted": round(ti / tc, 2),
-                     "same_result": rc == ri if acc == 7 else None}
+                     "same_result": rc == ri}
+        if acc != 7:                                   # f64 sum register: also how far apart, relative
+            fc, fi = (struct.unpack("<d", struct.pack("<Q", v & (2**64 - 1)))[0] for v in (rc, ri))
+            out[name]["rel_diff_vs_interpreted"] = abs(fc - fi) / abs(fi) if fi else abs(fc - fi)
     del x
     torch.cuda.empty_cache()
     return out
