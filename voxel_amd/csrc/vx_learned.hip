@@ -940,7 +940,8 @@ static bool learned_table(LearnedBufs& b, const uint64_t* kin, uint64_t n, uint6
 }
 
 // The learned CDF of n keys as a partitioner into nfine buckets (for the
-// range-partitioned join): the table is copied to cell_out (65538 entries).
+// range-partitioned join): the table (at most 2^14 + 1 entries) is copied
+// to cell_out, a buffer of 65538 entries.
 bool learned_partitioner(const uint64_t* kin, uint64_t n, uint64_t flip, int model, uint32_t nfine, uint2* cell_out,
                          TableDigit* dig, voxel_gpu_learned_stats* stats, StreamCtx* c) {
     voxel_gpu_learned_stats S;
