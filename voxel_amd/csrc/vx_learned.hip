@@ -52,7 +52,9 @@ int sort_pairs_u64_rowidx(const uint64_t* kin, uint64_t* kout, uint32_t* vout, u
 constexpr int kH = 32;                       // hidden width
 constexpr int kParams = 32 + 32 + 32 * 32 + 32 + 32 + 1;   // 1153
 constexpr int oW1 = 0, oB1 = 32, oW2 = 64, oB2 = 64 + 1024, oW3 = oB2 + 32, oB3 = oW3 + 32;
-constexpr int kGradThreads = 64;
+constexpr int kGradThreads = 64;             // samples per gradient workgroup
+constexpr int kGradLanes = 4;                // lanes per sample
+constexpr int kGradBlock = kGradThreads * kGradLanes;
 constexpr uint32_t kLeaves = 4096;           // linear models under the MLP
 constexpr int kMinBucketLog2 = 5;            // average bucket rows are kept at or above 32
 constexpr uint32_t kBigCap = 65536;          // groups over a wave's capacity that are fixed up
@@ -91,60 +93,89 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ P
 
 // Mean-squared-error gradient of one 64-sample slice per workgroup:
 // part[block][0..1152] = sum over the slice of dL/dparam, part[block][1153] = sum of squared errors.
-__global__ __launch_bounds__(kGradThreads) void k_mlp_grad(const float* __restrict__ Pg, const float* __restrict__ x,
-                                                           const float* __restrict__ yt, uint64_t n, float inv_n,
-                                                           float* __restrict__ part) {
+// Four lanes per sample (kGradLanes): lane g of a sample holds hidden units
+// 8g..8g+7 of the second layer, so a sample's forward and backward pass is
+// a quarter of the serial work; the output and the first layer's gradient
+// are summed over the four lanes by quad shuffles (every lane of the quad
+// gets the same sum).  Then the 1153 parameter gradients are summed over the
+// 64 samples in sample order from LDS.
+__global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const float* __restrict__ Pg, const float* __restrict__ x,
+                                                         const float* __restrict__ yt, uint64_t n, float inv_n,
+                                                         float* __restrict__ part) {
+    constexpr int Q = kH / kGradLanes;                   // second-layer units per lane
     __shared__ float P[kParams];
     __shared__ float H1[kGradThreads][kH + 1], DH1[kGradThreads][kH + 1], DH2[kGradThreads][kH + 1];
     __shared__ float H2[kGradThreads][kH + 1];
     __shared__ float DY[kGradThreads], X[kGradThreads], E2[kGradThreads];
-    for (int i = threadIdx.x; i < kParams; i += kGradThreads) P[i] = Pg[i];
+    for (int i = threadIdx.x; i < kParams; i += kGradBlock) P[i] = Pg[i];
     __syncthreads();
-    const int t = threadIdx.x;
-    const uint64_t i = (uint64_t)blockIdx.x * kGradThreads + t;
-    float h1[kH], h2[kH];
-    float xi = 0.0f, dy = 0.0f, e = 0.0f;
-    if (i < n) {
-        xi = x[i];
-        float y = mlp_fwd(P, xi, h1, h2);
-        e = y - yt[i];
-        dy = 2.0f * e * inv_n;
-    } else {
+    const int t = threadIdx.x, sp = t / kGradLanes, g = t % kGradLanes;
+    const uint64_t i = (uint64_t)blockIdx.x * kGradThreads + sp;
+    const bool live = i < n;
+    const float xi = live ? x[i] : 0.0f;
+    float h1[kH], h2[Q];
 #pragma unroll
-        for (int j = 0; j < kH; ++j) { h1[j] = 0.0f; h2[j] = 0.0f; }
+    for (int j = 0; j < kH; ++j) h1[j] = live ? fmaxf(P[oW1 + j] * xi + P[oB1 + j], 0.0f) : 0.0f;
+    float yp = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int k = g * Q + q;
+        float s = P[oB2 + k];
+#pragma unroll
+        for (int j = 0; j < kH; ++j) s += P[oW2 + k * kH + j] * h1[j];
+        h2[q] = live ? fmaxf(s, 0.0f) : 0.0f;
+        yp += P[oW3 + k] * h2[q];
     }
-    float dh2[kH], dh1[kH];
+    yp += __shfl_xor(yp, 1, kGradLanes);
+    yp += __shfl_xor(yp, 2, kGradLanes);
+    const float e = live ? (P[oB3] + yp) - yt[i] : 0.0f;
+    const float dy = 2.0f * e * inv_n;
+    float dh2[Q];
 #pragma unroll
-    for (int k = 0; k < kH; ++k) dh2[k] = h2[k] > 0.0f ? dy * P[oW3 + k] : 0.0f;
+    for (int q = 0; q < Q; ++q) dh2[q] = h2[q] > 0.0f ? dy * P[oW3 + g * Q + q] : 0.0f;
 #pragma unroll
     for (int j = 0; j < kH; ++j) {
         float s = 0.0f;
 #pragma unroll
-        for (int k = 0; k < kH; ++k) s += dh2[k] * P[oW2 + k * kH + j];
-        dh1[j] = h1[j] > 0.0f ? s : 0.0f;
+        for (int q = 0; q < Q; ++q) s += dh2[q] * P[oW2 + (g * Q + q) * kH + j];
+        s += __shfl_xor(s, 1, kGradLanes);
+        s += __shfl_xor(s, 2, kGradLanes);
+        if (j / Q == g) DH1[sp][j] = h1[j] > 0.0f ? s : 0.0f;
     }
 #pragma unroll
-    for (int j = 0; j < kH; ++j) { H1[t][j] = h1[j]; H2[t][j] = h2[j]; DH1[t][j] = dh1[j]; DH2[t][j] = dh2[j]; }
-    DY[t] = dy; X[t] = xi; E2[t] = e * e;
+    for (int q = 0; q < Q; ++q) {
+        H1[sp][g * Q + q] = h1[g * Q + q];
+        H2[sp][g * Q + q] = h2[q];
+        DH2[sp][g * Q + q] = dh2[q];
+    }
+    if (g == 0) { DY[sp] = dy; X[sp] = xi; E2[sp] = e * e; }
     __syncthreads();
     float* out = part + (uint64_t)blockIdx.x * (kParams + 1);
-    // dW2[k][j] = sum_s DH2[s][k] * H1[s][j]: 16 entries per thread, samples in order
-    for (int q = 0; q < 16; ++q) {
-        int idx = t * 16 + q, k = idx / kH, j = idx % kH;
-        float s = 0.0f;
-        for (int r = 0; r < kGradThreads; ++r) s += DH2[r][k] * H1[r][j];
-        out[oW2 + idx] = s;
-    }
-    if (t < kH) {
-        float w1 = 0.0f, b1 = 0.0f, b2 = 0.0f, w3 = 0.0f;
+    // dW2[k][j] = sum_s DH2[s][k] * H1[s][j]: four entries per thread (one k,
+    // four j), samples in order
+    {
+        const int idx = t * 4, k = idx / kH, j = idx % kH;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+#pragma unroll 8
         for (int r = 0; r < kGradThreads; ++r) {
-            w1 += DH1[r][t] * X[r];
-            b1 += DH1[r][t];
-            b2 += DH2[r][t];
-            w3 += DY[r] * H2[r][t];
+            const float d = DH2[r][k];
+            s0 += d * H1[r][j]; s1 += d * H1[r][j + 1]; s2 += d * H1[r][j + 2]; s3 += d * H1[r][j + 3];
         }
-        out[oW1 + t] = w1; out[oB1 + t] = b1; out[oB2 + t] = b2; out[oW3 + t] = w3;
-    } else if (t == kH) {
+        out[oW2 + idx] = s0; out[oW2 + idx + 1] = s1; out[oW2 + idx + 2] = s2; out[oW2 + idx + 3] = s3;
+    }
+    if (t < 4 * kH) {
+        // one of W1, B1, B2, W3 (by t / kH) for unit t % kH
+        const int u = t % kH, which = t / kH;
+        // a = the per-sample factor, b = the unit's value: (DH1, X), (DH1, 1), (DH2, 1), (H2, DY)
+        const float (*va)[kH + 1] = which <= 1 ? DH1 : which == 2 ? DH2 : H2;
+        float s = 0.0f;
+#pragma unroll 8
+        for (int r = 0; r < kGradThreads; ++r) {
+            const float f = which == 0 ? X[r] : which == 3 ? DY[r] : 1.0f;
+            s += va[r][u] * f;
+        }
+        out[(which == 0 ? oW1 : which == 1 ? oB1 : which == 2 ? oB2 : oW3) + u] = s;
+    } else if (t == 4 * kH) {
         float b3 = 0.0f, l = 0.0f;
         for (int r = 0; r < kGradThreads; ++r) { b3 += DY[r]; l += E2[r]; }
         out[oB3] = b3;
@@ -771,7 +802,7 @@ static void init_params(std::vector<float>& P, uint64_t seed) {
 static bool mlp_grad(LearnedBufs& b, const float* P, const float* x, const float* y, uint64_t n, float* grad,
                      float* loss, int adam, float lr, int step, hipStream_t st) {
     uint64_t blocks = (n + kGradThreads - 1) / kGradThreads;
-    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)blocks), dim3(kGradThreads), 0, st, P, x, y, n, 1.0f / (float)n, b.part);
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)blocks), dim3(kGradBlock), 0, st, P, x, y, n, 1.0f / (float)n, b.part);
     float b1c = 1.0f - std::pow(0.9f, (float)step), b2c = 1.0f - std::pow(0.999f, (float)step);
     hipLaunchKernelGGL(k_mlp_reduce, dim3((kParams + 1 + 63) / 64), dim3(256), 0, st, const_cast<float*>(P), b.m, b.v,
                        b.part, (uint32_t)blocks, adam, lr, b1c, b2c, grad, loss, 1.0f / (float)n);
@@ -811,7 +842,7 @@ static bool train_graph(LearnedBufs& b, uint64_t T, int steps, hipStream_t st) {
             hipKernelNodeParams kp = {};
             kp.func = reinterpret_cast<void*>(&k_mlp_grad);
             kp.gridDim = dim3(nblocks);
-            kp.blockDim = dim3(kGradThreads);
+            kp.blockDim = dim3(kGradBlock);
             kp.sharedMemBytes = 0;
             kp.kernelParams = ga;
             kp.extra = nullptr;
