@@ -107,12 +107,26 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const float* __restrict
     __shared__ float H1[kGradThreads][kH + 1], DH1[kGradThreads][kH + 1], DH2[kGradThreads][kH + 1];
     __shared__ float H2[kGradThreads][kH + 1];
     __shared__ float DY[kGradThreads], X[kGradThreads], E2[kGradThreads];
-    for (int i = threadIdx.x; i < kParams; i += kGradBlock) P[i] = Pg[i];
-    __syncthreads();
+    // every global load of the step issued before any is used (one memory
+    // round trip instead of one per parameter chunk)
+    constexpr int kPLoads = (kParams + kGradBlock - 1) / kGradBlock;
+    float pv[kPLoads];
+#pragma unroll
+    for (int u = 0; u < kPLoads; ++u) {
+        const int pi = (int)threadIdx.x + u * kGradBlock;
+        pv[u] = pi < kParams ? Pg[pi] : 0.0f;
+    }
     const int t = threadIdx.x, sp = t / kGradLanes, g = t % kGradLanes;
     const uint64_t i = (uint64_t)blockIdx.x * kGradThreads + sp;
     const bool live = i < n;
     const float xi = live ? x[i] : 0.0f;
+    const float yti = live ? yt[i] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kPLoads; ++u) {
+        const int pi = (int)threadIdx.x + u * kGradBlock;
+        if (pi < kParams) P[pi] = pv[u];
+    }
+    __syncthreads();
     float h1[kH], h2[Q];
 #pragma unroll
     for (int j = 0; j < kH; ++j) h1[j] = live ? fmaxf(P[oW1 + j] * xi + P[oB1 + j], 0.0f) : 0.0f;
@@ -128,7 +142,7 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const float* __restrict
     }
     yp += __shfl_xor(yp, 1, kGradLanes);
     yp += __shfl_xor(yp, 2, kGradLanes);
-    const float e = live ? (P[oB3] + yp) - yt[i] : 0.0f;
+    const float e = live ? (P[oB3] + yp) - yti : 0.0f;
     const float dy = 2.0f * e * inv_n;
     float dh2[Q];
 #pragma unroll
@@ -196,6 +210,13 @@ __global__ __launch_bounds__(256) void k_mlp_reduce(float* __restrict__ Pg, floa
     float g = 0.0f;
     if (p <= kParams) {
         uint32_t b = w;
+        for (; b + 60 < nblocks; b += 64) {                  // sixteen loads in flight, added in order
+            float a[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) a[u] = part[(uint64_t)(b + 4 * u) * (kParams + 1) + p];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) g += a[u];
+        }
         for (; b + 12 < nblocks; b += 16) {
             float a0 = part[(uint64_t)b * (kParams + 1) + p], a1 = part[(uint64_t)(b + 4) * (kParams + 1) + p];
             float a2 = part[(uint64_t)(b + 8) * (kParams + 1) + p], a3 = part[(uint64_t)(b + 12) * (kParams + 1) + p];
